@@ -1,0 +1,74 @@
+/* CPU restatement of the a5_streamer hot path (test infrastructure only).
+ *
+ * This is the oracle that the HIP library is checked against. It restates,
+ * in plain C with one scalar loop per reference loop and the reference's
+ * operand order, the Afivo tree core (afivo/src/m_aX_core.f90), ghost cells
+ * (m_aX_gc.f90), prolongation (m_aX_prolong.f90), restriction
+ * (m_aX_restrict.f90), multigrid (m_aX_mg.f90) and the streamer per-box
+ * physics (src/streamer_{2d,3d,cyl}.f90, src/m_streamer.f90,
+ * fosito/m_lookup_table.f90). Only tests, the smoke check and the CPU
+ * baseline leg of bench.py load it; the product never does.
+ *
+ * Conventions follow the reference: box ids are 1-based, 0 = no box,
+ * negative neighbour ids are physical boundaries, variable indices are
+ * 1-based. A box's cell-centred block is cc(0:nc+1, 0:nc+1[, 0:nc+1]) with
+ * i fastest, stored for every variable of a box contiguously.
+ */
+#ifndef A5_ORACLE_H
+#define A5_ORACLE_H
+
+typedef struct otree otree;
+
+otree *o_tree_new(int dim, int n_cell, int n_cc, int n_fc, double dr,
+                  const double *r_min, int coarsen_to, int lvls_max, int coord,
+                  int n_boxes);
+void o_tree_free(otree *t);
+void o_set_base(otree *t, int n, const int *ix_list, const int *nb_list);
+void o_adjust(otree *t, int n_flags, const int *flags, int *n_add, int *n_rm);
+void o_tidy_up(otree *t, double max_frac, double goal_frac, int n_clean_min,
+               int reorder);
+void o_info(const otree *t, int *out8);
+void o_get_boxes(const otree *t, int n, int *lvl, int *ix, int *parent,
+                 int *children, int *neighbors, int *in_use, int *n_cell,
+                 double *dr, double *r_min);
+void o_level_counts(const otree *t, int lvl, int *counts3);
+void o_level_lists(const otree *t, int lvl, int *ids, int *leaves,
+                   int *parents);
+int o_ref_info_lvl(const otree *t, int lvl, int *add, int *rm, int *n_add,
+                   int *n_rm);
+
+void o_set_cc(otree *t, int id, int iv, const double *data);
+void o_get_cc(const otree *t, int id, int iv, double *data);
+void o_set_fc(otree *t, int id, int dim, int ifc, const double *data);
+void o_get_fc(const otree *t, int id, int dim, int ifc, double *data);
+
+/* Multigrid options (mg$D_t, m_aX_mg.f90:20-52). bc_kind per physical face:
+ * 0 = Neumann (copy), 1 = Dirichlet zero (-c), 2 = value (2V - c). */
+void o_mg_setup(otree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
+                int n_down, int n_up, int n_base, const int *bc_kind,
+                const double *bc_val, int auto_ops);
+void o_fas_fmg(otree *t, int set_residual, int first_call);
+void o_fas_vcycle(otree *t, int max_lvl, int set_residual);
+void o_box_gsrb(otree *t, int id, int cntr);
+void o_box_op(otree *t, int id, int i_out);
+void o_box_rstr(otree *t, int id, int iv);
+void o_box_corr(otree *t, int id);
+/* rb: 0 = mg sides_rb, 1 = gc_interp, 2 = gc_interp_lim;
+ * bc: 0 = mg boundary rule, 1 = Neumann, 2 = Dirichlet zero */
+void o_gc_box(otree *t, int id, int iv, int rb, int bc);
+void o_gc_tree(otree *t, int iv, int rb, int bc);
+void o_gc2_box(otree *t, int id, int iv, double *out);
+void o_restrict_tree(otree *t, int iv);
+void o_consistent_fluxes(otree *t, int n, const int *f_ixs);
+void o_prolong1_to(otree *t, int id, int iv, int add);
+void o_prolong0_to(otree *t, int id, int iv);
+void o_tree_copy_cc(otree *t, int iv_from, int iv_to);
+void o_tree_minmax_cc(const otree *t, int iv, double *vmin, double *vmax);
+double o_min_dr(const otree *t);
+/* max|rhs - A phi| over leaves (uses i_tmp as scratch, like residual_box) */
+double o_max_residual(otree *t);
+
+/* Koren-limited flux limiter, src/m_streamer.f90:339-362 */
+double o_koren_mlim(double a, double b);
+
+#endif

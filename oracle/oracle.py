@@ -1,0 +1,236 @@
+"""ctypes view of oracle/liba5oracle.so, the C restatement of the a5_streamer
+hot path (tree core, ghost cells, multigrid, streamer physics).
+
+Test infrastructure only: tests, the smoke check and the CPU-baseline leg of
+bench.py use it as the checker. The API mirrors oracle/refharness.py so the
+same scenario can be run on the reference, the restatement and the HIP path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liba5oracle.so")
+
+_lib = None
+
+ip = C.POINTER(C.c_int)
+dp = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "liba5oracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liba5oracle.so is not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.o_tree_new.restype = C.c_void_p
+        L.o_tree_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                 dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.o_tree_free.argtypes = [C.c_void_p]
+        L.o_min_dr.restype = C.c_double
+        L.o_min_dr.argtypes = [C.c_void_p]
+        L.o_max_residual.restype = C.c_double
+        L.o_max_residual.argtypes = [C.c_void_p]
+        L.o_koren_mlim.restype = C.c_double
+        L.o_koren_mlim.argtypes = [C.c_double, C.c_double]
+        for name in dir(L):
+            pass
+        _lib = L
+    return _lib
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ip)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+class OTree:
+    """A tree of the C restatement; same methods as refharness.RefTree."""
+
+    def __init__(self, dim, n_cell, n_cc, n_fc, dr, r_min=None, coarsen_to=-1,
+                 lvls_max=30, coord=0, n_boxes=100):
+        self.dim = dim
+        self.n_cell = n_cell
+        self.L = lib()
+        r_min = np.zeros(dim) if r_min is None else np.asarray(r_min, float)
+        self._rm = _d(r_min)[0]
+        self.h = C.c_void_p(self.L.o_tree_new(dim, n_cell, n_cc, n_fc, dr,
+                                              self._rm.ctypes.data_as(dp),
+                                              coarsen_to, lvls_max, coord,
+                                              n_boxes))
+
+    def __del__(self):
+        try:
+            self.L.o_tree_free(self.h)
+        except Exception:
+            pass
+
+    def set_base(self, ix_list, nb_list):
+        ix, pix = _i(np.asarray(ix_list))
+        nb, pnb = _i(np.asarray(nb_list))
+        self.L.o_set_base(self.h, C.c_int(ix.shape[0]), pix, pnb)
+
+    def adjust(self, flags_by_id):
+        fl, pfl = _i(flags_by_id)
+        a, r = C.c_int(), C.c_int()
+        self.L.o_adjust(self.h, C.c_int(len(fl)), pfl, C.byref(a), C.byref(r))
+        return a.value, r.value
+
+    def ref_info_lvl(self, lvl):
+        a, r = C.c_int(), C.c_int()
+        self.L.o_ref_info_lvl(self.h, C.c_int(lvl), None, None, C.byref(a),
+                              C.byref(r))
+        add = np.zeros(max(a.value, 1), np.int32)
+        rm = np.zeros(max(r.value, 1), np.int32)
+        self.L.o_ref_info_lvl(self.h, C.c_int(lvl), _i(add)[1], _i(rm)[1],
+                              C.byref(a), C.byref(r))
+        return add[:a.value], rm[:r.value]
+
+    def tidy_up(self, max_frac, goal_frac, n_clean_min, reorder):
+        self.L.o_tidy_up(self.h, C.c_double(max_frac), C.c_double(goal_frac),
+                         C.c_int(n_clean_min), C.c_int(int(reorder)))
+
+    def info(self):
+        out = np.zeros(8, np.int32)
+        self.L.o_info(self.h, _i(out)[1])
+        keys = ("max_id", "max_lvl", "min_lvl", "lvls_max", "n_storage",
+                "n_cell", "coord", "ready")
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def boxes(self):
+        n = self.info()["max_id"]
+        D, nch, nnb = self.dim, 2 ** self.dim, 2 * self.dim
+        out = dict(lvl=np.zeros(n, np.int32), ix=np.zeros((n, D), np.int32),
+                   parent=np.zeros(n, np.int32),
+                   children=np.zeros((n, nch), np.int32),
+                   neighbors=np.zeros((n, nnb), np.int32),
+                   in_use=np.zeros(n, np.int32), n_cell=np.zeros(n, np.int32),
+                   dr=np.zeros(n), r_min=np.zeros((n, D)))
+        ptrs = [out[k].ctypes.data_as(ip) for k in
+                ("lvl", "ix", "parent", "children", "neighbors", "in_use",
+                 "n_cell")]
+        ptrs += [out[k].ctypes.data_as(dp) for k in ("dr", "r_min")]
+        self.L.o_get_boxes(self.h, C.c_int(n), *ptrs)
+        return out
+
+    def level(self, lvl):
+        cnt = np.zeros(3, np.int32)
+        self.L.o_level_counts(self.h, C.c_int(lvl), _i(cnt)[1])
+        ids = np.zeros(max(cnt[0], 1), np.int32)
+        lv = np.zeros(max(cnt[1], 1), np.int32)
+        pa = np.zeros(max(cnt[2], 1), np.int32)
+        if cnt[0] or cnt[1] or cnt[2]:
+            self.L.o_level_lists(self.h, C.c_int(lvl), _i(ids)[1], _i(lv)[1],
+                                 _i(pa)[1])
+        return ids[:cnt[0]], lv[:cnt[1]], pa[:cnt[2]]
+
+    def box_len(self, nc):
+        return (nc + 2) ** self.dim
+
+    def set_cc(self, id_, iv, data):
+        d, pd = _d(data)
+        self.L.o_set_cc(self.h, C.c_int(id_), C.c_int(iv), pd)
+
+    def get_cc(self, id_, iv, nc=None):
+        nc = self.n_cell if nc is None else nc
+        out = np.zeros(self.box_len(nc))
+        self.L.o_get_cc(self.h, C.c_int(id_), C.c_int(iv), _d(out)[1])
+        return out
+
+    def set_fc(self, id_, dim, ifc, data):
+        d, pd = _d(data)
+        self.L.o_set_fc(self.h, C.c_int(id_), C.c_int(dim), C.c_int(ifc), pd)
+
+    def get_fc(self, id_, dim, ifc):
+        nc = self.n_cell
+        out = np.zeros((nc + 1) * nc ** (self.dim - 1))
+        self.L.o_get_fc(self.h, C.c_int(id_), C.c_int(dim), C.c_int(ifc),
+                        _d(out)[1])
+        return out
+
+    def mg_setup(self, i_phi, i_tmp, i_rhs, i_eps=-1, n_down=2, n_up=2,
+                 n_base=2, bc_kind=None, bc_val=None, auto_ops=False):
+        nnb = 2 * self.dim
+        k = np.zeros(nnb, np.int32) if bc_kind is None else bc_kind
+        v = np.zeros(nnb) if bc_val is None else bc_val
+        ka, pk = _i(k)
+        va, pv = _d(v)
+        self.L.o_mg_setup(self.h, C.c_int(i_phi), C.c_int(i_tmp),
+                          C.c_int(i_rhs), C.c_int(i_eps), C.c_int(n_down),
+                          C.c_int(n_up), C.c_int(n_base), pk, pv,
+                          C.c_int(int(auto_ops)))
+
+    def fas_fmg(self, set_residual, first_call):
+        self.L.o_fas_fmg(self.h, C.c_int(int(set_residual)),
+                         C.c_int(int(first_call)))
+
+    def fas_vcycle(self, max_lvl, set_residual):
+        self.L.o_fas_vcycle(self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
+
+    def box_gsrb(self, id_, cntr):
+        self.L.o_box_gsrb(self.h, C.c_int(id_), C.c_int(cntr))
+
+    def box_op(self, id_, i_out):
+        self.L.o_box_op(self.h, C.c_int(id_), C.c_int(i_out))
+
+    def box_rstr(self, id_, iv):
+        self.L.o_box_rstr(self.h, C.c_int(id_), C.c_int(iv))
+
+    def box_corr(self, id_):
+        self.L.o_box_corr(self.h, C.c_int(id_))
+
+    def gc_box(self, id_, iv, rb, bc):
+        self.L.o_gc_box(self.h, C.c_int(id_), C.c_int(iv), C.c_int(rb),
+                        C.c_int(bc))
+
+    def gc_tree(self, iv, rb, bc):
+        self.L.o_gc_tree(self.h, C.c_int(iv), C.c_int(rb), C.c_int(bc))
+
+    def gc2_box(self, id_, iv):
+        nc = self.n_cell
+        out = np.zeros(nc ** (self.dim - 1) * 2 * self.dim)
+        self.L.o_gc2_box(self.h, C.c_int(id_), C.c_int(iv), _d(out)[1])
+        return out
+
+    def restrict_tree(self, iv):
+        self.L.o_restrict_tree(self.h, C.c_int(iv))
+
+    def consistent_fluxes(self, f_ixs):
+        a, p = _i(f_ixs)
+        self.L.o_consistent_fluxes(self.h, C.c_int(len(a)), p)
+
+    def prolong1_to(self, id_, iv, add=False):
+        self.L.o_prolong1_to(self.h, C.c_int(id_), C.c_int(iv), C.c_int(int(add)))
+
+    def prolong0_to(self, id_, iv):
+        self.L.o_prolong0_to(self.h, C.c_int(id_), C.c_int(iv))
+
+    def tree_copy_cc(self, a, b):
+        self.L.o_tree_copy_cc(self.h, C.c_int(a), C.c_int(b))
+
+    def tree_minmax(self, iv):
+        lo, hi = C.c_double(), C.c_double()
+        self.L.o_tree_minmax_cc(self.h, C.c_int(iv), C.byref(lo), C.byref(hi))
+        return lo.value, hi.value
+
+    def min_dr(self):
+        return self.L.o_min_dr(self.h)
+
+    def max_residual(self):
+        return self.L.o_max_residual(self.h)
+
+
+def koren_mlim(a, b):
+    return lib().o_koren_mlim(a, b)

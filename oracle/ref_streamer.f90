@@ -1,0 +1,111 @@
+! Test-only harness around the reference's streamer/fosito modules
+! (m_streamer, m_lookup_table, m_transport_data, m_morton), compiled by
+! oracle/Makefile into oracle/_ref/. Exposes C-callable entry points for the
+! Koren limiter, the uniform lookup table, the transport-data text reader and
+! Morton ordering. Nothing in the product links or loads this.
+
+module ref_streamer
+  use iso_c_binding
+  use m_lookup_table
+  use m_morton
+  use m_streamer, only: koren_mlim
+  implicit none
+  private
+
+  integer, parameter :: dp = kind(0.0d0)
+  type(LT_table_t), save :: tbl
+
+contains
+
+  function h_koren(a, b) result(r) bind(C, name="ref_koren_mlim")
+    real(c_double), value :: a, b
+    real(c_double)        :: r
+    r = koren_mlim(a, b)
+  end function h_koren
+
+  subroutine h_lt_create(x_min, x_max, n_rows, n_cols) bind(C, name="ref_lt_create")
+    real(c_double), value :: x_min, x_max
+    integer(c_int), value :: n_rows, n_cols
+    tbl = LT_create(x_min, x_max, n_rows, n_cols)
+  end subroutine h_lt_create
+
+  subroutine h_lt_set_col(col, n, xx, yy) bind(C, name="ref_lt_set_col")
+    integer(c_int), value :: col, n
+    real(c_double)        :: xx(n), yy(n)
+    call LT_set_col(tbl, col, xx, yy)
+  end subroutine h_lt_set_col
+
+  function h_lt_get_col(col, x) result(y) bind(C, name="ref_lt_get_col")
+    integer(c_int), value :: col
+    real(c_double), value :: x
+    real(c_double)        :: y
+    y = LT_get_col(tbl, col, x)
+  end function h_lt_get_col
+
+  subroutine h_lt_get_data(n_rows, n_cols, x_data, cols_rows) bind(C, name="ref_lt_get_data")
+    integer(c_int), value :: n_rows, n_cols
+    real(c_double)        :: x_data(n_rows), cols_rows(n_cols, n_rows)
+    call LT_get_data(tbl, x_data, cols_rows)
+  end subroutine h_lt_get_data
+
+  subroutine h_lin_interp(n, xx, yy, x, y) bind(C, name="ref_lin_interp_list")
+    integer(c_int), value       :: n
+    real(c_double)              :: xx(n), yy(n)
+    real(c_double), value       :: x
+    real(c_double), intent(out) :: y
+    call LT_lin_interp_list(xx, yy, x, y)
+  end subroutine h_lin_interp
+
+  ! Reads one transport-data entry; n_out is the number of rows found
+  subroutine h_td_read(fname, gname, dname, max_n, n_out, xx, yy) &
+       bind(C, name="ref_td_read")
+    use m_transport_data
+    character(kind=c_char) :: fname(*), gname(*), dname(*)
+    integer(c_int), value       :: max_n
+    integer(c_int), intent(out) :: n_out
+    real(c_double)              :: xx(max_n), yy(max_n)
+    real(dp), allocatable       :: x_data(:), y_data(:)
+    call TD_get_td_from_file(c2f(fname), c2f(gname), c2f(dname), x_data, y_data)
+    n_out = min(size(x_data), max_n)
+    xx(1:n_out) = x_data(1:n_out)
+    yy(1:n_out) = y_data(1:n_out)
+  end subroutine h_td_read
+
+  function c2f(s) result(f)
+    character(kind=c_char) :: s(*)
+    character(len=:), allocatable :: f
+    integer :: n, i
+    n = 0
+    do while (s(n+1) /= c_null_char)
+       n = n + 1
+    end do
+    allocate(character(len=n) :: f)
+    do i = 1, n
+       f(i:i) = s(i)
+    end do
+  end function c2f
+
+  function h_morton2(i, j) result(m) bind(C, name="ref_morton_from_ix2")
+    integer(c_int), value :: i, j
+    integer(c_int64_t)    :: m
+    m = morton_from_ix2([i, j])
+  end function h_morton2
+
+  function h_morton3(i, j, k) result(m) bind(C, name="ref_morton_from_ix3")
+    integer(c_int), value :: i, j, k
+    integer(c_int64_t)    :: m
+    m = morton_from_ix3([i, j, k])
+  end function h_morton3
+
+  subroutine h_morton_rank(n, mortons, ranks) bind(C, name="ref_morton_rank")
+    integer(c_int), value :: n
+    integer(c_int64_t)    :: mortons(n)
+    integer(c_int)        :: ranks(n)
+    integer(morton_k)     :: m(n)
+    integer               :: r(n)
+    m = mortons
+    call morton_rank(m, r)
+    ranks = r
+  end subroutine h_morton_rank
+
+end module ref_streamer
