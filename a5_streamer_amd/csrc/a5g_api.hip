@@ -1,0 +1,597 @@
+// GPU box pool management and the C ABI of the a5g library (include/a5g.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/a5g.h"
+#include "a5g_internal.hpp"
+#include "a5g_kernels.hpp"
+
+namespace a5g {
+
+static void hcheck(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int Pool::bs() const {
+  int n = tree.n_cell + 2;
+  return tree.dim == 2 ? n * n : n * n * n;
+}
+
+int Pool::fs() const {
+  int nc = tree.n_cell;
+  return tree.dim == 2 ? (nc + 1) * nc : (nc + 1) * nc * nc;
+}
+
+Dev Pool::dev() {
+  Dev a;
+  a.cc = cc;
+  a.fc = fc;
+  a.bs = bs();
+  a.fs = fs();
+  a.vs = (size_t)(dev_cap + 1) * a.bs;
+  a.fvs = (size_t)(dev_cap + 1) * tree.dim * a.fs;
+  a.coord = tree.coord;
+  a.lvl = d_lvl;
+  a.ix = d_ix;
+  a.parent = d_parent;
+  a.children = d_children;
+  a.nbs = d_nbs;
+  a.ncell = d_ncell;
+  a.dr = d_dr;
+  a.rmin = d_rmin;
+  for (int i = 0; i < 6; i++) {
+    a.bc_kind[i] = bc_kind[i];
+    a.bc_val[i] = bc_val[i];
+  }
+  return a;
+}
+
+template <typename T>
+static void realloc_dev(T** p, size_t n) {
+  if (*p) hcheck(hipFree(*p), "hipFree");
+  *p = nullptr;
+  hcheck(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+}
+
+void Pool::ensure_capacity(int need) {
+  if (host_only) return;
+  if (need <= dev_cap && cc) return;
+  int ncap = std::max(need, dev_cap + dev_cap / 2);
+  const size_t b = bs(), f = fs();
+  const int ncc = tree.n_cc, nfc = std::max(tree.n_fc, 1), D = tree.dim;
+  double* ncc_p = nullptr;
+  double* nfc_p = nullptr;
+  hcheck(hipMalloc((void**)&ncc_p, (size_t)ncc * (ncap + 1) * b * sizeof(double)), "hipMalloc cc");
+  hcheck(hipMalloc((void**)&nfc_p, (size_t)nfc * (ncap + 1) * D * f * sizeof(double)), "hipMalloc fc");
+  hcheck(hipMemsetAsync(ncc_p, 0, (size_t)ncc * (ncap + 1) * b * sizeof(double), stream), "memset");
+  hcheck(hipMemsetAsync(nfc_p, 0, (size_t)nfc * (ncap + 1) * D * f * sizeof(double), stream), "memset");
+  if (cc) {
+    for (int v = 0; v < ncc; v++)
+      hcheck(hipMemcpyAsync(ncc_p + (size_t)v * (ncap + 1) * b, cc + (size_t)v * (dev_cap + 1) * b,
+                            (size_t)(dev_cap + 1) * b * sizeof(double), hipMemcpyDeviceToDevice,
+                            stream), "copy cc");
+    for (int v = 0; v < nfc; v++)
+      hcheck(hipMemcpyAsync(nfc_p + (size_t)v * (ncap + 1) * D * f,
+                            fc + (size_t)v * (dev_cap + 1) * D * f,
+                            (size_t)(dev_cap + 1) * D * f * sizeof(double),
+                            hipMemcpyDeviceToDevice, stream), "copy fc");
+    hcheck(hipStreamSynchronize(stream), "sync");
+    hcheck(hipFree(cc), "free");
+    hcheck(hipFree(fc), "free");
+  }
+  cc = ncc_p;
+  fc = nfc_p;
+  dev_cap = ncap;
+  realloc_dev(&d_lvl, ncap + 1);
+  realloc_dev(&d_ix, (size_t)(ncap + 1) * 3);
+  realloc_dev(&d_parent, ncap + 1);
+  realloc_dev(&d_children, (size_t)(ncap + 1) * 8);
+  realloc_dev(&d_nbs, (size_t)(ncap + 1) * 6);
+  realloc_dev(&d_ncell, ncap + 1);
+  realloc_dev(&d_dr, ncap + 1);
+  realloc_dev(&d_rmin, (size_t)(ncap + 1) * 3);
+  topo_dirty = true;
+}
+
+void Pool::sync_topology() {
+  if (host_only) throw std::runtime_error("no GPU visible: cell data and kernels are unavailable");
+  if (!topo_dirty) return;
+  ensure_capacity(std::max(tree.max_id, tree.cap));
+  const int n = tree.max_id + 1;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    hcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "topology upload");
+  };
+  up(d_lvl, tree.lvl.data(), n * sizeof(int));
+  up(d_ix, tree.ix.data(), (size_t)n * 3 * sizeof(int));
+  up(d_parent, tree.parent.data(), n * sizeof(int));
+  up(d_children, tree.children.data(), (size_t)n * 8 * sizeof(int));
+  up(d_nbs, tree.nbs.data(), (size_t)n * 6 * sizeof(int));
+  up(d_ncell, tree.ncell.data(), n * sizeof(int));
+  up(d_dr, tree.dr.data(), n * sizeof(double));
+  up(d_rmin, tree.rmin.data(), (size_t)n * 3 * sizeof(double));
+  // level lists
+  const int nl = (int)tree.ids.size();
+  off_ids.assign(nl, 0);
+  off_leaves.assign(nl, 0);
+  off_parents.assign(nl, 0);
+  static thread_local std::vector<int> host;
+  host.clear();
+  for (int l = 0; l < nl; l++) {
+    off_ids[l] = host.size();
+    host.insert(host.end(), tree.ids[l].begin(), tree.ids[l].end());
+    off_leaves[l] = host.size();
+    host.insert(host.end(), tree.leaves[l].begin(), tree.leaves[l].end());
+    off_parents[l] = host.size();
+    host.insert(host.end(), tree.parents[l].begin(), tree.parents[l].end());
+  }
+  if (host.size() > lists_cap || !d_lists) {
+    lists_cap = std::max(host.size(), lists_cap * 3 / 2) + 64;
+    realloc_dev(&d_lists, lists_cap);
+  }
+  if (!host.empty()) up(d_lists, host.data(), host.size() * sizeof(int));
+  hcheck(hipStreamSynchronize(stream), "topology sync");
+  topo_dirty = false;
+}
+
+// new slot n <- old slot map[n] for every variable
+__global__ void k_gather(const double* __restrict__ src, double* __restrict__ dst,
+                         const int* __restrict__ map, int n_used, size_t per_box,
+                         size_t old_vs, size_t new_vs, int n_var) {
+  const size_t total = (size_t)n_used * per_box;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int nn = (int)(q / per_box) + 1;
+    const size_t w = q % per_box;
+    const int o = map[nn];
+    for (int v = 0; v < n_var; v++)
+      dst[(size_t)v * new_vs + (size_t)nn * per_box + w] =
+          src[(size_t)v * old_vs + (size_t)o * per_box + w];
+  }
+}
+
+void Pool::apply_moves(const std::vector<int>& moves, int n_used) {
+  if (host_only) return;
+  const int ncap = std::max(dev_cap, std::max(tree.cap, n_used));
+  const size_t b = bs(), f = (size_t)fs() * tree.dim;
+  const int ncc = tree.n_cc, nfc = std::max(tree.n_fc, 1);
+  int* d_map = nullptr;
+  realloc_dev(&d_map, moves.size());
+  hcheck(hipMemcpyAsync(d_map, moves.data(), moves.size() * sizeof(int), hipMemcpyHostToDevice,
+                        stream), "moves");
+  double *ncc_p = nullptr, *nfc_p = nullptr;
+  hcheck(hipMalloc((void**)&ncc_p, (size_t)ncc * (ncap + 1) * b * sizeof(double)), "hipMalloc");
+  hcheck(hipMalloc((void**)&nfc_p, (size_t)nfc * (ncap + 1) * f * sizeof(double)), "hipMalloc");
+  hcheck(hipMemsetAsync(ncc_p, 0, (size_t)ncc * (ncap + 1) * b * sizeof(double), stream), "memset");
+  hcheck(hipMemsetAsync(nfc_p, 0, (size_t)nfc * (ncap + 1) * f * sizeof(double), stream), "memset");
+  if (n_used > 0) {
+    hipLaunchKernelGGL(k_gather, dim3(1024), dim3(256), 0, stream, cc, ncc_p, d_map, n_used, b,
+                       (size_t)(dev_cap + 1) * b, (size_t)(ncap + 1) * b, ncc);
+    hipLaunchKernelGGL(k_gather, dim3(1024), dim3(256), 0, stream, fc, nfc_p, d_map, n_used, f,
+                       (size_t)(dev_cap + 1) * f, (size_t)(ncap + 1) * f, nfc);
+    hcheck(hipGetLastError(), "gather");
+  }
+  hcheck(hipStreamSynchronize(stream), "gather sync");
+  hcheck(hipFree(cc), "free");
+  hcheck(hipFree(fc), "free");
+  hcheck(hipFree(d_map), "free");
+  cc = ncc_p;
+  fc = nfc_p;
+  if (ncap != dev_cap) {
+    dev_cap = ncap;
+    realloc_dev(&d_lvl, ncap + 1);
+    realloc_dev(&d_ix, (size_t)(ncap + 1) * 3);
+    realloc_dev(&d_parent, ncap + 1);
+    realloc_dev(&d_children, (size_t)(ncap + 1) * 8);
+    realloc_dev(&d_nbs, (size_t)(ncap + 1) * 6);
+    realloc_dev(&d_ncell, ncap + 1);
+    realloc_dev(&d_dr, ncap + 1);
+    realloc_dev(&d_rmin, (size_t)(ncap + 1) * 3);
+  }
+  topo_dirty = true;
+}
+
+void Pool::release() {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
+  f(d_dr); f(d_rmin); f(d_lists); f(d_red);
+  if (h_red) (void)hipHostFree(h_red);
+  if (stream) (void)hipStreamDestroy(stream);
+  cc = fc = nullptr;
+  stream = nullptr;
+}
+
+}  // namespace a5g
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+struct a5g_tree {
+  a5g::Pool P;
+};
+
+static double* a5g_facep_host(const a5g::Dev& a, int ifc, int id, int d, int dim) {
+  return a.fc + (size_t)(ifc - 1) * a.fvs + ((size_t)id * dim + d) * a.fs;
+}
+
+static thread_local std::string g_err;
+
+#define API_BEGIN \
+  try {           \
+    g_err.clear();
+#define API_END                    \
+  return 0;                        \
+  }                                \
+  catch (const std::exception& e) { \
+    g_err = e.what();              \
+    return 1;                      \
+  }                                \
+  catch (...) {                    \
+    g_err = "unknown error";       \
+    return 1;                      \
+  }
+
+#define NEED(t) \
+  if (!(t)) throw std::invalid_argument("null a5g_tree handle");
+
+static void check_id(const a5g::Tree& T, int id) {
+  if (id < 1 || id > T.max_id || !T.in_use[id])
+    throw std::out_of_range("box id " + std::to_string(id) + " is not in use");
+}
+static void need_gpu(const a5g::Pool& P) {
+  if (P.host_only) throw std::runtime_error("no GPU visible: cell data and kernels are unavailable");
+}
+static void check_iv(const a5g::Tree& T, int iv) {
+  if (iv < 1 || iv > T.n_cc)
+    throw std::out_of_range("cell variable " + std::to_string(iv) + " out of range");
+}
+
+extern "C" {
+
+const char* a5g_last_error(void) { return g_err.c_str(); }
+
+int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, double dr,
+             const double* r_min, int coarsen_to, int lvls_max, int coord, int n_boxes) {
+  API_BEGIN
+  if (!t) throw std::invalid_argument("null output handle");
+  *t = nullptr;
+  auto* h = new a5g_tree();
+  try {
+    h->P.tree.init(dim, n_cell, n_var_cell, n_var_face, dr, r_min, coarsen_to,
+                   lvls_max > 0 ? lvls_max : 30, coord, n_boxes > 0 ? n_boxes : 100);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      (void)hipGetLastError();
+      h->P.host_only = true;
+    } else {
+      a5g::hcheck(hipStreamCreateWithFlags(&h->P.stream, hipStreamNonBlocking), "stream");
+      a5g::hcheck(hipMalloc((void**)&h->P.d_red, 64), "hipMalloc");
+      h->P.ensure_capacity(h->P.tree.cap);
+    }
+  } catch (...) {
+    h->P.release();
+    delete h;
+    throw;
+  }
+  *t = h;
+  API_END
+}
+
+int a5g_destroy(a5g_tree* t) {
+  API_BEGIN
+  NEED(t);
+  t->P.release();
+  delete t;
+  API_END
+}
+
+int a5g_set_base(a5g_tree* t, int n, const int* ix_list, const int* nb_list) {
+  API_BEGIN
+  NEED(t);
+  t->P.tree.set_base(n, ix_list, nb_list);
+  t->P.ensure_capacity(t->P.tree.cap);
+  t->P.topo_dirty = true;
+  API_END
+}
+
+int a5g_adjust_refinement(a5g_tree* t, int n_flags, const int* flags, int* n_add, int* n_rm) {
+  API_BEGIN
+  NEED(t);
+  int a = 0, r = 0;
+  t->P.tree.adjust_refinement(n_flags, flags, &a, &r);
+  t->P.ensure_capacity(std::max(t->P.tree.cap, t->P.tree.max_id));
+  t->P.topo_dirty = true;
+  if (n_add) *n_add = a;
+  if (n_rm) *n_rm = r;
+  API_END
+}
+
+int a5g_ref_info_lvl(a5g_tree* t, int lvl, int* add, int* rm, int* n_add, int* n_rm) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  int na = 0, nr = 0;
+  if (lvl >= 1 && lvl <= (int)T.ri_add.size()) {
+    na = (int)T.ri_add[lvl - 1].size();
+    nr = (int)T.ri_rm[lvl - 1].size();
+    if (add) std::copy(T.ri_add[lvl - 1].begin(), T.ri_add[lvl - 1].end(), add);
+    if (rm) std::copy(T.ri_rm[lvl - 1].begin(), T.ri_rm[lvl - 1].end(), rm);
+  }
+  if (n_add) *n_add = na;
+  if (n_rm) *n_rm = nr;
+  API_END
+}
+
+int a5g_tidy_up(a5g_tree* t, double max_frac, double goal_frac, int n_clean_min, int reorder) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (P.tree.tidy_up(max_frac, goal_frac, n_clean_min, reorder != 0)) {
+    P.apply_moves(P.tree.last_moves, P.tree.max_id);
+    P.topo_dirty = true;
+  }
+  API_END
+}
+
+int a5g_info(a5g_tree* t, int* out) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  out[0] = T.max_id;
+  out[1] = T.max_lvl;
+  out[2] = T.min_lvl;
+  out[3] = T.lvls_max;
+  out[4] = T.cap;
+  out[5] = T.n_cell;
+  out[6] = T.coord;
+  out[7] = T.ready ? 1 : 0;
+  API_END
+}
+
+int a5g_get_boxes(a5g_tree* t, int n, int* lvl, int* ix, int* parent, int* children,
+                  int* neighbors, int* in_use, int* n_cell, double* dr, double* r_min) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  const int D = T.dim, nch = T.nch, nnb = T.nnb;
+  for (int id = 1; id <= n; id++) {
+    const bool u = id <= T.max_id && T.in_use[id];
+    in_use[id - 1] = u;
+    lvl[id - 1] = u ? T.lvl[id] : 0;
+    parent[id - 1] = u ? T.parent[id] : 0;
+    n_cell[id - 1] = u ? T.ncell[id] : 0;
+    dr[id - 1] = u ? T.dr[id] : 0;
+    for (int d = 0; d < D; d++) {
+      ix[(id - 1) * D + d] = u ? T.ix[id * 3 + d] : 0;
+      r_min[(id - 1) * D + d] = u ? T.rmin[id * 3 + d] : 0;
+    }
+    for (int c = 0; c < nch; c++) children[(id - 1) * nch + c] = u ? T.children[id * 8 + c] : 0;
+    for (int b = 0; b < nnb; b++) neighbors[(id - 1) * nnb + b] = u ? T.nbs[id * 6 + b] : 0;
+  }
+  API_END
+}
+
+int a5g_level_counts(a5g_tree* t, int lvl, int* c) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  c[0] = c[1] = c[2] = 0;
+  if (lvl >= T.min_lvl && lvl <= T.lvls_max + 1) {
+    c[0] = (int)T.lv_ids(lvl).size();
+    c[1] = (int)T.lv_leaves(lvl).size();
+    c[2] = (int)T.lv_parents(lvl).size();
+  }
+  API_END
+}
+
+int a5g_level_lists(a5g_tree* t, int lvl, int* ids, int* leaves, int* parents) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  if (lvl < T.min_lvl || lvl > T.lvls_max + 1) throw std::out_of_range("level out of range");
+  if (ids) std::copy(T.lv_ids(lvl).begin(), T.lv_ids(lvl).end(), ids);
+  if (leaves) std::copy(T.lv_leaves(lvl).begin(), T.lv_leaves(lvl).end(), leaves);
+  if (parents) std::copy(T.lv_parents(lvl).begin(), T.lv_parents(lvl).end(), parents);
+  API_END
+}
+
+int a5g_set_cc(a5g_tree* t, int id, int iv, const double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_id(P.tree, id);
+  check_iv(P.tree, iv);
+  P.ensure_capacity(P.tree.max_id);
+  a5g::Dev a = P.dev();
+  double* dst = a.cc + (size_t)(iv - 1) * a.vs + (size_t)id * a.bs;
+  a5g::hcheck(hipMemcpyAsync(dst, data, P.tree.box_len(id) * sizeof(double),
+                             hipMemcpyHostToDevice, P.stream), "set_cc");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "set_cc sync");
+  API_END
+}
+
+int a5g_get_cc(a5g_tree* t, int id, int iv, double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_id(P.tree, id);
+  check_iv(P.tree, iv);
+  a5g::Dev a = P.dev();
+  const double* src = a.cc + (size_t)(iv - 1) * a.vs + (size_t)id * a.bs;
+  a5g::hcheck(hipMemcpyAsync(data, src, P.tree.box_len(id) * sizeof(double),
+                             hipMemcpyDeviceToHost, P.stream), "get_cc");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "get_cc sync");
+  API_END
+}
+
+int a5g_set_cc_all(a5g_tree* t, int iv, const double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_iv(P.tree, iv);
+  P.ensure_capacity(P.tree.max_id);
+  a5g::Dev a = P.dev();
+  a5g::hcheck(hipMemcpyAsync(a.cc + (size_t)(iv - 1) * a.vs + a.bs, data,
+                             (size_t)P.tree.max_id * a.bs * sizeof(double),
+                             hipMemcpyHostToDevice, P.stream), "set_cc_all");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "set_cc_all sync");
+  API_END
+}
+
+int a5g_get_cc_all(a5g_tree* t, int iv, double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_iv(P.tree, iv);
+  a5g::Dev a = P.dev();
+  a5g::hcheck(hipMemcpyAsync(data, a.cc + (size_t)(iv - 1) * a.vs + a.bs,
+                             (size_t)P.tree.max_id * a.bs * sizeof(double),
+                             hipMemcpyDeviceToHost, P.stream), "get_cc_all");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "get_cc_all sync");
+  API_END
+}
+
+int a5g_set_fc(a5g_tree* t, int id, int dim, int ifc, const double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_id(P.tree, id);
+  if (ifc < 1 || ifc > P.tree.n_fc || dim < 1 || dim > P.tree.dim)
+    throw std::out_of_range("face variable out of range");
+  a5g::Dev a = P.dev();
+  a5g::hcheck(hipMemcpyAsync(a5g_facep_host(a, ifc, id, dim - 1, P.tree.dim), data,
+                             a.fs * sizeof(double), hipMemcpyHostToDevice, P.stream), "set_fc");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "set_fc sync");
+  API_END
+}
+
+int a5g_get_fc(a5g_tree* t, int id, int dim, int ifc, double* data) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  check_id(P.tree, id);
+  if (ifc < 1 || ifc > P.tree.n_fc || dim < 1 || dim > P.tree.dim)
+    throw std::out_of_range("face variable out of range");
+  a5g::Dev a = P.dev();
+  a5g::hcheck(hipMemcpyAsync(data, a5g_facep_host(a, ifc, id, dim - 1, P.tree.dim),
+                             a.fs * sizeof(double), hipMemcpyDeviceToHost, P.stream), "get_fc");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "get_fc sync");
+  API_END
+}
+
+int a5g_sync(a5g_tree* t) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::hcheck(hipStreamSynchronize(t->P.stream), "sync");
+  API_END
+}
+
+int a5g_mg_setup(a5g_tree* t, int i_phi, int i_tmp, int i_rhs, int i_eps, int n_down, int n_up,
+                 int n_base, const int* bc_kind, const double* bc_val, int auto_ops) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  // mg$D_init_mg, m_aX_mg.f90:130-154
+  if (i_phi < 0) throw std::invalid_argument("mg_init_mg: i_phi not set");
+  if (i_tmp < 0) throw std::invalid_argument("mg_init_mg: i_tmp not set");
+  if (i_rhs < 0) throw std::invalid_argument("mg_init_mg: i_rhs not set");
+  check_iv(P.tree, i_phi);
+  check_iv(P.tree, i_tmp);
+  check_iv(P.tree, i_rhs);
+  if (i_eps > 0) check_iv(P.tree, i_eps);
+  if (!bc_kind) throw std::invalid_argument("mg_init_mg: sides_bc not set");
+  P.mg.i_phi = i_phi;
+  P.mg.i_tmp = i_tmp;
+  P.mg.i_rhs = i_rhs;
+  P.mg.i_eps = i_eps;
+  P.mg.n_down = n_down < 0 ? 2 : n_down;
+  P.mg.n_up = n_up < 0 ? 2 : n_up;
+  P.mg.n_base = n_base < 0 ? 2 : n_base;
+  P.mg.auto_ops = auto_ops;
+  for (int nb = 0; nb < 2 * P.tree.dim; nb++) {
+    if (bc_kind[nb] < 0 || bc_kind[nb] > 2) throw std::invalid_argument("bad boundary kind");
+    P.bc_kind[nb] = bc_kind[nb];
+    P.bc_val[nb] = bc_val ? bc_val[nb] : 0.0;
+  }
+  P.mg.ready = true;
+  API_END
+}
+
+int a5g_mg_set_bc_value(a5g_tree* t, int nb, double value) {
+  API_BEGIN
+  NEED(t);
+  if (nb < 1 || nb > 2 * t->P.tree.dim) throw std::out_of_range("face index out of range");
+  t->P.bc_val[nb - 1] = value;
+  API_END
+}
+
+int a5g_fas_fmg(a5g_tree* t, int set_residual, int first_call) {
+  API_BEGIN
+  NEED(t);
+  a5g::mg_fmg(t->P, set_residual != 0, first_call != 0);
+  API_END
+}
+
+int a5g_fas_vcycle(a5g_tree* t, int max_lvl, int set_residual) {
+  API_BEGIN
+  NEED(t);
+  a5g::mg_vcycle(t->P, max_lvl, set_residual != 0);
+  API_END
+}
+
+int a5g_mg_residual_norms(a5g_tree* t, double* max_res, double* max_rhs) {
+  API_BEGIN
+  NEED(t);
+  a5g::mg_residual_norms(t->P, max_res, max_rhs);
+  API_END
+}
+
+int a5g_gc_tree(a5g_tree* t, int iv, int rb, int bc) {
+  API_BEGIN
+  NEED(t);
+  check_iv(t->P.tree, iv);
+  if (rb < 0 || rb > 2 || bc < 0 || bc > 2) throw std::invalid_argument("bad ghost rule");
+  a5g::gc_tree(t->P, iv, rb, bc);
+  API_END
+}
+
+int a5g_restrict_tree(a5g_tree* t, int iv) {
+  API_BEGIN
+  NEED(t);
+  check_iv(t->P.tree, iv);
+  a5g::restrict_tree(t->P, iv);
+  API_END
+}
+
+int a5g_tree_copy_cc(a5g_tree* t, int iv_from, int iv_to) {
+  API_BEGIN
+  NEED(t);
+  check_iv(t->P.tree, iv_from);
+  check_iv(t->P.tree, iv_to);
+  a5g::tree_copy_cc(t->P, iv_from, iv_to);
+  API_END
+}
+
+int a5g_tree_minmax_cc(a5g_tree* t, int iv, double* vmin, double* vmax) {
+  API_BEGIN
+  NEED(t);
+  check_iv(t->P.tree, iv);
+  a5g::minmax_boxes(t->P, iv, false, false, true, vmin, vmax);
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// GPU-resident box pool of one a5g tree: cell/face data in HBM, device copies
+// of the topology and level lists, and the multigrid options.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "a5g_tree.hpp"
+
+namespace a5g {
+
+// Everything a kernel needs to address boxes; passed by value.
+struct Dev {
+  double* cc = nullptr;  // [n_cc][cap+1][bs]; box id at slot id
+  double* fc = nullptr;  // [n_fc][cap+1][dim][fs]
+  size_t vs = 0;         // per-variable stride of cc ((cap+1) * bs)
+  size_t fvs = 0;        // per-variable stride of fc ((cap+1) * dim * fs)
+  int bs = 0;            // box stride: (n_cell+2)^dim
+  int fs = 0;            // face-array stride: (n_cell+1) * n_cell^(dim-1)
+  int coord = 0;
+  const int* lvl = nullptr;
+  const int* ix = nullptr;        // [id*3 + d]
+  const int* parent = nullptr;
+  const int* children = nullptr;  // [id*8 + c]
+  const int* nbs = nullptr;       // [id*6 + nb-1]
+  const int* ncell = nullptr;
+  const double* dr = nullptr;
+  const double* rmin = nullptr;   // [id*3 + d]
+  int bc_kind[6] = {0, 0, 0, 0, 0, 0};
+  double bc_val[6] = {0, 0, 0, 0, 0, 0};
+};
+
+struct MgOpts {
+  int i_phi = -1, i_tmp = -1, i_rhs = -1, i_eps = -1;
+  int n_down = 2, n_up = 2, n_base = 2;
+  int auto_ops = 0;
+  bool ready = false;
+};
+
+struct Pool {
+  Tree tree;
+  // No GPU visible: the tree core still works (topology calls), every
+  // call that touches cell data fails.
+  bool host_only = false;
+  hipStream_t stream = nullptr;
+  int dev_cap = 0;  // boxes allocated on the device (slots 0..dev_cap)
+  double* cc = nullptr;
+  double* fc = nullptr;
+  // device topology
+  int *d_lvl = nullptr, *d_ix = nullptr, *d_parent = nullptr, *d_children = nullptr;
+  int *d_nbs = nullptr, *d_ncell = nullptr;
+  double *d_dr = nullptr, *d_rmin = nullptr;
+  int* d_lists = nullptr;  // concatenated level lists
+  size_t lists_cap = 0;
+  // host offsets into d_lists, per level: ids, leaves, parents
+  std::vector<size_t> off_ids, off_leaves, off_parents;
+  bool topo_dirty = true;
+  // reduction scratch
+  double* d_red = nullptr;
+  double* h_red = nullptr;  // pinned
+  MgOpts mg;
+  int bc_kind[6] = {0, 0, 0, 0, 0, 0};
+  double bc_val[6] = {0, 0, 0, 0, 0, 0};
+
+  int bs() const;
+  int fs() const;
+  Dev dev();
+  void ensure_capacity(int cap);  // grows GPU storage, keeping data
+  void sync_topology();           // uploads topology and level lists if dirty
+  void apply_moves(const std::vector<int>& moves, int n_used);  // after tidy_up
+  void release();
+  // level list on the device
+  const int* ids(int l) const { return d_lists + off_ids[l - tree.min_lvl]; }
+  const int* leaves(int l) const { return d_lists + off_leaves[l - tree.min_lvl]; }
+  const int* parents(int l) const { return d_lists + off_parents[l - tree.min_lvl]; }
+  int n_ids(int l) const { return (int)tree.lv_ids(l).size(); }
+  int n_leaves(int l) const { return (int)tree.lv_leaves(l).size(); }
+  int n_parents(int l) const { return (int)tree.lv_parents(l).size(); }
+  int nc_lvl(int l) const {
+    return l >= 1 ? tree.n_cell : tree.n_cell / (1 << (1 - l));
+  }
+};
+
+}  // namespace a5g

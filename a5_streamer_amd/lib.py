@@ -1,0 +1,221 @@
+"""ctypes binding of liba5g.so (include/a5g.h).
+
+Every call checks the returned status and raises A5GError with the
+library's message. Arrays are numpy; box ids and variable indices are
+1-based as in Afivo. There is no fallback: if the library is not built the
+import of Tree methods fails loudly.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(HERE, "liba5g.so")
+_lib = None
+
+ip = C.POINTER(C.c_int)
+dp = C.POINTER(C.c_double)
+
+
+class A5GError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return _LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise A5GError("liba5g.so is not built: run python -m a5_streamer_amd.build")
+        L = C.CDLL(_LIB)
+        L.a5g_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _call(name, *args):
+    L = load()
+    st = getattr(L, name)(*args)
+    if st != 0:
+        raise A5GError("%s: %s" % (name, L.a5g_last_error().decode()))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ip)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(dp)
+
+
+class Tree:
+    """One a5g tree: Afivo topology on the host, cell data in HBM."""
+
+    def __init__(self, dim, n_cell, n_cc, n_fc, dr, r_min=None, coarsen_to=-1,
+                 lvls_max=30, coord=0, n_boxes=100):
+        self.dim = dim
+        self.n_cell = n_cell
+        self.h = C.c_void_p()
+        r_min = np.zeros(dim) if r_min is None else np.asarray(r_min, float)
+        rm, prm = _d(r_min)
+        _call("a5g_init", C.byref(self.h), C.c_int(dim), C.c_int(n_cell),
+              C.c_int(n_cc), C.c_int(n_fc), C.c_double(dr), prm,
+              C.c_int(coarsen_to), C.c_int(lvls_max), C.c_int(coord),
+              C.c_int(n_boxes))
+
+    def close(self):
+        if self.h:
+            _call("a5g_destroy", self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- tree -----------------------------------------------------------
+    def set_base(self, ix_list, nb_list):
+        ix, pix = _i(np.asarray(ix_list))
+        nb, pnb = _i(np.asarray(nb_list))
+        _call("a5g_set_base", self.h, C.c_int(ix.shape[0]), pix, pnb)
+
+    def adjust(self, flags_by_id):
+        fl, pfl = _i(flags_by_id)
+        a, r = C.c_int(), C.c_int()
+        _call("a5g_adjust_refinement", self.h, C.c_int(len(fl)), pfl,
+              C.byref(a), C.byref(r))
+        return a.value, r.value
+
+    def ref_info_lvl(self, lvl):
+        a, r = C.c_int(), C.c_int()
+        _call("a5g_ref_info_lvl", self.h, C.c_int(lvl), None, None, C.byref(a),
+              C.byref(r))
+        add = np.zeros(max(a.value, 1), np.int32)
+        rm = np.zeros(max(r.value, 1), np.int32)
+        _call("a5g_ref_info_lvl", self.h, C.c_int(lvl), _i(add)[1], _i(rm)[1],
+              C.byref(a), C.byref(r))
+        return add[:a.value], rm[:r.value]
+
+    def tidy_up(self, max_frac, goal_frac, n_clean_min, reorder):
+        _call("a5g_tidy_up", self.h, C.c_double(max_frac), C.c_double(goal_frac),
+              C.c_int(n_clean_min), C.c_int(int(reorder)))
+
+    def info(self):
+        out = np.zeros(8, np.int32)
+        _call("a5g_info", self.h, _i(out)[1])
+        keys = ("max_id", "max_lvl", "min_lvl", "lvls_max", "n_storage",
+                "n_cell", "coord", "ready")
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def boxes(self):
+        n = self.info()["max_id"]
+        D, nch, nnb = self.dim, 2 ** self.dim, 2 * self.dim
+        out = dict(lvl=np.zeros(n, np.int32), ix=np.zeros((n, D), np.int32),
+                   parent=np.zeros(n, np.int32),
+                   children=np.zeros((n, nch), np.int32),
+                   neighbors=np.zeros((n, nnb), np.int32),
+                   in_use=np.zeros(n, np.int32), n_cell=np.zeros(n, np.int32),
+                   dr=np.zeros(n), r_min=np.zeros((n, D)))
+        ptrs = [out[k].ctypes.data_as(ip) for k in
+                ("lvl", "ix", "parent", "children", "neighbors", "in_use",
+                 "n_cell")]
+        ptrs += [out[k].ctypes.data_as(dp) for k in ("dr", "r_min")]
+        _call("a5g_get_boxes", self.h, C.c_int(n), *ptrs)
+        return out
+
+    def level(self, lvl):
+        cnt = np.zeros(3, np.int32)
+        _call("a5g_level_counts", self.h, C.c_int(lvl), _i(cnt)[1])
+        ids = np.zeros(max(cnt[0], 1), np.int32)
+        lv = np.zeros(max(cnt[1], 1), np.int32)
+        pa = np.zeros(max(cnt[2], 1), np.int32)
+        if cnt.any():
+            _call("a5g_level_lists", self.h, C.c_int(lvl), _i(ids)[1],
+                  _i(lv)[1], _i(pa)[1])
+        return ids[:cnt[0]], lv[:cnt[1]], pa[:cnt[2]]
+
+    # --- data -----------------------------------------------------------
+    def box_len(self, nc):
+        return (nc + 2) ** self.dim
+
+    def set_cc(self, id_, iv, data):
+        d, pd = _d(data)
+        _call("a5g_set_cc", self.h, C.c_int(id_), C.c_int(iv), pd)
+
+    def get_cc(self, id_, iv, nc=None):
+        nc = self.n_cell if nc is None else nc
+        out = np.zeros(self.box_len(nc))
+        _call("a5g_get_cc", self.h, C.c_int(id_), C.c_int(iv), _d(out)[1])
+        return out
+
+    def set_cc_all(self, iv, data):
+        d, pd = _d(data)
+        _call("a5g_set_cc_all", self.h, C.c_int(iv), pd)
+
+    def get_cc_all(self, iv):
+        n = self.info()["max_id"]
+        out = np.zeros((n, self.box_len(self.n_cell)))
+        _call("a5g_get_cc_all", self.h, C.c_int(iv), _d(out)[1])
+        return out
+
+    def set_fc(self, id_, dim, ifc, data):
+        d, pd = _d(data)
+        _call("a5g_set_fc", self.h, C.c_int(id_), C.c_int(dim), C.c_int(ifc), pd)
+
+    def get_fc(self, id_, dim, ifc):
+        nc = self.n_cell
+        out = np.zeros((nc + 1) * nc ** (self.dim - 1))
+        _call("a5g_get_fc", self.h, C.c_int(id_), C.c_int(dim), C.c_int(ifc),
+              _d(out)[1])
+        return out
+
+    def sync(self):
+        _call("a5g_sync", self.h)
+
+    # --- multigrid and grid ops -----------------------------------------
+    def mg_setup(self, i_phi, i_tmp, i_rhs, i_eps=-1, n_down=2, n_up=2,
+                 n_base=2, bc_kind=None, bc_val=None, auto_ops=False):
+        nnb = 2 * self.dim
+        k = np.zeros(nnb, np.int32) if bc_kind is None else bc_kind
+        v = np.zeros(nnb) if bc_val is None else bc_val
+        ka, pk = _i(k)
+        va, pv = _d(v)
+        _call("a5g_mg_setup", self.h, C.c_int(i_phi), C.c_int(i_tmp),
+              C.c_int(i_rhs), C.c_int(i_eps), C.c_int(n_down), C.c_int(n_up),
+              C.c_int(n_base), pk, pv, C.c_int(int(auto_ops)))
+
+    def mg_set_bc_value(self, nb, value):
+        _call("a5g_mg_set_bc_value", self.h, C.c_int(nb), C.c_double(value))
+
+    def fas_fmg(self, set_residual, first_call):
+        _call("a5g_fas_fmg", self.h, C.c_int(int(set_residual)),
+              C.c_int(int(first_call)))
+
+    def fas_vcycle(self, max_lvl, set_residual):
+        _call("a5g_fas_vcycle", self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
+
+    def residual_norms(self):
+        r, b = C.c_double(), C.c_double()
+        _call("a5g_mg_residual_norms", self.h, C.byref(r), C.byref(b))
+        return r.value, b.value
+
+    def gc_tree(self, iv, rb, bc):
+        _call("a5g_gc_tree", self.h, C.c_int(iv), C.c_int(rb), C.c_int(bc))
+
+    def restrict_tree(self, iv):
+        _call("a5g_restrict_tree", self.h, C.c_int(iv))
+
+    def tree_copy_cc(self, a, b):
+        _call("a5g_tree_copy_cc", self.h, C.c_int(a), C.c_int(b))
+
+    def tree_minmax(self, iv):
+        lo, hi = C.c_double(), C.c_double()
+        _call("a5g_tree_minmax_cc", self.h, C.c_int(iv), C.byref(lo), C.byref(hi))
+        return lo.value, hi.value

@@ -1,0 +1,113 @@
+/* a5g — MI355X (gfx950) implementation of the a5_streamer per-timestep hot
+ * path: Afivo's FAS multigrid Poisson solve, ghost cells, restriction and
+ * prolongation over quadtree/octree box lists, and the streamer's
+ * drift-diffusion flux/source update.
+ *
+ * Plain C ABI: opaque handle, plain pointers and sizes, int status (0 = ok;
+ * otherwise a5g_last_error() describes the failure). No exception crosses
+ * the boundary. Each entry point names the reference routine it stands in
+ * for; argument meaning and 1-based ids / variable indices follow Afivo
+ * (afivo/src/m_aX_t.f90:136-205): box id 0 = no box, negative neighbour ids
+ * are physical boundaries, a box's cell block is cc(0:nc+1,0:nc+1[,0:nc+1])
+ * with i fastest. One handle is used by one host thread at a time; all GPU
+ * work is ordered on the handle's own HIP stream.
+ */
+#ifndef A5G_H
+#define A5G_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct a5g_tree a5g_tree;
+
+/* Message of the last failure on this host thread ("" if none). */
+const char *a5g_last_error(void);
+
+/* ---- tree (afivo/src/m_aX_core.f90) --------------------------------- */
+
+/* a$D_init, m_aX_core.f90:47-116. coarsen_to <= 0: no extra coarse levels.
+ * coord 0 = Cartesian, 1 = cylindrical (2D only). */
+int a5g_init(a5g_tree **t, int dim, int n_cell, int n_var_cell,
+             int n_var_face, double dr, const double *r_min, int coarsen_to,
+             int lvls_max, int coord, int n_boxes);
+/* a$D_destroy, m_aX_core.f90:120-133 (also frees GPU memory) */
+int a5g_destroy(a5g_tree *t);
+/* a$D_set_base, m_aX_core.f90:137-220; ix_list[n][dim], nb_list[n][2*dim] */
+int a5g_set_base(a5g_tree *t, int n, const int *ix_list, const int *nb_list);
+/* a$D_adjust_refinement, m_aX_core.f90:473-537. The user's set_ref_flags
+ * callback is replaced by flags[id-1] (id = 1..n_flags): -1 derefine,
+ * 0 keep, 1 refine; boxes beyond n_flags keep their refinement. */
+int a5g_adjust_refinement(a5g_tree *t, int n_flags, const int *flags,
+                          int *n_add, int *n_rm);
+/* ref_info_t%lvls(lvl)%add / %rm of the last adjustment
+ * (m_aX_core.f90:540-601); pass NULL arrays to query the counts only. */
+int a5g_ref_info_lvl(a5g_tree *t, int lvl, int *add, int *rm, int *n_add,
+                     int *n_rm);
+/* a$D_tidy_up, m_aX_core.f90:224-325 (Morton reorder; moves GPU data) */
+int a5g_tidy_up(a5g_tree *t, double max_frac, double goal_frac,
+                int n_clean_min, int reorder);
+/* out8 = max_id, max_lvl, min_lvl, lvls_max, storage size, n_cell, coord,
+ * ready (the fields of a$D_t, m_aX_t.f90:173-186) */
+int a5g_info(a5g_tree *t, int *out8);
+/* Box records 1..n (box$D_t, m_aX_t.f90:136-161); arrays are [n][...] */
+int a5g_get_boxes(a5g_tree *t, int n, int *lvl, int *ix, int *parent,
+                  int *children, int *neighbors, int *in_use, int *n_cell,
+                  double *dr, double *r_min);
+/* tree%lvls(lvl): counts3 = sizes of ids, leaves, parents */
+int a5g_level_counts(a5g_tree *t, int lvl, int *counts3);
+int a5g_level_lists(a5g_tree *t, int lvl, int *ids, int *leaves,
+                    int *parents);
+
+/* ---- data movement (host <-> HBM) ----------------------------------- */
+
+/* cc(..., iv) of one box, ghosts included ((nc+2)^dim values) */
+int a5g_set_cc(a5g_tree *t, int id, int iv, const double *data);
+int a5g_get_cc(a5g_tree *t, int id, int iv, double *data);
+/* cc(..., iv) of boxes 1..max_id, each at stride (n_cell+2)^dim */
+int a5g_set_cc_all(a5g_tree *t, int iv, const double *data);
+int a5g_get_cc_all(a5g_tree *t, int iv, double *data);
+/* fx/fy/fz(..., ifc) of one box: dim = 1, 2, 3 */
+int a5g_set_fc(a5g_tree *t, int id, int dim, int ifc, const double *data);
+int a5g_get_fc(a5g_tree *t, int id, int dim, int ifc, double *data);
+/* Wait for all GPU work of the handle */
+int a5g_sync(a5g_tree *t);
+
+/* ---- multigrid (afivo/src/m_aX_mg.f90) ------------------------------ */
+
+/* mg$D_t options + mg$D_init_mg, m_aX_mg.f90:20-52, 130-154.
+ * bc_kind per physical face (2*dim entries, order lx hx ly hy [lz hz]):
+ * 0 = Neumann zero, 1 = Dirichlet zero, 2 = value bc_val (ghost 2V - c):
+ * the three forms of the drivers' sides_bc callbacks
+ * (src/streamer_2d.f90:620-638). auto_ops = 1 installs mg$D_auto_* as the
+ * drivers do; 0 keeps the plain Laplacian defaults. */
+int a5g_mg_setup(a5g_tree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
+                 int n_cycle_down, int n_cycle_up, int n_cycle_base,
+                 const int *bc_kind, const double *bc_val, int auto_ops);
+/* Change the value of a physical-face rule (the applied voltage) */
+int a5g_mg_set_bc_value(a5g_tree *t, int nb, double value);
+/* mg$D_fas_fmg, m_aX_mg.f90:164-203 */
+int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
+/* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
+int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
+/* max over leaf cells of |rhs - A phi| (scratch: i_tmp), and of |rhs| */
+int a5g_mg_residual_norms(a5g_tree *t, double *max_res, double *max_rhs);
+
+/* ---- ghost cells, restriction, copies (m_aX_gc/restrict/utils) -------- */
+
+/* a$D_gc_tree, m_aX_gc.f90:28-46. rb: 0 = mg$D_sides_rb, 1 = a$D_gc_interp,
+ * 2 = a$D_gc_interp_lim; bc: 0 = the multigrid face rules, 1 =
+ * a$D_gc_neumann, 2 = a$D_gc_dirichlet */
+int a5g_gc_tree(a5g_tree *t, int iv, int rb, int bc);
+/* a$D_restrict_tree, m_aX_restrict.f90:54-64 */
+int a5g_restrict_tree(a5g_tree *t, int iv);
+/* a$D_tree_copy_cc, m_aX_utils.f90:445-453 */
+int a5g_tree_copy_cc(a5g_tree *t, int iv_from, int iv_to);
+/* a$D_tree_min_cc / a$D_tree_max_cc, m_aX_utils.f90:559-619 */
+int a5g_tree_minmax_cc(a5g_tree *t, int iv, double *vmin, double *vmax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
