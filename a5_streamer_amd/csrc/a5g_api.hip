@@ -195,7 +195,37 @@ void Pool::apply_moves(const std::vector<int>& moves, int n_used) {
   topo_dirty = true;
 }
 
+int Pool::prof_begin(int kind, int lvl, double bytes) {
+  if (!prof_on) return -1;
+  auto take = [&]() {
+    hipEvent_t e;
+    if (!ev_pool.empty()) {
+      e = ev_pool.back();
+      ev_pool.pop_back();
+    } else {
+      hcheck(hipEventCreate(&e), "hipEventCreate");
+    }
+    return e;
+  };
+  ProfRec r{kind, lvl, bytes, take(), take()};
+  hcheck(hipEventRecord(r.a, stream), "hipEventRecord");
+  prof.push_back(r);
+  return (int)prof.size() - 1;
+}
+
+void Pool::prof_end(int idx) {
+  if (idx < 0) return;
+  hcheck(hipEventRecord(prof[idx].b, stream), "hipEventRecord");
+}
+
 void Pool::release() {
+  for (auto& r : prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto& e : ev_pool) (void)hipEventDestroy(e);
+  prof.clear();
+  ev_pool.clear();
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -591,6 +621,53 @@ int a5g_tree_minmax_cc(a5g_tree* t, int iv, double* vmin, double* vmax) {
   NEED(t);
   check_iv(t->P.tree, iv);
   a5g::minmax_boxes(t->P, iv, false, false, true, vmin, vmax);
+  API_END
+}
+
+int a5g_set_gaussians(a5g_tree* t, int iv, int n_blob, const double* blobs, double base) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  if (n_blob < 0 || (n_blob > 0 && !blobs)) throw std::invalid_argument("bad blob list");
+  a5g::set_gaussians(t->P, iv, n_blob, blobs, base);
+  API_END
+}
+
+int a5g_prof_enable(a5g_tree* t, int on) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  auto& P = t->P;
+  a5g::hcheck(hipStreamSynchronize(P.stream), "sync");
+  for (auto& r : P.prof) {
+    P.ev_pool.push_back(r.a);
+    P.ev_pool.push_back(r.b);
+  }
+  P.prof.clear();
+  P.prof_on = on != 0;
+  API_END
+}
+
+int a5g_prof_read(a5g_tree* t, int kind, int lvl, int* n_launch, double* ms, double* bytes) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  auto& P = t->P;
+  a5g::hcheck(hipStreamSynchronize(P.stream), "sync");
+  int cnt = 0;
+  double tot = 0, by = 0;
+  for (auto& r : P.prof) {
+    if (r.kind != kind || (lvl != -999 && r.lvl != lvl)) continue;
+    float m = 0;
+    a5g::hcheck(hipEventElapsedTime(&m, r.a, r.b), "hipEventElapsedTime");
+    cnt++;
+    tot += m;
+    by += r.bytes;
+  }
+  *n_launch = cnt;
+  *ms = tot;
+  *bytes = by;
   API_END
 }
 

@@ -15,4 +15,6 @@ void mg_fmg(Pool& P, bool set_residual, bool first_call);
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);
 
+void set_gaussians(Pool& P, int iv, int nblob, const double* blobs, double base);
+
 }  // namespace a5g

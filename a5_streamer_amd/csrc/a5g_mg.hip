@@ -522,8 +522,11 @@ static void gsrb_boxes(Pool& P, int lvl, int n_cycle) {
   Geo g = geo(n, ncol);
   Dev a = P.dev();
   const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
+  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
   for (int it = 1; it <= 2 * n_cycle; it++) {
     const int fused = it > 1;
+    // algorithmic bytes of a half-sweep: 1.5 words per cell
+    const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * 8 * cells);
     if (cyl)
       hipLaunchKernelGGL((k_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
                          P.ids(lvl), n, nc, it, fused, g.tpb, P.mg.i_phi, P.mg.i_rhs);
@@ -531,6 +534,7 @@ static void gsrb_boxes(Pool& P, int lvl, int n_cycle) {
       hipLaunchKernelGGL((k_gsrb<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
                          P.ids(lvl), n, nc, it, fused, g.tpb, P.mg.i_phi, P.mg.i_rhs);
     post_launch("gsrb");
+    P.prof_end(pr);
   }
   gc_level(P, lvl, P.mg.i_phi, 0, 0);
 }
@@ -543,6 +547,7 @@ void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
   const int work = 2 * D * (D == 2 ? nc : nc * nc);
   Geo g = geo(n, work);
   Dev a = P.dev();
+  const int pr = P.prof_begin(PK_GC, lvl, 0.0);
   if (D == 2)
     hipLaunchKernelGGL((k_gc<2>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
                        g.tpb, iv, rb, bc);
@@ -550,6 +555,7 @@ void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
     hipLaunchKernelGGL((k_gc<3>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
                        g.tpb, iv, rb, bc);
   post_launch("gc");
+  P.prof_end(pr);
 }
 
 template <int D>
@@ -561,6 +567,8 @@ static void resid_rstr(Pool& P, int lvl, int keep_res) {
   Geo g = geo(n, D == 2 ? hnc * hnc : hnc * hnc * hnc);
   Dev a = P.dev();
   const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
+  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
+  const int pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * 8 * cells);
   if (cyl)
     hipLaunchKernelGGL((k_resid_rstr<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
                        P.ids(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp, keep_res);
@@ -568,6 +576,7 @@ static void resid_rstr(Pool& P, int lvl, int keep_res) {
     hipLaunchKernelGGL((k_resid_rstr<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
                        P.ids(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp, keep_res);
   post_launch("resid_rstr");
+  P.prof_end(pr);
 }
 
 template <int D>
@@ -579,6 +588,8 @@ static void coarse_rhs(Pool& P, int lvl, int copy_tmp) {
   Geo g = geo(n, bl);
   Dev a = P.dev();
   const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
+  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
+  const int pr = P.prof_begin(PK_COARSE_RHS, lvl, 4.0 * 8 * cells);
   if (cyl)
     hipLaunchKernelGGL((k_coarse_rhs<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
                        P.parents(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp,
@@ -588,6 +599,7 @@ static void coarse_rhs(Pool& P, int lvl, int copy_tmp) {
                        P.parents(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp,
                        copy_tmp);
   post_launch("coarse_rhs");
+  P.prof_end(pr);
 }
 
 template <int D>
@@ -616,16 +628,22 @@ static void correct_children(Pool& P, int lvl) {
   const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
   Dev a = P.dev();
   Geo g = geo(n, bl);
+  const double pcells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
+  int pr = P.prof_begin(PK_CORR, lvl, 3.0 * 8 * pcells);
   hipLaunchKernelGGL(k_sub_whole, dim3(g.grid), dim3(256), 0, P.stream, a, P.parents(lvl), n,
                      bl, g.tpb, P.mg.i_phi, P.mg.i_tmp);
   post_launch("corr_prep");
+  P.prof_end(pr);
   // every child of these parents is a box of level lvl+1
   const int nch = P.n_ids(lvl + 1);
   const int cnc = P.nc_lvl(lvl + 1);
   Geo g2 = geo(nch, D == 2 ? cnc * cnc : cnc * cnc * cnc);
+  const double ccells = (double)nch * (D == 2 ? cnc * cnc : cnc * cnc * cnc);
+  pr = P.prof_begin(PK_CORR, lvl + 1, (2.0 + 1.0 / (1 << D)) * 8 * ccells);
   hipLaunchKernelGGL((k_corr<D>), dim3(g2.grid), dim3(256), 0, P.stream, a, P.ids(lvl + 1), nch,
                      cnc, g2.tpb, P.mg.i_phi, P.mg.i_tmp);
   post_launch("corr");
+  P.prof_end(pr);
 }
 
 static void copy_level(Pool& P, int lvl, int from, int to) {
@@ -634,9 +652,12 @@ static void copy_level(Pool& P, int lvl, int from, int to) {
   const int nc = P.nc_lvl(lvl);
   const int bl = P.tree.dim == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
   Geo g = geo(n, bl);
+  const int pr = P.prof_begin(PK_COPY, lvl,
+                              2.0 * 8 * n * (double)(P.tree.dim == 2 ? nc * nc : nc * nc * nc));
   hipLaunchKernelGGL(k_copy_whole, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
                      n, bl, g.tpb, from, to);
   post_launch("copy");
+  P.prof_end(pr);
 }
 
 template <int D>

@@ -32,6 +32,17 @@ struct Dev {
   double bc_val[6] = {0, 0, 0, 0, 0, 0};
 };
 
+// Per-launch timing with HIP events on the pool's stream (kernel classes
+// below), used by the benchmark to measure kernel durations in place.
+enum ProfKind { PK_GSRB = 0, PK_GC, PK_RESID_RSTR, PK_COARSE_RHS, PK_CORR, PK_COPY, PK_OTHER,
+                PK_COUNT };
+
+struct ProfRec {
+  int kind, lvl;
+  double bytes;  // algorithmic bytes of the launch
+  hipEvent_t a, b;
+};
+
 struct MgOpts {
   int i_phi = -1, i_tmp = -1, i_rhs = -1, i_eps = -1;
   int n_down = 2, n_up = 2, n_base = 2;
@@ -61,6 +72,11 @@ struct Pool {
   double* d_red = nullptr;
   double* h_red = nullptr;  // pinned
   MgOpts mg;
+  bool prof_on = false;
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> ev_pool;
+  int prof_begin(int kind, int lvl, double bytes);
+  void prof_end(int idx);
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
   double bc_val[6] = {0, 0, 0, 0, 0, 0};
 

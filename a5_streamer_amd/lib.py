@@ -215,6 +215,20 @@ class Tree:
     def tree_copy_cc(self, a, b):
         _call("a5g_tree_copy_cc", self.h, C.c_int(a), C.c_int(b))
 
+    def set_gaussians(self, iv, blobs, base=0.0):
+        b, pb = _d(np.asarray(blobs, float).reshape(-1, self.dim + 2))
+        _call("a5g_set_gaussians", self.h, C.c_int(iv), C.c_int(b.shape[0]), pb,
+              C.c_double(base))
+
+    def prof_enable(self, on):
+        _call("a5g_prof_enable", self.h, C.c_int(int(on)))
+
+    def prof_read(self, kind, lvl=-999):
+        n, ms, by = C.c_int(), C.c_double(), C.c_double()
+        _call("a5g_prof_read", self.h, C.c_int(kind), C.c_int(lvl), C.byref(n),
+              C.byref(ms), C.byref(by))
+        return n.value, ms.value, by.value
+
     def tree_minmax(self, iv):
         lo, hi = C.c_double(), C.c_double()
         _call("a5g_tree_minmax_cc", self.h, C.c_int(iv), C.byref(lo), C.byref(hi))

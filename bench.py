@@ -1,0 +1,203 @@
+#!/usr/bin/env python
+"""Headline benchmark: the isolated 3D FAS multigrid Poisson solve on a frozen
+octree (boxes of 16^3 cells, uniform level 4 = 4096 boxes plus a sub-volume
+refined to level 5; 64 seeded Gaussian charge blobs; phi = V / 0 on the
+bottom / top faces, Neumann sides), solved to 1e-10 relative residual.
+
+One step = one full solve from phi = 0: mg3_fas_fmg (first call) followed by
+mg3_fas_vcycle until max|rhs - A phi| / max|rhs| <= 1e-10 over the leaves,
+the schedule of afivo/src/m_aX_mg.f90 with the drivers' n_cycle_base = 8.
+The number of V-cycles is fixed in warm-up and the residual is re-checked
+after the timed steps. Throughput = leaf cells x solves / s.
+
+Prints one JSON line. Inputs are resident in HBM before timing starts.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KIND_NAMES = ["gsrb_half_sweep", "ghost_cells", "residual_restrict", "coarse_rhs",
+              "correction", "copy"]
+
+
+def build_gpu(W):
+    import a5_streamer_amd as A
+    T = A.Tree(3, W.n_cell, 3, 0, W.dr, coarsen_to=2, n_boxes=1000)
+    W.build(T)
+    kinds, vals = W.bc()
+    i_phi, i_tmp, i_rhs = W.ivs
+    T.mg_setup(i_phi, i_tmp, i_rhs, n_down=2, n_up=2, n_base=W.n_cycle_base,
+               bc_kind=kinds, bc_val=vals, auto_ops=True)
+    T.set_gaussians(i_rhs, W.blobs(), 0.0)
+    return T
+
+
+def solve(T, max_lvl, n_v):
+    T.fas_fmg(False, True)
+    for _ in range(n_v):
+        T.fas_vcycle(max_lvl, False)
+
+
+def cpu_baseline(W_small, n_v_small, budget_s=20.0):
+    """The reference (oracle/_ref, built from the reference sources) timed on
+    this host's cores on the same octree (1-3 solves, bounded by budget_s);
+    the rhs comes from the GPU library on the bit-identical tree."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refharness as R
+    if not R.available():
+        return None
+    import a5_streamer_amd as A
+    Tg = A.Tree(3, W_small.n_cell, 3, 0, W_small.dr, coarsen_to=2, n_boxes=1000)
+    W_small.build(Tg)
+    Tg.set_gaussians(3, W_small.blobs(), 0.0)
+    rhs = Tg.get_cc_all(3)
+    T = R.RefTree(3, W_small.n_cell, 3, 0, W_small.dr, coarsen_to=2, n_boxes=1000)
+    W_small.build(T)
+    b = T.boxes()
+    for i in range(len(b["lvl"])):
+        if b["in_use"][i]:
+            nc = int(b["n_cell"][i])
+            T.set_cc(i + 1, 3, rhs[i][: (nc + 2) ** 3])
+    kinds, vals = W_small.bc()
+    T.mg_setup(1, 2, 3, n_base=W_small.n_cycle_base, bc_kind=kinds, bc_val=vals,
+               auto_ops=True)
+    max_lvl = T.info()["max_lvl"]
+    cells = W_small.leaf_cells(T)
+    n_solves = 0
+    t0 = time.perf_counter()
+    while True:
+        T.fas_fmg(False, True)
+        for _ in range(n_v_small):
+            T.fas_vcycle(max_lvl, False)
+        n_solves += 1
+        if time.perf_counter() - t0 > budget_s / 3 or n_solves >= 3:
+            break
+    dt = time.perf_counter() - t0
+    Tg.close()
+    return {
+        "value": cells * n_solves / dt / 1e6,
+        "unit": "Mcell-solves/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        "kind": "reference",
+        "sample": "reference Fortran (flang -O2 -fopenmp) mg3_fas_fmg + %d mg3_fas_vcycle to "
+                  "1e-10 on the benchmark octree (%d leaf cells), %d solve(s) in %.1f s" % (
+                      n_v_small, cells, n_solves, dt),
+    }
+
+
+def converge_count(T, max_lvl, tol=1e-10, cap=60):
+    T.fas_fmg(False, True)
+    for n in range(cap + 1):
+        r, b = T.residual_norms()
+        if r <= tol * b:
+            return n, r / b
+        T.fas_vcycle(max_lvl, False)
+    raise RuntimeError("multigrid did not reach %g in %d V-cycles" % (tol, cap))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--small", action="store_true", help="smaller octree (testing)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from a5_streamer_amd.workloads import OctreePoisson
+    W = OctreePoisson(uniform_lvl=3, fine_lvl=4, refine_frac=(5, 5, 8)) if args.small \
+        else OctreePoisson()
+    T = build_gpu(W)
+    info = T.info()
+    max_lvl = info["max_lvl"]
+    cells = W.leaf_cells(T)
+    n_v, rel0 = converge_count(T, max_lvl)
+    for _ in range(args.warmup):
+        solve(T, max_lvl, n_v)
+    T.sync()
+
+    T.prof_enable(True)
+    T.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve(T, max_lvl, n_v)
+    T.sync()
+    dt = time.perf_counter() - t0
+    prof = {}
+    for k, name in enumerate(KIND_NAMES):
+        n, ms, by = T.prof_read(k, -999)
+        prof[name] = (n, ms, by)
+    n_f, ms_f, by_f = T.prof_read(0, max_lvl)
+    T.prof_enable(False)
+    res, rhs = T.residual_norms()
+    rel = res / rhs
+    if not rel <= 1e-10:
+        raise RuntimeError("timed solves did not converge: %g" % rel)
+
+    ms_step = dt * 1e3 / args.steps
+    value = cells * args.steps / dt / 1e6
+    dom = max(prof, key=lambda k: prof[k][1])
+    n, ms, by = prof[dom]
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {
+        "metric": "Mcell-steps/s (3D octree FAS-multigrid Poisson solve to 1e-10, FP64)",
+        "value": round(value, 3),
+        "unit": "Mcell-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (64 seeded Gaussian charge blobs)",
+        "config": {
+            "workload": "3D octree Poisson: 2x2x2 base, 16^3-cell boxes, uniform level %d "
+                        "+ %dx%dx%d boxes refined to level %d" % (
+                            W.uniform_lvl, *W.refine_boxes, W.fine_lvl),
+            "leaf_cells": cells,
+            "boxes": info["max_id"],
+            "v_cycles_per_solve": n_v,
+            "rel_residual": rel,
+            "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": n,
+            "avg_launch_us": round(ms * 1e3 / max(n, 1), 2),
+            "alg_bytes_per_launch": by / max(n, 1),
+            "finest_level": {"launches": n_f, "avg_launch_us": round(ms_f * 1e3 / max(n_f, 1), 2),
+                             "achieved_GBs": round(by_f / (ms_f * 1e-3) / 1e9, 1) if ms_f else 0},
+        },
+        "kernel_time_ms_per_step": {k: round(v[1] / args.steps, 3) for k, v in prof.items()},
+    }
+    T.close()
+    if not args.no_cpu_baseline:
+        # the FP64 GPU path is bit-identical to the reference, so the
+        # reference needs the same number of V-cycles
+        out["cpu_baseline"] = cpu_baseline(W, n_v)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

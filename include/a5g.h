@@ -106,6 +106,26 @@ int a5g_tree_copy_cc(a5g_tree *t, int iv_from, int iv_to);
 /* a$D_tree_min_cc / a$D_tree_max_cc, m_aX_utils.f90:559-619 */
 int a5g_tree_minmax_cc(a5g_tree *t, int iv, double *vmin, double *vmax);
 
+/* ---- initial conditions --------------------------------------------- */
+
+/* cc(..., iv) = base + sum_k amp_k exp(-|x - c_k|^2 / w_k^2) at every cell
+ * centre of every box, ghosts included; blobs[k] = (c_1..c_dim, w, amp).
+ * The Gaussian seed of m_geom.f90:82-86 (GM_gaussian) written for blobs. */
+int a5g_set_gaussians(a5g_tree *t, int iv, int n_blob, const double *blobs,
+                      double base);
+
+/* ---- measurement ----------------------------------------------------- */
+
+/* Start (on = 1) or stop timing every kernel launch with HIP events on the
+ * handle's stream; enabling clears earlier records. */
+int a5g_prof_enable(a5g_tree *t, int on);
+/* Launch count, summed duration (ms) and algorithmic HBM bytes of the
+ * recorded launches of one kernel class (0 Gauss-Seidel half-sweep, 1 ghost
+ * cells, 2 residual+restriction, 3 coarse rhs, 4 correction, 5 copy) on
+ * level lvl (-999: all levels). */
+int a5g_prof_read(a5g_tree *t, int kind, int lvl, int *n_launch, double *ms,
+                  double *bytes);
+
 #ifdef __cplusplus
 }
 #endif
