@@ -5,6 +5,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/a5g.h"
 #include "a5g_internal.hpp"
@@ -275,6 +276,36 @@ static void check_id(const a5g::Tree& T, int id) {
   if (id < 1 || id > T.max_id || !T.in_use[id])
     throw std::out_of_range("box id " + std::to_string(id) + " is not in use");
 }
+// Host conversion between the reference's lexicographic box block
+// cc(0:nc+1, ...) and the colour-split device layout (cidx in a5g_kernels.hpp).
+static void split_perm(int dim, int nc, std::vector<int>& perm) {
+  const int n = nc + 2;
+  const int bl = dim == 2 ? n * n : n * n * n, half = bl / 2;
+  perm.resize(bl);
+  for (int q = 0; q < bl; q++) {
+    const int i = q % n, j = (q / n) % n, k = dim == 3 ? q / (n * n) : 0;
+    perm[q] = (((i + j + k) & 1) ? half : 0) + (q >> 1);
+  }
+}
+static void lex_to_split(int dim, int nc, const double* in, double* out) {
+  static thread_local std::vector<int> perm;
+  static thread_local int key = -1;
+  if (key != dim * 1000 + nc) {
+    split_perm(dim, nc, perm);
+    key = dim * 1000 + nc;
+  }
+  for (size_t q = 0; q < perm.size(); q++) out[perm[q]] = in[q];
+}
+static void split_to_lex(int dim, int nc, const double* in, double* out) {
+  static thread_local std::vector<int> perm;
+  static thread_local int key = -1;
+  if (key != dim * 1000 + nc) {
+    split_perm(dim, nc, perm);
+    key = dim * 1000 + nc;
+  }
+  for (size_t q = 0; q < perm.size(); q++) out[q] = in[perm[q]];
+}
+
 static void need_gpu(const a5g::Pool& P) {
   if (P.host_only) throw std::runtime_error("no GPU visible: cell data and kernels are unavailable");
 }
@@ -442,7 +473,9 @@ int a5g_set_cc(a5g_tree* t, int id, int iv, const double* data) {
   P.ensure_capacity(P.tree.max_id);
   a5g::Dev a = P.dev();
   double* dst = a.cc + (size_t)(iv - 1) * a.vs + (size_t)id * a.bs;
-  a5g::hcheck(hipMemcpyAsync(dst, data, P.tree.box_len(id) * sizeof(double),
+  std::vector<double> tmp(P.tree.box_len(id));
+  lex_to_split(P.tree.dim, P.tree.ncell[id], data, tmp.data());
+  a5g::hcheck(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(double),
                              hipMemcpyHostToDevice, P.stream), "set_cc");
   a5g::hcheck(hipStreamSynchronize(P.stream), "set_cc sync");
   API_END
@@ -457,9 +490,11 @@ int a5g_get_cc(a5g_tree* t, int id, int iv, double* data) {
   check_iv(P.tree, iv);
   a5g::Dev a = P.dev();
   const double* src = a.cc + (size_t)(iv - 1) * a.vs + (size_t)id * a.bs;
-  a5g::hcheck(hipMemcpyAsync(data, src, P.tree.box_len(id) * sizeof(double),
+  std::vector<double> tmp(P.tree.box_len(id));
+  a5g::hcheck(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(double),
                              hipMemcpyDeviceToHost, P.stream), "get_cc");
   a5g::hcheck(hipStreamSynchronize(P.stream), "get_cc sync");
+  split_to_lex(P.tree.dim, P.tree.ncell[id], tmp.data(), data);
   API_END
 }
 
@@ -471,9 +506,15 @@ int a5g_set_cc_all(a5g_tree* t, int iv, const double* data) {
   check_iv(P.tree, iv);
   P.ensure_capacity(P.tree.max_id);
   a5g::Dev a = P.dev();
-  a5g::hcheck(hipMemcpyAsync(a.cc + (size_t)(iv - 1) * a.vs + a.bs, data,
-                             (size_t)P.tree.max_id * a.bs * sizeof(double),
-                             hipMemcpyHostToDevice, P.stream), "set_cc_all");
+  const auto& T = P.tree;
+  std::vector<double> tmp((size_t)T.max_id * a.bs, 0.0);
+  for (int id = 1; id <= T.max_id; id++)
+    if (T.in_use[id])
+      lex_to_split(T.dim, T.ncell[id], data + (size_t)(id - 1) * a.bs,
+                   tmp.data() + (size_t)(id - 1) * a.bs);
+  a5g::hcheck(hipMemcpyAsync(a.cc + (size_t)(iv - 1) * a.vs + a.bs, tmp.data(),
+                             tmp.size() * sizeof(double), hipMemcpyHostToDevice, P.stream),
+              "set_cc_all");
   a5g::hcheck(hipStreamSynchronize(P.stream), "set_cc_all sync");
   API_END
 }
@@ -485,10 +526,17 @@ int a5g_get_cc_all(a5g_tree* t, int iv, double* data) {
   need_gpu(P);
   check_iv(P.tree, iv);
   a5g::Dev a = P.dev();
-  a5g::hcheck(hipMemcpyAsync(data, a.cc + (size_t)(iv - 1) * a.vs + a.bs,
-                             (size_t)P.tree.max_id * a.bs * sizeof(double),
-                             hipMemcpyDeviceToHost, P.stream), "get_cc_all");
+  const auto& T = P.tree;
+  std::vector<double> tmp((size_t)T.max_id * a.bs);
+  a5g::hcheck(hipMemcpyAsync(tmp.data(), a.cc + (size_t)(iv - 1) * a.vs + a.bs,
+                             tmp.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream),
+              "get_cc_all");
   a5g::hcheck(hipStreamSynchronize(P.stream), "get_cc_all sync");
+  for (int id = 1; id <= T.max_id; id++) {
+    double* o = data + (size_t)(id - 1) * a.bs;
+    if (T.in_use[id]) split_to_lex(T.dim, T.ncell[id], tmp.data() + (size_t)(id - 1) * a.bs, o);
+    else std::fill(o, o + a.bs, 0.0);
+  }
   API_END
 }
 

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_gaussians(Dev a, const int* __restrict_
       const double w = g[D];
       v = v + g[D + 1] * exp(-r2 / (w * w));
     }
-    p[q] = v;
+    p[cidx<D>(nc, i, j, k)] = v;
   }
 }
 

@@ -18,11 +18,17 @@ __device__ __forceinline__ int d_nb_dix(int nb, int d) {
   return d_nb_dim(nb) != d ? 0 : (d_nb_low(nb) ? -1 : 1);
 }
 
-// flat index of cell (i, j, k) in a box with nc cells per dimension
+// Flat index of cell (i, j, k), ghosts included (0 <= i,j,k <= nc+1), in a
+// box with nc cells per dimension. Boxes are stored colour-split: the cells
+// with i+j+k even come first, then the odd ones, each half in lexicographic
+// order of the (even) row pairs. A red-black half-sweep then reads one half
+// and writes the other instead of touching every cache line of the box.
 template <int D>
 __device__ __forceinline__ int cidx(int nc, int i, int j, int k) {
   const int n = nc + 2;
-  return D == 2 ? i + n * j : i + n * (j + n * k);
+  const int lex = D == 2 ? i + n * j : i + n * (j + n * k);
+  const int half = D == 2 ? (n * n) >> 1 : (n * n * n) >> 1;
+  return (((i + j + k) & 1) ? half : 0) + (lex >> 1);
 }
 
 __device__ __forceinline__ double* boxp(const Dev& a, int iv, int id) {

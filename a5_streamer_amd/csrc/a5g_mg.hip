@@ -107,6 +107,115 @@ __global__ __launch_bounds__(256) void k_gsrb(Dev a, const int* __restrict__ lis
   }
 }
 
+// Consecutive entries of a box list go to the same XCD (the hardware deals
+// workgroups round-robin over the 8 XCDs), so Morton-adjacent boxes share an
+// L2 and neighbour faces are L2 hits.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, g = gridDim.x, per = g >> 3;
+  return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+}
+
+// Box-size-specialised half-sweep. The other colour's half of the box (ghost
+// slots included) is staged in LDS; when fused, that colour's face ghost
+// slots are first recomputed from the current state (the values the
+// reference's ghost pass after the previous half-sweep would store). Then
+// every lane updates cells of colour cntr&1 from LDS and rhs.
+template <int D, bool CYL, int NC>
+__global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
+                                                  int cntr, int fused, int i_phi, int i_rhs) {
+  constexpr int N2 = NC + 2;
+  constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
+  constexpr int HALF = BL / 2;
+  constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
+  constexpr int TPB = NCOL > 128 ? 256 : (NCOL > 64 ? 128 : 64);
+  constexpr int BPB = 256 / TPB;
+  constexpr int PER_FACE = D == 2 ? NC : NC * NC;
+  constexpr int NGH = 2 * D * PER_FACE;
+  constexpr int HNC = NC / 2;
+  __shared__ double sh[BPB * HALF];
+  const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
+  const int li = xcd_block() * BPB + slot;
+  const bool act = li < n;
+  const int id = act ? list[li] : 0;
+  const int c = cntr & 1, oc = c ^ 1;
+  double* s = sh + slot * HALF;
+  double* p = act ? boxp(a, i_phi, id) : nullptr;
+  if (act) {
+    for (int q = lt; q < HALF; q += TPB) s[q] = p[oc * HALF + q];
+    if (fused) {
+      for (int q = lt; q < NGH; q += TPB) {
+        const int nb = q / PER_FACE + 1;
+        const int t = q - (nb - 1) * PER_FACE;
+        const int d = d_nb_dim(nb);
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        int g[3] = {0, 0, 0};
+        g[d] = d_nb_low(nb) ? 0 : NC + 1;
+        g[ta] = t % NC + 1;
+        if (D == 3) g[tb] = t / NC + 1;
+        if (((g[0] + g[1] + g[2]) & 1) != oc) continue;
+        const int lex = D == 2 ? g[0] + N2 * g[1] : g[0] + N2 * (g[1] + N2 * g[2]);
+        s[lex >> 1] = ghost_mg<D>(a, p, i_phi, id, NC, nb, g[0], g[1], g[2]);
+      }
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  const double* r = boxp(a, i_rhs, id) + c * HALF;
+  double* pc = p + c * HALF;
+  const double dx2 = a.dr[id] * a.dr[id];
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
+  for (int q = lt; q < NCOL; q += TPB) {
+    const int mm = q % HNC, row = q / HNC;
+    const int j = row % NC + 1;
+    const int k = D == 3 ? row / NC + 1 : 0;
+    const int i = 2 * mm + 2 - ((c + j + k) & 1);
+    const int lex = D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
+    const int x = lex >> 1;
+    const double xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
+    const double yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
+    double v;
+    if (D == 2) {
+      if (CYL) {
+        const double den = i + ioff - 0.5;
+        const double rf1 = (double)(i + ioff - 1) / den;
+        const double rf2 = (double)(i + ioff) / den;
+        v = 0.25 * (rf1 * xm + rf2 * xp + yp + ym - dx2 * r[x]);
+      } else {
+        v = 0.25 * (xp + xm + yp + ym - dx2 * r[x]);
+      }
+    } else {
+      const double zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
+      const double sixth = 1 / 6.0;
+      v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * r[x]);
+    }
+    pc[x] = v;
+  }
+}
+
+template <int D, bool CYL>
+static bool launch_gsrb_lds(Pool& P, const int* list, int n, int nc, int cntr, int fused) {
+  auto go = [&](auto kern, int ncol) {
+    const int tpb = ncol > 128 ? 256 : (ncol > 64 ? 128 : 64);
+    const int bpb = 256 / tpb;
+    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, P.dev(), list, n,
+                       cntr, fused, P.mg.i_phi, P.mg.i_rhs);
+  };
+  const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
+  switch (nc) {
+    case 2: go(k_gsrb_lds<D, CYL, 2>, ncol); return true;
+    case 4: go(k_gsrb_lds<D, CYL, 4>, ncol); return true;
+    case 8: go(k_gsrb_lds<D, CYL, 8>, ncol); return true;
+    case 16: go(k_gsrb_lds<D, CYL, 16>, ncol); return true;
+    case 32:
+      if (D == 2) {
+        go(k_gsrb_lds<2, CYL, 32>, ncol);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Face ghost cells of one variable: a$D_gc_box (m_aX_gc.f90:50-68) with
 // rb = 0 mg$D_sides_rb, 1 a$D_gc_interp (:90-185), 2 a$D_gc_interp_lim
@@ -373,12 +482,13 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
   const int bl = D == 2 ? n2 * n2 : n2 * n2 * n2;
   for (int q = lt; q < bl; q += tpb) {
     const int i = q % n2, j = (q / n2) % n2, k = D == 3 ? q / (n2 * n2) : 0;
+    const int x = cidx<D>(nc, i, j, k);
     const bool inner = i >= 1 && i <= nc && j >= 1 && j <= nc &&
                        (D == 2 || (k >= 1 && k <= nc));
-    double rv = inner ? lpl_at<D, CYL>(p, nc, i, j, k, inv_dr_sq, ioff) : r[q];
-    rv = rv + t[q];
-    r[q] = rv;
-    if (copy_tmp) t[q] = p[q];
+    double rv = inner ? lpl_at<D, CYL>(p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
+    rv = rv + t[x];
+    r[x] = rv;
+    if (copy_tmp) t[x] = p[x];
   }
 }
 
@@ -495,7 +605,17 @@ __global__ __launch_bounds__(256) void k_minmax(Dev a, const int* __restrict__ l
   }
   lo = wave_min(lo);
   hi = wave_max(hi);
+  __shared__ double wl[4], wh[4];
   if ((threadIdx.x & 63) == 0) {
+    wl[threadIdx.x >> 6] = lo;
+    wh[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = fmin(lo, wl[w]);
+      hi = fmax(hi, wh[w]);
+    }
     atomicMin(&out[0], ord_key(lo));
     atomicMax(&out[1], ord_key(hi));
   }
@@ -527,7 +647,10 @@ static void gsrb_boxes(Pool& P, int lvl, int n_cycle) {
     const int fused = it > 1;
     // algorithmic bytes of a half-sweep: 1.5 words per cell
     const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * 8 * cells);
-    if (cyl)
+    const bool done = cyl ? launch_gsrb_lds<2, true>(P, P.ids(lvl), n, nc, it, fused)
+                          : launch_gsrb_lds<D, false>(P, P.ids(lvl), n, nc, it, fused);
+    if (done) {
+    } else if (cyl)
       hipLaunchKernelGGL((k_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
                          P.ids(lvl), n, nc, it, fused, g.tpb, P.mg.i_phi, P.mg.i_rhs);
     else
