@@ -230,6 +230,8 @@ void Pool::release() {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
+  f(d_table);
+  d_table = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
   f(d_dr); f(d_rmin); f(d_lists); f(d_red);
   if (h_red) (void)hipHostFree(h_red);
@@ -669,6 +671,215 @@ int a5g_tree_minmax_cc(a5g_tree* t, int iv, double* vmin, double* vmax) {
   NEED(t);
   check_iv(t->P.tree, iv);
   a5g::minmax_boxes(t->P, iv, false, false, true, vmin, vmax);
+  API_END
+}
+
+// ---- transport table (fosito/m_lookup_table.f90, m_find_index.f90) -------
+
+// FI_adaptive_r (fosito/m_find_index.f90:56-67): smallest 1-based ix with
+// val <= list(ix); linear search below 40 entries, bisection above
+static int fi_adaptive(int n, const double* list, double val) {
+  if (n < 40) {
+    int ix;
+    for (ix = 1; ix <= n; ix++)
+      if (val <= list[ix - 1]) break;
+    return ix;
+  }
+  int lo = 1, hi = n;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (val <= list[mid - 1]) hi = mid;
+    else lo = mid + 1;
+  }
+  if (val > list[lo - 1]) lo = n + 1;
+  return lo;
+}
+
+// LT_lin_interp_list (fosito/m_lookup_table.f90:201-221)
+static double lin_interp_list(int n, const double* x, const double* y, double xv) {
+  if (xv <= x[0]) return y[0];
+  if (xv >= x[n - 1]) return y[n - 1];
+  const int ix = fi_adaptive(n, x, xv);
+  const double temp = (xv - x[ix - 2]) / (x[ix - 1] - x[ix - 2]);
+  return (1 - temp) * y[ix - 2] + temp * y[ix - 1];
+}
+
+int a5g_table_create(a5g_tree* t, double x_min, double x_max, int n_rows, int n_cols) {
+  // LT_create, fosito/m_lookup_table.f90:46-60
+  API_BEGIN
+  NEED(t);
+  if (x_max <= x_min) throw std::invalid_argument("table: x_max should be > x_min");
+  if (n_rows <= 1) throw std::invalid_argument("table: n_rows should be bigger than 1");
+  if (n_cols < 1) throw std::invalid_argument("table: n_cols should be >= 1");
+  auto& P = t->P;
+  P.tbl_rows = n_rows;
+  P.tbl_cols = n_cols;
+  P.tbl_xmin = x_min;
+  P.tbl_inv_dx = (n_rows - 1) / (x_max - x_min);
+  P.h_table.assign((size_t)n_rows * n_cols, 0.0);
+  if (!P.host_only) a5g::set_table(P, n_rows, n_cols, P.tbl_xmin, P.tbl_inv_dx, P.h_table.data());
+  API_END
+}
+
+int a5g_table_set_col(a5g_tree* t, int col, int n, const double* xx, const double* yy) {
+  // LT_set_col / get_spaced_data / LT_get_xdata,
+  // fosito/m_lookup_table.f90:62-72, 180-198
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (P.h_table.empty()) throw std::logic_error("table: call a5g_table_create first");
+  if (col < 1 || col > P.tbl_cols) throw std::out_of_range("table column out of range");
+  if (n < 1) throw std::invalid_argument("table: empty data");
+  const double dx = 1 / P.tbl_inv_dx;
+  for (int ix = 1; ix <= P.tbl_rows; ix++) {
+    const double xv = P.tbl_xmin + (ix - 1) * dx;
+    P.h_table[(size_t)(col - 1) * P.tbl_rows + ix - 1] = lin_interp_list(n, xx, yy, xv);
+  }
+  if (!P.host_only)
+    a5g::set_table(P, P.tbl_rows, P.tbl_cols, P.tbl_xmin, P.tbl_inv_dx, P.h_table.data());
+  API_END
+}
+
+int a5g_table_get(a5g_tree* t, double* rows_cols, double* x_min, double* inv_dx) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (P.h_table.empty()) throw std::logic_error("table: none set");
+  std::copy(P.h_table.begin(), P.h_table.end(), rows_cols);
+  *x_min = P.tbl_xmin;
+  *inv_dx = P.tbl_inv_dx;
+  API_END
+}
+
+int a5g_set_table(a5g_tree* t, int n_rows, int n_cols, double x_min, double inv_dx,
+                  const double* rows_cols) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  P.h_table.assign(rows_cols, rows_cols + (size_t)n_rows * n_cols);
+  P.tbl_rows = n_rows;
+  P.tbl_cols = n_cols;
+  P.tbl_xmin = x_min;
+  P.tbl_inv_dx = inv_dx;
+  if (!P.host_only) a5g::set_table(P, n_rows, n_cols, x_min, inv_dx, rows_cols);
+  API_END
+}
+
+// ---- streamer physics -------------------------------------------------------
+
+int a5g_streamer_setup(a5g_tree* t, const int* v) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  a5g::StVars s;
+  s.elec = v[0]; s.pion = v[1]; s.elec_old = v[2]; s.pion_old = v[3]; s.phi = v[4];
+  s.fld = v[5]; s.rhs = v[6]; s.pho = v[7]; s.eps = v[8];
+  s.f_elec = v[9]; s.f_fld = v[10];
+  s.t_mob = v[11]; s.t_dif = v[12]; s.t_alpha = v[13]; s.t_eta = v[14];
+  s.dielectric_rule = v[15];
+  for (int iv : {s.elec, s.pion, s.elec_old, s.pion_old, s.phi, s.fld, s.rhs, s.eps})
+    check_iv(P.tree, iv);
+  if (s.pho > 0) check_iv(P.tree, s.pho);
+  if (s.f_elec < 1 || s.f_elec > P.tree.n_fc || s.f_fld < 1 || s.f_fld > P.tree.n_fc)
+    throw std::out_of_range("face variable out of range");
+  P.sv = s;
+  API_END
+}
+
+static void need_table(const a5g::Pool& P, const a5g::StVars& v) {
+  if (!P.d_table) throw std::logic_error("no transport table on the GPU");
+  const int mx = std::max(std::max(v.t_mob, v.t_dif), std::max(v.t_alpha, v.t_eta));
+  if (mx > P.tbl_cols) throw std::out_of_range("table column out of range");
+}
+
+int a5g_space_charge(a5g_tree* t, double fac) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::space_charge(t->P, t->P.sv, fac);
+  API_END
+}
+
+int a5g_fld_from_pot(a5g_tree* t) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::fld_from_pot(t->P, t->P.sv);
+  API_END
+}
+
+int a5g_fluxes_koren(a5g_tree* t, double dt) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  need_table(t->P, t->P.sv);
+  if (!(dt > 0)) throw std::invalid_argument("fluxes_koren: dt must be > 0");
+  a5g::fluxes_koren(t->P, t->P.sv, dt);
+  API_END
+}
+
+int a5g_consistent_fluxes(a5g_tree* t, int n, const int* f_ixs) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  for (int i = 0; i < n; i++) {
+    if (f_ixs[i] < 1 || f_ixs[i] > t->P.tree.n_fc)
+      throw std::out_of_range("face variable out of range");
+    a5g::consistent_fluxes(t->P, f_ixs[i]);
+  }
+  API_END
+}
+
+int a5g_update_solution(a5g_tree* t, double dt) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  need_table(t->P, t->P.sv);
+  a5g::update_solution(t->P, t->P.sv, dt);
+  API_END
+}
+
+int a5g_average_dens(a5g_tree* t) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::average_dens(t->P, t->P.sv);
+  API_END
+}
+
+int a5g_box_summaries(a5g_tree* t, double* out) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::box_summaries(t->P, t->P.sv, out);
+  API_END
+}
+
+int a5g_prolong1_to(a5g_tree* t, int n_ids, const int* ids, int iv) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  a5g::prolong1_boxes(t->P, n_ids, ids, iv);
+  API_END
+}
+
+int a5g_fill_cc(a5g_tree* t, int n_ids, const int* ids, int iv, double value) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  a5g::fill_boxes(t->P, n_ids, ids, iv, value);
+  API_END
+}
+
+int a5g_gc_boxes(a5g_tree* t, int n_ids, const int* ids, int iv, int rb, int bc) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  if (rb < 0 || rb > 2 || bc < 0 || bc > 2) throw std::invalid_argument("bad ghost rule");
+  a5g::gc_boxes(t->P, n_ids, ids, iv, rb, bc);
   API_END
 }
 

@@ -16,5 +16,18 @@ void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);
 
 void set_gaussians(Pool& P, int iv, int nblob, const double* blobs, double base);
+void gc_list(Pool& P, int lvl, const int* d_ids, int n, int iv, int rb, int bc);
+
+void set_table(Pool& P, int n_rows, int n_cols, double x_min, double inv_dx, const double* rc);
+void space_charge(Pool& P, const StVars& v, double fac);
+void fld_from_pot(Pool& P, const StVars& v);
+void fluxes_koren(Pool& P, const StVars& v, double dt);
+void consistent_fluxes(Pool& P, int f_elec);
+void update_solution(Pool& P, const StVars& v, double dt);
+void average_dens(Pool& P, const StVars& v);
+void box_summaries(Pool& P, const StVars& v, double* out_host);
+void prolong1_boxes(Pool& P, int n_ids, const int* ids, int iv);
+void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value);
+void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc);
 
 }  // namespace a5g

@@ -663,7 +663,10 @@ static void gsrb_boxes(Pool& P, int lvl, int n_cycle) {
 }
 
 void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
-  const int n = P.n_ids(lvl);
+  gc_list(P, lvl, P.ids(lvl), P.n_ids(lvl), iv, rb, bc);
+}
+
+void gc_list(Pool& P, int lvl, const int* list, int n, int iv, int rb, int bc) {
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
   const int D = P.tree.dim;
@@ -672,11 +675,11 @@ void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
   Dev a = P.dev();
   const int pr = P.prof_begin(PK_GC, lvl, 0.0);
   if (D == 2)
-    hipLaunchKernelGGL((k_gc<2>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
-                       g.tpb, iv, rb, bc);
+    hipLaunchKernelGGL((k_gc<2>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
+                       iv, rb, bc);
   else
-    hipLaunchKernelGGL((k_gc<3>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
-                       g.tpb, iv, rb, bc);
+    hipLaunchKernelGGL((k_gc<3>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
+                       iv, rb, bc);
   post_launch("gc");
   P.prof_end(pr);
 }

@@ -43,6 +43,16 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// Variable indices of the streamer models (src/m_streamer.f90:16-43): cell
+// variables, face variables, lookup-table columns and two driver options.
+struct StVars {
+  int elec = 1, pion = 2, elec_old = 3, pion_old = 4, phi = 5, fld = 6, rhs = 7, pho = -1,
+      eps = 9;
+  int f_elec = 1, f_fld = 2;
+  int t_mob = 1, t_dif = 2, t_alpha = 3, t_eta = 4;
+  int dielectric_rule = 1;
+};
+
 struct MgOpts {
   int i_phi = -1, i_tmp = -1, i_rhs = -1, i_eps = -1;
   int n_down = 2, n_up = 2, n_base = 2;
@@ -72,6 +82,12 @@ struct Pool {
   double* d_red = nullptr;
   double* h_red = nullptr;  // pinned
   MgOpts mg;
+  StVars sv;
+  // transport lookup table on the device: rows_cols(n_rows, n_cols)
+  double* d_table = nullptr;
+  int tbl_rows = 0, tbl_cols = 0;
+  double tbl_xmin = 0, tbl_inv_dx = 0;
+  std::vector<double> h_table;  // host copy while it is being built
   bool prof_on = false;
   std::vector<ProfRec> prof;
   std::vector<hipEvent_t> ev_pool;

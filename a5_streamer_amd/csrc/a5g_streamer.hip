@@ -1,0 +1,794 @@
+// Streamer physics kernels of the a5g library: space-charge right-hand side,
+// field from potential, Koren-limited drift-diffusion fluxes, consistent
+// fluxes at refinement boundaries, the density update with the
+// impact-ionisation source, trapezoidal averaging, per-box refinement
+// summaries and prolongation to new boxes. They restate the per-box
+// procedures of src/streamer_{2d,3d,cyl}.f90 with the reference's operand
+// order (the library is built with -ffp-contract=off), so the FP64 results
+// equal the reference's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "a5g_internal.hpp"
+#include "a5g_kernels.hpp"
+
+namespace a5g {
+
+#define BOXES_PER_BLOCK                                     \
+  const int bpb_ = blockDim.x / tpb;                        \
+  const int slot_ = threadIdx.x / tpb;                      \
+  const int lt = threadIdx.x - slot_ * tpb;                 \
+  const int li_ = blockIdx.x * bpb_ + slot_;                \
+  const bool act = li_ < n;                                 \
+  const int id = act ? list[li_] : 0;
+
+static int pow2c(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+struct G2 {
+  int tpb, grid;
+};
+static G2 g2(int n, int work) {
+  int tpb = std::min(256, std::max(64, pow2c(work)));
+  return {tpb, (n + 256 / tpb - 1) / (256 / tpb)};
+}
+
+// index into a face array fx(nc+1,nc[,nc]) / fy / fz of a box (1-based)
+template <int D>
+__device__ __forceinline__ int fidx(int nc, int d, int i, int j, int k) {
+  const int n0 = nc + (d == 0), n1 = nc + (d == 1);
+  return D == 2 ? (i - 1) + n0 * (j - 1) : (i - 1) + n0 * ((j - 1) + n1 * (k - 1));
+}
+
+// ---------------------------------------------------------------------------
+// lookup table (fosito/m_lookup_table.f90:100-161): the location is
+// low_ix = ceiling((x - x_min) inv_dx), low_frac = low_ix - frac, clamped to
+// the table; a column value is low_frac t(low_ix) + (1 - low_frac) t(low_ix+1)
+// ---------------------------------------------------------------------------
+
+struct Tbl {
+  const double* rc = nullptr;  // [col][row], 0-based
+  int n_rows = 0;
+  double x_min = 0, inv_dx = 0;
+};
+
+__device__ __forceinline__ void lt_loc(const Tbl& t, double x, int& li, double& lf) {
+  const double frac = (x - t.x_min) * t.inv_dx;
+  li = (int)ceil(frac);
+  lf = li - frac;
+  if (li < 1) {
+    li = 1;
+    lf = 1;
+  } else if (li >= t.n_rows) {
+    li = t.n_rows - 1;
+    lf = 0;
+  }
+}
+
+__device__ __forceinline__ double lt_col(const Tbl& t, int col, int li, double lf) {
+  const double* c = t.rc + (size_t)(col - 1) * t.n_rows;
+  return lf * c[li - 1] + (1 - lf) * c[li];
+}
+
+// ---------------------------------------------------------------------------
+// rhs = fac (n_e - n_i) on whole leaf boxes (compute_fld,
+// src/streamer_2d.f90:318-329)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_space_charge(Dev a, const int* __restrict__ list, int n,
+                                                      int bl, int tpb, int ie, int ip, int ir,
+                                                      double fac) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double* e = boxp(a, ie, id);
+  const double* p = boxp(a, ip, id);
+  double* r = boxp(a, ir, id);
+  for (int q = lt; q < bl; q += tpb) r[q] = fac * (e[q] - p[q]);
+}
+
+// ---------------------------------------------------------------------------
+// fld_from_pot (src/streamer_2d.f90:346-381, src/streamer_3d.f90:348-395):
+// face fields, with the permittivity-weighted form on the box faces, then the
+// cell field norm
+// ---------------------------------------------------------------------------
+
+template <int D>
+__global__ __launch_bounds__(256) void k_fld(Dev a, const int* __restrict__ list, int n, int nc,
+                                             int tpb, int iphi, int ieps, int ifld, int ffld) {
+  BOXES_PER_BLOCK
+  const int per = (nc + 1) * (D == 2 ? nc : nc * nc);
+  if (act) {
+    const double inv_dr = 1 / a.dr[id];
+    const double* p = boxp(a, iphi, id);
+    const double* e = boxp(a, ieps, id);
+    for (int q = lt; q < D * per; q += tpb) {
+      const int d = q / per, w = q - d * per;
+      const int n0 = nc + (d == 0), n1 = nc + (d == 1);
+      const int i = w % n0 + 1, j = (w / n0) % n1 + 1, k = D == 3 ? w / (n0 * n1) + 1 : 0;
+      int lo[3] = {i, j, k};
+      lo[d] -= 1;
+      const int ia = cidx<D>(nc, lo[0], lo[1], lo[2]);
+      const int ib = cidx<D>(nc, i, j, k);
+      const int fpos = d == 0 ? i : (d == 1 ? j : k);
+      double v;
+      if (fpos == 1) v = 2 * inv_dr * (p[ia] - p[ib]) * e[ia] / (e[ib] + e[ia]);
+      else if (fpos == nc + 1) v = 2 * inv_dr * (p[ia] - p[ib]) * e[ib] / (e[ib] + e[ia]);
+      else v = inv_dr * (p[ia] - p[ib]);
+      facep(a, ffld, id, d, D)[w] = v;
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  double* fl = boxp(a, ifld, id);
+  const double* F0 = facep(a, ffld, id, 0, D);
+  const double* F1 = facep(a, ffld, id, 1, D);
+  const double* F2 = D == 3 ? facep(a, ffld, id, 2, D) : nullptr;
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const double sx = F0[fidx<D>(nc, 0, i, j, k)] + F0[fidx<D>(nc, 0, i + 1, j, k)];
+    const double sy = F1[fidx<D>(nc, 1, i, j, k)] + F1[fidx<D>(nc, 1, i, j + 1, k)];
+    if (D == 2) {
+      fl[cidx<D>(nc, i, j, k)] = sqrt(0.25 * (sx * sx) + 0.25 * (sy * sy));
+    } else {
+      const double sz = F2[fidx<D>(nc, 2, i, j, k)] + F2[fidx<D>(nc, 2, i, j, k + 1)];
+      fl[cidx<D>(nc, i, j, k)] = 0.5 * sqrt(sx * sx + sy * sy + sz * sz);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Koren limiter, src/m_streamer.f90:339-362
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double koren_mlim(double x, double y) {
+  const double sixth = 1 / 6.0;
+  const double aa = x * x, ab = x * y;
+  if (ab <= 0) return 0;
+  if (aa <= 0.25 * ab) return x;
+  if (aa <= 2.5 * ab) return sixth * (y + 2 * x);
+  return y;
+}
+
+// Second ghost layer value beyond face nb at tangential cell (t1, t2) of box
+// id: a$D_gc2_box with a$D_gc2_prolong1 / a$D_gc2_neumann
+// (afivo/src/m_aX_gc.f90:407-622)
+template <int D>
+__device__ double gc2_value(const Dev& a, int iv, int id, int nc, int nb, int t1, int t2) {
+  const int d = d_nb_dim(nb);
+  const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+  const bool low = d_nb_low(nb);
+  const int nb_id = a.nbs[id * 6 + nb - 1];
+  int c[3] = {0, 0, 0};
+  c[ta] = t1;
+  if (D == 3) c[tb] = t2;
+  if (nb_id > 0) {
+    c[d] = low ? nc - 1 : 2;
+    return boxp(a, iv, nb_id)[cidx<D>(nc, c[0], c[1], c[2])];
+  }
+  if (nb_id < 0) {
+    c[d] = low ? 2 : nc - 1;
+    return boxp(a, iv, id)[cidx<D>(nc, c[0], c[1], c[2])];
+  }
+  const int pid = a.parent[id];
+  const int pn = a.nbs[pid * 6 + nb - 1];
+  const int pnc = a.ncell[pn];
+  int off[3];
+  child_off<D>(a, id, nc, nb, off);
+  const double* q = boxp(a, iv, pn);
+  const int ix = low ? -1 : nc + 2;
+  const int n1 = off[d] + ((ix + 1) >> 1), n2 = n1 + 1 - 2 * (ix & 1);
+  const int c1 = off[ta] + ((t1 + 1) >> 1), c2 = c1 + 1 - 2 * (t1 & 1);
+  if (D == 2) {
+    int s0[3], s1[3], s2[3];
+    s0[d] = n1; s0[ta] = c1;
+    s1[d] = n2; s1[ta] = c1;
+    s2[d] = n1; s2[ta] = c2;
+    // the shifted cells enter in the order dim 0 then dim 1
+    const double v0 = q[cidx<D>(pnc, s0[0], s0[1], 0)];
+    double va = q[cidx<D>(pnc, s1[0], s1[1], 0)];
+    double vb = q[cidx<D>(pnc, s2[0], s2[1], 0)];
+    if (d == 1) {
+      const double tmp = va;
+      va = vb;
+      vb = tmp;
+    }
+    return 0.5 * v0 + 0.25 * va + 0.25 * vb;
+  }
+  const int e1 = off[tb] + ((t2 + 1) >> 1), e2 = e1 + 1 - 2 * (t2 & 1);
+  int P[4][3];
+  for (int m = 0; m < 4; m++) {
+    P[m][d] = n1;
+    P[m][ta] = c1;
+    P[m][tb] = e1;
+  }
+  P[1][d] = n2;
+  P[2][ta] = c2;
+  P[3][tb] = e2;
+  // sum order: base, then the cell shifted in dim 0, dim 1, dim 2
+  double v[4];
+  v[0] = q[cidx<D>(pnc, P[0][0], P[0][1], P[0][2])];
+  int m = 1;
+  for (int dd = 0; dd < 3; dd++) {
+    const int w = dd == d ? 1 : (dd == ta ? 2 : 3);
+    v[m++] = q[cidx<D>(pnc, P[w][0], P[w][1], P[w][2])];
+  }
+  return 0.25 * v[0] + 0.25 * v[1] + 0.25 * v[2] + 0.25 * v[3];
+}
+
+// fluxes_koren (src/streamer_2d.f90:384-489, src/streamer_3d.f90:396-554,
+// src/streamer_cyl.f90:416-521) on leaves: one lane per face
+template <int D>
+__global__ __launch_bounds__(256) void k_koren(Dev a, Tbl tb, const int* __restrict__ list,
+                                               int n, int nc, int tpb, int ie, int ifld,
+                                               int fel, int ffld, int cmob, int cdif,
+                                               double fac) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double inv_dr = 1 / a.dr[id];
+  const double* ne = boxp(a, ie, id);
+  const double* E = boxp(a, ifld, id);
+  const int per = (nc + 1) * (D == 2 ? nc : nc * nc);
+  for (int q = lt; q < D * per; q += tpb) {
+    const int d = q / per, w = q - d * per;
+    const int n0 = nc + (d == 0), n1 = nc + (d == 1);
+    int c[3];
+    c[0] = w % n0 + 1;
+    c[1] = (w / n0) % n1 + 1;
+    c[2] = D == 3 ? w / (n0 * n1) + 1 : 0;
+    int m1[3] = {c[0], c[1], c[2]};
+    m1[d] -= 1;
+    const int pos = c[d];
+    const int ic = cidx<D>(nc, c[0], c[1], c[2]);
+    const int im = cidx<D>(nc, m1[0], m1[1], m1[2]);
+    const double fld_avg = 0.5 * (E[ic] + E[im]);
+    int li;
+    double lf;
+    lt_loc(tb, fld_avg, li, lf);
+    const double mob = lt_col(tb, cmob, li, lf);
+    const double dif = lt_col(tb, cdif, li, lf);
+    const double fld = facep(a, ffld, id, d, D)[w];
+    const double vd = -mob * fld;
+    const double n0v = ne[ic], nm1 = ne[im];
+    const double gradc = n0v - nm1;
+    const int ta = d == 0 ? 1 : 0, tbb = d == 2 ? 1 : 2;
+    double f;
+    if (vd < 0.0) {
+      double tmp;
+      if (pos == nc + 1) {
+        tmp = gc2_value<D>(a, ie, id, nc, 2 * d + 2, c[ta], D == 3 ? c[tbb] : 0);
+      } else {
+        int p1[3] = {c[0], c[1], c[2]};
+        p1[d] += 1;
+        tmp = ne[cidx<D>(nc, p1[0], p1[1], p1[2])];
+      }
+      const double gradn = tmp - n0v;
+      f = vd * (n0v - koren_mlim(gradc, gradn));
+      if (f < fac * fld) f = fac * fld;
+    } else {
+      double tmp;
+      if (pos == 1) {
+        tmp = gc2_value<D>(a, ie, id, nc, 2 * d + 1, c[ta], D == 3 ? c[tbb] : 0);
+      } else {
+        int m2[3] = {c[0], c[1], c[2]};
+        m2[d] -= 2;
+        tmp = ne[cidx<D>(nc, m2[0], m2[1], m2[2])];
+      }
+      const double gradp = nm1 - tmp;
+      f = vd * (nm1 + koren_mlim(gradc, gradp));
+      if (f > fac * fld) f = fac * fld;
+    }
+    facep(a, fel, id, d, D)[w] = f - dif * gradc * inv_dr;
+  }
+}
+
+// a$D_consistent_fluxes / flux_from_children (afivo/src/m_aX_core.f90:818-930):
+// one lane per coarse face cell of a (parent, direction) pair
+template <int D>
+__global__ __launch_bounds__(256) void k_flux_fix(Dev a, const int2* __restrict__ pairs, int n,
+                                                  int nc, int fel) {
+  const int hnc = nc >> 1;
+  const int per = D == 2 ? nc : nc * nc;  // coarse face cells per pair
+  const long total = (long)n * per;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
+       q += (long)gridDim.x * blockDim.x) {
+    const int pi = (int)(q / per), w = (int)(q - (long)pi * per);
+    const int id = pairs[pi].x, nb = pairs[pi].y;
+    const int d = d_nb_dim(nb);
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int nb_id = a.nbs[id * 6 + nb - 1];
+    const bool low = d_nb_low(nb);
+    const int ii = low ? 1 : nc + 1, inb = low ? nc + 1 : 1;
+    // coarse face cell (a_, b_) in 1..nc, and the child that covers it
+    const int a_ = w % nc + 1, b_ = D == 3 ? w / nc + 1 : 1;
+    const int ca = (a_ - 1) / hnc, cb = D == 3 ? (b_ - 1) / hnc : 0;
+    int cdix[3] = {0, 0, 0};
+    cdix[d] = low ? 0 : 1;
+    cdix[ta] = ca;
+    if (D == 3) cdix[tb] = cb;
+    const int ich = 1 + cdix[0] + 2 * cdix[1] + 4 * cdix[2];
+    const int cid = a.children[id * 8 + ich - 1];
+    const int sa = 2 * (a_ - ca * hnc) - 1, sb = D == 3 ? 2 * (b_ - cb * hnc) - 1 : 1;
+    const double* src = facep(a, fel, cid, d, D);
+    double* dst = facep(a, fel, nb_id, d, D);
+    int s[3], t[3];
+    s[d] = ii;
+    s[ta] = sa;
+    t[d] = inb;
+    t[ta] = a_;
+    if (D == 2) {
+      const double v0 = src[fidx<D>(nc, d, s[0], s[1], 0)];
+      s[ta] += 1;
+      const double v1 = src[fidx<D>(nc, d, s[0], s[1], 0)];
+      dst[fidx<D>(nc, d, t[0], t[1], 0)] = 0.5 * (v0 + v1);
+    } else {
+      s[tb] = sb;
+      t[tb] = b_;
+      const double v0 = src[fidx<D>(nc, d, s[0], s[1], s[2])];
+      s[ta] += 1;
+      const double v1 = src[fidx<D>(nc, d, s[0], s[1], s[2])];
+      s[ta] -= 1;
+      s[tb] += 1;
+      const double v2 = src[fidx<D>(nc, d, s[0], s[1], s[2])];
+      s[ta] += 1;
+      const double v3 = src[fidx<D>(nc, d, s[0], s[1], s[2])];
+      dst[fidx<D>(nc, d, t[0], t[1], t[2])] = 0.25 * (v0 + v1 + v2 + v3);
+    }
+  }
+}
+
+// NORM2 as the flang runtime evaluates it (running maximum m, scaled sum)
+template <int N>
+__device__ __forceinline__ double norm2_flang(const double* x) {
+  double mx = 0, s = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    const double ax = fabs(x[i]);
+    if (mx == 0) {
+      mx = ax;
+    } else if (ax > mx) {
+      const double q = mx / ax, qq = q * q;
+      s = s * qq;
+      s = s + qq;
+      mx = ax;
+    } else {
+      const double q = ax / mx;
+      s = s + q * q;
+    }
+  }
+  return mx * sqrt(1 + s);
+}
+
+// update_solution (src/streamer_2d.f90:499-542, src/streamer_3d.f90:566-615,
+// src/streamer_cyl.f90:531-570) on leaves: one lane per cell
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_update(Dev a, Tbl tb, const int* __restrict__ list,
+                                                int n, int nc, int tpb, int ie, int ip,
+                                                int ifld, int ieps, int ipho, int fel,
+                                                int calpha, int ceta, int diel, double dt) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double inv_dr = 1 / a.dr[id];
+  double* ne = boxp(a, ie, id);
+  double* ni = boxp(a, ip, id);
+  const double* E = boxp(a, ifld, id);
+  const double* eps = boxp(a, ieps, id);
+  const double* ph = ipho > 0 ? boxp(a, ipho, id) : nullptr;
+  const double* F0 = facep(a, fel, id, 0, D);
+  const double* F1 = facep(a, fel, id, 1, D);
+  const double* F2 = D == 3 ? facep(a, fel, id, 2, D) : nullptr;
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c = cidx<D>(nc, i, j, k);
+    int li;
+    double lf;
+    lt_loc(tb, E[c], li, lf);
+    const double alpha = lt_col(tb, calpha, li, lf);
+    const double eta = lt_col(tb, ceta, li, lf);
+    const double fxl = F0[fidx<D>(nc, 0, i, j, k)], fxh = F0[fidx<D>(nc, 0, i + 1, j, k)];
+    const double fyl = F1[fidx<D>(nc, 1, i, j, k)], fyh = F1[fidx<D>(nc, 1, i, j + 1, k)];
+    double src, sflux;
+    if (D == 2) {
+      const double df[2] = {fxl + fxh, fyl + fyh};
+      src = 0.5 * norm2_flang<2>(df) * (alpha - eta);
+      if (ph) src = src + ph[c];
+      if (CYL) {
+        const double den = i + ioff - 0.5;
+        const double rf1 = (double)(i + ioff - 1) / den, rf2 = (double)(i + ioff) / den;
+        sflux = (fyl - fyh + rf1 * fxl - rf2 * fxh) * inv_dr;
+      } else {
+        sflux = (fxl - fxh + fyl - fyh) * inv_dr;
+      }
+    } else {
+      const double fzl = F2[fidx<D>(nc, 2, i, j, k)], fzh = F2[fidx<D>(nc, 2, i, j, k + 1)];
+      const double df[3] = {fxl + fxh, fyl + fyh, fzl + fzh};
+      src = 0.5 * norm2_flang<3>(df) * (alpha - eta);
+      if (ph) src = src + ph[c];
+      sflux = (fxl - fxh + fyl - fyh + fzl - fzh) * inv_dr;
+    }
+    double e = ne[c] + (src + sflux) * dt;
+    double p = ni[c] + src * dt;
+    if (diel && eps[c] > 1.0) {
+      p = p - e;
+      e = 0;
+    }
+    ne[c] = e;
+    ni[c] = p;
+  }
+}
+
+// average_dens (src/streamer_2d.f90:492-496): whole boxes
+__global__ __launch_bounds__(256) void k_average(Dev a, const int* __restrict__ list, int n,
+                                                 int bl, int tpb, int ie, int ieo, int ip,
+                                                 int ipo) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  double* e = boxp(a, ie, id);
+  const double* eo = boxp(a, ieo, id);
+  double* p = boxp(a, ip, id);
+  const double* po = boxp(a, ipo, id);
+  for (int q = lt; q < bl; q += tpb) {
+    e[q] = 0.5 * (e[q] + eo[q]);
+    p[q] = 0.5 * (p[q] + po[q]);
+  }
+}
+
+// per-box maxima of set_ref_flags (src/streamer_2d.f90:186-192):
+// (dr^2 max|rhs|, max fld, max elec) over the interior
+template <int D>
+__global__ __launch_bounds__(256) void k_summary(Dev a, const int* __restrict__ list, int n,
+                                                 int nc, int ir, int ifld, int ie,
+                                                 double* __restrict__ out) {
+  const int li = blockIdx.x;
+  if (li >= n) return;
+  const int id = list[li];
+  const double* r = boxp(a, ir, id);
+  const double* E = boxp(a, ifld, id);
+  const double* ne = boxp(a, ie, id);
+  double mr = -1.7976931348623157e308, me = mr, mn = mr;
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c = cidx<D>(nc, i, j, k);
+    mr = fmax(mr, fabs(r[c]));
+    me = fmax(me, E[c]);
+    mn = fmax(mn, ne[c]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mr = fmax(mr, __shfl_xor(mr, o, 64));
+    me = fmax(me, __shfl_xor(me, o, 64));
+    mn = fmax(mn, __shfl_xor(mn, o, 64));
+  }
+  __shared__ double sm[3][4];
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][threadIdx.x >> 6] = mr;
+    sm[1][threadIdx.x >> 6] = me;
+    sm[2][threadIdx.x >> 6] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      mr = fmax(mr, sm[0][w]);
+      me = fmax(me, sm[1][w]);
+      mn = fmax(mn, sm[2][w]);
+    }
+    out[(size_t)(id - 1) * 3 + 0] = a.dr[id] * a.dr[id] * mr;
+    out[(size_t)(id - 1) * 3 + 1] = me;
+    out[(size_t)(id - 1) * 3 + 2] = mn;
+  }
+}
+
+// a$D_prolong1_to (afivo/src/m_aX_prolong.f90:99-191), add = false, for every
+// box of the list
+template <int D>
+__global__ __launch_bounds__(256) void k_prolong1(Dev a, const int* __restrict__ list, int n,
+                                                  int nc, int tpb, int iv) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const int hnc = nc >> 1;
+  const int pid = a.parent[id];
+  const int pnc = a.ncell[pid];
+  int off[3];
+  child_off<D>(a, id, nc, 0, off);
+  double* c = boxp(a, iv, id);
+  const double* p = boxp(a, iv, pid);
+  const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % hnc + 1, j = (q / hnc) % hnc + 1, k = D == 3 ? q / (hnc * hnc) + 1 : 0;
+    const int ic = i + off[0], jc = j + off[1], kc = D == 3 ? k + off[2] : 0;
+    const int fi = 2 * i - 1, fj = 2 * j - 1, fk = D == 3 ? 2 * k - 1 : 0;
+    if (D == 2) {
+      const double f0 = 0.5 * p[cidx<D>(pnc, ic, jc, 0)];
+      const double flx = 0.25 * p[cidx<D>(pnc, ic - 1, jc, 0)];
+      const double fhx = 0.25 * p[cidx<D>(pnc, ic + 1, jc, 0)];
+      const double fly = 0.25 * p[cidx<D>(pnc, ic, jc - 1, 0)];
+      const double fhy = 0.25 * p[cidx<D>(pnc, ic, jc + 1, 0)];
+      // the children start from zero (prolong1 without add)
+      c[cidx<D>(nc, fi, fj, 0)] = f0 + flx + fly + 0.0;
+      c[cidx<D>(nc, fi + 1, fj, 0)] = f0 + fhx + fly + 0.0;
+      c[cidx<D>(nc, fi, fj + 1, 0)] = f0 + flx + fhy + 0.0;
+      c[cidx<D>(nc, fi + 1, fj + 1, 0)] = f0 + fhx + fhy + 0.0;
+    } else {
+      const double f0 = 0.25 * p[cidx<D>(pnc, ic, jc, kc)];
+      const double flx = 0.25 * p[cidx<D>(pnc, ic - 1, jc, kc)];
+      const double fhx = 0.25 * p[cidx<D>(pnc, ic + 1, jc, kc)];
+      const double fly = 0.25 * p[cidx<D>(pnc, ic, jc - 1, kc)];
+      const double fhy = 0.25 * p[cidx<D>(pnc, ic, jc + 1, kc)];
+      const double flz = 0.25 * p[cidx<D>(pnc, ic, jc, kc - 1)];
+      const double fhz = 0.25 * p[cidx<D>(pnc, ic, jc, kc + 1)];
+      c[cidx<D>(nc, fi, fj, fk)] = f0 + flx + fly + flz + 0.0;
+      c[cidx<D>(nc, fi + 1, fj, fk)] = f0 + fhx + fly + flz + 0.0;
+      c[cidx<D>(nc, fi, fj + 1, fk)] = f0 + flx + fhy + flz + 0.0;
+      c[cidx<D>(nc, fi + 1, fj + 1, fk)] = f0 + fhx + fhy + flz + 0.0;
+      c[cidx<D>(nc, fi, fj, fk + 1)] = f0 + flx + fly + fhz + 0.0;
+      c[cidx<D>(nc, fi + 1, fj, fk + 1)] = f0 + fhx + fly + fhz + 0.0;
+      c[cidx<D>(nc, fi, fj + 1, fk + 1)] = f0 + flx + fhy + fhz + 0.0;
+      c[cidx<D>(nc, fi + 1, fj + 1, fk + 1)] = f0 + fhx + fhy + fhz + 0.0;
+    }
+  }
+}
+
+// cc(..., iv) = value over whole boxes
+__global__ __launch_bounds__(256) void k_fill(Dev a, const int* __restrict__ list, int n,
+                                              int bl, int tpb, int iv, double value) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  double* p = boxp(a, iv, id);
+  for (int q = lt; q < bl; q += tpb) p[q] = value;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static void chk(hipError_t e, const char* w) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
+}
+
+static Tbl device_table(Pool& P) {
+  if (!P.d_table) throw std::logic_error("no transport table: call a5g_set_table first");
+  Tbl t;
+  t.rc = P.d_table;
+  t.n_rows = P.tbl_rows;
+  t.x_min = P.tbl_xmin;
+  t.inv_dx = P.tbl_inv_dx;
+  return t;
+}
+
+void set_table(Pool& P, int n_rows, int n_cols, double x_min, double inv_dx, const double* rc) {
+  if (n_rows < 2 || n_cols < 1) throw std::invalid_argument("table needs >= 2 rows and >= 1 column");
+  if (P.d_table) chk(hipFree(P.d_table), "hipFree");
+  P.d_table = nullptr;
+  chk(hipMalloc((void**)&P.d_table, (size_t)n_rows * n_cols * sizeof(double)), "hipMalloc table");
+  chk(hipMemcpyAsync(P.d_table, rc, (size_t)n_rows * n_cols * sizeof(double),
+                     hipMemcpyHostToDevice, P.stream), "table upload");
+  chk(hipStreamSynchronize(P.stream), "table sync");
+  P.tbl_rows = n_rows;
+  P.tbl_cols = n_cols;
+  P.tbl_xmin = x_min;
+  P.tbl_inv_dx = inv_dx;
+}
+
+static int box_cells(int D, int nc, bool ghosts) {
+  const int m = ghosts ? nc + 2 : nc;
+  return D == 2 ? m * m : m * m * m;
+}
+
+void space_charge(Pool& P, const StVars& v, double fac) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  for (int lvl = 1; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl), bl = box_cells(D, nc, true);
+    G2 g = g2(n, bl);
+    hipLaunchKernelGGL(k_space_charge, dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                       P.leaves(lvl), n, bl, g.tpb, v.elec, v.pion, v.rhs, fac);
+    chk(hipGetLastError(), "space_charge");
+  }
+}
+
+void fld_from_pot(Pool& P, const StVars& v) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_ids(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    if (D == 2)
+      hipLaunchKernelGGL((k_fld<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, nc, g.tpb, v.phi, v.eps, v.fld, v.f_fld);
+    else
+      hipLaunchKernelGGL((k_fld<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, nc, g.tpb, v.phi, v.eps, v.fld, v.f_fld);
+    chk(hipGetLastError(), "fld_from_pot");
+  }
+}
+
+void fluxes_koren(Pool& P, const StVars& v, double dt) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  Tbl tb = device_table(P);
+  const double eps0 = 8.8541878176e-12, qe = 1.6022e-19;
+  const double fac = -0.8 * eps0 / (qe * dt);
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    if (D == 2)
+      hipLaunchKernelGGL((k_koren<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.fld, v.f_elec, v.f_fld, v.t_mob,
+                         v.t_dif, fac);
+    else
+      hipLaunchKernelGGL((k_koren<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.fld, v.f_elec, v.f_fld, v.t_mob,
+                         v.t_dif, fac);
+    chk(hipGetLastError(), "fluxes_koren");
+  }
+}
+
+void consistent_fluxes(Pool& P, int f_elec) {
+  // a$D_consistent_fluxes: (parent, direction) pairs whose neighbour is a leaf
+  P.sync_topology();
+  const auto& T = P.tree;
+  const int D = T.dim;
+  for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
+    std::vector<int2> pairs;
+    for (int id : T.lv_parents(lvl))
+      for (int nb = 1; nb <= T.nnb; nb++) {
+        const int nb_id = T.nbs[id * 6 + nb - 1];
+        if (nb_id > 0 && !T.has_children(nb_id)) pairs.push_back(make_int2(id, nb));
+      }
+    if (pairs.empty()) continue;
+    const int nc = P.nc_lvl(lvl);
+    int2* d_pairs = nullptr;
+    chk(hipMalloc((void**)&d_pairs, pairs.size() * sizeof(int2)), "hipMalloc");
+    chk(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                       P.stream), "pairs");
+    const long work = (long)pairs.size() * (D == 2 ? nc : nc * nc);
+    const int grid = (int)std::min<long>((work + 255) / 256, 65535);
+    if (D == 2)
+      hipLaunchKernelGGL((k_flux_fix<2>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
+                         (int)pairs.size(), nc, f_elec);
+    else
+      hipLaunchKernelGGL((k_flux_fix<3>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
+                         (int)pairs.size(), nc, f_elec);
+    chk(hipGetLastError(), "consistent_fluxes");
+    chk(hipStreamSynchronize(P.stream), "consistent_fluxes sync");
+    chk(hipFree(d_pairs), "hipFree");
+  }
+}
+
+void update_solution(Pool& P, const StVars& v, double dt) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  Tbl tb = device_table(P);
+  const bool cyl = D == 2 && P.tree.coord == 1;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, box_cells(D, nc, false));
+    if (D == 3)
+      hipLaunchKernelGGL((k_update<3, false>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.pion, v.fld, v.eps, v.pho,
+                         v.f_elec, v.t_alpha, v.t_eta, v.dielectric_rule, dt);
+    else if (cyl)
+      hipLaunchKernelGGL((k_update<2, true>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.pion, v.fld, v.eps, v.pho,
+                         v.f_elec, v.t_alpha, v.t_eta, v.dielectric_rule, dt);
+    else
+      hipLaunchKernelGGL((k_update<2, false>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.pion, v.fld, v.eps, v.pho,
+                         v.f_elec, v.t_alpha, v.t_eta, v.dielectric_rule, dt);
+    chk(hipGetLastError(), "update_solution");
+  }
+}
+
+void average_dens(Pool& P, const StVars& v) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_ids(lvl);
+    if (!n) continue;
+    const int bl = box_cells(D, P.nc_lvl(lvl), true);
+    G2 g = g2(n, bl);
+    hipLaunchKernelGGL(k_average, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl), n,
+                       bl, g.tpb, v.elec, v.elec_old, v.pion, v.pion_old);
+    chk(hipGetLastError(), "average_dens");
+  }
+}
+
+void box_summaries(Pool& P, const StVars& v, double* out_host) {
+  P.sync_topology();
+  const auto& T = P.tree;
+  const int D = T.dim;
+  double* d_out = nullptr;
+  const size_t bytes = (size_t)std::max(T.max_id, 1) * 3 * sizeof(double);
+  chk(hipMalloc((void**)&d_out, bytes), "hipMalloc");
+  chk(hipMemsetAsync(d_out, 0, bytes, P.stream), "memset");
+  for (int lvl = 1; lvl <= T.max_lvl; lvl++) {
+    const int n = P.n_ids(lvl);
+    if (!n) continue;
+    if (D == 2)
+      hipLaunchKernelGGL((k_summary<2>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, P.nc_lvl(lvl), v.rhs, v.fld, v.elec, d_out);
+    else
+      hipLaunchKernelGGL((k_summary<3>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, P.nc_lvl(lvl), v.rhs, v.fld, v.elec, d_out);
+    chk(hipGetLastError(), "box_summaries");
+  }
+  chk(hipMemcpyAsync(out_host, d_out, (size_t)T.max_id * 3 * sizeof(double),
+                     hipMemcpyDeviceToHost, P.stream), "summaries");
+  chk(hipStreamSynchronize(P.stream), "summaries sync");
+  chk(hipFree(d_out), "hipFree");
+}
+
+// device copy of a host id list, grouped by level; calls fn(level, ptr, n)
+template <typename F>
+static void by_level(Pool& P, int n_ids, const int* ids, F fn) {
+  const auto& T = P.tree;
+  for (int lvl = T.min_lvl; lvl <= T.max_lvl; lvl++) {
+    std::vector<int> sel;
+    for (int i = 0; i < n_ids; i++) {
+      const int id = ids[i];
+      if (id < 1 || id > T.max_id || !T.in_use[id])
+        throw std::out_of_range("box id " + std::to_string(id) + " is not in use");
+      if (T.lvl[id] == lvl) sel.push_back(id);
+    }
+    if (sel.empty()) continue;
+    int* d = nullptr;
+    chk(hipMalloc((void**)&d, sel.size() * sizeof(int)), "hipMalloc");
+    chk(hipMemcpyAsync(d, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, P.stream),
+        "ids");
+    fn(lvl, d, (int)sel.size());
+    chk(hipStreamSynchronize(P.stream), "sync");
+    chk(hipFree(d), "hipFree");
+  }
+}
+
+void prolong1_boxes(Pool& P, int n_ids, const int* ids, int iv) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  by_level(P, n_ids, ids, [&](int lvl, const int* d, int n) {
+    if (lvl <= P.tree.min_lvl) throw std::invalid_argument("prolong1: box without parent");
+    const int nc = P.nc_lvl(lvl), hnc = nc / 2;
+    G2 g = g2(n, D == 2 ? hnc * hnc : hnc * hnc * hnc);
+    if (D == 2)
+      hipLaunchKernelGGL((k_prolong1<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), d, n,
+                         nc, g.tpb, iv);
+    else
+      hipLaunchKernelGGL((k_prolong1<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), d, n,
+                         nc, g.tpb, iv);
+    chk(hipGetLastError(), "prolong1");
+  });
+}
+
+void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  by_level(P, n_ids, ids, [&](int lvl, const int* d, int n) {
+    const int bl = box_cells(D, P.nc_lvl(lvl), true);
+    G2 g = g2(n, bl);
+    hipLaunchKernelGGL(k_fill, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), d, n, bl, g.tpb, iv,
+                       value);
+    chk(hipGetLastError(), "fill");
+  });
+}
+
+void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc) {
+  P.sync_topology();
+  by_level(P, n_ids, ids, [&](int lvl, const int* d, int n) { gc_list(P, lvl, d, n, iv, rb, bc); });
+}
+
+}  // namespace a5g

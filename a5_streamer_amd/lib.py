@@ -215,6 +215,79 @@ class Tree:
     def tree_copy_cc(self, a, b):
         _call("a5g_tree_copy_cc", self.h, C.c_int(a), C.c_int(b))
 
+    # --- transport table and streamer physics -----------------------------
+    def table_create(self, x_min, x_max, n_rows, n_cols):
+        self._tbl = (n_rows, n_cols)
+        _call("a5g_table_create", self.h, C.c_double(x_min), C.c_double(x_max),
+              C.c_int(n_rows), C.c_int(n_cols))
+
+    def table_set_col(self, col, xx, yy):
+        xa, px = _d(xx)
+        ya, py = _d(yy)
+        _call("a5g_table_set_col", self.h, C.c_int(col), C.c_int(len(xa)), px, py)
+
+    def table_get(self):
+        n_rows, n_cols = self._tbl
+        rc = np.zeros(n_rows * n_cols)
+        xm, idx = C.c_double(), C.c_double()
+        _call("a5g_table_get", self.h, _d(rc)[1], C.byref(xm), C.byref(idx))
+        return rc.reshape(n_cols, n_rows), xm.value, idx.value
+
+    def set_table(self, rows_cols, x_min, inv_dx):
+        rc = np.asarray(rows_cols, float)
+        n_cols, n_rows = rc.shape
+        self._tbl = (n_rows, n_cols)
+        _call("a5g_set_table", self.h, C.c_int(n_rows), C.c_int(n_cols),
+              C.c_double(x_min), C.c_double(inv_dx), _d(rc)[1])
+
+    def streamer_setup(self, elec=1, pion=2, elec_old=3, pion_old=4, phi=5, fld=6,
+                       rhs=7, pho=-1, eps=9, f_elec=1, f_fld=2, t_mob=1, t_dif=2,
+                       t_alpha=3, t_eta=4, dielectric_rule=True):
+        v = np.array([elec, pion, elec_old, pion_old, phi, fld, rhs, pho, eps,
+                      f_elec, f_fld, t_mob, t_dif, t_alpha, t_eta,
+                      int(dielectric_rule)], np.int32)
+        _call("a5g_streamer_setup", self.h, _i(v)[1])
+
+    def space_charge(self, fac):
+        _call("a5g_space_charge", self.h, C.c_double(fac))
+
+    def fld_from_pot(self):
+        _call("a5g_fld_from_pot", self.h)
+
+    def fluxes_koren(self, dt):
+        _call("a5g_fluxes_koren", self.h, C.c_double(dt))
+
+    def consistent_fluxes(self, f_ixs):
+        a, p = _i(f_ixs)
+        _call("a5g_consistent_fluxes", self.h, C.c_int(len(a)), p)
+
+    def update_solution(self, dt):
+        _call("a5g_update_solution", self.h, C.c_double(dt))
+
+    def average_dens(self):
+        _call("a5g_average_dens", self.h)
+
+    def box_summaries(self):
+        n = self.info()["max_id"]
+        out = np.zeros((n, 3))
+        _call("a5g_box_summaries", self.h, _d(out)[1])
+        return out
+
+    def prolong1_to(self, ids, iv, add=False):
+        if add:
+            raise A5GError("prolong1_to: only add=False (prolong_to_new_boxes) is provided")
+        a, p = _i(np.atleast_1d(ids))
+        _call("a5g_prolong1_to", self.h, C.c_int(len(a)), p, C.c_int(iv))
+
+    def fill_cc(self, ids, iv, value):
+        a, p = _i(np.atleast_1d(ids))
+        _call("a5g_fill_cc", self.h, C.c_int(len(a)), p, C.c_int(iv), C.c_double(value))
+
+    def gc_boxes(self, ids, iv, rb, bc):
+        a, p = _i(np.atleast_1d(ids))
+        _call("a5g_gc_boxes", self.h, C.c_int(len(a)), p, C.c_int(iv), C.c_int(rb),
+              C.c_int(bc))
+
     def set_gaussians(self, iv, blobs, base=0.0):
         b, pb = _d(np.asarray(blobs, float).reshape(-1, self.dim + 2))
         _call("a5g_set_gaussians", self.h, C.c_int(iv), C.c_int(b.shape[0]), pb,

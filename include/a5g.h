@@ -106,6 +106,61 @@ int a5g_tree_copy_cc(a5g_tree *t, int iv_from, int iv_to);
 /* a$D_tree_min_cc / a$D_tree_max_cc, m_aX_utils.f90:559-619 */
 int a5g_tree_minmax_cc(a5g_tree *t, int iv, double *vmin, double *vmax);
 
+/* ---- transport lookup table (fosito/m_lookup_table.f90) --------------- */
+
+/* LT_create: a table of n_rows on the uniform grid [x_min, x_max] */
+int a5g_table_create(a5g_tree *t, double x_min, double x_max, int n_rows,
+                     int n_cols);
+/* LT_set_col: column col (1-based) from scattered data (xx, yy), resampled
+ * by linear interpolation with end clamping (LT_lin_interp_list) */
+int a5g_table_set_col(a5g_tree *t, int col, int n, const double *xx,
+                      const double *yy);
+/* The table as built: rows_cols(n_rows, n_cols) column-major, x_min,
+ * inv_dx = (n_rows - 1) / (x_max - x_min) */
+int a5g_table_get(a5g_tree *t, double *rows_cols, double *x_min,
+                  double *inv_dx);
+/* A table built by the caller (e.g. the drivers' ST_load_transport_data) */
+int a5g_set_table(a5g_tree *t, int n_rows, int n_cols, double x_min,
+                  double inv_dx, const double *rows_cols);
+
+/* ---- streamer physics (src/streamer_{2d,3d,cyl}.f90) ---------------- */
+
+/* Variable indices used by the calls below (src/m_streamer.f90:16-43),
+ * vars[16] = i_elec, i_pion, i_elec_old, i_pion_old, i_phi, i_fld, i_rhs,
+ * i_pho (<= 0: no photoionisation source), i_eps, f_elec, f_fld, and the
+ * table columns mobility, diffusion, alpha, eta, then 1 when the update
+ * moves electrons in dielectric cells to the ions (2D/3D drivers) or 0
+ * (cylindrical driver, src/streamer_cyl.f90:531-570). */
+int a5g_streamer_setup(a5g_tree *t, const int *vars);
+/* compute_fld's source term on leaves: rhs = fac (n_e - n_i), whole boxes
+ * (src/streamer_2d.f90:318-329) */
+int a5g_space_charge(a5g_tree *t, double fac);
+/* fld_from_pot on every box (src/streamer_2d.f90:346-381,
+ * src/streamer_3d.f90:348-395) */
+int a5g_fld_from_pot(a5g_tree *t);
+/* fluxes_koren on leaves (src/streamer_2d.f90:384-489,
+ * src/streamer_3d.f90:396-554, src/streamer_cyl.f90:416-521) */
+int a5g_fluxes_koren(a5g_tree *t, double dt);
+/* a$D_consistent_fluxes, afivo/src/m_aX_core.f90:818-930 */
+int a5g_consistent_fluxes(a5g_tree *t, int n, const int *f_ixs);
+/* update_solution on leaves (src/streamer_2d.f90:499-542,
+ * src/streamer_3d.f90:566-615, src/streamer_cyl.f90:531-570) */
+int a5g_update_solution(a5g_tree *t, double dt);
+/* average_dens on every box, whole boxes (src/streamer_2d.f90:492-496) */
+int a5g_average_dens(a5g_tree *t);
+/* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
+ * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
+int a5g_box_summaries(a5g_tree *t, double *out);
+/* a$D_prolong1_to (afivo/src/m_aX_prolong.f90:99-191, add = .false.) for
+ * the boxes ids[0..n-1] (prolong_to_new_boxes, src/streamer_2d.f90:592-617) */
+int a5g_prolong1_to(a5g_tree *t, int n_ids, const int *ids, int iv);
+/* cc(..., iv) = value on whole boxes (e.g. set_box_eps with eps = 1) */
+int a5g_fill_cc(a5g_tree *t, int n_ids, const int *ids, int iv, double value);
+/* a$D_gc_box for the boxes ids[0..n-1], level by level (rules as in
+ * a5g_gc_tree) */
+int a5g_gc_boxes(a5g_tree *t, int n_ids, const int *ids, int iv, int rb,
+                 int bc);
+
 /* ---- initial conditions --------------------------------------------- */
 
 /* cc(..., iv) = base + sum_k amp_k exp(-|x - c_k|^2 / w_k^2) at every cell

@@ -1768,3 +1768,367 @@ double o_koren_mlim(double a, double b) {
   if (aa <= 2.5 * ab) return sixth * (b + 2 * a);
   return b;
 }
+
+/* ------------------------------------------------------------------ */
+/* streamer physics                                                     */
+/* ------------------------------------------------------------------ */
+
+struct otable {
+  int n_rows, n_cols;
+  double x_min, inv_dx;
+  double *rc; /* rows_cols(n_rows, n_cols), column-major */
+};
+
+otable *o_lt_create(double x_min, double x_max, int n_rows, int n_cols) {
+  /* LT_create, fosito/m_lookup_table.f90:46-60 */
+  otable *t = calloc(1, sizeof(otable));
+  t->n_rows = n_rows;
+  t->n_cols = n_cols;
+  t->x_min = x_min;
+  t->inv_dx = (n_rows - 1) / (x_max - x_min);
+  t->rc = calloc((size_t)n_rows * n_cols, sizeof(double));
+  return t;
+}
+
+void o_lt_free(otable *t) {
+  if (!t) return;
+  free(t->rc);
+  free(t);
+}
+
+static int fi_adaptive(int n, const double *list, double val) {
+  /* FI_adaptive_r (fosito/m_find_index.f90:56-67): smallest 1-based ix with
+   * val <= list(ix), linear below 40 entries, bisection above */
+  if (n < 40) {
+    int ix;
+    for (ix = 1; ix <= n; ix++)
+      if (val <= list[ix - 1]) break;
+    return ix;
+  }
+  int lo = 1, hi = n;
+  while (lo < hi) {
+    int mid = lo + (hi - lo) / 2;
+    if (val <= list[mid - 1]) hi = mid;
+    else lo = mid + 1;
+  }
+  if (val > list[lo - 1]) lo = n + 1;
+  return lo;
+}
+
+static double lin_interp_list(int n, const double *x, const double *y, double xv) {
+  /* LT_lin_interp_list, fosito/m_lookup_table.f90:201-221 */
+  if (xv <= x[0]) return y[0];
+  if (xv >= x[n - 1]) return y[n - 1];
+  int ix = fi_adaptive(n, x, xv);
+  double temp = (xv - x[ix - 2]) / (x[ix - 1] - x[ix - 2]);
+  return (1 - temp) * y[ix - 2] + temp * y[ix - 1];
+}
+
+void o_lt_set_col(otable *t, int col, int n, const double *xx, const double *yy) {
+  /* LT_set_col / get_spaced_data / LT_get_xdata,
+   * fosito/m_lookup_table.f90:62-72, 180-198 */
+  double dx = 1 / t->inv_dx;
+  for (int ix = 1; ix <= t->n_rows; ix++) {
+    double xv = t->x_min + (ix - 1) * dx;
+    t->rc[(size_t)(col - 1) * t->n_rows + (ix - 1)] = lin_interp_list(n, xx, yy, xv);
+  }
+}
+
+static void lt_loc(const otable *t, double x, int *low_ix, double *low_frac) {
+  /* get_loc, fosito/m_lookup_table.f90:107-125 */
+  double frac = (x - t->x_min) * t->inv_dx;
+  int li = (int)ceil(frac);
+  double lf = li - frac;
+  if (li < 1) {
+    li = 1;
+    lf = 1;
+  } else if (li >= t->n_rows) {
+    li = t->n_rows - 1;
+    lf = 0;
+  }
+  *low_ix = li;
+  *low_frac = lf;
+}
+
+static double lt_col_at(const otable *t, int col, int li, double lf) {
+  /* LT_get_col_at_loc, fosito/m_lookup_table.f90:154-161 */
+  const double *c = t->rc + (size_t)(col - 1) * t->n_rows;
+  return lf * c[li - 1] + (1 - lf) * c[li];
+}
+
+double o_lt_get_col(const otable *t, int col, double x) {
+  int li;
+  double lf;
+  lt_loc(t, x, &li, &lf);
+  return lt_col_at(t, col, li, lf);
+}
+
+void o_lt_data(const otable *t, double *rc, double *x_min, double *inv_dx) {
+  memcpy(rc, t->rc, sizeof(double) * (size_t)t->n_rows * t->n_cols);
+  *x_min = t->x_min;
+  *inv_dx = t->inv_dx;
+}
+
+double o_norm2(int n, const double *x) {
+  /* NORM2 as the flang runtime evaluates it: running maximum m and
+   * s = sum((x/m)^2) over the others, result m sqrt(1 + s) */
+  double mx = 0, s = 0;
+  for (int i = 0; i < n; i++) {
+    double ax = fabs(x[i]);
+    if (mx == 0) {
+      mx = ax;
+    } else if (ax > mx) {
+      double q = mx / ax, qq = q * q;
+      s = s * qq;
+      s = s + qq;
+      mx = ax;
+    } else {
+      double q = ax / mx;
+      s = s + q * q;
+    }
+  }
+  return mx * sqrt(1 + s);
+}
+
+void o_space_charge(otree *t, const o_vars *v, double fac) {
+  /* compute_fld (src/streamer_2d.f90:318-329): leaves of levels >= 1 */
+  for (int lvl = 1; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int i = 0; i < lv->n; i++) {
+      int id = lv->v[i], n = blen(t, id);
+      double *r = CC(t, id, v->rhs), *e = CC(t, id, v->elec), *p = CC(t, id, v->pion);
+      for (int m = 0; m < n; m++) r[m] = fac * (e[m] - p[m]);
+    }
+  }
+}
+
+static void fld_box(otree *t, const o_vars *v, int id) {
+  /* fld_from_pot, src/streamer_2d.f90:346-381, src/streamer_3d.f90:348-395,
+   * src/streamer_cyl.f90:356-392 */
+  int nc = t->ncell[id], D = t->dim;
+  double inv_dr = 1 / t->dr[id];
+  const double *p = CC(t, id, v->phi), *e = CC(t, id, v->eps);
+  double *fl = CC(t, id, v->fld);
+  double *F[3];
+  for (int d = 0; d < D; d++) F[d] = FC(t, id, v->f_fld, d);
+  int kmax = D == 3 ? nc : 1;
+  for (int d = 0; d < D; d++) {
+    int n0 = nc + (d == 0), n1 = nc + (d == 1), n2 = D == 3 ? nc + (d == 2) : 1;
+    for (int k = 1; k <= n2; k++)
+      for (int j = 1; j <= n1; j++)
+        for (int i = 1; i <= n0; i++) {
+          int a[3] = {i, j, D == 3 ? k : 0}, b[3] = {i, j, D == 3 ? k : 0};
+          a[d] -= 1; /* cell on the low side of face (i, j, k) */
+          size_t ia = D == 2 ? I2(nc, a[0], a[1]) : I3(nc, a[0], a[1], a[2]);
+          size_t ib = D == 2 ? I2(nc, b[0], b[1]) : I3(nc, b[0], b[1], b[2]);
+          int fpos = (d == 0) ? i : (d == 1) ? j : k;
+          double val;
+          if (fpos == 1) {
+            val = 2 * inv_dr * (p[ia] - p[ib]) * e[ia] / (e[ib] + e[ia]);
+          } else if (fpos == nc + 1) {
+            val = 2 * inv_dr * (p[ia] - p[ib]) * e[ib] / (e[ib] + e[ia]);
+          } else {
+            val = inv_dr * (p[ia] - p[ib]);
+          }
+          F[d][FI(nc, D, d, i, j, k)] = val;
+        }
+  }
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) {
+        double sx = F[0][FI(nc, D, 0, i, j, k)] + F[0][FI(nc, D, 0, i + 1, j, k)];
+        double sy = F[1][FI(nc, D, 1, i, j, k)] + F[1][FI(nc, D, 1, i, j + 1, k)];
+        if (D == 2) {
+          fl[I2(nc, i, j)] = sqrt(0.25 * (sx * sx) + 0.25 * (sy * sy));
+        } else {
+          double sz = F[2][FI(nc, D, 2, i, j, k)] + F[2][FI(nc, D, 2, i, j, k + 1)];
+          fl[I3(nc, i, j, k)] = 0.5 * sqrt(sx * sx + sy * sy + sz * sz);
+        }
+      }
+}
+
+void o_fld_from_pot(otree *t, const o_vars *v) {
+  /* a$D_loop_box(tree, fld_from_pot): every box of every level */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int i = 0; i < ids->n; i++) fld_box(t, v, ids->v[i]);
+  }
+}
+
+static double koren_face(double vd, double c_up, double gradc, double g_other, double fac,
+                         double fld, int neg) {
+  /* one advective face flux of fluxes_koren (src/streamer_2d.f90:415-437) */
+  double f;
+  if (neg) {
+    f = vd * (c_up - o_koren_mlim(gradc, g_other));
+    if (f < fac * fld) f = fac * fld;
+  } else {
+    f = vd * (c_up + o_koren_mlim(gradc, g_other));
+    if (f > fac * fld) f = fac * fld;
+  }
+  return f;
+}
+
+void o_fluxes_koren(otree *t, const o_vars *v, const otable *tbl, double dt) {
+  /* fluxes_koren, src/streamer_2d.f90:384-489 (3D: src/streamer_3d.f90:396-554,
+   * cylindrical: src/streamer_cyl.f90:416-521), on leaves */
+  const double eps0 = 8.8541878176e-12, qe = 1.6022e-19;
+  double fac = -0.8 * eps0 / (qe * dt);
+  int D = t->dim;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id];
+      double inv_dr = 1 / t->dr[id];
+      size_t fl = D == 2 ? (size_t)nc : (size_t)nc * nc;
+      double *gc = malloc(sizeof(double) * fl * t->nnb);
+      o_gc2_box(t, id, v->elec, gc);
+      const double *n = CC(t, id, v->elec), *E = CC(t, id, v->fld);
+      int kmax = D == 3 ? nc : 1;
+      for (int d = 0; d < D; d++) {
+        double *Fe = FC(t, id, v->f_elec, d), *Ff = FC(t, id, v->f_fld, d);
+        int n0 = nc + (d == 0), n1 = nc + (d == 1), n2 = D == 3 ? nc + (d == 2) : 1;
+        int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        for (int k = 1; k <= n2; k++)
+          for (int j = 1; j <= n1; j++)
+            for (int i = 1; i <= n0; i++) {
+              int c[3] = {i, j, D == 3 ? k : 0};
+#define CI(a0, a1, a2) (D == 2 ? I2(nc, a0, a1) : I3(nc, a0, a1, a2))
+              int m1[3] = {c[0], c[1], c[2]}, m2[3] = {c[0], c[1], c[2]}, p1[3] = {c[0], c[1], c[2]};
+              m1[d] -= 1;
+              m2[d] -= 2;
+              p1[d] += 1;
+              int pos = c[d];
+              double fld_avg = 0.5 * (E[CI(c[0], c[1], c[2])] + E[CI(m1[0], m1[1], m1[2])]);
+              int li;
+              double lf;
+              lt_loc(tbl, fld_avg, &li, &lf);
+              double mob = lt_col_at(tbl, v->t_mob, li, lf);
+              double dif = lt_col_at(tbl, v->t_dif, li, lf);
+              size_t fi = FI(nc, D, d, i, j, k);
+              double fld = Ff[fi];
+              double vd = -mob * fld;
+              double nc0 = n[CI(c[0], c[1], c[2])], nm1 = n[CI(m1[0], m1[1], m1[2])];
+              double gradc = nc0 - nm1;
+              size_t gpos = (size_t)(c[ta] - 1) + (D == 3 ? (size_t)nc * (c[tb] - 1) : 0);
+              double f;
+              if (vd < 0.0) {
+                double tmp = pos == nc + 1 ? gc[(size_t)(2 * d + 1) * fl + gpos]
+                                           : n[CI(p1[0], p1[1], p1[2])];
+                f = koren_face(vd, nc0, gradc, tmp - nc0, fac, fld, 1);
+              } else {
+                double tmp = pos == 1 ? gc[(size_t)(2 * d) * fl + gpos]
+                                      : n[CI(m2[0], m2[1], m2[2])];
+                f = koren_face(vd, nm1, gradc, nm1 - tmp, fac, fld, 0);
+              }
+              Fe[fi] = f - dif * gradc * inv_dr;
+#undef CI
+            }
+      }
+      (void)kmax;
+      free(gc);
+    }
+  }
+}
+
+void o_update_solution(otree *t, const o_vars *v, const otable *tbl, double dt) {
+  /* update_solution, src/streamer_2d.f90:499-542 (3D :566-615, cylindrical
+   * src/streamer_cyl.f90:531-570), on leaves */
+  int D = t->dim;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id];
+      double inv_dr = 1 / t->dr[id];
+      double *ne = CC(t, id, v->elec), *ni = CC(t, id, v->pion);
+      const double *E = CC(t, id, v->fld), *eps = CC(t, id, v->eps);
+      const double *ph = v->photoi ? CC(t, id, v->pho) : NULL;
+      const double *F[3];
+      for (int d = 0; d < D; d++) F[d] = FC(t, id, v->f_elec, d);
+      int ioff = (IXP(t, id)[0] - 1) * nc;
+      int kmax = D == 3 ? nc : 1;
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            int li;
+            double lf;
+            lt_loc(tbl, E[c], &li, &lf);
+            double alpha = lt_col_at(tbl, v->t_alpha, li, lf);
+            double eta = lt_col_at(tbl, v->t_eta, li, lf);
+            double fxl = F[0][FI(nc, D, 0, i, j, k)], fxh = F[0][FI(nc, D, 0, i + 1, j, k)];
+            double fyl = F[1][FI(nc, D, 1, i, j, k)], fyh = F[1][FI(nc, D, 1, i, j + 1, k)];
+            double df[3];
+            df[0] = fxl + fxh;
+            df[1] = fyl + fyh;
+            double fzl = 0, fzh = 0;
+            if (D == 3) {
+              fzl = F[2][FI(nc, D, 2, i, j, k)];
+              fzh = F[2][FI(nc, D, 2, i, j, k + 1)];
+              df[2] = fzl + fzh;
+            }
+            double src = 0.5 * o_norm2(D, df) * (alpha - eta);
+            if (ph) src = src + ph[c];
+            double sflux;
+            if (D == 2 && t->coord == 1) {
+              double den = i + ioff - 0.5;
+              double rf1 = (double)(i + ioff - 1) / den, rf2 = (double)(i + ioff) / den;
+              sflux = (fyl - fyh + rf1 * fxl - rf2 * fxh) * inv_dr;
+            } else if (D == 2) {
+              sflux = (fxl - fxh + fyl - fyh) * inv_dr;
+            } else {
+              sflux = (fxl - fxh + fyl - fyh + fzl - fzh) * inv_dr;
+            }
+            ne[c] = ne[c] + (src + sflux) * dt;
+            ni[c] = ni[c] + src * dt;
+          }
+      if (v->dielectric_rule) {
+        for (int k = 1; k <= kmax; k++)
+          for (int j = 1; j <= nc; j++)
+            for (int i = 1; i <= nc; i++) {
+              size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+              if (eps[c] > 1.0) {
+                ni[c] = ni[c] - ne[c];
+                ne[c] = 0;
+              }
+            }
+      }
+    }
+  }
+}
+
+void o_average_dens(otree *t, const o_vars *v) {
+  /* average_dens (src/streamer_2d.f90:492-496) on every box, whole boxes */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = 0; q < ids->n; q++) {
+      int id = ids->v[q], n = blen(t, id);
+      double *e = CC(t, id, v->elec), *eo = CC(t, id, v->elec_old);
+      double *p = CC(t, id, v->pion), *po = CC(t, id, v->pion_old);
+      for (int m = 0; m < n; m++) e[m] = 0.5 * (e[m] + eo[m]);
+      for (int m = 0; m < n; m++) p[m] = 0.5 * (p[m] + po[m]);
+    }
+  }
+}
+
+void o_box_summaries(const otree *t, const o_vars *v, double *out) {
+  /* the maxima set_ref_flags takes per box (src/streamer_2d.f90:186-192) */
+  for (int id = 1; id <= t->max_id; id++) {
+    out[(id - 1) * 3 + 0] = out[(id - 1) * 3 + 1] = out[(id - 1) * 3 + 2] = 0;
+    if (!t->in_use[id] || t->lvl[id] < 1) continue;
+    int nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+    const double *r = CC(t, id, v->rhs), *E = CC(t, id, v->fld), *n = CC(t, id, v->elec);
+    double mr = -1.7976931348623157e308, me = mr, mn = mr;
+    for (int k = 1; k <= kmax; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++) {
+          size_t c = t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+          if (fabs(r[c]) > mr) mr = fabs(r[c]);
+          if (E[c] > me) me = E[c];
+          if (n[c] > mn) mn = n[c];
+        }
+    out[(id - 1) * 3 + 0] = t->dr[id] * t->dr[id] * mr;
+    out[(id - 1) * 3 + 1] = me;
+    out[(id - 1) * 3 + 2] = mn;
+  }
+}

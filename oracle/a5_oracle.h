@@ -71,4 +71,42 @@ double o_max_residual(otree *t);
 /* Koren-limited flux limiter, src/m_streamer.f90:339-362 */
 double o_koren_mlim(double a, double b);
 
+/* ---- streamer physics (src/streamer_{2d,3d,cyl}.f90, fosito) ---------- */
+
+/* Uniform lookup table, fosito/m_lookup_table.f90 (LT_table_t). */
+typedef struct otable otable;
+otable *o_lt_create(double x_min, double x_max, int n_rows, int n_cols);
+void o_lt_free(otable *t);
+/* LT_set_col: linear resampling of (xx, yy) onto the table grid */
+void o_lt_set_col(otable *t, int col, int n, const double *xx, const double *yy);
+double o_lt_get_col(const otable *t, int col, double x);
+/* rows_cols(n_rows, n_cols), column-major, and x_min / inv_dx */
+void o_lt_data(const otable *t, double *rows_cols, double *x_min, double *inv_dx);
+
+/* norm2 of the flang runtime (scaled sum of squares) */
+double o_norm2(int n, const double *x);
+
+/* Variable indices of the streamer models (src/m_streamer.f90:16-43). */
+typedef struct {
+  int elec, pion, elec_old, pion_old, phi, fld, rhs, pho, eps;
+  int f_elec, f_fld;
+  int t_mob, t_dif, t_alpha, t_eta; /* table columns */
+  int dielectric_rule;              /* 2D/3D update: n_e -> n_i in eps > 1 */
+  int photoi;                       /* add i_pho to the source term */
+} o_vars;
+
+/* compute_fld's right-hand side on leaves, whole boxes: rhs = fac (ne - ni) */
+void o_space_charge(otree *t, const o_vars *v, double fac);
+/* fld_from_pot on every box */
+void o_fld_from_pot(otree *t, const o_vars *v);
+/* fluxes_koren on leaves */
+void o_fluxes_koren(otree *t, const o_vars *v, const otable *tbl, double dt);
+/* update_solution on leaves */
+void o_update_solution(otree *t, const o_vars *v, const otable *tbl, double dt);
+/* average_dens on every box, whole boxes */
+void o_average_dens(otree *t, const o_vars *v);
+/* per-box maxima for the refinement criterion: out[id-1] = (dr^2 max|rhs|,
+ * max fld, max elec) over the interior, boxes of levels >= 1 */
+void o_box_summaries(const otree *t, const o_vars *v, double *out);
+
 #endif

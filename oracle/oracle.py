@@ -40,8 +40,17 @@ def lib():
         L.o_max_residual.argtypes = [C.c_void_p]
         L.o_koren_mlim.restype = C.c_double
         L.o_koren_mlim.argtypes = [C.c_double, C.c_double]
-        for name in dir(L):
-            pass
+        L.o_lt_create.restype = C.c_void_p
+        L.o_lt_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int]
+        L.o_lt_free.argtypes = [C.c_void_p]
+        L.o_lt_set_col.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp]
+        L.o_lt_get_col.restype = C.c_double
+        L.o_lt_get_col.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.o_lt_data.argtypes = [C.c_void_p, dp, dp, dp]
+        L.o_norm2.restype = C.c_double
+        L.o_norm2.argtypes = [C.c_int, dp]
+        for name in ("o_space_charge", "o_fld_from_pot", "o_average_dens"):
+            getattr(L, name).argtypes = None
         _lib = L
     return _lib
 
@@ -234,3 +243,77 @@ class OTree:
 
 def koren_mlim(a, b):
     return lib().o_koren_mlim(a, b)
+
+
+class OVars(C.Structure):
+    """Variable indices of the streamer models (src/m_streamer.f90:16-43)."""
+    _fields_ = [(n, C.c_int) for n in (
+        "elec", "pion", "elec_old", "pion_old", "phi", "fld", "rhs", "pho", "eps",
+        "f_elec", "f_fld", "t_mob", "t_dif", "t_alpha", "t_eta",
+        "dielectric_rule", "photoi")]
+
+
+def streamer_vars(dielectric_rule=True, photoi=False):
+    return OVars(1, 2, 3, 4, 5, 6, 7, 8, 9, 1, 2, 1, 2, 3, 4,
+                 int(dielectric_rule), int(photoi))
+
+
+class OTable:
+    """fosito's uniform lookup table (m_lookup_table.f90)."""
+
+    def __init__(self, x_min, x_max, n_rows, n_cols):
+        self.L = lib()
+        self.n_rows, self.n_cols = n_rows, n_cols
+        self.h = C.c_void_p(self.L.o_lt_create(x_min, x_max, n_rows, n_cols))
+
+    def __del__(self):
+        try:
+            self.L.o_lt_free(self.h)
+        except Exception:
+            pass
+
+    def set_col(self, col, xx, yy):
+        xa, px = _d(xx)
+        ya, py = _d(yy)
+        self.L.o_lt_set_col(self.h, col, len(xa), px, py)
+
+    def get_col(self, col, x):
+        return self.L.o_lt_get_col(self.h, col, x)
+
+    def data(self):
+        rc = np.zeros(self.n_rows * self.n_cols)
+        xm, idx = C.c_double(), C.c_double()
+        self.L.o_lt_data(self.h, _d(rc)[1], C.byref(xm), C.byref(idx))
+        return rc.reshape(self.n_cols, self.n_rows), xm.value, idx.value
+
+
+def norm2(x):
+    a, p = _d(x)
+    return lib().o_norm2(len(a), p)
+
+
+def space_charge(T, v, fac):
+    T.L.o_space_charge(T.h, C.byref(v), C.c_double(fac))
+
+
+def fld_from_pot(T, v):
+    T.L.o_fld_from_pot(T.h, C.byref(v))
+
+
+def fluxes_koren(T, v, tbl, dt):
+    T.L.o_fluxes_koren(T.h, C.byref(v), tbl.h, C.c_double(dt))
+
+
+def update_solution(T, v, tbl, dt):
+    T.L.o_update_solution(T.h, C.byref(v), tbl.h, C.c_double(dt))
+
+
+def average_dens(T, v):
+    T.L.o_average_dens(T.h, C.byref(v))
+
+
+def box_summaries(T, v):
+    n = T.info()["max_id"]
+    out = np.zeros((n, 3))
+    T.L.o_box_summaries(T.h, C.byref(v), _d(out)[1])
+    return out

@@ -85,6 +85,14 @@ contains
     end do
   end function c2f
 
+  ! the compiler's NORM2 intrinsic, which the update's source term uses
+  function h_norm2(n, x) result(r) bind(C, name="ref_norm2")
+    integer(c_int), value :: n
+    real(c_double)        :: x(n)
+    real(c_double)        :: r
+    r = norm2(x)
+  end function h_norm2
+
   function h_morton2(i, j) result(m) bind(C, name="ref_morton_from_ix2")
     integer(c_int), value :: i, j
     integer(c_int64_t)    :: m

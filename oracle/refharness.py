@@ -29,7 +29,7 @@ def lib():
                                "(make -C oracle ref)")
         _lib = C.CDLL(LIB_PATH)
         for name in ("ref_koren_mlim", "ref_lt_get_col", "ref2_min_dr",
-                     "ref3_min_dr"):
+                     "ref3_min_dr", "ref_norm2"):
             getattr(_lib, name).restype = C.c_double
         _lib.ref_morton_from_ix2.restype = C.c_int64
         _lib.ref_morton_from_ix3.restype = C.c_int64
@@ -220,6 +220,11 @@ class RefTree:
 
 def koren_mlim(a, b):
     return lib().ref_koren_mlim(C.c_double(a), C.c_double(b))
+
+
+def norm2(x):
+    a, p = _d(x)
+    return lib().ref_norm2(C.c_int(len(a)), p)
 
 
 def morton2(i, j):
