@@ -36,6 +36,8 @@ Dev Pool::dev() {
   a.fs = fs();
   a.vs = (size_t)(dev_cap + 1) * a.bs;
   a.fvs = (size_t)(dev_cap + 1) * tree.dim * a.fs;
+  a.cc32 = cc32;
+  a.vs32 = (size_t)(cap32 + 1) * a.bs;
   a.coord = tree.coord;
   a.lvl = d_lvl;
   a.ix = d_ix;
@@ -88,6 +90,9 @@ void Pool::ensure_capacity(int need) {
   cc = ncc_p;
   fc = nfc_p;
   dev_cap = ncap;
+  if (cc32) hcheck(hipFree(cc32), "hipFree");
+  cc32 = nullptr;
+  n32 = cap32 = 0;
   realloc_dev(&d_lvl, ncap + 1);
   realloc_dev(&d_ix, (size_t)(ncap + 1) * 3);
   realloc_dev(&d_parent, ncap + 1);
@@ -182,6 +187,9 @@ void Pool::apply_moves(const std::vector<int>& moves, int n_used) {
   hcheck(hipFree(d_map), "free");
   cc = ncc_p;
   fc = nfc_p;
+  if (cc32) hcheck(hipFree(cc32), "hipFree");
+  cc32 = nullptr;
+  n32 = cap32 = 0;
   if (ncap != dev_cap) {
     dev_cap = ncap;
     realloc_dev(&d_lvl, ncap + 1);
@@ -232,6 +240,8 @@ void Pool::release() {
   };
   f(d_table);
   d_table = nullptr;
+  f(cc32);
+  cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
   f(d_dr); f(d_rmin); f(d_lists); f(d_red);
   if (h_red) (void)hipHostFree(h_red);
@@ -633,6 +643,28 @@ int a5g_fas_vcycle(a5g_tree* t, int max_lvl, int set_residual) {
   API_END
 }
 
+int a5g_mg_solve_mixed(a5g_tree* t, double tol, int max_outer, int n_inner, int zero_start,
+                       int* n_outer, double* rel_res) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  auto& P = t->P;
+  if (zero_start) {
+    P.sync_topology();
+    std::vector<int> all;
+    for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++)
+      all.insert(all.end(), P.tree.lv_ids(l).begin(), P.tree.lv_ids(l).end());
+    a5g::fill_boxes(P, (int)all.size(), all.data(), P.mg.i_phi, 0.0);
+    a5g::gc_tree(P, P.mg.i_phi, 0, 0);
+  }
+  int it = 0;
+  double rel = 0;
+  a5g::mg_solve_mixed(P, tol, max_outer, n_inner, &it, &rel);
+  if (n_outer) *n_outer = it;
+  if (rel_res) *rel_res = rel;
+  API_END
+}
+
 int a5g_mg_residual_norms(a5g_tree* t, double* max_res, double* max_rhs) {
   API_BEGIN
   NEED(t);
@@ -663,6 +695,16 @@ int a5g_tree_copy_cc(a5g_tree* t, int iv_from, int iv_to) {
   check_iv(t->P.tree, iv_from);
   check_iv(t->P.tree, iv_to);
   a5g::tree_copy_cc(t->P, iv_from, iv_to);
+  API_END
+}
+
+int a5g_tree_maxdiff_cc(a5g_tree* t, int iv_a, int iv_b, double* maxdiff, double* maxabs_b) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv_a);
+  check_iv(t->P.tree, iv_b);
+  a5g::maxdiff_leaves(t->P, iv_a, iv_b, maxdiff, maxabs_b);
   API_END
 }
 

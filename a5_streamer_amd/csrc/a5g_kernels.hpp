@@ -35,6 +35,18 @@ __device__ __forceinline__ double* boxp(const Dev& a, int iv, int id) {
   return a.cc + (size_t)(iv - 1) * a.vs + (size_t)id * a.bs;
 }
 
+// The same box slot in the FP64 (cc) or FP32 (cc32) variable arrays
+template <typename T>
+__device__ __forceinline__ T* boxp_t(const Dev& a, int iv, int id);
+template <>
+__device__ __forceinline__ double* boxp_t<double>(const Dev& a, int iv, int id) {
+  return boxp(a, iv, id);
+}
+template <>
+__device__ __forceinline__ float* boxp_t<float>(const Dev& a, int iv, int id) {
+  return a.cc32 + (size_t)(iv - 1) * a.vs32 + (size_t)id * a.bs;
+}
+
 __device__ __forceinline__ double* facep(const Dev& a, int ifc, int id, int d, int dim) {
   return a.fc + (size_t)(ifc - 1) * a.fvs + ((size_t)id * dim + d) * a.fs;
 }
@@ -62,9 +74,9 @@ __device__ __forceinline__ void child_off(const Dev& a, int id, int nc, int nb, 
 // with a$D_gc_prolong0 / a$D_prolong0_to, m_aX_gc.f90:71-86,
 // m_aX_prolong.f90:37-78) or the physical rule (copy, -c, 2V - c).
 // `p` is the box's own data; only interior cells of p are read.
-template <int D>
-__device__ __forceinline__ double ghost_mg(const Dev& a, const double* p, int iv, int id,
-                                           int nc, int nb, int gi, int gj, int gk) {
+template <int D, typename T = double>
+__device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, int nc, int nb,
+                                      int gi, int gj, int gk) {
   const int nb_id = a.nbs[id * 6 + nb - 1];
   const int d = d_nb_dim(nb);
   int g[3] = {gi, gj, gk};
@@ -74,7 +86,7 @@ __device__ __forceinline__ double ghost_mg(const Dev& a, const double* p, int iv
   if (nb_id > 0) {
     int s[3] = {gi, gj, gk};
     s[d] = low ? nc : 1;
-    return boxp(a, iv, nb_id)[cidx<D>(nc, s[0], s[1], s[2])];
+    return boxp_t<T>(a, iv, nb_id)[cidx<D>(nc, s[0], s[1], s[2])];
   } else if (nb_id == 0) {
     const int pid = a.parent[id];
     const int pnc = a.ncell[pid];
@@ -83,10 +95,10 @@ __device__ __forceinline__ double ghost_mg(const Dev& a, const double* p, int iv
     int q[3];
 #pragma unroll
     for (int e = 0; e < 3; e++) q[e] = (e < D) ? off[e] + ((g[e] + 1) >> 1) : 0;
-    const double P = boxp(a, iv, pid)[cidx<D>(pnc, q[0], q[1], q[2])];
-    const double cv = p[cidx<D>(nc, c[0], c[1], c[2])];
+    const T P = boxp_t<T>(a, iv, pid)[cidx<D>(pnc, q[0], q[1], q[2])];
+    const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
     // shifted interior cells, in the order dim 0, 1, 2 of the reference
-    double t[3];
+    T t[3];
 #pragma unroll
     for (int e = 0; e < D; e++) {
       int s[3] = {c[0], c[1], c[2]};
@@ -94,14 +106,14 @@ __device__ __forceinline__ double ghost_mg(const Dev& a, const double* p, int iv
       else s[e] += -1 + 2 * (s[e] & 1);
       t[e] = p[cidx<D>(nc, s[0], s[1], s[2])];
     }
-    if (D == 2) return 0.5 * P + cv - 0.25 * (t[0] + t[1]);
-    return 0.5 * P + 1.25 * cv - 0.25 * (t[0] + t[1] + t[2]);
+    if (D == 2) return T(0.5) * P + cv - T(0.25) * (t[0] + t[1]);
+    return T(0.5) * P + T(1.25) * cv - T(0.25) * (t[0] + t[1] + t[2]);
   } else {
-    const double cv = p[cidx<D>(nc, c[0], c[1], c[2])];
+    const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
     const int kind = a.bc_kind[nb - 1];
     if (kind == 0) return cv;
     if (kind == 1) return -cv;
-    return 2 * a.bc_val[nb - 1] - cv;
+    return T(2) * (T)a.bc_val[nb - 1] - cv;
   }
 }
 

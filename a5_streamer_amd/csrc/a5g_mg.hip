@@ -58,14 +58,14 @@ static Geo geo(int n_boxes, int work) {
 // mg$D_box_gsrb_lpl (m_aX_mg.f90:769-811), mg2_box_gsrb_clpl (:1413-1439)
 // ---------------------------------------------------------------------------
 
-template <int D, bool CYL>
+template <int D, bool CYL, typename T>
 __global__ __launch_bounds__(256) void k_gsrb(Dev a, const int* __restrict__ list, int n,
                                               int nc, int cntr, int fused, int tpb,
                                               int i_phi, int i_rhs) {
   BOX_PROLOGUE
-  double* p = boxp(a, i_phi, id);
-  const double* r = boxp(a, i_rhs, id);
-  const double dx2 = a.dr[id] * a.dr[id];
+  T* p = boxp_t<T>(a, i_phi, id);
+  const T* r = boxp_t<T>(a, i_rhs, id);
+  const T dx2 = (T)(a.dr[id] * a.dr[id]);
   const int half = nc >> 1;
   const int ncol = (D == 2 ? nc * nc : nc * nc * nc) >> 1;
   const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
@@ -76,32 +76,32 @@ __global__ __launch_bounds__(256) void k_gsrb(Dev a, const int* __restrict__ lis
     const int i0 = 2 - ((cntr ^ (D == 3 ? k + j : j)) & 1);
     const int i = i0 + 2 * m;
     // neighbour value at (i+di, j+dj, k+dk); ghosts recomputed when fused
-    auto nbv = [&](int di, int dj, int dk, int nb) -> double {
+    auto nbv = [&](int di, int dj, int dk, int nb) -> T {
       const int ii = i + di, jj = j + dj, kk = k + dk;
       const bool ghost = (ii < 1 || ii > nc || jj < 1 || jj > nc ||
                           (D == 3 && (kk < 1 || kk > nc)));
-      if (ghost && fused) return ghost_mg<D>(a, p, i_phi, id, nc, nb, ii, jj, kk);
+      if (ghost && fused) return ghost_mg<D, T>(a, p, i_phi, id, nc, nb, ii, jj, kk);
       return p[cidx<D>(nc, ii, jj, kk)];
     };
     const int c0 = cidx<D>(nc, i, j, k);
     if (D == 2) {
       if (CYL) {
         const double den = i + ioff - 0.5;
-        const double rf1 = (double)(i + ioff - 1) / den;
-        const double rf2 = (double)(i + ioff) / den;
-        const double v = 0.25 * (rf1 * nbv(-1, 0, 0, 1) + rf2 * nbv(1, 0, 0, 2) +
-                                 nbv(0, 1, 0, 4) + nbv(0, -1, 0, 3) - dx2 * r[c0]);
+        const T rf1 = (T)((double)(i + ioff - 1) / den);
+        const T rf2 = (T)((double)(i + ioff) / den);
+        const T v = T(0.25) * (rf1 * nbv(-1, 0, 0, 1) + rf2 * nbv(1, 0, 0, 2) +
+                               nbv(0, 1, 0, 4) + nbv(0, -1, 0, 3) - dx2 * r[c0]);
         p[c0] = v;
       } else {
-        const double v = 0.25 * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
-                                 nbv(0, -1, 0, 3) - dx2 * r[c0]);
+        const T v = T(0.25) * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
+                               nbv(0, -1, 0, 3) - dx2 * r[c0]);
         p[c0] = v;
       }
     } else {
-      const double sixth = 1 / 6.0;
-      const double v = sixth * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
-                                nbv(0, -1, 0, 3) + nbv(0, 0, 1, 6) + nbv(0, 0, -1, 5) -
-                                dx2 * r[c0]);
+      const T sixth = T(1) / T(6);
+      const T v = sixth * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
+                           nbv(0, -1, 0, 3) + nbv(0, 0, 1, 6) + nbv(0, 0, -1, 5) -
+                           dx2 * r[c0]);
       p[c0] = v;
     }
   }
@@ -120,7 +120,7 @@ __device__ __forceinline__ int xcd_block() {
 // slots are first recomputed from the current state (the values the
 // reference's ghost pass after the previous half-sweep would store). Then
 // every lane updates cells of colour cntr&1 from LDS and rhs.
-template <int D, bool CYL, int NC>
+template <int D, bool CYL, int NC, typename T>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
   constexpr int N2 = NC + 2;
@@ -132,14 +132,14 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int NGH = 2 * D * PER_FACE;
   constexpr int HNC = NC / 2;
-  __shared__ double sh[BPB * HALF];
+  __shared__ T sh[BPB * HALF];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
   const int li = xcd_block() * BPB + slot;
   const bool act = li < n;
   const int id = act ? list[li] : 0;
   const int c = cntr & 1, oc = c ^ 1;
-  double* s = sh + slot * HALF;
-  double* p = act ? boxp(a, i_phi, id) : nullptr;
+  T* s = sh + slot * HALF;
+  T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   if (act) {
     for (int q = lt; q < HALF; q += TPB) s[q] = p[oc * HALF + q];
     if (fused) {
@@ -154,15 +154,15 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
         if (D == 3) g[tb] = t / NC + 1;
         if (((g[0] + g[1] + g[2]) & 1) != oc) continue;
         const int lex = D == 2 ? g[0] + N2 * g[1] : g[0] + N2 * (g[1] + N2 * g[2]);
-        s[lex >> 1] = ghost_mg<D>(a, p, i_phi, id, NC, nb, g[0], g[1], g[2]);
+        s[lex >> 1] = ghost_mg<D, T>(a, p, i_phi, id, NC, nb, g[0], g[1], g[2]);
       }
     }
   }
   __syncthreads();
   if (!act) return;
-  const double* r = boxp(a, i_rhs, id) + c * HALF;
-  double* pc = p + c * HALF;
-  const double dx2 = a.dr[id] * a.dr[id];
+  const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
+  T* pc = p + c * HALF;
+  const T dx2 = (T)(a.dr[id] * a.dr[id]);
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
   for (int q = lt; q < NCOL; q += TPB) {
     const int mm = q % HNC, row = q / HNC;
@@ -171,44 +171,45 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
     const int i = 2 * mm + 2 - ((c + j + k) & 1);
     const int lex = D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
     const int x = lex >> 1;
-    const double xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
-    const double yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
-    double v;
+    const T xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
+    const T yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
+    T v;
     if (D == 2) {
       if (CYL) {
         const double den = i + ioff - 0.5;
-        const double rf1 = (double)(i + ioff - 1) / den;
-        const double rf2 = (double)(i + ioff) / den;
-        v = 0.25 * (rf1 * xm + rf2 * xp + yp + ym - dx2 * r[x]);
+        const T rf1 = (T)((double)(i + ioff - 1) / den);
+        const T rf2 = (T)((double)(i + ioff) / den);
+        v = T(0.25) * (rf1 * xm + rf2 * xp + yp + ym - dx2 * r[x]);
       } else {
-        v = 0.25 * (xp + xm + yp + ym - dx2 * r[x]);
+        v = T(0.25) * (xp + xm + yp + ym - dx2 * r[x]);
       }
     } else {
-      const double zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
-      const double sixth = 1 / 6.0;
+      const T zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
+      const T sixth = T(1) / T(6);
       v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * r[x]);
     }
     pc[x] = v;
   }
 }
 
-template <int D, bool CYL>
-static bool launch_gsrb_lds(Pool& P, const int* list, int n, int nc, int cntr, int fused) {
+template <int D, bool CYL, typename T>
+static bool launch_gsrb_lds(Pool& P, const Dev& a, const int* list, int n, int nc, int cntr,
+                            int fused, int i_phi, int i_rhs) {
   auto go = [&](auto kern, int ncol) {
     const int tpb = ncol > 128 ? 256 : (ncol > 64 ? 128 : 64);
     const int bpb = 256 / tpb;
-    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, P.dev(), list, n,
-                       cntr, fused, P.mg.i_phi, P.mg.i_rhs);
+    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
+                       cntr, fused, i_phi, i_rhs);
   };
   const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
   switch (nc) {
-    case 2: go(k_gsrb_lds<D, CYL, 2>, ncol); return true;
-    case 4: go(k_gsrb_lds<D, CYL, 4>, ncol); return true;
-    case 8: go(k_gsrb_lds<D, CYL, 8>, ncol); return true;
-    case 16: go(k_gsrb_lds<D, CYL, 16>, ncol); return true;
+    case 2: go(k_gsrb_lds<D, CYL, 2, T>, ncol); return true;
+    case 4: go(k_gsrb_lds<D, CYL, 4, T>, ncol); return true;
+    case 8: go(k_gsrb_lds<D, CYL, 8, T>, ncol); return true;
+    case 16: go(k_gsrb_lds<D, CYL, 16, T>, ncol); return true;
     case 32:
       if (D == 2) {
-        go(k_gsrb_lds<2, CYL, 32>, ncol);
+        go(k_gsrb_lds<2, CYL, 32, T>, ncol);
         return true;
       }
       return false;
@@ -223,14 +224,14 @@ static bool launch_gsrb_lds(Pool& P, const int* list, int n, int nc, int cntr, i
 // 2 Dirichlet zero (:333-365).
 // ---------------------------------------------------------------------------
 
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list, int n,
                                             int nc, int tpb, int iv, int rb, int bc) {
   BOX_PROLOGUE
-  double* p = boxp(a, iv, id);
+  T* p = boxp_t<T>(a, iv, id);
   const int per_face = D == 2 ? nc : nc * nc;
   const int total = 2 * D * per_face;
-  const double sixth = 1 / 6.0, third = 1 / 3.0;
+  const T sixth = T(1) / T(6), third = T(1) / T(3);
   for (int q = lt; q < total; q += tpb) {
     const int nb = q / per_face + 1;
     const int t = q - (nb - 1) * per_face;
@@ -242,13 +243,13 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
     g[ta] = t % nc + 1;
     if (D == 3) g[tb] = t / nc + 1;
     const int nb_id = a.nbs[id * 6 + nb - 1];
-    double v;
+    T v;
     if (nb_id > 0 || (nb_id == 0 && rb == 0) || (nb_id < 0 && bc == 0)) {
-      v = ghost_mg<D>(a, p, iv, id, nc, nb, g[0], g[1], g[2]);
+      v = ghost_mg<D, T>(a, p, iv, id, nc, nb, g[0], g[1], g[2]);
     } else if (nb_id < 0) {
       int c[3] = {g[0], g[1], g[2]};
       c[d] = low ? 1 : nc;
-      const double cv = p[cidx<D>(nc, c[0], c[1], c[2])];
+      const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
       v = bc == 1 ? cv : -cv;
     } else {
       // coarse-neighbour interpolation
@@ -258,10 +259,10 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
       int off[3];
       child_off<D>(a, id, nc, nb, off);
       const int ixc = low ? nc : 1, ixf = low ? 1 : nc;
-      const double* qv = boxp(a, iv, pn);
+      const T* qv = boxp_t<T>(a, iv, pn);
       int f[3] = {g[0], g[1], g[2]};
       f[d] = ixf;
-      const double fv = p[cidx<D>(nc, f[0], f[1], f[2])];
+      const T fv = p[cidx<D>(nc, f[0], f[1], f[2])];
       if (D == 2) {
         const int x = g[ta];
         const int c1 = off[ta] + ((x + 1) >> 1), c2 = c1 + 1 - 2 * (x & 1);
@@ -269,10 +270,10 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
         s1[d] = s2[d] = ixc;
         s1[ta] = c1;
         s2[ta] = c2;
-        const double v1 = qv[cidx<D>(pnc, s1[0], s1[1], 0)];
-        const double v2 = qv[cidx<D>(pnc, s2[0], s2[1], 0)];
-        v = 0.5 * v1 + sixth * v2 + third * fv;
-        if (rb == 2 && v > 2 * v1) v = 2 * v1;
+        const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], 0)];
+        const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], 0)];
+        v = T(0.5) * v1 + sixth * v2 + third * fv;
+        if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
       } else {
         // the two tangential shifts enter in the order of the reference:
         // x-faces vary j then k, y-faces i then k, z-faces j then i
@@ -289,11 +290,11 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
           s2[ta] = a2; s2[tb] = b1;
           s3[ta] = a1; s3[tb] = b2;
         }
-        const double v1 = qv[cidx<D>(pnc, s1[0], s1[1], s1[2])];
-        const double v2 = qv[cidx<D>(pnc, s2[0], s2[1], s2[2])];
-        const double v3 = qv[cidx<D>(pnc, s3[0], s3[1], s3[2])];
+        const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], s1[2])];
+        const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], s2[2])];
+        const T v3 = qv[cidx<D>(pnc, s3[0], s3[1], s3[2])];
         v = third * v1 + sixth * v2 + sixth * v3 + third * fv;
-        if (rb == 2 && v > 2 * v1) v = 2 * v1;
+        if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
       }
     }
     p[cidx<D>(nc, g[0], g[1], g[2])] = v;
@@ -305,26 +306,26 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
 // mg2_box_clpl (:1442-1464)
 // ---------------------------------------------------------------------------
 
-template <int D, bool CYL>
-__device__ __forceinline__ double lpl_at(const double* p, int nc, int i, int j, int k,
-                                         double inv_dr_sq, int ioff) {
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T lpl_at(const T* p, int nc, int i, int j, int k, T inv_dr_sq,
+                                    int ioff) {
   if (D == 2) {
     if (CYL) {
       const double den = i + ioff - 0.5;
-      const double rf1 = (double)(i + ioff - 1) / den;
-      const double rf2 = (double)(i + ioff) / den;
+      const T rf1 = (T)((double)(i + ioff - 1) / den);
+      const T rf2 = (T)((double)(i + ioff) / den);
       return (rf1 * p[cidx<D>(nc, i - 1, j, 0)] + rf2 * p[cidx<D>(nc, i + 1, j, 0)] +
               p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-              4 * p[cidx<D>(nc, i, j, 0)]) * inv_dr_sq;
+              T(4) * p[cidx<D>(nc, i, j, 0)]) * inv_dr_sq;
     }
     return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, 0)] + p[cidx<D>(nc, i + 1, j, 0)] +
                         p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-                        4 * p[cidx<D>(nc, i, j, 0)]);
+                        T(4) * p[cidx<D>(nc, i, j, 0)]);
   }
   return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, k)] + p[cidx<D>(nc, i + 1, j, k)] +
                       p[cidx<D>(nc, i, j - 1, k)] + p[cidx<D>(nc, i, j + 1, k)] +
                       p[cidx<D>(nc, i, j, k - 1)] + p[cidx<D>(nc, i, j, k + 1)] -
-                      6 * p[cidx<D>(nc, i, j, k)]);
+                      T(6) * p[cidx<D>(nc, i, j, k)]);
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +336,7 @@ __device__ __forceinline__ double lpl_at(const double* p, int nc, int i, int j, 
 // (set_coarse_phi_rhs); 0 leaves tmp as it was (update_coarse restores it).
 // ---------------------------------------------------------------------------
 
-template <int D, bool CYL>
+template <int D, bool CYL, typename T>
 __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict__ list,
                                                     int n, int nc, int tpb, int i_phi,
                                                     int i_rhs, int i_tmp, int keep_res) {
@@ -345,12 +346,12 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
   const int pnc = a.ncell[pid];
   int off[3];
   child_off<D>(a, id, nc, 0, off);
-  const double* p = boxp(a, i_phi, id);
-  const double* r = boxp(a, i_rhs, id);
-  double* t = boxp(a, i_tmp, id);
-  double* pp = boxp(a, i_phi, pid);
-  double* pt = boxp(a, i_tmp, pid);
-  const double inv_dr_sq = 1 / (a.dr[id] * a.dr[id]);
+  const T* p = boxp_t<T>(a, i_phi, id);
+  const T* r = boxp_t<T>(a, i_rhs, id);
+  T* t = boxp_t<T>(a, i_tmp, id);
+  T* pp = boxp_t<T>(a, i_phi, pid);
+  T* pt = boxp_t<T>(a, i_tmp, pid);
+  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
   const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
   const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
   for (int q = lt; q < total; q += tpb) {
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
     const int cj = (q / hnc) % hnc + 1;
     const int ck = D == 3 ? q / (hnc * hnc) + 1 : 0;
     const int fi = 2 * ci - 1, fj = 2 * cj - 1, fk = D == 3 ? 2 * ck - 1 : 0;
-    double sr = 0, sp = 0;
+    T sr = 0, sp = 0;
 #pragma unroll
     for (int dk = 0; dk < (D == 3 ? 2 : 1); dk++)
 #pragma unroll
@@ -367,13 +368,13 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
         for (int di = 0; di < 2; di++) {
           const int ii = fi + di, jj = fj + dj, kk = fk + dk;
           const int c = cidx<D>(nc, ii, jj, kk);
-          const double res = r[c] - lpl_at<D, CYL>(p, nc, ii, jj, kk, inv_dr_sq, ioff);
+          const T res = r[c] - lpl_at<D, CYL, T>(p, nc, ii, jj, kk, inv_dr_sq, ioff);
           if (keep_res) t[c] = res;
           sr = sr + res;
           sp = sp + p[c];
         }
     const int pc = cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
-    const double w = D == 2 ? 0.25 : 0.125;
+    const T w = D == 2 ? T(0.25) : T(0.125);
     pt[pc] = w * sr;
     pp[pc] = w * sp;
   }
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
 
 // Plain restriction of one variable to the parent (mg$D_box_rstr_lpl), and
 // for init_phi_rhs (m_aX_mg.f90:526-549) the clearing of the child's phi.
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(256) void k_rstr_clear(Dev a, const int* __restrict__ list,
                                                     int n, int nc, int tpb, int iv,
                                                     int i_clear) {
@@ -391,15 +392,15 @@ __global__ __launch_bounds__(256) void k_rstr_clear(Dev a, const int* __restrict
   const int pnc = a.ncell[pid];
   int off[3];
   child_off<D>(a, id, nc, 0, off);
-  const double* c = boxp(a, iv, id);
-  double* pv = boxp(a, iv, pid);
+  const T* c = boxp_t<T>(a, iv, id);
+  T* pv = boxp_t<T>(a, iv, pid);
   const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
   for (int q = lt; q < total; q += tpb) {
     const int ci = q % hnc + 1;
     const int cj = (q / hnc) % hnc + 1;
     const int ck = D == 3 ? q / (hnc * hnc) + 1 : 0;
     const int fi = 2 * ci - 1, fj = 2 * cj - 1, fk = D == 3 ? 2 * ck - 1 : 0;
-    double s = 0;
+    T s = 0;
 #pragma unroll
     for (int dk = 0; dk < (D == 3 ? 2 : 1); dk++)
 #pragma unroll
@@ -407,11 +408,11 @@ __global__ __launch_bounds__(256) void k_rstr_clear(Dev a, const int* __restrict
 #pragma unroll
         for (int di = 0; di < 2; di++) s = s + c[cidx<D>(nc, fi + di, fj + dj, fk + dk)];
     pv[cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0)] =
-        (D == 2 ? 0.25 : 0.125) * s;
+        (D == 2 ? T(0.25) : T(0.125)) * s;
   }
   if (i_clear > 0) {
     // runs after all reads of this box (other lanes read `iv`, not i_clear)
-    double* z = boxp(a, i_clear, id);
+    T* z = boxp_t<T>(a, i_clear, id);
     const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
     for (int q = lt; q < bl; q += tpb) z[q] = 0;
   }
@@ -468,15 +469,15 @@ __global__ __launch_bounds__(256) void k_restrict_box(Dev a, const int* __restri
 // over the whole box (a$D_box_copy_cc); m_aX_mg.f90:478-485, 516-522.
 // ---------------------------------------------------------------------------
 
-template <int D, bool CYL>
+template <int D, bool CYL, typename T>
 __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict__ list,
                                                     int n, int nc, int tpb, int i_phi,
                                                     int i_rhs, int i_tmp, int copy_tmp) {
   BOX_PROLOGUE
-  const double* p = boxp(a, i_phi, id);
-  double* r = boxp(a, i_rhs, id);
-  double* t = boxp(a, i_tmp, id);
-  const double inv_dr_sq = 1 / (a.dr[id] * a.dr[id]);
+  const T* p = boxp_t<T>(a, i_phi, id);
+  T* r = boxp_t<T>(a, i_rhs, id);
+  T* t = boxp_t<T>(a, i_tmp, id);
+  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
   const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
   const int n2 = nc + 2;
   const int bl = D == 2 ? n2 * n2 : n2 * n2 * n2;
@@ -485,7 +486,7 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
     const int x = cidx<D>(nc, i, j, k);
     const bool inner = i >= 1 && i <= nc && j >= 1 && j <= nc &&
                        (D == 2 || (k >= 1 && k <= nc));
-    double rv = inner ? lpl_at<D, CYL>(p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
+    T rv = inner ? lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
     rv = rv + t[x];
     r[x] = rv;
     if (copy_tmp) t[x] = p[x];
@@ -494,18 +495,19 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
 
 // correct_children, first half (m_aX_mg.f90:385-392): tmp = phi - tmp on the
 // whole parent box
+template <typename T>
 __global__ __launch_bounds__(256) void k_sub_whole(Dev a, const int* __restrict__ list,
                                                    int n, int bl, int tpb, int i_phi,
                                                    int i_tmp) {
   BOX_PROLOGUE
-  const double* p = boxp(a, i_phi, id);
-  double* t = boxp(a, i_tmp, id);
+  const T* p = boxp_t<T>(a, i_phi, id);
+  T* t = boxp_t<T>(a, i_tmp, id);
   for (int q = lt; q < bl; q += tpb) t[q] = p[q] - t[q];
 }
 
 // correct_children, second half: mg$D_box_corr_lpl (m_aX_mg.f90:714-766) on
 // every child box in the list
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ list, int n,
                                               int nc, int tpb, int i_phi, int i_tmp) {
   BOX_PROLOGUE
@@ -513,8 +515,8 @@ __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ lis
   const int pnc = a.ncell[pid];
   int off[3];
   child_off<D>(a, id, nc, 0, off);
-  double* f = boxp(a, i_phi, id);
-  const double* c = boxp(a, i_tmp, pid);
+  T* f = boxp_t<T>(a, i_phi, id);
+  const T* c = boxp_t<T>(a, i_tmp, pid);
   const int total = D == 2 ? nc * nc : nc * nc * nc;
   for (int q = lt; q < total; q += tpb) {
     const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
@@ -522,42 +524,43 @@ __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ lis
     const int j1 = off[1] + ((j + 1) >> 1), j2 = j1 + 1 - 2 * (j & 1);
     const int x = cidx<D>(nc, i, j, k);
     if (D == 2) {
-      f[x] = f[x] + 0.5 * c[cidx<D>(pnc, i1, j1, 0)] +
-             0.25 * (c[cidx<D>(pnc, i2, j1, 0)] + c[cidx<D>(pnc, i1, j2, 0)]);
+      f[x] = f[x] + T(0.5) * c[cidx<D>(pnc, i1, j1, 0)] +
+             T(0.25) * (c[cidx<D>(pnc, i2, j1, 0)] + c[cidx<D>(pnc, i1, j2, 0)]);
     } else {
       const int k1 = off[2] + ((k + 1) >> 1), k2 = k1 + 1 - 2 * (k & 1);
-      f[x] = f[x] + 0.25 * (c[cidx<D>(pnc, i1, j1, k1)] + c[cidx<D>(pnc, i2, j1, k1)] +
-                            c[cidx<D>(pnc, i1, j2, k1)] + c[cidx<D>(pnc, i1, j1, k2)]);
+      f[x] = f[x] + T(0.25) * (c[cidx<D>(pnc, i1, j1, k1)] + c[cidx<D>(pnc, i2, j1, k1)] +
+                               c[cidx<D>(pnc, i1, j2, k1)] + c[cidx<D>(pnc, i1, j1, k2)]);
     }
   }
 }
 
 // whole-box copy (a$D_box_copy_cc, m_aX_utils.f90:421-429)
+template <typename T>
 __global__ __launch_bounds__(256) void k_copy_whole(Dev a, const int* __restrict__ list,
                                                     int n, int bl, int tpb, int from,
                                                     int to) {
   BOX_PROLOGUE
-  const double* s = boxp(a, from, id);
-  double* d = boxp(a, to, id);
+  const T* s = boxp_t<T>(a, from, id);
+  T* d = boxp_t<T>(a, to, id);
   for (int q = lt; q < bl; q += tpb) d[q] = s[q];
 }
 
 // residual_box (m_aX_mg.f90:551-565): tmp = rhs - A phi on the interior
-template <int D, bool CYL>
+template <int D, bool CYL, typename T>
 __global__ __launch_bounds__(256) void k_residual(Dev a, const int* __restrict__ list,
                                                   int n, int nc, int tpb, int i_phi,
                                                   int i_rhs, int i_tmp) {
   BOX_PROLOGUE
-  const double* p = boxp(a, i_phi, id);
-  const double* r = boxp(a, i_rhs, id);
-  double* t = boxp(a, i_tmp, id);
-  const double inv_dr_sq = 1 / (a.dr[id] * a.dr[id]);
+  const T* p = boxp_t<T>(a, i_phi, id);
+  const T* r = boxp_t<T>(a, i_rhs, id);
+  T* t = boxp_t<T>(a, i_tmp, id);
+  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
   const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
   const int total = D == 2 ? nc * nc : nc * nc * nc;
   for (int q = lt; q < total; q += tpb) {
     const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
     const int x = cidx<D>(nc, i, j, k);
-    t[x] = r[x] - lpl_at<D, CYL>(p, nc, i, j, k, inv_dr_sq, ioff);
+    t[x] = r[x] - lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff);
   }
 }
 
@@ -632,34 +635,43 @@ static void check(hipError_t e, const char* what) {
 
 static void post_launch(const char* what) { check(hipGetLastError(), what); }
 
-template <int D>
-static void gsrb_boxes(Pool& P, int lvl, int n_cycle) {
-  // gsrb_boxes, m_aX_mg.f90:403-426
-  const int n = P.n_ids(lvl);
+// One multigrid problem: its variables, its precision and whether the
+// physical boundary values are replaced by zero (the correction equation of
+// the mixed-precision solve, whose boundary data is homogeneous).
+struct Ctx {
+  int i_phi, i_tmp, i_rhs;
+  bool homogeneous;
+  bool cyl;  // cylindrical operators (2D, automatic operators installed)
+};
+
+static Ctx ctx64(const Pool& P) {
+  return {P.mg.i_phi, P.mg.i_tmp, P.mg.i_rhs, false,
+          P.tree.dim == 2 && P.tree.coord == 1 && P.mg.auto_ops != 0};
+}
+
+static Dev dev_for(Pool& P, const Ctx& c) {
+  Dev a = P.dev();
+  if (c.homogeneous)
+    for (int i = 0; i < 6; i++) a.bc_val[i] = 0.0;
+  return a;
+}
+
+static int cells_of(int D, int nc, bool ghosts) {
+  const int m = ghosts ? nc + 2 : nc;
+  return D == 2 ? m * m : m * m * m;
+}
+
+template <int D, typename T>
+static void gc_list_t(Pool& P, const Dev& a, int lvl, const int* list, int n, int iv, int rb,
+                      int bc) {
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
-  const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
-  Geo g = geo(n, ncol);
-  Dev a = P.dev();
-  const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
-  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
-  for (int it = 1; it <= 2 * n_cycle; it++) {
-    const int fused = it > 1;
-    // algorithmic bytes of a half-sweep: 1.5 words per cell
-    const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * 8 * cells);
-    const bool done = cyl ? launch_gsrb_lds<2, true>(P, P.ids(lvl), n, nc, it, fused)
-                          : launch_gsrb_lds<D, false>(P, P.ids(lvl), n, nc, it, fused);
-    if (done) {
-    } else if (cyl)
-      hipLaunchKernelGGL((k_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                         P.ids(lvl), n, nc, it, fused, g.tpb, P.mg.i_phi, P.mg.i_rhs);
-    else
-      hipLaunchKernelGGL((k_gsrb<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                         P.ids(lvl), n, nc, it, fused, g.tpb, P.mg.i_phi, P.mg.i_rhs);
-    post_launch("gsrb");
-    P.prof_end(pr);
-  }
-  gc_level(P, lvl, P.mg.i_phi, 0, 0);
+  Geo g = geo(n, 2 * D * (D == 2 ? nc : nc * nc));
+  const int pr = P.prof_begin(PK_GC, lvl, 0.0);
+  hipLaunchKernelGGL((k_gc<D, T>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
+                     iv, rb, bc);
+  post_launch("gc");
+  P.prof_end(pr);
 }
 
 void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
@@ -667,188 +679,210 @@ void gc_level(Pool& P, int lvl, int iv, int rb, int bc) {
 }
 
 void gc_list(Pool& P, int lvl, const int* list, int n, int iv, int rb, int bc) {
-  if (n == 0) return;
-  const int nc = P.nc_lvl(lvl);
-  const int D = P.tree.dim;
-  const int work = 2 * D * (D == 2 ? nc : nc * nc);
-  Geo g = geo(n, work);
   Dev a = P.dev();
-  const int pr = P.prof_begin(PK_GC, lvl, 0.0);
-  if (D == 2)
-    hipLaunchKernelGGL((k_gc<2>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
-                       iv, rb, bc);
-  else
-    hipLaunchKernelGGL((k_gc<3>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
-                       iv, rb, bc);
-  post_launch("gc");
-  P.prof_end(pr);
+  if (P.tree.dim == 2) gc_list_t<2, double>(P, a, lvl, list, n, iv, rb, bc);
+  else gc_list_t<3, double>(P, a, lvl, list, n, iv, rb, bc);
 }
 
-template <int D>
-static void resid_rstr(Pool& P, int lvl, int keep_res) {
+template <int D, typename T>
+static void fill_gc_phi(Pool& P, const Ctx& c, int lvl) {
+  // fill_gc_phi, m_aX_mg.f90:358-371
+  gc_list_t<D, T>(P, dev_for(P, c), lvl, P.ids(lvl), P.n_ids(lvl), c.i_phi, 0, 0);
+}
+
+template <int D, typename T>
+static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle) {
+  // gsrb_boxes, m_aX_mg.f90:403-426
+  const int n = P.n_ids(lvl);
+  if (n == 0) return;
+  const int nc = P.nc_lvl(lvl);
+  const int ncol = cells_of(D, nc, false) / 2;
+  Geo g = geo(n, ncol);
+  Dev a = dev_for(P, c);
+  const double cells = (double)n * cells_of(D, nc, false);
+  for (int it = 1; it <= 2 * n_cycle; it++) {
+    const int fused = it > 1;
+    // algorithmic bytes of a half-sweep: 1.5 words per cell
+    const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * sizeof(T) * cells);
+    const bool done =
+        c.cyl ? launch_gsrb_lds<2, true, T>(P, a, P.ids(lvl), n, nc, it, fused, c.i_phi, c.i_rhs)
+              : launch_gsrb_lds<D, false, T>(P, a, P.ids(lvl), n, nc, it, fused, c.i_phi,
+                                             c.i_rhs);
+    if (done) {
+    } else if (c.cyl)
+      hipLaunchKernelGGL((k_gsrb<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, it, fused, g.tpb, c.i_phi, c.i_rhs);
+    else
+      hipLaunchKernelGGL((k_gsrb<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, it, fused, g.tpb, c.i_phi, c.i_rhs);
+    post_launch("gsrb");
+    P.prof_end(pr);
+  }
+  fill_gc_phi<D, T>(P, c, lvl);
+}
+
+template <int D, typename T>
+static void resid_rstr(Pool& P, const Ctx& c, int lvl, int keep_res) {
   const int n = P.n_ids(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
   const int hnc = nc / 2;
   Geo g = geo(n, D == 2 ? hnc * hnc : hnc * hnc * hnc);
-  Dev a = P.dev();
-  const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
-  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
-  const int pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * 8 * cells);
-  if (cyl)
-    hipLaunchKernelGGL((k_resid_rstr<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.ids(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp, keep_res);
+  Dev a = dev_for(P, c);
+  const double cells = (double)n * cells_of(D, nc, false);
+  const int pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * sizeof(T) * cells);
+  if (c.cyl)
+    hipLaunchKernelGGL((k_resid_rstr<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                       P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
   else
-    hipLaunchKernelGGL((k_resid_rstr<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.ids(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp, keep_res);
+    hipLaunchKernelGGL((k_resid_rstr<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                       P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
   post_launch("resid_rstr");
   P.prof_end(pr);
 }
 
-template <int D>
-static void coarse_rhs(Pool& P, int lvl, int copy_tmp) {
+template <int D, typename T>
+static void coarse_rhs(Pool& P, const Ctx& c, int lvl, int copy_tmp) {
   const int n = P.n_parents(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
-  const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
-  Geo g = geo(n, bl);
-  Dev a = P.dev();
-  const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
-  const double cells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
-  const int pr = P.prof_begin(PK_COARSE_RHS, lvl, 4.0 * 8 * cells);
-  if (cyl)
-    hipLaunchKernelGGL((k_coarse_rhs<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.parents(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp,
-                       copy_tmp);
+  Geo g = geo(n, cells_of(D, nc, true));
+  Dev a = dev_for(P, c);
+  const double cells = (double)n * cells_of(D, nc, false);
+  const int pr = P.prof_begin(PK_COARSE_RHS, lvl, 4.0 * sizeof(T) * cells);
+  if (c.cyl)
+    hipLaunchKernelGGL((k_coarse_rhs<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                       P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
   else
-    hipLaunchKernelGGL((k_coarse_rhs<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.parents(lvl), n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp,
-                       copy_tmp);
+    hipLaunchKernelGGL((k_coarse_rhs<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                       P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
   post_launch("coarse_rhs");
   P.prof_end(pr);
 }
 
-template <int D>
-static void update_coarse(Pool& P, int lvl) {
+template <int D, typename T>
+static void update_coarse(Pool& P, const Ctx& c, int lvl) {
   // update_coarse, m_aX_mg.f90:430-486
-  resid_rstr<D>(P, lvl, 0);
-  gc_level(P, lvl - 1, P.mg.i_phi, 0, 0);
-  coarse_rhs<D>(P, lvl - 1, 1);
+  resid_rstr<D, T>(P, c, lvl, 0);
+  fill_gc_phi<D, T>(P, c, lvl - 1);
+  coarse_rhs<D, T>(P, c, lvl - 1, 1);
 }
 
-template <int D>
-static void set_coarse_phi_rhs(Pool& P, int lvl) {
+template <int D, typename T>
+static void set_coarse_phi_rhs(Pool& P, const Ctx& c, int lvl) {
   // set_coarse_phi_rhs, m_aX_mg.f90:490-523
-  if (lvl == P.tree.max_lvl) gc_level(P, lvl, P.mg.i_phi, 0, 0);
-  resid_rstr<D>(P, lvl, 1);
-  gc_level(P, lvl - 1, P.mg.i_phi, 0, 0);
-  coarse_rhs<D>(P, lvl - 1, 0);
+  if (lvl == P.tree.max_lvl) fill_gc_phi<D, T>(P, c, lvl);
+  resid_rstr<D, T>(P, c, lvl, 1);
+  fill_gc_phi<D, T>(P, c, lvl - 1);
+  coarse_rhs<D, T>(P, c, lvl - 1, 0);
 }
 
-template <int D>
-static void correct_children(Pool& P, int lvl) {
+template <int D, typename T>
+static void correct_children(Pool& P, const Ctx& c, int lvl) {
   // correct_children over the parents of lvl, m_aX_mg.f90:374-401
   const int n = P.n_parents(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
-  const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
-  Dev a = P.dev();
+  const int bl = cells_of(D, nc, true);
+  Dev a = dev_for(P, c);
   Geo g = geo(n, bl);
-  const double pcells = (double)n * (D == 2 ? nc * nc : nc * nc * nc);
-  int pr = P.prof_begin(PK_CORR, lvl, 3.0 * 8 * pcells);
-  hipLaunchKernelGGL(k_sub_whole, dim3(g.grid), dim3(256), 0, P.stream, a, P.parents(lvl), n,
-                     bl, g.tpb, P.mg.i_phi, P.mg.i_tmp);
+  int pr = P.prof_begin(PK_CORR, lvl, 3.0 * sizeof(T) * n * cells_of(D, nc, false));
+  hipLaunchKernelGGL((k_sub_whole<T>), dim3(g.grid), dim3(256), 0, P.stream, a, P.parents(lvl),
+                     n, bl, g.tpb, c.i_phi, c.i_tmp);
   post_launch("corr_prep");
   P.prof_end(pr);
   // every child of these parents is a box of level lvl+1
   const int nch = P.n_ids(lvl + 1);
   const int cnc = P.nc_lvl(lvl + 1);
-  Geo g2 = geo(nch, D == 2 ? cnc * cnc : cnc * cnc * cnc);
-  const double ccells = (double)nch * (D == 2 ? cnc * cnc : cnc * cnc * cnc);
-  pr = P.prof_begin(PK_CORR, lvl + 1, (2.0 + 1.0 / (1 << D)) * 8 * ccells);
-  hipLaunchKernelGGL((k_corr<D>), dim3(g2.grid), dim3(256), 0, P.stream, a, P.ids(lvl + 1), nch,
-                     cnc, g2.tpb, P.mg.i_phi, P.mg.i_tmp);
+  Geo g2 = geo(nch, cells_of(D, cnc, false));
+  pr = P.prof_begin(PK_CORR, lvl + 1,
+                    (2.0 + 1.0 / (1 << D)) * sizeof(T) * nch * cells_of(D, cnc, false));
+  hipLaunchKernelGGL((k_corr<D, T>), dim3(g2.grid), dim3(256), 0, P.stream, a, P.ids(lvl + 1),
+                     nch, cnc, g2.tpb, c.i_phi, c.i_tmp);
   post_launch("corr");
   P.prof_end(pr);
 }
 
-static void copy_level(Pool& P, int lvl, int from, int to) {
+template <typename T>
+static void copy_level_t(Pool& P, int lvl, int from, int to) {
   const int n = P.n_ids(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
-  const int bl = P.tree.dim == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
+  const int D = P.tree.dim;
+  const int bl = cells_of(D, nc, true);
   Geo g = geo(n, bl);
-  const int pr = P.prof_begin(PK_COPY, lvl,
-                              2.0 * 8 * n * (double)(P.tree.dim == 2 ? nc * nc : nc * nc * nc));
-  hipLaunchKernelGGL(k_copy_whole, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
-                     n, bl, g.tpb, from, to);
+  const int pr = P.prof_begin(PK_COPY, lvl, 2.0 * sizeof(T) * n * cells_of(D, nc, false));
+  hipLaunchKernelGGL((k_copy_whole<T>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                     P.ids(lvl), n, bl, g.tpb, from, to);
   post_launch("copy");
   P.prof_end(pr);
 }
 
-template <int D>
-static void residual_level(Pool& P, int lvl, bool leaves_only) {
+static void copy_level(Pool& P, int lvl, int from, int to) { copy_level_t<double>(P, lvl, from, to); }
+
+template <int D, typename T>
+static void residual_level(Pool& P, const Ctx& c, int lvl, bool leaves_only) {
   const int n = leaves_only ? P.n_leaves(lvl) : P.n_ids(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
-  Geo g = geo(n, D == 2 ? nc * nc : nc * nc * nc);
+  Geo g = geo(n, cells_of(D, nc, false));
   const int* list = leaves_only ? P.leaves(lvl) : P.ids(lvl);
-  const bool cyl = D == 2 && P.tree.coord == 1 && P.mg.auto_ops;
-  if (cyl)
-    hipLaunchKernelGGL((k_residual<2, true>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
-                       list, n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp);
+  Dev a = dev_for(P, c);
+  if (c.cyl)
+    hipLaunchKernelGGL((k_residual<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n,
+                       nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp);
   else
-    hipLaunchKernelGGL((k_residual<D, false>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
-                       list, n, nc, g.tpb, P.mg.i_phi, P.mg.i_rhs, P.mg.i_tmp);
+    hipLaunchKernelGGL((k_residual<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a, list,
+                       n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp);
   post_launch("residual");
 }
 
-template <int D>
-static void init_phi_rhs(Pool& P) {
+template <int D, typename T>
+static void init_phi_rhs(Pool& P, const Ctx& c) {
   // init_phi_rhs, m_aX_mg.f90:526-549
   for (int lvl = P.tree.max_lvl; lvl >= P.tree.min_lvl + 1; lvl--) {
     const int n = P.n_ids(lvl);
     if (n == 0) continue;
     const int nc = P.nc_lvl(lvl);
-    Geo g = geo(n, D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2));
-    hipLaunchKernelGGL((k_rstr_clear<D>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
-                       P.ids(lvl), n, nc, g.tpb, P.mg.i_rhs, P.mg.i_phi);
+    Geo g = geo(n, cells_of(D, nc, true));
+    hipLaunchKernelGGL((k_rstr_clear<D, T>), dim3(g.grid), dim3(256), 0, P.stream,
+                       dev_for(P, c), P.ids(lvl), n, nc, g.tpb, c.i_rhs, c.i_phi);
     post_launch("init_phi_rhs");
   }
 }
 
-template <int D>
-static void fas_vcycle(Pool& P, int max_lvl, bool set_residual) {
+template <int D, typename T>
+static void fas_vcycle(Pool& P, const Ctx& c, int max_lvl, bool set_residual) {
   // mg$D_fas_vcycle, m_aX_mg.f90:208-255
   const int min_lvl = P.tree.min_lvl;
   for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
-    gsrb_boxes<D>(P, lvl, P.mg.n_down);
-    update_coarse<D>(P, lvl);
+    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_down);
+    update_coarse<D, T>(P, c, lvl);
   }
-  gsrb_boxes<D>(P, min_lvl, P.mg.n_base);
+  gsrb_boxes<D, T>(P, c, min_lvl, P.mg.n_base);
   for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
-    correct_children<D>(P, lvl - 1);
-    gc_level(P, lvl, P.mg.i_phi, 0, 0);
-    gsrb_boxes<D>(P, lvl, P.mg.n_up);
+    correct_children<D, T>(P, c, lvl - 1);
+    fill_gc_phi<D, T>(P, c, lvl);
+    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_up);
   }
   if (set_residual)
-    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) residual_level<D>(P, lvl, false);
+    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) residual_level<D, T>(P, c, lvl, false);
 }
 
-template <int D>
-static void fas_fmg(Pool& P, bool set_residual, bool first_call) {
+template <int D, typename T>
+static void fas_fmg(Pool& P, const Ctx& c, bool set_residual, bool first_call) {
   // mg$D_fas_fmg, m_aX_mg.f90:164-203
   const int min_lvl = P.tree.min_lvl;
-  if (first_call) init_phi_rhs<D>(P);
+  if (first_call) init_phi_rhs<D, T>(P, c);
   else
-    for (int lvl = P.tree.max_lvl; lvl >= min_lvl + 1; lvl--) set_coarse_phi_rhs<D>(P, lvl);
+    for (int lvl = P.tree.max_lvl; lvl >= min_lvl + 1; lvl--) set_coarse_phi_rhs<D, T>(P, c, lvl);
   for (int lvl = min_lvl; lvl <= P.tree.max_lvl; lvl++) {
-    copy_level(P, lvl, P.mg.i_phi, P.mg.i_tmp);
+    copy_level_t<T>(P, lvl, c.i_phi, c.i_tmp);
     if (lvl > min_lvl) {
-      correct_children<D>(P, lvl - 1);
-      gc_level(P, lvl, P.mg.i_phi, 0, 0);
+      correct_children<D, T>(P, c, lvl - 1);
+      fill_gc_phi<D, T>(P, c, lvl);
     }
-    fas_vcycle<D>(P, lvl, set_residual && lvl == P.tree.max_lvl);
+    fas_vcycle<D, T>(P, c, lvl, set_residual && lvl == P.tree.max_lvl);
   }
 }
 
@@ -869,16 +903,18 @@ static void mg_check(Pool& P) {
 
 void mg_fmg(Pool& P, bool set_residual, bool first_call) {
   mg_check(P);
-  if (P.tree.dim == 2) fas_fmg<2>(P, set_residual, first_call);
-  else fas_fmg<3>(P, set_residual, first_call);
+  const Ctx c = ctx64(P);
+  if (P.tree.dim == 2) fas_fmg<2, double>(P, c, set_residual, first_call);
+  else fas_fmg<3, double>(P, c, set_residual, first_call);
 }
 
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual) {
   mg_check(P);
   if (max_lvl < P.tree.min_lvl || max_lvl > P.tree.max_lvl)
     throw std::invalid_argument("fas_vcycle: max_lvl outside the tree");
-  if (P.tree.dim == 2) fas_vcycle<2>(P, max_lvl, set_residual);
-  else fas_vcycle<3>(P, max_lvl, set_residual);
+  const Ctx c = ctx64(P);
+  if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, max_lvl, set_residual);
+  else fas_vcycle<3, double>(P, c, max_lvl, set_residual);
 }
 
 void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
@@ -918,15 +954,244 @@ void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
   *hi = unkey(out[1]);
 }
 
+
+// max over leaf interiors of |a - b| and of |b|
+template <int D>
+__global__ __launch_bounds__(256) void k_maxdiff(Dev a, const int* __restrict__ list, int n,
+                                                 int nc, int tpb, int ia, int ib,
+                                                 unsigned long long* out) {
+  const int bpb_ = blockDim.x / tpb;
+  const int slot_ = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot_ * tpb;
+  const int li_ = blockIdx.x * bpb_ + slot_;
+  double md = 0, mb = 0;
+  if (li_ < n) {
+    const int id = list[li_];
+    const double* x = boxp(a, ia, id);
+    const double* y = boxp(a, ib, id);
+    const int total = D == 2 ? nc * nc : nc * nc * nc;
+    for (int q = lt; q < total; q += tpb) {
+      const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+      const int c = cidx<D>(nc, i, j, k);
+      md = fmax(md, fabs(x[c] - y[c]));
+      mb = fmax(mb, fabs(y[c]));
+    }
+  }
+  md = wave_max(md);
+  mb = wave_max(mb);
+  __shared__ double w1[4], w2[4];
+  if ((threadIdx.x & 63) == 0) {
+    w1[threadIdx.x >> 6] = md;
+    w2[threadIdx.x >> 6] = mb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      md = fmax(md, w1[w]);
+      mb = fmax(mb, w2[w]);
+    }
+    atomicMax(&out[0], ord_key(md));
+    atomicMax(&out[1], ord_key(mb));
+  }
+}
+
+void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b) {
+  P.sync_topology();
+  unsigned long long init[2] = {0ull, 0ull};
+  check(hipMemcpyAsync(P.d_red, init, sizeof(init), hipMemcpyHostToDevice, P.stream), "init");
+  const int D = P.tree.dim;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (n == 0) continue;
+    const int nc = P.nc_lvl(lvl);
+    Geo g = geo(n, cells_of(D, nc, false));
+    if (D == 2)
+      hipLaunchKernelGGL((k_maxdiff<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                         P.leaves(lvl), n, nc, g.tpb, ia, ib, (unsigned long long*)P.d_red);
+    else
+      hipLaunchKernelGGL((k_maxdiff<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                         P.leaves(lvl), n, nc, g.tpb, ia, ib, (unsigned long long*)P.d_red);
+    post_launch("maxdiff");
+  }
+  unsigned long long out[2];
+  check(hipMemcpyAsync(out, P.d_red, sizeof(out), hipMemcpyDeviceToHost, P.stream), "read");
+  check(hipStreamSynchronize(P.stream), "sync");
+  double v[2];
+  for (int m = 0; m < 2; m++) {
+    unsigned long long u = (out[m] >> 63) ? (out[m] & 0x7fffffffffffffffull) : ~out[m];
+    std::memcpy(&v[m], &u, 8);
+  }
+  *maxdiff = v[0];
+  *maxabs_b = v[1];
+}
+
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs) {
   mg_check(P);
+  const Ctx c = ctx64(P);
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
-    if (P.tree.dim == 2) residual_level<2>(P, lvl, true);
-    else residual_level<3>(P, lvl, true);
+    if (P.tree.dim == 2) residual_level<2, double>(P, c, lvl, true);
+    else residual_level<3, double>(P, c, lvl, true);
   }
   double lo;
   minmax_boxes(P, P.mg.i_tmp, true, false, true, &lo, max_res);
   minmax_boxes(P, P.mg.i_rhs, true, false, true, &lo, max_rhs);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 FAS multigrid on the correction equation inside an
+// FP64 defect-correction loop. The FP64 residual, the coarsest-level data of
+// the outer problem and the update of phi stay FP64.
+// ---------------------------------------------------------------------------
+
+// FP64 residual r = rhs - A phi on leaf interiors, stored as FP32 right-hand
+// side of the correction problem; its max |r| is reduced on the way
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_resid_to32(Dev a, const int* __restrict__ list, int n,
+                                                    int nc, int tpb, int i_phi, int i_rhs,
+                                                    int j_rhs, unsigned long long* mx) {
+  const int bpb_ = blockDim.x / tpb;
+  const int slot_ = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot_ * tpb;
+  const int li_ = blockIdx.x * bpb_ + slot_;
+  double m = 0;
+  if (li_ < n) {
+    const int id = list[li_];
+    const double* p = boxp(a, i_phi, id);
+    const double* r = boxp(a, i_rhs, id);
+    float* o = boxp_t<float>(a, j_rhs, id);
+    const double inv_dr_sq = 1 / (a.dr[id] * a.dr[id]);
+    const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+    const int total = D == 2 ? nc * nc : nc * nc * nc;
+    for (int q = lt; q < total; q += tpb) {
+      const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+      const int x = cidx<D>(nc, i, j, k);
+      const double res = r[x] - lpl_at<D, CYL, double>(p, nc, i, j, k, inv_dr_sq, ioff);
+      o[x] = (float)res;
+      m = fmax(m, fabs(res));
+    }
+  }
+  m = wave_max(m);
+  __shared__ double wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wm[w]);
+    atomicMax(mx, ord_key(m));
+  }
+}
+
+// phi += e on box interiors
+template <int D>
+__global__ __launch_bounds__(256) void k_add32(Dev a, const int* __restrict__ list, int n,
+                                               int nc, int tpb, int i_phi, int j_e) {
+  BOX_PROLOGUE
+  double* p = boxp(a, i_phi, id);
+  const float* e = boxp_t<float>(a, j_e, id);
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int x = cidx<D>(nc, i, j, k);
+    p[x] = p[x] + (double)e[x];
+  }
+}
+
+// whole FP32 boxes set to zero
+__global__ __launch_bounds__(256) void k_zero32(Dev a, const int* __restrict__ list, int n,
+                                                int bl, int tpb, int j) {
+  BOX_PROLOGUE
+  float* p = boxp_t<float>(a, j, id);
+  for (int q = lt; q < bl; q += tpb) p[q] = 0.0f;
+}
+
+static void ensure32(Pool& P, int n32) {
+  const int need = std::max(P.dev_cap, 1);
+  if (P.cc32 && P.n32 >= n32 && P.cap32 == need) return;
+  if (P.cc32) check(hipFree(P.cc32), "hipFree");
+  P.cc32 = nullptr;
+  check(hipMalloc((void**)&P.cc32, (size_t)n32 * (need + 1) * P.bs() * sizeof(float)),
+        "hipMalloc cc32");
+  P.n32 = n32;
+  P.cap32 = need;
+}
+
+template <int D>
+static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* outer_done,
+                        double* rel_res) {
+  const Ctx c64 = ctx64(P);
+  // FP32 variables: 1 = correction e, 2 = tmp, 3 = rhs of the correction
+  const Ctx c32{1, 2, 3, true, c64.cyl};
+  ensure32(P, 3);
+  const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
+  double rhs_lo, rhs_max;
+  minmax_boxes(P, c64.i_rhs, true, false, true, &rhs_lo, &rhs_max);
+  Dev a = P.dev();
+  int it = 0;
+  double rel = 0;
+  for (;; it++) {
+    // r = rhs - A phi (FP64) -> FP32 rhs of the correction, and max |r|
+    unsigned long long init = 0;
+    check(hipMemcpyAsync(P.d_red, &init, sizeof(init), hipMemcpyHostToDevice, P.stream), "red");
+    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
+      const int n = P.n_leaves(lvl);
+      if (n == 0) continue;
+      const int nc = P.nc_lvl(lvl);
+      Geo g = geo(n, cells_of(D, nc, false));
+      const int pr = P.prof_begin(PK_OTHER, lvl, (2.0 * 8 + 4) * n * cells_of(D, nc, false));
+      if (c64.cyl)
+        hipLaunchKernelGGL((k_resid_to32<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                           P.leaves(lvl), n, nc, g.tpb, c64.i_phi, c64.i_rhs, c32.i_rhs,
+                           (unsigned long long*)P.d_red);
+      else
+        hipLaunchKernelGGL((k_resid_to32<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                           P.leaves(lvl), n, nc, g.tpb, c64.i_phi, c64.i_rhs, c32.i_rhs,
+                           (unsigned long long*)P.d_red);
+      post_launch("resid_to32");
+      P.prof_end(pr);
+    }
+    unsigned long long key;
+    check(hipMemcpyAsync(&key, P.d_red, sizeof(key), hipMemcpyDeviceToHost, P.stream), "red");
+    check(hipStreamSynchronize(P.stream), "sync");
+    unsigned long long u = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+    double rmax;
+    std::memcpy(&rmax, &u, 8);
+    rel = rhs_max > 0 ? rmax / rhs_max : rmax;
+    if (rel <= tol || it >= max_outer) break;
+    // e = 0 everywhere, then FMG (+ V-cycles) on A e = r with zero boundary data
+    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
+      const int n = P.n_ids(lvl);
+      if (n == 0) continue;
+      const int bl = cells_of(D, P.nc_lvl(lvl), true);
+      Geo g = geo(n, bl);
+      hipLaunchKernelGGL(k_zero32, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, bl,
+                         g.tpb, c32.i_phi);
+      post_launch("zero32");
+    }
+    fas_fmg<D, float>(P, c32, false, true);
+    for (int v = 1; v < n_inner; v++) fas_vcycle<D, float>(P, c32, max_lvl, false);
+    // phi += e (FP64), then the FP64 ghost layer of phi, coarse to fine
+    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
+      const int n = P.n_ids(lvl);
+      if (n == 0) continue;
+      const int nc = P.nc_lvl(lvl);
+      Geo g = geo(n, cells_of(D, nc, false));
+      const int pr = P.prof_begin(PK_OTHER, lvl, (2.0 * 8 + 4) * n * cells_of(D, nc, false));
+      hipLaunchKernelGGL((k_add32<D>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
+                         g.tpb, c64.i_phi, c32.i_phi);
+      post_launch("add32");
+      P.prof_end(pr);
+    }
+    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) fill_gc_phi<D, double>(P, c64, lvl);
+  }
+  *outer_done = it;
+  *rel_res = rel;
+}
+
+void mg_solve_mixed(Pool& P, double tol, int max_outer, int n_inner, int* outer_done,
+                    double* rel_res) {
+  mg_check(P);
+  if (n_inner < 1) throw std::invalid_argument("mixed solve: n_inner must be >= 1");
+  if (P.tree.dim == 2) mixed_solve<2>(P, tol, max_outer, n_inner, outer_done, rel_res);
+  else mixed_solve<3>(P, tol, max_outer, n_inner, outer_done, rel_res);
 }
 
 void gc_tree(Pool& P, int iv, int rb, int bc) {

@@ -15,6 +15,8 @@ namespace a5g {
 struct Dev {
   double* cc = nullptr;  // [n_cc][cap+1][bs]; box id at slot id
   double* fc = nullptr;  // [n_fc][cap+1][dim][fs]
+  float* cc32 = nullptr;  // FP32 scratch variables [n32][cap+1][bs]
+  size_t vs32 = 0;
   size_t vs = 0;         // per-variable stride of cc ((cap+1) * bs)
   size_t fvs = 0;        // per-variable stride of fc ((cap+1) * dim * fs)
   int bs = 0;            // box stride: (n_cell+2)^dim
@@ -69,6 +71,10 @@ struct Pool {
   int dev_cap = 0;  // boxes allocated on the device (slots 0..dev_cap)
   double* cc = nullptr;
   double* fc = nullptr;
+  // FP32 variables of the mixed-precision solve (scratch: their contents do
+  // not survive a capacity change or a tidy_up)
+  float* cc32 = nullptr;
+  int n32 = 0, cap32 = 0;
   // device topology
   int *d_lvl = nullptr, *d_ix = nullptr, *d_parent = nullptr, *d_children = nullptr;
   int *d_nbs = nullptr, *d_ncell = nullptr;

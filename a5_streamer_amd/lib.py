@@ -201,6 +201,12 @@ class Tree:
     def fas_vcycle(self, max_lvl, set_residual):
         _call("a5g_fas_vcycle", self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
 
+    def solve_mixed(self, tol=1e-10, max_outer=30, n_inner=1, zero_start=True):
+        it, rel = C.c_int(), C.c_double()
+        _call("a5g_mg_solve_mixed", self.h, C.c_double(tol), C.c_int(max_outer),
+              C.c_int(n_inner), C.c_int(int(zero_start)), C.byref(it), C.byref(rel))
+        return it.value, rel.value
+
     def residual_norms(self):
         r, b = C.c_double(), C.c_double()
         _call("a5g_mg_residual_norms", self.h, C.byref(r), C.byref(b))
@@ -301,6 +307,12 @@ class Tree:
         _call("a5g_prof_read", self.h, C.c_int(kind), C.c_int(lvl), C.byref(n),
               C.byref(ms), C.byref(by))
         return n.value, ms.value, by.value
+
+    def tree_maxdiff(self, iv_a, iv_b):
+        d, b = C.c_double(), C.c_double()
+        _call("a5g_tree_maxdiff_cc", self.h, C.c_int(iv_a), C.c_int(iv_b), C.byref(d),
+              C.byref(b))
+        return d.value, b.value
 
     def tree_minmax(self, iv):
         lo, hi = C.c_double(), C.c_double()

@@ -90,6 +90,15 @@ int a5g_mg_set_bc_value(a5g_tree *t, int nb, double value);
 int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 /* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
 int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
+/* Mixed-precision solve of the same problem to max|r| <= tol max|rhs| over
+ * the leaves: an FP64 defect-correction loop whose correction A e = r (zero
+ * boundary data) is solved by FP32 FAS multigrid — one FMG plus
+ * n_inner - 1 V-cycles, the schedule of mg$D_fas_fmg / mg$D_fas_vcycle —
+ * while the residual, the update phi += e and the ghost cells of phi stay
+ * FP64. zero_start = 1 starts from phi = 0. No reference counterpart: the
+ * reference solves in FP64 only (afivo/src/m_aX_t.f90:11). */
+int a5g_mg_solve_mixed(a5g_tree *t, double tol, int max_outer, int n_inner,
+                       int zero_start, int *n_outer, double *rel_res);
 /* max over leaf cells of |rhs - A phi| (scratch: i_tmp), and of |rhs| */
 int a5g_mg_residual_norms(a5g_tree *t, double *max_res, double *max_rhs);
 
@@ -160,6 +169,11 @@ int a5g_fill_cc(a5g_tree *t, int n_ids, const int *ids, int iv, double value);
  * a5g_gc_tree) */
 int a5g_gc_boxes(a5g_tree *t, int n_ids, const int *ids, int iv, int rb,
                  int bc);
+
+/* max over leaf cells of |cc(iv_a) - cc(iv_b)| and of |cc(iv_b)| (the
+ * error of one solution against another) */
+int a5g_tree_maxdiff_cc(a5g_tree *t, int iv_a, int iv_b, double *maxdiff,
+                        double *maxabs_b);
 
 /* ---- initial conditions --------------------------------------------- */
 
