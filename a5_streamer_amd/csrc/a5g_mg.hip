@@ -424,7 +424,8 @@ __global__ __launch_bounds__(256) void k_rstr_clear(Dev a, const int* __restrict
 // at m_aX_restrict.f90:106.
 template <int D>
 __global__ __launch_bounds__(256) void k_restrict_box(Dev a, const int* __restrict__ list,
-                                                      int n, int nc, int tpb, int iv) {
+                                                      int n, int nc, int tpb, int iv,
+                                                      bool plain) {
   BOX_PROLOGUE
   const int pid = a.parent[id];
   const int pnc = a.ncell[pid];
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256) void k_restrict_box(Dev a, const int* __restri
     const int ck = D == 3 ? q / (hnc * hnc) + 1 : 0;
     const int fi = 2 * ci - 1, fj = 2 * cj - 1, fk = D == 3 ? 2 * ck - 1 : 0;
     const int pc = cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
-    if (D == 2 && a.coord == 1) {
+    if (D == 2 && a.coord == 1 && !plain) {
       const double dr16 = 0.0625 * a.dr[pid];
       const double rr = a.rmin[pid * 3] + (ci - 0.5) * a.dr[pid];
       const double rfac = dr16 / rr;
@@ -1114,6 +1115,8 @@ static void ensure32(Pool& P, int n32) {
   P.cap32 = need;
 }
 
+static void restrict_levels(Pool& P, int iv, bool plain);
+
 template <int D>
 static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* outer_done,
                         double* rel_res) {
@@ -1168,18 +1171,23 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
     }
     fas_fmg<D, float>(P, c32, false, true);
     for (int v = 1; v < n_inner; v++) fas_vcycle<D, float>(P, c32, max_lvl, false);
-    // phi += e (FP64), then the FP64 ghost layer of phi, coarse to fine
+    // phi += e (FP64) on the leaves; parents then take the restriction of
+    // the new leaf values, which is where the FAS iteration leaves them at
+    // convergence (update_coarse, m_aX_mg.f90:430-486), so that the
+    // refinement-boundary ghosts of the next residual see the same composite
+    // problem as the FP64 solve. Then the FP64 ghost layer, coarse to fine.
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
-      const int n = P.n_ids(lvl);
+      const int n = P.n_leaves(lvl);
       if (n == 0) continue;
       const int nc = P.nc_lvl(lvl);
       Geo g = geo(n, cells_of(D, nc, false));
       const int pr = P.prof_begin(PK_OTHER, lvl, (2.0 * 8 + 4) * n * cells_of(D, nc, false));
-      hipLaunchKernelGGL((k_add32<D>), dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
-                         g.tpb, c64.i_phi, c32.i_phi);
+      hipLaunchKernelGGL((k_add32<D>), dim3(g.grid), dim3(256), 0, P.stream, a, P.leaves(lvl), n,
+                         nc, g.tpb, c64.i_phi, c32.i_phi);
       post_launch("add32");
       P.prof_end(pr);
     }
+    restrict_levels(P, c64.i_phi, true);
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) fill_gc_phi<D, double>(P, c64, lvl);
   }
   *outer_done = it;
@@ -1200,10 +1208,11 @@ void gc_tree(Pool& P, int iv, int rb, int bc) {
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) gc_level(P, lvl, iv, rb, bc);
 }
 
-void restrict_tree(Pool& P, int iv) {
-  // a$D_restrict_tree, m_aX_restrict.f90:54-64: children of the parents of
-  // lvl are exactly the boxes of lvl+1
-  P.sync_topology();
+// Restriction of iv from every level to its parents, finest first. plain
+// selects the multigrid's own restriction (mg$D_box_rstr_lpl,
+// m_aX_mg.f90:851-893: the 2^D average in every coordinate system) instead of
+// a$D_restrict_box (radius-weighted in cylindrical coordinates).
+static void restrict_levels(Pool& P, int iv, bool plain) {
   for (int lvl = P.tree.max_lvl - 1; lvl >= P.tree.min_lvl; lvl--) {
     const int n = P.n_ids(lvl + 1);
     if (n == 0) continue;
@@ -1212,14 +1221,21 @@ void restrict_tree(Pool& P, int iv) {
     if (P.tree.dim == 2) {
       Geo g = geo(n, hnc * hnc);
       hipLaunchKernelGGL((k_restrict_box<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
-                         P.ids(lvl + 1), n, nc, g.tpb, iv);
+                         P.ids(lvl + 1), n, nc, g.tpb, iv, plain);
     } else {
       Geo g = geo(n, hnc * hnc * hnc);
       hipLaunchKernelGGL((k_restrict_box<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
-                         P.ids(lvl + 1), n, nc, g.tpb, iv);
+                         P.ids(lvl + 1), n, nc, g.tpb, iv, plain);
     }
     post_launch("restrict");
   }
+}
+
+void restrict_tree(Pool& P, int iv) {
+  // a$D_restrict_tree, m_aX_restrict.f90:54-64: children of the parents of
+  // lvl are exactly the boxes of lvl+1
+  P.sync_topology();
+  restrict_levels(P, iv, false);
 }
 
 void tree_copy_cc(Pool& P, int from, int to) {

@@ -61,6 +61,7 @@ class Tree:
                  lvls_max=30, coord=0, n_boxes=100):
         self.dim = dim
         self.n_cell = n_cell
+        self.dr_base = dr
         self.h = C.c_void_p()
         r_min = np.zeros(dim) if r_min is None else np.asarray(r_min, float)
         rm, prm = _d(r_min)
@@ -313,6 +314,10 @@ class Tree:
         _call("a5g_tree_maxdiff_cc", self.h, C.c_int(iv_a), C.c_int(iv_b), C.byref(d),
               C.byref(b))
         return d.value, b.value
+
+    def min_dr(self):
+        """a$D_min_dr (afivo/src/m_aX_utils.f90:196-200): dr of the finest level."""
+        return self.dr_base * 0.5 ** (self.info()["max_lvl"] - 1)
 
     def tree_minmax(self, iv):
         lo, hi = C.c_double(), C.c_double()

@@ -1,0 +1,424 @@
+"""Host side of a streamer run, following the control flow of the reference
+drivers src/streamer_2d.f90:23-143, src/streamer_3d.f90:23-143 and
+src/streamer_cyl.f90:26-152 over the a5g C ABI.
+
+The per-box physics runs on the GPU; the host keeps what the drivers keep:
+configuration, initial conditions (evaluated on the host and uploaded, as
+set_init_cond/set_box_eps do per box), the refinement criterion (evaluated
+from per-box maxima that the GPU reduces), the time-step rule and the loop.
+The run is written against the Tree interface of a5_streamer_amd.lib, so any
+object with the same methods can drive it; the tests use that to step the
+same run through the CPU restatement and compare every state bit for bit.
+
+Output (Silo files, streamer properties) and Monte-Carlo photoionization are
+not part of this path: a configuration that asks for photoionization is
+rejected.
+"""
+import math
+
+import numpy as np
+
+EPS0 = 8.8541878176e-12  # fosito/m_units_constants.f90:34
+ELEC = 1.6022e-19        # fosito/m_units_constants.f90:35
+EPS_F64 = 2.220446049250313e-16  # epsilon(1.0_dp)
+
+# variable indices of src/m_streamer.f90:16-43
+I_ELEC, I_PION, I_ELEC_OLD, I_PION_OLD, I_PHI, I_FLD, I_RHS, I_PHO, I_EPS = range(1, 10)
+N_VAR_CELL, N_VAR_FACE = 9, 2
+F_ELEC, F_FLD = 1, 2
+T_MOB, T_DIF, T_ALPHA, T_ETA = 1, 2, 3, 4
+RB_MG, RB_INTERP, RB_INTERP_LIM = 0, 1, 2   # mg_sides_rb, gc_interp, gc_interp_lim
+BC_MG, BC_NEUMANN = 0, 1                    # the solver's sides_bc, gc_neumann
+BC_COPY, BC_NEG, BC_2V = 0, 1, 2            # sides_bc_pot rules per face
+N_FMG_CYCLES = 1                            # src/m_streamer.f90:46
+
+
+def norm2(v):
+    """NORM2 as the flang runtime evaluates it in double precision: running
+    maximum m and s = sum((x/m)^2) over the other entries, result
+    m sqrt(1 + s)."""
+    mx, s = 0.0, 0.0
+    for x in v:
+        ax = abs(x)
+        if mx == 0:
+            mx = ax
+        elif ax > mx:
+            q = mx / ax
+            qq = q * q
+            s = s * qq
+            s = s + qq
+            mx = ax
+        else:
+            q = ax / mx
+            s = s + q * q
+    return mx * math.sqrt(1 + s)
+
+
+def dist_line(r, r0, r1):
+    """GM_dist_line, src/m_geom.f90:29-46."""
+    d = len(r)
+    line_len2 = 0.0
+    for k in range(d):
+        t = r1[k] - r0[k]
+        line_len2 = line_len2 + t * t
+    temp = 0.0
+    for k in range(d):
+        temp = temp + (r[k] - r0[k]) * (r1[k] - r0[k])
+    if temp <= 0.0:
+        return norm2([r[k] - r0[k] for k in range(d)])
+    if temp >= line_len2:
+        return norm2([r[k] - r1[k] for k in range(d)])
+    proj = [r0[k] + temp / line_len2 * (r1[k] - r0[k]) for k in range(d)]
+    return norm2([r[k] - proj[k] for k in range(d)])
+
+
+_LOG_HALF_HUGE = math.log(0.5 * 1.7976931348623157e308)
+
+
+def dens_line(r, r0, r1, width, kind):
+    """GM_dens_line with GM_sigmoid/_gaussian/_smoothstep/_step
+    (src/m_geom.f90:48-110); kind 1..4 as GM_*_t (src/m_geom.f90:8-11)."""
+    dist = dist_line(r, r0, r1)
+    if kind == 1:
+        tmp = dist / width
+        if tmp > _LOG_HALF_HUGE:
+            return 0.0
+        return 2 / (1 + math.exp(tmp))
+    if kind == 2:
+        q = dist / width
+        return math.exp(-(q * q))
+    if kind == 3:
+        if dist < width:
+            return 1.0
+        if dist < 2 * width:
+            t = dist / width - 1
+            return 1 - (3 * (t * t) - 2 * (t * t * t))
+        return 0.0
+    if kind == 4:
+        return 1.0 if dist < width else 0.0
+    return 0.0
+
+
+class Table:
+    """Host copy of the uniform transport table for the scalar lookups the
+    drivers make (LT_get_col, fosito/m_lookup_table.f90:100-144)."""
+
+    def __init__(self, rows_cols, x_min, inv_dx):
+        self.rc = np.asarray(rows_cols)
+        self.n_rows = self.rc.shape[1]
+        self.x_min, self.inv_dx = x_min, inv_dx
+
+    def col(self, c, x):
+        frac = (x - self.x_min) * self.inv_dx
+        li = math.ceil(frac)
+        lf = li - frac
+        if li < 1:
+            li, lf = 1, 1.0
+        elif li >= self.n_rows:
+            li, lf = self.n_rows - 1, 0.0
+        t = self.rc[c - 1]
+        return lf * t[li - 1] + (1 - lf) * t[li]
+
+
+# per-driver constants (the lines in each src/streamer_*.f90 that differ)
+VARIANTS = {
+    "2d": dict(dim=2, coord=0, n_cycle_base=8, init_window=2.0e-9, crv_refine=1.0,
+               exit_on_rm=False, time_in_substep=False, dielectric_rule=True),
+    "3d": dict(dim=3, coord=0, n_cycle_base=8, init_window=2.5e-9, crv_refine=0.1,
+               exit_on_rm=True, time_in_substep=False, dielectric_rule=True),
+    "cyl": dict(dim=2, coord=1, n_cycle_base=4, init_window=5.0e-9, crv_refine=0.0,
+                exit_on_rm=False, time_in_substep=True, dielectric_rule=False),
+}
+
+
+class Config:
+    """Run parameters the drivers read from their config files
+    (ST_create_cfg, src/m_streamer.f90:145-228). The defaults here are the 2D
+    air streamer: 32 mm square domain of 8x8-cell boxes on a 4x4-box base
+    (1 mm cells), 2.5 MV/m, a Gaussian seed of peak 1e20 m^-3 and width 0.4 mm
+    at (16, 28) mm over 1e15 m^-3, no dielectric, 9 levels (3.9 um)."""
+
+    def __init__(self, variant="2d", **kw):
+        if variant not in VARIANTS:
+            raise KeyError("variant must be one of %s" % sorted(VARIANTS))
+        self.variant = variant
+        self.domain_len = 32e-3
+        self.box_size = 8
+        self.n_base = 4                # base boxes per direction, int or tuple;
+        #                                domain_len spans the first direction
+        self.lvls_max = 9
+        self.applied_fld = 2.5e6
+        self.bg_dens = 1e15
+        self.seed_dens = [1e20]
+        self.seed_r0 = [[16e-3, 28e-3]]
+        self.seed_r1 = [[16e-3, 28e-3]]
+        self.seed_width = [0.4e-3]
+        self.seed_falloff = [2]
+        self.dt_max = 1e-8
+        self.steps_amr = 2
+        self.epsilon_diel = 1.0
+        self.photoi = False
+        self.table_size = 1000
+        self.table_max_fld = 3e7
+        self.n_boxes = 1000
+        self.transport = None          # (x, [mob, dif, alpha, eta]) or None
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise KeyError("unknown config key %s" % k)
+            setattr(self, k, v)
+        if self.photoi:
+            raise ValueError("photoionization is not provided")
+
+    def transport_columns(self):
+        """Input rows of the transport table. The default is the air model of
+        the 2D benchmark: mobility 0.03 m^2/Vs, diffusion 0.2 m^2/s,
+        alpha(E) = 4.3e5 exp(-2e7/E) 1/m, eta = 0, given as 500 rows on
+        0..3e7 V/m (the reader's row limit, src/m_transport_data.f90:9)."""
+        if self.transport is not None:
+            return self.transport
+        x = np.linspace(0.0, self.table_max_fld, 500)
+        alpha = np.array([4.3e5 * math.exp(-2e7 / v) if v > 0 else 0.0 for v in x])
+        return x, [np.full(500, 0.03), np.full(500, 0.2), alpha, np.zeros(500)]
+
+
+def base_grid(dim, n):
+    """ix and neighbour lists of a block of base boxes, n per direction (an
+    int, or one count per direction), physical boundaries (-1) outside; ids
+    in lexicographic order, x fastest."""
+    shape = [n] * dim if np.isscalar(n) else list(n)
+    ix, idx = [], {}
+    for p in np.ndindex(*reversed(shape)):
+        q = tuple(reversed(p))
+        idx[q] = len(ix) + 1
+        ix.append([c + 1 for c in q])
+    nbl = []
+    for q in (tuple(c - 1 for c in r) for r in ix):
+        row = []
+        for d in range(dim):
+            for s in (-1, 1):
+                o = list(q)
+                o[d] += s
+                row.append(idx.get(tuple(o), -1))
+        nbl.append(row)
+    return np.array(ix, np.int32), np.array(nbl, np.int32)
+
+
+class StreamerRun:
+    """One streamer simulation on a tree made by ``tree_cls`` (the HIP
+    library's Tree, or any class with its constructor and methods)."""
+
+    def __init__(self, tree_cls, cfg):
+        c = cfg
+        v = VARIANTS[c.variant]
+        self.c, self.v = c, v
+        self.dim = v["dim"]
+        self.time = 0.0
+        self.dt = 0.0
+        self.step_no = 0
+        # init_tree: box_size cells per base box; one base box of the whole
+        # domain uses coarsen_to=2 as the drivers do
+        n0 = c.n_base if np.isscalar(c.n_base) else c.n_base[0]
+        dr = c.domain_len / (c.box_size * n0)
+        self.T = T = tree_cls(self.dim, c.box_size, N_VAR_CELL, N_VAR_FACE, dr,
+                              coarsen_to=2, lvls_max=c.lvls_max, coord=v["coord"],
+                              n_boxes=c.n_boxes)
+        T.set_base(*base_grid(self.dim, c.n_base))
+        # ST_load_transport_data (src/m_streamer.f90:273-333)
+        T.table_create(0.0, c.table_max_fld, c.table_size, 4)
+        x, cols = c.transport_columns()
+        for k, y in enumerate(cols, start=1):
+            T.table_set_col(k, x, y)
+        self.tbl = Table(*T.table_get())
+        T.streamer_setup(dielectric_rule=v["dielectric_rule"])
+        # sides_bc_pot: Neumann sides, grounded at the low end of the last
+        # axis, 2V - phi at its high end
+        self.voltage = -c.domain_len * c.applied_fld
+        kinds = [BC_COPY] * (2 * self.dim)
+        kinds[-2], kinds[-1] = BC_NEG, BC_2V
+        vals = [0.0] * (2 * self.dim)
+        vals[-1] = self.voltage
+        T.mg_setup(I_PHI, I_FLD, I_RHS, i_eps=I_EPS, n_base=v["n_cycle_base"],
+                   bc_kind=np.array(kinds, np.int32), bc_val=np.array(vals),
+                   auto_ops=True)
+
+    # --- set_init_cond / set_box_eps (src/streamer_2d.f90:218-268) -----------
+    def _eps_at(self, r):
+        c = self.c
+        if self.c.variant == "3d":
+            x = r[0] / c.domain_len
+            return c.epsilon_diel if 0.375 < x < 0.5 else 1.0
+        k = 0 if c.variant == "cyl" else 1
+        return c.epsilon_diel if r[k] / c.domain_len < 0.25 else 1.0
+
+    def _cell_coords(self, nc, dr, rmin):
+        n = nc + 2
+        axes = [rmin[d] + (np.arange(n) - 0.5) * dr for d in range(self.dim)]
+        # box layout: x fastest
+        return [tuple(axes[d][p[self.dim - 1 - d]] for d in range(self.dim))
+                for p in np.ndindex(*([n] * self.dim))]
+
+    def _box_fields(self, b, i, want_dens):
+        c = self.c
+        pts = self._cell_coords(int(b["n_cell"][i]), b["dr"][i], b["r_min"][i])
+        eps = np.array([self._eps_at(r) for r in pts])
+        if not want_dens:
+            return None, eps
+        elec = np.empty(len(pts))
+        for q, r in enumerate(pts):
+            val = c.bg_dens
+            for k in range(len(c.seed_dens)):
+                val = val + c.seed_dens[k] * dens_line(
+                    r, c.seed_r0[k], c.seed_r1[k], c.seed_width[k], c.seed_falloff[k])
+            elec[q] = val
+        return elec, eps
+
+    def set_init_cond(self):
+        T = self.T
+        b = T.boxes()
+        for i in range(len(b["lvl"])):
+            if not b["in_use"][i]:
+                continue
+            elec, eps = self._box_fields(b, i, True)
+            T.set_cc(i + 1, I_ELEC, elec)
+            T.set_cc(i + 1, I_PION, elec)
+            T.set_cc(i + 1, I_PHI, np.zeros_like(elec))
+            T.set_cc(i + 1, I_EPS, eps)
+
+    def _set_eps(self, ids):
+        if self.c.epsilon_diel == 1.0:
+            self.T.fill_cc(ids, I_EPS, 1.0)
+            return
+        b = self.T.boxes()
+        for i in ids:
+            self.T.set_cc(int(i), I_EPS, self._box_fields(b, int(i) - 1, False)[1])
+
+    # --- compute_fld (src/streamer_2d.f90:307-343) ---------------------------
+    def compute_fld(self, no_guess):
+        T = self.T
+        T.space_charge(ELEC / EPS0)
+        for i in range(N_FMG_CYCLES):
+            T.fas_fmg(False, no_guess and i == 0)
+        T.fld_from_pot()
+        T.gc_tree(I_FLD, RB_INTERP, BC_NEUMANN)
+
+    # --- set_ref_flags (src/streamer_2d.f90:177-216) ------------------------
+    def ref_flags(self):
+        T, c, v = self.T, self.c, self.v
+        b = T.boxes()
+        s = T.box_summaries()
+        n = len(b["lvl"])
+        flags = np.zeros(n, np.int32)
+        for i in range(n):
+            if not b["in_use"][i] or b["lvl"][i] < 1:
+                continue
+            crv_phi, max_fld = s[i, 0], s[i, 1]
+            dr = b["dr"][i]
+            adx = dr * self.tbl.col(T_ALPHA, max_fld)
+            f = 0
+            if adx < 0.1 and dr < 2.0e-5:
+                f = -1
+            if adx < 0.1 and crv_phi < 4.0 and dr < 5.0e-5:
+                f = -1
+            if self.time < v["init_window"]:
+                nc = int(b["n_cell"][i])
+                boxlen = nc * dr
+                centre = [b["r_min"][i][k] + 0.5 * nc * dr for k in range(self.dim)]
+                for k in range(len(c.seed_dens)):
+                    dist = dist_line(centre, c.seed_r0[k], c.seed_r1[k])
+                    if dist - c.seed_width[k] < boxlen and dr > 0.2 * c.seed_width[k]:
+                        f = 1
+            if adx > 1.0 and crv_phi > v["crv_refine"]:
+                f = 1
+            flags[i] = f
+        return flags
+
+    def adjust(self):
+        return self.T.adjust(self.ref_flags())
+
+    # --- prolong_to_new_boxes (src/streamer_2d.f90:592-617) ------------------
+    def prolong_to_new_boxes(self):
+        T = self.T
+        for lvl in range(1, T.info()["max_lvl"] + 1):
+            add, _ = T.ref_info_lvl(lvl)
+            if len(add) == 0:
+                continue
+            for iv in (I_ELEC, I_PION, I_PHI):
+                T.prolong1_to(add, iv)
+            self._set_eps(add)
+            T.gc_boxes(add, I_ELEC, RB_INTERP_LIM, BC_NEUMANN)
+            T.gc_boxes(add, I_PION, RB_INTERP_LIM, BC_NEUMANN)
+            T.gc_boxes(add, I_PHI, RB_MG, BC_MG)
+
+    # --- get_max_dt (src/streamer_2d.f90:271-303) ----------------------------
+    def max_dt(self):
+        T, t = self.T, self.tbl
+        min_fld, max_fld = T.tree_minmax(I_FLD)
+        dr_min = T.min_dr()
+        mobility = t.col(T_MOB, max_fld)
+        diff_coeff = t.col(T_DIF, max_fld)
+        alpha = t.col(T_ALPHA, max_fld)
+        dt_cfl = dr_min / (mobility * max_fld)
+        dt_dif = 0.25 * (dr_min * dr_min) / diff_coeff
+        # dt_drt (dielectric relaxation) is computed by the drivers but not
+        # part of the minimum
+        dt_alpha = 1 / max(mobility * max_fld * alpha, EPS_F64)
+        return 0.5 * min(1 / (1 / dt_cfl + 1 / dt_dif), dt_alpha, self.c.dt_max)
+
+    # --- the run ------------------------------------------------------------
+    def initialise(self, max_iter=50):
+        """Initial conditions, field and refinement until the mesh settles
+        (src/streamer_2d.f90:57-63)."""
+        for _ in range(max_iter):
+            self.set_init_cond()
+            self.compute_fld(True)
+            n_add, n_rm = self.adjust()
+            if n_add == 0 and (n_rm == 0 or not self.v["exit_on_rm"]):
+                return
+        raise RuntimeError("initial refinement did not settle")
+
+    def _euler_pair(self, dt, cyl_time):
+        T = self.T
+        for i in (1, 2):
+            if cyl_time:
+                self.time = self.time + dt
+            T.fluxes_koren(dt)
+            T.consistent_fluxes([F_ELEC])
+            T.update_solution(dt)
+            T.restrict_tree(I_ELEC)
+            T.restrict_tree(I_PION)
+            T.gc_tree(I_ELEC, RB_INTERP_LIM, BC_NEUMANN)
+            T.gc_tree(I_PION, RB_INTERP_LIM, BC_NEUMANN)
+            if i == 1:
+                self.compute_fld(False)
+        if cyl_time:
+            self.time = self.time - dt
+
+    def step_block(self):
+        """One pass of the outer loop: dt, steps_amr time steps, then the mesh
+        update (src/streamer_2d.f90:68-142). Returns (n_add, n_rm)."""
+        T = self.T
+        self.dt = self.max_dt()
+        if self.dt < 1e-14:
+            raise RuntimeError("dt getting too small, instability?")
+        dt = self.dt
+        cyl_time = self.v["time_in_substep"]
+        for _ in range(self.c.steps_amr):
+            if not cyl_time:
+                self.time = self.time + dt
+            T.tree_copy_cc(I_ELEC, I_ELEC_OLD)
+            T.tree_copy_cc(I_PION, I_PION_OLD)
+            self._euler_pair(dt, cyl_time)
+            T.average_dens()
+            self.compute_fld(False)
+            self.step_no += 1
+        n_add, n_rm = self.adjust()
+        if n_add > 0 or n_rm > 0:
+            self.prolong_to_new_boxes()
+            self.compute_fld(False)
+            T.tidy_up(0.9, 0.25, 5000, False)
+        return n_add, n_rm
+
+    def leaf_cells(self):
+        info = self.T.info()
+        n = sum(len(self.T.level(l)[1]) for l in range(1, info["max_lvl"] + 1))
+        return n * self.c.box_size ** self.dim

@@ -317,3 +317,51 @@ def box_summaries(T, v):
     out = np.zeros((n, 3))
     T.L.o_box_summaries(T.h, C.byref(v), _d(out)[1])
     return out
+
+
+class OStreamTree(OTree):
+    """OTree with the streamer calls under the names and argument forms of
+    a5_streamer_amd.lib.Tree, so one host driver can step either."""
+
+    def table_create(self, x_min, x_max, n_rows, n_cols):
+        self.tbl = OTable(x_min, x_max, n_rows, n_cols)
+
+    def table_set_col(self, col, xx, yy):
+        self.tbl.set_col(col, xx, yy)
+
+    def table_get(self):
+        return self.tbl.data()
+
+    def streamer_setup(self, dielectric_rule=True):
+        self.v = streamer_vars(dielectric_rule)
+
+    def space_charge(self, fac):
+        space_charge(self, self.v, fac)
+
+    def fld_from_pot(self):
+        fld_from_pot(self, self.v)
+
+    def fluxes_koren(self, dt):
+        fluxes_koren(self, self.v, self.tbl, dt)
+
+    def update_solution(self, dt):
+        update_solution(self, self.v, self.tbl, dt)
+
+    def average_dens(self):
+        average_dens(self, self.v)
+
+    def box_summaries(self):
+        return box_summaries(self, self.v)
+
+    def prolong1_to(self, ids, iv, add=False):
+        for i in np.atleast_1d(ids):
+            OTree.prolong1_to(self, int(i), iv, add)
+
+    def fill_cc(self, ids, iv, value):
+        nc = self.boxes()["n_cell"]
+        for i in np.atleast_1d(ids):
+            self.set_cc(int(i), iv, np.full(self.box_len(int(nc[int(i) - 1])), value))
+
+    def gc_boxes(self, ids, iv, rb, bc):
+        for i in np.atleast_1d(ids):
+            self.gc_box(int(i), iv, rb, bc)

@@ -1,0 +1,151 @@
+"""Host side of the streamer run (a5_streamer_amd/streamer.py): the scalar
+helpers against closed forms, the oracle and the reference's own routines,
+and the run loop stepped through the oracle with conservation checks."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from a5_streamer_amd import streamer as S
+
+
+def test_norm2_matches_oracle_and_reference(oracle_lib, request):
+    rng = np.random.default_rng(7)
+    cases = [np.zeros(2), np.array([0.0, -3.0]), np.array([3.0, 4.0]),
+             np.array([1e-300, 1e-300]), np.array([1e300, -1e300, 1e299])]
+    for _ in range(2000):
+        n = int(rng.integers(2, 4))
+        v = rng.standard_normal(n) * 10.0 ** rng.integers(-30, 30, n)
+        v[rng.random(n) < 0.1] = 0.0
+        cases.append(v)
+    ref = None
+    try:
+        ref = request.getfixturevalue("ref")
+    except pytest.skip.Exception:
+        pass
+    for v in cases:
+        a = S.norm2(list(v))
+        assert a == O.norm2(v)
+        if ref is not None:
+            assert a == ref.norm2(v)
+    # the scaled form stays finite where the plain sum of squares overflows
+    assert S.norm2([3e300, 4e300]) == pytest.approx(5e300, rel=1e-15)
+    assert S.norm2([3.0, 4.0]) == 5.0
+
+
+def test_dist_line_against_brute_force():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        r, r0, r1 = (rng.standard_normal(2) for _ in range(3))
+        t = np.linspace(0.0, 1.0, 20001)
+        pts = r0[None, :] + t[:, None] * (r1 - r0)[None, :]
+        brute = np.min(np.linalg.norm(pts - r[None, :], axis=1))
+        assert S.dist_line(list(r), list(r0), list(r1)) == pytest.approx(brute, abs=2e-4)
+    # point seed: the distance is the Euclidean one
+    assert S.dist_line([3.0, 4.0], [0.0, 0.0], [0.0, 0.0]) == 5.0
+
+
+def test_dens_line_profiles():
+    w = 0.5
+    for d in (0.0, 0.1, 0.49, 0.5, 0.9, 1.2):
+        r = [d, 0.0]
+        assert S.dens_line(r, [0, 0], [0, 0], w, 2) == math.exp(-(d / w) ** 2)
+        assert S.dens_line(r, [0, 0], [0, 0], w, 1) == pytest.approx(
+            2 / (1 + math.exp(d / w)), rel=1e-15)
+        assert S.dens_line(r, [0, 0], [0, 0], w, 4) == (1.0 if d < w else 0.0)
+    # smoothstep: 1 inside w, 0 beyond 2w, 1/2 at 1.5w, continuous at both ends
+    assert S.dens_line([0.2, 0], [0, 0], [0, 0], w, 3) == 1.0
+    assert S.dens_line([0.75, 0], [0, 0], [0, 0], w, 3) == pytest.approx(0.5)
+    assert S.dens_line([1.0 - 1e-12, 0], [0, 0], [0, 0], w, 3) == pytest.approx(0.0, abs=1e-9)
+    assert S.dens_line([1.1, 0], [0, 0], [0, 0], w, 3) == 0.0
+    # sigmoid far away underflows to exactly zero
+    assert S.dens_line([1e6, 0], [0, 0], [0, 0], 1e-3, 1) == 0.0
+
+
+def test_table_col_matches_oracle(oracle_lib):
+    cfg = S.Config()
+    x, cols = cfg.transport_columns()
+    t = O.OTable(0.0, 3e7, 1000, 4)
+    for k, y in enumerate(cols, start=1):
+        t.set_col(k, x, y)
+    host = S.Table(*t.data())
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-1e6, 3.2e7, 3000), [0.0, 3e7, -1.0, 4e7]])
+    for xv in xs:
+        for c in (1, 2, 3, 4):
+            assert host.col(c, xv) == t.get_col(c, xv)
+
+
+def test_base_grid_neighbours_are_symmetric():
+    for dim, n in ((2, 4), (2, (1, 2)), (3, 2), (3, (2, 1, 3))):
+        ix, nb = S.base_grid(dim, n)
+        shape = [n] * dim if np.isscalar(n) else list(n)
+        assert len(ix) == int(np.prod(shape))
+        for i in range(len(ix)):
+            for d in range(2 * dim):
+                j = nb[i, d]
+                if j > 0:
+                    assert nb[j - 1, d ^ 1] == i + 1
+                    assert abs(ix[j - 1] - ix[i]).sum() == 1
+                else:
+                    side = d % 2
+                    c = ix[i, d // 2]
+                    assert c == (1 if side == 0 else shape[d // 2])
+
+
+def _leaf_total(T, iv, dim):
+    b = T.boxes()
+    tot = 0.0
+    for lvl in range(1, T.info()["max_lvl"] + 1):
+        for i in T.level(lvl)[1]:
+            nc = int(b["n_cell"][i - 1])
+            v = T.get_cc(int(i), iv, nc).reshape([nc + 2] * dim)
+            inner = v[(slice(1, -1),) * dim]
+            tot += inner.sum() * b["dr"][i - 1] ** dim
+    return tot
+
+
+def test_run_conserves_electrons_without_sources(oracle_lib):
+    """With alpha = eta = 0, no background and the seed away from the walls,
+    the conservative flux update with consistent coarse-fine fluxes keeps the
+    electron count constant and the ions untouched."""
+    x = np.linspace(0.0, 3e7, 500)
+    cfg = S.Config(lvls_max=6, bg_dens=0.0, seed_width=[1.0e-3],
+                   seed_r0=[[16e-3, 16e-3]], seed_r1=[[16e-3, 16e-3]],
+                   transport=(x, [np.full(500, 0.03), np.full(500, 0.2),
+                                  np.zeros(500), np.zeros(500)]))
+    run = S.StreamerRun(O.OStreamTree, cfg)
+    run.initialise()
+    assert run.T.info()["max_lvl"] >= 3
+    e0 = _leaf_total(run.T, S.I_ELEC, 2)
+    i0 = _leaf_total(run.T, S.I_PION, 2)
+    t_sum = 0.0
+    for _ in range(3):
+        n_add, n_rm = run.step_block()
+        t_sum += 2 * run.dt
+        assert n_add == 0 and n_rm == 0  # mesh stays put without ionization
+    assert _leaf_total(run.T, S.I_ELEC, 2) == pytest.approx(e0, rel=1e-12)
+    assert _leaf_total(run.T, S.I_PION, 2) == i0
+    assert run.step_no == 6 and run.time == pytest.approx(t_sum, rel=1e-12)
+
+
+def test_run_refines_around_seed_then_steps(oracle_lib):
+    cfg = S.Config(lvls_max=7)
+    run = S.StreamerRun(O.OStreamTree, cfg)
+    run.initialise()
+    info = run.T.info()
+    b = run.T.boxes()
+    # seed refinement stops once dr <= 0.2 * width
+    fine = [i for i in range(len(b["lvl"])) if b["in_use"][i] and b["lvl"][i] == info["max_lvl"]]
+    assert b["dr"][fine[0]] <= 0.2 * cfg.seed_width[0] < 2 * b["dr"][fine[0]]
+    dt = run.max_dt()
+    assert 0 < dt <= 0.5 * cfg.dt_max
+    run.step_block()
+    lo, hi = run.T.tree_minmax(S.I_ELEC)
+    assert lo > 0 and hi < 1.01 * cfg.seed_dens[0]
+
+
+def test_rejects_photoionization():
+    with pytest.raises(ValueError):
+        S.Config(photoi=True)
