@@ -1,4 +1,6 @@
-"""Builds the a5g HIP library for gfx950 in-tree (a5_streamer_amd/liba5g.so).
+"""Builds the a5g HIP library for gfx950 in-tree (a5_streamer_amd/liba5g.so),
+then the Fortran binding and the 2D streamer driver on it
+(a5_streamer_amd/fortran, with flang).
 
     python -m a5_streamer_amd.build
 """
@@ -32,6 +34,10 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    fdir = os.path.join(HERE, "fortran")
+    fc = os.environ.get("FC", "flang")
+    # the driver binary links liba5g.so: rebuild it against the new library
+    subprocess.check_call(["make", "-s", "-B", "-C", fdir, "FC=" + fc])
     return OUT
 
 

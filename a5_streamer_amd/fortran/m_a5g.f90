@@ -1,0 +1,926 @@
+!> Fortran binding of the a5g C ABI (include/a5g.h) for the streamer drivers.
+!>
+!> The raw interfaces (c_*) bind every entry point of the C ABI with
+!> iso_c_binding. The public wrappers keep Afivo's and the drivers' names and
+!> argument meanings (afivo/src/m_aX_core.f90, m_aX_mg.f90, m_aX_gc.f90,
+!> m_aX_utils.f90, src/streamer_2d.f90) so a driver keeps its control flow:
+!>   - a2_*/a3_* and mg2_*/mg3_* are the same procedures (the tree handle
+!>     carries the dimension);
+!>   - callbacks become small integers: ghost-cell rules (a5_gc_*), the
+!>     multigrid face rules (a5_bc_*) and refinement flags by id;
+!>   - the drivers' box procedures (fld_from_pot, fluxes_koren,
+!>     update_solution, average_dens and compute_fld's rhs) are st_* calls.
+!> A non-zero status from the library stops the program with its message,
+!> which is the reference's fail-fast behaviour (stop "msg").
+module m_a5g
+  use iso_c_binding
+  implicit none
+  private
+
+  integer, parameter :: dp = kind(0.0d0)
+
+  ! ghost-cell rule at refinement boundaries (a5g_gc_tree: rb)
+  integer, parameter, public :: a5_gc_sides_rb   = 0 ! mg$D_sides_rb
+  integer, parameter, public :: a5_gc_interp     = 1 ! a$D_gc_interp
+  integer, parameter, public :: a5_gc_interp_lim = 2 ! a$D_gc_interp_lim
+  ! ghost-cell rule at physical boundaries (a5g_gc_tree: bc)
+  integer, parameter, public :: a5_gc_mg_bc      = 0 ! the multigrid face rules
+  integer, parameter, public :: a5_gc_neumann    = 1 ! a$D_gc_neumann
+  integer, parameter, public :: a5_gc_dirichlet  = 2 ! a$D_gc_dirichlet
+  ! multigrid face rules (the drivers' sides_bc_pot)
+  integer, parameter, public :: a5_bc_copy  = 0 ! ghost = interior (Neumann)
+  integer, parameter, public :: a5_bc_neg   = 1 ! ghost = -interior (grounded)
+  integer, parameter, public :: a5_bc_value = 2 ! ghost = 2 V - interior
+  ! refinement flags (afivo/src/m_afivo_types.f90)
+  integer, parameter, public :: a5_rm_ref = -1, a5_kp_ref = 0, a5_do_ref = 1
+  ! coordinate systems
+  integer, parameter, public :: a5_xyz = 0, a5_cyl = 1
+
+  !> A tree whose cell data lives on the GPU
+  type, public :: a5g_t
+     type(c_ptr) :: h      = c_null_ptr
+     integer     :: dim    = 0
+     integer     :: n_cell = 0
+  end type a5g_t
+
+  !> Multigrid options (mg$D_t, afivo/src/m_aX_mg.f90:20-52); the sides_bc
+  !> procedure becomes one rule and value per physical face
+  type, public :: a5g_mg_t
+     integer  :: i_phi        = -1
+     integer  :: i_tmp        = -1
+     integer  :: i_rhs        = -1
+     integer  :: i_eps        = -1
+     integer  :: n_cycle_down = 2
+     integer  :: n_cycle_up   = 2
+     integer  :: n_cycle_base = 2
+     integer  :: bc_kind(6)   = a5_bc_copy
+     real(dp) :: bc_val(6)    = 0.0_dp
+     logical  :: auto_ops     = .true.
+  end type a5g_mg_t
+
+  !> ref_info_t (afivo/src/m_aX_core.f90:540-601)
+  type, public :: a5g_lvl_info_t
+     integer, allocatable :: add(:)
+     integer, allocatable :: rm(:)
+  end type a5g_lvl_info_t
+
+  type, public :: a5g_ref_info_t
+     integer                           :: n_add = 0
+     integer                           :: n_rm  = 0
+     type(a5g_lvl_info_t), allocatable :: lvls(:)
+  end type a5g_ref_info_t
+
+  ! ------------------------------------------------------------------
+  ! raw interfaces, one per entry point of include/a5g.h
+  ! ------------------------------------------------------------------
+  interface
+     function c_last_error() bind(C, name="a5g_last_error") result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function c_last_error
+
+     function c_init(t, dim, n_cell, n_cc, n_fc, dr, r_min, coarsen_to, &
+          lvls_max, coord, n_boxes) bind(C, name="a5g_init") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr)           :: t
+       integer(c_int), value :: dim, n_cell, n_cc, n_fc, coarsen_to, lvls_max
+       integer(c_int), value :: coord, n_boxes
+       real(c_double), value :: dr
+       real(c_double)        :: r_min(*)
+       integer(c_int)        :: s
+     end function c_init
+
+     function c_destroy(t) bind(C, name="a5g_destroy") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: s
+     end function c_destroy
+
+     function c_set_base(t, n, ix_list, nb_list) bind(C, name="a5g_set_base") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: ix_list(*), nb_list(*)
+       integer(c_int)        :: s
+     end function c_set_base
+
+     function c_adjust_refinement(t, n, flags, n_add, n_rm) &
+          bind(C, name="a5g_adjust_refinement") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: flags(*), n_add, n_rm
+       integer(c_int)        :: s
+     end function c_adjust_refinement
+
+     function c_ref_info_lvl(t, lvl, add, rm, n_add, n_rm) &
+          bind(C, name="a5g_ref_info_lvl") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: lvl
+       type(c_ptr), value    :: add, rm
+       integer(c_int)        :: n_add, n_rm
+       integer(c_int)        :: s
+     end function c_ref_info_lvl
+
+     function c_tidy_up(t, max_frac, goal_frac, n_clean_min, reorder) &
+          bind(C, name="a5g_tidy_up") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: max_frac, goal_frac
+       integer(c_int), value :: n_clean_min, reorder
+       integer(c_int)        :: s
+     end function c_tidy_up
+
+     function c_info(t, out8) bind(C, name="a5g_info") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: out8(8)
+       integer(c_int)     :: s
+     end function c_info
+
+     function c_get_boxes(t, n, lvl, ix, parent, children, neighbors, in_use, &
+          n_cell, dr, r_min) bind(C, name="a5g_get_boxes") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: lvl(*), ix(*), parent(*), children(*)
+       integer(c_int)        :: neighbors(*), in_use(*), n_cell(*)
+       real(c_double)        :: dr(*), r_min(*)
+       integer(c_int)        :: s
+     end function c_get_boxes
+
+     function c_level_counts(t, lvl, counts3) bind(C, name="a5g_level_counts") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: lvl
+       integer(c_int)        :: counts3(3)
+       integer(c_int)        :: s
+     end function c_level_counts
+
+     function c_level_lists(t, lvl, ids, leaves, parents) &
+          bind(C, name="a5g_level_lists") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: lvl
+       integer(c_int)        :: ids(*), leaves(*), parents(*)
+       integer(c_int)        :: s
+     end function c_level_lists
+
+     function c_set_cc(t, id, iv, data) bind(C, name="a5g_set_cc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: id, iv
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_set_cc
+
+     function c_get_cc(t, id, iv, data) bind(C, name="a5g_get_cc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: id, iv
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_get_cc
+
+     function c_set_cc_all(t, iv, data) bind(C, name="a5g_set_cc_all") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_set_cc_all
+
+     function c_get_cc_all(t, iv, data) bind(C, name="a5g_get_cc_all") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_get_cc_all
+
+     function c_set_fc(t, id, dim, ifc, data) bind(C, name="a5g_set_fc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: id, dim, ifc
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_set_fc
+
+     function c_get_fc(t, id, dim, ifc, data) bind(C, name="a5g_get_fc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: id, dim, ifc
+       real(c_double)        :: data(*)
+       integer(c_int)        :: s
+     end function c_get_fc
+
+     function c_sync(t) bind(C, name="a5g_sync") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: s
+     end function c_sync
+
+     function c_mg_setup(t, i_phi, i_tmp, i_rhs, i_eps, n_down, n_up, n_base, &
+          bc_kind, bc_val, auto_ops) bind(C, name="a5g_mg_setup") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: i_phi, i_tmp, i_rhs, i_eps, n_down, n_up, n_base
+       integer(c_int)        :: bc_kind(*)
+       real(c_double)        :: bc_val(*)
+       integer(c_int), value :: auto_ops
+       integer(c_int)        :: s
+     end function c_mg_setup
+
+     function c_mg_set_bc_value(t, nb, value) bind(C, name="a5g_mg_set_bc_value") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: nb
+       real(c_double), value :: value
+       integer(c_int)        :: s
+     end function c_mg_set_bc_value
+
+     function c_fas_fmg(t, set_residual, first_call) bind(C, name="a5g_fas_fmg") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: set_residual, first_call
+       integer(c_int)        :: s
+     end function c_fas_fmg
+
+     function c_fas_vcycle(t, max_lvl, set_residual) bind(C, name="a5g_fas_vcycle") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: max_lvl, set_residual
+       integer(c_int)        :: s
+     end function c_fas_vcycle
+
+     function c_mg_solve_mixed(t, tol, max_outer, n_inner, zero_start, n_outer, &
+          rel_res) bind(C, name="a5g_mg_solve_mixed") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: tol
+       integer(c_int), value :: max_outer, n_inner, zero_start
+       integer(c_int)        :: n_outer
+       real(c_double)        :: rel_res
+       integer(c_int)        :: s
+     end function c_mg_solve_mixed
+
+     function c_mg_residual_norms(t, max_res, max_rhs) &
+          bind(C, name="a5g_mg_residual_norms") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: t
+       real(c_double)     :: max_res, max_rhs
+       integer(c_int)     :: s
+     end function c_mg_residual_norms
+
+     function c_gc_tree(t, iv, rb, bc) bind(C, name="a5g_gc_tree") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv, rb, bc
+       integer(c_int)        :: s
+     end function c_gc_tree
+
+     function c_restrict_tree(t, iv) bind(C, name="a5g_restrict_tree") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv
+       integer(c_int)        :: s
+     end function c_restrict_tree
+
+     function c_tree_copy_cc(t, iv_from, iv_to) bind(C, name="a5g_tree_copy_cc") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv_from, iv_to
+       integer(c_int)        :: s
+     end function c_tree_copy_cc
+
+     function c_tree_minmax_cc(t, iv, vmin, vmax) bind(C, name="a5g_tree_minmax_cc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv
+       real(c_double)        :: vmin, vmax
+       integer(c_int)        :: s
+     end function c_tree_minmax_cc
+
+     function c_table_create(t, x_min, x_max, n_rows, n_cols) &
+          bind(C, name="a5g_table_create") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: x_min, x_max
+       integer(c_int), value :: n_rows, n_cols
+       integer(c_int)        :: s
+     end function c_table_create
+
+     function c_table_set_col(t, col, n, xx, yy) bind(C, name="a5g_table_set_col") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: col, n
+       real(c_double)        :: xx(*), yy(*)
+       integer(c_int)        :: s
+     end function c_table_set_col
+
+     function c_table_get(t, rows_cols, x_min, inv_dx) bind(C, name="a5g_table_get") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: t
+       real(c_double)     :: rows_cols(*), x_min, inv_dx
+       integer(c_int)     :: s
+     end function c_table_get
+
+     function c_set_table(t, n_rows, n_cols, x_min, inv_dx, rows_cols) &
+          bind(C, name="a5g_set_table") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n_rows, n_cols
+       real(c_double), value :: x_min, inv_dx
+       real(c_double)        :: rows_cols(*)
+       integer(c_int)        :: s
+     end function c_set_table
+
+     function c_streamer_setup(t, vars) bind(C, name="a5g_streamer_setup") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: vars(16)
+       integer(c_int)     :: s
+     end function c_streamer_setup
+
+     function c_space_charge(t, fac) bind(C, name="a5g_space_charge") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: fac
+       integer(c_int)        :: s
+     end function c_space_charge
+
+     function c_fld_from_pot(t) bind(C, name="a5g_fld_from_pot") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: s
+     end function c_fld_from_pot
+
+     function c_fluxes_koren(t, dt) bind(C, name="a5g_fluxes_koren") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: dt
+       integer(c_int)        :: s
+     end function c_fluxes_koren
+
+     function c_consistent_fluxes(t, n, f_ixs) bind(C, name="a5g_consistent_fluxes") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: f_ixs(*)
+       integer(c_int)        :: s
+     end function c_consistent_fluxes
+
+     function c_update_solution(t, dt) bind(C, name="a5g_update_solution") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: dt
+       integer(c_int)        :: s
+     end function c_update_solution
+
+     function c_average_dens(t) bind(C, name="a5g_average_dens") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: s
+     end function c_average_dens
+
+     function c_box_summaries(t, out) bind(C, name="a5g_box_summaries") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: t
+       real(c_double)     :: out(*)
+       integer(c_int)     :: s
+     end function c_box_summaries
+
+     function c_prolong1_to(t, n, ids, iv) bind(C, name="a5g_prolong1_to") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n, iv
+       integer(c_int)        :: ids(*)
+       integer(c_int)        :: s
+     end function c_prolong1_to
+
+     function c_fill_cc(t, n, ids, iv, value) bind(C, name="a5g_fill_cc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n, iv
+       integer(c_int)        :: ids(*)
+       real(c_double), value :: value
+       integer(c_int)        :: s
+     end function c_fill_cc
+
+     function c_gc_boxes(t, n, ids, iv, rb, bc) bind(C, name="a5g_gc_boxes") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n, iv, rb, bc
+       integer(c_int)        :: ids(*)
+       integer(c_int)        :: s
+     end function c_gc_boxes
+
+     function c_tree_maxdiff_cc(t, iv_a, iv_b, maxdiff, maxabs_b) &
+          bind(C, name="a5g_tree_maxdiff_cc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv_a, iv_b
+       real(c_double)        :: maxdiff, maxabs_b
+       integer(c_int)        :: s
+     end function c_tree_maxdiff_cc
+
+     function c_set_gaussians(t, iv, n_blob, blobs, base) &
+          bind(C, name="a5g_set_gaussians") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv, n_blob
+       real(c_double)        :: blobs(*)
+       real(c_double), value :: base
+       integer(c_int)        :: s
+     end function c_set_gaussians
+
+     function c_prof_enable(t, on) bind(C, name="a5g_prof_enable") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: on
+       integer(c_int)        :: s
+     end function c_prof_enable
+
+     function c_prof_read(t, kind, lvl, n_launch, ms, bytes) &
+          bind(C, name="a5g_prof_read") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: kind, lvl
+       integer(c_int)        :: n_launch
+       real(c_double)        :: ms, bytes
+       integer(c_int)        :: s
+     end function c_prof_read
+  end interface
+
+  ! Afivo's names for the 2D and 3D trees
+  interface a2_destroy;           module procedure t_destroy;           end interface
+  interface a3_destroy;           module procedure t_destroy;           end interface
+  interface a2_set_base;          module procedure t_set_base;          end interface
+  interface a3_set_base;          module procedure t_set_base;          end interface
+  interface a2_adjust_refinement; module procedure t_adjust_refinement; end interface
+  interface a3_adjust_refinement; module procedure t_adjust_refinement; end interface
+  interface a2_tidy_up;           module procedure t_tidy_up;           end interface
+  interface a3_tidy_up;           module procedure t_tidy_up;           end interface
+  interface a2_gc_tree;           module procedure t_gc_tree;           end interface
+  interface a3_gc_tree;           module procedure t_gc_tree;           end interface
+  interface a2_gc_boxes;          module procedure t_gc_boxes;          end interface
+  interface a3_gc_boxes;          module procedure t_gc_boxes;          end interface
+  interface a2_restrict_tree;     module procedure t_restrict_tree;     end interface
+  interface a3_restrict_tree;     module procedure t_restrict_tree;     end interface
+  interface a2_tree_copy_cc;      module procedure t_tree_copy_cc;      end interface
+  interface a3_tree_copy_cc;      module procedure t_tree_copy_cc;      end interface
+  interface a2_tree_max_cc;       module procedure t_tree_max_cc;       end interface
+  interface a3_tree_max_cc;       module procedure t_tree_max_cc;       end interface
+  interface a2_tree_min_cc;       module procedure t_tree_min_cc;       end interface
+  interface a3_tree_min_cc;       module procedure t_tree_min_cc;       end interface
+  interface a2_consistent_fluxes; module procedure t_consistent_fluxes; end interface
+  interface a3_consistent_fluxes; module procedure t_consistent_fluxes; end interface
+  interface a2_prolong1_to;       module procedure t_prolong1_to;       end interface
+  interface a3_prolong1_to;       module procedure t_prolong1_to;       end interface
+  interface mg2_init_mg;          module procedure t_mg_init;           end interface
+  interface mg3_init_mg;          module procedure t_mg_init;           end interface
+  interface mg2_fas_fmg;          module procedure t_fas_fmg;           end interface
+  interface mg3_fas_fmg;          module procedure t_fas_fmg;           end interface
+  interface mg2_fas_vcycle;       module procedure t_fas_vcycle;        end interface
+  interface mg3_fas_vcycle;       module procedure t_fas_vcycle;        end interface
+
+  public :: a2_init, a3_init
+  public :: a2_destroy, a3_destroy, a2_set_base, a3_set_base
+  public :: a2_adjust_refinement, a3_adjust_refinement, a2_tidy_up, a3_tidy_up
+  public :: a2_gc_tree, a3_gc_tree, a2_gc_boxes, a3_gc_boxes
+  public :: a2_restrict_tree, a3_restrict_tree, a2_tree_copy_cc, a3_tree_copy_cc
+  public :: a2_tree_max_cc, a3_tree_max_cc, a2_tree_min_cc, a3_tree_min_cc
+  public :: a2_consistent_fluxes, a3_consistent_fluxes, a2_prolong1_to, a3_prolong1_to
+  public :: mg2_init_mg, mg3_init_mg, mg2_fas_fmg, mg3_fas_fmg
+  public :: mg2_fas_vcycle, mg3_fas_vcycle
+  public :: a5g_max_id, a5g_max_lvl, a5g_min_dr, a5g_level_leaves, a5g_box_info
+  public :: a5g_set_cc, a5g_get_cc, a5g_fill_cc, a5g_sync
+  public :: a5g_mixed_solve, a5g_residual_norms, a5g_set_bc_value
+  public :: st_setup, st_table_create, st_table_set_col, st_table_get
+  public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
+  public :: st_average_dens, st_box_summaries
+
+contains
+
+  !> Stop with the library's message when a call failed
+  subroutine check(s, what)
+    integer(c_int), intent(in)   :: s
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    character(len=:), allocatable   :: text
+    integer :: n
+
+    if (s == 0) return
+    call c_f_pointer(c_last_error(), msg, [4096])
+    n = 0
+    do while (n < 4096)
+       if (msg(n+1) == c_null_char) exit
+       n = n + 1
+    end do
+    allocate(character(len=n) :: text)
+    text = transfer(msg(1:n), text)
+    write(*, "(A)") "a5g error in " // what // ": " // text
+    error stop 1
+  end subroutine check
+
+  integer(c_int) function lg(x)
+    logical, intent(in) :: x
+    lg = merge(1, 0, x)
+  end function lg
+
+  ! ------------------------------------------------------------------
+  ! tree (afivo/src/m_aX_core.f90)
+  ! ------------------------------------------------------------------
+
+  !> a2_init (m_aX_core.f90:47-116) for a 2D tree on the GPU
+  subroutine a2_init(tree, n_cell, n_var_cell, n_var_face, dr, coarsen_to, &
+       n_boxes, coord, lvls_max, r_min)
+    type(a5g_t), intent(inout)     :: tree
+    integer, intent(in)            :: n_cell, n_var_cell, n_var_face
+    real(dp), intent(in)           :: dr
+    integer, intent(in), optional  :: coarsen_to, n_boxes, coord, lvls_max
+    real(dp), intent(in), optional :: r_min(2)
+    call t_init(tree, 2, n_cell, n_var_cell, n_var_face, dr, coarsen_to, &
+         n_boxes, coord, lvls_max, r_min)
+  end subroutine a2_init
+
+  !> a3_init for a 3D tree on the GPU
+  subroutine a3_init(tree, n_cell, n_var_cell, n_var_face, dr, coarsen_to, &
+       n_boxes, lvls_max, r_min)
+    type(a5g_t), intent(inout)     :: tree
+    integer, intent(in)            :: n_cell, n_var_cell, n_var_face
+    real(dp), intent(in)           :: dr
+    integer, intent(in), optional  :: coarsen_to, n_boxes, lvls_max
+    real(dp), intent(in), optional :: r_min(3)
+    call t_init(tree, 3, n_cell, n_var_cell, n_var_face, dr, coarsen_to, &
+         n_boxes, a5_xyz, lvls_max, r_min)
+  end subroutine a3_init
+
+  subroutine t_init(tree, dim, n_cell, n_var_cell, n_var_face, dr, coarsen_to, &
+       n_boxes, coord, lvls_max, r_min)
+    type(a5g_t), intent(inout)     :: tree
+    integer, intent(in)            :: dim, n_cell, n_var_cell, n_var_face
+    real(dp), intent(in)           :: dr
+    integer, intent(in), optional  :: coarsen_to, n_boxes, coord, lvls_max
+    real(dp), intent(in), optional :: r_min(:)
+    integer  :: ct, nb, cs, lm
+    real(dp) :: rm(3)
+
+    ! defaults of a$D_init (m_aX_core.f90:47-66)
+    ct = -1; if (present(coarsen_to)) ct = coarsen_to
+    nb = 1000; if (present(n_boxes)) nb = n_boxes
+    cs = a5_xyz; if (present(coord)) cs = coord
+    lm = 30; if (present(lvls_max)) lm = lvls_max
+    rm = 0.0_dp; if (present(r_min)) rm(1:dim) = r_min(1:dim)
+    call check(c_init(tree%h, dim, n_cell, n_var_cell, n_var_face, dr, rm, ct, &
+         lm, cs, nb), "a$D_init")
+    tree%dim = dim
+    tree%n_cell = n_cell
+  end subroutine t_init
+
+  subroutine t_destroy(tree)
+    type(a5g_t), intent(inout) :: tree
+    call check(c_destroy(tree%h), "a$D_destroy")
+    tree%h = c_null_ptr
+  end subroutine t_destroy
+
+  !> a$D_set_base (m_aX_core.f90:137-220): ix_list(dim, n), nb_list(2 dim, n)
+  subroutine t_set_base(tree, ix_list, nb_list)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: ix_list(:, :), nb_list(:, :)
+    call check(c_set_base(tree%h, size(ix_list, 2), ix_list, nb_list), "a$D_set_base")
+  end subroutine t_set_base
+
+  !> a$D_adjust_refinement (m_aX_core.f90:473-537). ref_flags(id) holds what
+  !> the user's set_ref_flags would have set (a5_rm_ref / a5_kp_ref /
+  !> a5_do_ref) for boxes 1..size(ref_flags).
+  subroutine t_adjust_refinement(tree, ref_flags, ref_info)
+    type(a5g_t), intent(inout)          :: tree
+    integer, intent(in)                 :: ref_flags(:)
+    type(a5g_ref_info_t), intent(inout) :: ref_info
+    integer(c_int) :: n_add, n_rm, na, nr
+    integer        :: lvl, max_lvl
+    integer(c_int), allocatable, target :: add(:), rm(:)
+
+    call check(c_adjust_refinement(tree%h, size(ref_flags), ref_flags, n_add, n_rm), &
+         "a$D_adjust_refinement")
+    ref_info%n_add = n_add
+    ref_info%n_rm  = n_rm
+    max_lvl = a5g_max_lvl(tree)
+    if (allocated(ref_info%lvls)) deallocate(ref_info%lvls)
+    allocate(ref_info%lvls(max_lvl + 1))
+    do lvl = 1, max_lvl + 1
+       call check(c_ref_info_lvl(tree%h, lvl, c_null_ptr, c_null_ptr, na, nr), "ref_info")
+       allocate(add(max(na, 1)), rm(max(nr, 1)))
+       call check(c_ref_info_lvl(tree%h, lvl, c_loc(add), c_loc(rm), na, nr), "ref_info")
+       ref_info%lvls(lvl)%add = add(1:na)
+       ref_info%lvls(lvl)%rm  = rm(1:nr)
+       deallocate(add, rm)
+    end do
+  end subroutine t_adjust_refinement
+
+  !> a$D_tidy_up (m_aX_core.f90:224-325)
+  subroutine t_tidy_up(tree, max_frac, goal_frac, n_clean_min, reorder)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: max_frac, goal_frac
+    integer, intent(in)        :: n_clean_min
+    logical, intent(in)        :: reorder
+    call check(c_tidy_up(tree%h, max_frac, goal_frac, n_clean_min, lg(reorder)), &
+         "a$D_tidy_up")
+  end subroutine t_tidy_up
+
+  integer function a5g_max_id(tree)
+    type(a5g_t), intent(in) :: tree
+    integer(c_int) :: out8(8)
+    call check(c_info(tree%h, out8), "info")
+    a5g_max_id = out8(1)
+  end function a5g_max_id
+
+  integer function a5g_max_lvl(tree)
+    type(a5g_t), intent(in) :: tree
+    integer(c_int) :: out8(8)
+    call check(c_info(tree%h, out8), "info")
+    a5g_max_lvl = out8(2)
+  end function a5g_max_lvl
+
+  !> a$D_min_dr (m_aX_utils.f90:196-200) from the finest box in use
+  real(dp) function a5g_min_dr(tree)
+    type(a5g_t), intent(in) :: tree
+    integer, allocatable  :: lvl(:), in_use(:), n_cell(:)
+    real(dp), allocatable :: dr(:), r_min(:, :)
+    integer :: n
+    call a5g_box_info(tree, lvl, in_use, n_cell, dr, r_min)
+    n = size(lvl)
+    a5g_min_dr = minval(dr, mask=(in_use /= 0))
+  end function a5g_min_dr
+
+  !> Level, use flag, cell count, dr and r_min of boxes 1..max_id
+  subroutine a5g_box_info(tree, lvl, in_use, n_cell, dr, r_min)
+    type(a5g_t), intent(in)               :: tree
+    integer, allocatable, intent(out)     :: lvl(:), in_use(:), n_cell(:)
+    real(dp), allocatable, intent(out)    :: dr(:), r_min(:, :)
+    integer, allocatable :: ix(:, :), parent(:), children(:, :), nbs(:, :)
+    integer :: n, d
+
+    n = a5g_max_id(tree)
+    d = tree%dim
+    allocate(lvl(n), in_use(n), n_cell(n), dr(n), r_min(d, n))
+    allocate(ix(d, n), parent(n), children(2**d, n), nbs(2*d, n))
+    call check(c_get_boxes(tree%h, n, lvl, ix, parent, children, nbs, in_use, &
+         n_cell, dr, r_min), "get_boxes")
+  end subroutine a5g_box_info
+
+  !> tree%lvls(lvl)%leaves
+  subroutine a5g_level_leaves(tree, lvl, leaves)
+    type(a5g_t), intent(in)           :: tree
+    integer, intent(in)               :: lvl
+    integer, allocatable, intent(out) :: leaves(:)
+    integer(c_int) :: cnt(3)
+    integer, allocatable :: ids(:), parents(:)
+    call check(c_level_counts(tree%h, lvl, cnt), "level_counts")
+    allocate(ids(max(cnt(1), 1)), leaves(max(cnt(2), 1)), parents(max(cnt(3), 1)))
+    call check(c_level_lists(tree%h, lvl, ids, leaves, parents), "level_lists")
+    leaves = leaves(1:cnt(2))
+  end subroutine a5g_level_leaves
+
+  ! ------------------------------------------------------------------
+  ! cell data
+  ! ------------------------------------------------------------------
+
+  !> cc(:, :[, :], iv) of box id, ghosts included, in Afivo's order
+  subroutine a5g_set_cc(tree, id, iv, cc)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: id, iv
+    real(dp), intent(in)       :: cc(*)
+    call check(c_set_cc(tree%h, id, iv, cc), "set_cc")
+  end subroutine a5g_set_cc
+
+  subroutine a5g_get_cc(tree, id, iv, cc)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: id, iv
+    real(dp), intent(out)      :: cc(*)
+    call check(c_get_cc(tree%h, id, iv, cc), "get_cc")
+  end subroutine a5g_get_cc
+
+  subroutine a5g_fill_cc(tree, ids, iv, value)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: ids(:), iv
+    real(dp), intent(in)       :: value
+    call check(c_fill_cc(tree%h, size(ids), ids, iv, value), "fill_cc")
+  end subroutine a5g_fill_cc
+
+  subroutine a5g_sync(tree)
+    type(a5g_t), intent(inout) :: tree
+    call check(c_sync(tree%h), "sync")
+  end subroutine a5g_sync
+
+  ! ------------------------------------------------------------------
+  ! multigrid (afivo/src/m_aX_mg.f90)
+  ! ------------------------------------------------------------------
+
+  !> mg$D_init_mg (m_aX_mg.f90:130-154) with the options in mg
+  subroutine t_mg_init(tree, mg)
+    type(a5g_t), intent(inout)  :: tree
+    type(a5g_mg_t), intent(in)  :: mg
+    call check(c_mg_setup(tree%h, mg%i_phi, mg%i_tmp, mg%i_rhs, mg%i_eps, &
+         mg%n_cycle_down, mg%n_cycle_up, mg%n_cycle_base, mg%bc_kind, mg%bc_val, &
+         lg(mg%auto_ops)), "mg$D_init_mg")
+  end subroutine t_mg_init
+
+  !> New value of a face rule a5_bc_value (the drivers' applied voltage)
+  subroutine a5g_set_bc_value(tree, nb, value)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: nb
+    real(dp), intent(in)       :: value
+    call check(c_mg_set_bc_value(tree%h, nb, value), "mg_set_bc_value")
+  end subroutine a5g_set_bc_value
+
+  !> mg$D_fas_fmg (m_aX_mg.f90:164-203)
+  subroutine t_fas_fmg(tree, mg, set_residual, first_call)
+    type(a5g_t), intent(inout) :: tree
+    type(a5g_mg_t), intent(in) :: mg
+    logical, intent(in)        :: set_residual, first_call
+    call check(c_fas_fmg(tree%h, lg(set_residual), lg(first_call)), "mg$D_fas_fmg")
+  end subroutine t_fas_fmg
+
+  !> mg$D_fas_vcycle (m_aX_mg.f90:208-255)
+  subroutine t_fas_vcycle(tree, mg, max_lvl, set_residual)
+    type(a5g_t), intent(inout) :: tree
+    type(a5g_mg_t), intent(in) :: mg
+    integer, intent(in)        :: max_lvl
+    logical, intent(in)        :: set_residual
+    call check(c_fas_vcycle(tree%h, max_lvl, lg(set_residual)), "mg$D_fas_vcycle")
+  end subroutine t_fas_vcycle
+
+  !> FP32 multigrid inside FP64 defect correction, to max|r| <= tol max|rhs|
+  subroutine a5g_mixed_solve(tree, tol, max_outer, n_inner, zero_start, n_outer, rel_res)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: tol
+    integer, intent(in)        :: max_outer, n_inner
+    logical, intent(in)        :: zero_start
+    integer, intent(out)       :: n_outer
+    real(dp), intent(out)      :: rel_res
+    integer(c_int) :: no
+    call check(c_mg_solve_mixed(tree%h, tol, max_outer, n_inner, lg(zero_start), no, &
+         rel_res), "mg_solve_mixed")
+    n_outer = no
+  end subroutine a5g_mixed_solve
+
+  subroutine a5g_residual_norms(tree, max_res, max_rhs)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(out)      :: max_res, max_rhs
+    call check(c_mg_residual_norms(tree%h, max_res, max_rhs), "mg_residual_norms")
+  end subroutine a5g_residual_norms
+
+  ! ------------------------------------------------------------------
+  ! ghost cells, restriction, prolongation, copies, reductions
+  ! ------------------------------------------------------------------
+
+  !> a$D_gc_tree (m_aX_gc.f90:28-46); rb = a5_gc_sides_rb / _interp /
+  !> _interp_lim, bc = a5_gc_mg_bc / _neumann / _dirichlet
+  subroutine t_gc_tree(tree, iv, rb, bc)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv, rb, bc
+    call check(c_gc_tree(tree%h, iv, rb, bc), "a$D_gc_tree")
+  end subroutine t_gc_tree
+
+  !> a$D_gc_box (m_aX_gc.f90:50-68) for a list of boxes
+  subroutine t_gc_boxes(tree, ids, iv, rb, bc)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: ids(:), iv, rb, bc
+    call check(c_gc_boxes(tree%h, size(ids), ids, iv, rb, bc), "a$D_gc_box")
+  end subroutine t_gc_boxes
+
+  !> a$D_restrict_tree (m_aX_restrict.f90:54-64)
+  subroutine t_restrict_tree(tree, iv)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv
+    call check(c_restrict_tree(tree%h, iv), "a$D_restrict_tree")
+  end subroutine t_restrict_tree
+
+  !> a$D_tree_copy_cc (m_aX_utils.f90:445-453)
+  subroutine t_tree_copy_cc(tree, iv_from, iv_to)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv_from, iv_to
+    call check(c_tree_copy_cc(tree%h, iv_from, iv_to), "a$D_tree_copy_cc")
+  end subroutine t_tree_copy_cc
+
+  !> a$D_tree_max_cc (m_aX_utils.f90:591-619)
+  subroutine t_tree_max_cc(tree, iv, cc_max)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv
+    real(dp), intent(out)      :: cc_max
+    real(dp) :: lo
+    call check(c_tree_minmax_cc(tree%h, iv, lo, cc_max), "a$D_tree_max_cc")
+  end subroutine t_tree_max_cc
+
+  !> a$D_tree_min_cc (m_aX_utils.f90:559-587)
+  subroutine t_tree_min_cc(tree, iv, cc_min)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv
+    real(dp), intent(out)      :: cc_min
+    real(dp) :: hi
+    call check(c_tree_minmax_cc(tree%h, iv, cc_min, hi), "a$D_tree_min_cc")
+  end subroutine t_tree_min_cc
+
+  !> a$D_consistent_fluxes (m_aX_core.f90:818-930)
+  subroutine t_consistent_fluxes(tree, f_ixs)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: f_ixs(:)
+    call check(c_consistent_fluxes(tree%h, size(f_ixs), f_ixs), "a$D_consistent_fluxes")
+  end subroutine t_consistent_fluxes
+
+  !> a$D_prolong1_to (m_aX_prolong.f90:99-191) for a list of new boxes
+  subroutine t_prolong1_to(tree, ids, iv)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: ids(:), iv
+    call check(c_prolong1_to(tree%h, size(ids), ids, iv), "a$D_prolong1_to")
+  end subroutine t_prolong1_to
+
+  ! ------------------------------------------------------------------
+  ! streamer (src/m_streamer.f90, src/streamer_{2d,3d,cyl}.f90)
+  ! ------------------------------------------------------------------
+
+  !> Variable indices (src/m_streamer.f90:16-43) and the dielectric rule of
+  !> update_solution (2D/3D drivers: .true., cylindrical driver: .false.)
+  subroutine st_setup(tree, i_elec, i_pion, i_elec_old, i_pion_old, i_phi, i_fld, &
+       i_rhs, i_pho, i_eps, f_elec, f_fld, i_mobility, i_diffusion, i_alpha, &
+       i_eta, dielectric_rule)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in) :: i_elec, i_pion, i_elec_old, i_pion_old, i_phi, i_fld
+    integer, intent(in) :: i_rhs, i_pho, i_eps, f_elec, f_fld
+    integer, intent(in) :: i_mobility, i_diffusion, i_alpha, i_eta
+    logical, intent(in) :: dielectric_rule
+    integer(c_int) :: v(16)
+    v = [i_elec, i_pion, i_elec_old, i_pion_old, i_phi, i_fld, i_rhs, i_pho, i_eps, &
+         f_elec, f_fld, i_mobility, i_diffusion, i_alpha, i_eta, lg(dielectric_rule)]
+    call check(c_streamer_setup(tree%h, v), "streamer_setup")
+  end subroutine st_setup
+
+  !> LT_create (fosito/m_lookup_table.f90:46-58) of the transport table
+  subroutine st_table_create(tree, x_min, x_max, n_rows, n_cols)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: x_min, x_max
+    integer, intent(in)        :: n_rows, n_cols
+    call check(c_table_create(tree%h, x_min, x_max, n_rows, n_cols), "LT_create")
+  end subroutine st_table_create
+
+  !> LT_set_col (fosito/m_lookup_table.f90:61-72)
+  subroutine st_table_set_col(tree, col, xx, yy)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: col
+    real(dp), intent(in)       :: xx(:), yy(:)
+    call check(c_table_set_col(tree%h, col, size(xx), xx, yy), "LT_set_col")
+  end subroutine st_table_set_col
+
+  !> The table as built: rows_cols(n_rows, n_cols), x_min, inv_dx
+  subroutine st_table_get(tree, rows_cols, x_min, inv_dx)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(out)      :: rows_cols(:, :), x_min, inv_dx
+    call check(c_table_get(tree%h, rows_cols, x_min, inv_dx), "table_get")
+  end subroutine st_table_get
+
+  !> compute_fld's source term (src/streamer_2d.f90:318-329)
+  subroutine st_space_charge(tree, fac)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: fac
+    call check(c_space_charge(tree%h, fac), "space_charge")
+  end subroutine st_space_charge
+
+  !> fld_from_pot on every box (src/streamer_2d.f90:346-381)
+  subroutine st_fld_from_pot(tree)
+    type(a5g_t), intent(inout) :: tree
+    call check(c_fld_from_pot(tree%h), "fld_from_pot")
+  end subroutine st_fld_from_pot
+
+  !> fluxes_koren on the leaves (src/streamer_2d.f90:384-489)
+  subroutine st_fluxes_koren(tree, dt)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: dt
+    call check(c_fluxes_koren(tree%h, dt), "fluxes_koren")
+  end subroutine st_fluxes_koren
+
+  !> update_solution on the leaves (src/streamer_2d.f90:499-542)
+  subroutine st_update_solution(tree, dt)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: dt
+    call check(c_update_solution(tree%h, dt), "update_solution")
+  end subroutine st_update_solution
+
+  !> average_dens on every box (src/streamer_2d.f90:492-496)
+  subroutine st_average_dens(tree)
+    type(a5g_t), intent(inout) :: tree
+    call check(c_average_dens(tree%h), "average_dens")
+  end subroutine st_average_dens
+
+  !> Per-box maxima read by set_ref_flags (src/streamer_2d.f90:186-192):
+  !> summ(:, id) = (dr^2 max|rhs|, max fld, max elec), boxes of level >= 1
+  subroutine st_box_summaries(tree, summ)
+    type(a5g_t), intent(inout)         :: tree
+    real(dp), allocatable, intent(out) :: summ(:, :)
+    allocate(summ(3, a5g_max_id(tree)))
+    call check(c_box_summaries(tree%h, summ), "box_summaries")
+  end subroutine st_box_summaries
+
+end module m_a5g

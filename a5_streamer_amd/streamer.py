@@ -176,7 +176,7 @@ class Config:
         0..3e7 V/m (the reader's row limit, src/m_transport_data.f90:9)."""
         if self.transport is not None:
             return self.transport
-        x = np.linspace(0.0, self.table_max_fld, 500)
+        x = np.array([self.table_max_fld * k / 499 for k in range(500)])
         alpha = np.array([4.3e5 * math.exp(-2e7 / v) if v > 0 else 0.0 for v in x])
         return x, [np.full(500, 0.03), np.full(500, 0.2), alpha, np.zeros(500)]
 
