@@ -1,6 +1,8 @@
 // GPU box pool management and the C ABI of the a5g library (include/a5g.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
@@ -99,6 +101,7 @@ void Pool::ensure_capacity(int need) {
   realloc_dev(&d_children, (size_t)(ncap + 1) * 8);
   realloc_dev(&d_nbs, (size_t)(ncap + 1) * 6);
   realloc_dev(&d_ncell, ncap + 1);
+  realloc_dev(&d_tag, ncap + 1);
   realloc_dev(&d_dr, ncap + 1);
   realloc_dev(&d_rmin, (size_t)(ncap + 1) * 3);
   topo_dirty = true;
@@ -118,6 +121,7 @@ void Pool::sync_topology() {
   up(d_children, tree.children.data(), (size_t)n * 8 * sizeof(int));
   up(d_nbs, tree.nbs.data(), (size_t)n * 6 * sizeof(int));
   up(d_ncell, tree.ncell.data(), n * sizeof(int));
+  up(d_tag, tree.tag.data(), n * sizeof(int));
   up(d_dr, tree.dr.data(), n * sizeof(double));
   up(d_rmin, tree.rmin.data(), (size_t)n * 3 * sizeof(double));
   // level lists
@@ -198,6 +202,7 @@ void Pool::apply_moves(const std::vector<int>& moves, int n_used) {
     realloc_dev(&d_children, (size_t)(ncap + 1) * 8);
     realloc_dev(&d_nbs, (size_t)(ncap + 1) * 6);
     realloc_dev(&d_ncell, ncap + 1);
+    realloc_dev(&d_tag, ncap + 1);
     realloc_dev(&d_dr, ncap + 1);
     realloc_dev(&d_rmin, (size_t)(ncap + 1) * 3);
   }
@@ -227,7 +232,13 @@ void Pool::prof_end(int idx) {
   hcheck(hipEventRecord(prof[idx].b, stream), "hipEventRecord");
 }
 
+void Pool::clear_graphs() {
+  for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+}
+
 void Pool::release() {
+  clear_graphs();
   for (auto& r : prof) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
@@ -243,7 +254,7 @@ void Pool::release() {
   f(cc32);
   cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
-  f(d_dr); f(d_rmin); f(d_lists); f(d_red);
+  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red);
   if (h_red) (void)hipHostFree(h_red);
   if (stream) (void)hipStreamDestroy(stream);
   cc = fc = nullptr;
@@ -345,6 +356,9 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       h->P.host_only = true;
     } else {
       a5g::hcheck(hipStreamCreateWithFlags(&h->P.stream, hipStreamNonBlocking), "stream");
+      // A5G_GRAPHS=0 runs every multigrid call as plain launches
+      const char* g = std::getenv("A5G_GRAPHS");
+      h->P.graphs_on = !(g && g[0] == '0');
       a5g::hcheck(hipMalloc((void**)&h->P.d_red, 64), "hipMalloc");
       h->P.ensure_capacity(h->P.tree.cap);
     }

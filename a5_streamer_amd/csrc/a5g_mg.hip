@@ -54,6 +54,97 @@ static Geo geo(int n_boxes, int work) {
   const int id = list[li_];
 
 // ---------------------------------------------------------------------------
+// Laplacian of phi at interior cell c0: mg$D_box_lpl (m_aX_mg.f90:814-848),
+// mg2_box_clpl (:1442-1464)
+// ---------------------------------------------------------------------------
+
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T lpl_at(const T* p, int nc, int i, int j, int k, T inv_dr_sq,
+                                    int ioff) {
+  if (D == 2) {
+    if (CYL) {
+      const double den = i + ioff - 0.5;
+      const T rf1 = (T)((double)(i + ioff - 1) / den);
+      const T rf2 = (T)((double)(i + ioff) / den);
+      return (rf1 * p[cidx<D>(nc, i - 1, j, 0)] + rf2 * p[cidx<D>(nc, i + 1, j, 0)] +
+              p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
+              T(4) * p[cidx<D>(nc, i, j, 0)]) * inv_dr_sq;
+    }
+    return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, 0)] + p[cidx<D>(nc, i + 1, j, 0)] +
+                        p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
+                        T(4) * p[cidx<D>(nc, i, j, 0)]);
+  }
+  return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, k)] + p[cidx<D>(nc, i + 1, j, k)] +
+                      p[cidx<D>(nc, i, j - 1, k)] + p[cidx<D>(nc, i, j + 1, k)] +
+                      p[cidx<D>(nc, i, j, k - 1)] + p[cidx<D>(nc, i, j, k + 1)] -
+                      T(6) * p[cidx<D>(nc, i, j, k)]);
+}
+
+// Operator with a permittivity eps: mg$D_box_lpld (m_aX_mg.f90:951-1001)
+// and mg2_box_clpld (:1467-1495) at interior cell (i, j, k). Neighbours in the
+// order i-1, i+1, j-1, j+1, k-1, k+1 of the reference's u(:) and a(:).
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T lpld_at(const double* e, const T* p, int nc, int i, int j, int k,
+                                     T inv_dr_sq, int ioff) {
+  const int c0 = cidx<D>(nc, i, j, k);
+  const int nb[6] = {cidx<D>(nc, i - 1, j, k), cidx<D>(nc, i + 1, j, k),
+                     cidx<D>(nc, i, j - 1, k), cidx<D>(nc, i, j + 1, k),
+                     D == 3 ? cidx<D>(nc, i, j, k - 1) : 0, D == 3 ? cidx<D>(nc, i, j, k + 1) : 0};
+  const T u0 = p[c0], a0 = (T)e[c0];
+  T rf[2] = {T(1), T(1)};
+  if (CYL) {
+    const double den = i + ioff - 0.5;
+    rf[0] = (T)((double)(i + ioff - 1) / den);
+    rf[1] = (T)((double)(i + ioff) / den);
+  }
+  T s = 0;
+#pragma unroll
+  for (int m = 0; m < 2 * D; m++) {
+    const T am = (T)e[nb[m]];
+    if (CYL && m < 2) s = s + rf[m] * a0 * am / (a0 + am) * (p[nb[m]] - u0);
+    else s = s + a0 * am / (a0 + am) * (p[nb[m]] - u0);
+  }
+  return inv_dr_sq * T(2) * s;
+}
+
+// The multigrid operator of box id: the permittivity form for boxes tagged
+// constant- or variable-eps (mg$D_auto_op, m_aX_mg.f90:603-635), else the
+// Laplacian.
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T op_at(const Dev& a, int id, const T* p, int nc, int i, int j, int k,
+                                   T inv_dr_sq, int ioff) {
+  if (a.tag != nullptr && a.tag[id] > kTagNormal)
+    return lpld_at<D, CYL, T>(boxp(a, a.i_eps, id), p, nc, i, j, k, inv_dr_sq, ioff);
+  return lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff);
+}
+
+// One Gauss-Seidel update with a permittivity: mg$D_box_gsrb_lpld
+// (m_aX_mg.f90:895-948), mg2_box_gsrb_clpld (:1499-1531). u[m], am[m]:
+// potential and eps of the neighbours in the order i-1, i+1, j-1, j+1, k-1,
+// k+1.
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T gsrb_lpld(const T* u, const T* am, T a0, T dx2, T rhs, T rf1,
+                                       T rf2) {
+  T c[2 * D];
+#pragma unroll
+  for (int m = 0; m < 2 * D; m++) c[m] = T(2) * a0 * am[m] / (a0 + am[m]);
+  T s1 = 0, s2 = 0;
+  if (CYL) {
+    const T rf[4] = {rf1, rf2, T(1), T(1)};
+#pragma unroll
+    for (int m = 0; m < 2 * D; m++) s1 = s1 + c[m] * rf[m] * u[m];
+#pragma unroll
+    for (int m = 0; m < 2 * D; m++) s2 = s2 + c[m] * rf[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 2 * D; m++) s1 = s1 + c[m] * u[m];
+#pragma unroll
+    for (int m = 0; m < 2 * D; m++) s2 = s2 + c[m];
+  }
+  return (s1 - dx2 * rhs) / s2;
+}
+
+// ---------------------------------------------------------------------------
 // Gauss-Seidel red-black half-sweep
 // mg$D_box_gsrb_lpl (m_aX_mg.f90:769-811), mg2_box_gsrb_clpl (:1413-1439)
 // ---------------------------------------------------------------------------
@@ -84,7 +175,25 @@ __global__ __launch_bounds__(256) void k_gsrb(Dev a, const int* __restrict__ lis
       return p[cidx<D>(nc, ii, jj, kk)];
     };
     const int c0 = cidx<D>(nc, i, j, k);
-    if (D == 2) {
+    if (a.tag != nullptr && a.tag[id] > kTagNormal) {
+      const double* ev = boxp(a, a.i_eps, id);
+      T uu[6], am[6];
+      uu[0] = nbv(-1, 0, 0, 1); uu[1] = nbv(1, 0, 0, 2);
+      uu[2] = nbv(0, -1, 0, 3); uu[3] = nbv(0, 1, 0, 4);
+      am[0] = (T)ev[cidx<D>(nc, i - 1, j, k)]; am[1] = (T)ev[cidx<D>(nc, i + 1, j, k)];
+      am[2] = (T)ev[cidx<D>(nc, i, j - 1, k)]; am[3] = (T)ev[cidx<D>(nc, i, j + 1, k)];
+      if (D == 3) {
+        uu[4] = nbv(0, 0, -1, 5); uu[5] = nbv(0, 0, 1, 6);
+        am[4] = (T)ev[cidx<D>(nc, i, j, k - 1)]; am[5] = (T)ev[cidx<D>(nc, i, j, k + 1)];
+      }
+      T rf1 = T(1), rf2 = T(1);
+      if (CYL) {
+        const double den = i + ioff - 0.5;
+        rf1 = (T)((double)(i + ioff - 1) / den);
+        rf2 = (T)((double)(i + ioff) / den);
+      }
+      p[c0] = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c0], dx2, r[c0], rf1, rf2);
+    } else if (D == 2) {
       if (CYL) {
         const double den = i + ioff - 0.5;
         const T rf1 = (T)((double)(i + ioff - 1) / den);
@@ -115,11 +224,35 @@ __device__ __forceinline__ int xcd_block() {
   return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
+// Copies N values from global memory to LDS with TPB lanes, issuing up to
+// eight loads per lane before the first LDS store so that they are in flight
+// together.
+template <int N, int TPB, typename T>
+__device__ __forceinline__ void stage_lds(T* __restrict__ s, const T* __restrict__ g, int lt) {
+  constexpr int NIT = (N + TPB - 1) / TPB;
+  constexpr int B = NIT < 8 ? NIT : 8;
+#pragma unroll
+  for (int i0 = 0; i0 < NIT; i0 += B) {
+    T v[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int q = lt + (i0 + u) * TPB;
+      if (i0 + u < NIT && q < N) v[u] = g[q];
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int q = lt + (i0 + u) * TPB;
+      if (i0 + u < NIT && q < N) s[q] = v[u];
+    }
+  }
+}
+
 // Box-size-specialised half-sweep. The other colour's half of the box (ghost
 // slots included) is staged in LDS; when fused, that colour's face ghost
 // slots are first recomputed from the current state (the values the
 // reference's ghost pass after the previous half-sweep would store). Then
-// every lane updates cells of colour cntr&1 from LDS and rhs.
+// every lane updates cells of colour cntr&1 from LDS and rhs (loaded into
+// registers before the barrier).
 template <int D, bool CYL, int NC, typename T>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
@@ -132,6 +265,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int NGH = 2 * D * PER_FACE;
   constexpr int HNC = NC / 2;
+  constexpr int NIT = (NCOL + TPB - 1) / TPB;
   __shared__ T sh[BPB * HALF];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
   const int li = xcd_block() * BPB + slot;
@@ -140,8 +274,29 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   const int c = cntr & 1, oc = c ^ 1;
   T* s = sh + slot * HALF;
   T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
+  // lex index of the q-th cell of colour c
+  auto lex_of = [&](int q, int& i) {
+    const int mm = q % HNC, row = q / HNC;
+    const int j = row % NC + 1;
+    const int k = D == 3 ? row / NC + 1 : 0;
+    i = 2 * mm + 2 - ((c + j + k) & 1);
+    return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
+  };
+  T rv[NIT];
   if (act) {
-    for (int q = lt; q < HALF; q += TPB) s[q] = p[oc * HALF + q];
+    stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
+    const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
+#pragma unroll
+    for (int u = 0; u < NIT; u++) {
+      const int q = lt + u * TPB;
+      int i;
+      if (q < NCOL) rv[u] = r[lex_of(q, i) >> 1];
+    }
+  }
+  // the staged half includes the ghost slots: they are overwritten below only
+  // after every lane has stored its part
+  if (fused) __syncthreads();
+  if (act) {
     if (fused) {
       for (int q = lt; q < NGH; q += TPB) {
         const int nb = q / PER_FACE + 1;
@@ -160,33 +315,53 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   }
   __syncthreads();
   if (!act) return;
-  const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
   T* pc = p + c * HALF;
   const T dx2 = (T)(a.dr[id] * a.dr[id]);
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
-  for (int q = lt; q < NCOL; q += TPB) {
-    const int mm = q % HNC, row = q / HNC;
-    const int j = row % NC + 1;
-    const int k = D == 3 ? row / NC + 1 : 0;
-    const int i = 2 * mm + 2 - ((c + j + k) & 1);
-    const int lex = D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
+  // boxes tagged constant- or variable-eps (mg$D_auto_gsrb,
+  // m_aX_mg.f90:568-600); eps is read in its own (colour-split) order
+  const double* ev = (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
+#pragma unroll
+  for (int u = 0; u < NIT; u++) {
+    const int q = lt + u * TPB;
+    if (q >= NCOL) break;
+    int i;
+    const int lex = lex_of(q, i);
     const int x = lex >> 1;
     const T xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
     const T yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
     T v;
-    if (D == 2) {
+    if (ev != nullptr) {
+      // the neighbours have the other colour: second half of the box
+      const double* eo = ev + oc * HALF;
+      T uu[6], am[6];
+      uu[0] = xm; uu[1] = xp; uu[2] = ym; uu[3] = yp;
+      am[0] = (T)eo[(lex - 1) >> 1]; am[1] = (T)eo[(lex + 1) >> 1];
+      am[2] = (T)eo[(lex - N2) >> 1]; am[3] = (T)eo[(lex + N2) >> 1];
+      if (D == 3) {
+        uu[4] = s[(lex - N2 * N2) >> 1]; uu[5] = s[(lex + N2 * N2) >> 1];
+        am[4] = (T)eo[(lex - N2 * N2) >> 1]; am[5] = (T)eo[(lex + N2 * N2) >> 1];
+      }
+      T rf1 = T(1), rf2 = T(1);
+      if (CYL) {
+        const double den = i + ioff - 0.5;
+        rf1 = (T)((double)(i + ioff - 1) / den);
+        rf2 = (T)((double)(i + ioff) / den);
+      }
+      v = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c * HALF + x], dx2, rv[u], rf1, rf2);
+    } else if (D == 2) {
       if (CYL) {
         const double den = i + ioff - 0.5;
         const T rf1 = (T)((double)(i + ioff - 1) / den);
         const T rf2 = (T)((double)(i + ioff) / den);
-        v = T(0.25) * (rf1 * xm + rf2 * xp + yp + ym - dx2 * r[x]);
+        v = T(0.25) * (rf1 * xm + rf2 * xp + yp + ym - dx2 * rv[u]);
       } else {
-        v = T(0.25) * (xp + xm + yp + ym - dx2 * r[x]);
+        v = T(0.25) * (xp + xm + yp + ym - dx2 * rv[u]);
       }
     } else {
       const T zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
       const T sixth = T(1) / T(6);
-      v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * r[x]);
+      v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * rv[u]);
     }
     pc[x] = v;
   }
@@ -302,33 +477,6 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
 }
 
 // ---------------------------------------------------------------------------
-// Laplacian of phi at interior cell c0: mg$D_box_lpl (m_aX_mg.f90:814-848),
-// mg2_box_clpl (:1442-1464)
-// ---------------------------------------------------------------------------
-
-template <int D, bool CYL, typename T>
-__device__ __forceinline__ T lpl_at(const T* p, int nc, int i, int j, int k, T inv_dr_sq,
-                                    int ioff) {
-  if (D == 2) {
-    if (CYL) {
-      const double den = i + ioff - 0.5;
-      const T rf1 = (T)((double)(i + ioff - 1) / den);
-      const T rf2 = (T)((double)(i + ioff) / den);
-      return (rf1 * p[cidx<D>(nc, i - 1, j, 0)] + rf2 * p[cidx<D>(nc, i + 1, j, 0)] +
-              p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-              T(4) * p[cidx<D>(nc, i, j, 0)]) * inv_dr_sq;
-    }
-    return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, 0)] + p[cidx<D>(nc, i + 1, j, 0)] +
-                        p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-                        T(4) * p[cidx<D>(nc, i, j, 0)]);
-  }
-  return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, k)] + p[cidx<D>(nc, i + 1, j, k)] +
-                      p[cidx<D>(nc, i, j - 1, k)] + p[cidx<D>(nc, i, j + 1, k)] +
-                      p[cidx<D>(nc, i, j, k - 1)] + p[cidx<D>(nc, i, j, k + 1)] -
-                      T(6) * p[cidx<D>(nc, i, j, k)]);
-}
-
-// ---------------------------------------------------------------------------
 // Residual + restriction to the parent (update_coarse / set_coarse_phi_rhs,
 // m_aX_mg.f90:430-523, with residual_box :551-565 and mg$D_box_rstr_lpl
 // :851-893). One lane per parent cell covered by the child: the 2^D fine
@@ -368,10 +516,87 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
         for (int di = 0; di < 2; di++) {
           const int ii = fi + di, jj = fj + dj, kk = fk + dk;
           const int c = cidx<D>(nc, ii, jj, kk);
-          const T res = r[c] - lpl_at<D, CYL, T>(p, nc, ii, jj, kk, inv_dr_sq, ioff);
+          const T res = r[c] - op_at<D, CYL, T>(a, id, p, nc, ii, jj, kk, inv_dr_sq, ioff);
           if (keep_res) t[c] = res;
           sr = sr + res;
           sp = sp + p[c];
+        }
+    const int pc = cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
+    const T w = D == 2 ? T(0.25) : T(0.125);
+    pt[pc] = w * sr;
+    pp[pc] = w * sp;
+  }
+}
+
+// The same pass with the child's potential staged in LDS and its face ghost
+// slots recomputed there from the current state (the values the ghost pass
+// that ends the reference's smoothing, fill_gc_phi after gsrb_boxes, would
+// store: nothing changes the child, its neighbours or the uncovered parent
+// cells in between), so that pass is not needed before this one.
+template <int D, bool CYL, int NC, typename T>
+__global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __restrict__ list,
+                                                        int n, int i_phi, int i_rhs, int i_tmp,
+                                                        int keep_res) {
+  constexpr int N2 = NC + 2;
+  constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
+  constexpr int HNC = NC / 2;
+  constexpr int TOTAL = D == 2 ? HNC * HNC : HNC * HNC * HNC;
+  constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int BPB = 256 / TPB;
+  constexpr int PER_FACE = D == 2 ? NC : NC * NC;
+  constexpr int NGH = 2 * D * PER_FACE;
+  __shared__ T sh[BPB * BL];
+  const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
+  const int li = xcd_block() * BPB + slot;
+  const bool act = li < n;
+  const int id = act ? list[li] : 0;
+  T* s = sh + slot * BL;
+  if (act) stage_lds<BL, TPB>(s, boxp_t<T>(a, i_phi, id), lt);
+  __syncthreads();
+  if (act) {
+    for (int q = lt; q < NGH; q += TPB) {
+      const int nb = q / PER_FACE + 1;
+      const int t = q - (nb - 1) * PER_FACE;
+      const int d = d_nb_dim(nb);
+      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int g[3] = {0, 0, 0};
+      g[d] = d_nb_low(nb) ? 0 : NC + 1;
+      g[ta] = t % NC + 1;
+      if (D == 3) g[tb] = t / NC + 1;
+      // reads interior cells of s only, writes a ghost slot
+      s[cidx<D>(NC, g[0], g[1], g[2])] = ghost_mg<D, T>(a, s, i_phi, id, NC, nb, g[0], g[1], g[2]);
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  const int pid = a.parent[id];
+  const int pnc = a.ncell[pid];
+  int off[3];
+  child_off<D>(a, id, NC, 0, off);
+  const T* r = boxp_t<T>(a, i_rhs, id);
+  T* tv = boxp_t<T>(a, i_tmp, id);
+  T* pp = boxp_t<T>(a, i_phi, pid);
+  T* pt = boxp_t<T>(a, i_tmp, pid);
+  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
+  for (int q = lt; q < TOTAL; q += TPB) {
+    const int ci = q % HNC + 1;
+    const int cj = (q / HNC) % HNC + 1;
+    const int ck = D == 3 ? q / (HNC * HNC) + 1 : 0;
+    const int fi = 2 * ci - 1, fj = 2 * cj - 1, fk = D == 3 ? 2 * ck - 1 : 0;
+    T sr = 0, sp = 0;
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); dk++)
+#pragma unroll
+      for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+        for (int di = 0; di < 2; di++) {
+          const int ii = fi + di, jj = fj + dj, kk = fk + dk;
+          const int c = cidx<D>(NC, ii, jj, kk);
+          const T res = r[c] - op_at<D, CYL, T>(a, id, s, NC, ii, jj, kk, inv_dr_sq, ioff);
+          if (keep_res) tv[c] = res;
+          sr = sr + res;
+          sp = sp + s[c];
         }
     const int pc = cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
     const T w = D == 2 ? T(0.25) : T(0.125);
@@ -487,7 +712,7 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
     const int x = cidx<D>(nc, i, j, k);
     const bool inner = i >= 1 && i <= nc && j >= 1 && j <= nc &&
                        (D == 2 || (k >= 1 && k <= nc));
-    T rv = inner ? lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
+    T rv = inner ? op_at<D, CYL, T>(a, id, p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
     rv = rv + t[x];
     r[x] = rv;
     if (copy_tmp) t[x] = p[x];
@@ -518,13 +743,41 @@ __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ lis
   child_off<D>(a, id, nc, 0, off);
   T* f = boxp_t<T>(a, i_phi, id);
   const T* c = boxp_t<T>(a, i_tmp, pid);
+  const double* ep = (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, pid) : nullptr;
   const int total = D == 2 ? nc * nc : nc * nc * nc;
   for (int q = lt; q < total; q += tpb) {
     const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
     const int i1 = off[0] + ((i + 1) >> 1), i2 = i1 + 1 - 2 * (i & 1);
     const int j1 = off[1] + ((j + 1) >> 1), j2 = j1 + 1 - 2 * (j & 1);
     const int x = cidx<D>(nc, i, j, k);
-    if (D == 2) {
+    if (ep != nullptr) {
+      // mg$D_box_corr_lpld, m_aX_mg.f90:1003-1071 (parent's eps)
+      if (D == 2) {
+        const int q0 = cidx<D>(pnc, i1, j1, 0);
+        const int qa[2] = {cidx<D>(pnc, i2, j1, 0), cidx<D>(pnc, i1, j2, 0)};
+        const T u0 = c[q0], a0 = (T)ep[q0];
+        T sm = 0;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+          const T am = (T)ep[qa[m]];
+          sm = sm + (a0 * u0 + am * c[qa[m]]) / (a0 + am);
+        }
+        f[x] = f[x] + T(0.5) * sm;
+      } else {
+        const int k1 = off[2] + ((k + 1) >> 1), k2 = k1 + 1 - 2 * (k & 1);
+        const int q0 = cidx<D>(pnc, i1, j1, k1);
+        const int qa[3] = {cidx<D>(pnc, i2, j1, k1), cidx<D>(pnc, i1, j2, k1),
+                           cidx<D>(pnc, i1, j1, k2)};
+        const T u0 = c[q0], a0 = (T)ep[q0];
+        T sm = 0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const T am = (T)ep[qa[m]];
+          sm = sm + (a0 * u0 + am * (T(1.5) * c[qa[m]] - T(0.5) * u0)) / (a0 + am);
+        }
+        f[x] = f[x] + (T(1) / T(3)) * sm;
+      }
+    } else if (D == 2) {
       f[x] = f[x] + T(0.5) * c[cidx<D>(pnc, i1, j1, 0)] +
              T(0.25) * (c[cidx<D>(pnc, i2, j1, 0)] + c[cidx<D>(pnc, i1, j2, 0)]);
     } else {
@@ -561,7 +814,7 @@ __global__ __launch_bounds__(256) void k_residual(Dev a, const int* __restrict__
   for (int q = lt; q < total; q += tpb) {
     const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
     const int x = cidx<D>(nc, i, j, k);
-    t[x] = r[x] - lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff);
+    t[x] = r[x] - op_at<D, CYL, T>(a, id, p, nc, i, j, k, inv_dr_sq, ioff);
   }
 }
 
@@ -625,6 +878,40 @@ __global__ __launch_bounds__(256) void k_minmax(Dev a, const int* __restrict__ l
   }
 }
 
+// Range of one variable over whole boxes (ghosts included): out[2 li] = min,
+// out[2 li + 1] = max for box list[li]. One workgroup per box.
+template <int D>
+__global__ __launch_bounds__(256) void k_box_range(Dev a, const int* __restrict__ list, int n,
+                                                   int iv, double* out) {
+  const int li = blockIdx.x;
+  if (li >= n) return;
+  const int id = list[li];
+  const double* p = boxp(a, iv, id);
+  const int m = a.ncell[id] + 2;
+  const int total = D == 2 ? m * m : m * m * m;
+  double lo = p[0], hi = p[0];
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    lo = fmin(lo, p[q]);
+    hi = fmax(hi, p[q]);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  __shared__ double wl[4], wh[4];
+  if ((threadIdx.x & 63) == 0) {
+    wl[threadIdx.x >> 6] = lo;
+    wh[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = fmin(lo, wl[w]);
+      hi = fmax(hi, wh[w]);
+    }
+    out[2 * li] = lo;
+    out[2 * li + 1] = hi;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -643,17 +930,23 @@ struct Ctx {
   int i_phi, i_tmp, i_rhs;
   bool homogeneous;
   bool cyl;  // cylindrical operators (2D, automatic operators installed)
+  bool eps;  // per-box operator tags (automatic operators and an eps variable)
 };
 
 static Ctx ctx64(const Pool& P) {
   return {P.mg.i_phi, P.mg.i_tmp, P.mg.i_rhs, false,
-          P.tree.dim == 2 && P.tree.coord == 1 && P.mg.auto_ops != 0};
+          P.tree.dim == 2 && P.tree.coord == 1 && P.mg.auto_ops != 0,
+          P.mg.auto_ops != 0 && P.mg.i_eps > 0};
 }
 
 static Dev dev_for(Pool& P, const Ctx& c) {
   Dev a = P.dev();
   if (c.homogeneous)
     for (int i = 0; i < 6; i++) a.bc_val[i] = 0.0;
+  if (c.eps) {
+    a.tag = P.d_tag;
+    a.i_eps = P.mg.i_eps;
+  }
   return a;
 }
 
@@ -691,9 +984,15 @@ static void fill_gc_phi(Pool& P, const Ctx& c, int lvl) {
   gc_list_t<D, T>(P, dev_for(P, c), lvl, P.ids(lvl), P.n_ids(lvl), c.i_phi, 0, 0);
 }
 
+// gsrb_boxes, m_aX_mg.f90:403-426: 2 n_cycle half-sweeps, each followed by
+// the ghost pass. Ghosts are recomputed inside every half-sweep after the
+// first; the first reads the stored ghosts unless first_fused (the caller
+// skipped a ghost pass that would have stored exactly what is recomputed).
+// store = false leaves the stored ghost layer as it was, for callers whose
+// next reader of these ghosts recomputes them (the residual pass).
 template <int D, typename T>
-static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle) {
-  // gsrb_boxes, m_aX_mg.f90:403-426
+static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_fused = false,
+                       bool store = true) {
   const int n = P.n_ids(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
@@ -702,7 +1001,7 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle) {
   Dev a = dev_for(P, c);
   const double cells = (double)n * cells_of(D, nc, false);
   for (int it = 1; it <= 2 * n_cycle; it++) {
-    const int fused = it > 1;
+    const int fused = it > 1 || first_fused;
     // algorithmic bytes of a half-sweep: 1.5 words per cell
     const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * sizeof(T) * cells);
     const bool done =
@@ -719,9 +1018,38 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle) {
     post_launch("gsrb");
     P.prof_end(pr);
   }
-  fill_gc_phi<D, T>(P, c, lvl);
+  if (store || n_cycle == 0) fill_gc_phi<D, T>(P, c, lvl);
 }
 
+template <int D, bool CYL, typename T>
+static bool launch_resid_rstr_lds(Pool& P, const Dev& a, const int* list, int n, int nc,
+                                  int i_phi, int i_rhs, int i_tmp, int keep_res) {
+  auto go = [&](auto kern, int total) {
+    const int tpb = total > 128 ? 256 : (total > 64 ? 128 : 64);
+    const int bpb = 256 / tpb;
+    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
+                       i_phi, i_rhs, i_tmp, keep_res);
+  };
+  const int hnc = nc / 2;
+  const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
+  switch (nc) {
+    case 2: go(k_resid_rstr_lds<D, CYL, 2, T>, total); return true;
+    case 4: go(k_resid_rstr_lds<D, CYL, 4, T>, total); return true;
+    case 8: go(k_resid_rstr_lds<D, CYL, 8, T>, total); return true;
+    case 16: go(k_resid_rstr_lds<D, CYL, 16, T>, total); return true;
+    case 32:
+      if (D == 2) {
+        go(k_resid_rstr_lds<2, CYL, 32, T>, total);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
+// Residual + restriction of level lvl into its parents. The face ghosts of
+// phi on lvl are recomputed inside the pass, so callers need not have
+// stored them since the last change of phi on lvl.
 template <int D, typename T>
 static void resid_rstr(Pool& P, const Ctx& c, int lvl, int keep_res) {
   const int n = P.n_ids(lvl);
@@ -731,13 +1059,23 @@ static void resid_rstr(Pool& P, const Ctx& c, int lvl, int keep_res) {
   Geo g = geo(n, D == 2 ? hnc * hnc : hnc * hnc * hnc);
   Dev a = dev_for(P, c);
   const double cells = (double)n * cells_of(D, nc, false);
-  const int pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * sizeof(T) * cells);
-  if (c.cyl)
-    hipLaunchKernelGGL((k_resid_rstr<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
-  else
-    hipLaunchKernelGGL((k_resid_rstr<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
+  int pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * sizeof(T) * cells);
+  const bool done =
+      c.cyl ? launch_resid_rstr_lds<2, true, T>(P, a, P.ids(lvl), n, nc, c.i_phi, c.i_rhs,
+                                                c.i_tmp, keep_res)
+            : launch_resid_rstr_lds<D, false, T>(P, a, P.ids(lvl), n, nc, c.i_phi, c.i_rhs,
+                                                 c.i_tmp, keep_res);
+  if (!done) {
+    P.prof_end(pr);
+    fill_gc_phi<D, T>(P, c, lvl);
+    pr = P.prof_begin(PK_RESID_RSTR, lvl, (2.0 + 2.0 / (1 << D)) * sizeof(T) * cells);
+    if (c.cyl)
+      hipLaunchKernelGGL((k_resid_rstr<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
+    else
+      hipLaunchKernelGGL((k_resid_rstr<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, keep_res);
+  }
   post_launch("resid_rstr");
   P.prof_end(pr);
 }
@@ -855,16 +1193,20 @@ static void init_phi_rhs(Pool& P, const Ctx& c) {
 template <int D, typename T>
 static void fas_vcycle(Pool& P, const Ctx& c, int max_lvl, bool set_residual) {
   // mg$D_fas_vcycle, m_aX_mg.f90:208-255
+  // Two ghost passes of the reference are folded into their readers: the one
+  // after the down-smoothing (the residual pass recomputes those ghosts) and
+  // the one after the correction (the first up-smoothing half-sweep does).
+  // Until the up-smoothing stores them again, the stored ghosts of a level
+  // are read by nobody.
   const int min_lvl = P.tree.min_lvl;
   for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
-    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_down);
+    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_down, false, false);
     update_coarse<D, T>(P, c, lvl);
   }
   gsrb_boxes<D, T>(P, c, min_lvl, P.mg.n_base);
   for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
     correct_children<D, T>(P, c, lvl - 1);
-    fill_gc_phi<D, T>(P, c, lvl);
-    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_up);
+    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_up, true, true);
   }
   if (set_residual)
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) residual_level<D, T>(P, c, lvl, false);
@@ -887,26 +1229,146 @@ static void fas_fmg(Pool& P, const Ctx& c, bool set_residual, bool first_call) {
   }
 }
 
+// mg$D_set_box_tag (m_aX_mg.f90:673-712) for the boxes that meet the
+// multigrid for the first time since they were created: variable-eps when eps
+// varies over the box (ghosts included), constant-eps when it is one value
+// other than 1, else normal. The multigrid never writes eps, so tagging every
+// such box at the start of a call gives the tags the reference sets at the
+// box's first operator call.
+static void ensure_tags(Pool& P) {
+  auto& T = P.tree;
+  std::vector<int> todo;
+  for (int id = 1; id <= T.max_id; id++)
+    if (T.in_use[id] && T.tag[id] == kTagInit) todo.push_back(id);
+  if (todo.empty()) return;
+  const int n = (int)todo.size();
+  int* d_list = nullptr;
+  double* d_out = nullptr;
+  check(hipMalloc((void**)&d_list, n * sizeof(int)), "hipMalloc tags");
+  check(hipMalloc((void**)&d_out, 2 * (size_t)n * sizeof(double)), "hipMalloc tags");
+  std::vector<double> out(2 * (size_t)n);
+  check(hipMemcpyAsync(d_list, todo.data(), n * sizeof(int), hipMemcpyHostToDevice, P.stream),
+        "tag list");
+  if (T.dim == 2)
+    hipLaunchKernelGGL(k_box_range<2>, dim3(n), dim3(256), 0, P.stream, P.dev(), d_list, n,
+                       P.mg.i_eps, d_out);
+  else
+    hipLaunchKernelGGL(k_box_range<3>, dim3(n), dim3(256), 0, P.stream, P.dev(), d_list, n,
+                       P.mg.i_eps, d_out);
+  post_launch("box_range");
+  check(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       P.stream), "tag range");
+  check(hipStreamSynchronize(P.stream), "tag sync");
+  check(hipFree(d_list), "hipFree");
+  check(hipFree(d_out), "hipFree");
+  for (int q = 0; q < n; q++) {
+    const double lo = out[2 * q], hi = out[2 * q + 1];
+    int tag = kTagNormal;
+    if (hi > lo) tag = kTagVeps;
+    else if (lo < 1 || lo > 1) tag = kTagCeps;
+    T.tag[todo[q]] = tag;
+  }
+  check(hipMemcpyAsync(P.d_tag, T.tag.data(), (T.max_id + 1) * sizeof(int),
+                       hipMemcpyHostToDevice, P.stream), "tag upload");
+}
+
+// ---------------------------------------------------------------------------
+// HIP graphs of whole multigrid calls. A call sequence (FMG, V-cycle, the
+// FP32 correction of the mixed solve) is captured on its first use and
+// replayed afterwards. Its kernel arguments are fixed at capture, so the
+// cache is dropped whenever the fingerprint of what they derive from changes:
+// the pool's device pointers and strides, the face rules, the level lists and
+// their offsets, and the multigrid options. Timed runs (events on) and
+// A5G_GRAPHS=0 use plain launches.
+// ---------------------------------------------------------------------------
+
+static uint64_t fnv(uint64_t h, uint64_t v) {
+  for (int b = 0; b < 8; b++) {
+    h ^= (v >> (8 * b)) & 0xff;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+static uint64_t state_fingerprint(Pool& P) {
+  uint64_t h = 1469598103934665603ull;
+  auto ptr = [&](const void* q) { h = fnv(h, (uint64_t)(uintptr_t)q); };
+  auto num = [&](uint64_t v) { h = fnv(h, v); };
+  ptr(P.cc); ptr(P.fc); ptr(P.cc32); num(P.n32); num(P.cap32); num(P.dev_cap);
+  ptr(P.d_lvl); ptr(P.d_ix); ptr(P.d_parent); ptr(P.d_children); ptr(P.d_nbs);
+  ptr(P.d_ncell); ptr(P.d_tag); ptr(P.d_dr); ptr(P.d_rmin); ptr(P.d_lists);
+  num(P.tree.coord);
+  for (int i = 0; i < 6; i++) {
+    num(P.bc_kind[i]);
+    uint64_t b;
+    std::memcpy(&b, &P.bc_val[i], 8);
+    num(b);
+  }
+  num(P.mg.i_phi); num(P.mg.i_tmp); num(P.mg.i_rhs); num(P.mg.i_eps);
+  num(P.mg.n_down); num(P.mg.n_up); num(P.mg.n_base); num(P.mg.auto_ops);
+  num((uint64_t)(int64_t)P.tree.min_lvl); num(P.tree.max_lvl);
+  for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++) {
+    const int k = l - P.tree.min_lvl;
+    num(P.off_ids[k]); num(P.off_leaves[k]); num(P.off_parents[k]);
+    num(P.n_ids(l)); num(P.n_leaves(l)); num(P.n_parents(l));
+  }
+  return h;
+}
+
+template <class F>
+static void run_captured(Pool& P, uint64_t key, F&& f) {
+  if (!P.graphs_on || P.prof_on) {
+    f();
+    return;
+  }
+  const uint64_t st = state_fingerprint(P);
+  if (st != P.graphs_state) {
+    P.clear_graphs();
+    P.graphs_state = st;
+  }
+  for (auto& g : P.graphs)
+    if (g.first == key) {
+      check(hipGraphLaunch(g.second, P.stream), "graph launch");
+      return;
+    }
+  check(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    f();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(P.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  check(hipStreamEndCapture(P.stream, &g), "end capture");
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  check(e, "graph instantiate");
+  P.graphs.push_back({key, ex});
+  check(hipGraphLaunch(ex, P.stream), "graph launch");
+}
+
+static uint64_t call_key(int kind, int a0, int a1, int a2) {
+  return ((uint64_t)kind << 48) | ((uint64_t)(a0 & 0xffff) << 32) |
+         ((uint64_t)(a1 & 0xffff) << 16) | (uint64_t)(a2 & 0xffff);
+}
+
 static void mg_check(Pool& P) {
   // check_mg, m_aX_mg.f90:156-159
   if (!P.mg.ready) throw std::logic_error("check_mg: you haven't called mg_init_mg");
   P.sync_topology();
-  if (P.mg.auto_ops && P.mg.i_eps > 0) {
-    // mg$D_set_box_tag (m_aX_mg.f90:673-712) gives every box the plain
-    // Laplacian when eps is 1 everywhere; boxes with eps /= 1 (the lpld
-    // operator family) are not implemented on the GPU.
-    double lo, hi;
-    minmax_boxes(P, P.mg.i_eps, false, true, false, &lo, &hi);
-    if (lo != 1.0 || hi != 1.0)
-      throw std::runtime_error("multigrid: boxes with permittivity /= 1 are not supported");
-  }
+  if (P.mg.auto_ops && P.mg.i_eps > 0) ensure_tags(P);
 }
 
 void mg_fmg(Pool& P, bool set_residual, bool first_call) {
   mg_check(P);
   const Ctx c = ctx64(P);
-  if (P.tree.dim == 2) fas_fmg<2, double>(P, c, set_residual, first_call);
-  else fas_fmg<3, double>(P, c, set_residual, first_call);
+  run_captured(P, call_key(1, set_residual, first_call, 0), [&] {
+    if (P.tree.dim == 2) fas_fmg<2, double>(P, c, set_residual, first_call);
+    else fas_fmg<3, double>(P, c, set_residual, first_call);
+  });
 }
 
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual) {
@@ -914,8 +1376,10 @@ void mg_vcycle(Pool& P, int max_lvl, bool set_residual) {
   if (max_lvl < P.tree.min_lvl || max_lvl > P.tree.max_lvl)
     throw std::invalid_argument("fas_vcycle: max_lvl outside the tree");
   const Ctx c = ctx64(P);
-  if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, max_lvl, set_residual);
-  else fas_vcycle<3, double>(P, c, max_lvl, set_residual);
+  run_captured(P, call_key(2, max_lvl - P.tree.min_lvl, set_residual, 0), [&] {
+    if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, max_lvl, set_residual);
+    else fas_vcycle<3, double>(P, c, max_lvl, set_residual);
+  });
 }
 
 void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
@@ -1066,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_resid_to32(Dev a, const int* __restrict
     for (int q = lt; q < total; q += tpb) {
       const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
       const int x = cidx<D>(nc, i, j, k);
-      const double res = r[x] - lpl_at<D, CYL, double>(p, nc, i, j, k, inv_dr_sq, ioff);
+      const double res = r[x] - op_at<D, CYL, double>(a, id, p, nc, i, j, k, inv_dr_sq, ioff);
       o[x] = (float)res;
       m = fmax(m, fabs(res));
     }
@@ -1122,12 +1586,13 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
                         double* rel_res) {
   const Ctx c64 = ctx64(P);
   // FP32 variables: 1 = correction e, 2 = tmp, 3 = rhs of the correction
-  const Ctx c32{1, 2, 3, true, c64.cyl};
+  const Ctx c32{1, 2, 3, true, c64.cyl, c64.eps};
   ensure32(P, 3);
   const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
   double rhs_lo, rhs_max;
   minmax_boxes(P, c64.i_rhs, true, false, true, &rhs_lo, &rhs_max);
-  Dev a = P.dev();
+  // the FP64 residual takes each box's operator (permittivity tags)
+  Dev a = dev_for(P, c64);
   int it = 0;
   double rel = 0;
   for (;; it++) {
@@ -1159,36 +1624,45 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
     std::memcpy(&rmax, &u, 8);
     rel = rhs_max > 0 ? rmax / rhs_max : rmax;
     if (rel <= tol || it >= max_outer) break;
-    // e = 0 everywhere, then FMG (+ V-cycles) on A e = r with zero boundary data
-    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
-      const int n = P.n_ids(lvl);
-      if (n == 0) continue;
-      const int bl = cells_of(D, P.nc_lvl(lvl), true);
-      Geo g = geo(n, bl);
-      hipLaunchKernelGGL(k_zero32, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, bl,
-                         g.tpb, c32.i_phi);
-      post_launch("zero32");
-    }
-    fas_fmg<D, float>(P, c32, false, true);
-    for (int v = 1; v < n_inner; v++) fas_vcycle<D, float>(P, c32, max_lvl, false);
-    // phi += e (FP64) on the leaves; parents then take the restriction of
-    // the new leaf values, which is where the FAS iteration leaves them at
+    // e = 0 everywhere, then FMG (+ V-cycles) on A e = r with zero boundary
+    // data; phi += e (FP64) on the leaves; parents then take the restriction
+    // of the new leaf values, which is where the FAS iteration leaves them at
     // convergence (update_coarse, m_aX_mg.f90:430-486), so that the
     // refinement-boundary ghosts of the next residual see the same composite
     // problem as the FP64 solve. Then the FP64 ghost layer, coarse to fine.
-    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
-      const int n = P.n_leaves(lvl);
-      if (n == 0) continue;
-      const int nc = P.nc_lvl(lvl);
-      Geo g = geo(n, cells_of(D, nc, false));
-      const int pr = P.prof_begin(PK_OTHER, lvl, (2.0 * 8 + 4) * n * cells_of(D, nc, false));
-      hipLaunchKernelGGL((k_add32<D>), dim3(g.grid), dim3(256), 0, P.stream, a, P.leaves(lvl), n,
-                         nc, g.tpb, c64.i_phi, c32.i_phi);
-      post_launch("add32");
-      P.prof_end(pr);
-    }
-    restrict_levels(P, c64.i_phi, true);
-    for (int lvl = min_lvl; lvl <= max_lvl; lvl++) fill_gc_phi<D, double>(P, c64, lvl);
+    const bool first = it == 0;
+    run_captured(P, call_key(3, first, n_inner, max_lvl - min_lvl), [&] {
+      for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
+        const int n = P.n_ids(lvl);
+        if (n == 0) continue;
+        const int bl = cells_of(D, P.nc_lvl(lvl), true);
+        Geo g = geo(n, bl);
+        hipLaunchKernelGGL(k_zero32, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n,
+                           bl, g.tpb, c32.i_phi);
+        post_launch("zero32");
+      }
+      // the first correction is the whole solution: FMG from the coarse
+      // levels; later ones start from e = 0 with V-cycles
+      if (first) {
+        fas_fmg<D, float>(P, c32, false, true);
+        for (int v = 1; v < n_inner; v++) fas_vcycle<D, float>(P, c32, max_lvl, false);
+      } else {
+        for (int v = 0; v < n_inner; v++) fas_vcycle<D, float>(P, c32, max_lvl, false);
+      }
+      for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
+        const int n = P.n_leaves(lvl);
+        if (n == 0) continue;
+        const int nc = P.nc_lvl(lvl);
+        Geo g = geo(n, cells_of(D, nc, false));
+        const int pr = P.prof_begin(PK_OTHER, lvl, (2.0 * 8 + 4) * n * cells_of(D, nc, false));
+        hipLaunchKernelGGL((k_add32<D>), dim3(g.grid), dim3(256), 0, P.stream, a, P.leaves(lvl),
+                           n, nc, g.tpb, c64.i_phi, c32.i_phi);
+        post_launch("add32");
+        P.prof_end(pr);
+      }
+      restrict_levels(P, c64.i_phi, true);
+      for (int lvl = min_lvl; lvl <= max_lvl; lvl++) fill_gc_phi<D, double>(P, c64, lvl);
+    });
   }
   *outer_done = it;
   *rel_res = rel;

@@ -32,6 +32,10 @@ struct Dev {
   const double* rmin = nullptr;   // [id*3 + d]
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
   double bc_val[6] = {0, 0, 0, 0, 0, 0};
+  // multigrid operator class per box (kTag*), set for the multigrid passes
+  // when the automatic operators and a permittivity variable are in use
+  const int* tag = nullptr;
+  int i_eps = -1;
 };
 
 // Per-launch timing with HIP events on the pool's stream (kernel classes
@@ -77,7 +81,7 @@ struct Pool {
   int n32 = 0, cap32 = 0;
   // device topology
   int *d_lvl = nullptr, *d_ix = nullptr, *d_parent = nullptr, *d_children = nullptr;
-  int *d_nbs = nullptr, *d_ncell = nullptr;
+  int *d_nbs = nullptr, *d_ncell = nullptr, *d_tag = nullptr;
   double *d_dr = nullptr, *d_rmin = nullptr;
   int* d_lists = nullptr;  // concatenated level lists
   size_t lists_cap = 0;
@@ -101,6 +105,13 @@ struct Pool {
   void prof_end(int idx);
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
   double bc_val[6] = {0, 0, 0, 0, 0, 0};
+  // Multigrid call sequences captured as HIP graphs and replayed while the
+  // state their kernel arguments were taken from (device pointers, level
+  // lists, options) is unchanged; graphs_state fingerprints that state.
+  bool graphs_on = true;
+  uint64_t graphs_state = 0;
+  std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
+  void clear_graphs();
 
   int bs() const;
   int fs() const;

@@ -24,6 +24,7 @@ void Tree::resize(int new_cap) {
   nbs.resize((size_t)(new_cap + 1) * 6, 0);
   dr.resize(new_cap + 1, 0.0);
   rmin.resize((size_t)(new_cap + 1) * 3, 0.0);
+  tag.resize(new_cap + 1, kTagInit);
   cap = new_cap;
 }
 
@@ -173,6 +174,7 @@ void Tree::add_children(int id, const int* c_ids) {
     ncell[c] = ncell[id];
     dr[c] = 0.5 * dr[id];
     in_use[c] = 1;
+    tag[c] = kTagInit;  // m_aX_core.f90:769
   }
   for (int nb = 1; nb <= nnb; nb++) {
     int v = nbs[id * 6 + nb - 1];
@@ -362,6 +364,7 @@ bool Tree::tidy_up(double max_frac, double goal_frac, int n_clean_min, bool reor
     in_use[n] = 1;
     ncell[n] = old.ncell[o];
     dr[n] = old.dr[o];
+    tag[n] = old.tag[o];
   }
   for (int l = min_lvl; l <= max_lvl; l++) set_leaves_parents(l);
   max_id = n_used;

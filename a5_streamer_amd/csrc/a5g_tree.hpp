@@ -17,6 +17,10 @@
 namespace a5g {
 
 constexpr int kNoBox = 0;
+// multigrid operator class of a box (mg$D_set_box_tag, m_aX_mg.f90:673-712):
+// set at the first multigrid call that meets the box, kept until the box is
+// created again by a refinement
+constexpr int kTagInit = -1, kTagNormal = 0, kTagCeps = 1, kTagVeps = 2;
 constexpr int kRmRef = -1, kKpRef = 0, kDoRef = 1, kDerefine = -2, kRefine = 2;
 
 struct Tree {
@@ -33,6 +37,7 @@ struct Tree {
   std::vector<int> children;  // [id*8 + c]
   std::vector<int> nbs;       // [id*6 + nb]
   std::vector<double> dr, rmin;  // rmin[id*3 + d]
+  std::vector<int> tag;          // kTag*
 
   // Per level, index lvl - min_lvl for lvl in min_lvl .. lvls_max+1
   std::vector<std::vector<int>> ids, leaves, parents;

@@ -92,11 +92,13 @@ int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
 /* Mixed-precision solve of the same problem to max|r| <= tol max|rhs| over
  * the leaves: an FP64 defect-correction loop whose correction A e = r (zero
- * boundary data) is solved by FP32 FAS multigrid — one FMG plus
- * n_inner - 1 V-cycles, the schedule of mg$D_fas_fmg / mg$D_fas_vcycle —
- * while the residual, the update phi += e and the ghost cells of phi stay
- * FP64. zero_start = 1 starts from phi = 0. No reference counterpart: the
- * reference solves in FP64 only (afivo/src/m_aX_t.f90:11). */
+ * boundary data) is solved by FP32 FAS multigrid with the schedule of
+ * mg$D_fas_fmg / mg$D_fas_vcycle — the first correction by one FMG plus
+ * n_inner - 1 V-cycles, later ones by n_inner V-cycles from e = 0 — while
+ * the residual, the update phi += e, the restriction of phi to the parents
+ * and the ghost cells of phi stay FP64. zero_start = 1 starts from phi = 0.
+ * No reference counterpart: the reference solves in FP64 only
+ * (afivo/src/m_aX_t.f90:11). */
 int a5g_mg_solve_mixed(a5g_tree *t, double tol, int max_outer, int n_inner,
                        int zero_start, int *n_outer, double *rel_res);
 /* max over leaf cells of |rhs - A phi| (scratch: i_tmp), and of |rhs| */

@@ -21,6 +21,10 @@ CASES = {
     "3d": dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4,
                seed_r0=[[4e-3, 4e-3, 5e-3]], seed_r1=[[4e-3, 4e-3, 6e-3]],
                seed_width=[0.6e-3], applied_fld=2.5e6, n_boxes=4000),
+    # the reference's default dielectric (eps 1.5 below y = 8 mm): constant-
+    # and variable-eps boxes in the field solve, the update's dielectric rule
+    "2d_dielectric": dict(variant="2d", lvls_max=7, epsilon_diel=1.5,
+                          seed_r0=[[16e-3, 12e-3]], seed_r1=[[16e-3, 12e-3]]),
     # 2D with a strong field so that the mesh grows and shrinks as it runs
     "2d_refining": dict(variant="2d", lvls_max=8, applied_fld=1.0e7,
                         seed_width=[0.5e-3], seed_dens=[5e19], bg_dens=1e12),

@@ -27,6 +27,12 @@ CASES = [
     (2, 4, 0, (1, 1, 1)),
     (3, 4, 0, (2, 1, 1)),
 ]
+# the box sizes of the benchmarks, where a lane handles several cells
+LARGE = [
+    (3, 16, 0, (1, 1, 1)),
+    (2, 16, 1, (1, 2, 1)),
+    (2, 32, 0, (1, 1, 1)),
+]
 
 
 def _pair(oracle_lib, dim, nc, coord, base, n_cc=4, seed=1):
@@ -50,7 +56,7 @@ def _same(ts, dim, ivs):
             np.testing.assert_array_equal(a[k], b[k], err_msg="box %d var %d" % (k, iv))
 
 
-@pytest.mark.parametrize("dim,nc,coord,base", CASES)
+@pytest.mark.parametrize("dim,nc,coord,base", CASES + LARGE)
 def test_fmg_vcycle_bit_exact(oracle_lib, dim, nc, coord, base):
     ts = _pair(oracle_lib, dim, nc, coord, base)
     kinds = np.array([0, 1, 2, 0, 1, 2][:2 * dim], np.int32)
@@ -96,3 +102,104 @@ def test_default_ops_cylindrical_is_cartesian(oracle_lib):
         T.mg_setup(1, 2, 3, bc_kind=np.zeros(4, np.int32))
         T.fas_fmg(True, True)
     _same(ts, 2, (1, 2, 3))
+
+
+@pytest.mark.parametrize("dim,nc,coord,base", CASES + LARGE[:1])
+def test_fmg_vcycle_variable_eps_bit_exact(oracle_lib, dim, nc, coord, base):
+    """Normal, constant-eps and variable-eps boxes in one tree: the operator
+    tags and the lpld/clpld operators, Gauss-Seidel and correction."""
+    from test_oracle_ref import eps_field
+    ts = _pair(oracle_lib, dim, nc, coord, base, n_cc=5, seed=5)
+    e = eps_field(ts[0], dim, 5, 1)
+    for i, v in e.items():
+        ts[1].set_cc(i, 5, v)
+    kinds = np.array([0, 0, 1, 2, 0, 2][:2 * dim], np.int32)
+    for T in ts:
+        T.mg_setup(1, 2, 3, i_eps=5, n_down=2, n_up=2, n_base=4, bc_kind=kinds,
+                   bc_val=np.linspace(-1, 2, 2 * dim), auto_ops=True)
+    for it in range(2):
+        for T in ts:
+            T.fas_fmg(True, it == 0)
+        _same(ts, dim, (1, 2, 3))
+    for T in ts:
+        T.fas_vcycle(ts[0].info()["max_lvl"], True)
+    _same(ts, dim, (1, 2, 3, 4, 5))
+    # refine further: new boxes start untagged and are tagged at their first
+    # multigrid call, with the eps they hold by then
+    mx = ts[0].info()["max_lvl"]
+    for T in ts:
+        b = T.boxes()
+        c = S.box_centres(b, nc)
+        fl = np.array([1 if (b["in_use"][i] and b["lvl"][i] == mx
+                             and np.linalg.norm(c[i] - 0.3) < 0.12) else 0
+                       for i in range(len(b["lvl"]))], np.int32)
+        T.adjust(fl)
+    data = S.fill_random(ts[0], dim, [1, 2, 3], seed=6)
+    for (i, iv), v in data.items():
+        ts[1].set_cc(i, iv, v)
+    e = eps_field(ts[0], dim, 5, 1)
+    for i, v in e.items():
+        ts[1].set_cc(i, 5, v)
+    for T in ts:
+        T.fas_fmg(False, False)
+    _same(ts, dim, (1, 2, 3))
+
+
+def test_graph_replays_match_plain_launches(oracle_lib, monkeypatch):
+    """Multigrid calls are captured as HIP graphs on first use and replayed
+    after: every replay, with other work and a topology change in between,
+    must equal the plain launches (A5G_GRAPHS=0) and the oracle."""
+    import a5_streamer_amd as A
+    dim, nc, coord, base = 3, 8, 0, (1, 1, 1)
+    ts = []
+    for M, graphs in ((O.OTree, None), (A.Tree, "1"), (A.Tree, "0")):
+        if graphs is not None:
+            monkeypatch.setenv("A5G_GRAPHS", graphs)
+        T = M(dim, nc, 5, 2, 1.0 / nc, coarsen_to=2, coord=coord)
+        S.build_tree(T, dim, nc, _rules, n_base=base)
+        ts.append(T)
+    data = S.fill_random(ts[0], dim, [1, 2, 3, 4, 5], seed=9)
+    for (i, iv), v in data.items():
+        for T in ts[1:]:
+            T.set_cc(i, iv, v)
+    kinds = np.array([0, 1, 2, 0, 1, 2], np.int32)
+    for T in ts:
+        T.mg_setup(1, 2, 3, n_base=4, bc_kind=kinds, bc_val=np.arange(6) + 0.5, auto_ops=True)
+    mx = ts[0].info()["max_lvl"]
+
+    def same(ivs):
+        for iv in ivs:
+            a = S.all_cc(ts[0], dim, iv)
+            for T in ts[1:]:
+                b = S.all_cc(T, dim, iv)
+                for k in a:
+                    np.testing.assert_array_equal(a[k], b[k], err_msg="box %d var %d" % (k, iv))
+
+    for T in ts:
+        T.fas_fmg(False, True)
+    same((1, 2, 3))
+    for rep in range(4):
+        for T in ts:
+            T.fas_vcycle(mx, rep % 2 == 1)
+            T.gc_tree(4, 1, 1)  # other work on the same stream between replays
+        same((1, 2, 3, 4))
+        for T in ts:
+            T.fas_fmg(False, False)
+        same((1, 2, 3))
+    # a topology change invalidates the captured graphs
+    for T in ts:
+        b = T.boxes()
+        c = S.box_centres(b, nc)
+        fl = np.array([1 if (b["in_use"][i] and b["lvl"][i] == mx and
+                             np.linalg.norm(c[i] - 0.3) < 0.1) else 0
+                       for i in range(len(b["lvl"]))], np.int32)
+        T.adjust(fl)
+    data = S.fill_random(ts[0], dim, [1, 2, 3], seed=10)
+    for (i, iv), v in data.items():
+        for T in ts[1:]:
+            T.set_cc(i, iv, v)
+    mx = ts[0].info()["max_lvl"]
+    for rep in range(3):
+        for T in ts:
+            T.fas_vcycle(mx, False)
+        same((1, 2, 3))

@@ -174,3 +174,51 @@ def test_koren_limiter(ref, oracle_lib):
                                                   (1, 0.4), (2.5, 1), (-1, -4), (1, 2.5)]
     for a, b in vals:
         assert O.koren_mlim(float(a), float(b)) == ref.koren_mlim(float(a), float(b))
+
+
+def eps_field(T, dim, iv, kind_seed):
+    """Permittivity by box position: a dielectric slab (eps 3) whose face
+    cuts through boxes (variable-eps boxes), a region of constant eps 2
+    (constant-eps boxes) and eps 1 elsewhere; ghosts included, as the
+    drivers' set_box_eps writes them."""
+    b = T.boxes()
+    out = {}
+    for i in range(len(b["lvl"])):
+        if not b["in_use"][i]:
+            continue
+        nc = int(b["n_cell"][i])
+        dr = b["dr"][i]
+        ax = [b["r_min"][i][d] + (np.arange(nc + 2) - 0.5) * dr for d in range(dim)]
+        g = np.meshgrid(*ax, indexing="ij")
+        e = np.ones(g[0].shape)
+        e[g[1] < 0.31 + 0.02 * kind_seed] = 3.0
+        e[(g[0] > 0.75) & (g[1] > 0.5)] = 2.0
+        v = e.ravel(order="F")
+        out[i + 1] = v
+        T.set_cc(i + 1, iv, v)
+    return out
+
+
+@pytest.mark.parametrize("dim,nc,coord,base", TREES[:4])
+def test_fmg_vcycle_variable_eps(ref, oracle_lib, dim, nc, coord, base):
+    """mg$D_auto_* with an eps variable: normal, constant-eps and
+    variable-eps boxes (m_aX_mg.f90:568-712, lpld :895-1071, clpld
+    :1467-1531) in one tree."""
+    ts = _both(ref, dim, nc, coord, base, n_cc=5)
+    data = S.fill_random(ts[0], dim, [1, 2, 3], seed=17)
+    for (i, iv), v in data.items():
+        ts[1].set_cc(i, iv, v)
+    e = eps_field(ts[0], dim, 5, 0)
+    for i, v in e.items():
+        ts[1].set_cc(i, 5, v)
+    kinds = np.array([0, 0, 1, 2, 0, 2][:2 * dim], np.int32)
+    for T in ts:
+        T.mg_setup(1, 2, 3, i_eps=5, n_down=2, n_up=2, n_base=4, bc_kind=kinds,
+                   bc_val=np.linspace(-1, 2, 2 * dim), auto_ops=True)
+    for it in range(3):
+        for T in ts:
+            T.fas_fmg(it != 1, it == 0)
+        _same_cc(ts, dim, (1, 2, 3))
+    for T in ts:
+        T.fas_vcycle(ts[0].info()["max_lvl"], True)
+    _same_cc(ts, dim, (1, 2, 3, 5))
