@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -247,6 +249,39 @@ __device__ __forceinline__ void stage_lds(T* __restrict__ s, const T* __restrict
   }
 }
 
+// Copies, for one colour half of a box of NC^D cells, the slots of the
+// interior rows (j, k in 1..NC; each row's NC/2 + 1 slots of that colour,
+// one of them an x ghost slot) from global memory to LDS at the same
+// positions. The y and z ghost rows are left out: callers recompute every
+// face ghost slot they read.
+template <int D, int NC, int TPB, typename T>
+__device__ __forceinline__ void stage_rows(T* __restrict__ s, const T* __restrict__ g, int lt) {
+  constexpr int N2 = NC + 2, RW = N2 / 2;
+  constexpr int NR = D == 2 ? NC : NC * NC;
+  constexpr int N = NR * RW;
+  constexpr int NIT = (N + TPB - 1) / TPB;
+  constexpr int B = NIT < 8 ? NIT : 8;
+#pragma unroll
+  for (int i0 = 0; i0 < NIT; i0 += B) {
+    T v[B];
+    int at[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int q = lt + (i0 + u) * TPB;
+      at[u] = -1;
+      if (i0 + u < NIT && q < N) {
+        const int row = q / RW, m = q - row * RW;
+        const int j = row % NC + 1, k = D == 3 ? row / NC + 1 : 0;
+        at[u] = ((N2 * (j + N2 * k)) >> 1) + m;
+        v[u] = g[at[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++)
+      if (at[u] >= 0) s[at[u]] = v[u];
+  }
+}
+
 // Box-size-specialised half-sweep. The other colour's half of the box (ghost
 // slots included) is staged in LDS; when fused, that colour's face ghost
 // slots are first recomputed from the current state (the values the
@@ -284,7 +319,10 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   };
   T rv[NIT];
   if (act) {
-    stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
+    // a fused half-sweep recomputes every face ghost it reads: only the
+    // interior rows of the other colour are loaded
+    if (fused) stage_rows<D, NC, TPB>(s, p + oc * HALF, lt);
+    else stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
     const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
@@ -551,7 +589,12 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   const bool act = li < n;
   const int id = act ? list[li] : 0;
   T* s = sh + slot * BL;
-  if (act) stage_lds<BL, TPB>(s, boxp_t<T>(a, i_phi, id), lt);
+  if (act) {
+    // interior rows of both colours; the face ghosts are recomputed below
+    const T* p = boxp_t<T>(a, i_phi, id);
+    stage_rows<D, NC, TPB>(s, p, lt);
+    stage_rows<D, NC, TPB>(s + BL / 2, p + BL / 2, lt);
+  }
   __syncthreads();
   if (act) {
     for (int q = lt; q < NGH; q += TPB) {
@@ -788,6 +831,114 @@ __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ lis
   }
 }
 
+// The same correction with the parent's region under the child staged in
+// LDS: (NC/2 + 2)^D values of the parent's tmp, ghosts included, read once
+// per child box instead of once per child cell.
+template <int D, int NC, typename T>
+__global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__ list, int n,
+                                                  int i_phi, int i_tmp) {
+  constexpr int R = NC / 2 + 2;
+  constexpr int RB = D == 2 ? R * R : R * R * R;
+  constexpr int TOTAL = D == 2 ? NC * NC : NC * NC * NC;
+  constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int BPB = 256 / TPB;
+  constexpr int NIT = (TOTAL + TPB - 1) / TPB;
+  __shared__ T sh[BPB * RB];
+  const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
+  const int li = xcd_block() * BPB + slot;
+  const bool act = li < n;
+  const int id = act ? list[li] : 0;
+  T* q = sh + slot * RB;
+  int off[3] = {0, 0, 0};
+  int pid = 0, pnc = 0;
+  if (act) {
+    pid = a.parent[id];
+    pnc = a.ncell[pid];
+    child_off<D>(a, id, NC, 0, off);
+    const T* c = boxp_t<T>(a, i_tmp, pid);
+    for (int t = lt; t < RB; t += TPB) {
+      const int x = t % R, y = (t / R) % R, z = D == 3 ? t / (R * R) : 0;
+      q[t] = c[cidx<D>(pnc, off[0] + x, off[1] + y, D == 3 ? off[2] + z : 0)];
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  T* f = boxp_t<T>(a, i_phi, id);
+  const double* ep = (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, pid) : nullptr;
+  auto L = [&](int x, int y, int z) { return D == 2 ? x + R * y : x + R * (y + R * z); };
+#pragma unroll
+  for (int u = 0; u < NIT; u++) {
+    const int t = lt + u * TPB;
+    if (t >= TOTAL) break;
+    const int i = t % NC + 1, j = (t / NC) % NC + 1, k = D == 3 ? t / (NC * NC) + 1 : 0;
+    // local parent indices: (i+1)/2 is the closest coarse cell, the other
+    // one lies on the side of the fine cell
+    const int i1 = (i + 1) >> 1, i2 = i1 + 1 - 2 * (i & 1);
+    const int j1 = (j + 1) >> 1, j2 = j1 + 1 - 2 * (j & 1);
+    const int x = cidx<D>(NC, i, j, k);
+    if (ep != nullptr) {
+      if (D == 2) {
+        const int p0 = cidx<D>(pnc, off[0] + i1, off[1] + j1, 0);
+        const int pa[2] = {cidx<D>(pnc, off[0] + i2, off[1] + j1, 0),
+                           cidx<D>(pnc, off[0] + i1, off[1] + j2, 0)};
+        const T ua[2] = {q[L(i2, j1, 0)], q[L(i1, j2, 0)]};
+        const T u0 = q[L(i1, j1, 0)], a0 = (T)ep[p0];
+        T sm = 0;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+          const T am = (T)ep[pa[m]];
+          sm = sm + (a0 * u0 + am * ua[m]) / (a0 + am);
+        }
+        f[x] = f[x] + T(0.5) * sm;
+      } else {
+        const int k1 = (k + 1) >> 1, k2 = k1 + 1 - 2 * (k & 1);
+        const int p0 = cidx<D>(pnc, off[0] + i1, off[1] + j1, off[2] + k1);
+        const int pa[3] = {cidx<D>(pnc, off[0] + i2, off[1] + j1, off[2] + k1),
+                           cidx<D>(pnc, off[0] + i1, off[1] + j2, off[2] + k1),
+                           cidx<D>(pnc, off[0] + i1, off[1] + j1, off[2] + k2)};
+        const T ua[3] = {q[L(i2, j1, k1)], q[L(i1, j2, k1)], q[L(i1, j1, k2)]};
+        const T u0 = q[L(i1, j1, k1)], a0 = (T)ep[p0];
+        T sm = 0;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          const T am = (T)ep[pa[m]];
+          sm = sm + (a0 * u0 + am * (T(1.5) * ua[m] - T(0.5) * u0)) / (a0 + am);
+        }
+        f[x] = f[x] + (T(1) / T(3)) * sm;
+      }
+    } else if (D == 2) {
+      f[x] = f[x] + T(0.5) * q[L(i1, j1, 0)] + T(0.25) * (q[L(i2, j1, 0)] + q[L(i1, j2, 0)]);
+    } else {
+      const int k1 = (k + 1) >> 1, k2 = k1 + 1 - 2 * (k & 1);
+      f[x] = f[x] + T(0.25) * (q[L(i1, j1, k1)] + q[L(i2, j1, k1)] + q[L(i1, j2, k1)] +
+                               q[L(i1, j1, k2)]);
+    }
+  }
+}
+
+template <int D, typename T>
+static bool launch_corr_lds(Pool& P, const Dev& a, const int* list, int n, int nc, int i_phi,
+                            int i_tmp) {
+  auto go = [&](auto kern, int total) {
+    const int tpb = total > 128 ? 256 : (total > 64 ? 128 : 64);
+    const int bpb = 256 / tpb;
+    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
+                       i_phi, i_tmp);
+  };
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  switch (nc) {
+    case 8: go(k_corr_lds<D, 8, T>, total); return true;
+    case 16: go(k_corr_lds<D, 16, T>, total); return true;
+    case 32:
+      if (D == 2) {
+        go(k_corr_lds<2, 32, T>, total);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
 // whole-box copy (a$D_box_copy_cc, m_aX_utils.f90:421-429)
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_whole(Dev a, const int* __restrict__ list,
@@ -931,12 +1082,13 @@ struct Ctx {
   bool homogeneous;
   bool cyl;  // cylindrical operators (2D, automatic operators installed)
   bool eps;  // per-box operator tags (automatic operators and an eps variable)
+  int n_down, n_up, n_base;  // smoothing cycles (mg$D_t%n_cycle_*)
 };
 
 static Ctx ctx64(const Pool& P) {
   return {P.mg.i_phi, P.mg.i_tmp, P.mg.i_rhs, false,
           P.tree.dim == 2 && P.tree.coord == 1 && P.mg.auto_ops != 0,
-          P.mg.auto_ops != 0 && P.mg.i_eps > 0};
+          P.mg.auto_ops != 0 && P.mg.i_eps > 0, P.mg.n_down, P.mg.n_up, P.mg.n_base};
 }
 
 static Dev dev_for(Pool& P, const Ctx& c) {
@@ -1136,8 +1288,9 @@ static void correct_children(Pool& P, const Ctx& c, int lvl) {
   Geo g2 = geo(nch, cells_of(D, cnc, false));
   pr = P.prof_begin(PK_CORR, lvl + 1,
                     (2.0 + 1.0 / (1 << D)) * sizeof(T) * nch * cells_of(D, cnc, false));
-  hipLaunchKernelGGL((k_corr<D, T>), dim3(g2.grid), dim3(256), 0, P.stream, a, P.ids(lvl + 1),
-                     nch, cnc, g2.tpb, c.i_phi, c.i_tmp);
+  if (!launch_corr_lds<D, T>(P, a, P.ids(lvl + 1), nch, cnc, c.i_phi, c.i_tmp))
+    hipLaunchKernelGGL((k_corr<D, T>), dim3(g2.grid), dim3(256), 0, P.stream, a,
+                       P.ids(lvl + 1), nch, cnc, g2.tpb, c.i_phi, c.i_tmp);
   post_launch("corr");
   P.prof_end(pr);
 }
@@ -1198,15 +1351,21 @@ static void fas_vcycle(Pool& P, const Ctx& c, int max_lvl, bool set_residual) {
   // the one after the correction (the first up-smoothing half-sweep does).
   // Until the up-smoothing stores them again, the stored ghosts of a level
   // are read by nobody.
+  // The FP32 correction of the mixed solve goes further: its ghost cells are
+  // never read from outside, so a level without parents (whose ghosts no
+  // correction reads) does not store them after the up-smoothing, and the
+  // next down-smoothing of that level recomputes them in its first
+  // half-sweep.
   const int min_lvl = P.tree.min_lvl;
+  auto lazy = [&](int lvl) { return c.homogeneous && P.n_parents(lvl) == 0; };
   for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
-    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_down, false, false);
+    gsrb_boxes<D, T>(P, c, lvl, c.n_down, lvl == max_lvl && lazy(lvl), false);
     update_coarse<D, T>(P, c, lvl);
   }
-  gsrb_boxes<D, T>(P, c, min_lvl, P.mg.n_base);
+  gsrb_boxes<D, T>(P, c, min_lvl, c.n_base);
   for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
     correct_children<D, T>(P, c, lvl - 1);
-    gsrb_boxes<D, T>(P, c, lvl, P.mg.n_up, true, true);
+    gsrb_boxes<D, T>(P, c, lvl, c.n_up, true, !(lvl == max_lvl && lazy(lvl)));
   }
   if (set_residual)
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) residual_level<D, T>(P, c, lvl, false);
@@ -1586,7 +1745,14 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
                         double* rel_res) {
   const Ctx c64 = ctx64(P);
   // FP32 variables: 1 = correction e, 2 = tmp, 3 = rhs of the correction
-  const Ctx c32{1, 2, 3, true, c64.cyl, c64.eps};
+  Ctx c32{1, 2, 3, true, c64.cyl, c64.eps, c64.n_down, c64.n_up, c64.n_base};
+  if (const char* e = std::getenv("A5G_MIXED_SMOOTH")) {
+    int d = 0, u = 0;
+    if (std::sscanf(e, "%d,%d", &d, &u) == 2 && d >= 0 && u >= 0) {
+      c32.n_down = d;
+      c32.n_up = u;
+    }
+  }
   ensure32(P, 3);
   const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
   double rhs_lo, rhs_max;
@@ -1631,7 +1797,8 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
     // refinement-boundary ghosts of the next residual see the same composite
     // problem as the FP64 solve. Then the FP64 ghost layer, coarse to fine.
     const bool first = it == 0;
-    run_captured(P, call_key(3, first, n_inner, max_lvl - min_lvl), [&] {
+    run_captured(P, call_key(3 + 16 * (c32.n_down * 16 + c32.n_up), first, n_inner,
+                             max_lvl - min_lvl), [&] {
       for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
         const int n = P.n_ids(lvl);
         if (n == 0) continue;

@@ -139,15 +139,48 @@ def profiled(T, fn):
     return prof
 
 
-def roofline(prof, steps):
-    dom = max(prof, key=lambda k: prof[k][1])
-    n, ms, by = prof[dom]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "octree16_gsrb_pmc_traffic.md")
+
+
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of the finest-level launch of `kernel`, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (None if
+    absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 6 and w[:2] == ["-", "traffic"] and w[2] == kernel and \
+                        w[3] == "finest" and w[4] == precision:
+                    return float(w[5])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(T, precision, fn):
+    """Roofline of the dominant kernel, the Gauss-Seidel half-sweep, on the
+    finest level (its largest launches): algorithmic bytes per launch over
+    the launch's mean duration from HIP events on the library's stream, for
+    one more call of fn. The same figures over all levels ride along."""
+    lvl = T.info()["max_lvl"]
+    T.prof_enable(True)
+    T.sync()
+    fn()
+    T.sync()
+    n, ms, by = T.prof_read(0, lvl)
+    na, msa, bya = T.prof_read(0, -999)
+    T.prof_enable(False)
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    tr = measured_traffic("gsrb_half_sweep", precision)
     return {
-        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+        "bound": "hbm", "kernel": "gsrb_half_sweep (finest level)", "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None, "launches": n, "avg_launch_us": round(ms * 1e3 / max(n, 1), 2),
+        "traffic": tr, "launches": n, "avg_launch_us": round(ms * 1e3 / max(n, 1), 2),
         "alg_bytes_per_launch": round(by / max(n, 1)),
+        "all_levels": {"launches": na, "avg_launch_us": round(msa * 1e3 / max(na, 1), 2),
+                       "alg_bytes_per_launch": round(bya / max(na, 1)),
+                       "achieved": round(bya / (msa * 1e-3) / 1e9, 1) if msa > 0 else 0.0},
     }
 
 
@@ -201,7 +234,7 @@ def main():
         },
         "fp64": {"value": round(v64, 3), "ms_per_step": round(dt64 * 1e3 / args.steps, 3),
                  "v_cycles_per_solve": n_v, "rel_residual": rel64,
-                 "roofline": roofline(prof64, 1),
+                 "roofline": roofline(T, "fp64", lambda: solve(T, max_lvl, n_v)),
                  "kernel_ms_per_step": {k: round(v[1], 3) for k, v in prof64.items()}},
     }
     head = "fp64"
@@ -223,7 +256,7 @@ def main():
                         "outer_iterations": info_mixed["it"], "rel_residual": info_mixed["rel"],
                         "rel_potential_error_vs_fp64": err,
                         "speedup_vs_fp64": round(vm / v64, 3),
-                        "roofline": roofline(profm, 1),
+                        "roofline": roofline(T, "mixed", one),
                         "kernel_ms_per_step": {k: round(v[1], 3) for k, v in profm.items()}}
         if err <= 1e-8:
             head = "mixed"
