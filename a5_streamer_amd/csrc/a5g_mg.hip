@@ -288,7 +288,7 @@ __device__ __forceinline__ void stage_rows(T* __restrict__ s, const T* __restric
 // reference's ghost pass after the previous half-sweep would store). Then
 // every lane updates cells of colour cntr&1 from LDS and rhs (loaded into
 // registers before the barrier).
-template <int D, bool CYL, int NC, typename T>
+template <int D, bool CYL, int NC, typename T, bool EPS>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
   constexpr int N2 = NC + 2;
@@ -318,11 +318,9 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
     return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
   };
   T rv[NIT];
-  if (act) {
-    // a fused half-sweep recomputes every face ghost it reads: only the
-    // interior rows of the other colour are loaded
-    if (fused) stage_rows<D, NC, TPB>(s, p + oc * HALF, lt);
-    else stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
+  if (act && !fused) {
+    // the first half-sweep of a call reads the stored ghost layer
+    stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
     const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
@@ -330,25 +328,76 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
       int i;
       if (q < NCOL) rv[u] = r[lex_of(q, i) >> 1];
     }
-  }
-  // the staged half includes the ghost slots: they are overwritten below only
-  // after every lane has stored its part
-  if (fused) __syncthreads();
-  if (act) {
-    if (fused) {
-      for (int q = lt; q < NGH; q += TPB) {
-        const int nb = q / PER_FACE + 1;
-        const int t = q - (nb - 1) * PER_FACE;
+  } else if (act) {
+    // A fused half-sweep recomputes the face ghosts of the other colour.
+    // Every global load (interior cells of colour oc, rhs of colour c, the
+    // neighbour cells that the ghosts copy) is issued before any LDS store;
+    // the staged slots and the ghost slots are disjoint.
+    constexpr int NGC = NGH / 2;  // face ghosts of one colour
+    constexpr int NITG = (NGC + TPB - 1) / TPB;
+    constexpr int GPF = PER_FACE / 2;
+    T sv[NIT];
+    int sa[NIT];
+    const T* po = p + oc * HALF;
+#pragma unroll
+    for (int u = 0; u < NIT; u++) {
+      const int q = lt + u * TPB;
+      sa[u] = -1;
+      if (q < NCOL) {
+        const int mm = q % HNC, row = q / HNC;
+        const int j = row % NC + 1;
+        const int k = D == 3 ? row / NC + 1 : 0;
+        const int i = 2 * mm + 2 - ((oc + j + k) & 1);
+        sa[u] = (D == 2 ? i + N2 * j : i + N2 * (j + N2 * k)) >> 1;
+        sv[u] = po[sa[u]];
+      }
+    }
+    const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
+#pragma unroll
+    for (int u = 0; u < NIT; u++) {
+      const int q = lt + u * TPB;
+      int i;
+      if (q < NCOL) rv[u] = r[lex_of(q, i) >> 1];
+    }
+    T gv[NITG];
+    int gs[NITG], gnb[NITG], gg[NITG][3];
+#pragma unroll
+    for (int u = 0; u < NITG; u++) {
+      const int q = lt + u * TPB;
+      gs[u] = -1;
+      if (q < NGC) {
+        const int nb = q / GPF + 1;
+        const int t = q - (nb - 1) * GPF;
         const int d = d_nb_dim(nb);
         const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
         int g[3] = {0, 0, 0};
         g[d] = d_nb_low(nb) ? 0 : NC + 1;
-        g[ta] = t % NC + 1;
-        if (D == 3) g[tb] = t / NC + 1;
-        if (((g[0] + g[1] + g[2]) & 1) != oc) continue;
-        const int lex = D == 2 ? g[0] + N2 * g[1] : g[0] + N2 * (g[1] + N2 * g[2]);
-        s[lex >> 1] = ghost_mg<D, T>(a, p, i_phi, id, NC, nb, g[0], g[1], g[2]);
+        // ghosts of colour oc: along ta every other cell of each tb row
+        const int y = D == 3 ? t / HNC + 1 : 0;
+        const int w = D == 3 ? t % HNC : t;
+        if (D == 3) g[tb] = y;
+        g[ta] = 2 * w + 2 - ((oc + g[d] + y) & 1);
+        gs[u] = (D == 2 ? g[0] + N2 * g[1] : g[0] + N2 * (g[1] + N2 * g[2])) >> 1;
+        gnb[u] = nb;
+        gg[u][0] = g[0]; gg[u][1] = g[1]; gg[u][2] = g[2];
+        const int nb_id = a.nbs[id * 6 + nb - 1];
+        if (nb_id > 0) {
+          int sc[3] = {g[0], g[1], g[2]};
+          sc[d] = d_nb_low(nb) ? NC : 1;
+          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx<D>(NC, sc[0], sc[1], sc[2])];
+          gnb[u] = 0;
+        }
       }
+    }
+#pragma unroll
+    for (int u = 0; u < NIT; u++)
+      if (sa[u] >= 0) s[sa[u]] = sv[u];
+#pragma unroll
+    for (int u = 0; u < NITG; u++) {
+      if (gs[u] < 0) continue;
+      // refinement-boundary and physical rules from the box's own cells
+      if (gnb[u] > 0) gv[u] = ghost_mg<D, T>(a, p, i_phi, id, NC, gnb[u], gg[u][0], gg[u][1], gg[u][2]);
+      s[gs[u]] = gv[u];
     }
   }
   __syncthreads();
@@ -358,7 +407,8 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
   // boxes tagged constant- or variable-eps (mg$D_auto_gsrb,
   // m_aX_mg.f90:568-600); eps is read in its own (colour-split) order
-  const double* ev = (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
+  const double* ev =
+      (EPS && a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
 #pragma unroll
   for (int u = 0; u < NIT; u++) {
     const int q = lt + u * TPB;
@@ -369,7 +419,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
     const T xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
     const T yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
     T v;
-    if (ev != nullptr) {
+    if (EPS && ev != nullptr) {
       // the neighbours have the other colour: second half of the box
       const double* eo = ev + oc * HALF;
       T uu[6], am[6];
@@ -405,9 +455,9 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   }
 }
 
-template <int D, bool CYL, typename T>
-static bool launch_gsrb_lds(Pool& P, const Dev& a, const int* list, int n, int nc, int cntr,
-                            int fused, int i_phi, int i_rhs) {
+template <int D, bool CYL, typename T, bool EPS>
+static bool launch_gsrb_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc, int cntr,
+                              int fused, int i_phi, int i_rhs) {
   auto go = [&](auto kern, int ncol) {
     const int tpb = ncol > 128 ? 256 : (ncol > 64 ? 128 : 64);
     const int bpb = 256 / tpb;
@@ -416,18 +466,28 @@ static bool launch_gsrb_lds(Pool& P, const Dev& a, const int* list, int n, int n
   };
   const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
   switch (nc) {
-    case 2: go(k_gsrb_lds<D, CYL, 2, T>, ncol); return true;
-    case 4: go(k_gsrb_lds<D, CYL, 4, T>, ncol); return true;
-    case 8: go(k_gsrb_lds<D, CYL, 8, T>, ncol); return true;
-    case 16: go(k_gsrb_lds<D, CYL, 16, T>, ncol); return true;
+    case 2: go(k_gsrb_lds<D, CYL, 2, T, EPS>, ncol); return true;
+    case 4: go(k_gsrb_lds<D, CYL, 4, T, EPS>, ncol); return true;
+    case 8: go(k_gsrb_lds<D, CYL, 8, T, EPS>, ncol); return true;
+    case 16: go(k_gsrb_lds<D, CYL, 16, T, EPS>, ncol); return true;
     case 32:
       if (D == 2) {
-        go(k_gsrb_lds<2, CYL, 32, T>, ncol);
+        go(k_gsrb_lds<2, CYL, 32, T, EPS>, ncol);
         return true;
       }
       return false;
     default: return false;
   }
+}
+
+// The permittivity branch is compiled in only when the operator tags are in
+// use, so the common kernel keeps its registers for loads in flight.
+template <int D, bool CYL, typename T>
+static bool launch_gsrb_lds(Pool& P, const Dev& a, const int* list, int n, int nc, int cntr,
+                            int fused, int i_phi, int i_rhs) {
+  if (a.tag != nullptr)
+    return launch_gsrb_lds_t<D, CYL, T, true>(P, a, list, n, nc, cntr, fused, i_phi, i_rhs);
+  return launch_gsrb_lds_t<D, CYL, T, false>(P, a, list, n, nc, cntr, fused, i_phi, i_rhs);
 }
 
 // ---------------------------------------------------------------------------
@@ -571,7 +631,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
 // that ends the reference's smoothing, fill_gc_phi after gsrb_boxes, would
 // store: nothing changes the child, its neighbours or the uncovered parent
 // cells in between), so that pass is not needed before this one.
-template <int D, bool CYL, int NC, typename T>
+template <int D, bool CYL, int NC, typename T, bool EPS>
 __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
                                                         int keep_res) {
@@ -588,16 +648,73 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   const int li = xcd_block() * BPB + slot;
   const bool act = li < n;
   const int id = act ? list[li] : 0;
+  constexpr int NCELL = D == 2 ? NC * NC : NC * NC * NC;
+  constexpr int NITC = (NCELL + TPB - 1) / TPB;
+  constexpr int NITG = (NGH + TPB - 1) / TPB;
   T* s = sh + slot * BL;
+  // Face ghosts that copy a neighbour's cell are loaded first, then the
+  // interior cells (both colours) in batches; all these LDS slots are
+  // disjoint. Ghosts of the other rules read the staged interior, after a
+  // barrier.
+  int gq[NITG];
   if (act) {
-    // interior rows of both colours; the face ghosts are recomputed below
     const T* p = boxp_t<T>(a, i_phi, id);
-    stage_rows<D, NC, TPB>(s, p, lt);
-    stage_rows<D, NC, TPB>(s + BL / 2, p + BL / 2, lt);
+    T gv[NITG];
+    int gs[NITG];
+#pragma unroll
+    for (int u = 0; u < NITG; u++) {
+      const int q = lt + u * TPB;
+      gs[u] = -1;
+      gq[u] = -1;
+      if (q < NGH) {
+        const int nb = q / PER_FACE + 1;
+        const int t = q - (nb - 1) * PER_FACE;
+        const int d = d_nb_dim(nb);
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        int g[3] = {0, 0, 0};
+        g[d] = d_nb_low(nb) ? 0 : NC + 1;
+        g[ta] = t % NC + 1;
+        if (D == 3) g[tb] = t / NC + 1;
+        const int nb_id = a.nbs[id * 6 + nb - 1];
+        if (nb_id > 0) {
+          int sc[3] = {g[0], g[1], g[2]};
+          sc[d] = d_nb_low(nb) ? NC : 1;
+          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx<D>(NC, sc[0], sc[1], sc[2])];
+          gs[u] = cidx<D>(NC, g[0], g[1], g[2]);
+        } else {
+          gq[u] = q;
+        }
+      }
+    }
+    constexpr int B = NITC < 8 ? NITC : 8;
+#pragma unroll
+    for (int i0 = 0; i0 < NITC; i0 += B) {
+      T v[B];
+      int at[B];
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        const int q = lt + (i0 + u) * TPB;
+        at[u] = -1;
+        if (i0 + u < NITC && q < NCELL) {
+          const int i = q % NC + 1, j = (q / NC) % NC + 1, k = D == 3 ? q / (NC * NC) + 1 : 0;
+          at[u] = cidx<D>(NC, i, j, k);
+          v[u] = p[at[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B; u++)
+        if (at[u] >= 0) s[at[u]] = v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NITG; u++)
+      if (gs[u] >= 0) s[gs[u]] = gv[u];
   }
   __syncthreads();
   if (act) {
-    for (int q = lt; q < NGH; q += TPB) {
+#pragma unroll
+    for (int u = 0; u < NITG; u++) {
+      const int q = gq[u];
+      if (q < 0) continue;
       const int nb = q / PER_FACE + 1;
       const int t = q - (nb - 1) * PER_FACE;
       const int d = d_nb_dim(nb);
@@ -636,7 +753,8 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
         for (int di = 0; di < 2; di++) {
           const int ii = fi + di, jj = fj + dj, kk = fk + dk;
           const int c = cidx<D>(NC, ii, jj, kk);
-          const T res = r[c] - op_at<D, CYL, T>(a, id, s, NC, ii, jj, kk, inv_dr_sq, ioff);
+          const T res = r[c] - (EPS ? op_at<D, CYL, T>(a, id, s, NC, ii, jj, kk, inv_dr_sq, ioff)
+                                    : lpl_at<D, CYL, T>(s, NC, ii, jj, kk, inv_dr_sq, ioff));
           if (keep_res) tv[c] = res;
           sr = sr + res;
           sp = sp + s[c];
@@ -1173,9 +1291,9 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
   if (store || n_cycle == 0) fill_gc_phi<D, T>(P, c, lvl);
 }
 
-template <int D, bool CYL, typename T>
-static bool launch_resid_rstr_lds(Pool& P, const Dev& a, const int* list, int n, int nc,
-                                  int i_phi, int i_rhs, int i_tmp, int keep_res) {
+template <int D, bool CYL, typename T, bool EPS>
+static bool launch_resid_rstr_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc,
+                                    int i_phi, int i_rhs, int i_tmp, int keep_res) {
   auto go = [&](auto kern, int total) {
     const int tpb = total > 128 ? 256 : (total > 64 ? 128 : 64);
     const int bpb = 256 / tpb;
@@ -1185,18 +1303,28 @@ static bool launch_resid_rstr_lds(Pool& P, const Dev& a, const int* list, int n,
   const int hnc = nc / 2;
   const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
   switch (nc) {
-    case 2: go(k_resid_rstr_lds<D, CYL, 2, T>, total); return true;
-    case 4: go(k_resid_rstr_lds<D, CYL, 4, T>, total); return true;
-    case 8: go(k_resid_rstr_lds<D, CYL, 8, T>, total); return true;
-    case 16: go(k_resid_rstr_lds<D, CYL, 16, T>, total); return true;
+    case 2: go(k_resid_rstr_lds<D, CYL, 2, T, EPS>, total); return true;
+    case 4: go(k_resid_rstr_lds<D, CYL, 4, T, EPS>, total); return true;
+    case 8: go(k_resid_rstr_lds<D, CYL, 8, T, EPS>, total); return true;
+    case 16: go(k_resid_rstr_lds<D, CYL, 16, T, EPS>, total); return true;
     case 32:
       if (D == 2) {
-        go(k_resid_rstr_lds<2, CYL, 32, T>, total);
+        go(k_resid_rstr_lds<2, CYL, 32, T, EPS>, total);
         return true;
       }
       return false;
     default: return false;
   }
+}
+
+template <int D, bool CYL, typename T>
+static bool launch_resid_rstr_lds(Pool& P, const Dev& a, const int* list, int n, int nc,
+                                  int i_phi, int i_rhs, int i_tmp, int keep_res) {
+  if (a.tag != nullptr)
+    return launch_resid_rstr_lds_t<D, CYL, T, true>(P, a, list, n, nc, i_phi, i_rhs, i_tmp,
+                                                    keep_res);
+  return launch_resid_rstr_lds_t<D, CYL, T, false>(P, a, list, n, nc, i_phi, i_rhs, i_tmp,
+                                                   keep_res);
 }
 
 // Residual + restriction of level lvl into its parents. The face ghosts of
@@ -1745,14 +1873,7 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
                         double* rel_res) {
   const Ctx c64 = ctx64(P);
   // FP32 variables: 1 = correction e, 2 = tmp, 3 = rhs of the correction
-  Ctx c32{1, 2, 3, true, c64.cyl, c64.eps, c64.n_down, c64.n_up, c64.n_base};
-  if (const char* e = std::getenv("A5G_MIXED_SMOOTH")) {
-    int d = 0, u = 0;
-    if (std::sscanf(e, "%d,%d", &d, &u) == 2 && d >= 0 && u >= 0) {
-      c32.n_down = d;
-      c32.n_up = u;
-    }
-  }
+  const Ctx c32{1, 2, 3, true, c64.cyl, c64.eps, c64.n_down, c64.n_up, c64.n_base};
   ensure32(P, 3);
   const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
   double rhs_lo, rhs_max;
@@ -1797,8 +1918,7 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
     // refinement-boundary ghosts of the next residual see the same composite
     // problem as the FP64 solve. Then the FP64 ghost layer, coarse to fine.
     const bool first = it == 0;
-    run_captured(P, call_key(3 + 16 * (c32.n_down * 16 + c32.n_up), first, n_inner,
-                             max_lvl - min_lvl), [&] {
+    run_captured(P, call_key(3, first, n_inner, max_lvl - min_lvl), [&] {
       for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
         const int n = P.n_ids(lvl);
         if (n == 0) continue;

@@ -135,6 +135,8 @@ static void resize_storage(otree *t, int new_cap) {
     memset(CHP(t, id), 0, sizeof(int) * 8);
     memset(NBP(t, id), 0, sizeof(int) * 6);
     memset(CC(t, id, 1), 0, sizeof(double) * t->n_cc * t->box_len);
+    memset(t->fc + (size_t)id * (t->n_fc > 0 ? t->n_fc : 1) * t->dim * t->face_len, 0,
+           sizeof(double) * (t->n_fc > 0 ? t->n_fc : 1) * t->dim * t->face_len);
   }
   if (new_cap < old) {
     /* shrinking is only used by tidy_up, which rewrites all boxes */

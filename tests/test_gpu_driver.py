@@ -58,9 +58,10 @@ def test_run_matches_oracle(oracle_lib, name):
     _same_state(runs[0].T, runs[1].T, dim)
     n_blocks = 2 if name == "3d" else 4
     changed = 0
-    for _ in range(n_blocks):
+    for blk in range(n_blocks):
+        pre = [(r.max_dt(), r.time, r.T.tree_minmax(S.I_FLD)) for r in runs]
         res = [r.step_block() for r in runs]
-        assert res[0] == res[1]
+        assert res[0] == res[1], (blk, res, pre)
         changed += res[0][0] + res[0][1]
         assert runs[0].dt == runs[1].dt and runs[0].time == runs[1].time
         _same_state(runs[0].T, runs[1].T, dim)

@@ -71,6 +71,27 @@ def test_host_tree_matches_reference(ref):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def test_top_level_leaf_list(ref, oracle_lib):
+    """When refinement reaches lvls_max, a$D_adjust_refinement leaves the loop
+    without its exit, so max_lvl + 1 is past the top level and the leaf and
+    parent lists of level lvls_max are not rebuilt (m_aX_core.f90:497-531).
+    The leaf loops of the streamer then skip those boxes until a tidy_up
+    renumbers them. The oracle and the library keep this behaviour."""
+    import a5_streamer_amd as A
+    ts = [M(2, 8, 3, 1, 1.0 / 8, coarsen_to=2, lvls_max=3)
+          for M in (ref.RefTree, O.OTree, A.Tree)]
+    for T in ts:
+        S.build_tree(T, 2, 8, lambda lvl, c: 1, n_base=(2, 1, 1))
+    for T in ts:
+        assert T.info()["max_lvl"] == 3
+        ids, leaves, parents = T.level(3)
+        assert len(ids) == 2 * 16 and len(leaves) == 0 and len(parents) == 0
+    for T in ts:
+        T.tidy_up(0.9, 0.25, 3, True)
+        ids, leaves, parents = T.level(3)
+        assert len(leaves) == len(ids) == 32
+
+
 def test_errors_are_reported():
     import a5_streamer_amd as A
     with pytest.raises(A.A5GError, match="n_cell should be even"):
