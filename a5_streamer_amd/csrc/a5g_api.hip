@@ -303,11 +303,11 @@ static void check_id(const a5g::Tree& T, int id) {
 // cc(0:nc+1, ...) and the colour-split device layout (cidx in a5g_kernels.hpp).
 static void split_perm(int dim, int nc, std::vector<int>& perm) {
   const int n = nc + 2;
-  const int bl = dim == 2 ? n * n : n * n * n, half = bl / 2;
+  const int bl = dim == 2 ? n * n : n * n * n;
   perm.resize(bl);
   for (int q = 0; q < bl; q++) {
     const int i = q % n, j = (q / n) % n, k = dim == 3 ? q / (n * n) : 0;
-    perm[q] = (((i + j + k) & 1) ? half : 0) + (q >> 1);
+    perm[q] = dim == 2 ? a5g::cidx<2>(nc, i, j, k) : a5g::cidx<3>(nc, i, j, k);
   }
 }
 static void lex_to_split(int dim, int nc, const double* in, double* out) {

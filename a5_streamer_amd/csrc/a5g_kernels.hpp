@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "a5g_layout.hpp"
 #include "a5g_pool.hpp"
 
 namespace a5g {
@@ -16,19 +17,6 @@ __device__ __forceinline__ int d_nb_dim(int nb) { return (nb - 1) >> 1; }
 __device__ __forceinline__ bool d_nb_low(int nb) { return ((nb - 1) & 1) == 0; }
 __device__ __forceinline__ int d_nb_dix(int nb, int d) {
   return d_nb_dim(nb) != d ? 0 : (d_nb_low(nb) ? -1 : 1);
-}
-
-// Flat index of cell (i, j, k), ghosts included (0 <= i,j,k <= nc+1), in a
-// box with nc cells per dimension. Boxes are stored colour-split: the cells
-// with i+j+k even come first, then the odd ones, each half in lexicographic
-// order of the (even) row pairs. A red-black half-sweep then reads one half
-// and writes the other instead of touching every cache line of the box.
-template <int D>
-__device__ __forceinline__ int cidx(int nc, int i, int j, int k) {
-  const int n = nc + 2;
-  const int lex = D == 2 ? i + n * j : i + n * (j + n * k);
-  const int half = D == 2 ? (n * n) >> 1 : (n * n * n) >> 1;
-  return (((i + j + k) & 1) ? half : 0) + (lex >> 1);
 }
 
 __device__ __forceinline__ double* boxp(const Dev& a, int iv, int id) {
@@ -86,7 +74,7 @@ __device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, 
   if (nb_id > 0) {
     int s[3] = {gi, gj, gk};
     s[d] = low ? nc : 1;
-    return boxp_t<T>(a, iv, nb_id)[cidx<D>(nc, s[0], s[1], s[2])];
+    return boxp_t<T>(a, iv, nb_id)[cidx_in<D>(nc, s[0], s[1], s[2])];
   } else if (nb_id == 0) {
     const int pid = a.parent[id];
     const int pnc = a.ncell[pid];
@@ -96,7 +84,7 @@ __device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, 
 #pragma unroll
     for (int e = 0; e < 3; e++) q[e] = (e < D) ? off[e] + ((g[e] + 1) >> 1) : 0;
     const T P = boxp_t<T>(a, iv, pid)[cidx<D>(pnc, q[0], q[1], q[2])];
-    const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
+    const T cv = p[cidx_in<D>(nc, c[0], c[1], c[2])];
     // shifted interior cells, in the order dim 0, 1, 2 of the reference
     T t[3];
 #pragma unroll
@@ -104,12 +92,12 @@ __device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, 
       int s[3] = {c[0], c[1], c[2]};
       if (e == d) s[e] += low ? 1 : -1;
       else s[e] += -1 + 2 * (s[e] & 1);
-      t[e] = p[cidx<D>(nc, s[0], s[1], s[2])];
+      t[e] = p[cidx_in<D>(nc, s[0], s[1], s[2])];
     }
     if (D == 2) return T(0.5) * P + cv - T(0.25) * (t[0] + t[1]);
     return T(0.5) * P + T(1.25) * cv - T(0.25) * (t[0] + t[1] + t[2]);
   } else {
-    const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
+    const T cv = p[cidx_in<D>(nc, c[0], c[1], c[2])];
     const int kind = a.bc_kind[nb - 1];
     if (kind == 0) return cv;
     if (kind == 1) return -cv;

@@ -226,68 +226,14 @@ __device__ __forceinline__ int xcd_block() {
   return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
-// Copies N values from global memory to LDS with TPB lanes, issuing up to
-// eight loads per lane before the first LDS store so that they are in flight
-// together.
-template <int N, int TPB, typename T>
-__device__ __forceinline__ void stage_lds(T* __restrict__ s, const T* __restrict__ g, int lt) {
-  constexpr int NIT = (N + TPB - 1) / TPB;
-  constexpr int B = NIT < 8 ? NIT : 8;
-#pragma unroll
-  for (int i0 = 0; i0 < NIT; i0 += B) {
-    T v[B];
-#pragma unroll
-    for (int u = 0; u < B; u++) {
-      const int q = lt + (i0 + u) * TPB;
-      if (i0 + u < NIT && q < N) v[u] = g[q];
-    }
-#pragma unroll
-    for (int u = 0; u < B; u++) {
-      const int q = lt + (i0 + u) * TPB;
-      if (i0 + u < NIT && q < N) s[q] = v[u];
-    }
-  }
-}
-
-// Copies, for one colour half of a box of NC^D cells, the slots of the
-// interior rows (j, k in 1..NC; each row's NC/2 + 1 slots of that colour,
-// one of them an x ghost slot) from global memory to LDS at the same
-// positions. The y and z ghost rows are left out: callers recompute every
-// face ghost slot they read.
-template <int D, int NC, int TPB, typename T>
-__device__ __forceinline__ void stage_rows(T* __restrict__ s, const T* __restrict__ g, int lt) {
-  constexpr int N2 = NC + 2, RW = N2 / 2;
-  constexpr int NR = D == 2 ? NC : NC * NC;
-  constexpr int N = NR * RW;
-  constexpr int NIT = (N + TPB - 1) / TPB;
-  constexpr int B = NIT < 8 ? NIT : 8;
-#pragma unroll
-  for (int i0 = 0; i0 < NIT; i0 += B) {
-    T v[B];
-    int at[B];
-#pragma unroll
-    for (int u = 0; u < B; u++) {
-      const int q = lt + (i0 + u) * TPB;
-      at[u] = -1;
-      if (i0 + u < NIT && q < N) {
-        const int row = q / RW, m = q - row * RW;
-        const int j = row % NC + 1, k = D == 3 ? row / NC + 1 : 0;
-        at[u] = ((N2 * (j + N2 * k)) >> 1) + m;
-        v[u] = g[at[u]];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < B; u++)
-      if (at[u] >= 0) s[at[u]] = v[u];
-  }
-}
-
-// Box-size-specialised half-sweep. The other colour's half of the box (ghost
-// slots included) is staged in LDS; when fused, that colour's face ghost
-// slots are first recomputed from the current state (the values the
-// reference's ghost pass after the previous half-sweep would store). Then
-// every lane updates cells of colour cntr&1 from LDS and rhs (loaded into
-// registers before the barrier).
+// Box-size-specialised half-sweep. The other colour's interior cells and
+// face ghosts are staged in LDS, in an order used only there: slot
+// lex(i, j, k) / 2 of the whole (NC+2)^D block, one colour, in which the
+// neighbours of a cell are fixed offsets. The ghosts are the stored ones on
+// the first half-sweep of a call; later half-sweeps recompute them from the
+// current state (the values the reference's ghost pass after the previous
+// half-sweep would store). Then every lane updates cells of colour cntr&1
+// from LDS and rhs (loaded into registers before the barrier).
 template <int D, bool CYL, int NC, typename T, bool EPS>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
@@ -298,9 +244,11 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   constexpr int TPB = NCOL > 128 ? 256 : (NCOL > 64 ? 128 : 64);
   constexpr int BPB = 256 / TPB;
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
-  constexpr int NGH = 2 * D * PER_FACE;
   constexpr int HNC = NC / 2;
   constexpr int NIT = (NCOL + TPB - 1) / TPB;
+  constexpr int NGC = D * PER_FACE;  // face ghosts of one colour
+  constexpr int NITG = (NGC + TPB - 1) / TPB;
+  constexpr int GPF = PER_FACE / 2;
   __shared__ T sh[BPB * HALF];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
   const int li = xcd_block() * BPB + slot;
@@ -309,33 +257,20 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   const int c = cntr & 1, oc = c ^ 1;
   T* s = sh + slot * HALF;
   T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
-  // lex index of the q-th cell of colour c
-  auto lex_of = [&](int q, int& i) {
+  // the cell of colour col in interior pair q (the q-th interior cell of
+  // that colour in HBM); returns its lexicographic index in the block
+  auto cell_of = [&](int q, int col, int& i, int& j, int& k) {
     const int mm = q % HNC, row = q / HNC;
-    const int j = row % NC + 1;
-    const int k = D == 3 ? row / NC + 1 : 0;
-    i = 2 * mm + 2 - ((c + j + k) & 1);
+    j = row % NC + 1;
+    k = D == 3 ? row / NC + 1 : 0;
+    i = 2 * mm + 2 - ((col + j + k) & 1);
     return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
   };
   T rv[NIT];
-  if (act && !fused) {
-    // the first half-sweep of a call reads the stored ghost layer
-    stage_lds<HALF, TPB>(s, p + oc * HALF, lt);
-    const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
-#pragma unroll
-    for (int u = 0; u < NIT; u++) {
-      const int q = lt + u * TPB;
-      int i;
-      if (q < NCOL) rv[u] = r[lex_of(q, i) >> 1];
-    }
-  } else if (act) {
-    // A fused half-sweep recomputes the face ghosts of the other colour.
+  if (act) {
     // Every global load (interior cells of colour oc, rhs of colour c, the
-    // neighbour cells that the ghosts copy) is issued before any LDS store;
-    // the staged slots and the ghost slots are disjoint.
-    constexpr int NGC = NGH / 2;  // face ghosts of one colour
-    constexpr int NITG = (NGC + TPB - 1) / TPB;
-    constexpr int GPF = PER_FACE / 2;
+    // ghost values or the neighbour cells they copy) is issued before any
+    // LDS store; the interior and ghost slots are disjoint.
     T sv[NIT];
     int sa[NIT];
     const T* po = p + oc * HALF;
@@ -344,20 +279,16 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
       const int q = lt + u * TPB;
       sa[u] = -1;
       if (q < NCOL) {
-        const int mm = q % HNC, row = q / HNC;
-        const int j = row % NC + 1;
-        const int k = D == 3 ? row / NC + 1 : 0;
-        const int i = 2 * mm + 2 - ((oc + j + k) & 1);
-        sa[u] = (D == 2 ? i + N2 * j : i + N2 * (j + N2 * k)) >> 1;
-        sv[u] = po[sa[u]];
+        int i, j, k;
+        sa[u] = cell_of(q, oc, i, j, k) >> 1;
+        sv[u] = po[q];
       }
     }
     const T* r = boxp_t<T>(a, i_rhs, id) + c * HALF;
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
       const int q = lt + u * TPB;
-      int i;
-      if (q < NCOL) rv[u] = r[lex_of(q, i) >> 1];
+      if (q < NCOL) rv[u] = r[q];
     }
     T gv[NITG];
     int gs[NITG], gnb[NITG], gg[NITG][3];
@@ -378,14 +309,20 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
         if (D == 3) g[tb] = y;
         g[ta] = 2 * w + 2 - ((oc + g[d] + y) & 1);
         gs[u] = (D == 2 ? g[0] + N2 * g[1] : g[0] + N2 * (g[1] + N2 * g[2])) >> 1;
-        gnb[u] = nb;
+        gnb[u] = 0;
         gg[u][0] = g[0]; gg[u][1] = g[1]; gg[u][2] = g[2];
-        const int nb_id = a.nbs[id * 6 + nb - 1];
-        if (nb_id > 0) {
-          int sc[3] = {g[0], g[1], g[2]};
-          sc[d] = d_nb_low(nb) ? NC : 1;
-          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx<D>(NC, sc[0], sc[1], sc[2])];
-          gnb[u] = 0;
+        if (!fused) {
+          // the first half-sweep of a call reads the stored ghost layer
+          gv[u] = p[cidx<D>(NC, g[0], g[1], g[2])];
+        } else {
+          const int nb_id = a.nbs[id * 6 + nb - 1];
+          if (nb_id > 0) {
+            int sc[3] = {g[0], g[1], g[2]};
+            sc[d] = d_nb_low(nb) ? NC : 1;
+            gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx_in<D>(NC, sc[0], sc[1], sc[2])];
+          } else {
+            gnb[u] = nb;
+          }
         }
       }
     }
@@ -406,29 +343,26 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   const T dx2 = (T)(a.dr[id] * a.dr[id]);
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
   // boxes tagged constant- or variable-eps (mg$D_auto_gsrb,
-  // m_aX_mg.f90:568-600); eps is read in its own (colour-split) order
+  // m_aX_mg.f90:568-600); eps is read from HBM
   const double* ev =
       (EPS && a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
 #pragma unroll
   for (int u = 0; u < NIT; u++) {
     const int q = lt + u * TPB;
     if (q >= NCOL) break;
-    int i;
-    const int lex = lex_of(q, i);
-    const int x = lex >> 1;
+    int i, j, k;
+    const int lex = cell_of(q, c, i, j, k);
     const T xp = s[(lex + 1) >> 1], xm = s[(lex - 1) >> 1];
     const T yp = s[(lex + N2) >> 1], ym = s[(lex - N2) >> 1];
     T v;
     if (EPS && ev != nullptr) {
-      // the neighbours have the other colour: second half of the box
-      const double* eo = ev + oc * HALF;
       T uu[6], am[6];
       uu[0] = xm; uu[1] = xp; uu[2] = ym; uu[3] = yp;
-      am[0] = (T)eo[(lex - 1) >> 1]; am[1] = (T)eo[(lex + 1) >> 1];
-      am[2] = (T)eo[(lex - N2) >> 1]; am[3] = (T)eo[(lex + N2) >> 1];
+      am[0] = (T)ev[cidx<D>(NC, i - 1, j, k)]; am[1] = (T)ev[cidx<D>(NC, i + 1, j, k)];
+      am[2] = (T)ev[cidx<D>(NC, i, j - 1, k)]; am[3] = (T)ev[cidx<D>(NC, i, j + 1, k)];
       if (D == 3) {
         uu[4] = s[(lex - N2 * N2) >> 1]; uu[5] = s[(lex + N2 * N2) >> 1];
-        am[4] = (T)eo[(lex - N2 * N2) >> 1]; am[5] = (T)eo[(lex + N2 * N2) >> 1];
+        am[4] = (T)ev[cidx<D>(NC, i, j, k - 1)]; am[5] = (T)ev[cidx<D>(NC, i, j, k + 1)];
       }
       T rf1 = T(1), rf2 = T(1);
       if (CYL) {
@@ -436,7 +370,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
         rf1 = (T)((double)(i + ioff - 1) / den);
         rf2 = (T)((double)(i + ioff) / den);
       }
-      v = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c * HALF + x], dx2, rv[u], rf1, rf2);
+      v = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c * HALF + q], dx2, rv[u], rf1, rf2);
     } else if (D == 2) {
       if (CYL) {
         const double den = i + ioff - 0.5;
@@ -451,7 +385,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
       const T sixth = T(1) / T(6);
       v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * rv[u]);
     }
-    pc[x] = v;
+    pc[q] = v;
   }
 }
 
