@@ -61,10 +61,11 @@ __device__ __forceinline__ void child_off(const Dev& a, int id, int nc, int nb, 
 // refinement-boundary extrapolation (mg$D_sides_rb, m_aX_mg.f90:262-356,
 // with a$D_gc_prolong0 / a$D_prolong0_to, m_aX_gc.f90:71-86,
 // m_aX_prolong.f90:37-78) or the physical rule (copy, -c, 2V - c).
-// `p` is the box's own data; only interior cells of p are read.
-template <int D, typename T = double>
-__device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, int nc, int nb,
-                                      int gi, int gj, int gk) {
+// own(i, j, k) gives the box's own interior cells (from HBM or from an LDS
+// copy); neighbours and the parent are read from HBM.
+template <int D, typename T, typename U>
+__device__ __forceinline__ T ghost_mg_u(const Dev& a, U own, int iv, int id, int nc, int nb,
+                                        int gi, int gj, int gk) {
   const int nb_id = a.nbs[id * 6 + nb - 1];
   const int d = d_nb_dim(nb);
   int g[3] = {gi, gj, gk};
@@ -84,7 +85,7 @@ __device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, 
 #pragma unroll
     for (int e = 0; e < 3; e++) q[e] = (e < D) ? off[e] + ((g[e] + 1) >> 1) : 0;
     const T P = boxp_t<T>(a, iv, pid)[cidx<D>(pnc, q[0], q[1], q[2])];
-    const T cv = p[cidx_in<D>(nc, c[0], c[1], c[2])];
+    const T cv = own(c[0], c[1], c[2]);
     // shifted interior cells, in the order dim 0, 1, 2 of the reference
     T t[3];
 #pragma unroll
@@ -92,17 +93,25 @@ __device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, 
       int s[3] = {c[0], c[1], c[2]};
       if (e == d) s[e] += low ? 1 : -1;
       else s[e] += -1 + 2 * (s[e] & 1);
-      t[e] = p[cidx_in<D>(nc, s[0], s[1], s[2])];
+      t[e] = own(s[0], s[1], s[2]);
     }
     if (D == 2) return T(0.5) * P + cv - T(0.25) * (t[0] + t[1]);
     return T(0.5) * P + T(1.25) * cv - T(0.25) * (t[0] + t[1] + t[2]);
   } else {
-    const T cv = p[cidx_in<D>(nc, c[0], c[1], c[2])];
+    const T cv = own(c[0], c[1], c[2]);
     const int kind = a.bc_kind[nb - 1];
     if (kind == 0) return cv;
     if (kind == 1) return -cv;
     return T(2) * (T)a.bc_val[nb - 1] - cv;
   }
+}
+
+// ghost_mg_u with the box's own cells in HBM (or an LDS copy in cidx order)
+template <int D, typename T = double>
+__device__ __forceinline__ T ghost_mg(const Dev& a, const T* p, int iv, int id, int nc, int nb,
+                                      int gi, int gj, int gk) {
+  return ghost_mg_u<D, T>(a, [&](int x, int y, int z) { return p[cidx_in<D>(nc, x, y, z)]; },
+                          iv, id, nc, nb, gi, gj, gk);
 }
 
 }  // namespace a5g

@@ -60,39 +60,40 @@ static Geo geo(int n_boxes, int work) {
 // mg2_box_clpl (:1442-1464)
 // ---------------------------------------------------------------------------
 
-template <int D, bool CYL, typename T>
-__device__ __forceinline__ T lpl_at(const T* p, int nc, int i, int j, int k, T inv_dr_sq,
-                                    int ioff) {
+// The operators read the potential through u(i, j, k), so that a kernel can
+// take it from HBM (cidx order) or from an LDS copy in another order.
+template <int D, bool CYL, typename T, typename U>
+__device__ __forceinline__ T lpl_u(U u, int i, int j, int k, T inv_dr_sq, int ioff) {
   if (D == 2) {
     if (CYL) {
       const double den = i + ioff - 0.5;
       const T rf1 = (T)((double)(i + ioff - 1) / den);
       const T rf2 = (T)((double)(i + ioff) / den);
-      return (rf1 * p[cidx<D>(nc, i - 1, j, 0)] + rf2 * p[cidx<D>(nc, i + 1, j, 0)] +
-              p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-              T(4) * p[cidx<D>(nc, i, j, 0)]) * inv_dr_sq;
+      return (rf1 * u(i - 1, j, 0) + rf2 * u(i + 1, j, 0) + u(i, j - 1, 0) + u(i, j + 1, 0) -
+              T(4) * u(i, j, 0)) * inv_dr_sq;
     }
-    return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, 0)] + p[cidx<D>(nc, i + 1, j, 0)] +
-                        p[cidx<D>(nc, i, j - 1, 0)] + p[cidx<D>(nc, i, j + 1, 0)] -
-                        T(4) * p[cidx<D>(nc, i, j, 0)]);
+    return inv_dr_sq * (u(i - 1, j, 0) + u(i + 1, j, 0) + u(i, j - 1, 0) + u(i, j + 1, 0) -
+                        T(4) * u(i, j, 0));
   }
-  return inv_dr_sq * (p[cidx<D>(nc, i - 1, j, k)] + p[cidx<D>(nc, i + 1, j, k)] +
-                      p[cidx<D>(nc, i, j - 1, k)] + p[cidx<D>(nc, i, j + 1, k)] +
-                      p[cidx<D>(nc, i, j, k - 1)] + p[cidx<D>(nc, i, j, k + 1)] -
-                      T(6) * p[cidx<D>(nc, i, j, k)]);
+  return inv_dr_sq * (u(i - 1, j, k) + u(i + 1, j, k) + u(i, j - 1, k) + u(i, j + 1, k) +
+                      u(i, j, k - 1) + u(i, j, k + 1) - T(6) * u(i, j, k));
+}
+
+template <int D, bool CYL, typename T>
+__device__ __forceinline__ T lpl_at(const T* p, int nc, int i, int j, int k, T inv_dr_sq,
+                                    int ioff) {
+  return lpl_u<D, CYL, T>([&](int x, int y, int z) { return p[cidx<D>(nc, x, y, z)]; }, i, j,
+                          k, inv_dr_sq, ioff);
 }
 
 // Operator with a permittivity eps: mg$D_box_lpld (m_aX_mg.f90:951-1001)
 // and mg2_box_clpld (:1467-1495) at interior cell (i, j, k). Neighbours in the
 // order i-1, i+1, j-1, j+1, k-1, k+1 of the reference's u(:) and a(:).
-template <int D, bool CYL, typename T>
-__device__ __forceinline__ T lpld_at(const double* e, const T* p, int nc, int i, int j, int k,
-                                     T inv_dr_sq, int ioff) {
-  const int c0 = cidx<D>(nc, i, j, k);
-  const int nb[6] = {cidx<D>(nc, i - 1, j, k), cidx<D>(nc, i + 1, j, k),
-                     cidx<D>(nc, i, j - 1, k), cidx<D>(nc, i, j + 1, k),
-                     D == 3 ? cidx<D>(nc, i, j, k - 1) : 0, D == 3 ? cidx<D>(nc, i, j, k + 1) : 0};
-  const T u0 = p[c0], a0 = (T)e[c0];
+template <int D, bool CYL, typename T, typename U>
+__device__ __forceinline__ T lpld_u(const double* e, int nc, U u, int i, int j, int k,
+                                    T inv_dr_sq, int ioff) {
+  const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+  const T u0 = u(i, j, k), a0 = (T)e[cidx_in<D>(nc, i, j, k)];
   T rf[2] = {T(1), T(1)};
   if (CYL) {
     const double den = i + ioff - 0.5;
@@ -102,9 +103,10 @@ __device__ __forceinline__ T lpld_at(const double* e, const T* p, int nc, int i,
   T s = 0;
 #pragma unroll
   for (int m = 0; m < 2 * D; m++) {
-    const T am = (T)e[nb[m]];
-    if (CYL && m < 2) s = s + rf[m] * a0 * am / (a0 + am) * (p[nb[m]] - u0);
-    else s = s + a0 * am / (a0 + am) * (p[nb[m]] - u0);
+    const T am = (T)e[cidx<D>(nc, i + di[m], j + dj[m], k + dk[m])];
+    const T um = u(i + di[m], j + dj[m], k + dk[m]);
+    if (CYL && m < 2) s = s + rf[m] * a0 * am / (a0 + am) * (um - u0);
+    else s = s + a0 * am / (a0 + am) * (um - u0);
   }
   return inv_dr_sq * T(2) * s;
 }
@@ -112,12 +114,19 @@ __device__ __forceinline__ T lpld_at(const double* e, const T* p, int nc, int i,
 // The multigrid operator of box id: the permittivity form for boxes tagged
 // constant- or variable-eps (mg$D_auto_op, m_aX_mg.f90:603-635), else the
 // Laplacian.
+template <int D, bool CYL, typename T, typename U>
+__device__ __forceinline__ T op_u(const Dev& a, int id, int nc, U u, int i, int j, int k,
+                                  T inv_dr_sq, int ioff) {
+  if (a.tag != nullptr && a.tag[id] > kTagNormal)
+    return lpld_u<D, CYL, T>(boxp(a, a.i_eps, id), nc, u, i, j, k, inv_dr_sq, ioff);
+  return lpl_u<D, CYL, T>(u, i, j, k, inv_dr_sq, ioff);
+}
+
 template <int D, bool CYL, typename T>
 __device__ __forceinline__ T op_at(const Dev& a, int id, const T* p, int nc, int i, int j, int k,
                                    T inv_dr_sq, int ioff) {
-  if (a.tag != nullptr && a.tag[id] > kTagNormal)
-    return lpld_at<D, CYL, T>(boxp(a, a.i_eps, id), p, nc, i, j, k, inv_dr_sq, ioff);
-  return lpl_at<D, CYL, T>(p, nc, i, j, k, inv_dr_sq, ioff);
+  return op_u<D, CYL, T>(a, id, nc, [&](int x, int y, int z) { return p[cidx<D>(nc, x, y, z)]; },
+                         i, j, k, inv_dr_sq, ioff);
 }
 
 // One Gauss-Seidel update with a permittivity: mg$D_box_gsrb_lpld
@@ -431,6 +440,75 @@ static bool launch_gsrb_lds(Pool& P, const Dev& a, const int* list, int n, int n
 // 2 Dirichlet zero (:333-365).
 // ---------------------------------------------------------------------------
 
+// A ghost value of k_gc for the faces without a same-level neighbour
+// (nb_id <= 0): refinement boundary or physical boundary.
+template <int D, typename T>
+__device__ __forceinline__ T gc_boundary(const Dev& a, const T* p, int iv, int id, int nc, int nb,
+                                         const int g[3], int rb, int bc) {
+  const int nb_id = a.nbs[id * 6 + nb - 1];
+  const int d = d_nb_dim(nb);
+  const bool low = d_nb_low(nb);
+  const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+  const T sixth = T(1) / T(6), third = T(1) / T(3);
+  if ((nb_id == 0 && rb == 0) || (nb_id < 0 && bc == 0))
+    return ghost_mg<D, T>(a, p, iv, id, nc, nb, g[0], g[1], g[2]);
+  if (nb_id < 0) {
+    int c[3] = {g[0], g[1], g[2]};
+    c[d] = low ? 1 : nc;
+    const T cv = p[cidx_in<D>(nc, c[0], c[1], c[2])];
+    return bc == 1 ? cv : -cv;
+  }
+  // coarse-neighbour interpolation
+  const int pid = a.parent[id];
+  const int pn = a.nbs[pid * 6 + nb - 1];
+  const int pnc = a.ncell[pn];
+  int off[3];
+  child_off<D>(a, id, nc, nb, off);
+  const int ixc = low ? nc : 1, ixf = low ? 1 : nc;
+  const T* qv = boxp_t<T>(a, iv, pn);
+  int f[3] = {g[0], g[1], g[2]};
+  f[d] = ixf;
+  const T fv = p[cidx_in<D>(nc, f[0], f[1], f[2])];
+  T v;
+  if (D == 2) {
+    const int x = g[ta];
+    const int c1 = off[ta] + ((x + 1) >> 1), c2 = c1 + 1 - 2 * (x & 1);
+    int s1[3], s2[3];
+    s1[d] = s2[d] = ixc;
+    s1[ta] = c1;
+    s2[ta] = c2;
+    const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], 0)];
+    const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], 0)];
+    v = T(0.5) * v1 + sixth * v2 + third * fv;
+    if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
+  } else {
+    // the two tangential shifts enter in the order of the reference:
+    // x-faces vary j then k, y-faces i then k, z-faces j then i
+    const int x = g[ta], y = g[tb];
+    const int a1 = off[ta] + ((x + 1) >> 1), a2 = a1 + 1 - 2 * (x & 1);
+    const int b1 = off[tb] + ((y + 1) >> 1), b2 = b1 + 1 - 2 * (y & 1);
+    int s1[3], s2[3], s3[3];
+    s1[d] = s2[d] = s3[d] = ixc;
+    s1[ta] = a1; s1[tb] = b1;
+    if (d == 2) {
+      s2[ta] = a1; s2[tb] = b2;
+      s3[ta] = a2; s3[tb] = b1;
+    } else {
+      s2[ta] = a2; s2[tb] = b1;
+      s3[ta] = a1; s3[tb] = b2;
+    }
+    const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], s1[2])];
+    const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], s2[2])];
+    const T v3 = qv[cidx<D>(pnc, s3[0], s3[1], s3[2])];
+    v = third * v1 + sixth * v2 + sixth * v3 + third * fv;
+    if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
+  }
+  return v;
+}
+
+// Every ghost read here is an interior cell (of the box, a neighbour, the
+// parent or its neighbour) and every write a ghost slot of the box, so the
+// neighbour copies of four ghosts per lane are loaded before any store.
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list, int n,
                                             int nc, int tpb, int iv, int rb, int bc) {
@@ -438,73 +516,44 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
   T* p = boxp_t<T>(a, iv, id);
   const int per_face = D == 2 ? nc : nc * nc;
   const int total = 2 * D * per_face;
-  const T sixth = T(1) / T(6), third = T(1) / T(3);
-  for (int q = lt; q < total; q += tpb) {
-    const int nb = q / per_face + 1;
-    const int t = q - (nb - 1) * per_face;
-    const int d = d_nb_dim(nb);
-    const bool low = d_nb_low(nb);
-    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    int g[3] = {0, 0, 0};
-    g[d] = low ? 0 : nc + 1;
-    g[ta] = t % nc + 1;
-    if (D == 3) g[tb] = t / nc + 1;
-    const int nb_id = a.nbs[id * 6 + nb - 1];
-    T v;
-    if (nb_id > 0 || (nb_id == 0 && rb == 0) || (nb_id < 0 && bc == 0)) {
-      v = ghost_mg<D, T>(a, p, iv, id, nc, nb, g[0], g[1], g[2]);
-    } else if (nb_id < 0) {
-      int c[3] = {g[0], g[1], g[2]};
-      c[d] = low ? 1 : nc;
-      const T cv = p[cidx<D>(nc, c[0], c[1], c[2])];
-      v = bc == 1 ? cv : -cv;
-    } else {
-      // coarse-neighbour interpolation
-      const int pid = a.parent[id];
-      const int pn = a.nbs[pid * 6 + nb - 1];
-      const int pnc = a.ncell[pn];
-      int off[3];
-      child_off<D>(a, id, nc, nb, off);
-      const int ixc = low ? nc : 1, ixf = low ? 1 : nc;
-      const T* qv = boxp_t<T>(a, iv, pn);
-      int f[3] = {g[0], g[1], g[2]};
-      f[d] = ixf;
-      const T fv = p[cidx<D>(nc, f[0], f[1], f[2])];
-      if (D == 2) {
-        const int x = g[ta];
-        const int c1 = off[ta] + ((x + 1) >> 1), c2 = c1 + 1 - 2 * (x & 1);
-        int s1[3], s2[3];
-        s1[d] = s2[d] = ixc;
-        s1[ta] = c1;
-        s2[ta] = c2;
-        const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], 0)];
-        const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], 0)];
-        v = T(0.5) * v1 + sixth * v2 + third * fv;
-        if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
-      } else {
-        // the two tangential shifts enter in the order of the reference:
-        // x-faces vary j then k, y-faces i then k, z-faces j then i
-        const int x = g[ta], y = g[tb];
-        const int a1 = off[ta] + ((x + 1) >> 1), a2 = a1 + 1 - 2 * (x & 1);
-        const int b1 = off[tb] + ((y + 1) >> 1), b2 = b1 + 1 - 2 * (y & 1);
-        int s1[3], s2[3], s3[3];
-        s1[d] = s2[d] = s3[d] = ixc;
-        s1[ta] = a1; s1[tb] = b1;
-        if (d == 2) {
-          s2[ta] = a1; s2[tb] = b2;
-          s3[ta] = a2; s3[tb] = b1;
+  constexpr int B = 4;
+  for (int q0 = lt; q0 < total; q0 += B * tpb) {
+    T v[B];
+    int x[B], gg[B][3], bnb[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int q = q0 + u * tpb;
+      x[u] = -1;
+      bnb[u] = 0;
+      if (q < total) {
+        const int nb = q / per_face + 1;
+        const int t = q - (nb - 1) * per_face;
+        const int d = d_nb_dim(nb);
+        const bool low = d_nb_low(nb);
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        int g[3] = {0, 0, 0};
+        g[d] = low ? 0 : nc + 1;
+        g[ta] = t % nc + 1;
+        if (D == 3) g[tb] = t / nc + 1;
+        x[u] = cidx<D>(nc, g[0], g[1], g[2]);
+        gg[u][0] = g[0]; gg[u][1] = g[1]; gg[u][2] = g[2];
+        const int nb_id = a.nbs[id * 6 + nb - 1];
+        if (nb_id > 0) {
+          // sides_from_nb (m_aX_gc.f90:368-402)
+          int s[3] = {g[0], g[1], g[2]};
+          s[d] = low ? nc : 1;
+          v[u] = boxp_t<T>(a, iv, nb_id)[cidx_in<D>(nc, s[0], s[1], s[2])];
         } else {
-          s2[ta] = a2; s2[tb] = b1;
-          s3[ta] = a1; s3[tb] = b2;
+          bnb[u] = nb;
         }
-        const T v1 = qv[cidx<D>(pnc, s1[0], s1[1], s1[2])];
-        const T v2 = qv[cidx<D>(pnc, s2[0], s2[1], s2[2])];
-        const T v3 = qv[cidx<D>(pnc, s3[0], s3[1], s3[2])];
-        v = third * v1 + sixth * v2 + sixth * v3 + third * fv;
-        if (rb == 2 && v > T(2) * v1) v = T(2) * v1;
       }
     }
-    p[cidx<D>(nc, g[0], g[1], g[2])] = v;
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      if (x[u] < 0) continue;
+      if (bnb[u] > 0) v[u] = gc_boundary<D, T>(a, p, iv, id, nc, bnb[u], gg[u], rb, bc);
+      p[x[u]] = v[u];
+    }
   }
 }
 
@@ -560,18 +609,21 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
   }
 }
 
-// The same pass with the child's potential staged in LDS and its face ghost
-// slots recomputed there from the current state (the values the ghost pass
-// that ends the reference's smoothing, fill_gc_phi after gsrb_boxes, would
-// store: nothing changes the child, its neighbours or the uncovered parent
-// cells in between), so that pass is not needed before this one.
+// The same pass with the child's potential staged in LDS, in lexicographic
+// order of the whole (NC+2)^D block, and its face ghosts recomputed there
+// from the current state (the values the ghost pass that ends the
+// reference's smoothing, fill_gc_phi after gsrb_boxes, would store: nothing
+// changes the child, its neighbours or the uncovered parent cells in
+// between), so that pass is not needed before this one.
 template <int D, bool CYL, int NC, typename T, bool EPS>
 __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
                                                         int keep_res) {
   constexpr int N2 = NC + 2;
   constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
+  constexpr int HALF = BL / 2;
   constexpr int HNC = NC / 2;
+  constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
   constexpr int TOTAL = D == 2 ? HNC * HNC : HNC * HNC * HNC;
   constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
   constexpr int BPB = 256 / TPB;
@@ -582,14 +634,16 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   const int li = xcd_block() * BPB + slot;
   const bool act = li < n;
   const int id = act ? list[li] : 0;
-  constexpr int NCELL = D == 2 ? NC * NC : NC * NC * NC;
+  constexpr int NCELL = 2 * NCOL;
   constexpr int NITC = (NCELL + TPB - 1) / TPB;
   constexpr int NITG = (NGH + TPB - 1) / TPB;
   T* s = sh + slot * BL;
+  auto L = [](int i, int j, int k) { return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k); };
+  auto own = [&](int i, int j, int k) { return s[L(i, j, k)]; };
   // Face ghosts that copy a neighbour's cell are loaded first, then the
-  // interior cells (both colours) in batches; all these LDS slots are
-  // disjoint. Ghosts of the other rules read the staged interior, after a
-  // barrier.
+  // interior cells (each colour a contiguous run in HBM) in batches; all
+  // these LDS slots are disjoint. Ghosts of the other rules read the staged
+  // interior, after a barrier.
   int gq[NITG];
   if (act) {
     const T* p = boxp_t<T>(a, i_phi, id);
@@ -613,8 +667,8 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
         if (nb_id > 0) {
           int sc[3] = {g[0], g[1], g[2]};
           sc[d] = d_nb_low(nb) ? NC : 1;
-          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx<D>(NC, sc[0], sc[1], sc[2])];
-          gs[u] = cidx<D>(NC, g[0], g[1], g[2]);
+          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx_in<D>(NC, sc[0], sc[1], sc[2])];
+          gs[u] = L(g[0], g[1], g[2]);
         } else {
           gq[u] = q;
         }
@@ -627,12 +681,16 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
       int at[B];
 #pragma unroll
       for (int u = 0; u < B; u++) {
-        const int q = lt + (i0 + u) * TPB;
+        const int e = lt + (i0 + u) * TPB;
         at[u] = -1;
-        if (i0 + u < NITC && q < NCELL) {
-          const int i = q % NC + 1, j = (q / NC) % NC + 1, k = D == 3 ? q / (NC * NC) + 1 : 0;
-          at[u] = cidx<D>(NC, i, j, k);
-          v[u] = p[at[u]];
+        if (i0 + u < NITC && e < NCELL) {
+          // the cell of colour col in interior pair q
+          const int col = e >= NCOL, q = e - col * NCOL;
+          const int mm = q % HNC, row = q / HNC;
+          const int j = row % NC + 1, k = D == 3 ? row / NC + 1 : 0;
+          const int i = 2 * mm + 2 - ((col + j + k) & 1);
+          at[u] = L(i, j, k);
+          v[u] = p[col * HALF + q];
         }
       }
 #pragma unroll
@@ -658,7 +716,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
       g[ta] = t % NC + 1;
       if (D == 3) g[tb] = t / NC + 1;
       // reads interior cells of s only, writes a ghost slot
-      s[cidx<D>(NC, g[0], g[1], g[2])] = ghost_mg<D, T>(a, s, i_phi, id, NC, nb, g[0], g[1], g[2]);
+      s[L(g[0], g[1], g[2])] = ghost_mg_u<D, T>(a, own, i_phi, id, NC, nb, g[0], g[1], g[2]);
     }
   }
   __syncthreads();
@@ -686,14 +744,14 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
 #pragma unroll
         for (int di = 0; di < 2; di++) {
           const int ii = fi + di, jj = fj + dj, kk = fk + dk;
-          const int c = cidx<D>(NC, ii, jj, kk);
-          const T res = r[c] - (EPS ? op_at<D, CYL, T>(a, id, s, NC, ii, jj, kk, inv_dr_sq, ioff)
-                                    : lpl_at<D, CYL, T>(s, NC, ii, jj, kk, inv_dr_sq, ioff));
+          const int c = cidx_in<D>(NC, ii, jj, kk);
+          const T res = r[c] - (EPS ? op_u<D, CYL, T>(a, id, NC, own, ii, jj, kk, inv_dr_sq, ioff)
+                                    : lpl_u<D, CYL, T>(own, ii, jj, kk, inv_dr_sq, ioff));
           if (keep_res) tv[c] = res;
           sr = sr + res;
-          sp = sp + s[c];
+          sp = sp + own(ii, jj, kk);
         }
-    const int pc = cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
+    const int pc = cidx_in<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
     const T w = D == 2 ? T(0.25) : T(0.125);
     pt[pc] = w * sr;
     pp[pc] = w * sp;
@@ -811,6 +869,105 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
     rv = rv + t[x];
     r[x] = rv;
     if (copy_tmp) t[x] = p[x];
+  }
+}
+
+// The same pass with phi staged in LDS (lexicographic order, interior and
+// face ghosts) for the operator; the interior and the ghost slots of each
+// colour are visited as contiguous runs.
+template <int D, bool CYL, int NC, typename T, bool EPS>
+__global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __restrict__ list,
+                                                        int n, int i_phi, int i_rhs, int i_tmp,
+                                                        int copy_tmp) {
+  constexpr int N2 = NC + 2;
+  constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
+  constexpr int HALF = BL / 2;
+  constexpr int HNC = NC / 2;
+  constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
+  constexpr int NGC = HALF - NCOL;  // ghost slots of one colour
+  constexpr int PER_FACE = D == 2 ? NC : NC * NC;
+  constexpr int NGH = 2 * D * PER_FACE;
+  constexpr int TPB = BL > 128 ? 256 : (BL > 64 ? 128 : 64);
+  constexpr int BPB = 256 / TPB;
+  __shared__ T sh[BPB * BL];
+  const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
+  const int li = xcd_block() * BPB + slot;
+  const bool act = li < n;
+  const int id = act ? list[li] : 0;
+  T* s = sh + slot * BL;
+  auto L = [](int i, int j, int k) { return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k); };
+  auto own = [&](int i, int j, int k) { return s[L(i, j, k)]; };
+  auto cell = [](int e, int& i, int& j, int& k) {
+    const int col = e >= NCOL, q = e - col * NCOL;
+    const int mm = q % HNC, row = q / HNC;
+    j = row % NC + 1;
+    k = D == 3 ? row / NC + 1 : 0;
+    i = 2 * mm + 2 - ((col + j + k) & 1);
+    return col * HALF + q;
+  };
+  const T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
+  if (act) {
+    for (int e = lt; e < 2 * NCOL; e += TPB) {
+      int i, j, k;
+      const int x = cell(e, i, j, k);
+      s[L(i, j, k)] = p[x];
+    }
+    for (int q = lt; q < NGH; q += TPB) {
+      const int nb = q / PER_FACE + 1;
+      const int t = q - (nb - 1) * PER_FACE;
+      const int d = d_nb_dim(nb);
+      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int g[3] = {0, 0, 0};
+      g[d] = d_nb_low(nb) ? 0 : NC + 1;
+      g[ta] = t % NC + 1;
+      if (D == 3) g[tb] = t / NC + 1;
+      s[L(g[0], g[1], g[2])] = p[cidx<D>(NC, g[0], g[1], g[2])];
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  T* r = boxp_t<T>(a, i_rhs, id);
+  T* t = boxp_t<T>(a, i_tmp, id);
+  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
+  for (int e = lt; e < 2 * NCOL; e += TPB) {
+    int i, j, k;
+    const int x = cell(e, i, j, k);
+    T rv = EPS ? op_u<D, CYL, T>(a, id, NC, own, i, j, k, inv_dr_sq, ioff)
+               : lpl_u<D, CYL, T>(own, i, j, k, inv_dr_sq, ioff);
+    rv = rv + t[x];
+    r[x] = rv;
+    if (copy_tmp) t[x] = own(i, j, k);
+  }
+  for (int e = lt; e < 2 * NGC; e += TPB) {
+    const int col = e >= NGC;
+    const int x = col * HALF + NCOL + (e - col * NGC);
+    r[x] = r[x] + t[x];
+    if (copy_tmp) t[x] = p[x];
+  }
+}
+
+template <int D, bool CYL, typename T, bool EPS>
+static bool launch_coarse_rhs_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc,
+                                    int i_phi, int i_rhs, int i_tmp, int copy_tmp) {
+  auto go = [&](auto kern, int bl) {
+    const int tpb = bl > 128 ? 256 : (bl > 64 ? 128 : 64);
+    const int bpb = 256 / tpb;
+    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
+                       i_phi, i_rhs, i_tmp, copy_tmp);
+  };
+  const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
+  switch (nc) {
+    case 4: go(k_coarse_rhs_lds<D, CYL, 4, T, EPS>, bl); return true;
+    case 8: go(k_coarse_rhs_lds<D, CYL, 8, T, EPS>, bl); return true;
+    case 16: go(k_coarse_rhs_lds<D, CYL, 16, T, EPS>, bl); return true;
+    case 32:
+      if (D == 2) {
+        go(k_coarse_rhs_lds<2, CYL, 32, T, EPS>, bl);
+        return true;
+      }
+      return false;
+    default: return false;
   }
 }
 
@@ -1303,12 +1460,28 @@ static void coarse_rhs(Pool& P, const Ctx& c, int lvl, int copy_tmp) {
   Dev a = dev_for(P, c);
   const double cells = (double)n * cells_of(D, nc, false);
   const int pr = P.prof_begin(PK_COARSE_RHS, lvl, 4.0 * sizeof(T) * cells);
+  bool done;
   if (c.cyl)
-    hipLaunchKernelGGL((k_coarse_rhs<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
+    done = c.eps ? launch_coarse_rhs_lds_t<2, true, T, true>(P, a, P.parents(lvl), n, nc,
+                                                              c.i_phi, c.i_rhs, c.i_tmp, copy_tmp)
+                 : launch_coarse_rhs_lds_t<2, true, T, false>(P, a, P.parents(lvl), n, nc,
+                                                               c.i_phi, c.i_rhs, c.i_tmp,
+                                                               copy_tmp);
   else
-    hipLaunchKernelGGL((k_coarse_rhs<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                       P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
+    done = c.eps ? launch_coarse_rhs_lds_t<D, false, T, true>(P, a, P.parents(lvl), n, nc,
+                                                               c.i_phi, c.i_rhs, c.i_tmp,
+                                                               copy_tmp)
+                 : launch_coarse_rhs_lds_t<D, false, T, false>(P, a, P.parents(lvl), n, nc,
+                                                                c.i_phi, c.i_rhs, c.i_tmp,
+                                                                copy_tmp);
+  if (!done) {
+    if (c.cyl)
+      hipLaunchKernelGGL((k_coarse_rhs<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
+    else
+      hipLaunchKernelGGL((k_coarse_rhs<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.parents(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, c.i_tmp, copy_tmp);
+  }
   post_launch("coarse_rhs");
   P.prof_end(pr);
 }
