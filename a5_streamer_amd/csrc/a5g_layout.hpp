@@ -15,15 +15,25 @@ namespace a5g {
 // Flat index of cell (i, j, k), ghosts included (0 <= i,j,k <= nc+1), in a
 // box with nc (even) cells per dimension. Boxes are stored colour-split: the
 // cells with i+j+k even fill the first half of the slot, the odd ones the
-// second half. Within a colour half, the nc^D/2 interior cells come first,
-// in lexicographic order of the interior row pairs (i = 2m+1, 2m+2), so the
-// q-th interior cell of either colour sits at q; the ghost cells follow, in
-// lexicographic order of the ghost pairs (a row on the ghost layer holds
-// (nc+2)/2 pairs, any other row one pair: its cells i = 0 and nc+1).
-//  * A red-black half-sweep reads the other colour's interior and writes
-//    its own colour's interior as contiguous runs, and no ghost slot shares
-//    a cache line with them.
-//  * The ghost layer of a colour is one contiguous run after its interior.
+// second half. Within a colour half:
+//  * the nc^D/2 interior cells come first. Each interior row (j, k) holds
+//    nc/2 cells of a colour, one of them on the x boundary (i = 1 or nc).
+//    The boundary cells of all rows come first (slot = row), then the other
+//    nc/2 - 1 cells of each row in row order (isl below);
+//  * the ghost cells follow, in lexicographic order of the ghost pairs (a row
+//    on the ghost layer holds (nc+2)/2 pairs, any other row one pair: its
+//    cells i = 0 and nc+1).
+// A red-black half-sweep then reads the other colour's interior and writes
+// its own as contiguous runs that share no cache line with a ghost, and the
+// face a neighbour reads is dense in every direction: the x faces are every
+// other entry of the boundary block.
+template <int D>
+A5G_HD int isl(int nc, int i, int j, int k) {
+  const int row = D == 2 ? j - 1 : (j - 1) + nc * (k - 1);
+  const int nrow = D == 2 ? nc : nc * nc;
+  return (i == 1 || i == nc) ? row : nrow + row * ((nc >> 1) - 1) + ((i - 2) >> 1);
+}
+
 template <int D>
 A5G_HD int cidx(int nc, int i, int j, int k) {
   const int n = nc + 2;
@@ -32,10 +42,7 @@ A5G_HD int cidx(int nc, int i, int j, int k) {
   const bool in_i = (unsigned)(i - 1) < (unsigned)nc;
   const bool in_j = (unsigned)(j - 1) < (unsigned)nc;
   const bool in_k = D == 2 || (unsigned)(k - 1) < (unsigned)nc;
-  if (in_i && in_j && in_k) {
-    const int q = D == 2 ? (i - 1) + nc * (j - 1) : (i - 1) + nc * ((j - 1) + nc * (k - 1));
-    return base + (q >> 1);
-  }
+  if (in_i && in_j && in_k) return base + isl<D>(nc, i, j, k);
   const int n_in = D == 2 ? (nc * nc) >> 1 : (nc * nc * nc) >> 1;
   const int hr = n >> 1;  // pairs in a row on the ghost layer
   // ghost pairs of a plane (2D: the box) before row j, plus the pair of (i, j)
@@ -53,8 +60,27 @@ template <int D>
 A5G_HD int cidx_in(int nc, int i, int j, int k) {
   const int n = nc + 2;
   const int half = D == 2 ? (n * n) >> 1 : (n * n * n) >> 1;
-  const int q = D == 2 ? (i - 1) + nc * (j - 1) : (i - 1) + nc * ((j - 1) + nc * (k - 1));
-  return (((i + j + k) & 1) ? half : 0) + (q >> 1);
+  return (((i + j + k) & 1) ? half : 0) + isl<D>(nc, i, j, k);
+}
+
+// The interior cell of colour col at slot q of that colour's half
+// (0 <= q < nc^D/2): the inverse of isl.
+template <int D>
+A5G_HD void icell(int nc, int q, int col, int& i, int& j, int& k) {
+  const int nrow = D == 2 ? nc : nc * nc;
+  const int w = (nc >> 1) - 1;  // cells of a colour in a row, boundary cell excluded
+  int row, m = -1;
+  if (q < nrow) {
+    row = q;
+  } else {
+    row = (q - nrow) / w;
+    m = q - nrow - row * w;
+  }
+  j = row % nc + 1;
+  k = D == 3 ? row / nc + 1 : 0;
+  const bool odd = ((1 + j + k) & 1) == col;  // the colour's cells have odd i
+  if (m < 0) i = odd ? 1 : nc;
+  else i = odd ? 3 + 2 * m : 2 + 2 * m;
 }
 
 }  // namespace a5g

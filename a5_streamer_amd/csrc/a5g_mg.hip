@@ -266,13 +266,10 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   const int c = cntr & 1, oc = c ^ 1;
   T* s = sh + slot * HALF;
   T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
-  // the cell of colour col in interior pair q (the q-th interior cell of
-  // that colour in HBM); returns its lexicographic index in the block
+  // the interior cell of colour col at slot q of its half; returns its
+  // lexicographic index in the block
   auto cell_of = [&](int q, int col, int& i, int& j, int& k) {
-    const int mm = q % HNC, row = q / HNC;
-    j = row % NC + 1;
-    k = D == 3 ? row / NC + 1 : 0;
-    i = 2 * mm + 2 - ((col + j + k) & 1);
+    icell<D>(NC, q, col, i, j, k);
     return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
   };
   T rv[NIT];
@@ -684,11 +681,10 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
         const int e = lt + (i0 + u) * TPB;
         at[u] = -1;
         if (i0 + u < NITC && e < NCELL) {
-          // the cell of colour col in interior pair q
+          // the interior cell of colour col at slot q of its half
           const int col = e >= NCOL, q = e - col * NCOL;
-          const int mm = q % HNC, row = q / HNC;
-          const int j = row % NC + 1, k = D == 3 ? row / NC + 1 : 0;
-          const int i = 2 * mm + 2 - ((col + j + k) & 1);
+          int i, j, k;
+          icell<D>(NC, q, col, i, j, k);
           at[u] = L(i, j, k);
           v[u] = p[col * HALF + q];
         }
@@ -882,7 +878,6 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
   constexpr int N2 = NC + 2;
   constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
   constexpr int HALF = BL / 2;
-  constexpr int HNC = NC / 2;
   constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
   constexpr int NGC = HALF - NCOL;  // ghost slots of one colour
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
@@ -899,10 +894,7 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
   auto own = [&](int i, int j, int k) { return s[L(i, j, k)]; };
   auto cell = [](int e, int& i, int& j, int& k) {
     const int col = e >= NCOL, q = e - col * NCOL;
-    const int mm = q % HNC, row = q / HNC;
-    j = row % NC + 1;
-    k = D == 3 ? row / NC + 1 : 0;
-    i = 2 * mm + 2 - ((col + j + k) & 1);
+    icell<D>(NC, q, col, i, j, k);
     return col * HALF + q;
   };
   const T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;

@@ -1,7 +1,8 @@
 // Host check of the box cell order (a5g_layout.hpp): for every box size the
 // library supports, cidx is a bijection of the (nc+2)^D block onto the slot,
 // colour c fills slots [c*half, (c+1)*half), its interior cells come first
-// in interior-pair order, and cidx_in agrees with cidx on the interior.
+// with the x-boundary cell of each row at slot = row, icell inverts the
+// interior order, and cidx_in agrees with cidx on the interior.
 #include <cstdio>
 #include <vector>
 
@@ -22,8 +23,11 @@ static int check(int nc) {
         if (x < col * half || x >= (col + 1) * half) bad++;
         else seen[x]++;
         if (in) {
-          const int q = D == 2 ? (i - 1) + nc * (j - 1) : (i - 1) + nc * ((j - 1) + nc * (k - 1));
-          if (x - col * half != q / 2) bad++;
+          const int q = x - col * half, row = D == 2 ? j - 1 : (j - 1) + nc * (k - 1);
+          int ii, jj, kk;
+          a5g::icell<D>(nc, q, col, ii, jj, kk);
+          if (ii != i || jj != j || (D == 3 && kk != k)) bad++;
+          if ((i == 1 || i == nc) && q != row) bad++;
           if (a5g::cidx_in<D>(nc, i, j, k) != x) bad++;
         } else if (x - col * half < n_in) {
           bad++;
