@@ -642,8 +642,33 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   // these LDS slots are disjoint. Ghosts of the other rules read the staged
   // interior, after a barrier.
   int gq[NITG];
+  // rhs of the 2^D fine cells of each parent cell a lane restricts, loaded
+  // with the staging loads so that the compute phase reads only LDS
+  constexpr int NITP = (TOTAL + TPB - 1) / TPB;
+  constexpr int NF = D == 2 ? 4 : 8;
+  T rq[NITP][NF];
+  auto fine = [](int q, int f, int& ii, int& jj, int& kk) {
+    const int ci = q % HNC + 1, cj = (q / HNC) % HNC + 1;
+    const int ck = D == 3 ? q / (HNC * HNC) + 1 : 0;
+    ii = 2 * ci - 1 + (f & 1);
+    jj = 2 * cj - 1 + ((f >> 1) & 1);
+    kk = D == 3 ? 2 * ck - 1 + (f >> 2) : 0;
+  };
   if (act) {
     const T* p = boxp_t<T>(a, i_phi, id);
+    const T* r = boxp_t<T>(a, i_rhs, id);
+#pragma unroll
+    for (int u = 0; u < NITP; u++) {
+      const int q = lt + u * TPB;
+      if (q < TOTAL) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          int ii, jj, kk;
+          fine(q, f, ii, jj, kk);
+          rq[u][f] = r[cidx_in<D>(NC, ii, jj, kk)];
+        }
+      }
+    }
     T gv[NITG];
     int gs[NITG];
 #pragma unroll
@@ -721,32 +746,31 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   const int pnc = a.ncell[pid];
   int off[3];
   child_off<D>(a, id, NC, 0, off);
-  const T* r = boxp_t<T>(a, i_rhs, id);
   T* tv = boxp_t<T>(a, i_tmp, id);
   T* pp = boxp_t<T>(a, i_phi, pid);
   T* pt = boxp_t<T>(a, i_tmp, pid);
   const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
-  for (int q = lt; q < TOTAL; q += TPB) {
+#pragma unroll
+  for (int u = 0; u < NITP; u++) {
+    const int q = lt + u * TPB;
+    if (q >= TOTAL) break;
     const int ci = q % HNC + 1;
     const int cj = (q / HNC) % HNC + 1;
     const int ck = D == 3 ? q / (HNC * HNC) + 1 : 0;
-    const int fi = 2 * ci - 1, fj = 2 * cj - 1, fk = D == 3 ? 2 * ck - 1 : 0;
+    // Fortran's sum over cc(i_f:i_f+1, j_f:j_f+1[, k_f:k_f+1]) runs in array
+    // order, i fastest (m_aX_mg.f90:872-890)
     T sr = 0, sp = 0;
 #pragma unroll
-    for (int dk = 0; dk < (D == 3 ? 2 : 1); dk++)
-#pragma unroll
-      for (int dj = 0; dj < 2; dj++)
-#pragma unroll
-        for (int di = 0; di < 2; di++) {
-          const int ii = fi + di, jj = fj + dj, kk = fk + dk;
-          const int c = cidx_in<D>(NC, ii, jj, kk);
-          const T res = r[c] - (EPS ? op_u<D, CYL, T>(a, id, NC, own, ii, jj, kk, inv_dr_sq, ioff)
+    for (int f = 0; f < NF; f++) {
+      int ii, jj, kk;
+      fine(q, f, ii, jj, kk);
+      const T res = rq[u][f] - (EPS ? op_u<D, CYL, T>(a, id, NC, own, ii, jj, kk, inv_dr_sq, ioff)
                                     : lpl_u<D, CYL, T>(own, ii, jj, kk, inv_dr_sq, ioff));
-          if (keep_res) tv[c] = res;
-          sr = sr + res;
-          sp = sp + own(ii, jj, kk);
-        }
+      if (keep_res) tv[cidx_in<D>(NC, ii, jj, kk)] = res;
+      sr = sr + res;
+      sp = sp + own(ii, jj, kk);
+    }
     const int pc = cidx_in<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
     const T w = D == 2 ? T(0.25) : T(0.125);
     pt[pc] = w * sr;
@@ -1044,6 +1068,8 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
   constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
   constexpr int BPB = 256 / TPB;
   constexpr int NIT = (TOTAL + TPB - 1) / TPB;
+  constexpr int NCOL = TOTAL / 2;
+  constexpr int HALF = (D == 2 ? (NC + 2) * (NC + 2) : (NC + 2) * (NC + 2) * (NC + 2)) / 2;
   __shared__ T sh[BPB * RB];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
   const int li = xcd_block() * BPB + slot;
@@ -1052,10 +1078,19 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
   T* q = sh + slot * RB;
   int off[3] = {0, 0, 0};
   int pid = 0, pnc = 0;
+  // child cells in slot order (each colour's interior a contiguous run);
+  // their potential is loaded with the staging loads
+  T fv[NIT];
+  T* f = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   if (act) {
     pid = a.parent[id];
     pnc = a.ncell[pid];
     child_off<D>(a, id, NC, 0, off);
+#pragma unroll
+    for (int u = 0; u < NIT; u++) {
+      const int t = lt + u * TPB;
+      if (t < TOTAL) fv[u] = f[(t >= NCOL ? HALF - NCOL : 0) + t];
+    }
     const T* c = boxp_t<T>(a, i_tmp, pid);
     for (int t = lt; t < RB; t += TPB) {
       const int x = t % R, y = (t / R) % R, z = D == 3 ? t / (R * R) : 0;
@@ -1064,19 +1099,20 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
   }
   __syncthreads();
   if (!act) return;
-  T* f = boxp_t<T>(a, i_phi, id);
   const double* ep = (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, pid) : nullptr;
   auto L = [&](int x, int y, int z) { return D == 2 ? x + R * y : x + R * (y + R * z); };
 #pragma unroll
   for (int u = 0; u < NIT; u++) {
     const int t = lt + u * TPB;
     if (t >= TOTAL) break;
-    const int i = t % NC + 1, j = (t / NC) % NC + 1, k = D == 3 ? t / (NC * NC) + 1 : 0;
+    const int col = t >= NCOL;
+    int i, j, k;
+    icell<D>(NC, t - col * NCOL, col, i, j, k);
     // local parent indices: (i+1)/2 is the closest coarse cell, the other
     // one lies on the side of the fine cell
     const int i1 = (i + 1) >> 1, i2 = i1 + 1 - 2 * (i & 1);
     const int j1 = (j + 1) >> 1, j2 = j1 + 1 - 2 * (j & 1);
-    const int x = cidx<D>(NC, i, j, k);
+    const int x = col * HALF + (t - col * NCOL);
     if (ep != nullptr) {
       if (D == 2) {
         const int p0 = cidx<D>(pnc, off[0] + i1, off[1] + j1, 0);
@@ -1090,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
           const T am = (T)ep[pa[m]];
           sm = sm + (a0 * u0 + am * ua[m]) / (a0 + am);
         }
-        f[x] = f[x] + T(0.5) * sm;
+        f[x] = fv[u] + T(0.5) * sm;
       } else {
         const int k1 = (k + 1) >> 1, k2 = k1 + 1 - 2 * (k & 1);
         const int p0 = cidx<D>(pnc, off[0] + i1, off[1] + j1, off[2] + k1);
@@ -1105,13 +1141,13 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
           const T am = (T)ep[pa[m]];
           sm = sm + (a0 * u0 + am * (T(1.5) * ua[m] - T(0.5) * u0)) / (a0 + am);
         }
-        f[x] = f[x] + (T(1) / T(3)) * sm;
+        f[x] = fv[u] + (T(1) / T(3)) * sm;
       }
     } else if (D == 2) {
-      f[x] = f[x] + T(0.5) * q[L(i1, j1, 0)] + T(0.25) * (q[L(i2, j1, 0)] + q[L(i1, j2, 0)]);
+      f[x] = fv[u] + T(0.5) * q[L(i1, j1, 0)] + T(0.25) * (q[L(i2, j1, 0)] + q[L(i1, j2, 0)]);
     } else {
       const int k1 = (k + 1) >> 1, k2 = k1 + 1 - 2 * (k & 1);
-      f[x] = f[x] + T(0.25) * (q[L(i1, j1, k1)] + q[L(i2, j1, k1)] + q[L(i1, j2, k1)] +
+      f[x] = fv[u] + T(0.25) * (q[L(i1, j1, k1)] + q[L(i2, j1, k1)] + q[L(i1, j2, k1)] +
                                q[L(i1, j1, k2)]);
     }
   }
