@@ -626,12 +626,24 @@ int a5g_mg_setup(a5g_tree* t, int i_phi, int i_tmp, int i_rhs, int i_eps, int n_
   P.mg.n_up = n_up < 0 ? 2 : n_up;
   P.mg.n_base = n_base < 0 ? 2 : n_base;
   P.mg.auto_ops = auto_ops;
+  P.mg.helm = 0;  // a new setup starts from the Laplacian
   for (int nb = 0; nb < 2 * P.tree.dim; nb++) {
     if (bc_kind[nb] < 0 || bc_kind[nb] > 2) throw std::invalid_argument("bad boundary kind");
     P.bc_kind[nb] = bc_kind[nb];
     P.bc_val[nb] = bc_val ? bc_val[nb] : 0.0;
   }
   P.mg.ready = true;
+  API_END
+}
+
+int a5g_mg_set_helmholtz(a5g_tree* t, double kappa2) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (!(kappa2 >= 0)) throw std::invalid_argument("mg_set_helmholtz: need kappa2 >= 0");
+  if (kappa2 > 0 && P.mg.auto_ops && P.mg.i_eps > 0)
+    throw std::invalid_argument("mg_set_helmholtz: not combined with a permittivity");
+  P.mg.helm = kappa2;
   API_END
 }
 
@@ -892,6 +904,24 @@ int a5g_update_solution(a5g_tree* t, double dt) {
   need_gpu(t->P);
   need_table(t->P, t->P.sv);
   a5g::update_solution(t->P, t->P.sv, dt);
+  API_END
+}
+
+int a5g_photoi_helmholtz(a5g_tree* t, double coeff, int n_terms, const double* amp,
+                         const double* kappa, int i_phi, int i_rhs, int i_tmp, double tol,
+                         int max_cycles, int* cycles) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  need_table(P, P.sv);
+  if (P.sv.pho <= 0) throw std::invalid_argument("photoionization: i_pho not set");
+  if (!amp || !kappa) throw std::invalid_argument("photoionization: null coefficients");
+  check_iv(P.tree, i_phi);
+  check_iv(P.tree, i_rhs);
+  check_iv(P.tree, i_tmp);
+  a5g::photoi_helmholtz(P, P.sv, coeff, n_terms, amp, kappa, i_phi, i_rhs, i_tmp, tol,
+                        max_cycles, cycles);
   API_END
 }
 

@@ -29,6 +29,11 @@ void consistent_fluxes(Pool& P, int f_elec);
 void update_solution(Pool& P, const StVars& v, double dt);
 void average_dens(Pool& P, const StVars& v);
 void box_summaries(Pool& P, const StVars& v, double* out_host);
+void photoi_rate(Pool& P, const StVars& v, double coeff);
+void axpy_leaves(Pool& P, int iin, int iout, double s, bool add);
+void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const double* amp,
+                      const double* kappa, int i_phi, int i_rhs, int i_tmp, double tol,
+                      int max_cycles, int* cycles_done);
 void prolong1_boxes(Pool& P, int n_ids, const int* ids, int iv);
 void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value);
 void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc);

@@ -114,12 +114,22 @@ __device__ __forceinline__ T lpld_u(const double* e, int nc, U u, int i, int j, 
 // The multigrid operator of box id: the permittivity form for boxes tagged
 // constant- or variable-eps (mg$D_auto_op, m_aX_mg.f90:603-635), else the
 // Laplacian.
+// The Laplacian, or with a Helmholtz coefficient lpl(u) - helm * u (the
+// operator of the photoionization solves)
+template <int D, bool CYL, typename T, typename U>
+__device__ __forceinline__ T lplh_u(const Dev& a, U u, int i, int j, int k, T inv_dr_sq,
+                                    int ioff) {
+  const T v = lpl_u<D, CYL, T>(u, i, j, k, inv_dr_sq, ioff);
+  if (a.helm != 0) return v - (T)a.helm * u(i, j, k);
+  return v;
+}
+
 template <int D, bool CYL, typename T, typename U>
 __device__ __forceinline__ T op_u(const Dev& a, int id, int nc, U u, int i, int j, int k,
                                   T inv_dr_sq, int ioff) {
   if (a.tag != nullptr && a.tag[id] > kTagNormal)
     return lpld_u<D, CYL, T>(boxp(a, a.i_eps, id), nc, u, i, j, k, inv_dr_sq, ioff);
-  return lpl_u<D, CYL, T>(u, i, j, k, inv_dr_sq, ioff);
+  return lplh_u<D, CYL, T>(a, u, i, j, k, inv_dr_sq, ioff);
 }
 
 template <int D, bool CYL, typename T>
@@ -204,25 +214,25 @@ __global__ __launch_bounds__(256) void k_gsrb(Dev a, const int* __restrict__ lis
         rf2 = (T)((double)(i + ioff) / den);
       }
       p[c0] = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c0], dx2, r[c0], rf1, rf2);
-    } else if (D == 2) {
-      if (CYL) {
-        const double den = i + ioff - 0.5;
-        const T rf1 = (T)((double)(i + ioff - 1) / den);
-        const T rf2 = (T)((double)(i + ioff) / den);
-        const T v = T(0.25) * (rf1 * nbv(-1, 0, 0, 1) + rf2 * nbv(1, 0, 0, 2) +
-                               nbv(0, 1, 0, 4) + nbv(0, -1, 0, 3) - dx2 * r[c0]);
-        p[c0] = v;
-      } else {
-        const T v = T(0.25) * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
-                               nbv(0, -1, 0, 3) - dx2 * r[c0]);
-        p[c0] = v;
-      }
     } else {
-      const T sixth = T(1) / T(6);
-      const T v = sixth * (nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) +
-                           nbv(0, -1, 0, 3) + nbv(0, 0, 1, 6) + nbv(0, 0, -1, 5) -
-                           dx2 * r[c0]);
-      p[c0] = v;
+      T num;
+      if (D == 2) {
+        if (CYL) {
+          const double den = i + ioff - 0.5;
+          const T rf1 = (T)((double)(i + ioff - 1) / den);
+          const T rf2 = (T)((double)(i + ioff) / den);
+          num = rf1 * nbv(-1, 0, 0, 1) + rf2 * nbv(1, 0, 0, 2) + nbv(0, 1, 0, 4) +
+                nbv(0, -1, 0, 3) - dx2 * r[c0];
+        } else {
+          num = nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) + nbv(0, -1, 0, 3) -
+                dx2 * r[c0];
+        }
+      } else {
+        num = nbv(1, 0, 0, 2) + nbv(-1, 0, 0, 1) + nbv(0, 1, 0, 4) + nbv(0, -1, 0, 3) +
+              nbv(0, 0, 1, 6) + nbv(0, 0, -1, 5) - dx2 * r[c0];
+      }
+      if (a.helm != 0) p[c0] = num / (T(2 * D) + (T)a.helm * dx2);
+      else p[c0] = (D == 2 ? T(0.25) : T(1) / T(6)) * num;
     }
   }
 }
@@ -377,19 +387,25 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
         rf2 = (T)((double)(i + ioff) / den);
       }
       v = gsrb_lpld<D, CYL, T>(uu, am, (T)ev[c * HALF + q], dx2, rv[u], rf1, rf2);
-    } else if (D == 2) {
-      if (CYL) {
-        const double den = i + ioff - 0.5;
-        const T rf1 = (T)((double)(i + ioff - 1) / den);
-        const T rf2 = (T)((double)(i + ioff) / den);
-        v = T(0.25) * (rf1 * xm + rf2 * xp + yp + ym - dx2 * rv[u]);
-      } else {
-        v = T(0.25) * (xp + xm + yp + ym - dx2 * rv[u]);
-      }
     } else {
-      const T zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
-      const T sixth = T(1) / T(6);
-      v = sixth * (xp + xm + yp + ym + zp + zm - dx2 * rv[u]);
+      // (sum of the neighbours - h^2 rhs) / 2D, or with a Helmholtz term
+      // divided by 2D + helm h^2
+      T num;
+      if (D == 2) {
+        if (CYL) {
+          const double den = i + ioff - 0.5;
+          const T rf1 = (T)((double)(i + ioff - 1) / den);
+          const T rf2 = (T)((double)(i + ioff) / den);
+          num = rf1 * xm + rf2 * xp + yp + ym - dx2 * rv[u];
+        } else {
+          num = xp + xm + yp + ym - dx2 * rv[u];
+        }
+      } else {
+        const T zp = s[(lex + N2 * N2) >> 1], zm = s[(lex - N2 * N2) >> 1];
+        num = xp + xm + yp + ym + zp + zm - dx2 * rv[u];
+      }
+      if (a.helm != 0) v = num / (T(2 * D) + (T)a.helm * dx2);
+      else v = (D == 2 ? T(0.25) : T(1) / T(6)) * num;
     }
     pc[q] = v;
   }
@@ -766,7 +782,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
       int ii, jj, kk;
       fine(q, f, ii, jj, kk);
       const T res = rq[u][f] - (EPS ? op_u<D, CYL, T>(a, id, NC, own, ii, jj, kk, inv_dr_sq, ioff)
-                                    : lpl_u<D, CYL, T>(own, ii, jj, kk, inv_dr_sq, ioff));
+                                    : lplh_u<D, CYL, T>(a, own, ii, jj, kk, inv_dr_sq, ioff));
       if (keep_res) tv[cidx_in<D>(NC, ii, jj, kk)] = res;
       sr = sr + res;
       sp = sp + own(ii, jj, kk);
@@ -950,7 +966,7 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
     int i, j, k;
     const int x = cell(e, i, j, k);
     T rv = EPS ? op_u<D, CYL, T>(a, id, NC, own, i, j, k, inv_dr_sq, ioff)
-               : lpl_u<D, CYL, T>(own, i, j, k, inv_dr_sq, ioff);
+               : lplh_u<D, CYL, T>(a, own, i, j, k, inv_dr_sq, ioff);
     rv = rv + t[x];
     r[x] = rv;
     if (copy_tmp) t[x] = own(i, j, k);
@@ -1320,12 +1336,14 @@ struct Ctx {
   bool cyl;  // cylindrical operators (2D, automatic operators installed)
   bool eps;  // per-box operator tags (automatic operators and an eps variable)
   int n_down, n_up, n_base;  // smoothing cycles (mg$D_t%n_cycle_*)
+  double helm;               // Helmholtz coefficient (0: Laplacian)
 };
 
 static Ctx ctx64(const Pool& P) {
   return {P.mg.i_phi, P.mg.i_tmp, P.mg.i_rhs, false,
           P.tree.dim == 2 && P.tree.coord == 1 && P.mg.auto_ops != 0,
-          P.mg.auto_ops != 0 && P.mg.i_eps > 0, P.mg.n_down, P.mg.n_up, P.mg.n_base};
+          P.mg.auto_ops != 0 && P.mg.i_eps > 0, P.mg.n_down, P.mg.n_up, P.mg.n_base,
+          P.mg.helm};
 }
 
 static Dev dev_for(Pool& P, const Ctx& c) {
@@ -1336,6 +1354,7 @@ static Dev dev_for(Pool& P, const Ctx& c) {
     a.tag = P.d_tag;
     a.i_eps = P.mg.i_eps;
   }
+  a.helm = c.helm;
   return a;
 }
 
@@ -1728,6 +1747,11 @@ static uint64_t state_fingerprint(Pool& P) {
   }
   num(P.mg.i_phi); num(P.mg.i_tmp); num(P.mg.i_rhs); num(P.mg.i_eps);
   num(P.mg.n_down); num(P.mg.n_up); num(P.mg.n_base); num(P.mg.auto_ops);
+  {
+    uint64_t hb;
+    std::memcpy(&hb, &P.mg.helm, 8);
+    num(hb);
+  }
   num((uint64_t)(int64_t)P.tree.min_lvl); num(P.tree.max_lvl);
   for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++) {
     const int k = l - P.tree.min_lvl;
@@ -2008,7 +2032,7 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
                         double* rel_res) {
   const Ctx c64 = ctx64(P);
   // FP32 variables: 1 = correction e, 2 = tmp, 3 = rhs of the correction
-  const Ctx c32{1, 2, 3, true, c64.cyl, c64.eps, c64.n_down, c64.n_up, c64.n_base};
+  const Ctx c32{1, 2, 3, true, c64.cyl, c64.eps, c64.n_down, c64.n_up, c64.n_base, c64.helm};
   ensure32(P, 3);
   const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
   double rhs_lo, rhs_max;
@@ -2096,6 +2120,81 @@ void mg_solve_mixed(Pool& P, double tol, int max_outer, int n_inner, int* outer_
   if (n_inner < 1) throw std::invalid_argument("mixed solve: n_inner must be >= 1");
   if (P.tree.dim == 2) mixed_solve<2>(P, tol, max_outer, n_inner, outer_done, rel_res);
   else mixed_solve<3>(P, tol, max_outer, n_inner, outer_done, rel_res);
+}
+
+// Photoionization by the three-term Helmholtz model (Bourdon et al. 2007,
+// Plasma Sources Sci. Technol. 16, 656) in place of the reference's Monte-Carlo
+// photons (set_photoionization, src/streamer_2d.f90:544-565): the production
+// rate I (photoi_rate) is turned into the source sum_j S_j, where
+// (lpl - kappa_j^2) S_j = -amp_j I with S_j = 0 on the domain boundary. Each
+// term is one FMG plus V-cycles until max|r| <= tol max|rhs| on the leaves.
+// The work variables carry phi, rhs and tmp of these solves; the caller's
+// multigrid options are restored afterwards. The leaf rhs of term j + 1 is
+// the leaf rhs of term j times amp_{j+1} / amp_j (a FAS solve leaves the rhs
+// of leaves unchanged), so that i_pho can sum the terms from the start.
+void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const double* amp,
+                      const double* kappa, int i_phi, int i_rhs, int i_tmp, double tol,
+                      int max_cycles, int* cycles_done) {
+  if (n_terms < 1) throw std::invalid_argument("photoionization: need at least one term");
+  for (int j = 0; j < n_terms; j++)
+    if (!(amp[j] > 0) || !(kappa[j] >= 0))
+      throw std::invalid_argument("photoionization: need amp > 0 and kappa >= 0");
+  if (i_phi == v.pho || i_rhs == v.pho || i_tmp == v.pho)
+    throw std::invalid_argument("photoionization: work variables must differ from i_pho");
+  P.sync_topology();
+  photoi_rate(P, v, coeff);
+  const MgOpts saved = P.mg;
+  int bk[6];
+  double bv[6];
+  for (int i = 0; i < 6; i++) {
+    bk[i] = P.bc_kind[i];
+    bv[i] = P.bc_val[i];
+    P.bc_kind[i] = 1;  // S = 0 on the boundary: ghost = -c
+    P.bc_val[i] = 0;
+  }
+  // the operator family of the field solve (cylindrical when it has the
+  // automatic operators), without permittivity
+  P.mg.i_phi = i_phi;
+  P.mg.i_rhs = i_rhs;
+  P.mg.i_tmp = i_tmp;
+  P.mg.i_eps = -1;
+  P.mg.ready = true;
+  int total = 0;
+  try {
+    axpy_leaves(P, v.pho, i_rhs, -amp[0], false);
+    axpy_leaves(P, v.pho, v.pho, 0.0, false);  // i_pho = 0 on leaves
+    for (int j = 0; j < n_terms; j++) {
+      if (j > 0) axpy_leaves(P, i_rhs, i_rhs, amp[j] / amp[j - 1], false);
+      P.mg.helm = kappa[j] * kappa[j];
+      const Ctx c = ctx64(P);
+      if (P.tree.dim == 2) fas_fmg<2, double>(P, c, false, true);
+      else fas_fmg<3, double>(P, c, false, true);
+      for (int cyc = 0;; cyc++) {
+        double res, rhs;
+        mg_residual_norms(P, &res, &rhs);
+        if (res <= tol * rhs) break;
+        if (cyc >= max_cycles)
+          throw std::runtime_error("photoionization: Helmholtz solve did not converge");
+        if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, P.tree.max_lvl, false);
+        else fas_vcycle<3, double>(P, c, P.tree.max_lvl, false);
+        total++;
+      }
+      axpy_leaves(P, i_phi, v.pho, 1.0, true);
+    }
+  } catch (...) {
+    P.mg = saved;
+    for (int i = 0; i < 6; i++) {
+      P.bc_kind[i] = bk[i];
+      P.bc_val[i] = bv[i];
+    }
+    throw;
+  }
+  P.mg = saved;
+  for (int i = 0; i < 6; i++) {
+    P.bc_kind[i] = bk[i];
+    P.bc_val[i] = bv[i];
+  }
+  if (cycles_done) *cycles_done = total;
 }
 
 void gc_tree(Pool& P, int iv, int rb, int bc) {

@@ -36,6 +36,9 @@ struct Dev {
   // when the automatic operators and a permittivity variable are in use
   const int* tag = nullptr;
   int i_eps = -1;
+  // Helmholtz term of the multigrid operator: lpl(phi) - helm * phi (0: the
+  // Laplacian)
+  double helm = 0;
 };
 
 // Per-launch timing with HIP events on the pool's stream (kernel classes
@@ -63,6 +66,7 @@ struct MgOpts {
   int i_phi = -1, i_tmp = -1, i_rhs = -1, i_eps = -1;
   int n_down = 2, n_up = 2, n_base = 2;
   int auto_ops = 0;
+  double helm = 0;  // Helmholtz coefficient kappa^2 (a5g_mg_set_helmholtz)
   bool ready = false;
 };
 

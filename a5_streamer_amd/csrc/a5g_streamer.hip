@@ -549,6 +549,54 @@ __global__ __launch_bounds__(256) void k_fill(Dev a, const int* __restrict__ lis
 // host side
 // ---------------------------------------------------------------------------
 
+// Photon production rate of the photoionization model (set_photoi_rate,
+// src/streamer_2d.f90:567-589, with src/streamer_3d.f90 and _cyl alike):
+// |E| mu alpha n_e coeff on leaf interiors, clamped at zero; coeff is eta
+// times the quenching factor p_q / (p + p_q).
+template <int D>
+__global__ __launch_bounds__(256) void k_photoi_rate(Dev a, Tbl tb, const int* __restrict__ list,
+                                                     int n, int nc, int tpb, int ifld, int ie,
+                                                     int ipho, int cmob, int calpha,
+                                                     double coeff) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double* E = boxp(a, ifld, id);
+  const double* ne = boxp(a, ie, id);
+  double* out = boxp(a, ipho, id);
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c = cidx_in<D>(nc, i, j, k);
+    int li;
+    double lf;
+    lt_loc(tb, E[c], li, lf);
+    const double alpha = lt_col(tb, calpha, li, lf);
+    const double mob = lt_col(tb, cmob, li, lf);
+    double v = E[c] * mob * alpha * ne[c] * coeff;
+    if (v < 0) v = 0;
+    out[c] = v;
+  }
+}
+
+// out = s * in (add = 0) or out = out + s * in (add = 1) on leaf interiors
+template <int D>
+__global__ __launch_bounds__(256) void k_axpy_in(Dev a, const int* __restrict__ list, int n,
+                                                 int nc, int tpb, int iin, int iout, double s,
+                                                 int add) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double* x = boxp(a, iin, id);
+  double* y = boxp(a, iout, id);
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c = cidx_in<D>(nc, i, j, k);
+    y[c] = add ? y[c] + s * x[c] : s * x[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+
 static void chk(hipError_t e, const char* w) {
   if (e != hipSuccess) throw std::runtime_error(std::string(w) + ": " + hipGetErrorString(e));
 }
@@ -706,6 +754,45 @@ void average_dens(Pool& P, const StVars& v) {
     hipLaunchKernelGGL(k_average, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl), n,
                        bl, g.tpb, v.elec, v.elec_old, v.pion, v.pion_old);
     chk(hipGetLastError(), "average_dens");
+  }
+}
+
+void photoi_rate(Pool& P, const StVars& v, double coeff) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  Tbl tb = device_table(P);
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, box_cells(D, nc, false));
+    if (D == 3)
+      hipLaunchKernelGGL((k_photoi_rate<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.fld, v.elec, v.pho, v.t_mob, v.t_alpha,
+                         coeff);
+    else
+      hipLaunchKernelGGL((k_photoi_rate<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.fld, v.elec, v.pho, v.t_mob, v.t_alpha,
+                         coeff);
+    chk(hipGetLastError(), "photoi_rate");
+  }
+}
+
+void axpy_leaves(Pool& P, int iin, int iout, double s, bool add) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, box_cells(D, nc, false));
+    if (D == 3)
+      hipLaunchKernelGGL((k_axpy_in<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                         P.leaves(lvl), n, nc, g.tpb, iin, iout, s, (int)add);
+    else
+      hipLaunchKernelGGL((k_axpy_in<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
+                         P.leaves(lvl), n, nc, g.tpb, iin, iout, s, (int)add);
+    chk(hipGetLastError(), "axpy_leaves");
   }
 }
 

@@ -240,6 +240,13 @@ module m_a5g
        integer(c_int)        :: s
      end function c_mg_set_bc_value
 
+     function c_mg_set_helmholtz(t, kappa2) bind(C, name="a5g_mg_set_helmholtz") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: kappa2
+       integer(c_int)        :: s
+     end function c_mg_set_helmholtz
+
      function c_fas_fmg(t, set_residual, first_call) bind(C, name="a5g_fas_fmg") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -384,6 +391,17 @@ module m_a5g
        integer(c_int)     :: s
      end function c_average_dens
 
+     function c_photoi_helmholtz(t, coeff, n_terms, amp, kappa, i_phi, i_rhs, i_tmp, tol, &
+          max_cycles, cycles) bind(C, name="a5g_photoi_helmholtz") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: coeff, tol
+       integer(c_int), value :: n_terms, i_phi, i_rhs, i_tmp, max_cycles
+       real(c_double)        :: amp(*), kappa(*)
+       integer(c_int)        :: cycles
+       integer(c_int)        :: s
+     end function c_photoi_helmholtz
+
      function c_box_summaries(t, out) bind(C, name="a5g_box_summaries") result(s)
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: t
@@ -499,7 +517,7 @@ module m_a5g
   public :: a5g_mixed_solve, a5g_residual_norms, a5g_set_bc_value
   public :: st_setup, st_table_create, st_table_set_col, st_table_get
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
-  public :: st_average_dens, st_box_summaries
+  public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
 
 contains
 
@@ -736,6 +754,15 @@ contains
     call check(c_mg_set_bc_value(tree%h, nb, value), "mg_set_bc_value")
   end subroutine a5g_set_bc_value
 
+  !> Helmholtz operator lpl(phi) - kappa2 phi for the following multigrid
+  !> calls (0 restores the Laplacian); the custom box_op / box_gsrb pattern of
+  !> afivo/examples/tutorial/implicit.f90:147-193
+  subroutine a5g_set_helmholtz(tree, kappa2)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: kappa2
+    call check(c_mg_set_helmholtz(tree%h, kappa2), "mg_set_helmholtz")
+  end subroutine a5g_set_helmholtz
+
   !> mg$D_fas_fmg (m_aX_mg.f90:164-203)
   subroutine t_fas_fmg(tree, mg, set_residual, first_call)
     type(a5g_t), intent(inout) :: tree
@@ -907,6 +934,24 @@ contains
     real(dp), intent(in)       :: dt
     call check(c_update_solution(tree%h, dt), "update_solution")
   end subroutine st_update_solution
+
+  !> Photoionization source in i_pho by the three-term Helmholtz model, in
+  !> place of set_photoionization's Monte-Carlo photons
+  !> (src/streamer_2d.f90:544-589): coeff = eta p_q / (p + p_q), amp(j) =
+  !> A_j p_O2^2 and kappa(j) = lambda_j p_O2 in SI units; i_phi, i_rhs and
+  !> i_tmp are the work variables of the solves.
+  subroutine st_photoi_helmholtz(tree, coeff, amp, kappa, i_phi, i_rhs, i_tmp, tol, &
+       max_cycles, cycles)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: coeff, amp(:), kappa(:), tol
+    integer, intent(in)        :: i_phi, i_rhs, i_tmp, max_cycles
+    integer, intent(out)       :: cycles
+    integer(c_int)             :: cyc
+    if (size(amp) /= size(kappa)) error stop "st_photoi_helmholtz: amp and kappa differ"
+    call check(c_photoi_helmholtz(tree%h, coeff, size(amp), amp, kappa, i_phi, i_rhs, &
+         i_tmp, tol, max_cycles, cyc), "photoi_helmholtz")
+    cycles = cyc
+  end subroutine st_photoi_helmholtz
 
   !> average_dens on every box (src/streamer_2d.f90:492-496)
   subroutine st_average_dens(tree)

@@ -61,6 +61,7 @@ class Tree:
                  lvls_max=30, coord=0, n_boxes=100):
         self.dim = dim
         self.n_cell = n_cell
+        self.n_cc = n_cc
         self.dr_base = dr
         self.h = C.c_void_p()
         r_min = np.zeros(dim) if r_min is None else np.asarray(r_min, float)
@@ -195,6 +196,9 @@ class Tree:
     def mg_set_bc_value(self, nb, value):
         _call("a5g_mg_set_bc_value", self.h, C.c_int(nb), C.c_double(value))
 
+    def mg_set_helmholtz(self, kappa2):
+        _call("a5g_mg_set_helmholtz", self.h, C.c_double(kappa2))
+
     def fas_fmg(self, set_residual, first_call):
         _call("a5g_fas_fmg", self.h, C.c_int(int(set_residual)),
               C.c_int(int(first_call)))
@@ -273,6 +277,18 @@ class Tree:
 
     def average_dens(self):
         _call("a5g_average_dens", self.h)
+
+    def photoi_helmholtz(self, coeff, amp, kappa, i_phi, i_rhs, i_tmp, tol=1e-8,
+                         max_cycles=30):
+        a, pa = _d(amp)
+        k, pk = _d(kappa)
+        if len(a) != len(k):
+            raise A5GError("photoi_helmholtz: amp and kappa differ in length")
+        cyc = C.c_int()
+        _call("a5g_photoi_helmholtz", self.h, C.c_double(coeff), C.c_int(len(a)), pa, pk,
+              C.c_int(i_phi), C.c_int(i_rhs), C.c_int(i_tmp), C.c_double(tol),
+              C.c_int(max_cycles), C.byref(cyc))
+        return cyc.value
 
     def box_summaries(self):
         n = self.info()["max_id"]

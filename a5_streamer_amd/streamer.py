@@ -11,8 +11,9 @@ object with the same methods can drive it; the tests use that to step the
 same run through the CPU restatement and compare every state bit for bit.
 
 Output (Silo files, streamer properties) and Monte-Carlo photoionization are
-not part of this path: a configuration that asks for photoionization is
-rejected.
+not part of this path. Photoionization is provided by the three-term
+Helmholtz model instead (Config.photoi), at the place where the drivers call
+set_photoionization.
 """
 import math
 
@@ -25,6 +26,15 @@ EPS_F64 = 2.220446049250313e-16  # epsilon(1.0_dp)
 # variable indices of src/m_streamer.f90:16-43
 I_ELEC, I_PION, I_ELEC_OLD, I_PION_OLD, I_PHI, I_FLD, I_RHS, I_PHO, I_EPS = range(1, 10)
 N_VAR_CELL, N_VAR_FACE = 9, 2
+# work variables of the photoionization solves (phi, rhs, tmp), after the
+# reference's nine
+I_PH_PHI, I_PH_RHS, I_PH_TMP = 10, 11, 12
+UC_TORR_TO_BAR = 133.322368 * 1.0e-5  # fosito/m_units_constants.f90:44
+# Three-term fit of the photoionization absorption function of air (Bourdon
+# et al. 2007, Plasma Sources Sci. Technol. 16, 656, table 1):
+# A_j in cm^-2 Torr^-2 and lambda_j in cm^-1 Torr^-1
+HELMHOLTZ_A = (1.986e-4, 0.0051, 0.4886)
+HELMHOLTZ_LAMBDA = (0.0553, 0.1460, 0.89)
 F_ELEC, F_FLD = 1, 2
 T_MOB, T_DIF, T_ALPHA, T_ETA = 1, 2, 3, 4
 RB_MG, RB_INTERP, RB_INTERP_LIM = 0, 1, 2   # mg_sides_rb, gc_interp, gc_interp_lim
@@ -157,7 +167,11 @@ class Config:
         self.dt_max = 1e-8
         self.steps_amr = 2
         self.epsilon_diel = 1.0
-        self.photoi = False
+        self.photoi = False            # Helmholtz photoionization
+        self.photoi_eta = 0.05         # src/m_streamer.f90:196-203
+        self.photoi_frac_o2 = 0.2
+        self.gas_pressure = 1.0        # bar, src/m_streamer.f90:160-161
+        self.photoi_tol = 1e-6         # relative residual of each term's solve
         self.table_size = 1000
         self.table_max_fld = 3e7
         self.n_boxes = 1000
@@ -166,8 +180,18 @@ class Config:
             if not hasattr(self, k):
                 raise KeyError("unknown config key %s" % k)
             setattr(self, k, v)
-        if self.photoi:
-            raise ValueError("photoionization is not provided")
+
+    def photoi_model(self):
+        """(coeff, amp, kappa) of the Helmholtz photoionization: coeff = eta
+        times the quenching factor of set_photoionization
+        (src/streamer_2d.f90:551-560), amp_j = A_j p_O2^2 (1/m^2) and kappa_j =
+        lambda_j p_O2 (1/m) with p_O2 in Torr."""
+        p_quench = 30.0 * UC_TORR_TO_BAR
+        coeff = self.photoi_eta * (p_quench / (self.gas_pressure + p_quench))
+        p_o2 = self.photoi_frac_o2 * self.gas_pressure / UC_TORR_TO_BAR
+        amp = [a * p_o2 ** 2 * 1e4 for a in HELMHOLTZ_A]
+        kappa = [lam * p_o2 * 1e2 for lam in HELMHOLTZ_LAMBDA]
+        return coeff, amp, kappa
 
     def transport_columns(self):
         """Input rows of the transport table. The default is the air model of
@@ -219,7 +243,8 @@ class StreamerRun:
         # domain uses coarsen_to=2 as the drivers do
         n0 = c.n_base if np.isscalar(c.n_base) else c.n_base[0]
         dr = c.domain_len / (c.box_size * n0)
-        self.T = T = tree_cls(self.dim, c.box_size, N_VAR_CELL, N_VAR_FACE, dr,
+        n_cc = N_VAR_CELL + (3 if c.photoi else 0)
+        self.T = T = tree_cls(self.dim, c.box_size, n_cc, N_VAR_FACE, dr,
                               coarsen_to=2, lvls_max=c.lvls_max, coord=v["coord"],
                               n_boxes=c.n_boxes)
         T.set_base(*base_grid(self.dim, c.n_base))
@@ -229,7 +254,8 @@ class StreamerRun:
         for k, y in enumerate(cols, start=1):
             T.table_set_col(k, x, y)
         self.tbl = Table(*T.table_get())
-        T.streamer_setup(dielectric_rule=v["dielectric_rule"])
+        T.streamer_setup(dielectric_rule=v["dielectric_rule"],
+                         pho=I_PHO if c.photoi else -1)
         # sides_bc_pot: Neumann sides, grounded at the low end of the last
         # axis, 2V - phi at its high end
         self.voltage = -c.domain_len * c.applied_fld
@@ -365,14 +391,25 @@ class StreamerRun:
         return 0.5 * min(1 / (1 / dt_cfl + 1 / dt_dif), dt_alpha, self.c.dt_max)
 
     # --- the run ------------------------------------------------------------
+    def set_photoionization(self):
+        """set_photoionization (src/streamer_2d.f90:544-565) with the
+        Helmholtz model in place of Monte-Carlo photons: the production rate
+        of set_photoi_rate, then one multigrid solve per term. Returns the
+        V-cycles run."""
+        coeff, amp, kappa = self.c.photoi_model()
+        return self.T.photoi_helmholtz(coeff, np.array(amp), np.array(kappa), I_PH_PHI,
+                                       I_PH_RHS, I_PH_TMP, self.c.photoi_tol, 40)
+
     def initialise(self, max_iter=50):
         """Initial conditions, field and refinement until the mesh settles
-        (src/streamer_2d.f90:57-63)."""
+        (src/streamer_2d.f90:57-66)."""
         for _ in range(max_iter):
             self.set_init_cond()
             self.compute_fld(True)
             n_add, n_rm = self.adjust()
             if n_add == 0 and (n_rm == 0 or not self.v["exit_on_rm"]):
+                if self.c.photoi:
+                    self.set_photoionization()
                 return
         raise RuntimeError("initial refinement did not settle")
 
@@ -405,6 +442,8 @@ class StreamerRun:
         for _ in range(self.c.steps_amr):
             if not cyl_time:
                 self.time = self.time + dt
+            if self.c.photoi:
+                self.set_photoionization()
             T.tree_copy_cc(I_ELEC, I_ELEC_OLD)
             T.tree_copy_cc(I_PION, I_PION_OLD)
             self._euler_pair(dt, cyl_time)

@@ -158,6 +158,25 @@ def measured_traffic(kernel, precision):
     return None
 
 
+def copy_peak(T, reps=5):
+    """Measured HBM rate of a plain copy kernel on this GPU: the whole-box copy
+    of one variable (ghosts included) on the finest level, bytes read plus
+    written over the mean HIP-event duration. Overwrites variable 4."""
+    lvl = T.info()["max_lvl"]
+    n_box = len(T.level(lvl)[0])
+    nc = T.n_cell
+    whole = n_box * (nc + 2) ** T.dim * 8.0 * 2
+    T.tree_copy_cc(1, 4)
+    T.prof_enable(True)
+    T.sync()
+    for _ in range(reps):
+        T.tree_copy_cc(1, 4)
+    T.sync()
+    n, ms, _ = T.prof_read(5, lvl)
+    T.prof_enable(False)
+    return round(whole / (ms * 1e-3 / n) / 1e9, 1) if n else None
+
+
 def roofline(T, precision, fn):
     """Roofline of the dominant kernel, the Gauss-Seidel half-sweep, on the
     finest level (its largest launches): algorithmic bytes per launch over
@@ -267,6 +286,10 @@ def main():
     out["dtype"] = "f64" if head == "fp64" else "f32+f64"
     out["config"]["precision"] = head
     out["roofline"] = H["roofline"]
+    # the nominal peak above, and the rate a plain copy reaches here
+    cp = copy_peak(T)
+    out["roofline"]["copy_peak_measured"] = cp
+    out["roofline"]["frac_of_copy_peak"] = round(out["roofline"]["achieved"] / cp, 4) if cp else None
     T.close()
     if not args.no_cpu_baseline:
         # the FP64 GPU path is bit-identical to the reference, so the

@@ -86,6 +86,13 @@ int a5g_mg_setup(a5g_tree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
                  const int *bc_kind, const double *bc_val, int auto_ops);
 /* Change the value of a physical-face rule (the applied voltage) */
 int a5g_mg_set_bc_value(a5g_tree *t, int nb, double value);
+/* Helmholtz operator lpl(phi) - kappa2 phi for the multigrid calls below
+ * (smoother (sum of the neighbours - h^2 rhs) / (2D + kappa2 h^2)); 0 restores
+ * the Laplacian. No counterpart in the reference's drivers: it is the custom
+ * mg%box_op / mg%box_gsrb pair that Afivo's tutorial installs for an implicit
+ * operator (afivo/examples/tutorial/implicit.f90:147-193). Not combined with
+ * a permittivity (auto_ops with i_eps > 0). */
+int a5g_mg_set_helmholtz(a5g_tree *t, double kappa2);
 /* mg$D_fas_fmg, m_aX_mg.f90:164-203 */
 int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 /* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
@@ -159,6 +166,20 @@ int a5g_consistent_fluxes(a5g_tree *t, int n, const int *f_ixs);
 int a5g_update_solution(a5g_tree *t, double dt);
 /* average_dens on every box, whole boxes (src/streamer_2d.f90:492-496) */
 int a5g_average_dens(a5g_tree *t);
+/* Photoionization source into i_pho by the Helmholtz model, replacing
+ * set_photoionization's Monte-Carlo photons (src/streamer_2d.f90:544-565,
+ * src/m_photons.f90:207-627), which have no GPU counterpart here:
+ * I = |E| mu alpha n_e coeff, clamped at 0 (set_photoi_rate, :567-589,
+ * coeff = eta p_q / (p + p_q)), then for j < n_terms the multigrid solve of
+ * (lpl - kappa_j^2) S_j = -amp_j I with S_j = 0 on the domain boundary, and
+ * i_pho = sum_j S_j on leaves. amp_j = A_j p_O2^2 (m^-2), kappa_j =
+ * lambda_j p_O2 (m^-1) of the three-term fit of Bourdon et al. 2007. Each
+ * solve is one FMG plus V-cycles until max|r| <= tol max|rhs|; i_phi, i_rhs
+ * and i_tmp are its work variables. The multigrid options of the field solve
+ * are kept. *cycles = V-cycles run in total. */
+int a5g_photoi_helmholtz(a5g_tree *t, double coeff, int n_terms, const double *amp,
+                         const double *kappa, int i_phi, int i_rhs, int i_tmp,
+                         double tol, int max_cycles, int *cycles);
 /* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
  * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
 int a5g_box_summaries(a5g_tree *t, double *out);

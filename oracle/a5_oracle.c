@@ -54,6 +54,7 @@ struct otree {
   ivec *ri_add, *ri_rm;
   /* multigrid options */
   int i_phi, i_tmp, i_rhs, i_eps, n_down, n_up, n_base, auto_ops, mg_ok;
+  double helm; /* Helmholtz coefficient kappa^2 of the operator (0: Laplacian) */
   int bc_kind[6];
   double bc_val[6];
 };
@@ -1299,11 +1300,21 @@ void o_mg_setup(otree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
   t->n_up = n_up < 0 ? 2 : n_up;
   t->n_base = n_base < 0 ? 2 : n_base;
   t->auto_ops = auto_ops;
+  t->helm = 0;
   for (int nb = 0; nb < t->nnb; nb++) {
     t->bc_kind[nb] = bc_kind ? bc_kind[nb] : 0;
     t->bc_val[nb] = bc_val ? bc_val[nb] : 0;
   }
   t->mg_ok = 1;
+}
+
+void o_mg_set_helmholtz(otree *t, double kappa2) {
+  /* The Helmholtz operator lpl(phi) - kappa2 phi of the photoionization
+   * model. No reference counterpart (the reference's photoionization is
+   * Monte-Carlo); written from the discrete operator, the pattern of a custom
+   * mg%box_op / mg%box_gsrb (afivo/examples/tutorial/implicit.f90:147-193).
+   * Pinned by a direct sparse solve and by manufactured solutions. */
+  t->helm = kappa2;
 }
 
 static int box_tag(otree *t, int id) {
@@ -1341,18 +1352,19 @@ static void gsrb_lpl(otree *t, int id, int cntr, int cyl) {
         for (int i = i0; i <= nc; i += 2) {
           double den = i + ioff - 0.5;
           double rf1 = (double)(i + ioff - 1) / den, rf2 = (double)(i + ioff) / den;
-          p[I2(nc, i, j)] = 0.25 * (rf1 * p[I2(nc, i - 1, j)] + rf2 * p[I2(nc, i + 1, j)] +
-                                    p[I2(nc, i, j + 1)] + p[I2(nc, i, j - 1)] -
-                                    dx2 * r[I2(nc, i, j)]);
+          double num = rf1 * p[I2(nc, i - 1, j)] + rf2 * p[I2(nc, i + 1, j)] +
+                       p[I2(nc, i, j + 1)] + p[I2(nc, i, j - 1)] - dx2 * r[I2(nc, i, j)];
+          p[I2(nc, i, j)] = t->helm != 0 ? num / (4 + t->helm * dx2) : 0.25 * num;
         }
       }
     } else {
       for (int j = 1; j <= nc; j++) {
         int i0 = 2 - ((cntr ^ j) & 1);
-        for (int i = i0; i <= nc; i += 2)
-          p[I2(nc, i, j)] = 0.25 * (p[I2(nc, i + 1, j)] + p[I2(nc, i - 1, j)] +
-                                    p[I2(nc, i, j + 1)] + p[I2(nc, i, j - 1)] -
-                                    dx2 * r[I2(nc, i, j)]);
+        for (int i = i0; i <= nc; i += 2) {
+          double num = p[I2(nc, i + 1, j)] + p[I2(nc, i - 1, j)] + p[I2(nc, i, j + 1)] +
+                       p[I2(nc, i, j - 1)] - dx2 * r[I2(nc, i, j)];
+          p[I2(nc, i, j)] = t->helm != 0 ? num / (4 + t->helm * dx2) : 0.25 * num;
+        }
       }
     }
   } else {
@@ -1360,11 +1372,12 @@ static void gsrb_lpl(otree *t, int id, int cntr, int cyl) {
     for (int k = 1; k <= nc; k++)
       for (int j = 1; j <= nc; j++) {
         int i0 = 2 - ((cntr ^ (k + j)) & 1);
-        for (int i = i0; i <= nc; i += 2)
-          p[I3(nc, i, j, k)] = sixth * (p[I3(nc, i + 1, j, k)] + p[I3(nc, i - 1, j, k)] +
-                                        p[I3(nc, i, j + 1, k)] + p[I3(nc, i, j - 1, k)] +
-                                        p[I3(nc, i, j, k + 1)] + p[I3(nc, i, j, k - 1)] -
-                                        dx2 * r[I3(nc, i, j, k)]);
+        for (int i = i0; i <= nc; i += 2) {
+          double num = p[I3(nc, i + 1, j, k)] + p[I3(nc, i - 1, j, k)] +
+                       p[I3(nc, i, j + 1, k)] + p[I3(nc, i, j - 1, k)] +
+                       p[I3(nc, i, j, k + 1)] + p[I3(nc, i, j, k - 1)] - dx2 * r[I3(nc, i, j, k)];
+          p[I3(nc, i, j, k)] = t->helm != 0 ? num / (6 + t->helm * dx2) : sixth * num;
+        }
       }
   }
 }
@@ -1492,8 +1505,22 @@ static void box_lpld(otree *t, int id, int i_out) {
 
 void o_box_op(otree *t, int id, int i_out) {
   /* mg$D_auto_op (m_aX_mg.f90:603-635), or the default mg$D_box_lpl */
-  if (box_tag(t, id) == TAG_NORMAL) box_lpl(t, id, i_out, t->auto_ops && t->coord == 1);
-  else box_lpld(t, id, i_out);
+  if (box_tag(t, id) == TAG_NORMAL) {
+    box_lpl(t, id, i_out, t->auto_ops && t->coord == 1);
+    if (t->helm != 0) {
+      /* the Helmholtz term: lpl(phi) - kappa2 phi */
+      int nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+      double *p = CC(t, id, t->i_phi), *o = CC(t, id, i_out);
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            o[c] = o[c] - t->helm * p[c];
+          }
+    }
+  } else {
+    box_lpld(t, id, i_out);
+  }
 }
 
 void o_box_rstr(otree *t, int id, int iv) {
@@ -2031,6 +2058,123 @@ void o_fluxes_koren(otree *t, const o_vars *v, const otable *tbl, double dt) {
       free(gc);
     }
   }
+}
+
+void o_photoi_rate(otree *t, const o_vars *v, const otable *tbl, double coeff) {
+  /* set_photoi_rate, src/streamer_2d.f90:567-589: |E| mu alpha n_e coeff on
+   * leaves, clamped at zero (3D and cylindrical drivers alike) */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+      const double *E = CC(t, id, v->fld), *ne = CC(t, id, v->elec);
+      double *out = CC(t, id, v->pho);
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            int li;
+            double lf;
+            lt_loc(tbl, E[c], &li, &lf);
+            double alpha = lt_col_at(tbl, v->t_alpha, li, lf);
+            double mob = lt_col_at(tbl, v->t_mob, li, lf);
+            double tmp = E[c] * mob * alpha * ne[c] * coeff;
+            if (tmp < 0) tmp = 0;
+            out[c] = tmp;
+          }
+    }
+  }
+}
+
+void o_axpy_leaves(otree *t, int iin, int iout, double s, int add) {
+  /* out = s in (add = 0) or out += s in on leaf interiors */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+      const double *x = CC(t, id, iin);
+      double *y = CC(t, id, iout);
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            y[c] = add ? y[c] + s * x[c] : s * x[c];
+          }
+    }
+  }
+}
+
+static double leaf_maxabs(otree *t, int iv) {
+  double mx = 0;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+      const double *x = CC(t, id, iv);
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            double a = fabs(x[t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k)]);
+            if (a > mx) mx = a;
+          }
+    }
+  }
+  return mx;
+}
+
+int o_photoi_helmholtz(otree *t, const o_vars *v, const otable *tbl, double coeff,
+                       int n_terms, const double *amp, const double *kappa, int i_phi,
+                       int i_rhs, int i_tmp, double tol, int max_cycles, int *cycles) {
+  /* The three-term Helmholtz photoionization model (Bourdon et al. 2007) in
+   * place of the reference's Monte-Carlo photons: no reference counterpart,
+   * restated from the model (see o_mg_set_helmholtz). Returns 0, or -1 when a
+   * solve does not converge within max_cycles V-cycles. */
+  o_photoi_rate(t, v, tbl, coeff);
+  int s_phi = t->i_phi, s_rhs = t->i_rhs, s_tmp = t->i_tmp, s_eps = t->i_eps, s_ok = t->mg_ok;
+  int s_kind[6];
+  double s_val[6], s_helm = t->helm;
+  for (int i = 0; i < 6; i++) {
+    s_kind[i] = t->bc_kind[i];
+    s_val[i] = t->bc_val[i];
+    t->bc_kind[i] = 1;
+    t->bc_val[i] = 0;
+  }
+  t->i_phi = i_phi;
+  t->i_rhs = i_rhs;
+  t->i_tmp = i_tmp;
+  t->i_eps = -1;
+  t->mg_ok = 1;
+  int total = 0, status = 0;
+  o_axpy_leaves(t, v->pho, i_rhs, -amp[0], 0);
+  o_axpy_leaves(t, v->pho, v->pho, 0.0, 0);
+  for (int j = 0; j < n_terms && status == 0; j++) {
+    if (j > 0) o_axpy_leaves(t, i_rhs, i_rhs, amp[j] / amp[j - 1], 0);
+    t->helm = kappa[j] * kappa[j];
+    o_fas_fmg(t, 0, 1);
+    for (int cyc = 0;; cyc++) {
+      double res = o_max_residual(t), rhs = leaf_maxabs(t, i_rhs);
+      if (res <= tol * rhs) break;
+      if (cyc >= max_cycles) {
+        status = -1;
+        break;
+      }
+      o_fas_vcycle(t, t->max_lvl, 0);
+      total++;
+    }
+    if (status == 0) o_axpy_leaves(t, i_phi, v->pho, 1.0, 1);
+  }
+  t->i_phi = s_phi;
+  t->i_rhs = s_rhs;
+  t->i_tmp = s_tmp;
+  t->i_eps = s_eps;
+  t->mg_ok = s_ok;
+  t->helm = s_helm;
+  for (int i = 0; i < 6; i++) {
+    t->bc_kind[i] = s_kind[i];
+    t->bc_val[i] = s_val[i];
+  }
+  if (cycles) *cycles = total;
+  return status;
 }
 
 void o_update_solution(otree *t, const o_vars *v, const otable *tbl, double dt) {

@@ -48,6 +48,8 @@ void o_mg_setup(otree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
                 int n_down, int n_up, int n_base, const int *bc_kind,
                 const double *bc_val, int auto_ops);
 void o_fas_fmg(otree *t, int set_residual, int first_call);
+/* Helmholtz operator lpl(phi) - kappa2 phi (0: the Laplacian) */
+void o_mg_set_helmholtz(otree *t, double kappa2);
 void o_fas_vcycle(otree *t, int max_lvl, int set_residual);
 void o_box_gsrb(otree *t, int id, int cntr);
 void o_box_op(otree *t, int id, int i_out);
@@ -95,6 +97,13 @@ typedef struct {
   int photoi;                       /* add i_pho to the source term */
 } o_vars;
 
+/* photon production rate (set_photoi_rate) into v->pho on leaves */
+void o_photoi_rate(otree *t, const o_vars *v, const otable *tbl, double coeff);
+void o_axpy_leaves(otree *t, int iin, int iout, double s, int add);
+/* Helmholtz photoionization source into v->pho (three-term model) */
+int o_photoi_helmholtz(otree *t, const o_vars *v, const otable *tbl, double coeff,
+                       int n_terms, const double *amp, const double *kappa, int i_phi,
+                       int i_rhs, int i_tmp, double tol, int max_cycles, int *cycles);
 /* compute_fld's right-hand side on leaves, whole boxes: rhs = fac (ne - ni) */
 void o_space_charge(otree *t, const o_vars *v, double fac);
 /* fld_from_pot on every box */

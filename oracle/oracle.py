@@ -72,6 +72,7 @@ class OTree:
                  lvls_max=30, coord=0, n_boxes=100):
         self.dim = dim
         self.n_cell = n_cell
+        self.n_cc = n_cc
         self.L = lib()
         r_min = np.zeros(dim) if r_min is None else np.asarray(r_min, float)
         self._rm = _d(r_min)[0]
@@ -180,6 +181,9 @@ class OTree:
                           C.c_int(i_rhs), C.c_int(i_eps), C.c_int(n_down),
                           C.c_int(n_up), C.c_int(n_base), pk, pv,
                           C.c_int(int(auto_ops)))
+
+    def mg_set_helmholtz(self, kappa2):
+        self.L.o_mg_set_helmholtz(self.h, C.c_double(kappa2))
 
     def fas_fmg(self, set_residual, first_call):
         self.L.o_fas_fmg(self.h, C.c_int(int(set_residual)),
@@ -332,8 +336,8 @@ class OStreamTree(OTree):
     def table_get(self):
         return self.tbl.data()
 
-    def streamer_setup(self, dielectric_rule=True):
-        self.v = streamer_vars(dielectric_rule)
+    def streamer_setup(self, dielectric_rule=True, pho=-1):
+        self.v = streamer_vars(dielectric_rule, photoi=pho > 0)
 
     def space_charge(self, fac):
         space_charge(self, self.v, fac)
@@ -349,6 +353,21 @@ class OStreamTree(OTree):
 
     def average_dens(self):
         average_dens(self, self.v)
+
+    def photoi_helmholtz(self, coeff, amp, kappa, i_phi, i_rhs, i_tmp, tol=1e-8,
+                         max_cycles=30):
+        a, pa = _d(amp)
+        k, pk = _d(kappa)
+        v = OVars(*[getattr(self.v, f) for f, _ in OVars._fields_])
+        v.photoi = 1
+        cyc = C.c_int()
+        st = self.L.o_photoi_helmholtz(self.h, C.byref(v), self.tbl.h, C.c_double(coeff),
+                                       C.c_int(len(a)), pa, pk, C.c_int(i_phi), C.c_int(i_rhs),
+                                       C.c_int(i_tmp), C.c_double(tol), C.c_int(max_cycles),
+                                       C.byref(cyc))
+        if st != 0:
+            raise RuntimeError("photoionization: Helmholtz solve did not converge")
+        return cyc.value
 
     def box_summaries(self):
         return box_summaries(self, self.v)

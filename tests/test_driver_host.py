@@ -146,6 +146,35 @@ def test_run_refines_around_seed_then_steps(oracle_lib):
     assert lo > 0 and hi < 1.01 * cfg.seed_dens[0]
 
 
-def test_rejects_photoionization():
-    with pytest.raises(ValueError):
-        S.Config(photoi=True)
+def test_photoionization_model_units():
+    """coeff = eta p_q / (p + p_q) with p_q = 30 Torr (set_photoionization,
+    src/streamer_2d.f90:551-556); amp_j = A_j p_O2^2 and kappa_j = lambda_j
+    p_O2 converted from Torr and cm to SI, p_O2 = 0.2 bar = 150.0 Torr."""
+    coeff, amp, kappa = S.Config(photoi=True).photoi_model()
+    torr = 133.322368e-5
+    assert coeff == pytest.approx(0.05 * (30 * torr) / (1.0 + 30 * torr), rel=1e-15)
+    p = 0.2 / torr
+    assert p == pytest.approx(150.0123, rel=1e-6)
+    assert kappa == pytest.approx([0.0553 * p * 100, 0.1460 * p * 100, 0.89 * p * 100])
+    assert amp == pytest.approx([1.986e-4 * p * p * 1e4, 0.0051 * p * p * 1e4,
+                                 0.4886 * p * p * 1e4])
+    assert kappa[0] == pytest.approx(829.6, rel=1e-3)
+
+
+def test_photoionization_run_on_the_oracle(oracle_lib):
+    """The driver with Helmholtz photoionization steps on the oracle: the
+    source is positive where the discharge ionizes and the electron count
+    grows at least as fast as without it."""
+    kw = dict(variant="2d", lvls_max=5)
+    runs = [S.StreamerRun(O.OStreamTree, S.Config(photoi=p, **kw)) for p in (False, True)]
+    for r in runs:
+        r.initialise()
+        r.step_block()
+    T = runs[1].T
+    b = T.boxes()
+    pho = max(T.get_cc(i + 1, S.I_PHO).max() for i in range(len(b["lvl"]))
+              if b["in_use"][i] and b["children"][i][0] == 0)
+    assert pho > 0
+    ne = [sum(r.T.get_cc(i + 1, S.I_ELEC).sum() for i in range(len(r.T.boxes()["lvl"]))
+              if r.T.boxes()["in_use"][i]) for r in runs]
+    assert ne[1] >= ne[0]

@@ -28,6 +28,8 @@ CASES = {
     # 2D with a strong field so that the mesh grows and shrinks as it runs
     "2d_refining": dict(variant="2d", lvls_max=8, applied_fld=1.0e7,
                         seed_width=[0.5e-3], seed_dens=[5e19], bg_dens=1e12),
+    # Helmholtz photoionization at every step (three multigrid solves)
+    "2d_photoi": dict(variant="2d", lvls_max=7, photoi=True),
 }
 
 
@@ -40,7 +42,10 @@ def _same_state(ta, tb, dim):
         if not ba["in_use"][i]:
             continue
         nc = int(ba["n_cell"][i])
-        for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD, S.I_RHS):
+        ivs = (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD, S.I_RHS)
+        if ta.n_cc > S.N_VAR_CELL:
+            ivs += (S.I_PHO,)
+        for iv in ivs:
             np.testing.assert_array_equal(
                 ta.get_cc(i + 1, iv, nc), tb.get_cc(i + 1, iv, nc),
                 err_msg="box %d lvl %d var %d" % (i + 1, ba["lvl"][i], iv))
