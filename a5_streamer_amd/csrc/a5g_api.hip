@@ -146,6 +146,7 @@ void Pool::sync_topology() {
   if (!host.empty()) up(d_lists, host.data(), host.size() * sizeof(int));
   hcheck(hipStreamSynchronize(stream), "topology sync");
   topo_dirty = false;
+  topo_version++;
 }
 
 // new slot n <- old slot map[n] for every variable
@@ -254,7 +255,8 @@ void Pool::release() {
   f(cc32);
   cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
-  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red);
+  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs);
+  d_flux_pairs = nullptr;
   if (h_red) (void)hipHostFree(h_red);
   if (stream) (void)hipStreamDestroy(stream);
   cc = fc = nullptr;
@@ -359,7 +361,8 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // A5G_GRAPHS=0 runs every multigrid call as plain launches
       const char* g = std::getenv("A5G_GRAPHS");
       h->P.graphs_on = !(g && g[0] == '0');
-      a5g::hcheck(hipMalloc((void**)&h->P.d_red, 64), "hipMalloc");
+      a5g::hcheck(hipMalloc((void**)&h->P.d_red, 2 * a5g::kRedSlots * sizeof(unsigned long long)),
+                  "hipMalloc");
       h->P.ensure_capacity(h->P.tree.cap);
     }
   } catch (...) {

@@ -1277,8 +1277,9 @@ __global__ __launch_bounds__(256) void k_minmax(Dev a, const int* __restrict__ l
       lo = fmin(lo, wl[w]);
       hi = fmax(hi, wh[w]);
     }
-    atomicMin(&out[0], ord_key(lo));
-    atomicMax(&out[1], ord_key(hi));
+    const int sl = 2 * (blockIdx.x % kRedSlots);
+    atomicMin(&out[sl], ord_key(lo));
+    atomicMax(&out[sl + 1], ord_key(hi));
   }
 }
 
@@ -1828,12 +1829,39 @@ void mg_vcycle(Pool& P, int max_lvl, bool set_residual) {
   });
 }
 
+// The slots of the reduction scratch: set every pair to (lo_init, hi_init)
+// before the launches, combine them after.
+static void red_init(Pool& P, unsigned long long lo_init, unsigned long long hi_init) {
+  static thread_local std::vector<unsigned long long> init;
+  init.assign(2 * kRedSlots, 0);
+  for (int s = 0; s < kRedSlots; s++) {
+    init[2 * s] = lo_init;
+    init[2 * s + 1] = hi_init;
+  }
+  check(hipMemcpyAsync(P.d_red, init.data(), init.size() * sizeof(unsigned long long),
+                       hipMemcpyHostToDevice, P.stream), "reduction init");
+}
+
+// min over the even slots and max over the odd ones (or max of both when
+// both_max), after the stream has drained
+static void red_read(Pool& P, bool both_max, unsigned long long* k0, unsigned long long* k1) {
+  std::vector<unsigned long long> out(2 * kRedSlots);
+  check(hipMemcpyAsync(out.data(), P.d_red, out.size() * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, P.stream), "reduction read");
+  check(hipStreamSynchronize(P.stream), "reduction sync");
+  unsigned long long a = out[0], b = out[1];
+  for (int s = 1; s < kRedSlots; s++) {
+    a = both_max ? std::max(a, out[2 * s]) : std::min(a, out[2 * s]);
+    b = std::max(b, out[2 * s + 1]);
+  }
+  *k0 = a;
+  *k1 = b;
+}
+
 void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
                   double* lo, double* hi) {
   P.sync_topology();
-  unsigned long long init[2] = {~0ull, 0ull};
-  check(hipMemcpyAsync(P.d_red, init, sizeof(init), hipMemcpyHostToDevice, P.stream),
-        "minmax init");
+  red_init(P, ~0ull, 0ull);
   const int D = P.tree.dim;
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
     const int n = leaves_only ? P.n_leaves(lvl) : P.n_ids(lvl);
@@ -1852,9 +1880,7 @@ void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
     post_launch("minmax");
   }
   unsigned long long out[2];
-  check(hipMemcpyAsync(out, P.d_red, sizeof(out), hipMemcpyDeviceToHost, P.stream),
-        "minmax read");
-  check(hipStreamSynchronize(P.stream), "minmax sync");
+  red_read(P, false, &out[0], &out[1]);
   auto unkey = [](unsigned long long k) {
     unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
     double v;
@@ -1901,15 +1927,15 @@ __global__ __launch_bounds__(256) void k_maxdiff(Dev a, const int* __restrict__ 
       md = fmax(md, w1[w]);
       mb = fmax(mb, w2[w]);
     }
-    atomicMax(&out[0], ord_key(md));
-    atomicMax(&out[1], ord_key(mb));
+    const int sl = 2 * (blockIdx.x % kRedSlots);
+    atomicMax(&out[sl], ord_key(md));
+    atomicMax(&out[sl + 1], ord_key(mb));
   }
 }
 
 void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b) {
   P.sync_topology();
-  unsigned long long init[2] = {0ull, 0ull};
-  check(hipMemcpyAsync(P.d_red, init, sizeof(init), hipMemcpyHostToDevice, P.stream), "init");
+  red_init(P, 0ull, 0ull);
   const int D = P.tree.dim;
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
     const int n = P.n_leaves(lvl);
@@ -1925,8 +1951,7 @@ void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b) 
     post_launch("maxdiff");
   }
   unsigned long long out[2];
-  check(hipMemcpyAsync(out, P.d_red, sizeof(out), hipMemcpyDeviceToHost, P.stream), "read");
-  check(hipStreamSynchronize(P.stream), "sync");
+  red_read(P, true, &out[0], &out[1]);
   double v[2];
   for (int m = 0; m < 2; m++) {
     unsigned long long u = (out[m] >> 63) ? (out[m] & 0x7fffffffffffffffull) : ~out[m];
@@ -1987,7 +2012,7 @@ __global__ __launch_bounds__(256) void k_resid_to32(Dev a, const int* __restrict
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wm[w]);
-    atomicMax(mx, ord_key(m));
+    atomicMax(&mx[blockIdx.x % kRedSlots], ord_key(m));
   }
 }
 
@@ -2043,8 +2068,7 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
   double rel = 0;
   for (;; it++) {
     // r = rhs - A phi (FP64) -> FP32 rhs of the correction, and max |r|
-    unsigned long long init = 0;
-    check(hipMemcpyAsync(P.d_red, &init, sizeof(init), hipMemcpyHostToDevice, P.stream), "red");
+    red_init(P, 0ull, 0ull);
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
       const int n = P.n_leaves(lvl);
       if (n == 0) continue;
@@ -2062,9 +2086,10 @@ static void mixed_solve(Pool& P, double tol, int max_outer, int n_inner, int* ou
       post_launch("resid_to32");
       P.prof_end(pr);
     }
-    unsigned long long key;
-    check(hipMemcpyAsync(&key, P.d_red, sizeof(key), hipMemcpyDeviceToHost, P.stream), "red");
-    check(hipStreamSynchronize(P.stream), "sync");
+    // the residual maxima sit in the first kRedSlots words
+    unsigned long long k0, k1, key;
+    red_read(P, true, &k0, &k1);
+    key = std::max(k0, k1);
     unsigned long long u = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
     double rmax;
     std::memcpy(&rmax, &u, 8);
@@ -2213,6 +2238,10 @@ static void restrict_levels(Pool& P, int iv, bool plain) {
     if (n == 0) continue;
     const int nc = P.nc_lvl(lvl + 1);
     const int hnc = nc / 2;
+    // 1 + 2^-D words per child cell (read the child, write the parent)
+    const int pr = P.prof_begin(PK_RESTRICT, lvl + 1,
+                                (1.0 + 1.0 / (1 << P.tree.dim)) * 8 * n *
+                                    cells_of(P.tree.dim, nc, false));
     if (P.tree.dim == 2) {
       Geo g = geo(n, hnc * hnc);
       hipLaunchKernelGGL((k_restrict_box<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
@@ -2223,6 +2252,7 @@ static void restrict_levels(Pool& P, int iv, bool plain) {
                          P.ids(lvl + 1), n, nc, g.tpb, iv, plain);
     }
     post_launch("restrict");
+    P.prof_end(pr);
   }
 }
 

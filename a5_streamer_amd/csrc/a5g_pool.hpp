@@ -44,7 +44,9 @@ struct Dev {
 // Per-launch timing with HIP events on the pool's stream (kernel classes
 // below), used by the benchmark to measure kernel durations in place.
 enum ProfKind { PK_GSRB = 0, PK_GC, PK_RESID_RSTR, PK_COARSE_RHS, PK_CORR, PK_COPY, PK_OTHER,
-                PK_COUNT };
+                // the streamer's box procedures
+                PK_SPACE_CHARGE, PK_FIELD, PK_KOREN, PK_FLUX_FIX, PK_UPDATE, PK_AVERAGE,
+                PK_RESTRICT, PK_PHOTOI_RATE, PK_COUNT };
 
 struct ProfRec {
   int kind, lvl;
@@ -70,6 +72,8 @@ struct MgOpts {
   bool ready = false;
 };
 
+constexpr int kRedSlots = 256;
+
 struct Pool {
   Tree tree;
   // No GPU visible: the tree core still works (topology calls), every
@@ -92,7 +96,15 @@ struct Pool {
   // host offsets into d_lists, per level: ids, leaves, parents
   std::vector<size_t> off_ids, off_leaves, off_parents;
   bool topo_dirty = true;
-  // reduction scratch
+  uint64_t topo_version = 0;  // counts topology uploads
+  // (parent, direction) pairs of a$D_consistent_fluxes per level, on the
+  // device, for topology version flux_pairs_version
+  int2* d_flux_pairs = nullptr;
+  size_t flux_pairs_cap = 0;
+  std::vector<size_t> flux_pairs_off, flux_pairs_n;
+  uint64_t flux_pairs_version = ~0ull;
+  // reduction scratch: kRedSlots pairs of ordered-integer keys, one pair per
+  // group of workgroups, so that atomics do not all meet on one address
   double* d_red = nullptr;
   double* h_red = nullptr;  // pinned
   MgOpts mg;

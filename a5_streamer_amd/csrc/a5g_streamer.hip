@@ -638,9 +638,12 @@ void space_charge(Pool& P, const StVars& v, double fac) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl), bl = box_cells(D, nc, true);
     G2 g = g2(n, bl);
+    // algorithmic bytes: 3 words per leaf cell (read n_e, n_i, write rhs)
+    const int pr = P.prof_begin(PK_SPACE_CHARGE, lvl, 3.0 * 8 * n * box_cells(D, nc, false));
     hipLaunchKernelGGL(k_space_charge, dim3(g.grid), dim3(256), 0, P.stream, P.dev(),
                        P.leaves(lvl), n, bl, g.tpb, v.elec, v.pion, v.rhs, fac);
     chk(hipGetLastError(), "space_charge");
+    P.prof_end(pr);
   }
 }
 
@@ -652,6 +655,8 @@ void fld_from_pot(Pool& P, const StVars& v) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    // 2 + D words per cell (read phi and eps, write D face fields and |E|)
+    const int pr = P.prof_begin(PK_FIELD, lvl, (2.0 + D) * 8 * n * box_cells(D, nc, false));
     if (D == 2)
       hipLaunchKernelGGL((k_fld<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
                          n, nc, g.tpb, v.phi, v.eps, v.fld, v.f_fld);
@@ -659,6 +664,7 @@ void fld_from_pot(Pool& P, const StVars& v) {
       hipLaunchKernelGGL((k_fld<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
                          n, nc, g.tpb, v.phi, v.eps, v.fld, v.f_fld);
     chk(hipGetLastError(), "fld_from_pot");
+    P.prof_end(pr);
   }
 }
 
@@ -673,6 +679,9 @@ void fluxes_koren(Pool& P, const StVars& v, double dt) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    // 2 + 2D words per leaf cell (read n_e, |E| and D face fields, write D
+    // face fluxes)
+    const int pr = P.prof_begin(PK_KOREN, lvl, (2.0 + 2 * D) * 8 * n * box_cells(D, nc, false));
     if (D == 2)
       hipLaunchKernelGGL((k_koren<2>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
                          P.leaves(lvl), n, nc, g.tpb, v.elec, v.fld, v.f_elec, v.f_fld, v.t_mob,
@@ -682,38 +691,60 @@ void fluxes_koren(Pool& P, const StVars& v, double dt) {
                          P.leaves(lvl), n, nc, g.tpb, v.elec, v.fld, v.f_elec, v.f_fld, v.t_mob,
                          v.t_dif, fac);
     chk(hipGetLastError(), "fluxes_koren");
+    P.prof_end(pr);
   }
 }
 
 void consistent_fluxes(Pool& P, int f_elec) {
-  // a$D_consistent_fluxes: (parent, direction) pairs whose neighbour is a leaf
+  // a$D_consistent_fluxes: (parent, direction) pairs whose neighbour is a
+  // leaf, per level; the list is rebuilt when the topology changes
   P.sync_topology();
   const auto& T = P.tree;
   const int D = T.dim;
-  for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
+  const int nl = T.max_lvl - T.min_lvl;
+  if (P.flux_pairs_version != P.topo_version) {
     std::vector<int2> pairs;
-    for (int id : T.lv_parents(lvl))
-      for (int nb = 1; nb <= T.nnb; nb++) {
-        const int nb_id = T.nbs[id * 6 + nb - 1];
-        if (nb_id > 0 && !T.has_children(nb_id)) pairs.push_back(make_int2(id, nb));
-      }
-    if (pairs.empty()) continue;
+    P.flux_pairs_off.assign(std::max(nl, 0), 0);
+    P.flux_pairs_n.assign(std::max(nl, 0), 0);
+    for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
+      P.flux_pairs_off[lvl - T.min_lvl] = pairs.size();
+      for (int id : T.lv_parents(lvl))
+        for (int nb = 1; nb <= T.nnb; nb++) {
+          const int nb_id = T.nbs[id * 6 + nb - 1];
+          if (nb_id > 0 && !T.has_children(nb_id)) pairs.push_back(make_int2(id, nb));
+        }
+      P.flux_pairs_n[lvl - T.min_lvl] = pairs.size() - P.flux_pairs_off[lvl - T.min_lvl];
+    }
+    if (pairs.size() > P.flux_pairs_cap || !P.d_flux_pairs) {
+      if (P.d_flux_pairs) chk(hipFree(P.d_flux_pairs), "hipFree");
+      P.d_flux_pairs = nullptr;
+      P.flux_pairs_cap = std::max<size_t>(pairs.size(), 64);
+      chk(hipMalloc((void**)&P.d_flux_pairs, P.flux_pairs_cap * sizeof(int2)), "hipMalloc");
+    }
+    if (!pairs.empty()) {
+      chk(hipMemcpyAsync(P.d_flux_pairs, pairs.data(), pairs.size() * sizeof(int2),
+                         hipMemcpyHostToDevice, P.stream), "pairs");
+      chk(hipStreamSynchronize(P.stream), "pairs sync");
+    }
+    P.flux_pairs_version = P.topo_version;
+  }
+  for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
+    const size_t np = P.flux_pairs_n[lvl - T.min_lvl];
+    if (np == 0) continue;
+    const int2* d_pairs = P.d_flux_pairs + P.flux_pairs_off[lvl - T.min_lvl];
     const int nc = P.nc_lvl(lvl);
-    int2* d_pairs = nullptr;
-    chk(hipMalloc((void**)&d_pairs, pairs.size() * sizeof(int2)), "hipMalloc");
-    chk(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
-                       P.stream), "pairs");
-    const long work = (long)pairs.size() * (D == 2 ? nc : nc * nc);
+    const long work = (long)np * (D == 2 ? nc : nc * nc);
     const int grid = (int)std::min<long>((work + 255) / 256, 65535);
+    // one coarse face written per 2^(D-1) fine faces read
+    const int pr = P.prof_begin(PK_FLUX_FIX, lvl, (1.0 + (1 << (D - 1))) * 8 * work / (1 << (D - 1)));
     if (D == 2)
       hipLaunchKernelGGL((k_flux_fix<2>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
-                         (int)pairs.size(), nc, f_elec);
+                         (int)np, nc, f_elec);
     else
       hipLaunchKernelGGL((k_flux_fix<3>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
-                         (int)pairs.size(), nc, f_elec);
+                         (int)np, nc, f_elec);
     chk(hipGetLastError(), "consistent_fluxes");
-    chk(hipStreamSynchronize(P.stream), "consistent_fluxes sync");
-    chk(hipFree(d_pairs), "hipFree");
+    P.prof_end(pr);
   }
 }
 
@@ -727,6 +758,10 @@ void update_solution(Pool& P, const StVars& v, double dt) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, box_cells(D, nc, false));
+    // 6 + D words per leaf cell (5 + D cylindrical): read n_e, n_i, |E|, eps
+    // and the 2D face fluxes counted once per cell, write n_e, n_i
+    const int pr = P.prof_begin(PK_UPDATE, lvl,
+                                ((cyl ? 5.0 : 6.0) + D) * 8 * n * box_cells(D, nc, false));
     if (D == 3)
       hipLaunchKernelGGL((k_update<3, false>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
                          P.leaves(lvl), n, nc, g.tpb, v.elec, v.pion, v.fld, v.eps, v.pho,
@@ -740,6 +775,7 @@ void update_solution(Pool& P, const StVars& v, double dt) {
                          P.leaves(lvl), n, nc, g.tpb, v.elec, v.pion, v.fld, v.eps, v.pho,
                          v.f_elec, v.t_alpha, v.t_eta, v.dielectric_rule, dt);
     chk(hipGetLastError(), "update_solution");
+    P.prof_end(pr);
   }
 }
 
@@ -751,9 +787,12 @@ void average_dens(Pool& P, const StVars& v) {
     if (!n) continue;
     const int bl = box_cells(D, P.nc_lvl(lvl), true);
     G2 g = g2(n, bl);
+    // 6 words per cell (read 4, write 2)
+    const int pr = P.prof_begin(PK_AVERAGE, lvl, 6.0 * 8 * n * box_cells(D, P.nc_lvl(lvl), false));
     hipLaunchKernelGGL(k_average, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), P.ids(lvl), n,
                        bl, g.tpb, v.elec, v.elec_old, v.pion, v.pion_old);
     chk(hipGetLastError(), "average_dens");
+    P.prof_end(pr);
   }
 }
 
@@ -766,6 +805,7 @@ void photoi_rate(Pool& P, const StVars& v, double coeff) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, box_cells(D, nc, false));
+    const int pr = P.prof_begin(PK_PHOTOI_RATE, lvl, 3.0 * 8 * n * box_cells(D, nc, false));
     if (D == 3)
       hipLaunchKernelGGL((k_photoi_rate<3>), dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb,
                          P.leaves(lvl), n, nc, g.tpb, v.fld, v.elec, v.pho, v.t_mob, v.t_alpha,
@@ -775,6 +815,7 @@ void photoi_rate(Pool& P, const StVars& v, double coeff) {
                          P.leaves(lvl), n, nc, g.tpb, v.fld, v.elec, v.pho, v.t_mob, v.t_alpha,
                          coeff);
     chk(hipGetLastError(), "photoi_rate");
+    P.prof_end(pr);
   }
 }
 

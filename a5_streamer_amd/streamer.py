@@ -283,12 +283,37 @@ class StreamerRun:
         return [tuple(axes[d][p[self.dim - 1 - d]] for d in range(self.dim))
                 for p in np.ndindex(*([n] * self.dim))]
 
+    def _far_from_seeds(self, nc, dr, rmin):
+        """True when every cell centre of the box (ghosts included) lies more
+        than 30 widths from every Gaussian seed segment: exp(-(d/w)^2) is then
+        exactly 0 in double precision (it underflows below d/w = 27.3), so each
+        density is exactly bg_dens."""
+        c = self.c
+        lo = np.asarray(rmin) - 0.5 * dr
+        hi = np.asarray(rmin) + (nc + 0.5) * dr
+        for k in range(len(c.seed_dens)):
+            if c.seed_falloff[k] != 2:
+                return False
+            s_lo = np.minimum(c.seed_r0[k], c.seed_r1[k])
+            s_hi = np.maximum(c.seed_r0[k], c.seed_r1[k])
+            gap = np.maximum(0.0, np.maximum(s_lo - hi, lo - s_hi))
+            if math.sqrt(float(np.sum(gap * gap))) <= 30 * c.seed_width[k]:
+                return False
+        return True
+
     def _box_fields(self, b, i, want_dens):
         c = self.c
-        pts = self._cell_coords(int(b["n_cell"][i]), b["dr"][i], b["r_min"][i])
-        eps = np.array([self._eps_at(r) for r in pts])
+        nc, dr, rmin = int(b["n_cell"][i]), b["dr"][i], b["r_min"][i]
+        n_all = (nc + 2) ** self.dim
+        if c.epsilon_diel == 1.0:
+            eps = np.ones(n_all)
+        else:
+            eps = np.array([self._eps_at(r) for r in self._cell_coords(nc, dr, rmin)])
         if not want_dens:
             return None, eps
+        if self._far_from_seeds(nc, dr, rmin):
+            return np.full(n_all, float(c.bg_dens)), eps
+        pts = self._cell_coords(nc, dr, rmin)
         elec = np.empty(len(pts))
         for q, r in enumerate(pts):
             val = c.bg_dens

@@ -203,6 +203,81 @@ def roofline(T, precision, fn):
     }
 
 
+# kernel classes of the streamer step (the library's event-timing kinds)
+STEP_KINDS = {0: "gsrb_half_sweep", 1: "ghost_cells", 2: "residual_restrict",
+              3: "coarse_rhs", 4: "correction", 5: "copy", 7: "space_charge",
+              8: "field_from_potential", 9: "koren_flux", 10: "flux_fix", 11: "update",
+              12: "average", 13: "restrict", 14: "photoi_rate"}
+
+
+def streamer_step(args):
+    """The full 3D streamer step on one GPU: the 3D driver (a 64 mm cube of
+    8^3-cell boxes on a base of step_base^3 boxes, 2 MV/m, the air seed and transport of
+    the 2D benchmark moved to (32, 32, 56) mm, Helmholtz photoionization)
+    after its initial refinement, stepped through the library. Prints one
+    JSON line: leaf Mcell-steps/s and, per kernel class, the time share and
+    the rate of algorithmic bytes over the event-timed launch durations."""
+    import a5_streamer_amd as A
+    from a5_streamer_amd import streamer as ST
+    cfg = ST.Config(variant="3d", domain_len=64e-3, n_base=args.step_base,
+                    lvls_max=args.step_lvls,
+                    applied_fld=2.0e6, seed_r0=[[32e-3, 32e-3, 56e-3]],
+                    seed_r1=[[32e-3, 32e-3, 56e-3]], photoi=not args.no_photoi,
+                    n_boxes=20000)
+    run = ST.StreamerRun(A.Tree, cfg)
+    t0 = time.perf_counter()
+    run.initialise()
+    t_init = time.perf_counter() - t0
+    cells = run.leaf_cells()
+    blocks = max(1, args.steps // cfg.steps_amr)
+    for _ in range(args.warmup):
+        run.step_block()
+    T = run.T
+    T.sync()
+    c0 = run.leaf_cells()
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        run.step_block()
+    T.sync()
+    dt = time.perf_counter() - t0
+    c1 = run.leaf_cells()
+    n_steps = blocks * cfg.steps_amr
+    T.prof_enable(True)
+    T.sync()
+    t1 = time.perf_counter()
+    run.step_block()
+    T.sync()
+    t_prof = time.perf_counter() - t1
+    kinds = {}
+    tot_ms = 0.0
+    for k, name in STEP_KINDS.items():
+        n, ms, by = T.prof_read(k, -999)
+        if n:
+            kinds[name] = {"launches": n, "ms_per_step": round(ms / cfg.steps_amr, 3),
+                           "alg_GBps": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
+            tot_ms += ms
+    T.prof_enable(False)
+    for v in kinds.values():
+        v["share"] = round(v["ms_per_step"] * cfg.steps_amr / tot_ms, 4)
+    line = {
+        "metric": "3D streamer step, leaf Mcell-steps/s",
+        "value": round(0.5 * (c0 + c1) * n_steps / dt / 1e6, 3),
+        "unit": "Mcell-steps/s", "steps": n_steps, "warmup_blocks": args.warmup,
+        "ms_per_step": round(dt * 1e3 / n_steps, 3), "dtype": "f64",
+        "config": {"workload": "3D streamer: 64 mm cube, 8^3-cell boxes, %d^3 base boxes, "
+                               "2 MV/m, seed at (32, 32, 56) mm, lvls_max %d, %s" % (
+                                   args.step_base, args.step_lvls, "Helmholtz photoionization"
+                                   if cfg.photoi else "no photoionization"),
+                   "leaf_cells_start": c0, "leaf_cells_end": c1, "leaf_cells_init": cells,
+                   "init_s": round(t_init, 1)},
+        "kernels": kinds,
+        "profiled_block_s": round(t_prof, 3),
+        "kernel_ms_per_step": round(tot_ms / cfg.steps_amr, 3),
+    }
+    print(json.dumps(line), flush=True)
+    T.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,11 +286,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="smaller octree (testing)")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--streamer-step", action="store_true",
+                    help="measure the full 3D streamer step instead (own JSON line)")
+    ap.add_argument("--step-lvls", type=int, default=7)
+    ap.add_argument("--step-base", type=int, default=16)
+    ap.add_argument("--no-photoi", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.streamer_step:
+        streamer_step(args)
+        return
 
     from a5_streamer_amd.workloads import OctreePoisson
     W = OctreePoisson(uniform_lvl=3, fine_lvl=4, refine_frac=(5, 5, 8)) if args.small \

@@ -178,3 +178,25 @@ def test_photoionization_run_on_the_oracle(oracle_lib):
     ne = [sum(r.T.get_cc(i + 1, S.I_ELEC).sum() for i in range(len(r.T.boxes()["lvl"]))
               if r.T.boxes()["in_use"][i]) for r in runs]
     assert ne[1] >= ne[0]
+
+
+def test_far_field_initial_densities_are_exact():
+    """Boxes more than 30 seed widths from every Gaussian seed take the
+    background density without evaluating the seeds: the per-cell values
+    there are exactly bg_dens, so both give the same bits."""
+    cfg = S.Config(variant="3d", domain_len=64e-3, n_base=8, lvls_max=5,
+                   seed_r0=[[32e-3, 32e-3, 56e-3]], seed_r1=[[32e-3, 32e-3, 58e-3]])
+    run = S.StreamerRun.__new__(S.StreamerRun)
+    run.c, run.dim = cfg, 3
+    rng = np.random.default_rng(0)
+    n_fast = 0
+    for _ in range(60):
+        rmin = rng.uniform(0, 56e-3, 3)
+        dr = 1e-3 * 0.5 ** int(rng.integers(0, 4))
+        b = {"n_cell": [8], "dr": [dr], "r_min": [rmin]}
+        n_fast += run._far_from_seeds(8, dr, rmin)
+        slow = np.array([cfg.bg_dens + cfg.seed_dens[0] * S.dens_line(
+            p, cfg.seed_r0[0], cfg.seed_r1[0], cfg.seed_width[0], 2)
+            for p in run._cell_coords(8, dr, rmin)])
+        np.testing.assert_array_equal(run._box_fields(b, 0, True)[0], slow)
+    assert 0 < n_fast < 60
