@@ -41,6 +41,10 @@ struct Geo {
   int tpb, grid;
 };
 
+// Bytes of potential and rhs above which a level's half-sweeps stream
+// (three times the Infinity Cache)
+constexpr double kStreamBytes = 768.0 * (1 << 20);
+
 static Geo geo(int n_boxes, int work) {
   int tpb = std::min(256, std::max(64, pow2ceil(work)));
   int bpb = 256 / tpb;
@@ -253,7 +257,7 @@ __device__ __forceinline__ int xcd_block() {
 // current state (the values the reference's ghost pass after the previous
 // half-sweep would store). Then every lane updates cells of colour cntr&1
 // from LDS and rhs (loaded into registers before the barrier).
-template <int D, bool CYL, int NC, typename T, bool EPS>
+template <int D, bool CYL, int NC, typename T, bool EPS, bool NT>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
   constexpr int N2 = NC + 2;
@@ -304,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
       const int q = lt + u * TPB;
-      if (q < NCOL) rv[u] = r[q];
+      if (q < NCOL) rv[u] = NT ? __builtin_nontemporal_load(r + q) : r[q];
     }
     T gv[NITG];
     int gs[NITG], gnb[NITG], gg[NITG][3];
@@ -407,7 +411,8 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
       if (a.helm != 0) v = num / (T(2 * D) + (T)a.helm * dx2);
       else v = (D == 2 ? T(0.25) : T(1) / T(6)) * num;
     }
-    pc[q] = v;
+    if (NT) __builtin_nontemporal_store(v, pc + q);
+    else pc[q] = v;
   }
 }
 
@@ -421,14 +426,25 @@ static bool launch_gsrb_lds_t(Pool& P, const Dev& a, const int* list, int n, int
                        cntr, fused, i_phi, i_rhs);
   };
   const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
+  // A level whose potential and rhs far exceed the 256 MB Infinity Cache
+  // streams: the rhs is read and the new colour written with non-temporal
+  // accesses, which keeps L2 for the neighbour faces the next boxes read.
+  // Smaller levels stay cached from one half-sweep to the next.
+  const double box_cells = D == 2 ? (nc + 2.0) * (nc + 2) : (nc + 2.0) * (nc + 2) * (nc + 2);
+  const double level_bytes = 2.0 * n * box_cells * sizeof(T);
+  const bool nt = level_bytes > kStreamBytes;
   switch (nc) {
-    case 2: go(k_gsrb_lds<D, CYL, 2, T, EPS>, ncol); return true;
-    case 4: go(k_gsrb_lds<D, CYL, 4, T, EPS>, ncol); return true;
-    case 8: go(k_gsrb_lds<D, CYL, 8, T, EPS>, ncol); return true;
-    case 16: go(k_gsrb_lds<D, CYL, 16, T, EPS>, ncol); return true;
+    case 2: go(k_gsrb_lds<D, CYL, 2, T, EPS, false>, ncol); return true;
+    case 4: go(k_gsrb_lds<D, CYL, 4, T, EPS, false>, ncol); return true;
+    case 8: go(k_gsrb_lds<D, CYL, 8, T, EPS, false>, ncol); return true;
+    case 16:
+      if (nt) go(k_gsrb_lds<D, CYL, 16, T, EPS, true>, ncol);
+      else go(k_gsrb_lds<D, CYL, 16, T, EPS, false>, ncol);
+      return true;
     case 32:
       if (D == 2) {
-        go(k_gsrb_lds<2, CYL, 32, T, EPS>, ncol);
+        if (nt) go(k_gsrb_lds<2, CYL, 32, T, EPS, true>, ncol);
+        else go(k_gsrb_lds<2, CYL, 32, T, EPS, false>, ncol);
         return true;
       }
       return false;
