@@ -324,16 +324,34 @@ class StreamerRun:
         return elec, eps
 
     def set_init_cond(self):
+        """set_init_cond (src/streamer_2d.f90:218-248) on every box in use.
+        A tree with whole-variable uploads (set_cc_all) gets one upload per
+        variable; box slots are box_len(n_cell) long, a coarser box filling
+        the start of its slot."""
         T = self.T
         b = T.boxes()
-        for i in range(len(b["lvl"])):
+        n = len(b["lvl"])
+        whole = hasattr(T, "set_cc_all")
+        if whole:
+            bl = T.box_len(T.n_cell)
+            elec_all, eps_all = np.zeros((n, bl)), np.zeros((n, bl))
+        for i in range(n):
             if not b["in_use"][i]:
                 continue
             elec, eps = self._box_fields(b, i, True)
+            if whole:
+                elec_all[i, :len(elec)] = elec
+                eps_all[i, :len(eps)] = eps
+                continue
             T.set_cc(i + 1, I_ELEC, elec)
             T.set_cc(i + 1, I_PION, elec)
             T.set_cc(i + 1, I_PHI, np.zeros_like(elec))
             T.set_cc(i + 1, I_EPS, eps)
+        if whole:
+            T.set_cc_all(I_ELEC, elec_all)
+            T.set_cc_all(I_PION, elec_all)
+            T.set_cc_all(I_PHI, np.zeros((n, bl)))
+            T.set_cc_all(I_EPS, eps_all)
 
     def _set_eps(self, ids):
         if self.c.epsilon_diel == 1.0:
