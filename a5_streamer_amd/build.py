@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liba5g.so")
 
-SOURCES = ["a5g_tree.cpp", "a5g_api.hip", "a5g_mg.hip", "a5g_init.hip", "a5g_streamer.hip"]
+SOURCES = ["a5g_tree.cpp", "a5g_api.hip", "a5g_mg.hip", "a5g_init.hip", "a5g_streamer.hip",
+           "a5g_diag.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -238,6 +238,17 @@ void Pool::clear_graphs() {
   graphs.clear();
 }
 
+void Pool::ensure_diag(int n) {
+  if (n <= diag_cap) return;
+  if (d_diag) hcheck(hipFree(d_diag), "hipFree diag");
+  d_diag = nullptr;
+  diag_cap = 0;
+  const int cap = std::max(n, 1024);
+  hcheck(hipMalloc((void**)&d_diag, ((size_t)cap + (cap + 1) / 2 + 4) * sizeof(double)),
+         "hipMalloc diag");
+  diag_cap = cap;
+}
+
 void Pool::release() {
   clear_graphs();
   for (auto& r : prof) {
@@ -255,9 +266,10 @@ void Pool::release() {
   f(cc32);
   cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
-  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs);
+  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag);
   d_flux_pairs = nullptr;
-  if (h_red) (void)hipHostFree(h_red);
+  d_diag = nullptr;
+  diag_cap = 0;
   if (stream) (void)hipStreamDestroy(stream);
   cc = fc = nullptr;
   stream = nullptr;
@@ -941,6 +953,52 @@ int a5g_box_summaries(a5g_tree* t, double* out) {
   NEED(t);
   need_gpu(t->P);
   a5g::box_summaries(t->P, t->P.sv, out);
+  API_END
+}
+
+int a5g_get_loc(a5g_tree* t, const double* r, int max_lvl, int* id, int* ix) {
+  API_BEGIN
+  NEED(t);
+  if (!r || !id || !ix) throw std::invalid_argument("get_loc: null argument");
+  if (!t->P.tree.ready) throw std::runtime_error("get_loc: the base is not set");
+  *id = t->P.tree.get_loc(r, max_lvl, ix);
+  API_END
+}
+
+int a5g_reduction_loc(a5g_tree* t, int kind, int var, int d, double thr, double init_val,
+                      double* val, int* id, int* ix) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  const auto& T = t->P.tree;
+  if (!val || !id || !ix) throw std::invalid_argument("reduction_loc: null argument");
+  if (d < 1 || d > T.dim) throw std::out_of_range("reduction_loc: dimension out of range");
+  if (kind == 1) {
+    if (var < 1 || var > T.n_fc)
+      throw std::out_of_range("face variable " + std::to_string(var) + " out of range");
+  } else if (kind == 2) {
+    check_iv(T, var);
+  } else {
+    throw std::invalid_argument("reduction_loc: kind must be 1 or 2");
+  }
+  a5g::reduction_loc(t->P, kind, var, d - 1, thr, init_val, val, id, ix);
+  API_END
+}
+
+int a5g_cc_axis(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc, int cap,
+                double* out, int* n_points) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  if (!n_points) throw std::invalid_argument("cc_axis: null n_points");
+  if (n_cc < 0 || n_fc < 0 || (n_cc && !ixs_cc) || (n_fc && !ixs_fc))
+    throw std::invalid_argument("cc_axis: bad variable lists");
+  for (int v = 0; v < n_cc; v++) check_iv(T, ixs_cc[v]);
+  for (int v = 0; v < n_fc; v++)
+    if (ixs_fc[v] < 1 || ixs_fc[v] > T.n_fc)
+      throw std::out_of_range("face variable " + std::to_string(ixs_fc[v]) + " out of range");
+  if (out) need_gpu(t->P);
+  *n_points = a5g::cc_axis(t->P, n_cc, ixs_cc, n_fc, ixs_fc, cap, out);
   API_END
 }
 

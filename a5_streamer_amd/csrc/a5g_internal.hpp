@@ -37,5 +37,9 @@ void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const
 void prolong1_boxes(Pool& P, int n_ids, const int* ids, int iv);
 void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value);
 void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc);
+void reduction_loc(Pool& P, int kind, int var, int d, double thr, double init_val, double* val,
+                   int* id, int* ix);
+int cc_axis(Pool& P, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc, int cap,
+            double* out);
 
 }  // namespace a5g

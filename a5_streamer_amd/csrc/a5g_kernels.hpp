@@ -39,6 +39,13 @@ __device__ __forceinline__ double* facep(const Dev& a, int ifc, int id, int d, i
   return a.fc + (size_t)(ifc - 1) * a.fvs + ((size_t)id * dim + d) * a.fs;
 }
 
+// index into a face array fx(nc+1,nc[,nc]) / fy / fz of a box (1-based)
+template <int D>
+__device__ __forceinline__ int fidx(int nc, int d, int i, int j, int k) {
+  const int n0 = nc + (d == 0), n1 = nc + (d == 1);
+  return D == 2 ? (i - 1) + n0 * (j - 1) : (i - 1) + n0 * ((j - 1) + n1 * (k - 1));
+}
+
 // a$D_get_child_offset, m_aX_utils.f90:227-239
 template <int D>
 __device__ __forceinline__ void child_off(const Dev& a, int id, int nc, int nb, int off[3]) {

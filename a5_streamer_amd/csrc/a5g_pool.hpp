@@ -106,7 +106,11 @@ struct Pool {
   // reduction scratch: kRedSlots pairs of ordered-integer keys, one pair per
   // group of workgroups, so that atomics do not all meet on one address
   double* d_red = nullptr;
-  double* h_red = nullptr;  // pinned
+  // scratch of the diagnostics (a5g_diag.hip): diag_cap doubles, then as
+  // many ints, then 4 doubles
+  double* d_diag = nullptr;
+  int diag_cap = 0;
+  void ensure_diag(int n);
   MgOpts mg;
   StVars sv;
   // transport lookup table on the device: rows_cols(n_rows, n_cols)

@@ -39,13 +39,6 @@ static G2 g2(int n, int work) {
   return {tpb, (n + 256 / tpb - 1) / (256 / tpb)};
 }
 
-// index into a face array fx(nc+1,nc[,nc]) / fy / fz of a box (1-based)
-template <int D>
-__device__ __forceinline__ int fidx(int nc, int d, int i, int j, int k) {
-  const int n0 = nc + (d == 0), n1 = nc + (d == 1);
-  return D == 2 ? (i - 1) + n0 * (j - 1) : (i - 1) + n0 * ((j - 1) + n1 * (k - 1));
-}
-
 // ---------------------------------------------------------------------------
 // lookup table (fosito/m_lookup_table.f90:100-161): the location is
 // low_ix = ceiling((x - x_min) inv_dx), low_frac = low_ix - frac, clamped to

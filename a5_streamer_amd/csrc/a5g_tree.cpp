@@ -372,4 +372,38 @@ bool Tree::tidy_up(double max_frac, double goal_frac, int n_clean_min, bool reor
   return true;
 }
 
+// a$D_get_loc (afivo/src/m_aX_utils.f90:244-281): the first level-1 box in
+// list order holding r (a$D_r_inside, :203-216, faces included), then down
+// through child_that_contains (:284-298) while the box has children and its
+// level is below max_lvl; the cell is a$D_cc_ix (:301-306).
+int Tree::get_loc(const double* r, int max_lvl, int* ix) const {
+  const int lmax = max_lvl < 1 ? lvls_max : max_lvl;
+  int id = -1;
+  for (int c : lv_ids(1)) {
+    bool in = true;
+    for (int d = 0; d < dim; d++) {
+      const double lo = rmin[c * 3 + d], hi = lo + dr[c] * ncell[c];
+      if (!(r[d] >= lo && r[d] <= hi)) in = false;
+    }
+    if (in) {
+      id = c;
+      break;
+    }
+  }
+  if (id < 0) {
+    for (int d = 0; d < dim; d++) ix[d] = -1;
+    return -1;
+  }
+  while (lvl[id] < lmax && has_children(id)) {
+    int i_ch = 1;
+    for (int d = 0; d < dim; d++) {
+      const double cntr = rmin[id * 3 + d] + dr[id] * (ncell[id] >> 1);
+      if (r[d] > cntr) i_ch += 1 << d;
+    }
+    id = children[id * 8 + i_ch - 1];
+  }
+  for (int d = 0; d < dim; d++) ix[d] = (int)std::ceil((r[d] - rmin[id * 3 + d]) / dr[id]);
+  return id;
+}
+
 }  // namespace a5g

@@ -63,6 +63,9 @@ struct Tree {
   const std::vector<int>& lv_leaves(int l) const { return leaves[l - min_lvl]; }
   const std::vector<int>& lv_parents(int l) const { return parents[l - min_lvl]; }
   bool has_children(int id) const { return children[id * 8] != kNoBox; }
+  // leaf box (levels up to max_lvl; max_lvl < 1: lvls_max) and cell ix
+  // holding point r; -1 (and ix = -1) outside the level-1 boxes
+  int get_loc(const double* r, int max_lvl, int* ix) const;
   int box_len(int id) const;
 
  private:

@@ -70,6 +70,18 @@ module m_a5g
      type(a5g_lvl_info_t), allocatable :: lvls(:)
   end type a5g_ref_info_t
 
+  !> a$D_loc_t (afivo/src/m_aX_t.f90): a leaf box and a cell of it; ix(1:dim)
+  !> is used
+  type, public :: a5g_loc_t
+     integer :: id    = -1
+     integer :: ix(3) = -1
+  end type a5g_loc_t
+
+  ! per-box rules of st_reduction_loc (get_streamer_properties,
+  ! src/streamer_cyl.f90:831-903)
+  integer, parameter, public :: a5_rl_face_mean = 1 ! box_maxfld_r / box_maxfld_z
+  integer, parameter, public :: a5_rl_coord_max = 2 ! box_fld_maxr / box_fld_maxz
+
   ! ------------------------------------------------------------------
   ! raw interfaces, one per entry point of include/a5g.h
   ! ------------------------------------------------------------------
@@ -453,6 +465,37 @@ module m_a5g
        integer(c_int)        :: s
      end function c_set_gaussians
 
+     function c_get_loc(t, r, max_lvl, id, ix) bind(C, name="a5g_get_loc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double)        :: r(*)
+       integer(c_int), value :: max_lvl
+       integer(c_int)        :: id, ix(*)
+       integer(c_int)        :: s
+     end function c_get_loc
+
+     function c_reduction_loc(t, kind, var, d, thr, init_val, val, id, ix) &
+          bind(C, name="a5g_reduction_loc") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: kind, var, d
+       real(c_double), value :: thr, init_val
+       real(c_double)        :: val
+       integer(c_int)        :: id, ix(*)
+       integer(c_int)        :: s
+     end function c_reduction_loc
+
+     function c_cc_axis(t, n_cc, ixs_cc, n_fc, ixs_fc, cap, out, n_points) &
+          bind(C, name="a5g_cc_axis") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n_cc, n_fc, cap
+       integer(c_int)        :: ixs_cc(*), ixs_fc(*)
+       real(c_double)        :: out(*)
+       integer(c_int)        :: n_points
+       integer(c_int)        :: s
+     end function c_cc_axis
+
      function c_prof_enable(t, on) bind(C, name="a5g_prof_enable") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -502,6 +545,8 @@ module m_a5g
   interface mg3_fas_fmg;          module procedure t_fas_fmg;           end interface
   interface mg2_fas_vcycle;       module procedure t_fas_vcycle;        end interface
   interface mg3_fas_vcycle;       module procedure t_fas_vcycle;        end interface
+  interface a2_get_loc;           module procedure t_get_loc;           end interface
+  interface a3_get_loc;           module procedure t_get_loc;           end interface
 
   public :: a2_init, a3_init
   public :: a2_destroy, a3_destroy, a2_set_base, a3_set_base
@@ -518,6 +563,7 @@ module m_a5g
   public :: st_setup, st_table_create, st_table_set_col, st_table_get
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
+  public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
 
 contains
 
@@ -967,5 +1013,56 @@ contains
     allocate(summ(3, a5g_max_id(tree)))
     call check(c_box_summaries(tree%h, summ), "box_summaries")
   end subroutine st_box_summaries
+
+  ! ------------------------------------------------------------------
+  ! streamer diagnostics (src/streamer_cyl.f90:682-956)
+  ! ------------------------------------------------------------------
+
+  !> a$D_get_loc (afivo/src/m_aX_utils.f90:244-281)
+  function t_get_loc(tree, rr, max_lvl) result(loc)
+    type(a5g_t), intent(in)       :: tree
+    real(dp), intent(in)          :: rr(:)
+    integer, intent(in), optional :: max_lvl
+    type(a5g_loc_t)               :: loc
+    integer(c_int) :: m
+    m = 0
+    if (present(max_lvl)) m = max(max_lvl, 1)
+    call check(c_get_loc(tree%h, rr, m, loc%id, loc%ix), "a$D_get_loc")
+  end function t_get_loc
+
+  !> a$D_reduction_loc (afivo/src/m_aX_utils.f90:499-555) with reduce_max and
+  !> the per-box rule kind (a5_rl_face_mean on face variable var,
+  !> a5_rl_coord_max on cells whose cc(var) > thr) along dimension d.
+  !> out_loc keeps its value when no box exceeds init_val, as an intent(out)
+  !> location without default initialisation does in the reference.
+  subroutine st_reduction_loc(tree, kind, var, d, thr, init_val, out_val, out_loc)
+    type(a5g_t), intent(inout)     :: tree
+    integer, intent(in)            :: kind, var, d
+    real(dp), intent(in)           :: thr, init_val
+    real(dp), intent(out)          :: out_val
+    type(a5g_loc_t), intent(inout) :: out_loc
+    integer(c_int) :: id, ix(3)
+    call check(c_reduction_loc(tree%h, kind, var, d, thr, init_val, out_val, id, ix), &
+         "a$D_reduction_loc")
+    if (id /= -1) then
+       out_loc%id = id
+       out_loc%ix = ix
+    end if
+  end subroutine st_reduction_loc
+
+  !> get_cc_axis (src/streamer_cyl.f90:905-956)
+  subroutine st_get_cc_axis(tree, ixs_cc, ixs_fc, axis_data)
+    type(a5g_t), intent(inout)           :: tree
+    integer, intent(in)                  :: ixs_cc(:), ixs_fc(:)
+    real(dp), allocatable, intent(inout) :: axis_data(:, :)
+    integer(c_int) :: n
+    real(dp)       :: none(1)
+    call check(c_cc_axis(tree%h, size(ixs_cc), ixs_cc, size(ixs_fc), ixs_fc, 0, none, n), &
+         "get_cc_axis")
+    if (allocated(axis_data)) deallocate(axis_data)
+    allocate(axis_data(size(ixs_cc) + size(ixs_fc) + 1, n))
+    if (n > 0) call check(c_cc_axis(tree%h, size(ixs_cc), ixs_cc, size(ixs_fc), ixs_fc, n, &
+         axis_data, n), "get_cc_axis")
+  end subroutine st_get_cc_axis
 
 end module m_a5g

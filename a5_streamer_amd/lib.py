@@ -339,3 +339,38 @@ class Tree:
         lo, hi = C.c_double(), C.c_double()
         _call("a5g_tree_minmax_cc", self.h, C.c_int(iv), C.byref(lo), C.byref(hi))
         return lo.value, hi.value
+
+    # --- streamer diagnostics (src/streamer_cyl.f90:682-956) ---------------
+    def get_loc(self, r, max_lvl=0):
+        """(id, ix) of the leaf cell holding point r (a$D_get_loc); (-1,
+        [-1, ...]) outside the base boxes. max_lvl < 1: no level limit."""
+        ix = np.zeros(self.dim, np.int32)
+        idv = C.c_int()
+        _call("a5g_get_loc", self.h, _d(np.asarray(r, float))[1], C.c_int(max_lvl),
+              C.byref(idv), _i(ix)[1])
+        return idv.value, ix
+
+    def reduction_loc(self, kind, var, d, thr=0.0, init_val=-1e99):
+        """(value, id, ix) of the located maximum over the leaves
+        (a$D_reduction_loc with the box procedures of get_streamer_properties):
+        kind 1 = 0.5 |f(ix) + f(ix + e_d)| of face variable var, kind 2 = the
+        cell-centre coordinate d where cc(var) > thr; d is 1-based."""
+        ix = np.zeros(self.dim, np.int32)
+        val, idv = C.c_double(), C.c_int()
+        _call("a5g_reduction_loc", self.h, C.c_int(kind), C.c_int(var), C.c_int(d),
+              C.c_double(thr), C.c_double(init_val), C.byref(val), C.byref(idv), _i(ix)[1])
+        return val.value, idv.value, ix
+
+    def cc_axis(self, ixs_cc, ixs_fc):
+        """get_cc_axis (2D): array (1 + len(ixs_cc) + len(ixs_fc), n) of the
+        cell-centre y, the cell values and the face averages along x = 0."""
+        cc, pcc = _i(np.atleast_1d(ixs_cc))
+        fc, pfc = _i(np.atleast_1d(ixs_fc))
+        n = C.c_int()
+        _call("a5g_cc_axis", self.h, C.c_int(len(cc)), pcc, C.c_int(len(fc)), pfc,
+              C.c_int(0), None, C.byref(n))
+        out = np.zeros((n.value, len(cc) + len(fc) + 1))
+        if n.value:
+            _call("a5g_cc_axis", self.h, C.c_int(len(cc)), pcc, C.c_int(len(fc)), pfc,
+                  C.c_int(n.value), _d(out)[1], C.byref(n))
+        return out.T

@@ -10,8 +10,9 @@ The run is written against the Tree interface of a5_streamer_amd.lib, so any
 object with the same methods can drive it; the tests use that to step the
 same run through the CPU restatement and compare every state bit for bit.
 
-Output (Silo files, streamer properties) and Monte-Carlo photoionization are
-not part of this path. Photoionization is provided by the three-term
+The streamer diagnostics of the cylindrical driver (streamer_properties,
+cc_axis) use the GPU's located reductions. Silo output and Monte-Carlo
+photoionization are not part of this path. Photoionization is provided by the three-term
 Helmholtz model instead (Config.photoi), at the place where the drivers call
 set_photoionization.
 """
@@ -30,6 +31,7 @@ N_VAR_CELL, N_VAR_FACE = 9, 2
 # reference's nine
 I_PH_PHI, I_PH_RHS, I_PH_TMP = 10, 11, 12
 UC_TORR_TO_BAR = 133.322368 * 1.0e-5  # fosito/m_units_constants.f90:44
+UC_PI = math.pi                        # acos(-1), fosito/m_units_constants.f90:24
 # Three-term fit of the photoionization absorption function of air (Bourdon
 # et al. 2007, Plasma Sources Sci. Technol. 16, 656, table 1):
 # A_j in cm^-2 Torr^-2 and lambda_j in cm^-1 Torr^-1
@@ -158,6 +160,12 @@ class Config:
         #                                domain_len spans the first direction
         self.lvls_max = 9
         self.applied_fld = 2.5e6
+        # time dependence of the applied field (ST_get_fld,
+        # src/m_streamer.f90:419-432; defaults :167-174)
+        self.fld_mod_t0 = 1.0e99
+        self.fld_sin_amplitude = 0.0
+        self.fld_sin_freq = 0.2e9
+        self.fld_lin_deriv = 0.0
         self.bg_dens = 1e15
         self.seed_dens = [1e20]
         self.seed_r0 = [[16e-3, 28e-3]]
@@ -180,6 +188,15 @@ class Config:
             if not hasattr(self, k):
                 raise KeyError("unknown config key %s" % k)
             setattr(self, k, v)
+
+    def get_fld(self, time):
+        """ST_get_fld (src/m_streamer.f90:419-432): the applied field at a
+        time, with its linear and sinusoidal modulation after fld_mod_t0."""
+        if time > self.fld_mod_t0:
+            return (self.applied_fld + (time - self.fld_mod_t0) * self.fld_lin_deriv
+                    + self.fld_sin_amplitude * math.sin(
+                        (time - self.fld_mod_t0) * 2 * UC_PI * self.fld_sin_freq))
+        return self.applied_fld
 
     def photoi_model(self):
         """(coeff, amp, kappa) of the Helmholtz photoionization: coeff = eta
@@ -365,6 +382,11 @@ class StreamerRun:
     def compute_fld(self, no_guess):
         T = self.T
         T.space_charge(ELEC / EPS0)
+        # ST_applied_voltage = -ST_domain_len * ST_get_fld(ST_time)
+        voltage = -self.c.domain_len * self.c.get_fld(self.time)
+        if voltage != self.voltage:
+            self.voltage = voltage
+            T.mg_set_bc_value(2 * self.dim, voltage)
         for i in range(N_FMG_CYCLES):
             T.fas_fmg(False, no_guess and i == 0)
         T.fld_from_pot()
@@ -499,6 +521,105 @@ class StreamerRun:
             self.compute_fld(False)
             T.tidy_up(0.9, 0.25, 5000, False)
         return n_add, n_rm
+
+    # --- output diagnostics (src/streamer_cyl.f90:647-956) --------------------
+    def _cell(self, loc, iv):
+        """tree%boxes(loc%id)%cc(loc%ix(1), loc%ix(2), iv): index 0 is the
+        ghost layer, which a$D_get_loc returns on the low face of a box."""
+        i, ix = loc
+        nc = self.c.box_size
+        return self.T.get_cc(i, iv, nc).reshape([nc + 2] * self.dim, order="F")[tuple(ix)]
+
+    def _r_loc(self, loc, b):
+        """a$D_r_loc / a$D_r_cc (afivo/src/m_aX_utils.f90:309-323)"""
+        i, ix = loc
+        return b["r_min"][i - 1] + (np.asarray(ix) - 0.5) * b["dr"][i - 1]
+
+    def streamer_properties(self):
+        """get_streamer_properties (src/streamer_cyl.f90:682-828) of the
+        cylindrical run: 39 named values from located maxima of the field
+        (on the GPU, a5g_reduction_loc) and point lookups. Two behaviours of
+        the reference are kept: the mobility of the S values is read from
+        the alpha column (:727-728, :736-737), and a location that no box
+        improves on keeps its previous value (an intent(out) a2_loc_t
+        without default initialisation, afivo/src/m_aX_utils.f90:502-553).
+        Raises where the reference stops: the head (the location of max E_r)
+        below 0.1 of the domain, or above 0.9 of it after 1 ns."""
+        if self.c.variant != "cyl":
+            raise ValueError("streamer properties are defined for the cylindrical run")
+        T, c, tb = self.T, self.c, self.tbl
+        b = T.boxes()
+        L = c.domain_len
+        names, props = [], []
+
+        def add(n, v):
+            names.append(n)
+            props.append(float(v))
+
+        add("E_bg", c.get_fld(self.time))
+        ez_max, *loc_ez = T.reduction_loc(1, F_FLD, 2, 0.0, -1.0e99)
+        add("Ez_max", ez_max)
+        er_max, *loc_er = T.reduction_loc(1, F_FLD, 1, 0.0, -1.0e99)
+        add("Er_max", er_max)
+        rz = self._r_loc(loc_er, b)
+        add("Er_max(r)", rz[0])
+        add("Er_max(z)", rz[1])
+        z_head = rz[1]
+        er_max_norm = self._cell(loc_er, I_FLD)
+        add("Er_max(E)", er_max_norm)
+        for tag, e in (("Ez_max", ez_max), ("Er_max", er_max_norm)):
+            alpha = tb.col(T_ALPHA, e)
+            mu = tb.col(T_ALPHA, e)
+            add(tag + "(alpha)", alpha)
+            add(tag + "(S)", alpha * mu * e)
+        if (self.time > 1.0e-9 and rz[1] > 0.9 * L) or rz[1] < 0.1 * L:
+            raise RuntimeError("Simulation has reached boundary")
+        loc_dens = T.get_loc([0.0, rz[1]])
+        add("n_e(head)", self._cell(loc_dens, I_ELEC))
+        phi_head = self._cell(loc_dens, I_PHI)
+        add("dphi", phi_head - (rz[1] / L) * self.voltage)
+        rz = self._r_loc(loc_ez, b)
+        add("Ez_max(z)", rz[1])
+        loc = None
+        for dz, n in ((2.0e-3, "dphi(2mm)"), (4.0e-3, "dphi(4mm)")):
+            loc = T.get_loc([0.0, rz[1] + dz])
+            add(n, phi_head - self._cell(loc, I_PHI) if loc[0] > 0 else 0.0)
+        for i in range(4, 15, 2):
+            thr = i * 1e6
+            v, *new = T.reduction_loc(2, I_FLD, 1, thr, 0.0)
+            loc = tuple(new) if new[0] != -1 else loc
+            add("r_max(%de6)" % i, v)
+            add("dphi_r(%de6)" % i, phi_head - self._cell(loc, I_PHI) if loc[0] > 0 else 0.0)
+            v, *new = T.reduction_loc(2, I_FLD, 2, thr, 0.0)
+            loc = tuple(new) if new[0] != -1 else loc
+            add("z_max(%de6)" % i, v - z_head)
+            add("dphi_z(%de6)" % i, phi_head - self._cell(loc, I_PHI) if loc[0] > 0 else 0.0)
+        return names, np.array(props)
+
+    def cc_axis(self):
+        """get_cc_axis(tree, [i_elec, i_pion], [f_fld, f_elec])
+        (src/streamer_cyl.f90:659,905-956): rows z, n_e, n_i, E_z, flux_z."""
+        return self.T.cc_axis([I_ELEC, I_PION], [F_FLD, F_ELEC])
+
+    def write_streamer_properties(self, fname_stats, fname_axis, out_cnt):
+        """write_streamer_properties (src/streamer_cyl.f90:647-680): the
+        first call writes the header line of the statistics file, later
+        calls append (out_cnt, time, properties); the axis file is rewritten
+        with one point per line. Numbers are written with 17 significant
+        digits (the reference uses list-directed output)."""
+        names, props = self.streamer_properties()
+        axis = self.cc_axis()
+        if out_cnt == 1:
+            with open(fname_stats, "w") as f:
+                f.write(" ix_out      time     " + "".join(n.ljust(15) for n in names) + "\n")
+        else:
+            with open(fname_stats, "a") as f:
+                f.write(" %d %.17g " % (out_cnt, self.time)
+                        + " ".join("%.17g" % v for v in props) + "\n")
+        with open(fname_axis, "w") as f:
+            for p in range(axis.shape[1]):
+                f.write(" ".join("%.17g" % v for v in axis[:, p]) + "\n")
+        return names, props
 
     def leaf_cells(self):
         info = self.T.info()

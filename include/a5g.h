@@ -206,15 +206,52 @@ int a5g_tree_maxdiff_cc(a5g_tree *t, int iv_a, int iv_b, double *maxdiff,
 int a5g_set_gaussians(a5g_tree *t, int iv, int n_blob, const double *blobs,
                       double base);
 
+/* ---- streamer diagnostics -------------------------------------------- */
+
+/* a$D_get_loc (afivo/src/m_aX_utils.f90:244-281): the first level-1 box in
+ * list order that holds point r (faces included), then down the children
+ * holding it while the box's level is below max_lvl (max_lvl < 1: lvls_max,
+ * the absent argument); ix = ceiling((r - r_min) / dr) per dimension
+ * (a$D_cc_ix, :301-306). id = -1 and ix = -1 outside the level-1 boxes.
+ * Host tree only: no GPU call. */
+int a5g_get_loc(a5g_tree *t, const double *r, int max_lvl, int *id, int *ix);
+/* a$D_reduction_loc (afivo/src/m_aX_utils.f90:499-555) with reduce_max for
+ * the per-box procedures of get_streamer_properties
+ * (src/streamer_cyl.f90:831-903), over the leaves on the GPU.
+ *   kind 1 (box_maxfld_r, box_maxfld_z): per box the first cell in array
+ *     order (x fastest) of the largest |f(ix) + f(ix + e_d)| of face variable
+ *     var along dimension d (1-based); the box value is half of it.
+ *   kind 2 (box_fld_maxr, box_fld_maxz): per box the largest cell-centre
+ *     coordinate d over cells whose cc(var) exceeds thr, first such cell; 0
+ *     and no cell where no coordinate above 0 qualifies.
+ * Leaves are taken level by level in list order, and a box replaces the
+ * running value (init_val at first) only where it is larger: the outcome of
+ * the reference on one OpenMP thread. Writes the value, the box id and cell
+ * (id = -1, ix = -1 when no box exceeded init_val). */
+int a5g_reduction_loc(a5g_tree *t, int kind, int var, int d, double thr,
+                      double init_val, double *val, int *id, int *ix);
+/* get_cc_axis (src/streamer_cyl.f90:905-956), 2D: walking up the line x = 0
+ * from y = 0 through the leaf boxes a$D_get_loc finds there, each box's
+ * first column of cells gives nc points: y of the cell centre, cc(1, j) of
+ * ixs_cc, 0.5 (fy(1, j) + fy(1, j+1)) of ixs_fc. Sets *n_points. With out
+ * NULL or cap = 0 nothing else happens; otherwise writes
+ * out(n_cc + n_fc + 1, *n_points) column-major, failing when *n_points
+ * exceeds cap. */
+int a5g_cc_axis(a5g_tree *t, int n_cc, const int *ixs_cc, int n_fc,
+                const int *ixs_fc, int cap, double *out, int *n_points);
+
 /* ---- measurement ----------------------------------------------------- */
 
 /* Start (on = 1) or stop timing every kernel launch with HIP events on the
  * handle's stream; enabling clears earlier records. */
 int a5g_prof_enable(a5g_tree *t, int on);
 /* Launch count, summed duration (ms) and algorithmic HBM bytes of the
- * recorded launches of one kernel class (0 Gauss-Seidel half-sweep, 1 ghost
- * cells, 2 residual+restriction, 3 coarse rhs, 4 correction, 5 copy) on
- * level lvl (-999: all levels). */
+ * recorded launches of one kernel class on level lvl (-999: all levels).
+ * Classes: 0 Gauss-Seidel half-sweep, 1 ghost cells, 2 residual +
+ * restriction, 3 coarse rhs, 4 correction, 5 copy, 6 other multigrid
+ * kernels, 7 space charge, 8 field from potential, 9 Koren fluxes,
+ * 10 consistent fluxes, 11 density update, 12 averaging, 13 restriction of
+ * the densities, 14 photoionization rate. */
 int a5g_prof_read(a5g_tree *t, int kind, int lvl, int *n_launch, double *ms,
                   double *bytes);
 

@@ -1285,6 +1285,161 @@ double o_min_dr(const otree *t) {
 }
 
 /* ------------------------------------------------------------------ */
+/* locations and located reductions                                    */
+/* ------------------------------------------------------------------ */
+
+/* a$D_r_inside without the optional margin (m_aX_utils.f90:203-216) */
+static int r_inside(const otree *t, int id, const double *r) {
+  for (int d = 0; d < t->dim; d++) {
+    double lo = RMP(t, id)[d], hi = lo + t->dr[id] * t->ncell[id];
+    if (!(r[d] >= lo && r[d] <= hi)) return 0;
+  }
+  return 1;
+}
+
+void o_get_loc(const otree *t, const double *r, int max_lvl, int *id_out, int *ix) {
+  /* a$D_get_loc (m_aX_utils.f90:244-281) with child_that_contains
+     (:284-298) and a$D_cc_ix (:301-306); max_lvl < 1 stands for the absent
+     argument (tree%lvls_max) */
+  int lvl_max = max_lvl < 1 ? t->lvls_max : max_lvl;
+  const ivec *l1 = &t->ids[LV(t, 1)];
+  int id = -1;
+  for (int i = 0; i < l1->n; i++)
+    if (r_inside(t, l1->v[i], r)) {
+      id = l1->v[i];
+      break;
+    }
+  if (id < 0) {
+    *id_out = -1;
+    for (int d = 0; d < t->dim; d++) ix[d] = -1;
+    return;
+  }
+  while (t->lvl[id] < lvl_max && CHP(t, id)[0] != 0) {
+    int i_ch = 1;
+    for (int d = 0; d < t->dim; d++) {
+      double cntr = RMP(t, id)[d] + t->dr[id] * (t->ncell[id] >> 1);
+      if (r[d] > cntr) i_ch += 1 << d;
+    }
+    id = CHP(t, id)[i_ch - 1];
+  }
+  *id_out = id;
+  for (int d = 0; d < t->dim; d++) ix[d] = (int)ceil((r[d] - RMP(t, id)[d]) / t->dr[id]);
+}
+
+/* Per-box procedures of get_streamer_properties (src/streamer_cyl.f90):
+   kind 1 is box_maxfld_r / box_maxfld_z (:831-842, :892-903) along
+   dimension d, kind 2 is box_fld_maxr / box_fld_maxz (:844-890). */
+static void box_rloc(const otree *t, int id, int kind, int var, int d, double thr,
+                     double *val, int *ix) {
+  int nc = t->ncell[id], kmax = t->dim == 3 ? nc : 1;
+  for (int e = 0; e < t->dim; e++) ix[e] = -1;
+  if (kind == 1) {
+    /* maxloc(abs(f(1:nc along d) + f(2:nc+1 along d))): the first maximum in
+       array order, value 0.5 * abs(...) there */
+    const double *f = FC(t, id, var, d);
+    double best = 0;
+    int first = 1;
+    for (int k = 1; k <= kmax; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++) {
+          int q[3] = {i, j, k};
+          q[d]++;
+          double a = fabs(f[FI(nc, t->dim, d, i, j, k)] + f[FI(nc, t->dim, d, q[0], q[1], q[2])]);
+          if (first || a > best) {
+            best = a;
+            ix[0] = i;
+            ix[1] = j;
+            if (t->dim == 3) ix[2] = k;
+            first = 0;
+          }
+        }
+    int q[3] = {ix[0], ix[1], t->dim == 3 ? ix[2] : 1};
+    int p[3] = {q[0], q[1], q[2]};
+    p[d]++;
+    *val = 0.5 * fabs(f[FI(nc, t->dim, d, q[0], q[1], q[2])] + f[FI(nc, t->dim, d, p[0], p[1], p[2])]);
+  } else {
+    /* the largest cell-centre coordinate d (a$D_r_cc, m_aX_utils.f90:309-314)
+       over cells with cc(var) > thr; 0 and no cell where none is */
+    const double *c = CC(t, id, var);
+    double v = 0;
+    for (int k = 1; k <= kmax; k++)
+      for (int j = 1; j <= nc; j++)
+        for (int i = 1; i <= nc; i++) {
+          size_t q = t->dim == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+          if (c[q] > thr) {
+            int w[3] = {i, j, k};
+            double rc = RMP(t, id)[d] + (w[d] - 0.5) * t->dr[id];
+            if (rc > v) {
+              v = rc;
+              ix[0] = i;
+              ix[1] = j;
+              if (t->dim == 3) ix[2] = k;
+            }
+          }
+        }
+    *val = v;
+  }
+}
+
+void o_reduction_loc(const otree *t, int kind, int var, int d, double thr, double init_val,
+                     double *val, int *id_out, int *ix) {
+  /* a$D_reduction_loc (m_aX_utils.f90:499-555) with reduction = max
+     (reduce_max, src/streamer_cyl.f90:826-829), in the order of one thread:
+     levels upwards, leaves in list order; a box takes over where
+     max(box value, running value) differs from the running value */
+  double my = init_val;
+  int my_id = -1, my_ix[3] = {-1, -1, -1};
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    const ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int i = 0; i < lv->n; i++) {
+      double v;
+      int bix[3];
+      box_rloc(t, lv->v[i], kind, var, d, thr, &v, bix);
+      double nv = v > my ? v : my;
+      if (fabs(nv - my) > 0) {
+        my_id = lv->v[i];
+        for (int e = 0; e < 3; e++) my_ix[e] = bix[e];
+        my = v;
+      }
+    }
+  }
+  *val = my;
+  *id_out = my_id;
+  for (int e = 0; e < t->dim; e++) ix[e] = my_ix[e];
+}
+
+int o_cc_axis(const otree *t, int n_cc, const int *ixs_cc, int n_fc, const int *ixs_fc,
+              double *out, int cap) {
+  /* get_cc_axis (src/streamer_cyl.f90:905-956), 2D: the leaf boxes met
+     walking up the line x = 0 from y = 0, each giving its first column of
+     cells: y of the centre, cc(1, j, ixs_cc), 0.5 (fy(1, j) + fy(1, j+1))
+     of ixs_fc. out(n_cc + n_fc + 1, n) column-major; returns n, writing
+     only when out is given and n <= cap. */
+  int nc = t->n_cell, rows = n_cc + n_fc + 1, cnt = 0;
+  double r[2] = {0.0, 0.0};
+  for (;;) {
+    int id, ix[2];
+    o_get_loc(t, r, 0, &id, ix);
+    if (id == -1) break;
+    double box_z = RMP(t, id)[1], box_dz = t->dr[id];
+    if (out && cnt + nc <= cap) {
+      for (int j = 1; j <= nc; j++) {
+        double *o = out + (size_t)(cnt + j - 1) * rows;
+        o[0] = box_z + (j - 0.5) * box_dz;
+        for (int v = 0; v < n_cc; v++) o[1 + v] = CC(t, id, ixs_cc[v])[I2(nc, 1, j)];
+        for (int v = 0; v < n_fc; v++) {
+          const double *f = FC(t, id, ixs_fc[v], 1);
+          o[1 + n_cc + v] = 0.5 * (f[FI(nc, 2, 1, 1, j, 0)] + f[FI(nc, 2, 1, 1, j + 1, 0)]);
+        }
+      }
+    }
+    cnt += nc;
+    r[1] = box_z + (nc + 1) * box_dz;
+  }
+  return cnt;
+}
+
+/* ------------------------------------------------------------------ */
 /* multigrid (m_aX_mg.f90)                                              */
 /* ------------------------------------------------------------------ */
 
@@ -1306,6 +1461,12 @@ void o_mg_setup(otree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
     t->bc_val[nb] = bc_val ? bc_val[nb] : 0;
   }
   t->mg_ok = 1;
+}
+
+void o_mg_set_bc_value(otree *t, int nb, double value) {
+  /* the value V of the rule 2V - c on face nb (the drivers' sides_bc_pot
+     reads ST_applied_voltage at each call, src/streamer_2d.f90:620-638) */
+  t->bc_val[nb - 1] = value;
 }
 
 void o_mg_set_helmholtz(otree *t, double kappa2) {

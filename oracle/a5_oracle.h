@@ -49,6 +49,7 @@ void o_mg_setup(otree *t, int i_phi, int i_tmp, int i_rhs, int i_eps,
                 const double *bc_val, int auto_ops);
 void o_fas_fmg(otree *t, int set_residual, int first_call);
 /* Helmholtz operator lpl(phi) - kappa2 phi (0: the Laplacian) */
+void o_mg_set_bc_value(otree *t, int nb, double value);
 void o_mg_set_helmholtz(otree *t, double kappa2);
 void o_fas_vcycle(otree *t, int max_lvl, int set_residual);
 void o_box_gsrb(otree *t, int id, int cntr);
@@ -67,6 +68,16 @@ void o_prolong0_to(otree *t, int id, int iv);
 void o_tree_copy_cc(otree *t, int iv_from, int iv_to);
 void o_tree_minmax_cc(const otree *t, int iv, double *vmin, double *vmax);
 double o_min_dr(const otree *t);
+/* a$D_get_loc: leaf box and cell holding point r (max_lvl < 1: no limit) */
+void o_get_loc(const otree *t, const double *r, int max_lvl, int *id, int *ix);
+/* a$D_reduction_loc with max for the box procedures of
+   get_streamer_properties: kind 1 = 0.5|f(ix) + f(ix+e_d)| of face
+   variable var, kind 2 = cell-centre coordinate d where cc(var) > thr */
+void o_reduction_loc(const otree *t, int kind, int var, int d, double thr, double init_val,
+                     double *val, int *id, int *ix);
+/* get_cc_axis (2D): returns the number of points */
+int o_cc_axis(const otree *t, int n_cc, const int *ixs_cc, int n_fc, const int *ixs_fc,
+              double *out, int cap);
 /* max|rhs - A phi| over leaves (uses i_tmp as scratch, like residual_box) */
 double o_max_residual(otree *t);
 

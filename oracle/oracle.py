@@ -49,6 +49,9 @@ def lib():
         L.o_lt_data.argtypes = [C.c_void_p, dp, dp, dp]
         L.o_norm2.restype = C.c_double
         L.o_norm2.argtypes = [C.c_int, dp]
+        L.o_reduction_loc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, dp, ip, ip]
+        L.o_cc_axis.restype = C.c_int
         for name in ("o_space_charge", "o_fld_from_pot", "o_average_dens"):
             getattr(L, name).argtypes = None
         _lib = L
@@ -182,6 +185,9 @@ class OTree:
                           C.c_int(n_up), C.c_int(n_base), pk, pv,
                           C.c_int(int(auto_ops)))
 
+    def mg_set_bc_value(self, nb, value):
+        self.L.o_mg_set_bc_value(self.h, C.c_int(nb), C.c_double(value))
+
     def mg_set_helmholtz(self, kappa2):
         self.L.o_mg_set_helmholtz(self.h, C.c_double(kappa2))
 
@@ -243,6 +249,33 @@ class OTree:
 
     def max_residual(self):
         return self.L.o_max_residual(self.h)
+
+    def get_loc(self, r, max_lvl=0):
+        """(id, ix) of the leaf cell holding point r (a$D_get_loc);
+        (-1, [-1, ...]) outside the base boxes."""
+        ix, pix = _i(np.zeros(self.dim))
+        idv = C.c_int()
+        self.L.o_get_loc(self.h, _d(np.asarray(r, float))[1], C.c_int(max_lvl),
+                         C.byref(idv), pix)
+        return idv.value, ix
+
+    def reduction_loc(self, kind, var, d, thr=0.0, init_val=-1e99):
+        """(value, id, ix) of the located maximum (a$D_reduction_loc)."""
+        ix, pix = _i(np.zeros(self.dim))
+        val, idv = C.c_double(), C.c_int()
+        self.L.o_reduction_loc(self.h, C.c_int(kind), C.c_int(var), C.c_int(d - 1),
+                               C.c_double(thr), C.c_double(init_val), C.byref(val),
+                               C.byref(idv), pix)
+        return val.value, idv.value, ix
+
+    def cc_axis(self, ixs_cc, ixs_fc):
+        """get_cc_axis: array (len(ixs_cc) + len(ixs_fc) + 1, n)."""
+        cc, pcc = _i(ixs_cc)
+        fc, pfc = _i(ixs_fc)
+        n = self.L.o_cc_axis(self.h, len(cc), pcc, len(fc), pfc, None, 0)
+        out = np.zeros((n, len(cc) + len(fc) + 1))
+        self.L.o_cc_axis(self.h, len(cc), pcc, len(fc), pfc, _d(out)[1] if n else None, n)
+        return out.T
 
 
 def koren_mlim(a, b):

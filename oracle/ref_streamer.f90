@@ -9,6 +9,7 @@ module ref_streamer
   use m_lookup_table
   use m_morton
   use m_streamer, only: koren_mlim
+  use omp_lib, only: omp_set_num_threads
   implicit none
   private
 
@@ -115,5 +116,11 @@ contains
     call morton_rank(m, r)
     ranks = r
   end subroutine h_morton_rank
+
+  ! OpenMP threads of the reference routines called after this
+  subroutine h_set_threads(n) bind(C, name="ref_set_threads")
+    integer(c_int), value :: n
+    call omp_set_num_threads(n)
+  end subroutine h_set_threads
 
 end module ref_streamer

@@ -214,8 +214,27 @@ class RefTree:
         self.f("tree_minmax_cc")(C.c_int(iv), C.byref(lo), C.byref(hi))
         return lo.value, hi.value
 
+    def get_loc(self, r, max_lvl=0):
+        ix = np.zeros(self.dim, np.int32)
+        idv = C.c_int()
+        self.f("get_loc")(_d(np.asarray(r, float))[1], C.c_int(max_lvl), C.byref(idv),
+                          _i(ix)[1])
+        return idv.value, ix
+
+    def reduction_loc(self, kind, var, d, thr=0.0, init_val=-1e99):
+        ix = np.zeros(self.dim, np.int32)
+        val, idv = C.c_double(), C.c_int()
+        self.f("reduction_loc")(C.c_int(kind), C.c_int(var), C.c_int(d), C.c_double(thr),
+                                C.c_double(init_val), C.byref(val), C.byref(idv), _i(ix)[1])
+        return val.value, idv.value, ix
+
     def min_dr(self):
         return self.f("min_dr")()
+
+
+def set_threads(n):
+    """OpenMP threads of the reference routines called afterwards."""
+    lib().ref_set_threads(C.c_int(n))
 
 
 def koren_mlim(a, b):

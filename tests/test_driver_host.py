@@ -200,3 +200,73 @@ def test_far_field_initial_densities_are_exact():
             for p in run._cell_coords(8, dr, rmin)])
         np.testing.assert_array_equal(run._box_fields(b, 0, True)[0], slow)
     assert 0 < n_fast < 60
+
+
+def test_applied_field_modulation():
+    """ST_get_fld (src/m_streamer.f90:419-432): constant up to fld_mod_t0,
+    then linear plus sinusoidal."""
+    c = S.Config(fld_mod_t0=1e-9, fld_lin_deriv=1e15, fld_sin_amplitude=1e5,
+                 fld_sin_freq=0.25e9)
+    assert c.get_fld(0.5e-9) == c.applied_fld == c.get_fld(1e-9)
+    t = 2e-9  # a quarter period after t0: the sine is 1
+    assert c.get_fld(t) == pytest.approx(2.5e6 + 1e-9 * 1e15 + 1e5, rel=1e-14)
+    assert S.Config().get_fld(1e-6) == 2.5e6
+
+
+def _cyl_run(lvls_max=6):
+    # a square domain: the reference's boundary test compares the head
+    # height with the domain length
+    cfg = S.Config(variant="cyl", domain_len=16e-3, n_base=1, lvls_max=lvls_max,
+                   seed_r0=[[0.0, 12e-3]], seed_r1=[[0.0, 12e-3]], applied_fld=2.5e6)
+    run = S.StreamerRun(O.OStreamTree, cfg)
+    run.initialise()
+    run.step_block()
+    return run
+
+
+def _leaf_arrays(T, dim, nc):
+    """Per leaf box: (id, cell centres (nc, nc, 2), |E| cells, fy of f_fld)."""
+    b = T.boxes()
+    out = []
+    for lvl in range(1, T.info()["max_lvl"] + 1):
+        for i in T.level(lvl)[1]:
+            dr, rmin = b["dr"][i - 1], b["r_min"][i - 1]
+            ax = [rmin[d] + (np.arange(1, nc + 1) - 0.5) * dr for d in range(2)]
+            cen = np.stack(np.meshgrid(*ax, indexing="ij"), axis=-1)
+            E = T.get_cc(int(i), S.I_FLD).reshape(nc + 2, nc + 2, order="F")[1:-1, 1:-1]
+            fy = T.get_fc(int(i), 2, S.F_FLD).reshape(nc, nc + 1, order="F")
+            fx = T.get_fc(int(i), 1, S.F_FLD).reshape(nc + 1, nc, order="F")
+            out.append((int(i), cen, E, fx, fy))
+    return out
+
+
+def test_streamer_properties_on_the_oracle(oracle_lib, tmp_path):
+    """get_streamer_properties of a cylindrical run on the oracle: 39 named
+    values whose located maxima equal brute force over the leaf cells, and
+    the axis profile with its five rows."""
+    run = _cyl_run()
+    names, props = run.streamer_properties()
+    assert len(names) == len(set(names)) == 39
+    p = dict(zip(names, props))
+    assert p["E_bg"] == 2.5e6
+    leaves = _leaf_arrays(run.T, 2, 8)
+    ez = max((0.5 * np.abs(fy[:, :-1] + fy[:, 1:])).max() for _, _, _, _, fy in leaves)
+    er = max((0.5 * np.abs(fx[:-1, :] + fx[1:, :])).max() for _, _, _, fx, _ in leaves)
+    assert p["Ez_max"] == ez and p["Er_max"] == er
+    for i in range(4, 15, 2):
+        r = [cen[..., 0][E > i * 1e6].max() for _, cen, E, _, _ in leaves if (E > i * 1e6).any()]
+        z = [cen[..., 1][E > i * 1e6].max() for _, cen, E, _, _ in leaves if (E > i * 1e6).any()]
+        assert p["r_max(%de6)" % i] == pytest.approx(max(r, default=0.0), abs=1e-15)
+        assert p["z_max(%de6)" % i] + p["Er_max(z)"] == pytest.approx(max(z, default=0.0),
+                                                                     abs=1e-15)
+    # the head lies inside the domain and the S values use alpha twice
+    assert 0.1 * 16e-3 <= p["Er_max(z)"] <= 0.9 * 16e-3
+    assert p["Ez_max(S)"] == p["Ez_max(alpha)"] ** 2 * p["Ez_max"]
+    ax = run.cc_axis()
+    assert ax.shape[0] == 5 and np.all(np.diff(ax[0]) > 0)
+    n1, _ = run.write_streamer_properties(tmp_path / "s.txt", tmp_path / "a.txt", 1)
+    run.write_streamer_properties(tmp_path / "s.txt", tmp_path / "a.txt", 2)
+    lines = (tmp_path / "s.txt").read_text().splitlines()
+    assert len(lines) == 2 and lines[0].split()[2:] == n1
+    assert len(lines[1].split()) == 2 + 39
+    assert len((tmp_path / "a.txt").read_text().splitlines()) == ax.shape[1]

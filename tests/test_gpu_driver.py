@@ -30,6 +30,14 @@ CASES = {
                         seed_width=[0.5e-3], seed_dens=[5e19], bg_dens=1e12),
     # Helmholtz photoionization at every step (three multigrid solves)
     "2d_photoi": dict(variant="2d", lvls_max=7, photoi=True),
+    # cylindrical on a square domain, with the streamer properties and the
+    # axis profile compared after every block
+    "cyl_square": dict(variant="cyl", domain_len=16e-3, n_base=1, lvls_max=6,
+                       seed_r0=[[0.0, 12e-3]], seed_r1=[[0.0, 12e-3]], applied_fld=2.5e6),
+    # an applied field that rises with time: the boundary value of every
+    # field solve changes
+    "2d_modulated": dict(variant="2d", lvls_max=6, fld_mod_t0=0.0, fld_lin_deriv=1e15,
+                         fld_sin_amplitude=1e5),
 }
 
 
@@ -70,5 +78,12 @@ def test_run_matches_oracle(oracle_lib, name):
         changed += res[0][0] + res[0][1]
         assert runs[0].dt == runs[1].dt and runs[0].time == runs[1].time
         _same_state(runs[0].T, runs[1].T, dim)
+        if name == "cyl_square":
+            (na, pa), (nb, pb) = [r.streamer_properties() for r in runs]
+            assert na == nb
+            np.testing.assert_array_equal(pa, pb)
+            np.testing.assert_array_equal(runs[0].cc_axis(), runs[1].cc_axis())
     if name == "2d_refining":
         assert changed > 0
+    if name == "2d_modulated":
+        assert runs[1].voltage != -runs[1].c.domain_len * runs[1].c.applied_fld
