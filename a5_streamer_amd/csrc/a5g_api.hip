@@ -1002,6 +1002,74 @@ int a5g_cc_axis(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int* i
   API_END
 }
 
+static void check_out_vars(const a5g::Tree& T, int n_cc, const int* ixs_cc, int n_fc,
+                           const int* ixs_fc) {
+  if (n_cc < 0 || n_fc < 0 || (n_cc && !ixs_cc) || (n_fc && !ixs_fc))
+    throw std::invalid_argument("bad variable lists");
+  for (int v = 0; v < n_cc; v++) check_iv(T, ixs_cc[v]);
+  for (int v = 0; v < n_fc; v++)
+    if (ixs_fc[v] < 1 || ixs_fc[v] > T.n_fc)
+      throw std::out_of_range("face variable " + std::to_string(ixs_fc[v]) + " out of range");
+}
+
+int a5g_leaf_cells(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc,
+                   long long cap, double* out, long long* n_cells) {
+  API_BEGIN
+  NEED(t);
+  const auto& T = t->P.tree;
+  if (!n_cells) throw std::invalid_argument("leaf_cells: null n_cells");
+  check_out_vars(T, n_cc, ixs_cc, n_fc, ixs_fc);
+  long long n_leaves = 0;
+  for (int l = 1; l <= T.max_lvl; l++) n_leaves += (long long)T.lv_leaves(l).size();
+  const long long cpb = T.dim == 2 ? (long long)T.n_cell * T.n_cell
+                                   : (long long)T.n_cell * T.n_cell * T.n_cell;
+  *n_cells = n_leaves * cpb;
+  if (!out || cap == 0) return 0;
+  need_gpu(t->P);
+  const long long cols = n_cc + (n_fc ? T.dim : 0);
+  if (*n_cells * cols > cap)
+    throw std::length_error("leaf_cells: " + std::to_string(*n_cells * cols) +
+                            " values, room for " + std::to_string(cap));
+  std::vector<double> h;
+  a5g::leaf_cells(t->P, n_cc, ixs_cc, n_fc, ixs_fc, h);
+  std::copy(h.begin(), h.end(), out);
+  API_END
+}
+
+int a5g_write_vtk(a5g_tree* t, const char* filename, int n_cc, const char* const* cc_names,
+                  const int* ixs_cc, int n_fc, const char* const* fc_names, const int* ixs_fc,
+                  int n_cycle, double time) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  const auto& T = t->P.tree;
+  if (!T.ready) throw std::runtime_error("Tree not ready");
+  if (!filename) throw std::invalid_argument("write_vtk: null filename");
+  std::vector<int> all;
+  if (!ixs_cc) {
+    if (n_cc != T.n_cc)
+      throw std::invalid_argument("a$D_write_vtk: size(cc_names) /= n_var_cell");
+    for (int v = 1; v <= T.n_cc; v++) all.push_back(v);
+    ixs_cc = all.data();
+  }
+  check_out_vars(T, n_cc, ixs_cc, n_fc, ixs_fc);
+  if ((n_cc && !cc_names) || (n_fc && !fc_names))
+    throw std::invalid_argument("write_vtk: null names");
+  a5g::write_vtk(t->P, filename, n_cc, cc_names, ixs_cc, n_fc, fc_names, ixs_fc, n_cycle, time);
+  API_END
+}
+
+int a5g_dump_leaves(a5g_tree* t, const char* filename, int n_var, const int* ivs, int cycle,
+                    double time) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  if (!filename) throw std::invalid_argument("dump_leaves: null filename");
+  check_out_vars(t->P.tree, n_var, ivs, 0, nullptr);
+  a5g::dump_leaves(t->P, filename, n_var, ivs, cycle, time);
+  API_END
+}
+
 int a5g_prolong1_to(a5g_tree* t, int n_ids, const int* ids, int iv) {
   API_BEGIN
   NEED(t);

@@ -1,7 +1,9 @@
-// Streamer diagnostics of the a5g library: the located maximum over the
-// leaves (a$D_reduction_loc, afivo/src/m_aX_utils.f90:499-555) with the
-// per-box procedures of get_streamer_properties (src/streamer_cyl.f90:
-// 831-903), and the axis profile of get_cc_axis (:905-956).
+// Streamer diagnostics and output of the a5g library: the located maximum
+// over the leaves (a$D_reduction_loc, afivo/src/m_aX_utils.f90:499-555) with
+// the per-box procedures of get_streamer_properties (src/streamer_cyl.f90:
+// 831-903), the axis profile of get_cc_axis (:905-956), and the leaf cells
+// of a$D_write_vtk (afivo/src/m_aX_io.f90:20-209) written as a VTK
+// unstructured grid or as a raw leaf-box dump.
 //
 // The reduction runs in two launches: one workgroup per leaf box finds the
 // box's value and cell, then one workgroup picks the box. Both keep the
@@ -12,6 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -237,6 +243,272 @@ int cc_axis(Pool& P, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc, i
     }
   }
   return n_pts;
+}
+
+}  // namespace a5g
+
+// ---------------------------------------------------------------------------
+// Output: the interior cells of the leaves in the order of a$D_write_vtk
+// (afivo/src/m_aX_io.f90:20-209): leaves level by level in list order, cells
+// x fastest. One column of n_cells values per cell variable, then, with face
+// variables, one column per dimension d holding 0.5 times the sum, in array
+// element order, of the section f_d(two faces of the cell along d, ifc_used).
+// ---------------------------------------------------------------------------
+
+namespace a5g {
+
+template <int D>
+__global__ __launch_bounds__(256) void k_leaf_cells(Dev a, const int* __restrict__ list, int n,
+                                                    int nc, long long base,
+                                                    const int* __restrict__ vars, int n_cc,
+                                                    int n_fc, long long n_cells,
+                                                    double* __restrict__ out) {
+  const int cpb = D == 2 ? nc * nc : nc * nc * nc;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * cpb) return;
+  const int li = (int)(t / cpb), q = (int)(t - (long long)li * cpb);
+  const int id = list[li];
+  const int p[3] = {q % nc + 1, (q / nc) % nc + 1, D == 3 ? q / (nc * nc) + 1 : 1};
+  const long long c = base + t;
+  for (int v = 0; v < n_cc; v++)
+    out[v * n_cells + c] = boxp(a, vars[v], id)[cidx_in<D>(nc, p[0], p[1], D == 3 ? p[2] : 0)];
+  if (!n_fc) return;
+  for (int d = 0; d < D; d++) {
+    int h[3] = {p[0], p[1], p[2]};
+    h[d]++;
+    double s = 0.0;
+    for (int v = 0; v < n_fc; v++) {
+      const double* f = facep(a, vars[n_cc + v], id, d, D);
+      s = s + f[fidx<D>(nc, d, p[0], p[1], p[2])];
+      s = s + f[fidx<D>(nc, d, h[0], h[1], h[2])];
+    }
+    out[(n_cc + d) * n_cells + c] = 0.5 * s;
+  }
+}
+
+long long leaf_cells(Pool& P, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc,
+                     std::vector<double>& host) {
+  P.sync_topology();
+  const auto& T = P.tree;
+  const int D = T.dim, nc = T.n_cell;
+  const int cpb = D == 2 ? nc * nc : nc * nc * nc;
+  long long n_leaves = 0;
+  for (int l = 1; l <= T.max_lvl; l++) n_leaves += P.n_leaves(l);
+  const long long n_cells = n_leaves * cpb;
+  const int cols = n_cc + (n_fc ? D : 0);
+  host.assign((size_t)(n_cells * cols), 0.0);
+  if (!n_cells || !cols) return n_cells;
+  const long long n_out = n_cells * cols;
+  const int nv = n_cc + n_fc;
+  if (n_out + nv > (long long)INT32_MAX) throw std::length_error("leaf_cells: output too large");
+  P.ensure_diag((int)(n_out + nv));
+  double* d_out = P.d_diag;
+  int* d_vars = reinterpret_cast<int*>(P.d_diag + n_out);
+  std::vector<int> hv(ixs_cc, ixs_cc + n_cc);
+  hv.insert(hv.end(), ixs_fc, ixs_fc + n_fc);
+  chk_d(hipMemcpyAsync(d_vars, hv.data(), hv.size() * sizeof(int), hipMemcpyHostToDevice,
+                       P.stream), "leaf_cells vars");
+  long long base = 0;
+  for (int l = 1; l <= T.max_lvl; l++) {
+    const int m = P.n_leaves(l);
+    if (!m) continue;
+    const long long work = (long long)m * cpb;
+    const dim3 grid((unsigned)((work + 255) / 256));
+    if (D == 2)
+      hipLaunchKernelGGL((k_leaf_cells<2>), grid, dim3(256), 0, P.stream, P.dev(), P.leaves(l), m,
+                         nc, base, d_vars, n_cc, n_fc, n_cells, d_out);
+    else
+      hipLaunchKernelGGL((k_leaf_cells<3>), grid, dim3(256), 0, P.stream, P.dev(), P.leaves(l), m,
+                         nc, base, d_vars, n_cc, n_fc, n_cells, d_out);
+    chk_d(hipGetLastError(), "leaf_cells");
+    base += work;
+  }
+  chk_d(hipMemcpyAsync(host.data(), d_out, (size_t)n_out * sizeof(double),
+                       hipMemcpyDeviceToHost, P.stream), "leaf_cells copy");
+  chk_d(hipStreamSynchronize(P.stream), "leaf_cells sync");
+  return n_cells;
+}
+
+}  // namespace a5g
+
+namespace a5g {
+
+// A VTK XML file laid out as the reference's m_vtk writes it
+// (afivo/src/m_vtk.f90): the XML part with two-space indentation, then one
+// raw appended block holding, per array, an int32 byte count and the data.
+struct VtkOut {
+  std::string xml;
+  std::vector<char> app;
+  int indent = 0;
+  long long offset = 0;
+  void line(const std::string& s) { xml += std::string(indent, ' ') + s + "\n"; }
+  void open(const std::string& tag) {
+    line("<" + tag + ">");
+    indent += 2;
+  }
+  void close(const std::string& tag) {
+    indent -= 2;
+    line("</" + tag + ">");
+  }
+  template <typename V>
+  void append(const V* p, size_t n) {
+    const int32_t nb = (int32_t)(n * sizeof(V));
+    const char* b = reinterpret_cast<const char*>(&nb);
+    app.insert(app.end(), b, b + 4);
+    const char* q = reinterpret_cast<const char*>(p);
+    app.insert(app.end(), q, q + n * sizeof(V));
+    offset += 4 + (long long)(n * sizeof(V));
+  }
+};
+
+// The value line of TIME: the reference writes it list-directed (a leading
+// blank, the indentation, a separator blank, the value); here the value has
+// the fewest significant digits that read back to the same double.
+static std::string num_line(int indent, double v) {
+  char b[64];
+  for (int p = 1; p <= 17; p++) {
+    std::snprintf(b, sizeof b, "%.*G", p, v);
+    if (std::strtod(b, nullptr) == v) break;
+  }
+  return std::string(indent + 2, ' ') + b;
+}
+
+void write_vtk(Pool& P, const char* filename, int n_cc, const char* const* cc_names,
+               const int* ixs_cc, int n_fc, const char* const* fc_names, const int* ixs_fc,
+               int n_cycle, double time) {
+  const auto& T = P.tree;
+  const int D = T.dim, bc = T.n_cell, bn = T.n_cell + 1;
+  const int n_ch = 1 << D;
+  const int nodes_per_box = D == 2 ? bn * bn : bn * bn * bn;
+  const int cells_per_box = D == 2 ? bc * bc : bc * bc * bc;
+  std::vector<int> leaves;
+  for (int l = 1; l <= T.max_lvl; l++)
+    for (int id : T.lv_leaves(l)) leaves.push_back(id);
+  const long long n_grids = (long long)leaves.size();
+  const long long n_nodes = nodes_per_box * n_grids, n_cells = cells_per_box * n_grids;
+  if (n_ch * n_cells > (long long)INT32_MAX / 4)
+    throw std::length_error("write_vtk: more cells than an int32 VTK block holds");
+  std::vector<double> cells;
+  leaf_cells(P, n_cc, ixs_cc, n_fc, ixs_fc, cells);
+  // node coordinates (Float32, three components), connectivity, offsets, types
+  std::vector<float> wr((size_t)(3 * n_nodes), 0.0f);
+  std::vector<int32_t> connects((size_t)(n_ch * n_cells)), offsets((size_t)n_cells),
+      types((size_t)n_cells, D == 2 ? 8 : 11);
+  for (long long g = 0; g < n_grids; g++) {
+    const int id = leaves[g];
+    const double* r0 = &T.rmin[id * 3];
+    const double dr = T.dr[id];
+    const long long node_ix = g * nodes_per_box, cell_ix = g * cells_per_box;
+    const int kn = D == 3 ? bn : 1, kc = D == 3 ? bc : 1;
+    for (int k = 1; k <= kn; k++)
+      for (int j = 1; j <= bn; j++)
+        for (int i = 1; i <= bn; i++) {
+          const long long nn = node_ix + (long long)(k - 1) * bn * bn + (j - 1) * bn + (i - 1);
+          const int w[3] = {i - 1, j - 1, k - 1};
+          for (int d = 0; d < D; d++) wr[3 * nn + d] = (float)(r0[d] + w[d] * dr);
+        }
+    for (int k = 1; k <= kc; k++)
+      for (int j = 1; j <= bc; j++)
+        for (int i = 1; i <= bc; i++) {
+          const long long n_ix = node_ix + (long long)(k - 1) * bn * bn + (j - 1) * bn + i - 1;
+          const long long c_ix = cell_ix + (long long)(k - 1) * bc * bc + (j - 1) * bc + i;
+          offsets[c_ix - 1] = (int32_t)(n_ch * c_ix);
+          int32_t* cn = &connects[n_ch * (c_ix - 1)];
+          const long long bn2 = (long long)bn * bn;
+          const long long q[8] = {n_ix, n_ix + 1, n_ix + bn, n_ix + bn + 1,
+                                  n_ix + bn2, n_ix + bn2 + 1, n_ix + bn2 + bn, n_ix + bn2 + bn + 1};
+          for (int c = 0; c < n_ch; c++) cn[c] = (int32_t)q[c];
+        }
+  }
+  VtkOut v;
+  v.xml = "<?xml version=\"1.0\"?>\n";
+  v.line("<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">");
+  v.indent = 2;
+  v.open("UnstructuredGrid");
+  v.open("FieldData");
+  v.line("<DataArray type=\"Float64\" Name=\"TIME\" NumberOfTuples=\"1\" format=\"ascii\">");
+  v.indent += 2;
+  v.xml += num_line(v.indent, time) + "\n";
+  v.close("DataArray");
+  v.line("<DataArray type=\"Int32\" Name=\"CYCLE\" NumberOfTuples=\"1\" format=\"ascii\">");
+  v.indent += 2;
+  v.xml += std::string(v.indent + 2, ' ') + std::to_string(n_cycle) + "\n";
+  v.close("DataArray");
+  v.close("FieldData");
+  v.line("<Piece NumberOfPoints=\"" + std::to_string(n_nodes) + "\" NumberOfCells=\"" +
+         std::to_string(n_cells) + "\">");
+  v.indent += 2;
+  v.open("Points");
+  v.line("<DataArray type=\"Float32\" NumberOfComponents=\"3\" Name=\"Points\" "
+         "format=\"appended\" offset=\"" + std::to_string(v.offset) + "\"/>");
+  v.append(wr.data(), wr.size());
+  v.close("Points");
+  v.open("Cells");
+  v.line("<DataArray type=\"Int32\" Name=\"connectivity\" format=\"appended\" offset=\"" +
+         std::to_string(v.offset) + "\"/>");
+  v.append(connects.data(), connects.size());
+  v.line("<DataArray type=\"Int32\" Name=\"offsets\" format=\"appended\" offset=\"" +
+         std::to_string(v.offset) + "\"/>");
+  v.append(offsets.data(), offsets.size());
+  v.line("<DataArray type=\"Int32\" Name=\"types\" format=\"appended\" offset=\"" +
+         std::to_string(v.offset) + "\"/>");
+  v.append(types.data(), types.size());
+  v.close("Cells");
+  v.open("CellData");
+  // cell variables, then per face variable one column per dimension; every
+  // face column of dimension d holds the same sum over all face variables
+  for (int c = 0; c < n_cc + n_fc * D; c++) {
+    const char* name = c < n_cc ? cc_names[c] : fc_names[c - n_cc];
+    const int col = c < n_cc ? c : n_cc + (c - n_cc) % D;
+    v.line("<DataArray type=\"Float64\" Name=\"" + std::string(name) +
+           "\" NumberOfComponents=\"1\" format=\"appended\" offset=\"" +
+           std::to_string(v.offset) + "\"/>");
+    v.append(cells.data() + (size_t)col * n_cells, (size_t)n_cells);
+  }
+  v.close("CellData");
+  v.close("Piece");
+  v.close("UnstructuredGrid");
+  v.line("<AppendedData encoding=\"raw\">");
+  const std::string fname = std::string(filename) + ".vtu";
+  FILE* f = std::fopen(fname.c_str(), "wb");
+  if (!f) throw std::runtime_error("write_vtk: cannot open " + fname);
+  bool ok = std::fwrite(v.xml.data(), 1, v.xml.size(), f) == v.xml.size();
+  ok = ok && std::fputc('_', f) != EOF;
+  ok = ok && std::fwrite(v.app.data(), 1, v.app.size(), f) == v.app.size();
+  std::string tail = "\n" + std::string(v.indent, ' ') + "</AppendedData>\n</VTKFile>\n";
+  ok = ok && std::fwrite(tail.data(), 1, tail.size(), f) == tail.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) throw std::runtime_error("write_vtk: cannot write " + fname);
+}
+
+// Raw leaf-box dump (little endian): "A5GLEAF1", int32 dim, n_cell,
+// n_leaves, n_var, coord, cycle, float64 time, int32 var[n_var]; per leaf
+// int32 id, lvl, ix[3] and float64 dr, r_min[3]; then float64
+// data[n_var][n_leaves][n_cell^dim], cells x fastest, interior only.
+void dump_leaves(Pool& P, const char* filename, int n_var, const int* ivs, int cycle,
+                 double time) {
+  const auto& T = P.tree;
+  std::vector<int> leaves;
+  for (int l = 1; l <= T.max_lvl; l++)
+    for (int id : T.lv_leaves(l)) leaves.push_back(id);
+  std::vector<double> cells;
+  leaf_cells(P, n_var, ivs, 0, nullptr, cells);
+  FILE* f = std::fopen(filename, "wb");
+  if (!f) throw std::runtime_error(std::string("dump_leaves: cannot open ") + filename);
+  bool ok = std::fwrite("A5GLEAF1", 1, 8, f) == 8;
+  const int32_t head[6] = {T.dim, T.n_cell, (int32_t)leaves.size(), n_var, T.coord, cycle};
+  ok = ok && std::fwrite(head, 4, 6, f) == 6;
+  ok = ok && std::fwrite(&time, 8, 1, f) == 1;
+  std::vector<int32_t> iv(ivs, ivs + n_var);
+  ok = ok && std::fwrite(iv.data(), 4, iv.size(), f) == iv.size();
+  for (int id : leaves) {
+    const int32_t rec[5] = {id, T.lvl[id], T.ix[id * 3], T.ix[id * 3 + 1], T.ix[id * 3 + 2]};
+    const double geo[4] = {T.dr[id], T.rmin[id * 3], T.rmin[id * 3 + 1], T.rmin[id * 3 + 2]};
+    ok = ok && std::fwrite(rec, 4, 5, f) == 5 && std::fwrite(geo, 8, 4, f) == 4;
+  }
+  ok = ok && std::fwrite(cells.data(), 8, cells.size(), f) == cells.size();
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) throw std::runtime_error(std::string("dump_leaves: cannot write ") + filename);
 }
 
 }  // namespace a5g

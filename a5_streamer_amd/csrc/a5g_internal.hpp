@@ -1,6 +1,8 @@
 // Internal entry points shared by the a5g translation units.
 #pragma once
 
+#include <vector>
+
 #include "a5g_pool.hpp"
 
 namespace a5g {
@@ -41,5 +43,12 @@ void reduction_loc(Pool& P, int kind, int var, int d, double thr, double init_va
                    int* id, int* ix);
 int cc_axis(Pool& P, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc, int cap,
             double* out);
+long long leaf_cells(Pool& P, int n_cc, const int* ixs_cc, int n_fc, const int* ixs_fc,
+                     std::vector<double>& host);
+void write_vtk(Pool& P, const char* filename, int n_cc, const char* const* cc_names,
+               const int* ixs_cc, int n_fc, const char* const* fc_names, const int* ixs_fc,
+               int n_cycle, double time);
+void dump_leaves(Pool& P, const char* filename, int n_var, const int* ivs, int cycle,
+                 double time);
 
 }  // namespace a5g

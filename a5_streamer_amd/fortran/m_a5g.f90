@@ -496,6 +496,42 @@ module m_a5g
        integer(c_int)        :: s
      end function c_cc_axis
 
+     function c_leaf_cells(t, n_cc, ixs_cc, n_fc, ixs_fc, cap, out, n_cells) &
+          bind(C, name="a5g_leaf_cells") result(s)
+       import :: c_ptr, c_int, c_double, c_long_long
+       type(c_ptr), value         :: t
+       integer(c_int), value      :: n_cc, n_fc
+       integer(c_int)             :: ixs_cc(*), ixs_fc(*)
+       integer(c_long_long), value :: cap
+       real(c_double)             :: out(*)
+       integer(c_long_long)       :: n_cells
+       integer(c_int)             :: s
+     end function c_leaf_cells
+
+     function c_write_vtk(t, filename, n_cc, cc_names, ixs_cc, n_fc, fc_names, ixs_fc, &
+          n_cycle, time) bind(C, name="a5g_write_vtk") result(s)
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value     :: t
+       character(kind=c_char) :: filename(*)
+       integer(c_int), value  :: n_cc, n_fc, n_cycle
+       type(c_ptr)            :: cc_names(*), fc_names(*)
+       type(c_ptr), value     :: ixs_cc
+       integer(c_int)         :: ixs_fc(*)
+       real(c_double), value  :: time
+       integer(c_int)         :: s
+     end function c_write_vtk
+
+     function c_dump_leaves(t, filename, n_var, ivs, cycle, time) &
+          bind(C, name="a5g_dump_leaves") result(s)
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value     :: t
+       character(kind=c_char) :: filename(*)
+       integer(c_int), value  :: n_var, cycle
+       integer(c_int)         :: ivs(*)
+       real(c_double), value  :: time
+       integer(c_int)         :: s
+     end function c_dump_leaves
+
      function c_prof_enable(t, on) bind(C, name="a5g_prof_enable") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -547,6 +583,8 @@ module m_a5g
   interface mg3_fas_vcycle;       module procedure t_fas_vcycle;        end interface
   interface a2_get_loc;           module procedure t_get_loc;           end interface
   interface a3_get_loc;           module procedure t_get_loc;           end interface
+  interface a2_write_vtk;         module procedure t_write_vtk;         end interface
+  interface a3_write_vtk;         module procedure t_write_vtk;         end interface
 
   public :: a2_init, a3_init
   public :: a2_destroy, a3_destroy, a2_set_base, a3_set_base
@@ -564,6 +602,7 @@ module m_a5g
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
+  public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
 
 contains
 
@@ -1064,5 +1103,91 @@ contains
     if (n > 0) call check(c_cc_axis(tree%h, size(ixs_cc), ixs_cc, size(ixs_fc), ixs_fc, n, &
          axis_data, n), "get_cc_axis")
   end subroutine st_get_cc_axis
+
+  ! ------------------------------------------------------------------
+  ! output
+  ! ------------------------------------------------------------------
+
+  !> a$D_write_vtk (afivo/src/m_aX_io.f90:20-209): the leaves as a VTK
+  !> unstructured grid in <dir>/<filename>.vtu
+  subroutine t_write_vtk(tree, filename, cc_names, n_cycle, time, ixs_cc, &
+       fc_names, ixs_fc, dir)
+    type(a5g_t), intent(inout)             :: tree
+    character(len=*), intent(in)           :: filename, cc_names(:)
+    integer, intent(in)                    :: n_cycle
+    real(dp), intent(in)                   :: time
+    integer, intent(in), optional          :: ixs_cc(:), ixs_fc(:)
+    character(len=*), intent(in), optional :: fc_names(:), dir
+    character(len=:), allocatable          :: fname
+    character(kind=c_char), allocatable, target :: ccb(:, :), fcb(:, :)
+    type(c_ptr), allocatable               :: ccp(:), fcp(:)
+    integer(c_int), allocatable, target    :: icc(:)
+    integer(c_int), allocatable            :: ifc(:)
+    type(c_ptr)                            :: picc
+    integer                                :: i, n_fc
+
+    if (present(ixs_cc)) then
+       if (size(ixs_cc) /= size(cc_names)) &
+            stop "a$D_write_vtk: size(cc_names) /= size(ixs_cc)"
+       icc = ixs_cc
+       picc = c_loc(icc)
+    else
+       picc = c_null_ptr
+    end if
+    n_fc = 0
+    allocate(ifc(1), fcp(1))
+    ifc = 0
+    fcp = c_null_ptr
+    if (present(fc_names)) then
+       if (.not. present(ixs_fc)) stop "a$D_write_vtk: ixs_fc not present (but fc_names is)"
+       if (size(ixs_fc) * tree%dim /= size(fc_names)) &
+            stop "a$D_write_vtk: size(fc_names) /= size(ixs_fc) * $D"
+       n_fc = size(ixs_fc)
+       ifc = ixs_fc
+       call c_strings(fc_names, fcb, fcp)
+    end if
+    call c_strings(cc_names, ccb, ccp)
+    fname = trim(filename)
+    if (present(dir)) then
+       i = len_trim(dir)
+       if (i > 0) then
+          if (dir(i:i) == "/") then
+             fname = trim(dir) // fname
+          else
+             fname = trim(dir) // "/" // fname
+          end if
+       end if
+    end if
+    call check(c_write_vtk(tree%h, fname // c_null_char, size(cc_names), ccp, picc, n_fc, &
+         fcp, ifc, n_cycle, time), "a$D_write_vtk")
+  end subroutine t_write_vtk
+
+  !> NUL-terminated copies of trimmed names and pointers to them
+  subroutine c_strings(names, buf, ptrs)
+    character(len=*), intent(in)                                  :: names(:)
+    character(kind=c_char), allocatable, target, intent(inout)    :: buf(:, :)
+    type(c_ptr), allocatable, intent(inout)                       :: ptrs(:)
+    integer :: i, j, n
+    allocate(buf(len(names) + 1, size(names)), ptrs(max(size(names), 1)))
+    ptrs = c_null_ptr
+    do i = 1, size(names)
+       n = len_trim(names(i))
+       do j = 1, n
+          buf(j, i) = names(i)(j:j)
+       end do
+       buf(n + 1, i) = c_null_char
+       ptrs(i) = c_loc(buf(1, i))
+    end do
+  end subroutine c_strings
+
+  !> The raw leaf-box dump of the variables ivs (see a5g_dump_leaves)
+  subroutine a5g_dump_leaves(tree, filename, ivs, cycle, time)
+    type(a5g_t), intent(inout)   :: tree
+    character(len=*), intent(in) :: filename
+    integer, intent(in)          :: ivs(:), cycle
+    real(dp), intent(in)         :: time
+    call check(c_dump_leaves(tree%h, trim(filename) // c_null_char, size(ivs), ivs, cycle, &
+         time), "dump_leaves")
+  end subroutine a5g_dump_leaves
 
 end module m_a5g

@@ -340,6 +340,43 @@ class Tree:
         _call("a5g_tree_minmax_cc", self.h, C.c_int(iv), C.byref(lo), C.byref(hi))
         return lo.value, hi.value
 
+    # --- output (afivo/src/m_aX_io.f90:20-209, m_vtk.f90) -------------------
+    def leaf_cells(self, ixs_cc, ixs_fc=()):
+        """a$D_write_vtk's cell data: array (n_cells, columns), leaves level
+        by level in list order, cells x fastest; the face columns (one per
+        dimension) are 0.5 * the sum over the cell's two faces and over all
+        face variables ixs_fc."""
+        cc, pcc = _i(np.atleast_1d(np.asarray(ixs_cc, np.int32)))
+        fc, pfc = _i(np.atleast_1d(np.asarray(ixs_fc, np.int32)))
+        n = C.c_longlong()
+        _call("a5g_leaf_cells", self.h, C.c_int(len(cc)), pcc, C.c_int(len(fc)), pfc,
+              C.c_longlong(0), None, C.byref(n))
+        cols = len(cc) + (self.dim if len(fc) else 0)
+        out = np.zeros((cols, n.value))
+        if out.size:
+            _call("a5g_leaf_cells", self.h, C.c_int(len(cc)), pcc, C.c_int(len(fc)), pfc,
+                  C.c_longlong(out.size), _d(out)[1], C.byref(n))
+        return out.T
+
+    def write_vtk(self, filename, cc_names, n_cycle, time, ixs_cc=None, fc_names=(),
+                  ixs_fc=()):
+        """a$D_write_vtk: <filename>.vtu with the leaves as pixels/voxels."""
+        cn = (C.c_char_p * max(len(cc_names), 1))(*[n.encode() for n in cc_names])
+        fn = (C.c_char_p * max(len(fc_names), 1))(*[n.encode() for n in fc_names])
+        if len(fc_names) != len(ixs_fc) * self.dim:
+            raise A5GError("write_vtk: size(fc_names) /= size(ixs_fc) * dim")
+        icc = None if ixs_cc is None else _i(np.asarray(ixs_cc))
+        ifc = _i(np.atleast_1d(np.asarray(ixs_fc, np.int32)))
+        _call("a5g_write_vtk", self.h, str(filename).encode(), C.c_int(len(cc_names)), cn,
+              None if icc is None else icc[1], C.c_int(len(ixs_fc)), fn, ifc[1],
+              C.c_int(n_cycle), C.c_double(time))
+
+    def dump_leaves(self, filename, ivs, cycle=0, time=0.0):
+        """The raw leaf-box dump of cell variables ivs (read_leaf_dump)."""
+        a, p = _i(np.atleast_1d(ivs))
+        _call("a5g_dump_leaves", self.h, str(filename).encode(), C.c_int(len(a)), p,
+              C.c_int(cycle), C.c_double(time))
+
     # --- streamer diagnostics (src/streamer_cyl.f90:682-956) ---------------
     def get_loc(self, r, max_lvl=0):
         """(id, ix) of the leaf cell holding point r (a$D_get_loc); (-1,
@@ -374,3 +411,33 @@ class Tree:
             _call("a5g_cc_axis", self.h, C.c_int(len(cc)), pcc, C.c_int(len(fc)), pfc,
                   C.c_int(n.value), _d(out)[1], C.byref(n))
         return out.T
+
+
+def read_leaf_dump(path):
+    """Read a raw leaf-box dump (a5g_dump_leaves): a dict with the header
+    fields, per-leaf id, lvl, ix (n, 3), dr, r_min (n, 3), the variable
+    indices and data (n_var, n_leaves) + (n_cell,) * dim in (x, y[, z])
+    index order."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    if raw[:8] != b"A5GLEAF1":
+        raise A5GError("%s is not an a5g leaf dump" % path)
+    dim, nc, n, nv, coord, cycle = np.frombuffer(raw, "<i4", 6, 8)
+    time = float(np.frombuffer(raw, "<f8", 1, 32)[0])
+    off = 40
+    ivs = np.frombuffer(raw, "<i4", nv, off)
+    off += 4 * nv
+    rec = np.dtype([("id", "<i4"), ("lvl", "<i4"), ("ix", "<i4", 3), ("dr", "<f8"),
+                    ("r_min", "<f8", 3)])
+    leaves = np.frombuffer(raw, rec, n, off)
+    off += rec.itemsize * n
+    cells = nc ** dim
+    data = np.frombuffer(raw, "<f8", nv * n * cells, off)
+    if off + data.nbytes != len(raw):
+        raise A5GError("%s: size does not match its header" % path)
+    # x fastest in the file: reverse the axes so that data[v, leaf][i, j(, k)]
+    data = data.reshape((nv, n) + (nc,) * dim).transpose((0, 1) + tuple(range(dim + 1, 1, -1)))
+    return dict(dim=int(dim), n_cell=int(nc), coord=int(coord), cycle=int(cycle), time=time,
+                ivs=ivs.copy(), id=leaves["id"].copy(), lvl=leaves["lvl"].copy(),
+                ix=leaves["ix"][:, :dim].copy(), dr=leaves["dr"].copy(),
+                r_min=leaves["r_min"][:, :dim].copy(), data=np.ascontiguousarray(data))

@@ -240,6 +240,41 @@ int a5g_reduction_loc(a5g_tree *t, int kind, int var, int d, double thr,
 int a5g_cc_axis(a5g_tree *t, int n_cc, const int *ixs_cc, int n_fc,
                 const int *ixs_fc, int cap, double *out, int *n_points);
 
+/* ---- output ----------------------------------------------------------- */
+
+/* The interior cells of the leaves in the order of a$D_write_vtk
+ * (afivo/src/m_aX_io.f90:20-209): leaves level by level in list order,
+ * cells x fastest. Columns of *n_cells values: cc(ixs_cc(v)) for each cell
+ * variable, then, when n_fc > 0, one column per dimension d holding
+ * 0.5 * sum of f_d over the cell's two faces along d and over all face
+ * variables ixs_fc, in array element order. Sets *n_cells. With out NULL or
+ * cap = 0 nothing else happens; otherwise writes out(*n_cells, columns)
+ * column-major, failing when *n_cells * columns exceeds cap. */
+int a5g_leaf_cells(a5g_tree *t, int n_cc, const int *ixs_cc, int n_fc,
+                   const int *ixs_fc, long long cap, double *out,
+                   long long *n_cells);
+/* a$D_write_vtk (afivo/src/m_aX_io.f90:20-209) with m_vtk
+ * (afivo/src/m_vtk.f90): the leaves as a VTK XML unstructured grid of
+ * pixels (2D) or voxels (3D) in <filename>.vtu, raw appended data. Cell
+ * variables ixs_cc (NULL: all, n_cc then equals the number of cell
+ * variables) named cc_names; with n_fc > 0 face variables ixs_fc whose
+ * n_fc * dim names fc_names label the face-average columns as the reference
+ * does (each column of dimension d holds the sum over all face variables).
+ * The TIME value is written with the fewest digits that read back to the
+ * same double (the reference writes it list-directed). */
+int a5g_write_vtk(a5g_tree *t, const char *filename, int n_cc,
+                  const char *const *cc_names, const int *ixs_cc, int n_fc,
+                  const char *const *fc_names, const int *ixs_fc, int n_cycle,
+                  double time);
+/* Raw leaf-box dump, the fixture and viewer format of this library (no
+ * counterpart in the reference, whose Silo output cannot be built here).
+ * Little endian: "A5GLEAF1", int32 dim, n_cell, n_leaves, n_var, coord,
+ * cycle, float64 time, int32 ivs[n_var]; per leaf (order of a5g_leaf_cells)
+ * int32 id, lvl, ix[3], float64 dr, r_min[3]; then float64
+ * data[n_var][n_leaves][n_cell^dim], cells x fastest, interior only. */
+int a5g_dump_leaves(a5g_tree *t, const char *filename, int n_var, const int *ivs,
+                    int cycle, double time);
+
 /* ---- measurement ----------------------------------------------------- */
 
 /* Start (on = 1) or stop timing every kernel launch with HIP events on the

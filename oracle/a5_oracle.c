@@ -1439,6 +1439,81 @@ int o_cc_axis(const otree *t, int n_cc, const int *ixs_cc, int n_fc, const int *
   return cnt;
 }
 
+long long o_leaf_cells(const otree *t, int n_cc, const int *ixs_cc, int n_fc,
+                       const int *ixs_fc, double *out) {
+  /* The cell data of a$D_write_vtk (m_aX_io.f90:87-174): leaves level by
+     level in list order, cells x fastest; columns cc(ixs_cc(v)), then with
+     face variables one column per dimension d: 0.5 * sum(f_d(two faces of
+     the cell along d, ixs_fc)), the section summed in array element order
+     from zero. out(n_cells, columns) column-major when given; returns
+     n_cells. */
+  int D = t->dim, nc = t->n_cell, cpb = D == 2 ? nc * nc : nc * nc * nc;
+  long long n_cells = 0;
+  for (int lvl = 1; lvl <= t->max_lvl; lvl++) n_cells += (long long)t->leaves[LV(t, lvl)].n * cpb;
+  if (!out) return n_cells;
+  long long c = 0;
+  for (int lvl = 1; lvl <= t->max_lvl; lvl++) {
+    const ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int n = 0; n < lv->n; n++) {
+      int id = lv->v[n];
+      for (int k = 1; k <= (D == 3 ? nc : 1); k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++, c++) {
+            size_t q = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            for (int v = 0; v < n_cc; v++) out[v * n_cells + c] = CC(t, id, ixs_cc[v])[q];
+            if (!n_fc) continue;
+            for (int d = 0; d < D; d++) {
+              int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+              hi[d]++;
+              double s = 0.0;
+              for (int v = 0; v < n_fc; v++) {
+                const double *f = FC(t, id, ixs_fc[v], d);
+                s = s + f[FI(nc, D, d, lo[0], lo[1], lo[2])];
+                s = s + f[FI(nc, D, d, hi[0], hi[1], hi[2])];
+              }
+              out[(n_cc + d) * n_cells + c] = 0.5 * s;
+            }
+          }
+    }
+  }
+  return n_cells;
+}
+
+void o_vtk_geometry(const otree *t, double *coords, int *connects, int *offsets, int *types) {
+  /* The node coordinates (dim per node), connectivity (0-based node
+     indices, 2^dim per cell), offsets and cell types (8 pixel, 11 voxel) of
+     a$D_write_vtk (m_aX_io.f90:106-174) */
+  int D = t->dim, bc = t->n_cell, bn = bc + 1, n_ch = 1 << D;
+  int npb = D == 2 ? bn * bn : bn * bn * bn, cpb = D == 2 ? bc * bc : bc * bc * bc;
+  long long g = 0;
+  for (int lvl = 1; lvl <= t->max_lvl; lvl++) {
+    const ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int n = 0; n < lv->n; n++, g++) {
+      int id = lv->v[n];
+      long long node_ix = g * npb, cell_ix = g * cpb;
+      for (int k = 1; k <= (D == 3 ? bn : 1); k++)
+        for (int j = 1; j <= bn; j++)
+          for (int i = 1; i <= bn; i++) {
+            long long nn = node_ix + (long long)(k - 1) * bn * bn + (long long)(j - 1) * bn + i - 1;
+            int w[3] = {i - 1, j - 1, k - 1};
+            for (int d = 0; d < D; d++) coords[D * nn + d] = RMP(t, id)[d] + w[d] * t->dr[id];
+          }
+      for (int k = 1; k <= (D == 3 ? bc : 1); k++)
+        for (int j = 1; j <= bc; j++)
+          for (int i = 1; i <= bc; i++) {
+            long long n_ix = node_ix + (long long)(k - 1) * bn * bn + (long long)(j - 1) * bn + i - 1;
+            long long c_ix = cell_ix + (long long)(k - 1) * bc * bc + (long long)(j - 1) * bc + i;
+            long long b2 = (long long)bn * bn;
+            long long q[8] = {n_ix, n_ix + 1, n_ix + bn, n_ix + bn + 1,
+                              n_ix + b2, n_ix + b2 + 1, n_ix + b2 + bn, n_ix + b2 + bn + 1};
+            for (int e = 0; e < n_ch; e++) connects[n_ch * (c_ix - 1) + e] = (int)q[e];
+            offsets[c_ix - 1] = (int)(n_ch * c_ix);
+            types[c_ix - 1] = D == 2 ? 8 : 11;
+          }
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* multigrid (m_aX_mg.f90)                                              */
 /* ------------------------------------------------------------------ */

@@ -75,6 +75,11 @@ void o_get_loc(const otree *t, const double *r, int max_lvl, int *id, int *ix);
    variable var, kind 2 = cell-centre coordinate d where cc(var) > thr */
 void o_reduction_loc(const otree *t, int kind, int var, int d, double thr, double init_val,
                      double *val, int *id, int *ix);
+/* a$D_write_vtk's cell data (returns n_cells; out may be NULL) and its
+   node coordinates, connectivity, offsets and cell types */
+long long o_leaf_cells(const otree *t, int n_cc, const int *ixs_cc, int n_fc,
+                       const int *ixs_fc, double *out);
+void o_vtk_geometry(const otree *t, double *coords, int *connects, int *offsets, int *types);
 /* get_cc_axis (2D): returns the number of points */
 int o_cc_axis(const otree *t, int n_cc, const int *ixs_cc, int n_fc, const int *ixs_fc,
               double *out, int cap);

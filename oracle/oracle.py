@@ -52,6 +52,7 @@ def lib():
         L.o_reduction_loc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_double, dp, ip, ip]
         L.o_cc_axis.restype = C.c_int
+        L.o_leaf_cells.restype = C.c_longlong
         for name in ("o_space_charge", "o_fld_from_pot", "o_average_dens"):
             getattr(L, name).argtypes = None
         _lib = L
@@ -267,6 +268,32 @@ class OTree:
                                C.c_double(thr), C.c_double(init_val), C.byref(val),
                                C.byref(idv), pix)
         return val.value, idv.value, ix
+
+    def leaf_cells(self, ixs_cc, ixs_fc):
+        """a$D_write_vtk's cell data: array (n_cells, columns)."""
+        cc, pcc = _i(np.atleast_1d(ixs_cc))
+        fc, pfc = _i(np.atleast_1d(ixs_fc))
+        n = self.L.o_leaf_cells(self.h, len(cc), pcc, len(fc), pfc, None)
+        cols = len(cc) + (self.dim if len(fc) else 0)
+        out = np.zeros((cols, n))
+        if n and cols:
+            self.L.o_leaf_cells(self.h, len(cc), pcc, len(fc), pfc, _d(out)[1])
+        return out.T
+
+    def vtk_geometry(self):
+        """(coords (n_nodes, dim), connects, offsets, types) of a$D_write_vtk."""
+        D, nc = self.dim, self.n_cell
+        n_grids = sum(len(self.level(l)[1]) for l in range(1, self.info()["max_lvl"] + 1))
+        n_nodes, n_cells = n_grids * (nc + 1) ** D, n_grids * nc ** D
+        coords = np.zeros((n_nodes, D))
+        conn = np.zeros(n_cells * 2 ** D, np.int32)
+        offs = np.zeros(n_cells, np.int32)
+        types = np.zeros(n_cells, np.int32)
+        if n_grids:
+            self.L.o_vtk_geometry(self.h, coords.ctypes.data_as(dp),
+                                  conn.ctypes.data_as(ip), offs.ctypes.data_as(ip),
+                                  types.ctypes.data_as(ip))
+        return coords, conn, offs, types
 
     def cc_axis(self, ixs_cc, ixs_fc):
         """get_cc_axis: array (len(ixs_cc) + len(ixs_fc) + 1, n)."""

@@ -10,6 +10,8 @@ module ref_streamer
   use m_morton
   use m_streamer, only: koren_mlim
   use omp_lib, only: omp_set_num_threads
+  use m_vtk, only: vtk_t, vtk_ini_xml, vtk_dat_xml, vtk_geo_xml, vtk_con_xml, &
+       vtk_var_r8_xml, vtk_geo_xml_close, vtk_end_xml
   implicit none
   private
 
@@ -116,6 +118,45 @@ contains
     call morton_rank(m, r)
     ranks = r
   end subroutine h_morton_rank
+
+  ! A VTK unstructured grid written by the reference's m_vtk with the calls of
+  ! a$D_write_vtk (afivo/src/m_aX_io.f90:190-203); names are 100 characters
+  ! each, blank padded
+  subroutine h_vtk_write(fname, n_nodes, n_cells, n_dim, coords, connects, offsets, &
+       cell_types, n_vars, names, vars, cycl, time) bind(C, name="ref_vtk_write")
+    character(kind=c_char), intent(in) :: fname(*)
+    integer(c_int), value              :: n_nodes, n_cells, n_dim, n_vars, cycl
+    real(c_double), intent(in)         :: coords(n_dim * n_nodes), vars(n_cells, n_vars)
+    integer(c_int), intent(in)         :: connects(2**n_dim * n_cells), offsets(n_cells)
+    integer(c_int), intent(in)         :: cell_types(n_cells)
+    character(kind=c_char), intent(in) :: names(100 * n_vars)
+    real(c_double), value              :: time
+    type(vtk_t)                        :: vtkf
+    character(len=400)                 :: f
+    character(len=100)                 :: nm
+    integer                            :: i, n
+
+    f = ''
+    do i = 1, 400
+       if (fname(i) == c_null_char) exit
+       f(i:i) = fname(i)
+    end do
+    call vtk_ini_xml(vtkf, trim(f), 'UnstructuredGrid')
+    call vtk_dat_xml(vtkf, "UnstructuredGrid", .true.)
+    call vtk_geo_xml(vtkf, coords, n_nodes, n_cells, n_dim, cycl, time)
+    call vtk_con_xml(vtkf, connects, offsets, cell_types, n_cells)
+    call vtk_dat_xml(vtkf, "CellData", .true.)
+    do n = 1, n_vars
+       do i = 1, 100
+          nm(i:i) = names(100 * (n - 1) + i)
+       end do
+       call vtk_var_r8_xml(vtkf, trim(nm), vars(:, n), n_cells)
+    end do
+    call vtk_dat_xml(vtkf, "CellData", .false.)
+    call vtk_geo_xml_close(vtkf)
+    call vtk_dat_xml(vtkf, "UnstructuredGrid", .false.)
+    call vtk_end_xml(vtkf)
+  end subroutine h_vtk_write
 
   ! OpenMP threads of the reference routines called after this
   subroutine h_set_threads(n) bind(C, name="ref_set_threads")

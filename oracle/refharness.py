@@ -232,6 +232,21 @@ class RefTree:
         return self.f("min_dr")()
 
 
+def vtk_write(fname, coords, connects, offsets, types, names, variables, cycle, time):
+    """A .vtu file written by the reference's m_vtk with the call sequence of
+    a$D_write_vtk; coords (n_nodes, dim), variables (n_cells, n_vars)."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    n_nodes, dim = coords.shape
+    v = np.asfortranarray(variables, np.float64)
+    n_cells, n_vars = v.shape
+    nm = b"".join(n.encode().ljust(100) for n in names)
+    lib().ref_vtk_write(str(fname).encode() + b"\0", C.c_int(n_nodes), C.c_int(n_cells),
+                        C.c_int(dim), _d(coords.ravel())[1], _i(connects)[1], _i(offsets)[1],
+                        _i(types)[1], C.c_int(n_vars), C.c_char_p(nm),
+                        v.ctypes.data_as(C.POINTER(C.c_double)), C.c_int(cycle),
+                        C.c_double(time))
+
+
 def set_threads(n):
     """OpenMP threads of the reference routines called afterwards."""
     lib().ref_set_threads(C.c_int(n))
