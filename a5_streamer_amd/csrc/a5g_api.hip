@@ -53,7 +53,24 @@ Dev Pool::dev() {
     a.bc_kind[i] = bc_kind[i];
     a.bc_val[i] = bc_val[i];
   }
+  a.n_inst = n_inst;
+  a.inst_stride = inst_stride;
+  a.inst_val = d_inst;
   return a;
+}
+
+// The instance of a box: its position along the last dimension, a quarter
+// cell inside the box so that rounding of r_min cannot move it across
+int Pool::inst_of(int id) const {
+  const int d = tree.dim - 1;
+  const int g = (int)std::floor((tree.rmin[id * 3 + d] + 0.25 * tree.dr[id]) / inst_stride);
+  return std::min(std::max(g, 0), n_inst - 1);
+}
+
+bool Pool::all_active() const {
+  for (char c : inst_active)
+    if (!c) return false;
+  return true;
 }
 
 template <typename T>
@@ -109,35 +126,57 @@ void Pool::ensure_capacity(int need) {
 
 void Pool::sync_topology() {
   if (host_only) throw std::runtime_error("no GPU visible: cell data and kernels are unavailable");
-  if (!topo_dirty) return;
-  ensure_capacity(std::max(tree.max_id, tree.cap));
-  const int n = tree.max_id + 1;
+  if (!topo_dirty && !lists_dirty) return;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     hcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "topology upload");
   };
-  up(d_lvl, tree.lvl.data(), n * sizeof(int));
-  up(d_ix, tree.ix.data(), (size_t)n * 3 * sizeof(int));
-  up(d_parent, tree.parent.data(), n * sizeof(int));
-  up(d_children, tree.children.data(), (size_t)n * 8 * sizeof(int));
-  up(d_nbs, tree.nbs.data(), (size_t)n * 6 * sizeof(int));
-  up(d_ncell, tree.ncell.data(), n * sizeof(int));
-  up(d_tag, tree.tag.data(), n * sizeof(int));
-  up(d_dr, tree.dr.data(), n * sizeof(double));
-  up(d_rmin, tree.rmin.data(), (size_t)n * 3 * sizeof(double));
-  // level lists
+  if (topo_dirty) {
+    ensure_capacity(std::max(tree.max_id, tree.cap));
+    const int n = tree.max_id + 1;
+    up(d_lvl, tree.lvl.data(), n * sizeof(int));
+    up(d_ix, tree.ix.data(), (size_t)n * 3 * sizeof(int));
+    up(d_parent, tree.parent.data(), n * sizeof(int));
+    up(d_children, tree.children.data(), (size_t)n * 8 * sizeof(int));
+    up(d_nbs, tree.nbs.data(), (size_t)n * 6 * sizeof(int));
+    up(d_ncell, tree.ncell.data(), n * sizeof(int));
+    up(d_tag, tree.tag.data(), n * sizeof(int));
+    up(d_dr, tree.dr.data(), n * sizeof(double));
+    up(d_rmin, tree.rmin.data(), (size_t)n * 3 * sizeof(double));
+  }
+  // level lists (with instances, only the boxes of active instances)
   const int nl = (int)tree.ids.size();
   off_ids.assign(nl, 0);
   off_leaves.assign(nl, 0);
   off_parents.assign(nl, 0);
+  const bool filter = n_inst > 0 && !all_active();
+  auto pick = [&](const std::vector<std::vector<int>>& src, std::vector<std::vector<int>>& dst) {
+    dst.assign(src.size(), {});
+    for (size_t l = 0; l < src.size(); l++) {
+      if (!filter) {
+        dst[l] = src[l];
+        continue;
+      }
+      for (int id : src[l])
+        if (inst_active[inst_of(id)]) dst[l].push_back(id);
+    }
+  };
+  if (n_inst) {
+    pick(tree.ids, act_ids);
+    pick(tree.leaves, act_leaves);
+    pick(tree.parents, act_parents);
+  }
+  const auto& L_ids = n_inst ? act_ids : tree.ids;
+  const auto& L_leaves = n_inst ? act_leaves : tree.leaves;
+  const auto& L_parents = n_inst ? act_parents : tree.parents;
   static thread_local std::vector<int> host;
   host.clear();
   for (int l = 0; l < nl; l++) {
     off_ids[l] = host.size();
-    host.insert(host.end(), tree.ids[l].begin(), tree.ids[l].end());
+    host.insert(host.end(), L_ids[l].begin(), L_ids[l].end());
     off_leaves[l] = host.size();
-    host.insert(host.end(), tree.leaves[l].begin(), tree.leaves[l].end());
+    host.insert(host.end(), L_leaves[l].begin(), L_leaves[l].end());
     off_parents[l] = host.size();
-    host.insert(host.end(), tree.parents[l].begin(), tree.parents[l].end());
+    host.insert(host.end(), L_parents[l].begin(), L_parents[l].end());
   }
   if (host.size() > lists_cap || !d_lists) {
     lists_cap = std::max(host.size(), lists_cap * 3 / 2) + 64;
@@ -146,6 +185,7 @@ void Pool::sync_topology() {
   if (!host.empty()) up(d_lists, host.data(), host.size() * sizeof(int));
   hcheck(hipStreamSynchronize(stream), "topology sync");
   topo_dirty = false;
+  lists_dirty = false;
   topo_version++;
 }
 
@@ -266,7 +306,8 @@ void Pool::release() {
   f(cc32);
   cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
-  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag);
+  f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag); f(d_inst);
+  d_inst = nullptr;
   d_flux_pairs = nullptr;
   d_diag = nullptr;
   diag_cap = 0;
@@ -694,7 +735,7 @@ int a5g_mg_solve_mixed(a5g_tree* t, double tol, int max_outer, int n_inner, int 
     P.sync_topology();
     std::vector<int> all;
     for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++)
-      all.insert(all.end(), P.tree.lv_ids(l).begin(), P.tree.lv_ids(l).end());
+      all.insert(all.end(), P.lids(l).begin(), P.lids(l).end());
     a5g::fill_boxes(P, (int)all.size(), all.data(), P.mg.i_phi, 0.0);
     a5g::gc_tree(P, P.mg.i_phi, 0, 0);
   }
@@ -1020,7 +1061,8 @@ int a5g_leaf_cells(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int
   if (!n_cells) throw std::invalid_argument("leaf_cells: null n_cells");
   check_out_vars(T, n_cc, ixs_cc, n_fc, ixs_fc);
   long long n_leaves = 0;
-  for (int l = 1; l <= T.max_lvl; l++) n_leaves += (long long)T.lv_leaves(l).size();
+  if (t->P.n_inst && !t->P.host_only) t->P.sync_topology();
+  for (int l = 1; l <= T.max_lvl; l++) n_leaves += (long long)t->P.lleaves(l).size();
   const long long cpb = T.dim == 2 ? (long long)T.n_cell * T.n_cell
                                    : (long long)T.n_cell * T.n_cell * T.n_cell;
   *n_cells = n_leaves * cpb;
@@ -1067,6 +1109,64 @@ int a5g_dump_leaves(a5g_tree* t, const char* filename, int n_var, const int* ivs
   if (!filename) throw std::invalid_argument("dump_leaves: null filename");
   check_out_vars(t->P.tree, n_var, ivs, 0, nullptr);
   a5g::dump_leaves(t->P, filename, n_var, ivs, cycle, time);
+  API_END
+}
+
+int a5g_set_instances(a5g_tree* t, int n_inst, double stride) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (n_inst < 0) throw std::invalid_argument("set_instances: negative count");
+  if (n_inst > 0 && !(stride > 0)) throw std::invalid_argument("set_instances: stride must be > 0");
+  if (P.d_inst) a5g::hcheck(hipFree(P.d_inst), "hipFree");
+  P.d_inst = nullptr;
+  P.n_inst = n_inst;
+  P.inst_stride = n_inst ? stride : 0.0;
+  P.h_inst.assign((size_t)n_inst * a5g::kInstVals, 0.0);
+  P.inst_active.assign(n_inst, 1);
+  if (n_inst && !P.host_only) {
+    a5g::hcheck(hipMalloc((void**)&P.d_inst, P.h_inst.size() * sizeof(double)), "hipMalloc");
+    a5g::hcheck(hipMemcpyAsync(P.d_inst, P.h_inst.data(), P.h_inst.size() * sizeof(double),
+                          hipMemcpyHostToDevice, P.stream), "instances");
+    a5g::hcheck(hipStreamSynchronize(P.stream), "instances sync");
+  }
+  P.topo_dirty = true;
+  API_END
+}
+
+int a5g_set_instance_values(a5g_tree* t, int what, int n, const double* values) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  if (!P.n_inst) throw std::logic_error("set_instance_values: no instances set");
+  if (n != P.n_inst || !values) throw std::invalid_argument("set_instance_values: one value per instance");
+  if (what < 0 || what > 2 * P.tree.dim) throw std::out_of_range("set_instance_values: what");
+  // the constants and the expression of fluxes_koren's clamp factor
+  const double eps0 = 8.8541878176e-12, qe = 1.6022e-19;
+  for (int g = 0; g < n; g++) {
+    double* v = &P.h_inst[(size_t)g * a5g::kInstVals];
+    if (what == 0) {
+      v[a5g::kInstDt] = values[g];
+      v[a5g::kInstKorenFac] = -0.8 * eps0 / (qe * values[g]);
+    } else {
+      v[a5g::kInstBc + what - 1] = values[g];
+    }
+  }
+  a5g::hcheck(hipMemcpyAsync(P.d_inst, P.h_inst.data(), P.h_inst.size() * sizeof(double),
+                        hipMemcpyHostToDevice, P.stream), "instance values");
+  a5g::hcheck(hipStreamSynchronize(P.stream), "instance values sync");
+  API_END
+}
+
+int a5g_set_active_instances(a5g_tree* t, int n, const int* active) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (!P.n_inst) throw std::logic_error("set_active_instances: no instances set");
+  if (active && n != P.n_inst) throw std::invalid_argument("set_active_instances: one flag per instance");
+  for (int g = 0; g < P.n_inst; g++) P.inst_active[g] = active ? (active[g] != 0) : 1;
+  P.lists_dirty = true;
   API_END
 }
 

@@ -127,7 +127,7 @@ void reduction_loc(Pool& P, int kind, int var, int d, double thr, double init_va
   const int D = T.dim;
   std::vector<int> order;  // leaf ids in the reference's order
   for (int l = T.min_lvl; l <= T.max_lvl; l++)
-    for (int id : T.lv_leaves(l)) order.push_back(id);
+    for (int id : P.lleaves(l)) order.push_back(id);
   const int n = (int)order.size();
   for (int e = 0; e < D; e++) ix[e] = -1;
   *val = init_val;
@@ -376,6 +376,7 @@ static std::string num_line(int indent, double v) {
 void write_vtk(Pool& P, const char* filename, int n_cc, const char* const* cc_names,
                const int* ixs_cc, int n_fc, const char* const* fc_names, const int* ixs_fc,
                int n_cycle, double time) {
+  P.sync_topology();
   const auto& T = P.tree;
   const int D = T.dim, bc = T.n_cell, bn = T.n_cell + 1;
   const int n_ch = 1 << D;
@@ -383,7 +384,7 @@ void write_vtk(Pool& P, const char* filename, int n_cc, const char* const* cc_na
   const int cells_per_box = D == 2 ? bc * bc : bc * bc * bc;
   std::vector<int> leaves;
   for (int l = 1; l <= T.max_lvl; l++)
-    for (int id : T.lv_leaves(l)) leaves.push_back(id);
+    for (int id : P.lleaves(l)) leaves.push_back(id);
   const long long n_grids = (long long)leaves.size();
   const long long n_nodes = nodes_per_box * n_grids, n_cells = cells_per_box * n_grids;
   if (n_ch * n_cells > (long long)INT32_MAX / 4)
@@ -487,10 +488,11 @@ void write_vtk(Pool& P, const char* filename, int n_cc, const char* const* cc_na
 // data[n_var][n_leaves][n_cell^dim], cells x fastest, interior only.
 void dump_leaves(Pool& P, const char* filename, int n_var, const int* ivs, int cycle,
                  double time) {
+  P.sync_topology();
   const auto& T = P.tree;
   std::vector<int> leaves;
   for (int l = 1; l <= T.max_lvl; l++)
-    for (int id : T.lv_leaves(l)) leaves.push_back(id);
+    for (int id : P.lleaves(l)) leaves.push_back(id);
   std::vector<double> cells;
   leaf_cells(P, n_var, ivs, 0, nullptr, cells);
   FILE* f = std::fopen(filename, "wb");

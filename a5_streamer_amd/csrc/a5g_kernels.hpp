@@ -39,6 +39,14 @@ __device__ __forceinline__ double* facep(const Dev& a, int ifc, int id, int d, i
   return a.fc + (size_t)(ifc - 1) * a.fvs + ((size_t)id * dim + d) * a.fs;
 }
 
+// The instance of box id (Pool::inst_of): its position along the last
+// dimension, a quarter cell inside the box
+template <int D>
+__device__ __forceinline__ int inst_of(const Dev& a, int id) {
+  const int g = (int)floor((a.rmin[id * 3 + D - 1] + 0.25 * a.dr[id]) / a.inst_stride);
+  return min(max(g, 0), a.n_inst - 1);
+}
+
 // index into a face array fx(nc+1,nc[,nc]) / fy / fz of a box (1-based)
 template <int D>
 __device__ __forceinline__ int fidx(int nc, int d, int i, int j, int k) {
@@ -109,7 +117,9 @@ __device__ __forceinline__ T ghost_mg_u(const Dev& a, U own, int iv, int id, int
     const int kind = a.bc_kind[nb - 1];
     if (kind == 0) return cv;
     if (kind == 1) return -cv;
-    return T(2) * (T)a.bc_val[nb - 1] - cv;
+    const double V = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstBc + nb - 1]
+                              : a.bc_val[nb - 1];
+    return T(2) * (T)V - cv;
   }
 }
 

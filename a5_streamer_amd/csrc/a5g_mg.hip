@@ -1365,8 +1365,10 @@ static Ctx ctx64(const Pool& P) {
 
 static Dev dev_for(Pool& P, const Ctx& c) {
   Dev a = P.dev();
-  if (c.homogeneous)
+  if (c.homogeneous) {
     for (int i = 0; i < 6; i++) a.bc_val[i] = 0.0;
+    a.n_inst = 0;  // the correction problem has zero boundary values everywhere
+  }
   if (c.eps) {
     a.tag = P.d_tag;
     a.i_eps = P.mg.i_eps;
@@ -1559,9 +1561,9 @@ static void update_coarse(Pool& P, const Ctx& c, int lvl) {
 }
 
 template <int D, typename T>
-static void set_coarse_phi_rhs(Pool& P, const Ctx& c, int lvl) {
+static void set_coarse_phi_rhs(Pool& P, const Ctx& c, int lvl, int max_lvl) {
   // set_coarse_phi_rhs, m_aX_mg.f90:490-523
-  if (lvl == P.tree.max_lvl) fill_gc_phi<D, T>(P, c, lvl);
+  if (lvl == max_lvl) fill_gc_phi<D, T>(P, c, lvl);
   resid_rstr<D, T>(P, c, lvl, 1);
   fill_gc_phi<D, T>(P, c, lvl - 1);
   coarse_rhs<D, T>(P, c, lvl - 1, 0);
@@ -1673,17 +1675,19 @@ static void fas_vcycle(Pool& P, const Ctx& c, int max_lvl, bool set_residual) {
 template <int D, typename T>
 static void fas_fmg(Pool& P, const Ctx& c, bool set_residual, bool first_call) {
   // mg$D_fas_fmg, m_aX_mg.f90:164-203
-  const int min_lvl = P.tree.min_lvl;
+  // tree%max_lvl of the reference; with instances the finest level of the
+  // active ones, so that each run takes the V-cycles its own tree would
+  const int min_lvl = P.tree.min_lvl, max_lvl = P.mg_max_lvl();
   if (first_call) init_phi_rhs<D, T>(P, c);
   else
-    for (int lvl = P.tree.max_lvl; lvl >= min_lvl + 1; lvl--) set_coarse_phi_rhs<D, T>(P, c, lvl);
-  for (int lvl = min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) set_coarse_phi_rhs<D, T>(P, c, lvl, max_lvl);
+  for (int lvl = min_lvl; lvl <= max_lvl; lvl++) {
     copy_level_t<T>(P, lvl, c.i_phi, c.i_tmp);
     if (lvl > min_lvl) {
       correct_children<D, T>(P, c, lvl - 1);
       fill_gc_phi<D, T>(P, c, lvl);
     }
-    fas_vcycle<D, T>(P, c, lvl, set_residual && lvl == P.tree.max_lvl);
+    fas_vcycle<D, T>(P, c, lvl, set_residual && lvl == max_lvl);
   }
 }
 
@@ -1762,6 +1766,12 @@ static uint64_t state_fingerprint(Pool& P) {
     std::memcpy(&b, &P.bc_val[i], 8);
     num(b);
   }
+  num(P.n_inst); ptr(P.d_inst);
+  {
+    uint64_t sb;
+    std::memcpy(&sb, &P.inst_stride, 8);
+    num(sb);
+  }
   num(P.mg.i_phi); num(P.mg.i_tmp); num(P.mg.i_rhs); num(P.mg.i_eps);
   num(P.mg.n_down); num(P.mg.n_up); num(P.mg.n_base); num(P.mg.auto_ops);
   {
@@ -1769,6 +1779,13 @@ static uint64_t state_fingerprint(Pool& P) {
     std::memcpy(&hb, &P.mg.helm, 8);
     num(hb);
   }
+  return h;
+}
+
+// The level lists a captured call reads: counts and offsets per level
+static uint64_t lists_fingerprint(Pool& P) {
+  uint64_t h = 1469598103934665603ull;
+  auto num = [&](uint64_t v) { h = fnv(h, v); };
   num((uint64_t)(int64_t)P.tree.min_lvl); num(P.tree.max_lvl);
   for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++) {
     const int k = l - P.tree.min_lvl;
@@ -1777,6 +1794,9 @@ static uint64_t state_fingerprint(Pool& P) {
   }
   return h;
 }
+
+// graphs kept for earlier level-list states (instances switched on and off)
+constexpr size_t kMaxGraphs = 48;
 
 template <class F>
 static void run_captured(Pool& P, uint64_t key, F&& f) {
@@ -1789,11 +1809,16 @@ static void run_captured(Pool& P, uint64_t key, F&& f) {
     P.clear_graphs();
     P.graphs_state = st;
   }
+  key = fnv(lists_fingerprint(P), key);
   for (auto& g : P.graphs)
     if (g.first == key) {
       check(hipGraphLaunch(g.second, P.stream), "graph launch");
       return;
     }
+  if (P.graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(P.graphs.front().second);
+    P.graphs.erase(P.graphs.begin());
+  }
   check(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal), "begin capture");
   try {
     f();

@@ -39,7 +39,18 @@ struct Dev {
   // Helmholtz term of the multigrid operator: lpl(phi) - helm * phi (0: the
   // Laplacian)
   double helm = 0;
+  // Independent runs in one tree (instances), stacked along the last
+  // dimension one every inst_stride (n_inst = 0: a single run, the scalar
+  // values apply). inst_val holds kInstVals values per instance.
+  int n_inst = 0;
+  double inst_stride = 0;
+  const double* inst_val = nullptr;
 };
+
+// Per-instance values: the time step, the Koren flux clamp factor
+// -0.8 eps0 / (e dt) and the value V of the rule 2V - c on faces 1..6.
+constexpr int kInstVals = 8;
+constexpr int kInstDt = 0, kInstKorenFac = 1, kInstBc = 2;
 
 // Per-launch timing with HIP events on the pool's stream (kernel classes
 // below), used by the benchmark to measure kernel durations in place.
@@ -96,6 +107,7 @@ struct Pool {
   // host offsets into d_lists, per level: ids, leaves, parents
   std::vector<size_t> off_ids, off_leaves, off_parents;
   bool topo_dirty = true;
+  bool lists_dirty = false;  // only the level lists need an upload (a mask change)
   uint64_t topo_version = 0;  // counts topology uploads
   // (parent, direction) pairs of a$D_consistent_fluxes per level, on the
   // device, for topology version flux_pairs_version
@@ -133,6 +145,28 @@ struct Pool {
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   void clear_graphs();
 
+  // instances (a5g_set_instances): the device level lists hold only the
+  // boxes of active instances
+  int n_inst = 0;
+  double inst_stride = 0;
+  double* d_inst = nullptr;
+  std::vector<double> h_inst;
+  std::vector<char> inst_active;
+  std::vector<std::vector<int>> act_ids, act_leaves, act_parents;
+  int inst_of(int id) const;
+  bool all_active() const;
+  // the level lists the device holds (all boxes, or those of active
+  // instances); valid after sync_topology
+  const std::vector<int>& lids(int l) const {
+    return n_inst && !topo_dirty && !lists_dirty ? act_ids[l - tree.min_lvl] : tree.lv_ids(l);
+  }
+  const std::vector<int>& lleaves(int l) const {
+    return n_inst && !topo_dirty && !lists_dirty ? act_leaves[l - tree.min_lvl] : tree.lv_leaves(l);
+  }
+  const std::vector<int>& lparents(int l) const {
+    return n_inst && !topo_dirty && !lists_dirty ? act_parents[l - tree.min_lvl] : tree.lv_parents(l);
+  }
+
   int bs() const;
   int fs() const;
   Dev dev();
@@ -144,9 +178,18 @@ struct Pool {
   const int* ids(int l) const { return d_lists + off_ids[l - tree.min_lvl]; }
   const int* leaves(int l) const { return d_lists + off_leaves[l - tree.min_lvl]; }
   const int* parents(int l) const { return d_lists + off_parents[l - tree.min_lvl]; }
-  int n_ids(int l) const { return (int)tree.lv_ids(l).size(); }
-  int n_leaves(int l) const { return (int)tree.lv_leaves(l).size(); }
-  int n_parents(int l) const { return (int)tree.lv_parents(l).size(); }
+  int n_ids(int l) const { return (int)lids(l).size(); }
+  int n_leaves(int l) const { return (int)lleaves(l).size(); }
+  int n_parents(int l) const { return (int)lparents(l).size(); }
+  // the finest level of the multigrid: with instances the finest level
+  // holding boxes of an active instance (after sync_topology)
+  int mg_max_lvl() const {
+    if (!n_inst) return tree.max_lvl;
+    int m = tree.min_lvl;
+    for (int l = tree.min_lvl; l <= tree.max_lvl; l++)
+      if (n_ids(l) > 0) m = l;
+    return m;
+  }
   int nc_lvl(int l) const {
     return l >= 1 ? tree.n_cell : tree.n_cell / (1 << (1 - l));
   }

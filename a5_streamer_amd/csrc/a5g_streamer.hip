@@ -221,9 +221,12 @@ template <int D>
 __global__ __launch_bounds__(256) void k_koren(Dev a, Tbl tb, const int* __restrict__ list,
                                                int n, int nc, int tpb, int ie, int ifld,
                                                int fel, int ffld, int cmob, int cdif,
-                                               double fac) {
+                                               double fac_all) {
   BOXES_PER_BLOCK
   if (!act) return;
+  // the clamp factor of the box's own run when the tree holds several
+  const double fac = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstKorenFac]
+                              : fac_all;
   const double inv_dr = 1 / a.dr[id];
   const double* ne = boxp(a, ie, id);
   const double* E = boxp(a, ifld, id);
@@ -364,9 +367,12 @@ template <int D, bool CYL>
 __global__ __launch_bounds__(256) void k_update(Dev a, Tbl tb, const int* __restrict__ list,
                                                 int n, int nc, int tpb, int ie, int ip,
                                                 int ifld, int ieps, int ipho, int fel,
-                                                int calpha, int ceta, int diel, double dt) {
+                                                int calpha, int ceta, int diel,
+                                                double dt_all) {
   BOXES_PER_BLOCK
   if (!act) return;
+  // the time step of the box's own run when the tree holds several
+  const double dt = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstDt] : dt_all;
   const double inv_dr = 1 / a.dr[id];
   double* ne = boxp(a, ie, id);
   double* ni = boxp(a, ip, id);
@@ -701,7 +707,7 @@ void consistent_fluxes(Pool& P, int f_elec) {
     P.flux_pairs_n.assign(std::max(nl, 0), 0);
     for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
       P.flux_pairs_off[lvl - T.min_lvl] = pairs.size();
-      for (int id : T.lv_parents(lvl))
+      for (int id : P.lparents(lvl))
         for (int nb = 1; nb <= T.nnb; nb++) {
           const int nb_id = T.nbs[id * 6 + nb - 1];
           if (nb_id > 0 && !T.has_children(nb_id)) pairs.push_back(make_int2(id, nb));

@@ -532,6 +532,32 @@ module m_a5g
        integer(c_int)         :: s
      end function c_dump_leaves
 
+     function c_set_instances(t, n_inst, stride) bind(C, name="a5g_set_instances") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n_inst
+       real(c_double), value :: stride
+       integer(c_int)        :: s
+     end function c_set_instances
+
+     function c_set_instance_values(t, what, n, values) &
+          bind(C, name="a5g_set_instance_values") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: what, n
+       real(c_double)        :: values(*)
+       integer(c_int)        :: s
+     end function c_set_instance_values
+
+     function c_set_active_instances(t, n, active) &
+          bind(C, name="a5g_set_active_instances") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: active(*)
+       integer(c_int)        :: s
+     end function c_set_active_instances
+
      function c_prof_enable(t, on) bind(C, name="a5g_prof_enable") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -603,6 +629,7 @@ module m_a5g
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
+  public :: a5g_set_instances, a5g_set_instance_values, a5g_set_active_instances
 
 contains
 
@@ -1189,5 +1216,35 @@ contains
     call check(c_dump_leaves(tree%h, trim(filename) // c_null_char, size(ivs), ivs, cycle, &
          time), "dump_leaves")
   end subroutine a5g_dump_leaves
+
+  ! ------------------------------------------------------------------
+  ! independent runs in one tree
+  ! ------------------------------------------------------------------
+
+  !> n_inst runs stacked along the last dimension, one every stride
+  subroutine a5g_set_instances(tree, n_inst, stride)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: n_inst
+    real(dp), intent(in)       :: stride
+    call check(c_set_instances(tree%h, n_inst, stride), "set_instances")
+  end subroutine a5g_set_instances
+
+  !> Per-run time steps (what = 0) or values V of the rule 2V - c on face what
+  subroutine a5g_set_instance_values(tree, what, values)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: what
+    real(dp), intent(in)       :: values(:)
+    call check(c_set_instance_values(tree%h, what, size(values), values), &
+         "set_instance_values")
+  end subroutine a5g_set_instance_values
+
+  !> Restrict the tree-wide calls to the runs with active(g)
+  subroutine a5g_set_active_instances(tree, active)
+    type(a5g_t), intent(inout) :: tree
+    logical, intent(in)        :: active(:)
+    integer(c_int) :: a(size(active))
+    a = merge(1, 0, active)
+    call check(c_set_active_instances(tree%h, size(a), a), "set_active_instances")
+  end subroutine a5g_set_active_instances
 
 end module m_a5g

@@ -340,6 +340,26 @@ class Tree:
         _call("a5g_tree_minmax_cc", self.h, C.c_int(iv), C.byref(lo), C.byref(hi))
         return lo.value, hi.value
 
+    # --- independent runs in one tree ----------------------------------------
+    def set_instances(self, n_inst, stride):
+        """n_inst independent runs stacked along the last dimension, one every
+        stride (a5g_set_instances)."""
+        _call("a5g_set_instances", self.h, C.c_int(n_inst), C.c_double(stride))
+
+    def set_instance_values(self, what, values):
+        """Per-run time steps (what = 0) or values V of the rule 2V - c on
+        face what (1..2 dim)."""
+        v, p = _d(np.atleast_1d(values))
+        _call("a5g_set_instance_values", self.h, C.c_int(what), C.c_int(len(v)), p)
+
+    def set_active_instances(self, active=None):
+        """Restrict the tree-wide calls to the runs with active[g] (None: all)."""
+        if active is None:
+            _call("a5g_set_active_instances", self.h, C.c_int(0), None)
+            return
+        a, p = _i(np.asarray(active, np.int32))
+        _call("a5g_set_active_instances", self.h, C.c_int(len(a)), p)
+
     # --- output (afivo/src/m_aX_io.f90:20-209, m_vtk.f90) -------------------
     def leaf_cells(self, ixs_cc, ixs_fc=()):
         """a$D_write_vtk's cell data: array (n_cells, columns), leaves level

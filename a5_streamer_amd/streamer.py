@@ -393,8 +393,30 @@ class StreamerRun:
         T.gc_tree(I_FLD, RB_INTERP, BC_NEUMANN)
 
     # --- set_ref_flags (src/streamer_2d.f90:177-216) ------------------------
+    def box_flag(self, time, summary, dr, nc, rmin):
+        """The refinement flag of one box from its maxima (dr^2 max|rhs|,
+        max |E|), its size and position."""
+        c, v = self.c, self.v
+        crv_phi, max_fld = summary[0], summary[1]
+        adx = dr * self.tbl.col(T_ALPHA, max_fld)
+        f = 0
+        if adx < 0.1 and dr < 2.0e-5:
+            f = -1
+        if adx < 0.1 and crv_phi < 4.0 and dr < 5.0e-5:
+            f = -1
+        if time < v["init_window"]:
+            boxlen = nc * dr
+            centre = [rmin[k] + 0.5 * nc * dr for k in range(self.dim)]
+            for k in range(len(c.seed_dens)):
+                dist = dist_line(centre, c.seed_r0[k], c.seed_r1[k])
+                if dist - c.seed_width[k] < boxlen and dr > 0.2 * c.seed_width[k]:
+                    f = 1
+        if adx > 1.0 and crv_phi > v["crv_refine"]:
+            f = 1
+        return f
+
     def ref_flags(self):
-        T, c, v = self.T, self.c, self.v
+        T = self.T
         b = T.boxes()
         s = T.box_summaries()
         n = len(b["lvl"])
@@ -402,25 +424,8 @@ class StreamerRun:
         for i in range(n):
             if not b["in_use"][i] or b["lvl"][i] < 1:
                 continue
-            crv_phi, max_fld = s[i, 0], s[i, 1]
-            dr = b["dr"][i]
-            adx = dr * self.tbl.col(T_ALPHA, max_fld)
-            f = 0
-            if adx < 0.1 and dr < 2.0e-5:
-                f = -1
-            if adx < 0.1 and crv_phi < 4.0 and dr < 5.0e-5:
-                f = -1
-            if self.time < v["init_window"]:
-                nc = int(b["n_cell"][i])
-                boxlen = nc * dr
-                centre = [b["r_min"][i][k] + 0.5 * nc * dr for k in range(self.dim)]
-                for k in range(len(c.seed_dens)):
-                    dist = dist_line(centre, c.seed_r0[k], c.seed_r1[k])
-                    if dist - c.seed_width[k] < boxlen and dr > 0.2 * c.seed_width[k]:
-                        f = 1
-            if adx > 1.0 and crv_phi > v["crv_refine"]:
-                f = 1
-            flags[i] = f
+            flags[i] = self.box_flag(self.time, s[i], b["dr"][i], int(b["n_cell"][i]),
+                                     b["r_min"][i])
         return flags
 
     def adjust(self):
@@ -625,3 +630,291 @@ class StreamerRun:
         info = self.T.info()
         n = sum(len(self.T.level(l)[1]) for l in range(1, info["max_lvl"] + 1))
         return n * self.c.box_size ** self.dim
+
+
+class BatchRun:
+    """Many independent streamer runs advanced together in one tree
+    (a5g_set_instances): run g owns the base boxes stacked along the last
+    axis in [g H, (g + 1) H), H the height of one run's domain, with no
+    neighbour links between runs. Every library call then treats all runs
+    in one launch, and the time step and applied voltage are taken per run.
+    The state of each run is the state StreamerRun reaches alone:
+
+    - the host evaluates initial conditions and refinement flags in each
+      run's own coordinates, recomputed the way a tree holding only that run
+      computes them (base position from its own ix, then the children's
+      offsets), so every double is the same;
+    - a run whose initial refinement has settled is left out of further
+      initial iterations (a5g_set_active_instances) and keeps its mesh;
+    - after a mesh update the field is solved again only for the runs whose
+      mesh changed.
+
+    The runs share the variant, domain, box size, base, level limit,
+    transport data and steps per mesh update; field, seeds, background and
+    dt_max may differ. Photoionization, a dielectric and the 3D driver's
+    stop on removal are not supported here."""
+
+    def __init__(self, tree_cls, cfgs):
+        c0 = cfgs[0]
+        for c in cfgs:
+            for k in ("variant", "domain_len", "box_size", "n_base", "lvls_max",
+                      "steps_amr", "table_size", "table_max_fld"):
+                if getattr(c, k) != getattr(c0, k):
+                    raise ValueError("runs of a batch must share %s" % k)
+            if c.photoi or c.epsilon_diel != 1.0:
+                raise ValueError("a batch has no photoionization and no dielectric")
+        self.v = v = VARIANTS[c0.variant]
+        if v["exit_on_rm"]:
+            raise ValueError("the %s driver is not batched" % c0.variant)
+        self.cfgs = cfgs
+        self.K = K = len(cfgs)
+        self.dim = D = v["dim"]
+        shape = [c0.n_base] * D if np.isscalar(c0.n_base) else list(c0.n_base)
+        self.n_last = shape[-1]
+        self.dr_base = c0.domain_len / (c0.box_size * shape[0])
+        self.H = self.n_last * c0.box_size * self.dr_base
+        self.T = T = tree_cls(D, c0.box_size, N_VAR_CELL, N_VAR_FACE, self.dr_base, coarsen_to=2,
+                              lvls_max=c0.lvls_max, coord=v["coord"], n_boxes=c0.n_boxes * K)
+        ix1, nb1 = base_grid(D, c0.n_base)
+        n1 = len(ix1)
+        ix = np.concatenate([ix1 + np.array([0] * (D - 1) + [g * self.n_last]) for g in range(K)])
+        nb = np.concatenate([np.where(nb1 > 0, nb1 + g * n1, nb1) for g in range(K)])
+        T.set_base(ix.astype(np.int32), nb.astype(np.int32))
+        # one StreamerRun per run holds its configuration and scalar state;
+        # its tree is this shared one
+        self.runs = []
+        for c in cfgs:
+            r = StreamerRun.__new__(StreamerRun)
+            r.c, r.v, r.dim, r.T = c, v, D, T
+            r.time, r.dt, r.step_no = 0.0, 0.0, 0
+            r.voltage = -c.domain_len * c.applied_fld
+            self.runs.append(r)
+        T.table_create(0.0, c0.table_max_fld, c0.table_size, 4)
+        x, cols = c0.transport_columns()
+        for k, y in enumerate(cols, start=1):
+            T.table_set_col(k, x, y)
+        tbl = Table(*T.table_get())
+        for r in self.runs:
+            r.tbl = tbl
+        T.streamer_setup(dielectric_rule=v["dielectric_rule"], pho=-1)
+        kinds = [BC_COPY] * (2 * D)
+        kinds[-2], kinds[-1] = BC_NEG, BC_2V
+        T.mg_setup(I_PHI, I_FLD, I_RHS, i_eps=I_EPS, n_base=v["n_cycle_base"],
+                   bc_kind=np.array(kinds, np.int32), bc_val=np.zeros(2 * D), auto_ops=True)
+        T.set_instances(K, self.H)
+        self.voltages = np.array([r.voltage for r in self.runs])
+        T.set_instance_values(2 * D, self.voltages)
+        self.active = np.ones(K, bool)
+
+    # --- geometry -------------------------------------------------------------
+    def _boxes(self):
+        """boxes() plus each box's run and its r_min in the run's own
+        coordinates (last axis recomputed as a one-run tree computes it:
+        from the base box's own ix, then the children's offsets)."""
+        b = self.T.boxes()
+        D = self.dim
+        use = b["in_use"] != 0
+        g = np.floor((b["r_min"][:, D - 1] + 0.25 * b["dr"]) / self.H).astype(np.int64)
+        g = np.clip(g, 0, self.K - 1)
+        loc = b["r_min"].copy()
+        lv = b["lvl"]
+        for lvl in range(int(lv[use].min()) if use.any() else 1, int(lv[use].max()) + 1 if use.any() else 1):
+            sel = np.nonzero(use & (lv == lvl))[0]
+            if lvl <= 1:
+                ixl = b["ix"][sel, D - 1].astype(np.int64) - g[sel] * self.n_last
+                loc[sel, D - 1] = 0.0 + (ixl - 1) * self.dr_base * self.cfgs[0].box_size
+            else:
+                p = b["parent"][sel] - 1
+                dix = (b["ix"][sel, D - 1] - 1) & 1
+                loc[sel, D - 1] = loc[p, D - 1] + 0.5 * b["dr"][p] * dix * b["n_cell"][p]
+        b["run"], b["r_loc"] = g, loc
+        self.inst_lvl = np.zeros(self.K, np.int64)
+        np.maximum.at(self.inst_lvl, g[use], lv[use])
+        return b
+
+    def _set_active(self, mask):
+        self.active = np.asarray(mask, bool)
+        self.T.set_active_instances(self.active.astype(np.int32))
+
+    # --- the driver steps, per run where they differ ------------------------
+    def set_init_cond(self, b):
+        """set_init_cond on the boxes of active runs: densities and eps are
+        uploaded for every box (a settled run gets the values it holds), phi
+        is cleared on the active runs' boxes only."""
+        T = self.T
+        n = len(b["lvl"])
+        bl = T.box_len(T.n_cell)
+        elec_all, eps_all = np.zeros((n, bl)), np.zeros((n, bl))
+        clear = []
+        for i in range(n):
+            if not b["in_use"][i]:
+                continue
+            r = self.runs[b["run"][i]]
+            bb = {"n_cell": b["n_cell"], "dr": b["dr"], "r_min": b["r_loc"]}
+            elec, eps = r._box_fields(bb, i, True)
+            elec_all[i, :len(elec)] = elec
+            eps_all[i, :len(eps)] = eps
+            if self.active[b["run"][i]]:
+                clear.append(i + 1)
+        T.set_cc_all(I_ELEC, elec_all)
+        T.set_cc_all(I_PION, elec_all)
+        T.set_cc_all(I_EPS, eps_all)
+        if clear:
+            T.fill_cc(np.array(clear, np.int32), I_PHI, 0.0)
+
+    def compute_fld(self, no_guess):
+        """compute_fld of the active runs. The FMG of a run goes up to its own
+        finest level, so it is called once per group of runs with the same
+        finest level, that group active."""
+        T, D = self.T, self.dim
+        T.space_charge(ELEC / EPS0)
+        volt = np.array([-r.c.domain_len * r.c.get_fld(r.time) for r in self.runs])
+        if np.any(volt != self.voltages):
+            self.voltages = volt
+            T.set_instance_values(2 * D, volt)
+        for r, vg in zip(self.runs, volt):
+            r.voltage = vg
+        active = self.active.copy()
+        levels = sorted(set(self.inst_lvl[active].tolist()))
+        for m in levels:
+            if len(levels) > 1:
+                self._set_active(active & (self.inst_lvl == m))
+            for i in range(N_FMG_CYCLES):
+                T.fas_fmg(False, no_guess and i == 0)
+        if len(levels) > 1:
+            self._set_active(active)
+        T.fld_from_pot()
+        T.gc_tree(I_FLD, RB_INTERP, BC_NEUMANN)
+
+    def adjust(self, b):
+        """Refinement flags of the active runs (the others keep their mesh);
+        returns the boxes added and removed per run."""
+        T = self.T
+        s = T.box_summaries()
+        n = len(b["lvl"])
+        flags = np.zeros(n, np.int32)
+        for i in range(n):
+            g = b["run"][i]
+            if not b["in_use"][i] or b["lvl"][i] < 1 or not self.active[g]:
+                continue
+            flags[i] = self.runs[g].box_flag(self.runs[g].time, s[i], b["dr"][i],
+                                             int(b["n_cell"][i]), b["r_loc"][i])
+        T.adjust(flags)
+        add_k, rm_k = np.zeros(self.K, int), np.zeros(self.K, int)
+        nb = T.boxes()
+        added = []
+        for lvl in range(1, T.info()["max_lvl"] + 1):
+            add, rm = T.ref_info_lvl(lvl)
+            for i in add:
+                gi = int(np.floor((nb["r_min"][i - 1][self.dim - 1] + 0.25 * nb["dr"][i - 1])
+                                  / self.H))
+                add_k[min(max(gi, 0), self.K - 1)] += 1
+            for i in rm:
+                rm_k[b["run"][i - 1]] += 1
+            added.append((lvl, add))
+        return add_k, rm_k, added
+
+    def max_dt(self, b):
+        """get_max_dt per run from the per-box maxima of its leaves."""
+        s = self.T.box_summaries()
+        dts = np.zeros(self.K)
+        for g, r in enumerate(self.runs):
+            sel = (b["run"] == g) & (b["in_use"] != 0) & (b["lvl"] >= 1)
+            leaf = sel & (b["children"][:, 0] == 0)
+            max_fld = s[leaf, 1].max()
+            dr_min = self.dr_base * 0.5 ** (int(b["lvl"][sel].max()) - 1)
+            t = r.tbl
+            mobility = t.col(T_MOB, max_fld)
+            diff_coeff = t.col(T_DIF, max_fld)
+            alpha = t.col(T_ALPHA, max_fld)
+            dt_cfl = dr_min / (mobility * max_fld)
+            dt_dif = 0.25 * (dr_min * dr_min) / diff_coeff
+            dt_alpha = 1 / max(mobility * max_fld * alpha, EPS_F64)
+            dts[g] = 0.5 * min(1 / (1 / dt_cfl + 1 / dt_dif), dt_alpha, r.c.dt_max)
+        return dts
+
+    def initialise(self, max_iter=50):
+        """The initial loop of every run (src/streamer_2d.f90:57-66); a run
+        leaves it when its refinement adds no box."""
+        settled = np.zeros(self.K, bool)
+        for _ in range(max_iter):
+            self._set_active(~settled)
+            b = self._boxes()
+            self.set_init_cond(b)
+            self.compute_fld(True)
+            add_k, _, _ = self.adjust(b)
+            self._boxes()
+            settled |= self.active & (add_k == 0)
+            if settled.all():
+                self._set_active(np.ones(self.K, bool))
+                return
+        raise RuntimeError("initial refinement did not settle")
+
+    def prolong_to_new_boxes(self, added):
+        T = self.T
+        for lvl, add in added:
+            if len(add) == 0:
+                continue
+            for iv in (I_ELEC, I_PION, I_PHI):
+                T.prolong1_to(add, iv)
+            T.fill_cc(add, I_EPS, 1.0)
+            T.gc_boxes(add, I_ELEC, RB_INTERP_LIM, BC_NEUMANN)
+            T.gc_boxes(add, I_PION, RB_INTERP_LIM, BC_NEUMANN)
+            T.gc_boxes(add, I_PHI, RB_MG, BC_MG)
+
+    def step_block(self):
+        """One block for every run: its own dt, steps_amr steps, the mesh
+        update. Returns (added, removed) boxes per run."""
+        T = self.T
+        b = self._boxes()
+        dts = self.max_dt(b)
+        if np.any(dts < 1e-14):
+            raise RuntimeError("dt getting too small, instability?")
+        for r, dt in zip(self.runs, dts):
+            r.dt = dt
+        T.set_instance_values(0, dts)
+        cyl_time = self.v["time_in_substep"]
+        for _ in range(self.cfgs[0].steps_amr):
+            if not cyl_time:
+                for r in self.runs:
+                    r.time = r.time + r.dt
+            T.tree_copy_cc(I_ELEC, I_ELEC_OLD)
+            T.tree_copy_cc(I_PION, I_PION_OLD)
+            for i in (1, 2):
+                if cyl_time:
+                    for r in self.runs:
+                        r.time = r.time + r.dt
+                T.fluxes_koren(1.0)
+                T.consistent_fluxes([F_ELEC])
+                T.update_solution(1.0)
+                T.restrict_tree(I_ELEC)
+                T.restrict_tree(I_PION)
+                T.gc_tree(I_ELEC, RB_INTERP_LIM, BC_NEUMANN)
+                T.gc_tree(I_PION, RB_INTERP_LIM, BC_NEUMANN)
+                if i == 1:
+                    self.compute_fld(False)
+            if cyl_time:
+                for r in self.runs:
+                    r.time = r.time - r.dt
+            T.average_dens()
+            self.compute_fld(False)
+            for r in self.runs:
+                r.step_no += 1
+        b = self._boxes()
+        add_k, rm_k, added = self.adjust(b)
+        changed = (add_k > 0) | (rm_k > 0)
+        if changed.any():
+            self.prolong_to_new_boxes(added)
+            self._boxes()
+            self._set_active(changed)
+            self.compute_fld(False)
+            self._set_active(np.ones(self.K, bool))
+            T.tidy_up(0.9, 0.25, 5000, False)
+        return add_k, rm_k
+
+    def leaf_cells(self):
+        """Leaf cells per run."""
+        b = self._boxes()
+        leaf = (b["in_use"] != 0) & (b["lvl"] >= 1) & (b["children"][:, 0] == 0)
+        nc = self.cfgs[0].box_size ** self.dim
+        return np.array([int(np.sum(leaf & (b["run"] == g))) * nc for g in range(self.K)])

@@ -206,6 +206,29 @@ int a5g_tree_maxdiff_cc(a5g_tree *t, int iv_a, int iv_b, double *maxdiff,
 int a5g_set_gaussians(a5g_tree *t, int iv, int n_blob, const double *blobs,
                       double base);
 
+/* ---- independent runs in one tree ------------------------------------ */
+
+/* Instances: n_inst independent runs in one tree, the base boxes of run g
+ * placed along the last dimension in [g * stride, (g + 1) * stride) with no
+ * neighbour links between runs (physical boundaries). Every kernel then
+ * treats all runs in one launch; the time step, the Koren clamp factor and
+ * the boundary values of the rule 2V - c are taken per run
+ * (a5g_set_instance_values). A box belongs to run
+ * floor((r_min_last + dr / 4) / stride). n_inst = 0 returns to one run.
+ * No reference counterpart: the reference runs one streamer per process;
+ * this is the batched form of the cylindrical runs. */
+int a5g_set_instances(a5g_tree *t, int n_inst, double stride);
+/* Per-run values, one per instance: what = 0 the time step (the argument dt
+ * of a5g_fluxes_koren and a5g_update_solution, which is then ignored),
+ * what = nb in 1..2*dim the value V of the rule 2V - c on face nb (instead
+ * of the value a5g_mg_setup / a5g_mg_set_bc_value gave). */
+int a5g_set_instance_values(a5g_tree *t, int what, int n, const double *values);
+/* Restrict every following call that works on the tree's level lists
+ * (multigrid, ghost cells, restriction, fluxes, update, reductions) to the
+ * runs with active[g] != 0; active NULL makes all runs active. Calls given
+ * explicit box lists are not affected. */
+int a5g_set_active_instances(a5g_tree *t, int n, const int *active);
+
 /* ---- streamer diagnostics -------------------------------------------- */
 
 /* a$D_get_loc (afivo/src/m_aX_utils.f90:244-281): the first level-1 box in
