@@ -84,6 +84,46 @@ def dist_line(r, r0, r1):
     return norm2([r[k] - proj[k] for k in range(d)])
 
 
+def norm2_rows(v):
+    """norm2 of each row of v (n, d), entry by entry in the same order and
+    with the same operations as norm2, so each result is the same double."""
+    v = np.asarray(v, float)
+    mx = np.zeros(v.shape[0])
+    s = np.zeros(v.shape[0])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for k in range(v.shape[1]):
+            ax = np.abs(v[:, k])
+            first = mx == 0
+            up = ~first & (ax > mx)
+            rest = ~first & ~up
+            q = mx / ax
+            qq = q * q
+            p = ax / mx
+            s = np.where(up, s * qq + qq, np.where(rest, s + p * p, s))
+            mx = np.where(first | up, ax, mx)
+    return mx * np.sqrt(1 + s)
+
+
+def dist_line_rows(r, r0, r1):
+    """dist_line of each row of r (n, d) to the segment r0-r1, with the
+    operations of dist_line."""
+    r = np.asarray(r, float)
+    d = r.shape[1]
+    line_len2 = 0.0
+    for k in range(d):
+        t = r1[k] - r0[k]
+        line_len2 = line_len2 + t * t
+    temp = np.zeros(r.shape[0])
+    for k in range(d):
+        temp = temp + (r[:, k] - r0[k]) * (r1[k] - r0[k])
+    to0 = norm2_rows(np.stack([r[:, k] - r0[k] for k in range(d)], axis=1))
+    to1 = norm2_rows(np.stack([r[:, k] - r1[k] for k in range(d)], axis=1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        proj = [r0[k] + temp / line_len2 * (r1[k] - r0[k]) for k in range(d)]
+    top = norm2_rows(np.stack([r[:, k] - proj[k] for k in range(d)], axis=1))
+    return np.where(temp <= 0.0, to0, np.where(temp >= line_len2, to1, top))
+
+
 _LOG_HALF_HUGE = math.log(0.5 * 1.7976931348623157e308)
 
 
@@ -128,6 +168,17 @@ class Table:
             li, lf = 1, 1.0
         elif li >= self.n_rows:
             li, lf = self.n_rows - 1, 0.0
+        t = self.rc[c - 1]
+        return lf * t[li - 1] + (1 - lf) * t[li]
+
+    def col_rows(self, c, x):
+        """col for an array of x, with the same operations."""
+        frac = (np.asarray(x, float) - self.x_min) * self.inv_dx
+        li = np.ceil(frac)
+        lf = li - frac
+        low, high = li < 1, li >= self.n_rows
+        li = np.where(low, 1, np.where(high, self.n_rows - 1, li)).astype(np.int64)
+        lf = np.where(low, 1.0, np.where(high, 0.0, lf))
         t = self.rc[c - 1]
         return lf * t[li - 1] + (1 - lf) * t[li]
 
@@ -791,15 +842,7 @@ class BatchRun:
         returns the boxes added and removed per run."""
         T = self.T
         s = T.box_summaries()
-        n = len(b["lvl"])
-        flags = np.zeros(n, np.int32)
-        for i in range(n):
-            g = b["run"][i]
-            if not b["in_use"][i] or b["lvl"][i] < 1 or not self.active[g]:
-                continue
-            flags[i] = self.runs[g].box_flag(self.runs[g].time, s[i], b["dr"][i],
-                                             int(b["n_cell"][i]), b["r_loc"][i])
-        T.adjust(flags)
+        T.adjust(self.flags(b, s))
         add_k, rm_k = np.zeros(self.K, int), np.zeros(self.K, int)
         nb = T.boxes()
         added = []
@@ -814,15 +857,52 @@ class BatchRun:
             added.append((lvl, add))
         return add_k, rm_k, added
 
+    def flags(self, b, s):
+        """set_ref_flags of every box of the active runs (StreamerRun.box_flag
+        with array operations, each result the same), keep for the others."""
+        c0, v = self.cfgs[0], self.v
+        g = b["run"]
+        ok = (b["in_use"] != 0) & (b["lvl"] >= 1) & self.active[g]
+        n_seed = {len(c.seed_dens) for c in self.cfgs}
+        if len(n_seed) != 1:
+            raise ValueError("runs of a batch must have the same number of seeds")
+        dr, nc = b["dr"], b["n_cell"]
+        crv_phi, max_fld = s[:, 0], s[:, 1]
+        adx = dr * self.runs[0].tbl.col_rows(T_ALPHA, max_fld)
+        f = np.zeros(len(dr), np.int32)
+        f[(adx < 0.1) & (dr < 2.0e-5)] = -1
+        f[(adx < 0.1) & (crv_phi < 4.0) & (dr < 5.0e-5)] = -1
+        times = np.array([r.time for r in self.runs])
+        early = ok & (times[g] < v["init_window"])
+        if early.any():
+            idx = np.nonzero(early)[0]
+            boxlen = nc[idx] * dr[idx]
+            centre = np.stack([b["r_loc"][idx, k] + 0.5 * nc[idx] * dr[idx]
+                               for k in range(self.dim)], axis=1)
+            for k in range(n_seed.pop()):
+                for gi in np.unique(g[idx]):
+                    c = self.cfgs[gi]
+                    m = g[idx] == gi
+                    dist = dist_line_rows(centre[m], c.seed_r0[k], c.seed_r1[k])
+                    hit = (dist - c.seed_width[k] < boxlen[m]) & (dr[idx][m] > 0.2 * c.seed_width[k])
+                    f[idx[m][hit]] = 1
+        f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
+        f[~ok] = 0
+        return f
+
     def max_dt(self, b):
         """get_max_dt per run from the per-box maxima of its leaves."""
         s = self.T.box_summaries()
         dts = np.zeros(self.K)
+        sel = (b["in_use"] != 0) & (b["lvl"] >= 1)
+        leaf = sel & (b["children"][:, 0] == 0)
+        fld_k = np.full(self.K, -np.inf)
+        np.maximum.at(fld_k, b["run"][leaf], s[leaf, 1])
+        lvl_k = np.zeros(self.K, np.int64)
+        np.maximum.at(lvl_k, b["run"][sel], b["lvl"][sel])
         for g, r in enumerate(self.runs):
-            sel = (b["run"] == g) & (b["in_use"] != 0) & (b["lvl"] >= 1)
-            leaf = sel & (b["children"][:, 0] == 0)
-            max_fld = s[leaf, 1].max()
-            dr_min = self.dr_base * 0.5 ** (int(b["lvl"][sel].max()) - 1)
+            max_fld = float(fld_k[g])
+            dr_min = self.dr_base * 0.5 ** (int(lvl_k[g]) - 1)
             t = r.tbl
             mobility = t.col(T_MOB, max_fld)
             diff_coeff = t.col(T_DIF, max_fld)

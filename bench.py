@@ -280,6 +280,92 @@ def streamer_step(args):
     T.close()
 
 
+def splitmix64(seed):
+    """SplitMix64 (Steele, Lea and Flood 2014), as uniform doubles in [0, 1)."""
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield (z ^ (z >> 31)) / 2.0 ** 64
+
+
+def batch_configs(n, lvls_max):
+    """The batched cylindrical workload: n runs of 16 x 32 mm (1 x 2 base
+    boxes of 8 x 8 cells), run k drawn from SplitMix64 seeded 20261203 + k:
+    applied field in [1.5, 2.5] MV/m, seed width in [0.2, 0.8] mm, background
+    in [1e14, 1e16] m^-3 (log-uniform), the seed on the axis at z = 24 mm."""
+    from a5_streamer_amd import streamer as ST
+    cfgs = []
+    for k in range(n):
+        g = splitmix64(20261203 + k)
+        fld, w, bg = 1.5e6 + 1.0e6 * next(g), 0.2e-3 + 0.6e-3 * next(g), 10 ** (14 + 2 * next(g))
+        cfgs.append(ST.Config(variant="cyl", domain_len=16e-3, n_base=(1, 2),
+                              lvls_max=lvls_max, applied_fld=fld, seed_width=[w], bg_dens=bg,
+                              seed_r0=[[0.0, 24e-3]], seed_r1=[[0.0, 24e-3]]))
+    return cfgs
+
+
+def batch_runs(args):
+    """Many cylindrical runs in one tree (BatchRun over a5g_set_instances)
+    against the same runs stepped one by one, each on its own tree. Prints
+    one JSON line: aggregate leaf Mcell-steps/s of both and their ratio."""
+    import a5_streamer_amd as A
+    from a5_streamer_amd import streamer as ST
+    cfgs = batch_configs(args.batch, args.batch_lvls)
+    blocks = max(1, args.steps // cfgs[0].steps_amr)
+    t0 = time.perf_counter()
+    B = ST.BatchRun(A.Tree, cfgs)
+    B.initialise()
+    t_init = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        B.step_block()
+    B.T.sync()
+    cells = []
+    t0 = time.perf_counter()
+    for _ in range(blocks):
+        cells.append(int(B.leaf_cells().sum()))
+        B.step_block()
+    B.T.sync()
+    dt_b = time.perf_counter() - t0
+    steps = blocks * cfgs[0].steps_amr
+    v_b = sum(cells) * cfgs[0].steps_amr / dt_b / 1e6
+    boxes = int(B.T.info()["max_id"])
+    B.T.close()
+    # the same runs one by one
+    dt_s, cells_s = 0.0, 0
+    for c in cfgs:
+        r = ST.StreamerRun(A.Tree, c)
+        r.initialise()
+        for _ in range(args.warmup):
+            r.step_block()
+        r.T.sync()
+        t0 = time.perf_counter()
+        for _ in range(blocks):
+            cells_s += r.leaf_cells() * c.steps_amr
+            r.step_block()
+        r.T.sync()
+        dt_s += time.perf_counter() - t0
+        r.T.close()
+    v_s = cells_s / dt_s / 1e6
+    line = {
+        "metric": "batched cylindrical runs, aggregate leaf Mcell-steps/s",
+        "value": round(v_b, 3), "unit": "Mcell-steps/s", "steps": steps,
+        "warmup_blocks": args.warmup, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d cylindrical runs of 16 x 32 mm, 1 x 2 base boxes of 8 x 8 "
+                               "cells, lvls_max %d, SplitMix64 seed 20261203 + k: field "
+                               "1.5-2.5 MV/m, seed width 0.2-0.8 mm, background 1e14-1e16"
+                               % (len(cfgs), args.batch_lvls),
+                   "boxes_in_batch_tree": boxes, "leaf_cells_start": cells[0],
+                   "init_s": round(t_init, 1)},
+        "ms_per_step": round(dt_b * 1e3 / steps, 3),
+        "one_by_one": {"value": round(v_s, 3), "ms_per_step_all_runs": round(dt_s * 1e3 / steps, 3)},
+        "speedup_vs_one_by_one": round(v_b / v_s, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +379,9 @@ def main():
     ap.add_argument("--step-lvls", type=int, default=7)
     ap.add_argument("--step-base", type=int, default=16)
     ap.add_argument("--no-photoi", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="measure this many batched cylindrical runs instead (own JSON line)")
+    ap.add_argument("--batch-lvls", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -300,6 +389,9 @@ def main():
         sys.exit(2)
     if args.streamer_step:
         streamer_step(args)
+        return
+    if args.batch:
+        batch_runs(args)
         return
 
     from a5_streamer_amd.workloads import OctreePoisson
