@@ -1760,17 +1760,27 @@ static uint64_t state_fingerprint(Pool& P) {
   ptr(P.d_lvl); ptr(P.d_ix); ptr(P.d_parent); ptr(P.d_children); ptr(P.d_nbs);
   ptr(P.d_ncell); ptr(P.d_tag); ptr(P.d_dr); ptr(P.d_rmin); ptr(P.d_lists);
   num(P.tree.coord);
-  for (int i = 0; i < 6; i++) {
-    num(P.bc_kind[i]);
-    uint64_t b;
-    std::memcpy(&b, &P.bc_val[i], 8);
-    num(b);
-  }
   num(P.n_inst); ptr(P.d_inst);
   {
     uint64_t sb;
     std::memcpy(&sb, &P.inst_stride, 8);
     num(sb);
+  }
+  return h;
+}
+
+// What else a captured call's arguments derive from and that switches back
+// and forth (the field solve and the photoionization terms, groups of
+// instances): the face rules, the multigrid options, and the counts and
+// offsets of the level lists. Graphs of several such states are kept.
+static uint64_t variant_fingerprint(Pool& P) {
+  uint64_t h = 1469598103934665603ull;
+  auto num = [&](uint64_t v) { h = fnv(h, v); };
+  for (int i = 0; i < 6; i++) {
+    num(P.bc_kind[i]);
+    uint64_t b;
+    std::memcpy(&b, &P.bc_val[i], 8);
+    num(b);
   }
   num(P.mg.i_phi); num(P.mg.i_tmp); num(P.mg.i_rhs); num(P.mg.i_eps);
   num(P.mg.n_down); num(P.mg.n_up); num(P.mg.n_base); num(P.mg.auto_ops);
@@ -1779,13 +1789,6 @@ static uint64_t state_fingerprint(Pool& P) {
     std::memcpy(&hb, &P.mg.helm, 8);
     num(hb);
   }
-  return h;
-}
-
-// The level lists a captured call reads: counts and offsets per level
-static uint64_t lists_fingerprint(Pool& P) {
-  uint64_t h = 1469598103934665603ull;
-  auto num = [&](uint64_t v) { h = fnv(h, v); };
   num((uint64_t)(int64_t)P.tree.min_lvl); num(P.tree.max_lvl);
   for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++) {
     const int k = l - P.tree.min_lvl;
@@ -1809,7 +1812,7 @@ static void run_captured(Pool& P, uint64_t key, F&& f) {
     P.clear_graphs();
     P.graphs_state = st;
   }
-  key = fnv(lists_fingerprint(P), key);
+  key = fnv(variant_fingerprint(P), key);
   for (auto& g : P.graphs)
     if (g.first == key) {
       check(hipGraphLaunch(g.second, P.stream), "graph launch");
@@ -2233,16 +2236,23 @@ void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const
       if (j > 0) axpy_leaves(P, i_rhs, i_rhs, amp[j] / amp[j - 1], false);
       P.mg.helm = kappa[j] * kappa[j];
       const Ctx c = ctx64(P);
-      if (P.tree.dim == 2) fas_fmg<2, double>(P, c, false, true);
-      else fas_fmg<3, double>(P, c, false, true);
+      // captured like the field solve's calls; the Helmholtz coefficient
+      // keeps each term's graphs apart
+      run_captured(P, call_key(1, 0, 1, 0), [&] {
+        if (P.tree.dim == 2) fas_fmg<2, double>(P, c, false, true);
+        else fas_fmg<3, double>(P, c, false, true);
+      });
+      const int mx = P.mg_max_lvl();
       for (int cyc = 0;; cyc++) {
         double res, rhs;
         mg_residual_norms(P, &res, &rhs);
         if (res <= tol * rhs) break;
         if (cyc >= max_cycles)
           throw std::runtime_error("photoionization: Helmholtz solve did not converge");
-        if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, P.tree.max_lvl, false);
-        else fas_vcycle<3, double>(P, c, P.tree.max_lvl, false);
+        run_captured(P, call_key(2, mx - P.tree.min_lvl, 0, 0), [&] {
+          if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, mx, false);
+          else fas_vcycle<3, double>(P, c, mx, false);
+        });
         total++;
       }
       axpy_leaves(P, i_phi, v.pho, 1.0, true);
