@@ -9,6 +9,24 @@
  * fosito/m_lookup_table.f90). Only tests, the smoke check and the CPU
  * baseline leg of bench.py load it; the product never does.
  *
+ * How each part is pinned (the tests named run on the CPU):
+ *  - tree core, ghost cells, restriction, prolongation, multigrid, Koren
+ *    limiter, lookup table, NORM2, Morton keys, point location and the
+ *    located reduction: bit for bit against the reference compiled from its
+ *    own sources (oracle/_ref; tests/test_oracle_ref.py,
+ *    tests/test_oracle_diag.py);
+ *  - the streamer box procedures the reference cannot link (field,
+ *    fluxes, update, averaging, space charge, refinement maxima): closed
+ *    forms and conservation (tests/test_oracle_math.py,
+ *    tests/test_driver_host.py);
+ *  - the Helmholtz operator and photoionization (no reference
+ *    counterpart): smoother closed form, direct sparse solve, manufactured
+ *    solution and far field (tests/test_oracle_helmholtz.py);
+ *  - the axis walk and the output arrays: the reference's own a2_get_loc
+ *    and m_vtk, geometry and brute force (tests/test_oracle_diag.py,
+ *    tests/test_oracle_output.py).
+ * No function is left without such a check.
+ *
  * Conventions follow the reference: box ids are 1-based, 0 = no box,
  * negative neighbour ids are physical boundaries, variable indices are
  * 1-based. A box's cell-centred block is cc(0:nc+1, 0:nc+1[, 0:nc+1]) with
