@@ -250,7 +250,7 @@ void Pool::apply_moves(const std::vector<int>& moves, int n_used) {
   topo_dirty = true;
 }
 
-int Pool::prof_begin(int kind, int lvl, double bytes) {
+int Pool::prof_begin(int kind, int lvl, double bytes, int count) {
   if (!prof_on) return -1;
   auto take = [&]() {
     hipEvent_t e;
@@ -262,7 +262,7 @@ int Pool::prof_begin(int kind, int lvl, double bytes) {
     }
     return e;
   };
-  ProfRec r{kind, lvl, bytes, take(), take()};
+  ProfRec r{kind, lvl, bytes, take(), take(), count};
   hcheck(hipEventRecord(r.a, stream), "hipEventRecord");
   prof.push_back(r);
   return (int)prof.size() - 1;
@@ -1235,7 +1235,7 @@ int a5g_prof_read(a5g_tree* t, int kind, int lvl, int* n_launch, double* ms, dou
     if (r.kind != kind || (lvl != -999 && r.lvl != lvl)) continue;
     float m = 0;
     a5g::hcheck(hipEventElapsedTime(&m, r.a, r.b), "hipEventElapsedTime");
-    cnt++;
+    cnt += r.count;
     tot += m;
     by += r.bytes;
   }

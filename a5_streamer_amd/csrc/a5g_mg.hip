@@ -1427,10 +1427,14 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
   Geo g = geo(n, ncol);
   Dev a = dev_for(P, c);
   const double cells = (double)n * cells_of(D, nc, false);
+  // one event pair around the consecutive half-sweeps (an event between two
+  // launches holds the second back by a few microseconds); algorithmic bytes
+  // of a half-sweep: 1.5 words per cell
+  const int pr = n_cycle > 0
+                     ? P.prof_begin(PK_GSRB, lvl, 2 * n_cycle * 1.5 * sizeof(T) * cells, 2 * n_cycle)
+                     : -1;
   for (int it = 1; it <= 2 * n_cycle; it++) {
     const int fused = it > 1 || first_fused;
-    // algorithmic bytes of a half-sweep: 1.5 words per cell
-    const int pr = P.prof_begin(PK_GSRB, lvl, 1.5 * sizeof(T) * cells);
     const bool done =
         c.cyl ? launch_gsrb_lds<2, true, T>(P, a, P.ids(lvl), n, nc, it, fused, c.i_phi, c.i_rhs)
               : launch_gsrb_lds<D, false, T>(P, a, P.ids(lvl), n, nc, it, fused, c.i_phi,
@@ -1443,8 +1447,8 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
       hipLaunchKernelGGL((k_gsrb<D, false, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
                          P.ids(lvl), n, nc, it, fused, g.tpb, c.i_phi, c.i_rhs);
     post_launch("gsrb");
-    P.prof_end(pr);
   }
+  P.prof_end(pr);
   if (store || n_cycle == 0) fill_gc_phi<D, T>(P, c, lvl);
 }
 

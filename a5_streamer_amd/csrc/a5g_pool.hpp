@@ -61,8 +61,9 @@ enum ProfKind { PK_GSRB = 0, PK_GC, PK_RESID_RSTR, PK_COARSE_RHS, PK_CORR, PK_CO
 
 struct ProfRec {
   int kind, lvl;
-  double bytes;  // algorithmic bytes of the launch
+  double bytes;  // algorithmic bytes of the launches
   hipEvent_t a, b;
+  int count;     // launches between the two events
 };
 
 // Variable indices of the streamer models (src/m_streamer.f90:16-43): cell
@@ -133,7 +134,8 @@ struct Pool {
   bool prof_on = false;
   std::vector<ProfRec> prof;
   std::vector<hipEvent_t> ev_pool;
-  int prof_begin(int kind, int lvl, double bytes);
+  // events around `count` consecutive launches of one kernel class
+  int prof_begin(int kind, int lvl, double bytes, int count = 1);
   void prof_end(int idx);
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
   double bc_val[6] = {0, 0, 0, 0, 0, 0};

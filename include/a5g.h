@@ -309,7 +309,9 @@ int a5g_prof_enable(a5g_tree *t, int on);
  * restriction, 3 coarse rhs, 4 correction, 5 copy, 6 other multigrid
  * kernels, 7 space charge, 8 field from potential, 9 Koren fluxes,
  * 10 consistent fluxes, 11 density update, 12 averaging, 13 restriction of
- * the densities, 14 photoionization rate. */
+ * the densities, 14 photoionization rate. The consecutive half-sweeps of
+ * one smoothing call are timed by one event pair, every other launch by its
+ * own. */
 int a5g_prof_read(a5g_tree *t, int kind, int lvl, int *n_launch, double *ms,
                   double *bytes);
 
