@@ -14,9 +14,10 @@ One step = one full solve from phi = 0 to max|rhs - A phi| / max|rhs| <=
           correction: FMG + 3 V-cycles, later ones: 4 V-cycles); its
           potential is compared with the FP64 solution (max relative error
           over leaf cells).
-The headline value is the faster of the forms that meet the error gate of
-1e-8 relative potential error (FP64 always does; mixed when its error is at
-most 1e-8); both forms, their ratio and their rooflines are in the line.
+The headline value is FP64's unless the mixed form meets the error gate of
+1e-8 relative potential error and is faster by more than the run-to-run
+spread of these timings (about 1%; the rule takes 2%); both forms, their
+ratio and their rooflines are in the line.
 Throughput = leaf cells x solves / s. Prints one JSON line. Inputs are
 resident in HBM before timing starts.
 """
@@ -454,7 +455,7 @@ def main():
                         "speedup_vs_fp64": round(vm / v64, 3),
                         "roofline": roofline(T, "mixed", one),
                         "kernel_ms_per_step": {k: round(v[1], 3) for k, v in profm.items()}}
-        if err <= 1e-8 and vm > v64:
+        if err <= 1e-8 and vm > 1.02 * v64:
             head = "mixed"
     H = out[head]
     out["value"] = H["value"]
