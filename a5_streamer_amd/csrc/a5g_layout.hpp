@@ -18,20 +18,22 @@ namespace a5g {
 // second half. Within a colour half:
 //  * the nc^D/2 interior cells come first. Each interior row (j, k) holds
 //    nc/2 cells of a colour, one of them on the x boundary (i = 1 or nc).
-//    The boundary cells of all rows come first (slot = row), then the other
-//    nc/2 - 1 cells of each row in row order (isl below);
+//    The boundary cells come first: those at i = 1 (slot = row / 2), then
+//    those at i = nc (slot = nrow / 2 + row / 2); then the other nc/2 - 1
+//    cells of each row in row order (isl below);
 //  * the ghost cells follow, in lexicographic order of the ghost pairs (a row
 //    on the ghost layer holds (nc+2)/2 pairs, any other row one pair: its
 //    cells i = 0 and nc+1).
 // A red-black half-sweep then reads the other colour's interior and writes
 // its own as contiguous runs that share no cache line with a ghost, and the
-// face a neighbour reads is dense in every direction: the x faces are every
-// other entry of the boundary block.
+// face a neighbour reads is one dense run in every direction.
 template <int D>
 A5G_HD int isl(int nc, int i, int j, int k) {
   const int row = D == 2 ? j - 1 : (j - 1) + nc * (k - 1);
   const int nrow = D == 2 ? nc : nc * nc;
-  return (i == 1 || i == nc) ? row : nrow + row * ((nc >> 1) - 1) + ((i - 2) >> 1);
+  if (i == 1) return row >> 1;
+  if (i == nc) return (nrow >> 1) + (row >> 1);
+  return nrow + row * ((nc >> 1) - 1) + ((i - 2) >> 1);
 }
 
 template <int D>
@@ -68,19 +70,23 @@ A5G_HD int cidx_in(int nc, int i, int j, int k) {
 template <int D>
 A5G_HD void icell(int nc, int q, int col, int& i, int& j, int& k) {
   const int nrow = D == 2 ? nc : nc * nc;
-  const int w = (nc >> 1) - 1;  // cells of a colour in a row, boundary cell excluded
-  int row, m = -1;
   if (q < nrow) {
-    row = q;
-  } else {
-    row = (q - nrow) / w;
-    m = q - nrow - row * w;
+    // a boundary cell: rows 2p and 2p+1 share slot p of their side; the
+    // colour picks the row
+    const bool low = q < (nrow >> 1);
+    const int r2 = (low ? q : q - (nrow >> 1)) << 1;
+    k = D == 3 ? r2 / nc + 1 : 0;
+    i = low ? 1 : nc;
+    const int j0 = r2 % nc + 1;
+    j = ((i + j0 + k) & 1) == col ? j0 : j0 + 1;
+    return;
   }
+  const int w = (nc >> 1) - 1;  // cells of a colour in a row, boundary cell excluded
+  const int row = (q - nrow) / w, m = q - nrow - row * w;
   j = row % nc + 1;
   k = D == 3 ? row / nc + 1 : 0;
   const bool odd = ((1 + j + k) & 1) == col;  // the colour's cells have odd i
-  if (m < 0) i = odd ? 1 : nc;
-  else i = odd ? 3 + 2 * m : 2 + 2 * m;
+  i = odd ? 3 + 2 * m : 2 + 2 * m;
 }
 
 }  // namespace a5g

@@ -1,8 +1,10 @@
 // Host check of the box cell order (a5g_layout.hpp): for every box size the
 // library supports, cidx is a bijection of the (nc+2)^D block onto the slot,
 // colour c fills slots [c*half, (c+1)*half), its interior cells come first
-// with the x-boundary cell of each row at slot = row, icell inverts the
-// interior order, and cidx_in agrees with cidx on the interior.
+// with the x-boundary cells of the rows at slot = row / 2 (i = 1) and
+// nrow / 2 + row / 2 (i = nc), so each x face of a colour is one dense run,
+// icell inverts the interior order, and cidx_in agrees with cidx on the
+// interior.
 #include <cstdio>
 #include <vector>
 
@@ -27,7 +29,9 @@ static int check(int nc) {
           int ii, jj, kk;
           a5g::icell<D>(nc, q, col, ii, jj, kk);
           if (ii != i || jj != j || (D == 3 && kk != k)) bad++;
-          if ((i == 1 || i == nc) && q != row) bad++;
+          const int nrow = D == 2 ? nc : nc * nc;
+          if (i == 1 && q != row / 2) bad++;
+          if (i == nc && q != nrow / 2 + row / 2) bad++;
           if (a5g::cidx_in<D>(nc, i, j, k) != x) bad++;
         } else if (x - col * half < n_in) {
           bad++;
