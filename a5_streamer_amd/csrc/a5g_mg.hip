@@ -1110,8 +1110,24 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
   T* q = sh + slot * RB;
   int off[3] = {0, 0, 0};
   int pid = 0, pnc = 0;
-  // child cells in slot order (each colour's interior a contiguous run);
-  // their potential is loaded with the staging loads
+  // child cells in slot order (each colour's interior a contiguous run),
+  // except that the x-boundary cells go in row order: neighbouring lanes
+  // then read parent values from different LDS banks. Their potential is
+  // loaded with the staging loads.
+  constexpr int NROW = D == 2 ? NC : NC * NC;
+  auto cell = [&](int t, int& i, int& j, int& k) {
+    const int col = t >= NCOL;
+    int qs = t - col * NCOL;
+    if (qs < NROW) {
+      j = qs % NC + 1;
+      k = D == 3 ? qs / NC + 1 : 0;
+      i = ((1 + j + k) & 1) == col ? 1 : NC;
+      qs = isl<D>(NC, i, j, k);
+    } else {
+      icell<D>(NC, qs, col, i, j, k);
+    }
+    return col * HALF + qs;
+  };
   T fv[NIT];
   T* f = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   if (act) {
@@ -1121,7 +1137,8 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
 #pragma unroll
     for (int u = 0; u < NIT; u++) {
       const int t = lt + u * TPB;
-      if (t < TOTAL) fv[u] = f[(t >= NCOL ? HALF - NCOL : 0) + t];
+      int i, j, k;
+      if (t < TOTAL) fv[u] = f[cell(t, i, j, k)];
     }
     const T* c = boxp_t<T>(a, i_tmp, pid);
     for (int t = lt; t < RB; t += TPB) {
@@ -1137,14 +1154,12 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
   for (int u = 0; u < NIT; u++) {
     const int t = lt + u * TPB;
     if (t >= TOTAL) break;
-    const int col = t >= NCOL;
     int i, j, k;
-    icell<D>(NC, t - col * NCOL, col, i, j, k);
+    const int x = cell(t, i, j, k);
     // local parent indices: (i+1)/2 is the closest coarse cell, the other
     // one lies on the side of the fine cell
     const int i1 = (i + 1) >> 1, i2 = i1 + 1 - 2 * (i & 1);
     const int j1 = (j + 1) >> 1, j2 = j1 + 1 - 2 * (j & 1);
-    const int x = col * HALF + (t - col * NCOL);
     if (ep != nullptr) {
       if (D == 2) {
         const int p0 = cidx<D>(pnc, off[0] + i1, off[1] + j1, 0);
