@@ -124,6 +124,26 @@ def dist_line_rows(r, r0, r1):
     return np.where(temp <= 0.0, to0, np.where(temp >= line_len2, to1, top))
 
 
+def dist_line_rows_per(r, r0, r1):
+    """dist_line of each row of r (n, d) to its own segment r0[row]-r1[row],
+    with the operations of dist_line."""
+    r, r0, r1 = (np.asarray(x, float) for x in (r, r0, r1))
+    d = r.shape[1]
+    line_len2 = np.zeros(r.shape[0])
+    for k in range(d):
+        t = r1[:, k] - r0[:, k]
+        line_len2 = line_len2 + t * t
+    temp = np.zeros(r.shape[0])
+    for k in range(d):
+        temp = temp + (r[:, k] - r0[:, k]) * (r1[:, k] - r0[:, k])
+    to0 = norm2_rows(np.stack([r[:, k] - r0[:, k] for k in range(d)], axis=1))
+    to1 = norm2_rows(np.stack([r[:, k] - r1[:, k] for k in range(d)], axis=1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        proj = [r0[:, k] + temp / line_len2 * (r1[:, k] - r0[:, k]) for k in range(d)]
+    top = norm2_rows(np.stack([r[:, k] - proj[k] for k in range(d)], axis=1))
+    return np.where(temp <= 0.0, to0, np.where(temp >= line_len2, to1, top))
+
+
 _LOG_HALF_HUGE = math.log(0.5 * 1.7976931348623157e308)
 
 
@@ -466,18 +486,35 @@ class StreamerRun:
             f = 1
         return f
 
+    def flags_rows(self, time, s, dr, nc, rmin, ok):
+        """box_flag of many boxes of this run at once, with array forms of
+        the same operations (each flag the same): s the per-box maxima, dr,
+        nc, rmin (n, dim) the boxes, ok the boxes to flag (others keep)."""
+        c, v = self.c, self.v
+        crv_phi, max_fld = s[:, 0], s[:, 1]
+        adx = dr * self.tbl.col_rows(T_ALPHA, max_fld)
+        f = np.zeros(len(dr), np.int32)
+        f[(adx < 0.1) & (dr < 2.0e-5)] = -1
+        f[(adx < 0.1) & (crv_phi < 4.0) & (dr < 5.0e-5)] = -1
+        if time < v["init_window"] and ok.any():
+            idx = np.nonzero(ok)[0]
+            boxlen = nc[idx] * dr[idx]
+            centre = np.stack([rmin[idx, k] + 0.5 * nc[idx] * dr[idx] for k in range(self.dim)],
+                              axis=1)
+            for k in range(len(c.seed_dens)):
+                dist = dist_line_rows(centre, c.seed_r0[k], c.seed_r1[k])
+                hit = (dist - c.seed_width[k] < boxlen) & (dr[idx] > 0.2 * c.seed_width[k])
+                f[idx[hit]] = 1
+        f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
+        f[~ok] = 0
+        return f
+
     def ref_flags(self):
         T = self.T
         b = T.boxes()
         s = T.box_summaries()
-        n = len(b["lvl"])
-        flags = np.zeros(n, np.int32)
-        for i in range(n):
-            if not b["in_use"][i] or b["lvl"][i] < 1:
-                continue
-            flags[i] = self.box_flag(self.time, s[i], b["dr"][i], int(b["n_cell"][i]),
-                                     b["r_min"][i])
-        return flags
+        ok = (b["in_use"] != 0) & (b["lvl"] >= 1)
+        return self.flags_rows(self.time, s, b["dr"], b["n_cell"], b["r_min"], ok)
 
     def adjust(self):
         return self.T.adjust(self.ref_flags())
@@ -859,7 +896,8 @@ class BatchRun:
 
     def flags(self, b, s):
         """set_ref_flags of every box of the active runs (StreamerRun.box_flag
-        with array operations, each result the same), keep for the others."""
+        with array operations over all runs at once, each flag the same), keep
+        for the others."""
         c0, v = self.cfgs[0], self.v
         g = b["run"]
         ok = (b["in_use"] != 0) & (b["lvl"] >= 1) & self.active[g]
@@ -879,13 +917,15 @@ class BatchRun:
             boxlen = nc[idx] * dr[idx]
             centre = np.stack([b["r_loc"][idx, k] + 0.5 * nc[idx] * dr[idx]
                                for k in range(self.dim)], axis=1)
+            gi_of = g[idx]
             for k in range(n_seed.pop()):
-                for gi in np.unique(g[idx]):
-                    c = self.cfgs[gi]
-                    m = g[idx] == gi
-                    dist = dist_line_rows(centre[m], c.seed_r0[k], c.seed_r1[k])
-                    hit = (dist - c.seed_width[k] < boxlen[m]) & (dr[idx][m] > 0.2 * c.seed_width[k])
-                    f[idx[m][hit]] = 1
+                # seed k of each box's run, gathered per box
+                r0 = np.array([c.seed_r0[k] for c in self.cfgs], float)[gi_of]
+                r1 = np.array([c.seed_r1[k] for c in self.cfgs], float)[gi_of]
+                w = np.array([c.seed_width[k] for c in self.cfgs], float)[gi_of]
+                dist = dist_line_rows_per(centre, r0, r1)
+                hit = (dist - w < boxlen) & (dr[idx] > 0.2 * w)
+                f[idx[hit]] = 1
         f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
         f[~ok] = 0
         return f

@@ -78,3 +78,14 @@ def test_batch_flags_are_the_box_flags(oracle_lib):
             want = B.runs[g].box_flag(B.runs[g].time, s[i], dr[i], 8, b["r_loc"][i])
         assert got[i] == want, i
     assert set(got.tolist()) == {-1, 0, 1}
+
+
+def test_dist_line_rows_per_is_dist_line():
+    rng = np.random.default_rng(6)
+    n = 2000
+    r = rng.uniform(-1, 1, (n, 2))
+    r0 = rng.uniform(-1, 1, (n, 2))
+    r1 = np.where(rng.random((n, 1)) < 0.3, r0, rng.uniform(-1, 1, (n, 2)))
+    got = S.dist_line_rows_per(r, r0, r1)
+    for i in range(n):
+        assert got[i] == S.dist_line(list(r[i]), list(r0[i]), list(r1[i]))

@@ -270,3 +270,24 @@ def test_streamer_properties_on_the_oracle(oracle_lib, tmp_path):
     assert len(lines) == 2 and lines[0].split()[2:] == n1
     assert len(lines[1].split()) == 2 + 39
     assert len((tmp_path / "a.txt").read_text().splitlines()) == ax.shape[1]
+
+
+def test_ref_flags_are_the_box_flags(oracle_lib):
+    """The array form of set_ref_flags gives each box the flag box_flag
+    gives it, on a refining cylindrical run inside and after the initial
+    seed window."""
+    cfg = S.Config(variant="cyl", domain_len=16e-3, n_base=(1, 2), lvls_max=8, applied_fld=1e7,
+                   seed_width=[0.3e-3], seed_dens=[5e19], bg_dens=1e12,
+                   seed_r0=[[0.0, 22e-3]], seed_r1=[[0.0, 23e-3]])
+    run = S.StreamerRun(O.OStreamTree, cfg)
+    run.initialise()
+    for t in (run.time, 1.0):
+        run.time = t
+        b, s = run.T.boxes(), run.T.box_summaries()
+        got = run.ref_flags()
+        for i in range(len(b["lvl"])):
+            want = 0
+            if b["in_use"][i] and b["lvl"][i] >= 1:
+                want = run.box_flag(t, s[i], b["dr"][i], int(b["n_cell"][i]), b["r_min"][i])
+            assert got[i] == want, (t, i)
+        assert set(got.tolist()) >= {0, 1} or set(got.tolist()) >= {-1, 0}
