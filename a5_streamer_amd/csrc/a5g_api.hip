@@ -274,8 +274,29 @@ void Pool::prof_end(int idx) {
 }
 
 void Pool::clear_graphs() {
+  if (!graphs.empty()) {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto s : side)
+      if (s) (void)hipStreamSynchronize(s);
+  }
   for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
   graphs.clear();
+}
+
+void Pool::ensure_side(int n) {
+  if (n > kMaxTerms - 1) throw std::invalid_argument("too many side streams");
+  if (!ev_fork) hcheck(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
+  for (int k = 0; k < n; k++) {
+    if (!side[k]) hcheck(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking), "side stream");
+    if (!ev_join[k])
+      hcheck(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming), "event");
+  }
+}
+
+void Pool::sync_streams() {
+  if (stream) hcheck(hipStreamSynchronize(stream), "stream sync");
+  for (auto s : side)
+    if (s) hcheck(hipStreamSynchronize(s), "stream sync");
 }
 
 void Pool::ensure_diag(int n) {
@@ -311,6 +332,14 @@ void Pool::release() {
   d_flux_pairs = nullptr;
   d_diag = nullptr;
   diag_cap = 0;
+  for (int k = 0; k < kMaxTerms - 1; k++) {
+    if (side[k]) (void)hipStreamDestroy(side[k]);
+    if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+    side[k] = nullptr;
+    ev_join[k] = nullptr;
+  }
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  ev_fork = nullptr;
   if (stream) (void)hipStreamDestroy(stream);
   cc = fc = nullptr;
   stream = nullptr;
@@ -978,6 +1007,30 @@ int a5g_photoi_helmholtz(a5g_tree* t, double coeff, int n_terms, const double* a
   check_iv(P.tree, i_tmp);
   a5g::photoi_helmholtz(P, P.sv, coeff, n_terms, amp, kappa, i_phi, i_rhs, i_tmp, tol,
                         max_cycles, cycles);
+  API_END
+}
+
+int a5g_photoi_helmholtz_terms(a5g_tree* t, double coeff, int n_terms, const double* amp,
+                               const double* kappa, const int* i_terms, const int* i_rhs,
+                               const int* i_tmp, int warm, double tol, int max_cycles,
+                               int* cycles) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  need_gpu(P);
+  need_table(P, P.sv);
+  if (P.sv.pho <= 0) throw std::invalid_argument("photoionization: i_pho not set");
+  if (!amp || !kappa || !i_terms || !i_rhs || !i_tmp)
+    throw std::invalid_argument("photoionization: null coefficients or variables");
+  if (n_terms < 1 || n_terms > a5g::kMaxTerms)
+    throw std::invalid_argument("photoionization: need 1 to 4 terms");
+  for (int j = 0; j < n_terms; j++) {
+    check_iv(P.tree, i_terms[j]);
+    check_iv(P.tree, i_rhs[j]);
+    check_iv(P.tree, i_tmp[j]);
+  }
+  a5g::photoi_helmholtz_terms(P, P.sv, coeff, n_terms, amp, kappa, i_terms, i_rhs, i_tmp,
+                              warm != 0, tol, max_cycles, cycles);
   API_END
 }
 

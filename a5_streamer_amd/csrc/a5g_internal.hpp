@@ -36,6 +36,10 @@ void axpy_leaves(Pool& P, int iin, int iout, double s, bool add);
 void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const double* amp,
                       const double* kappa, int i_phi, int i_rhs, int i_tmp, double tol,
                       int max_cycles, int* cycles_done);
+void photoi_helmholtz_terms(Pool& P, const StVars& v, double coeff, int n_terms,
+                            const double* amp, const double* kappa, const int* i_terms,
+                            const int* i_rhs, const int* i_tmp, bool warm, double tol,
+                            int max_cycles, int* cycles_done);
 void prolong1_boxes(Pool& P, int n_ids, const int* ids, int iv);
 void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value);
 void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc);

@@ -21,6 +21,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "a5g_internal.hpp"
 #include "a5g_kernels.hpp"
@@ -1838,6 +1840,8 @@ static void run_captured(Pool& P, uint64_t key, F&& f) {
       return;
     }
   if (P.graphs.size() >= kMaxGraphs) {
+    // the oldest graph may still run on a side stream
+    P.sync_streams();
     (void)hipGraphExecDestroy(P.graphs.front().second);
     P.graphs.erase(P.graphs.begin());
   }
@@ -2289,6 +2293,164 @@ void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const
     P.bc_kind[i] = bk[i];
     P.bc_val[i] = bv[i];
   }
+  if (cycles_done) *cycles_done = total;
+}
+
+// Runs f(j) for every j of js, each on its own stream: the first on the
+// handle's stream, the others on side streams that wait for everything queued
+// before and that the handle's stream waits for afterwards. Work of the
+// different j must touch different variables. Graphs captured inside f run on
+// the stream they were captured on, so independent terms overlap on the GPU.
+template <class F>
+static void branches(Pool& P, const std::vector<int>& js, F&& f) {
+  if (js.empty()) return;
+  if (js.size() == 1) {
+    f(js[0]);
+    return;
+  }
+  P.ensure_side((int)js.size() - 1);
+  hipStream_t main = P.stream;
+  check(hipEventRecord(P.ev_fork, main), "fork event");
+  for (size_t k = 1; k < js.size(); k++)
+    check(hipStreamWaitEvent(P.side[k - 1], P.ev_fork, 0), "fork wait");
+  try {
+    f(js[0]);
+    for (size_t k = 1; k < js.size(); k++) {
+      P.stream = P.side[k - 1];
+      f(js[k]);
+      P.stream = main;
+      check(hipEventRecord(P.ev_join[k - 1], P.side[k - 1]), "join event");
+      check(hipStreamWaitEvent(main, P.ev_join[k - 1], 0), "join wait");
+    }
+  } catch (...) {
+    P.stream = main;
+    (void)hipDeviceSynchronize();
+    throw;
+  }
+}
+
+// The Helmholtz photoionization with each term in variables of its own: S_j
+// in i_terms[j], its right-hand side -amp_j I in i_rhs[j] and its multigrid
+// scratch in i_tmp[j]. The terms then share no data, so their solves run
+// side by side (branches): on a mesh whose levels are mostly small, a
+// V-cycle is a chain of short launches and three chains overlap. Each term
+// goes through the same steps as a solve on its own: the start (cold: FMG
+// from zero, the reference's mg$D_fas_fmg with first_call; warm: the
+// solution an earlier call left in i_terms[j], restricted to every parent
+// with the multigrid's restriction and its ghost cells filled level by level
+// upwards), then V-cycles until max|r| <= tol max|rhs_j| on the leaves.
+// i_pho = sum_j S_j, added in term order.
+void photoi_helmholtz_terms(Pool& P, const StVars& v, double coeff, int n_terms,
+                            const double* amp, const double* kappa, const int* i_terms,
+                            const int* i_rhs, const int* i_tmp, bool warm, double tol,
+                            int max_cycles, int* cycles_done) {
+  if (n_terms < 1 || n_terms > kMaxTerms)
+    throw std::invalid_argument("photoionization: need 1 to " + std::to_string(kMaxTerms) +
+                                " terms");
+  for (int j = 0; j < n_terms; j++)
+    if (!(amp[j] > 0) || !(kappa[j] >= 0))
+      throw std::invalid_argument("photoionization: need amp > 0 and kappa >= 0");
+  {
+    std::vector<int> all;
+    for (int j = 0; j < n_terms; j++) {
+      all.push_back(i_terms[j]);
+      all.push_back(i_rhs[j]);
+      all.push_back(i_tmp[j]);
+    }
+    all.push_back(v.pho);
+    std::vector<int> sorted = all;
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      throw std::invalid_argument("photoionization: the term, rhs and tmp variables of all "
+                                  "terms and i_pho must all differ");
+  }
+  P.sync_topology();
+  photoi_rate(P, v, coeff);
+  const MgOpts saved = P.mg;
+  int bk[6];
+  double bv[6];
+  for (int i = 0; i < 6; i++) {
+    bk[i] = P.bc_kind[i];
+    bv[i] = P.bc_val[i];
+    P.bc_kind[i] = 1;  // S = 0 on the boundary: ghost = -c
+    P.bc_val[i] = 0;
+  }
+  P.mg.i_eps = -1;
+  P.mg.ready = true;
+  auto term = [&](int j) {
+    P.mg.i_phi = i_terms[j];
+    P.mg.i_rhs = i_rhs[j];
+    P.mg.i_tmp = i_tmp[j];
+    P.mg.helm = kappa[j] * kappa[j];
+    return ctx64(P);
+  };
+  const auto restore = [&] {
+    P.mg = saved;
+    for (int i = 0; i < 6; i++) {
+      P.bc_kind[i] = bk[i];
+      P.bc_val[i] = bv[i];
+    }
+  };
+  int total = 0;
+  try {
+    for (int j = 0; j < n_terms; j++) axpy_leaves(P, v.pho, i_rhs[j], -amp[j], false);
+    axpy_leaves(P, v.pho, v.pho, 0.0, false);  // i_pho = 0 on leaves
+    const int mx = P.mg_max_lvl();
+    std::vector<int> active;
+    for (int j = 0; j < n_terms; j++) active.push_back(j);
+    // captured like the field solve's calls; each term's variables and
+    // Helmholtz coefficient keep its graphs apart
+    branches(P, active, [&](int j) {
+      const Ctx c = term(j);
+      if (warm) {
+        run_captured(P, call_key(4, mx - P.tree.min_lvl, 0, 0), [&] {
+          restrict_levels(P, c.i_phi, true);
+          for (int lvl = P.tree.min_lvl; lvl <= mx; lvl++) {
+            if (P.tree.dim == 2) fill_gc_phi<2, double>(P, c, lvl);
+            else fill_gc_phi<3, double>(P, c, lvl);
+          }
+        });
+      } else {
+        run_captured(P, call_key(1, 0, 1, 0), [&] {
+          if (P.tree.dim == 2) fas_fmg<2, double>(P, c, false, true);
+          else fas_fmg<3, double>(P, c, false, true);
+        });
+      }
+    });
+    // the leaf rhs of a FAS solve does not change: its maximum once per term
+    double rhs_max[kMaxTerms], lo;
+    for (int j = 0; j < n_terms; j++) minmax_boxes(P, i_rhs[j], true, false, true, &lo, &rhs_max[j]);
+    for (int cyc = 0;; cyc++) {
+      std::vector<int> next;
+      for (int j : active) {
+        const Ctx c = term(j);
+        for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+          if (P.tree.dim == 2) residual_level<2, double>(P, c, lvl, true);
+          else residual_level<3, double>(P, c, lvl, true);
+        }
+        double res;
+        minmax_boxes(P, c.i_tmp, true, false, true, &lo, &res);
+        if (!(res <= tol * rhs_max[j])) next.push_back(j);
+      }
+      active.swap(next);
+      if (active.empty()) break;
+      if (cyc >= max_cycles)
+        throw std::runtime_error("photoionization: Helmholtz solve did not converge");
+      branches(P, active, [&](int j) {
+        const Ctx c = term(j);
+        run_captured(P, call_key(2, mx - P.tree.min_lvl, 0, 0), [&] {
+          if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, mx, false);
+          else fas_vcycle<3, double>(P, c, mx, false);
+        });
+      });
+      total += (int)active.size();
+    }
+    for (int j = 0; j < n_terms; j++) axpy_leaves(P, i_terms[j], v.pho, 1.0, true);
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
   if (cycles_done) *cycles_done = total;
 }
 

@@ -85,6 +85,8 @@ struct MgOpts {
 };
 
 constexpr int kRedSlots = 256;
+// independent solves that may run side by side (photoionization terms)
+constexpr int kMaxTerms = 4;
 
 struct Pool {
   Tree tree;
@@ -145,7 +147,16 @@ struct Pool {
   bool graphs_on = true;
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
+  // waits for every stream of the handle, then destroys the graphs (one may
+  // still run on a side stream otherwise)
   void clear_graphs();
+  // side streams for independent solves run side by side, created on first
+  // use; the handle's stream waits for them after every such section
+  hipStream_t side[kMaxTerms - 1] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kMaxTerms - 1] = {};
+  void ensure_side(int n);
+  void sync_streams();
 
   // instances (a5g_set_instances): the device level lists hold only the
   // boxes of active instances

@@ -414,6 +414,18 @@ module m_a5g
        integer(c_int)        :: s
      end function c_photoi_helmholtz
 
+     function c_photoi_helmholtz_terms(t, coeff, n_terms, amp, kappa, i_terms, i_rhs, i_tmp, &
+          warm, tol, max_cycles, cycles) bind(C, name="a5g_photoi_helmholtz_terms") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: coeff, tol
+       integer(c_int), value :: n_terms, warm, max_cycles
+       real(c_double)        :: amp(*), kappa(*)
+       integer(c_int)        :: i_terms(*), i_rhs(*), i_tmp(*)
+       integer(c_int)        :: cycles
+       integer(c_int)        :: s
+     end function c_photoi_helmholtz_terms
+
      function c_box_summaries(t, out) bind(C, name="a5g_box_summaries") result(s)
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: t
@@ -627,6 +639,7 @@ module m_a5g
   public :: st_setup, st_table_create, st_table_set_col, st_table_get
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
+  public :: st_photoi_helmholtz_terms
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
   public :: a5g_set_instances, a5g_set_instance_values, a5g_set_active_instances
@@ -1064,6 +1077,32 @@ contains
          i_tmp, tol, max_cycles, cyc), "photoi_helmholtz")
     cycles = cyc
   end subroutine st_photoi_helmholtz
+
+  !> The same model with term j in variables of its own: solution
+  !> i_terms(j), right-hand side i_rhs(j), multigrid scratch i_tmp(j); the
+  !> terms' solves run side by side. warm = .true. starts each term's
+  !> V-cycles from the solution an earlier call left in i_terms(j) (and
+  !> prolonged to new boxes) instead of an FMG from zero.
+  subroutine st_photoi_helmholtz_terms(tree, coeff, amp, kappa, i_terms, i_rhs, i_tmp, warm, &
+       tol, max_cycles, cycles)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: coeff, amp(:), kappa(:), tol
+    integer, intent(in)        :: i_terms(:), i_rhs(:), i_tmp(:), max_cycles
+    logical, intent(in)        :: warm
+    integer, intent(out)       :: cycles
+    integer(c_int)             :: cyc, iw
+    integer(c_int)             :: it(size(i_terms)), ir(size(i_terms)), im(size(i_terms))
+    if (size(amp) /= size(kappa) .or. size(amp) /= size(i_terms) .or. &
+         size(amp) /= size(i_rhs) .or. size(amp) /= size(i_tmp)) &
+         error stop "st_photoi_helmholtz_terms: amp, kappa and the variables differ"
+    it = i_terms
+    ir = i_rhs
+    im = i_tmp
+    iw = merge(1, 0, warm)
+    call check(c_photoi_helmholtz_terms(tree%h, coeff, size(amp), amp, kappa, it, ir, im, &
+         iw, tol, max_cycles, cyc), "photoi_helmholtz_terms")
+    cycles = cyc
+  end subroutine st_photoi_helmholtz_terms
 
   !> average_dens on every box (src/streamer_2d.f90:492-496)
   subroutine st_average_dens(tree)

@@ -290,6 +290,25 @@ class Tree:
               C.c_int(max_cycles), C.byref(cyc))
         return cyc.value
 
+    def photoi_helmholtz_terms(self, coeff, amp, kappa, i_terms, i_rhs, i_tmp, warm,
+                               tol=1e-8, max_cycles=30):
+        """The Helmholtz photoionization with term j in variables of its own
+        (solution i_terms[j], rhs i_rhs[j], scratch i_tmp[j]), the terms'
+        solves side by side; warm starts each term's V-cycles from what an
+        earlier call left in i_terms[j] instead of an FMG from zero."""
+        a, pa = _d(amp)
+        k, pk = _d(kappa)
+        iv = [np.ascontiguousarray(x, dtype=np.int32) for x in (i_terms, i_rhs, i_tmp)]
+        if not all(len(x) == len(a) for x in [k] + iv):
+            raise A5GError("photoi_helmholtz_terms: amp, kappa and the variables differ in "
+                           "length")
+        cyc = C.c_int()
+        pi = [x.ctypes.data_as(C.POINTER(C.c_int)) for x in iv]
+        _call("a5g_photoi_helmholtz_terms", self.h, C.c_double(coeff), C.c_int(len(a)), pa, pk,
+              pi[0], pi[1], pi[2], C.c_int(int(bool(warm))), C.c_double(tol),
+              C.c_int(max_cycles), C.byref(cyc))
+        return cyc.value
+
     def box_summaries(self):
         n = self.info()["max_id"]
         out = np.zeros((n, 3))

@@ -27,9 +27,12 @@ EPS_F64 = 2.220446049250313e-16  # epsilon(1.0_dp)
 # variable indices of src/m_streamer.f90:16-43
 I_ELEC, I_PION, I_ELEC_OLD, I_PION_OLD, I_PHI, I_FLD, I_RHS, I_PHO, I_EPS = range(1, 10)
 N_VAR_CELL, N_VAR_FACE = 9, 2
-# work variables of the photoionization solves (phi, rhs, tmp), after the
-# reference's nine
-I_PH_PHI, I_PH_RHS, I_PH_TMP = 10, 11, 12
+# variables of the three photoionization terms, after the reference's nine:
+# each term's solution (kept from one step to the next), right-hand side and
+# multigrid scratch
+I_PH_TERMS = (10, 11, 12)
+I_PH_RHS = (13, 14, 15)
+I_PH_TMP = (16, 17, 18)
 UC_TORR_TO_BAR = 133.322368 * 1.0e-5  # fosito/m_units_constants.f90:44
 UC_PI = math.pi                        # acos(-1), fosito/m_units_constants.f90:24
 # Three-term fit of the photoionization absorption function of air (Bourdon
@@ -251,6 +254,7 @@ class Config:
         self.photoi_frac_o2 = 0.2
         self.gas_pressure = 1.0        # bar, src/m_streamer.f90:160-161
         self.photoi_tol = 1e-6         # relative residual of each term's solve
+        self.photoi_warm = True        # V-cycles from the last step's solutions
         self.table_size = 1000
         self.table_max_fld = 3e7
         self.n_boxes = 1000
@@ -331,7 +335,9 @@ class StreamerRun:
         # domain uses coarsen_to=2 as the drivers do
         n0 = c.n_base if np.isscalar(c.n_base) else c.n_base[0]
         dr = c.domain_len / (c.box_size * n0)
-        n_cc = N_VAR_CELL + (3 if c.photoi else 0)
+        n_cc = N_VAR_CELL + (3 * len(I_PH_TERMS) if c.photoi else 0)
+        self.photoi_solved = False     # the term variables hold solutions
+        self.photoi_cycles = 0         # V-cycles of the photoionization solves
         self.T = T = tree_cls(self.dim, c.box_size, n_cc, N_VAR_FACE, dr,
                               coarsen_to=2, lvls_max=c.lvls_max, coord=v["coord"],
                               n_boxes=c.n_boxes)
@@ -528,6 +534,10 @@ class StreamerRun:
                 continue
             for iv in (I_ELEC, I_PION, I_PHI):
                 T.prolong1_to(add, iv)
+            if self.c.photoi:
+                # the next photoionization solves start from these
+                for iv in I_PH_TERMS:
+                    T.prolong1_to(add, iv)
             self._set_eps(add)
             T.gc_boxes(add, I_ELEC, RB_INTERP_LIM, BC_NEUMANN)
             T.gc_boxes(add, I_PION, RB_INTERP_LIM, BC_NEUMANN)
@@ -552,11 +562,18 @@ class StreamerRun:
     def set_photoionization(self):
         """set_photoionization (src/streamer_2d.f90:544-565) with the
         Helmholtz model in place of Monte-Carlo photons: the production rate
-        of set_photoi_rate, then one multigrid solve per term. Returns the
+        of set_photoi_rate, then one multigrid solve per term in variables
+        of its own (the library runs the terms side by side). After the
+        first call each solve starts from the term's solution of the call
+        before (Config.photoi_warm): V-cycles only, no FMG. Returns the
         V-cycles run."""
         coeff, amp, kappa = self.c.photoi_model()
-        return self.T.photoi_helmholtz(coeff, np.array(amp), np.array(kappa), I_PH_PHI,
-                                       I_PH_RHS, I_PH_TMP, self.c.photoi_tol, 40)
+        warm = self.c.photoi_warm and self.photoi_solved
+        cyc = self.T.photoi_helmholtz_terms(coeff, np.array(amp), np.array(kappa), I_PH_TERMS,
+                                            I_PH_RHS, I_PH_TMP, warm, self.c.photoi_tol, 40)
+        self.photoi_solved = True
+        self.photoi_cycles += cyc
+        return cyc
 
     def initialise(self, max_iter=50):
         """Initial conditions, field and refinement until the mesh settles

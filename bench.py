@@ -226,7 +226,7 @@ def streamer_step(args):
                     lvls_max=args.step_lvls,
                     applied_fld=2.0e6, seed_r0=[[32e-3, 32e-3, 56e-3]],
                     seed_r1=[[32e-3, 32e-3, 56e-3]], photoi=not args.no_photoi,
-                    n_boxes=20000)
+                    photoi_warm=not args.photoi_cold, n_boxes=20000)
     run = ST.StreamerRun(A.Tree, cfg)
     t0 = time.perf_counter()
     run.initialise()
@@ -238,11 +238,13 @@ def streamer_step(args):
     T = run.T
     T.sync()
     c0 = run.leaf_cells()
+    cyc0 = run.photoi_cycles
     t0 = time.perf_counter()
     for _ in range(blocks):
         run.step_block()
     T.sync()
     dt = time.perf_counter() - t0
+    cyc1 = run.photoi_cycles
     c1 = run.leaf_cells()
     n_steps = blocks * cfg.steps_amr
     T.prof_enable(True)
@@ -273,6 +275,9 @@ def streamer_step(args):
                                    if cfg.photoi else "no photoionization"),
                    "leaf_cells_start": c0, "leaf_cells_end": c1, "leaf_cells_init": cells,
                    "init_s": round(t_init, 1)},
+        "photoi": {"start": "cold (FMG)" if args.photoi_cold else "warm (last step's)",
+                   "v_cycles_per_step": round((cyc1 - cyc0) / n_steps, 2)}
+        if cfg.photoi else None,
         "kernels": kinds,
         "profiled_block_s": round(t_prof, 3),
         "kernel_ms_per_step": round(tot_ms / cfg.steps_amr, 3),
@@ -380,6 +385,8 @@ def main():
     ap.add_argument("--step-lvls", type=int, default=7)
     ap.add_argument("--step-base", type=int, default=16)
     ap.add_argument("--no-photoi", action="store_true")
+    ap.add_argument("--photoi-cold", action="store_true",
+                    help="solve every photoionization term by FMG from zero each step")
     ap.add_argument("--batch", type=int, default=0,
                     help="measure this many batched cylindrical runs instead (own JSON line)")
     ap.add_argument("--batch-lvls", type=int, default=10)

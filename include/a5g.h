@@ -10,7 +10,8 @@
  * (afivo/src/m_aX_t.f90:136-205): box id 0 = no box, negative neighbour ids
  * are physical boundaries, a box's cell block is cc(0:nc+1,0:nc+1[,0:nc+1])
  * with i fastest. One handle is used by one host thread at a time; all GPU
- * work is ordered on the handle's own HIP stream.
+ * work is ordered on the handle's own HIP stream (independent photoionization
+ * terms run on side streams that it waits for before it goes on).
  */
 #ifndef A5G_H
 #define A5G_H
@@ -180,6 +181,24 @@ int a5g_average_dens(a5g_tree *t);
 int a5g_photoi_helmholtz(a5g_tree *t, double coeff, int n_terms, const double *amp,
                          const double *kappa, int i_phi, int i_rhs, int i_tmp,
                          double tol, int max_cycles, int *cycles);
+/* The same model with every term in variables of its own: S_j in
+ * i_terms[j], its right-hand side -amp_j I in i_rhs[j] and its multigrid
+ * scratch in i_tmp[j] (1 <= n_terms <= 4; all 3 n_terms variables and i_pho
+ * distinct). The terms then share no data and their solves run side by side
+ * on the GPU. S_j stays in i_terms[j] for the next call. warm = 0: each term
+ * is solved as above (FMG from zero, then V-cycles). warm != 0: i_terms[j]
+ * holds the solution of an earlier call, prolonged to boxes created since
+ * (a$D_prolong1_to); it is restricted to every parent with the multigrid's
+ * restriction (mg$D_box_rstr_lpl, afivo/src/m_aX_mg.f90:851-893), its ghost
+ * cells are filled level by level upwards, and V-cycles alone bring max|r|
+ * under tol max|rhs_j|. Same replacement of set_photoionization
+ * (src/streamer_2d.f90:544-565) as a5g_photoi_helmholtz; the source moves
+ * little in one time step, so an earlier solution is close and the FMG is
+ * not needed. i_pho = sum_j S_j, added in term order. */
+int a5g_photoi_helmholtz_terms(a5g_tree *t, double coeff, int n_terms, const double *amp,
+                               const double *kappa, const int *i_terms, const int *i_rhs,
+                               const int *i_tmp, int warm, double tol, int max_cycles,
+                               int *cycles);
 /* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
  * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
 int a5g_box_summaries(a5g_tree *t, double *out);

@@ -2413,6 +2413,75 @@ int o_photoi_helmholtz(otree *t, const o_vars *v, const otable *tbl, double coef
   return status;
 }
 
+int o_photoi_helmholtz_terms(otree *t, const o_vars *v, const otable *tbl, double coeff,
+                             int n_terms, const double *amp, const double *kappa,
+                             const int *i_terms, const int *i_rhs, const int *i_tmp, int warm,
+                             double tol, int max_cycles, int *cycles) {
+  /* The same model with term j in variables of its own: S_j in i_terms[j],
+   * rhs_j = -amp_j I in i_rhs[j], scratch in i_tmp[j]; the terms are solved
+   * one after the other here (they share no data). Cold: FMG from zero
+   * (first_call). Warm: i_terms[j] holds an earlier solution; it is
+   * restricted to every parent (o_box_rstr, the multigrid's restriction),
+   * ghost cells are filled level by level upwards, and V-cycles alone reach
+   * the tolerance. max|rhs_j| is taken once: a FAS solve leaves the leaf rhs
+   * as it is. Returns 0, or -1 when a solve does not converge. */
+  o_photoi_rate(t, v, tbl, coeff);
+  int s_phi = t->i_phi, s_rhs = t->i_rhs, s_tmp = t->i_tmp, s_eps = t->i_eps, s_ok = t->mg_ok;
+  int s_kind[6];
+  double s_val[6], s_helm = t->helm;
+  for (int i = 0; i < 6; i++) {
+    s_kind[i] = t->bc_kind[i];
+    s_val[i] = t->bc_val[i];
+    t->bc_kind[i] = 1;
+    t->bc_val[i] = 0;
+  }
+  t->i_eps = -1;
+  t->mg_ok = 1;
+  int total = 0, status = 0;
+  for (int j = 0; j < n_terms; j++) o_axpy_leaves(t, v->pho, i_rhs[j], -amp[j], 0);
+  o_axpy_leaves(t, v->pho, v->pho, 0.0, 0);
+  for (int j = 0; j < n_terms && status == 0; j++) {
+    t->i_phi = i_terms[j];
+    t->i_rhs = i_rhs[j];
+    t->i_tmp = i_tmp[j];
+    t->helm = kappa[j] * kappa[j];
+    if (warm) {
+      for (int lvl = t->max_lvl; lvl >= t->min_lvl + 1; lvl--) {
+        ivec *ids = &t->ids[LV(t, lvl)];
+        for (int i = 0; i < ids->n; i++) o_box_rstr(t, ids->v[i], t->i_phi);
+      }
+      for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) fill_gc_phi(t, lvl);
+    } else {
+      o_fas_fmg(t, 0, 1);
+    }
+    double rhs = leaf_maxabs(t, i_rhs[j]);
+    for (int cyc = 0;; cyc++) {
+      double res = o_max_residual(t);
+      if (res <= tol * rhs) break;
+      if (cyc >= max_cycles) {
+        status = -1;
+        break;
+      }
+      o_fas_vcycle(t, t->max_lvl, 0);
+      total++;
+    }
+  }
+  if (status == 0)
+    for (int j = 0; j < n_terms; j++) o_axpy_leaves(t, i_terms[j], v->pho, 1.0, 1);
+  t->i_phi = s_phi;
+  t->i_rhs = s_rhs;
+  t->i_tmp = s_tmp;
+  t->i_eps = s_eps;
+  t->mg_ok = s_ok;
+  t->helm = s_helm;
+  for (int i = 0; i < 6; i++) {
+    t->bc_kind[i] = s_kind[i];
+    t->bc_val[i] = s_val[i];
+  }
+  if (cycles) *cycles = total;
+  return status;
+}
+
 void o_update_solution(otree *t, const o_vars *v, const otable *tbl, double dt) {
   /* update_solution, src/streamer_2d.f90:499-542 (3D :566-615, cylindrical
    * src/streamer_cyl.f90:531-570), on leaves */

@@ -138,6 +138,13 @@ void o_axpy_leaves(otree *t, int iin, int iout, double s, int add);
 int o_photoi_helmholtz(otree *t, const o_vars *v, const otable *tbl, double coeff,
                        int n_terms, const double *amp, const double *kappa, int i_phi,
                        int i_rhs, int i_tmp, double tol, int max_cycles, int *cycles);
+/* the same with term j in variables of its own (solution i_terms[j], rhs
+ * i_rhs[j], scratch i_tmp[j]); warm: V-cycles from the solution an earlier
+ * call left in i_terms[j], instead of an FMG from zero */
+int o_photoi_helmholtz_terms(otree *t, const o_vars *v, const otable *tbl, double coeff,
+                             int n_terms, const double *amp, const double *kappa,
+                             const int *i_terms, const int *i_rhs, const int *i_tmp, int warm,
+                             double tol, int max_cycles, int *cycles);
 /* compute_fld's right-hand side on leaves, whole boxes: rhs = fac (ne - ni) */
 void o_space_charge(otree *t, const o_vars *v, double fac);
 /* fld_from_pot on every box */

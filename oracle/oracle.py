@@ -429,6 +429,25 @@ class OStreamTree(OTree):
             raise RuntimeError("photoionization: Helmholtz solve did not converge")
         return cyc.value
 
+    def photoi_helmholtz_terms(self, coeff, amp, kappa, i_terms, i_rhs, i_tmp, warm,
+                               tol=1e-8, max_cycles=30):
+        a, pa = _d(amp)
+        k, pk = _d(kappa)
+        iv = [np.ascontiguousarray(x, dtype=np.int32) for x in (i_terms, i_rhs, i_tmp)]
+        if not all(len(x) == len(a) for x in [k] + iv):
+            raise ValueError("photoi_helmholtz_terms: amp, kappa and the variables differ")
+        v = OVars(*[getattr(self.v, f) for f, _ in OVars._fields_])
+        v.photoi = 1
+        cyc = C.c_int()
+        pi = [x.ctypes.data_as(C.POINTER(C.c_int)) for x in iv]
+        st = self.L.o_photoi_helmholtz_terms(
+            self.h, C.byref(v), self.tbl.h, C.c_double(coeff), C.c_int(len(a)), pa, pk,
+            pi[0], pi[1], pi[2], C.c_int(int(bool(warm))), C.c_double(tol),
+            C.c_int(max_cycles), C.byref(cyc))
+        if st != 0:
+            raise RuntimeError("photoionization: Helmholtz solve did not converge")
+        return cyc.value
+
     def box_summaries(self):
         return box_summaries(self, self.v)
 

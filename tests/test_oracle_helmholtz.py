@@ -153,7 +153,7 @@ def test_photoi_uniform_source_far_field(oracle_lib):
     coeff, amp, kappa = cfg.photoi_model()
     nc, nb = 8, 8
     dr = 32e-3 / (nc * nb)
-    T = O.OStreamTree(2, nc, 12, 2, dr, coarsen_to=2, n_boxes=200)
+    T = O.OStreamTree(2, nc, 18, 2, dr, coarsen_to=2, n_boxes=200)
     T.set_base(*ST.base_grid(2, nb))
     T.table_create(0.0, cfg.table_max_fld, cfg.table_size, 4)
     x, cols = cfg.transport_columns()
@@ -166,8 +166,8 @@ def test_photoi_uniform_source_far_field(oracle_lib):
         T.set_cc(i, ST.I_FLD, np.full(bl, E0))
         T.set_cc(i, ST.I_ELEC, np.full(bl, ne0))
     T.mg_setup(ST.I_PHI, ST.I_FLD, ST.I_RHS, bc_kind=np.zeros(4, np.int32))
-    T.photoi_helmholtz(coeff, np.array(amp), np.array(kappa), ST.I_PH_PHI, ST.I_PH_RHS,
-                       ST.I_PH_TMP, tol=1e-10, max_cycles=60)
+    T.photoi_helmholtz(coeff, np.array(amp), np.array(kappa), ST.I_PH_TERMS[0],
+                       ST.I_PH_RHS[0], ST.I_PH_TMP[0], tol=1e-10, max_cycles=60)
     tb = ST.Table(*T.table_get())
     I0 = E0 * tb.col(ST.T_MOB, E0) * tb.col(ST.T_ALPHA, E0) * ne0 * coeff
     want = sum(a * I0 / k ** 2 for a, k in zip(amp, kappa))
@@ -181,3 +181,48 @@ def test_photoi_uniform_source_far_field(oracle_lib):
     np.testing.assert_allclose(got, want, rtol=1e-4)
     # the multigrid options of the field solve are restored
     T.fas_fmg(False, True)
+
+
+def _leaf_interiors(T, dim, iv):
+    b = T.boxes()
+    out = []
+    for i in range(len(b["lvl"])):
+        if b["in_use"][i] and b["children"][i][0] == 0:
+            nc = int(b["n_cell"][i])
+            a = T.get_cc(i + 1, iv).reshape([nc + 2] * dim, order="F")
+            out.append(a[(slice(1, -1),) * dim].ravel())
+    return np.concatenate(out)
+
+
+def test_warm_start_of_the_terms():
+    """Each term kept in its own variable: a warm call on an unchanged source
+    runs no V-cycle and gives i_pho bit for bit. After two step blocks
+    (the mesh refined, the term variables prolonged to the new boxes) a warm
+    call and a cold one (FMG from zero) on the same state agree term by term
+    within the error that the residual tolerance allows: for (L - kappa^2) S
+    = f the discrete maximum principle gives |S| <= max|f| / kappa^2, so two
+    solutions whose residuals are each at most tol max|rhs_j| differ by at
+    most 2 tol max|rhs_j| / kappa_j^2 (a factor 10 is left for the
+    coarse-fine interpolation, which is not monotone)."""
+    cfg = ST.Config(variant="2d", photoi=True, lvls_max=6)
+    r = ST.StreamerRun(O.OStreamTree, cfg)
+    r.initialise()
+    p0 = _leaf_interiors(r.T, 2, ST.I_PHO)
+    assert r.set_photoionization() == 0
+    assert np.array_equal(_leaf_interiors(r.T, 2, ST.I_PHO), p0)
+    for _ in range(2):
+        n_add, n_rm = r.step_block()
+    assert r.photoi_solved
+    warm_cyc = r.set_photoionization()
+    warm = [_leaf_interiors(r.T, 2, iv) for iv in ST.I_PH_TERMS]
+    coeff, amp, kappa = cfg.photoi_model()
+    # i_rhs[j] holds -amp_j I on leaves after the solves
+    rate = np.abs(_leaf_interiors(r.T, 2, ST.I_PH_RHS[0])).max() / amp[0]
+    r.c.photoi_warm = False
+    cold_cyc = r.set_photoionization()
+    cold = [_leaf_interiors(r.T, 2, iv) for iv in ST.I_PH_TERMS]
+    assert 0 < warm_cyc <= 40 and cold_cyc > 0
+    for j in range(3):
+        bound = 2 * cfg.photoi_tol * amp[j] * rate / kappa[j] ** 2
+        assert np.abs(warm[j] - cold[j]).max() <= 10 * bound, j
+    assert np.abs(warm[0]).max() > 1e3 * 10 * 2 * cfg.photoi_tol * amp[0] * rate / kappa[0] ** 2
