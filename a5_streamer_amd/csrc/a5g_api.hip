@@ -293,6 +293,12 @@ void Pool::ensure_side(int n) {
   }
 }
 
+void Pool::ensure_term_red() {
+  if (d_term_red) return;
+  hcheck(hipMalloc((void**)&d_term_red, (size_t)kMaxTerms * 2 * kRedSlots * sizeof(unsigned long long)),
+         "hipMalloc term reductions");
+}
+
 void Pool::sync_streams() {
   if (stream) hcheck(hipStreamSynchronize(stream), "stream sync");
   for (auto s : side)
@@ -328,6 +334,8 @@ void Pool::release() {
   cc32 = nullptr;
   f(cc); f(fc); f(d_lvl); f(d_ix); f(d_parent); f(d_children); f(d_nbs); f(d_ncell);
   f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag); f(d_inst);
+  f(d_term_red);
+  d_term_red = nullptr;
   d_inst = nullptr;
   d_flux_pairs = nullptr;
   d_diag = nullptr;

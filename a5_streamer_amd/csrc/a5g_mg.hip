@@ -1316,6 +1316,49 @@ __global__ __launch_bounds__(256) void k_minmax(Dev a, const int* __restrict__ l
   }
 }
 
+// max over leaf interiors of |rhs - A phi| (resid) or of |rhs|, into
+// out[blockIdx.x % kRedSlots] as ordered-integer keys. The residual is the
+// expression of k_residual, so the maximum equals that of the stored
+// residual. Used by the convergence checks of solves that run side by side,
+// each with slots of its own.
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_leaf_absmax(Dev a, const int* __restrict__ list, int n,
+                                                     int nc, int tpb, int i_phi, int i_rhs,
+                                                     int resid, unsigned long long* out) {
+  const int bpb_ = blockDim.x / tpb;
+  const int slot_ = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot_ * tpb;
+  const int li_ = blockIdx.x * bpb_ + slot_;
+  double hi = 0.0;
+  if (li_ < n) {
+    const int id = list[li_];
+    const double* p = boxp(a, i_phi, id);
+    const double* r = boxp(a, i_rhs, id);
+    const double inv_dr_sq = 1 / (a.dr[id] * a.dr[id]);
+    const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+    const int total = D == 2 ? nc * nc : nc * nc * nc;
+    for (int q = lt; q < total; q += tpb) {
+      const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+      const int x = cidx<D>(nc, i, j, k);
+      const double v =
+          resid ? r[x] - op_at<D, CYL, double>(a, id, p, nc, i, j, k, inv_dr_sq, ioff) : r[x];
+      hi = fmax(hi, fabs(v));
+    }
+  }
+  hi = wave_max(hi);
+  __shared__ double wh[4];
+  if ((threadIdx.x & 63) == 0) wh[threadIdx.x >> 6] = hi;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) hi = fmax(hi, wh[w]);
+    atomicMax(&out[blockIdx.x % kRedSlots], ord_key(hi));
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
 // Range of one variable over whole boxes (ghosts included): out[2 li] = min,
 // out[2 li + 1] = max for box list[li]. One workgroup per box.
 template <int D>
@@ -2296,6 +2339,29 @@ void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const
   if (cycles_done) *cycles_done = total;
 }
 
+// Maximum over leaf interiors of |rhs - A phi| (resid) or |rhs| for the
+// problem c, into the kRedSlots keys at out (cleared by a kernel first, so
+// that a graph can hold the whole check).
+template <int D>
+static void leaf_absmax(Pool& P, const Ctx& c, bool resid, unsigned long long* out) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, P.stream, out, kRedSlots, 0ull);
+  post_launch("fill");
+  const Dev a = dev_for(P, c);
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (n == 0) continue;
+    const int nc = P.nc_lvl(lvl);
+    Geo g = geo(n, cells_of(D, nc, false));
+    if (c.cyl)
+      hipLaunchKernelGGL((k_leaf_absmax<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.leaves(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, (int)resid, out);
+    else
+      hipLaunchKernelGGL((k_leaf_absmax<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.leaves(lvl), n, nc, g.tpb, c.i_phi, c.i_rhs, (int)resid, out);
+    post_launch("leaf_absmax");
+  }
+}
+
 // Runs f(j) for every j of js, each on its own stream: the first on the
 // handle's stream, the others on side streams that wait for everything queued
 // before and that the handle's stream waits for afterwards. Work of the
@@ -2398,51 +2464,76 @@ void photoi_helmholtz_terms(Pool& P, const StVars& v, double coeff, int n_terms,
     const int mx = P.mg_max_lvl();
     std::vector<int> active;
     for (int j = 0; j < n_terms; j++) active.push_back(j);
+    // Each term's slots: kRedSlots keys of max|r|, then kRedSlots of
+    // max|rhs|. The checks run inside the branches, so one copy and one wait
+    // per section read them all.
+    P.ensure_term_red();
+    auto res_slots = [&](int j) { return P.d_term_red + (size_t)j * 2 * kRedSlots; };
+    std::vector<unsigned long long> keys((size_t)kMaxTerms * 2 * kRedSlots);
+    auto read_keys = [&] {
+      check(hipMemcpyAsync(keys.data(), P.d_term_red, keys.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, P.stream), "check read");
+      check(hipStreamSynchronize(P.stream), "check sync");
+    };
+    auto key_max = [&](int j, int half) {
+      unsigned long long k = 0;
+      for (int sl = 0; sl < kRedSlots; sl++)
+        k = std::max(k, keys[(size_t)j * 2 * kRedSlots + half * kRedSlots + sl]);
+      if (k == 0) return 0.0;  // no leaf cell
+      const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+      double d;
+      std::memcpy(&d, &u, 8);
+      return d;
+    };
     // captured like the field solve's calls; each term's variables and
     // Helmholtz coefficient keep its graphs apart
     branches(P, active, [&](int j) {
       const Ctx c = term(j);
-      if (warm) {
-        run_captured(P, call_key(4, mx - P.tree.min_lvl, 0, 0), [&] {
+      run_captured(P, call_key(warm ? 4 : 5, mx - P.tree.min_lvl, j, 0), [&] {
+        if (warm) {
           restrict_levels(P, c.i_phi, true);
           for (int lvl = P.tree.min_lvl; lvl <= mx; lvl++) {
             if (P.tree.dim == 2) fill_gc_phi<2, double>(P, c, lvl);
             else fill_gc_phi<3, double>(P, c, lvl);
           }
-        });
-      } else {
-        run_captured(P, call_key(1, 0, 1, 0), [&] {
+        } else {
           if (P.tree.dim == 2) fas_fmg<2, double>(P, c, false, true);
           else fas_fmg<3, double>(P, c, false, true);
-        });
-      }
+        }
+        // the leaf rhs of a FAS solve does not change: its maximum once
+        if (P.tree.dim == 2) {
+          leaf_absmax<2>(P, c, false, res_slots(j) + kRedSlots);
+          leaf_absmax<2>(P, c, true, res_slots(j));
+        } else {
+          leaf_absmax<3>(P, c, false, res_slots(j) + kRedSlots);
+          leaf_absmax<3>(P, c, true, res_slots(j));
+        }
+      });
     });
-    // the leaf rhs of a FAS solve does not change: its maximum once per term
-    double rhs_max[kMaxTerms], lo;
-    for (int j = 0; j < n_terms; j++) minmax_boxes(P, i_rhs[j], true, false, true, &lo, &rhs_max[j]);
+    read_keys();
+    double rhs_max[kMaxTerms];
+    for (int j = 0; j < n_terms; j++) rhs_max[j] = key_max(j, 1);
     for (int cyc = 0;; cyc++) {
       std::vector<int> next;
-      for (int j : active) {
-        const Ctx c = term(j);
-        for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
-          if (P.tree.dim == 2) residual_level<2, double>(P, c, lvl, true);
-          else residual_level<3, double>(P, c, lvl, true);
-        }
-        double res;
-        minmax_boxes(P, c.i_tmp, true, false, true, &lo, &res);
-        if (!(res <= tol * rhs_max[j])) next.push_back(j);
-      }
+      for (int j : active)
+        if (!(key_max(j, 0) <= tol * rhs_max[j])) next.push_back(j);
       active.swap(next);
       if (active.empty()) break;
       if (cyc >= max_cycles)
         throw std::runtime_error("photoionization: Helmholtz solve did not converge");
       branches(P, active, [&](int j) {
         const Ctx c = term(j);
-        run_captured(P, call_key(2, mx - P.tree.min_lvl, 0, 0), [&] {
-          if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, mx, false);
-          else fas_vcycle<3, double>(P, c, mx, false);
+        run_captured(P, call_key(6, mx - P.tree.min_lvl, j, 0), [&] {
+          if (P.tree.dim == 2) {
+            fas_vcycle<2, double>(P, c, mx, false);
+            leaf_absmax<2>(P, c, true, res_slots(j));
+          } else {
+            fas_vcycle<3, double>(P, c, mx, false);
+            leaf_absmax<3>(P, c, true, res_slots(j));
+          }
         });
       });
+      read_keys();
       total += (int)active.size();
     }
     for (int j = 0; j < n_terms; j++) axpy_leaves(P, i_terms[j], v.pho, 1.0, true);

@@ -157,6 +157,10 @@ struct Pool {
   hipEvent_t ev_join[kMaxTerms - 1] = {};
   void ensure_side(int n);
   void sync_streams();
+  // 2 kRedSlots reduction keys per independent solve (allocated once and
+  // kept: captured graphs hold the pointer)
+  unsigned long long* d_term_red = nullptr;
+  void ensure_term_red();
 
   // instances (a5g_set_instances): the device level lists hold only the
   // boxes of active instances
