@@ -194,16 +194,42 @@ def _leaf_interiors(T, dim, iv):
     return np.concatenate(out)
 
 
+def _warm_cold_tight(r, dim):
+    """Each term's leaf values after a warm solve and a cold one at the
+    configured tolerance, then a cold one to 1e-12, all on the same state."""
+    out = []
+    tol = r.c.photoi_tol
+    for warm, t in ((True, tol), (False, tol), (False, 1e-12)):
+        r.c.photoi_warm, r.c.photoi_tol = warm, t
+        cyc = r.set_photoionization()
+        assert cyc > 0
+        out.append([_leaf_interiors(r.T, dim, iv) for iv in ST.I_PH_TERMS])
+    r.c.photoi_warm, r.c.photoi_tol = True, tol
+    return out
+
+
+def check_warm_as_accurate(warm, cold, tight):
+    """The warm start is as accurate as the cold solve at the same residual
+    tolerance: per term, its error against the tightly converged solution is
+    within 4 times the cold solve's, or below 1e-6 of the solution's size.
+    (On these refined meshes the discrete maximum principle does not bound
+    the error by max|r| / kappa^2: the refinement-boundary ghosts are not
+    monotone, and the 3D case is hundreds of times over that bound for both
+    starts.)"""
+    for j in range(len(tight)):
+        scale = np.abs(tight[j]).max()
+        ew = np.abs(warm[j] - tight[j]).max()
+        ec = np.abs(cold[j] - tight[j]).max()
+        assert ew <= max(4 * ec, 1e-6 * scale), (j, ew / scale, ec / scale)
+
+
 def test_warm_start_of_the_terms():
-    """Each term kept in its own variable: a warm call on an unchanged source
-    runs no V-cycle and gives i_pho bit for bit. After two step blocks
-    (the mesh refined, the term variables prolonged to the new boxes) a warm
-    call and a cold one (FMG from zero) on the same state agree term by term
-    within the error that the residual tolerance allows: for (L - kappa^2) S
-    = f the discrete maximum principle gives |S| <= max|f| / kappa^2, so two
-    solutions whose residuals are each at most tol max|rhs_j| differ by at
-    most 2 tol max|rhs_j| / kappa_j^2 (a factor 10 is left for the
-    coarse-fine interpolation, which is not monotone)."""
+    """Each term kept in its own variable: here a warm call on an unchanged
+    source runs no V-cycle (restricting the stored solution to the parents
+    and refilling its ghosts leaves the residual under the tolerance on this
+    mesh) and gives i_pho bit for bit. After two step blocks (the mesh
+    refined, the term variables prolonged to the new boxes) the warm start
+    is as accurate as a cold one (FMG from zero)."""
     cfg = ST.Config(variant="2d", photoi=True, lvls_max=6)
     r = ST.StreamerRun(O.OStreamTree, cfg)
     r.initialise()
@@ -211,18 +237,6 @@ def test_warm_start_of_the_terms():
     assert r.set_photoionization() == 0
     assert np.array_equal(_leaf_interiors(r.T, 2, ST.I_PHO), p0)
     for _ in range(2):
-        n_add, n_rm = r.step_block()
+        r.step_block()
     assert r.photoi_solved
-    warm_cyc = r.set_photoionization()
-    warm = [_leaf_interiors(r.T, 2, iv) for iv in ST.I_PH_TERMS]
-    coeff, amp, kappa = cfg.photoi_model()
-    # i_rhs[j] holds -amp_j I on leaves after the solves
-    rate = np.abs(_leaf_interiors(r.T, 2, ST.I_PH_RHS[0])).max() / amp[0]
-    r.c.photoi_warm = False
-    cold_cyc = r.set_photoionization()
-    cold = [_leaf_interiors(r.T, 2, iv) for iv in ST.I_PH_TERMS]
-    assert 0 < warm_cyc <= 40 and cold_cyc > 0
-    for j in range(3):
-        bound = 2 * cfg.photoi_tol * amp[j] * rate / kappa[j] ** 2
-        assert np.abs(warm[j] - cold[j]).max() <= 10 * bound, j
-    assert np.abs(warm[0]).max() > 1e3 * 10 * 2 * cfg.photoi_tol * amp[0] * rate / kappa[0] ** 2
+    check_warm_as_accurate(*_warm_cold_tight(r, 2))
