@@ -178,6 +178,33 @@ void Pool::sync_topology() {
     off_parents[l] = host.size();
     host.insert(host.end(), L_parents[l].begin(), L_parents[l].end());
   }
+  // every group's lists after the active ones
+  const int ng = n_inst ? n_groups : 0;
+  auto split = [&](const std::vector<std::vector<int>>& src,
+                   std::vector<std::vector<std::vector<int>>>& dst) {
+    dst.assign(ng, std::vector<std::vector<int>>(nl));
+    if (ng == 0) return;  // no groups: inst_group may be empty
+    for (int l = 0; l < nl; l++)
+      for (int id : src[l]) {
+        const int g = inst_group[inst_of(id)];
+        if (g >= 0) dst[g][l].push_back(id);
+      }
+  };
+  split(tree.ids, g_ids);
+  split(tree.leaves, g_leaves);
+  split(tree.parents, g_parents);
+  g_off_ids.assign(ng, std::vector<size_t>(nl));
+  g_off_leaves.assign(ng, std::vector<size_t>(nl));
+  g_off_parents.assign(ng, std::vector<size_t>(nl));
+  for (int g = 0; g < ng; g++)
+    for (int l = 0; l < nl; l++) {
+      g_off_ids[g][l] = host.size();
+      host.insert(host.end(), g_ids[g][l].begin(), g_ids[g][l].end());
+      g_off_leaves[g][l] = host.size();
+      host.insert(host.end(), g_leaves[g][l].begin(), g_leaves[g][l].end());
+      g_off_parents[g][l] = host.size();
+      host.insert(host.end(), g_parents[g][l].begin(), g_parents[g][l].end());
+    }
   if (host.size() > lists_cap || !d_lists) {
     lists_cap = std::max(host.size(), lists_cap * 3 / 2) + 64;
     realloc_dev(&d_lists, lists_cap);
@@ -1182,6 +1209,8 @@ int a5g_set_instances(a5g_tree* t, int n_inst, double stride) {
   if (P.d_inst) a5g::hcheck(hipFree(P.d_inst), "hipFree");
   P.d_inst = nullptr;
   P.n_inst = n_inst;
+  P.inst_group.clear();
+  P.n_groups = 0;
   P.inst_stride = n_inst ? stride : 0.0;
   P.h_inst.assign((size_t)n_inst * a5g::kInstVals, 0.0);
   P.inst_active.assign(n_inst, 1);
@@ -1228,6 +1257,35 @@ int a5g_set_active_instances(a5g_tree* t, int n, const int* active) {
   if (active && n != P.n_inst) throw std::invalid_argument("set_active_instances: one flag per instance");
   for (int g = 0; g < P.n_inst; g++) P.inst_active[g] = active ? (active[g] != 0) : 1;
   P.lists_dirty = true;
+  API_END
+}
+
+int a5g_set_instance_groups(a5g_tree* t, int n, const int* group) {
+  API_BEGIN
+  NEED(t);
+  auto& P = t->P;
+  if (n != 0 && n != P.n_inst)
+    throw std::invalid_argument("instance groups: need one entry per instance (or n = 0)");
+  if (n > 0 && !group) throw std::invalid_argument("instance groups: null array");
+  std::vector<int> g(group, group + n);
+  int ng = 0;
+  for (int v : g) {
+    if (v < -1) throw std::invalid_argument("instance groups: a group is -1 or >= 0");
+    ng = std::max(ng, v + 1);
+  }
+  if (g != P.inst_group) {
+    P.inst_group = g;
+    P.n_groups = ng;
+    P.lists_dirty = true;
+  }
+  API_END
+}
+
+int a5g_fas_fmg_groups(a5g_tree* t, int set_residual, int first_call) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::mg_fmg_groups(t->P, set_residual != 0, first_call != 0);
   API_END
 }
 

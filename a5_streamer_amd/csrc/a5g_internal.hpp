@@ -14,6 +14,7 @@ void tree_copy_cc(Pool& P, int from, int to);
 void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
                   double* lo, double* hi);
 void mg_fmg(Pool& P, bool set_residual, bool first_call);
+void mg_fmg_groups(Pool& P, bool set_residual, bool first_call);
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);
 void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b);

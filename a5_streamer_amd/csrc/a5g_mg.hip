@@ -1855,8 +1855,10 @@ static uint64_t variant_fingerprint(Pool& P) {
   }
   num((uint64_t)(int64_t)P.tree.min_lvl); num(P.tree.max_lvl);
   for (int l = P.tree.min_lvl; l <= P.tree.max_lvl; l++) {
-    const int k = l - P.tree.min_lvl;
-    num(P.off_ids[k]); num(P.off_leaves[k]); num(P.off_parents[k]);
+    // offsets of the lists in use (the active ones, or one group's)
+    num((uint64_t)(P.ids(l) - P.d_lists));
+    num((uint64_t)(P.leaves(l) - P.d_lists));
+    num((uint64_t)(P.parents(l) - P.d_lists));
     num(P.n_ids(l)); num(P.n_leaves(l)); num(P.n_parents(l));
   }
   return h;
@@ -2543,6 +2545,38 @@ void photoi_helmholtz_terms(Pool& P, const StVars& v, double coeff, int n_terms,
   }
   restore();
   if (cycles_done) *cycles_done = total;
+}
+
+// mg$D_fas_fmg of every group of instances (a5g_set_instance_groups), each
+// group up to its own finest level, as if it were the only active one. The
+// groups share no boxes, so up to kMaxTerms of them run side by side.
+void mg_fmg_groups(Pool& P, bool set_residual, bool first_call) {
+  mg_check(P);
+  if (P.n_inst == 0 || P.n_groups < 1)
+    throw std::logic_error("fas_fmg_groups: no instance groups set");
+  std::vector<int> gs;
+  for (int g = 0; g < P.n_groups; g++) {
+    bool any = false;
+    for (const auto& l : P.g_ids[g]) any = any || !l.empty();
+    if (any) gs.push_back(g);
+  }
+  const Ctx c = ctx64(P);
+  for (size_t w = 0; w < gs.size(); w += kMaxTerms) {
+    const std::vector<int> wave(gs.begin() + w, gs.begin() + std::min(gs.size(), w + kMaxTerms));
+    branches(P, wave, [&](int g) {
+      P.cur_group = g;
+      try {
+        run_captured(P, call_key(1, set_residual, first_call, 0), [&] {
+          if (P.tree.dim == 2) fas_fmg<2, double>(P, c, set_residual, first_call);
+          else fas_fmg<3, double>(P, c, set_residual, first_call);
+        });
+      } catch (...) {
+        P.cur_group = -1;
+        throw;
+      }
+      P.cur_group = -1;
+    });
+  }
 }
 
 void gc_tree(Pool& P, int iv, int rb, int bc) {

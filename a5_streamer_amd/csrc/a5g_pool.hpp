@@ -175,14 +175,27 @@ struct Pool {
   // the level lists the device holds (all boxes, or those of active
   // instances); valid after sync_topology
   const std::vector<int>& lids(int l) const {
+    if (cur_group >= 0) return g_ids[cur_group][l - tree.min_lvl];
     return n_inst && !topo_dirty && !lists_dirty ? act_ids[l - tree.min_lvl] : tree.lv_ids(l);
   }
   const std::vector<int>& lleaves(int l) const {
+    if (cur_group >= 0) return g_leaves[cur_group][l - tree.min_lvl];
     return n_inst && !topo_dirty && !lists_dirty ? act_leaves[l - tree.min_lvl] : tree.lv_leaves(l);
   }
   const std::vector<int>& lparents(int l) const {
+    if (cur_group >= 0) return g_parents[cur_group][l - tree.min_lvl];
     return n_inst && !topo_dirty && !lists_dirty ? act_parents[l - tree.min_lvl] : tree.lv_parents(l);
   }
+  // Groups of instances (a5g_set_instance_groups): the group of every
+  // instance (-1: none). After sync_topology every group's level lists sit
+  // at offsets of their own in d_lists, and cur_group >= 0 (set only while
+  // one group's work is issued) makes the accessors return them, so groups
+  // of runs can be solved side by side.
+  std::vector<int> inst_group;
+  int n_groups = 0;
+  int cur_group = -1;
+  std::vector<std::vector<std::vector<int>>> g_ids, g_leaves, g_parents;
+  std::vector<std::vector<size_t>> g_off_ids, g_off_leaves, g_off_parents;
 
   int bs() const;
   int fs() const;
@@ -192,9 +205,18 @@ struct Pool {
   void apply_moves(const std::vector<int>& moves, int n_used);  // after tidy_up
   void release();
   // level list on the device
-  const int* ids(int l) const { return d_lists + off_ids[l - tree.min_lvl]; }
-  const int* leaves(int l) const { return d_lists + off_leaves[l - tree.min_lvl]; }
-  const int* parents(int l) const { return d_lists + off_parents[l - tree.min_lvl]; }
+  const int* ids(int l) const {
+    const int k = l - tree.min_lvl;
+    return d_lists + (cur_group >= 0 ? g_off_ids[cur_group][k] : off_ids[k]);
+  }
+  const int* leaves(int l) const {
+    const int k = l - tree.min_lvl;
+    return d_lists + (cur_group >= 0 ? g_off_leaves[cur_group][k] : off_leaves[k]);
+  }
+  const int* parents(int l) const {
+    const int k = l - tree.min_lvl;
+    return d_lists + (cur_group >= 0 ? g_off_parents[cur_group][k] : off_parents[k]);
+  }
   int n_ids(int l) const { return (int)lids(l).size(); }
   int n_leaves(int l) const { return (int)lleaves(l).size(); }
   int n_parents(int l) const { return (int)lparents(l).size(); }

@@ -570,6 +570,23 @@ module m_a5g
        integer(c_int)        :: s
      end function c_set_active_instances
 
+     function c_set_instance_groups(t, n, group) &
+          bind(C, name="a5g_set_instance_groups") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n
+       integer(c_int)        :: group(*)
+       integer(c_int)        :: s
+     end function c_set_instance_groups
+
+     function c_fas_fmg_groups(t, set_residual, first_call) &
+          bind(C, name="a5g_fas_fmg_groups") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: set_residual, first_call
+       integer(c_int)        :: s
+     end function c_fas_fmg_groups
+
      function c_prof_enable(t, on) bind(C, name="a5g_prof_enable") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -643,6 +660,7 @@ module m_a5g
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
   public :: a5g_set_instances, a5g_set_instance_values, a5g_set_active_instances
+  public :: a5g_set_instance_groups, mg_fas_fmg_groups
 
 contains
 
@@ -1285,5 +1303,25 @@ contains
     a = merge(1, 0, active)
     call check(c_set_active_instances(tree%h, size(a), a), "set_active_instances")
   end subroutine a5g_set_active_instances
+
+  !> Groups of runs for mg_fas_fmg_groups: group(g) >= 0 is run g's group,
+  !> -1 leaves it out; an empty array clears the groups
+  subroutine a5g_set_instance_groups(tree, group)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: group(:)
+    integer(c_int) :: a(max(size(group), 1))
+    a = 0
+    a(1:size(group)) = group
+    call check(c_set_instance_groups(tree%h, size(group), a), "set_instance_groups")
+  end subroutine a5g_set_instance_groups
+
+  !> mg$D_fas_fmg (afivo/src/m_aX_mg.f90:164-203) of every group of runs, each
+  !> up to its own finest level, the groups side by side
+  subroutine mg_fas_fmg_groups(tree, set_residual, first_call)
+    type(a5g_t), intent(inout) :: tree
+    logical, intent(in)        :: set_residual, first_call
+    call check(c_fas_fmg_groups(tree%h, merge(1, 0, set_residual), merge(1, 0, first_call)), &
+         "fas_fmg_groups")
+  end subroutine mg_fas_fmg_groups
 
 end module m_a5g

@@ -379,6 +379,21 @@ class Tree:
         a, p = _i(np.asarray(active, np.int32))
         _call("a5g_set_active_instances", self.h, C.c_int(len(a)), p)
 
+    def set_instance_groups(self, group=None):
+        """Groups of runs for fas_fmg_groups (group[g] >= 0, or -1 to leave
+        run g out; None clears them)."""
+        if group is None:
+            _call("a5g_set_instance_groups", self.h, C.c_int(0), None)
+            return
+        a, p = _i(np.asarray(group, np.int32))
+        _call("a5g_set_instance_groups", self.h, C.c_int(len(a)), p)
+
+    def fas_fmg_groups(self, set_residual, first_call):
+        """mg$D_fas_fmg of every group of runs up to its own finest level,
+        the groups side by side."""
+        _call("a5g_fas_fmg_groups", self.h, C.c_int(int(set_residual)),
+              C.c_int(int(first_call)))
+
     # --- output (afivo/src/m_aX_io.f90:20-209, m_vtk.f90) -------------------
     def leaf_cells(self, ixs_cc, ixs_fc=()):
         """a$D_write_vtk's cell data: array (n_cells, columns), leaves level

@@ -773,6 +773,9 @@ class BatchRun:
             raise ValueError("the %s driver is not batched" % c0.variant)
         self.cfgs = cfgs
         self.K = K = len(cfgs)
+        # field solves of runs with different finest levels side by side
+        # (fas_fmg_groups) rather than one group after the other
+        self.groups_side_by_side = True
         self.dim = D = v["dim"]
         shape = [c0.n_base] * D if np.isscalar(c0.n_base) else list(c0.n_base)
         self.n_last = shape[-1]
@@ -881,13 +884,22 @@ class BatchRun:
             r.voltage = vg
         active = self.active.copy()
         levels = sorted(set(self.inst_lvl[active].tolist()))
-        for m in levels:
-            if len(levels) > 1:
-                self._set_active(active & (self.inst_lvl == m))
+        if len(levels) > 1 and self.groups_side_by_side and hasattr(T, "fas_fmg_groups"):
+            # one group per finest level, the groups' FMGs side by side
+            group = np.full(self.K, -1, np.int32)
+            for gi, m in enumerate(levels):
+                group[active & (self.inst_lvl == m)] = gi
+            T.set_instance_groups(group)
             for i in range(N_FMG_CYCLES):
-                T.fas_fmg(False, no_guess and i == 0)
-        if len(levels) > 1:
-            self._set_active(active)
+                T.fas_fmg_groups(False, no_guess and i == 0)
+        else:
+            for m in levels:
+                if len(levels) > 1:
+                    self._set_active(active & (self.inst_lvl == m))
+                for i in range(N_FMG_CYCLES):
+                    T.fas_fmg(False, no_guess and i == 0)
+            if len(levels) > 1:
+                self._set_active(active)
         T.fld_from_pot()
         T.gc_tree(I_FLD, RB_INTERP, BC_NEUMANN)
 

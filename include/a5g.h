@@ -247,6 +247,15 @@ int a5g_set_instance_values(a5g_tree *t, int what, int n, const double *values);
  * runs with active[g] != 0; active NULL makes all runs active. Calls given
  * explicit box lists are not affected. */
 int a5g_set_active_instances(a5g_tree *t, int n, const int *active);
+/* Groups of runs for a5g_fas_fmg_groups: group[g] >= 0 is run g's group, -1
+ * leaves it out; n = 0 clears the groups. No reference counterpart (one run
+ * per process there). */
+int a5g_set_instance_groups(a5g_tree *t, int n, const int *group);
+/* mg$D_fas_fmg (afivo/src/m_aX_mg.f90:164-203) of every group of runs, each
+ * up to its own finest level as if it were the only active group, the
+ * groups side by side on the GPU (they share no boxes). Gives each run what
+ * a5g_fas_fmg gives it with only its group active. */
+int a5g_fas_fmg_groups(a5g_tree *t, int set_residual, int first_call);
 
 /* ---- streamer diagnostics -------------------------------------------- */
 
