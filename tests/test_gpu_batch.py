@@ -64,3 +64,28 @@ def test_batch_matches_separate_runs(oracle_lib):
                 np.testing.assert_array_equal(mine[k], ref[k], err_msg="block %d run %d box %s"
                                               % (blk, g, k))
     assert list(batch.leaf_cells()) == [r.leaf_cells() for r in solo]
+
+
+def test_groups_side_by_side_match_groups_in_turn():
+    """The field solves of run groups side by side (a5g_fas_fmg_groups) give
+    every box, ghosts included, the doubles that the groups solved one after
+    the other with only that group active give. The runs reach different
+    finest levels, so the grouped path is taken."""
+    import a5_streamer_amd as A
+    print("batch groups side by side vs in turn", flush=True)
+    runs = []
+    for side in (True, False):
+        r = S.BatchRun(A.Tree, _cfgs())
+        r.groups_side_by_side = side
+        r.initialise()
+        for _ in range(2):
+            r.step_block()
+        runs.append(r)
+    assert len(set(runs[0].inst_lvl.tolist())) > 1
+    b = [r.T.boxes() for r in runs]
+    assert np.array_equal(b[0]["in_use"], b[1]["in_use"])
+    for i in np.nonzero(b[0]["in_use"])[0]:
+        for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD, S.I_RHS):
+            np.testing.assert_array_equal(runs[0].T.get_cc(int(i) + 1, iv),
+                                          runs[1].T.get_cc(int(i) + 1, iv),
+                                          err_msg="box %d var %d" % (i + 1, iv))
