@@ -302,7 +302,8 @@ void Pool::prof_end(int idx) {
 
 void Pool::clear_graphs() {
   if (!graphs.empty()) {
-    if (stream) (void)hipStreamSynchronize(stream);
+    if (home) (void)hipStreamSynchronize(home);
+    if (stream && stream != home) (void)hipStreamSynchronize(stream);
     for (auto s : side)
       if (s) (void)hipStreamSynchronize(s);
   }
@@ -327,7 +328,8 @@ void Pool::ensure_term_red() {
 }
 
 void Pool::sync_streams() {
-  if (stream) hcheck(hipStreamSynchronize(stream), "stream sync");
+  if (home) hcheck(hipStreamSynchronize(home), "stream sync");
+  if (stream && stream != home) hcheck(hipStreamSynchronize(stream), "stream sync");
   for (auto s : side)
     if (s) hcheck(hipStreamSynchronize(s), "stream sync");
 }
@@ -375,9 +377,10 @@ void Pool::release() {
   }
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   ev_fork = nullptr;
-  if (stream) (void)hipStreamDestroy(stream);
+  if (home) (void)hipStreamDestroy(home);
   cc = fc = nullptr;
   stream = nullptr;
+  home = nullptr;
 }
 
 }  // namespace a5g
@@ -475,6 +478,7 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       h->P.host_only = true;
     } else {
       a5g::hcheck(hipStreamCreateWithFlags(&h->P.stream, hipStreamNonBlocking), "stream");
+      h->P.home = h->P.stream;
       // A5G_GRAPHS=0 runs every multigrid call as plain launches
       const char* g = std::getenv("A5G_GRAPHS");
       h->P.graphs_on = !(g && g[0] == '0');

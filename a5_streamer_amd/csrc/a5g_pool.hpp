@@ -94,6 +94,9 @@ struct Pool {
   // call that touches cell data fails.
   bool host_only = false;
   hipStream_t stream = nullptr;
+  // the handle's own stream; `stream` is a side stream while independent
+  // work is issued there
+  hipStream_t home = nullptr;
   int dev_cap = 0;  // boxes allocated on the device (slots 0..dev_cap)
   double* cc = nullptr;
   double* fc = nullptr;
