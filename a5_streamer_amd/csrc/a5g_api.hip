@@ -482,6 +482,14 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // A5G_GRAPHS=0 runs every multigrid call as plain launches
       const char* g = std::getenv("A5G_GRAPHS");
       h->P.graphs_on = !(g && g[0] == '0');
+      const char* sb = std::getenv("A5G_STREAM_BYTES");
+      if (sb && sb[0]) {
+        char* end = nullptr;
+        const double v = std::strtod(sb, &end);
+        if (end == sb || *end != 0 || !(v >= 0))
+          throw std::invalid_argument("A5G_STREAM_BYTES must be a byte count >= 0");
+        h->P.stream_bytes = v;
+      }
       a5g::hcheck(hipMalloc((void**)&h->P.d_red, 2 * a5g::kRedSlots * sizeof(unsigned long long)),
                   "hipMalloc");
       h->P.ensure_capacity(h->P.tree.cap);
@@ -752,7 +760,9 @@ int a5g_mg_setup(a5g_tree* t, int i_phi, int i_tmp, int i_rhs, int i_eps, int n_
   P.mg.auto_ops = auto_ops;
   P.mg.helm = 0;  // a new setup starts from the Laplacian
   for (int nb = 0; nb < 2 * P.tree.dim; nb++) {
-    if (bc_kind[nb] < 0 || bc_kind[nb] > 2) throw std::invalid_argument("bad boundary kind");
+    if (bc_kind[nb] < 0 || bc_kind[nb] > 3) throw std::invalid_argument("bad boundary kind");
+    if (bc_kind[nb] == a5g::kBcRow2V && (P.tree.dim != 2 || nb < 2))
+      throw std::invalid_argument("boundary kind 3 (whole ghost row) is for the y faces of a 2D tree");
     P.bc_kind[nb] = bc_kind[nb];
     P.bc_val[nb] = bc_val ? bc_val[nb] : 0.0;
   }

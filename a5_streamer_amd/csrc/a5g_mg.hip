@@ -43,9 +43,6 @@ struct Geo {
   int tpb, grid;
 };
 
-// Bytes of potential and rhs above which a level's half-sweeps stream
-// (three times the Infinity Cache)
-constexpr double kStreamBytes = 768.0 * (1 << 20);
 
 static Geo geo(int n_boxes, int work) {
   int tpb = std::min(256, std::max(64, pow2ceil(work)));
@@ -434,7 +431,7 @@ static bool launch_gsrb_lds_t(Pool& P, const Dev& a, const int* list, int n, int
   // Smaller levels stay cached from one half-sweep to the next.
   const double box_cells = D == 2 ? (nc + 2.0) * (nc + 2) : (nc + 2.0) * (nc + 2) * (nc + 2);
   const double level_bytes = 2.0 * n * box_cells * sizeof(T);
-  const bool nt = level_bytes > kStreamBytes;
+  const bool nt = level_bytes > P.stream_bytes;
   switch (nc) {
     case 2: go(k_gsrb_lds<D, CYL, 2, T, EPS, false>, ncol); return true;
     case 4: go(k_gsrb_lds<D, CYL, 4, T, EPS, false>, ncol); return true;
@@ -584,6 +581,30 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
       if (x[u] < 0) continue;
       if (bnb[u] > 0) v[u] = gc_boundary<D, T>(a, p, iv, id, nc, bnb[u], gg[u], rb, bc);
       p[x[u]] = v[u];
+    }
+  }
+  // The rule 2V - c over a whole y ghost row (the cylindrical driver's
+  // sides_bc_pot, src/streamer_cyl.f90:408-410, cc(:, nc+1) = 2V - cc(:, nc))
+  // also writes the two corner ghosts, from the x ghosts that the same
+  // a2_gc_box call has just filled (faces go in the order lx, hx, ly, hy).
+  // Lanes 0, 1 take the corners of face ly, lanes 2, 3 those of hy; each
+  // recomputes its x ghost from the interior cells it is made of.
+  if (D == 2 && bc == 0 && lt < 4) {
+    const int nb = 3 + (lt >> 1);
+    if (a.nbs[id * 6 + nb - 1] < 0 && a.bc_kind[nb - 1] == kBcRow2V) {
+      const int side = lt & 1;
+      const int xnb = 1 + side;
+      const int g[3] = {side ? nc + 1 : 0, nb == 3 ? 1 : nc, 0};
+      const int xid = a.nbs[id * 6 + xnb - 1];
+      T xg;
+      if (xid > 0) {
+        xg = boxp_t<T>(a, iv, xid)[cidx_in<D>(nc, side ? 1 : nc, g[1], 0)];
+      } else {
+        xg = gc_boundary<D, T>(a, p, iv, id, nc, xnb, g, rb, bc);
+      }
+      const double V = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstBc + nb - 1]
+                                : a.bc_val[nb - 1];
+      p[cidx<D>(nc, g[0], nb == 3 ? 0 : nc + 1, 0)] = T(2) * (T)V - xg;
     }
   }
 }

@@ -52,6 +52,10 @@ struct Dev {
 constexpr int kInstVals = 8;
 constexpr int kInstDt = 0, kInstKorenFac = 1, kInstBc = 2;
 
+// Physical-face rules of the potential (bc_kind): copy, -c, 2V - c on the
+// face, and 2V - c over the whole ghost row of a 2D y face, corners included
+constexpr int kBcCopy = 0, kBcNeg = 1, kBc2V = 2, kBcRow2V = 3;
+
 // Per-launch timing with HIP events on the pool's stream (kernel classes
 // below), used by the benchmark to measure kernel durations in place.
 enum ProfKind { PK_GSRB = 0, PK_GC, PK_RESID_RSTR, PK_COARSE_RHS, PK_CORR, PK_COPY, PK_OTHER,
@@ -148,6 +152,11 @@ struct Pool {
   // state their kernel arguments were taken from (device pointers, level
   // lists, options) is unchanged; graphs_state fingerprints that state.
   bool graphs_on = true;
+  // Bytes of potential and rhs of a level above which its half-sweeps
+  // stream (non-temporal rhs loads and stores of the new colour): three
+  // times the 256 MB Infinity Cache. A5G_STREAM_BYTES at a5g_init sets
+  // another threshold (0: every level of 16^3 or 32^2 boxes streams).
+  double stream_bytes = 768.0 * (1 << 20);
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   // waits for every stream of the handle, then destroys the graphs (one may

@@ -44,7 +44,10 @@ F_ELEC, F_FLD = 1, 2
 T_MOB, T_DIF, T_ALPHA, T_ETA = 1, 2, 3, 4
 RB_MG, RB_INTERP, RB_INTERP_LIM = 0, 1, 2   # mg_sides_rb, gc_interp, gc_interp_lim
 BC_MG, BC_NEUMANN = 0, 1                    # the solver's sides_bc, gc_neumann
-BC_COPY, BC_NEG, BC_2V = 0, 1, 2            # sides_bc_pot rules per face
+# sides_bc_pot rules per face: copy, -c, 2V - c on the face, and 2V - c over
+# the whole ghost row with its corner ghosts (the cylindrical driver's high
+# z face, src/streamer_cyl.f90:408-410)
+BC_COPY, BC_NEG, BC_2V, BC_ROW_2V = 0, 1, 2, 3
 N_FMG_CYCLES = 1                            # src/m_streamer.f90:46
 
 
@@ -209,11 +212,12 @@ class Table:
 # per-driver constants (the lines in each src/streamer_*.f90 that differ)
 VARIANTS = {
     "2d": dict(dim=2, coord=0, n_cycle_base=8, init_window=2.0e-9, crv_refine=1.0,
-               exit_on_rm=False, time_in_substep=False, dielectric_rule=True),
+               exit_on_rm=False, time_in_substep=False, dielectric_rule=True, bc_top=BC_2V),
     "3d": dict(dim=3, coord=0, n_cycle_base=8, init_window=2.5e-9, crv_refine=0.1,
-               exit_on_rm=True, time_in_substep=False, dielectric_rule=True),
+               exit_on_rm=True, time_in_substep=False, dielectric_rule=True, bc_top=BC_2V),
     "cyl": dict(dim=2, coord=1, n_cycle_base=4, init_window=5.0e-9, crv_refine=0.0,
-                exit_on_rm=False, time_in_substep=True, dielectric_rule=False),
+                exit_on_rm=False, time_in_substep=True, dielectric_rule=False,
+                bc_top=BC_ROW_2V),
 }
 
 
@@ -354,7 +358,7 @@ class StreamerRun:
         # axis, 2V - phi at its high end
         self.voltage = -c.domain_len * c.applied_fld
         kinds = [BC_COPY] * (2 * self.dim)
-        kinds[-2], kinds[-1] = BC_NEG, BC_2V
+        kinds[-2], kinds[-1] = BC_NEG, v["bc_top"]
         vals = [0.0] * (2 * self.dim)
         vals[-1] = self.voltage
         T.mg_setup(I_PHI, I_FLD, I_RHS, i_eps=I_EPS, n_base=v["n_cycle_base"],
@@ -806,7 +810,7 @@ class BatchRun:
             r.tbl = tbl
         T.streamer_setup(dielectric_rule=v["dielectric_rule"], pho=-1)
         kinds = [BC_COPY] * (2 * D)
-        kinds[-2], kinds[-1] = BC_NEG, BC_2V
+        kinds[-2], kinds[-1] = BC_NEG, VARIANTS["cyl"]["bc_top"]
         T.mg_setup(I_PHI, I_FLD, I_RHS, i_eps=I_EPS, n_base=v["n_cycle_base"],
                    bc_kind=np.array(kinds, np.int32), bc_val=np.zeros(2 * D), auto_ops=True)
         T.set_instances(K, self.H)

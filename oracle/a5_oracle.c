@@ -884,7 +884,8 @@ static void bc_dirichlet(otree *t, int id, int nb, int iv) {
 static void bc_mg(otree *t, int id, int nb, int iv) {
   /* The three forms of the drivers' potential boundary callbacks
    * (src/streamer_2d.f90:620-638, src/streamer_3d.f90:693-715,
-   * src/streamer_cyl.f90:395-413): copy, -c, 2V - c */
+   * src/streamer_cyl.f90:395-413): copy, -c, 2V - c on the face (kinds 0-2),
+   * and 2V - c over the whole ghost row of a 2D y face (kind 3) */
   double *a = CC(t, id, iv);
   int kind = t->bc_kind[nb - 1];
   double v = t->bc_val[nb - 1];
@@ -893,6 +894,14 @@ static void bc_mg(otree *t, int id, int nb, int iv) {
   else if (kind == 1) a[g] = -a[c];
   else a[g] = 2 * v - a[c];
   FACE_LOOP_END
+  if (kind == 3 && t->dim == 2 && nb_dim(nb) == 1) {
+    /* the cylindrical driver's rule covers the whole ghost row,
+     * cc(:, nc+1) = 2V - cc(:, nc) (src/streamer_cyl.f90:408-410): the two
+     * corner ghosts too, from the x ghosts of this pass */
+    int nc = t->ncell[id], gy = nb_low(nb) ? 0 : nc + 1, cy = nb_low(nb) ? 1 : nc;
+    a[I2(nc, 0, gy)] = 2 * v - a[I2(nc, 0, cy)];
+    a[I2(nc, nc + 1, gy)] = 2 * v - a[I2(nc, nc + 1, cy)];
+  }
 }
 
 static void mg_sides_rb(otree *t, int id, int nb, int iv) {

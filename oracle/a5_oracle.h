@@ -15,10 +15,13 @@
  *    located reduction: bit for bit against the reference compiled from its
  *    own sources (oracle/_ref; tests/test_oracle_ref.py,
  *    tests/test_oracle_diag.py);
- *  - the streamer box procedures the reference cannot link (field,
- *    fluxes, update, averaging, space charge, refinement maxima): closed
- *    forms and conservation (tests/test_oracle_math.py,
- *    tests/test_driver_host.py);
+ *  - the streamer box procedures (space charge, field, fluxes, update,
+ *    averaging, photoionization rate, refinement maxima): bit for bit
+ *    against the reference drivers' own internal procedures and main loop,
+ *    built from src/streamer_{2d,3d,cyl}.f90 into oracle/_ref as modules
+ *    (tests/test_oracle_ref_drivers.py), on random data (every branch of
+ *    the limiter and of the flux clamp) and over whole runs; also
+ *    conservation (tests/test_driver_host.py);
  *  - the Helmholtz operator and photoionization (no reference
  *    counterpart): smoother closed form, direct sparse solve, manufactured
  *    solution and far field (tests/test_oracle_helmholtz.py);

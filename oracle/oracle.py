@@ -376,6 +376,10 @@ def average_dens(T, v):
     T.L.o_average_dens(T.h, C.byref(v))
 
 
+def photoi_rate(T, v, tbl, coeff):
+    T.L.o_photoi_rate(T.h, C.byref(v), tbl.h, C.c_double(coeff))
+
+
 def box_summaries(T, v):
     n = T.info()["max_id"]
     out = np.zeros((n, 3))

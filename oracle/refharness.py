@@ -61,6 +61,13 @@ class RefTree:
                        C.c_int(lvls_max), C.c_int(coord), C.c_int(n_boxes))
         self.n_cell = n_cell
 
+    @classmethod
+    def view(cls, dim, n_cell):
+        """A view of the harness tree of dimension dim as it is (no a$D_init)."""
+        t = cls.__new__(cls)
+        t.dim, t.p, t.L, t.n_cell = dim, "ref%d_" % dim, lib(), n_cell
+        return t
+
     def f(self, name):
         return getattr(self.L, self.p + name)
 
@@ -309,3 +316,93 @@ def td_read(fname, gas, name, max_n=1000):
                       name.encode() + b"\0", C.c_int(max_n), C.byref(n),
                       _d(xx)[1], _d(yy)[1])
     return xx[:n.value], yy[:n.value]
+
+
+# --- the reference's driver programs (oracle/ref_drivers.f90) ---------------
+
+DRIVER_VARIANTS = {"2d": 1, "3d": 2, "cyl": 3}
+
+
+class RefDriver:
+    """One of the reference's driver programs (src/streamer_2d.f90,
+    src/streamer_3d.f90, src/streamer_cyl.f90) built as a module: its box
+    procedures act on the harness tree of its dimension (a RefTree), and
+    run() executes its main program from init_tree to the end of its time
+    loop on its own tree, which then becomes the harness tree."""
+
+    def __init__(self, variant):
+        self.variant = variant
+        self.vid = DRIVER_VARIANTS[variant]
+        self.dim = 3 if variant == "3d" else 2
+        self.L = lib()
+        self.L.refdrv_time.restype = C.c_double
+        self.L.refdrv_max_dt.restype = C.c_double
+
+    def config(self, domain_len, box_size, applied_fld, dt_max=1e-8, end_time=0.0,
+               dt_out=1e99, epsilon_diel=1.0, gas_pressure=1.0, fld_mod_t0=1e99,
+               fld_sin_amplitude=0.0, fld_sin_freq=0.2e9, fld_lin_deriv=0.0, time=0.0,
+               steps_amr=2, photoi=False):
+        d = np.array([domain_len, applied_fld, dt_max, end_time, dt_out, epsilon_diel,
+                      gas_pressure, fld_mod_t0, fld_sin_amplitude, fld_sin_freq,
+                      fld_lin_deriv, time], float)
+        iv = np.array([box_size, steps_amr, int(bool(photoi))], np.int32)
+        self.L.refdrv_config(_d(d)[1], _i(iv)[1])
+
+    def init_cond(self, bg_dens, r0, r1, dens, width, falloff):
+        r0 = np.asarray(r0, float).reshape(-1, self.dim)
+        r1 = np.asarray(r1, float).reshape(-1, self.dim)
+        n = r0.shape[0]
+        # (n, dim) row-major has the memory layout of Fortran's r0(dim, n)
+        self.L.refdrv_init_cond(C.c_int(self.dim), C.c_int(n), C.c_double(bg_dens),
+                                _d(r0)[1], _d(r1)[1], _d(np.asarray(dens, float))[1],
+                                _d(np.asarray(width, float))[1],
+                                _i(np.asarray(falloff))[1])
+
+    def table(self, n_rows, max_fld, x, cols):
+        x = np.asarray(x, float)
+        # cols(n, 4) column-major = the four columns one after the other
+        c = np.ascontiguousarray(np.stack([np.asarray(y, float) for y in cols]))
+        self.L.refdrv_table(C.c_int(n_rows), C.c_double(max_fld), C.c_int(len(x)),
+                            _d(x)[1], _d(c)[1])
+
+    def time(self):
+        dt = C.c_double()
+        t = self.L.refdrv_time(C.byref(dt))
+        return t, dt.value
+
+    def run(self, n_cell):
+        self.L.refdrv_run(C.c_int(self.vid))
+        return RefTree.view(self.dim, n_cell)
+
+    def mg_init(self):
+        self.L.refdrv_mg_init(C.c_int(self.vid))
+
+    def compute_fld(self, n_cycles, no_guess):
+        self.L.refdrv_compute_fld(C.c_int(self.vid), C.c_int(n_cycles),
+                                  C.c_int(int(bool(no_guess))))
+
+    def fld_from_pot(self):
+        self.L.refdrv_fld_from_pot(C.c_int(self.vid))
+
+    def fluxes_koren(self, dt):
+        self.L.refdrv_fluxes_koren(C.c_int(self.vid), C.c_double(dt))
+
+    def update_solution(self, dt):
+        self.L.refdrv_update_solution(C.c_int(self.vid), C.c_double(dt))
+
+    def average_dens(self):
+        self.L.refdrv_average_dens(C.c_int(self.vid))
+
+    def photoi_rate(self, coeff):
+        self.L.refdrv_photoi_rate(C.c_int(self.vid), C.c_double(coeff))
+
+    def set_init_cond(self):
+        self.L.refdrv_set_init_cond(C.c_int(self.vid))
+
+    def ref_flags(self, n):
+        out = np.zeros(n, np.int32)
+        self.L.refdrv_ref_flags(C.c_int(self.vid), C.c_int(n), _i(out)[1])
+        return out
+
+    def max_dt(self):
+        return self.L.refdrv_max_dt(C.c_int(self.vid))

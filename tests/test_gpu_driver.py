@@ -38,6 +38,21 @@ CASES = {
     # field solve changes
     "2d_modulated": dict(variant="2d", lvls_max=6, fld_mod_t0=0.0, fld_lin_deriv=1e15,
                          fld_sin_amplitude=1e5),
+    # the drivers' own set-up (one base box, 30 levels allowed): the
+    # configurations whose runs the oracle matches against the reference's
+    # driver programs (tests/test_oracle_ref_drivers.py)
+    "2d_one_base": dict(variant="2d", n_base=1, lvls_max=30),
+    "cyl_one_base": dict(variant="cyl", n_base=1, lvls_max=30, domain_len=16e-3,
+                         seed_r0=[[0.0, 12e-3]], seed_r1=[[0.0, 12e-3]], applied_fld=2.5e6),
+    "3d_one_base": dict(variant="3d", n_base=1, lvls_max=30, domain_len=8e-3, n_boxes=5000,
+                        seed_r0=[[4e-3, 4e-3, 5e-3]], seed_r1=[[4e-3, 4e-3, 6e-3]],
+                        seed_width=[1.0e-3], applied_fld=2.5e6, epsilon_diel=1.5),
+    # the production 3D streamer (64 mm cube of one base box of 8^3 cells,
+    # 2.0 MV/m, the 2D seed moved to the top, Helmholtz photoionization) on
+    # six levels instead of its full depth
+    "3d_production_6lvl": dict(variant="3d", domain_len=64e-3, n_base=1, lvls_max=6,
+                               applied_fld=2.0e6, seed_r0=[[32e-3, 32e-3, 56e-3]],
+                               seed_r1=[[32e-3, 32e-3, 56e-3]], photoi=True, n_boxes=5000),
 }
 
 
@@ -69,7 +84,7 @@ def test_run_matches_oracle(oracle_lib, name):
         r.initialise()
     dim = runs[0].dim
     _same_state(runs[0].T, runs[1].T, dim)
-    n_blocks = 2 if name == "3d" else 4
+    n_blocks = 2 if runs[0].dim == 3 else 4
     changed = 0
     for blk in range(n_blocks):
         pre = [(r.max_dt(), r.time, r.T.tree_minmax(S.I_FLD)) for r in runs]
