@@ -345,6 +345,20 @@ void Pool::ensure_diag(int n) {
   diag_cap = cap;
 }
 
+void* Pool::work(size_t bytes) {
+  if (bytes > work_bytes || !d_work) {
+    // no kernel of an earlier call may still read the old buffer: hipFree
+    // waits for the device
+    if (d_work) hcheck(hipFree(d_work), "hipFree work");
+    d_work = nullptr;
+    work_bytes = 0;
+    const size_t want = std::max(bytes + bytes / 2, (size_t)1 << 16);
+    hcheck(hipMalloc(&d_work, want), "hipMalloc work");
+    work_bytes = want;
+  }
+  return d_work;
+}
+
 void Pool::release() {
   clear_graphs();
   for (auto& r : prof) {
@@ -365,6 +379,9 @@ void Pool::release() {
   f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag); f(d_inst);
   f(d_term_red);
   d_term_red = nullptr;
+  f(d_work);
+  d_work = nullptr;
+  work_bytes = 0;
   d_inst = nullptr;
   d_flux_pairs = nullptr;
   d_diag = nullptr;
@@ -492,6 +509,7 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       }
       a5g::hcheck(hipMalloc((void**)&h->P.d_red, 2 * a5g::kRedSlots * sizeof(unsigned long long)),
                   "hipMalloc");
+      a5g::noalloc_init();
       h->P.ensure_capacity(h->P.tree.cap);
     }
   } catch (...) {
@@ -1096,6 +1114,15 @@ int a5g_box_summaries(a5g_tree* t, double* out) {
   NEED(t);
   need_gpu(t->P);
   a5g::box_summaries(t->P, t->P.sv, out);
+  API_END
+}
+
+int a5g_box_summaries_noalloc(a5g_tree* t, double* out) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  if (!out) throw std::invalid_argument("box_summaries_noalloc: null output");
+  a5g::box_summaries_noalloc(t->P, t->P.sv, out);
   API_END
 }
 

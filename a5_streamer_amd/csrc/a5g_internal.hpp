@@ -32,6 +32,12 @@ void consistent_fluxes(Pool& P, int f_elec);
 void update_solution(Pool& P, const StVars& v, double dt);
 void average_dens(Pool& P, const StVars& v);
 void box_summaries(Pool& P, const StVars& v, double* out_host);
+// the same with the library's fixed device scratch instead of allocations
+// (a5g_box_summaries_noalloc); noalloc_init looks the scratch up for the
+// current device
+constexpr int kNoAllocMaxBoxes = 131072;
+void noalloc_init();
+void box_summaries_noalloc(Pool& P, const StVars& v, double* out_host);
 void photoi_rate(Pool& P, const StVars& v, double coeff);
 void axpy_leaves(Pool& P, int iin, int iout, double s, bool add);
 void photoi_helmholtz(Pool& P, const StVars& v, double coeff, int n_terms, const double* amp,

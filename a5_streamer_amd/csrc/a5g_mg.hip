@@ -1789,10 +1789,9 @@ static void ensure_tags(Pool& P) {
     if (T.in_use[id] && T.tag[id] == kTagInit) todo.push_back(id);
   if (todo.empty()) return;
   const int n = (int)todo.size();
-  int* d_list = nullptr;
-  double* d_out = nullptr;
-  check(hipMalloc((void**)&d_list, n * sizeof(int)), "hipMalloc tags");
-  check(hipMalloc((void**)&d_out, 2 * (size_t)n * sizeof(double)), "hipMalloc tags");
+  // the pool's work buffer: the ranges first, then the id list
+  double* d_out = (double*)P.work(2 * (size_t)n * sizeof(double) + n * sizeof(int));
+  int* d_list = (int*)(d_out + 2 * (size_t)n);
   std::vector<double> out(2 * (size_t)n);
   check(hipMemcpyAsync(d_list, todo.data(), n * sizeof(int), hipMemcpyHostToDevice, P.stream),
         "tag list");
@@ -1806,8 +1805,6 @@ static void ensure_tags(Pool& P) {
   check(hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost,
                        P.stream), "tag range");
   check(hipStreamSynchronize(P.stream), "tag sync");
-  check(hipFree(d_list), "hipFree");
-  check(hipFree(d_out), "hipFree");
   for (int q = 0; q < n; q++) {
     const double lo = out[2 * q], hi = out[2 * q + 1];
     int tag = kTagNormal;

@@ -133,6 +133,13 @@ struct Pool {
   double* d_diag = nullptr;
   int diag_cap = 0;
   void ensure_diag(int n);
+  // Device buffer for the short-lived data of one call (per-box outputs,
+  // id lists uploaded for a kernel): grown when a call needs more, kept
+  // between calls and freed in release(), so that the stepping loop does
+  // not allocate and free (hipFree waits for the whole device)
+  void* d_work = nullptr;
+  size_t work_bytes = 0;
+  void* work(size_t bytes);
   MgOpts mg;
   StVars sv;
   // transport lookup table on the device: rows_cols(n_rows, n_cols)

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -836,14 +837,22 @@ void axpy_leaves(Pool& P, int iin, int iout, double s, bool add) {
   }
 }
 
-void box_summaries(Pool& P, const StVars& v, double* out_host) {
-  P.sync_topology();
+// out[0..n) = 0
+__global__ void k_zero(double* __restrict__ out, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    out[q] = 0.0;
+}
+
+// The per-box maxima of every box of levels >= 1 into d_out (3 max_id
+// values, zero for other boxes), then copied to the host.
+static void summaries_into(Pool& P, const StVars& v, double* d_out, double* out_host) {
   const auto& T = P.tree;
   const int D = T.dim;
-  double* d_out = nullptr;
-  const size_t bytes = (size_t)std::max(T.max_id, 1) * 3 * sizeof(double);
-  chk(hipMalloc((void**)&d_out, bytes), "hipMalloc");
-  chk(hipMemsetAsync(d_out, 0, bytes, P.stream), "memset");
+  const size_t n_out = (size_t)std::max(T.max_id, 1) * 3;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n_out + 255) / 256, 1024)),
+                     dim3(256), 0, P.stream, d_out, n_out);
+  chk(hipGetLastError(), "box_summaries zero");
   for (int lvl = 1; lvl <= T.max_lvl; lvl++) {
     const int n = P.n_ids(lvl);
     if (!n) continue;
@@ -858,7 +867,55 @@ void box_summaries(Pool& P, const StVars& v, double* out_host) {
   chk(hipMemcpyAsync(out_host, d_out, (size_t)T.max_id * 3 * sizeof(double),
                      hipMemcpyDeviceToHost, P.stream), "summaries");
   chk(hipStreamSynchronize(P.stream), "summaries sync");
-  chk(hipFree(d_out), "hipFree");
+}
+
+void box_summaries(Pool& P, const StVars& v, double* out_host) {
+  P.sync_topology();
+  const size_t n_out = (size_t)std::max(P.tree.max_id, 1) * 3;
+  summaries_into(P, v, (double*)P.work(n_out * sizeof(double)), out_host);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation summaries (a5g_box_summaries_noalloc): the
+// 3 maxima of up to kNoAllocMaxBoxes boxes in a file-scope device array,
+// which each GPU the library runs on holds until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the device (noalloc_ptr), looked up once per device by a5g_init.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) double scratch[3 * kNoAllocMaxBoxes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_init() {
+  int dev = 0;
+  chk(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return;
+  void* p = nullptr;
+  chk(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch");
+  g_noalloc_ptr[dev] = (double*)p;
+}
+
+void box_summaries_noalloc(Pool& P, const StVars& v, double* out_host) {
+  if (P.topo_dirty || P.lists_dirty)
+    throw std::logic_error(
+        "box_summaries_noalloc: the tree changed since its last upload; a call that may "
+        "allocate (a5g_box_summaries or any kernel call) has to upload it first");
+  if (P.tree.max_id > kNoAllocMaxBoxes)
+    throw std::length_error("box_summaries_noalloc: " + std::to_string(P.tree.max_id) +
+                            " boxes, the scratch holds " + std::to_string(kNoAllocMaxBoxes));
+  int dev = 0;
+  chk(hipGetDevice(&dev), "hipGetDevice");
+  double* ptr = (dev >= 0 && dev < kMaxDevices) ? g_noalloc_ptr[dev] : nullptr;
+  if (!ptr) throw std::logic_error("box_summaries_noalloc: no scratch on this device (a5g_init "
+                                   "was called with another device current)");
+  summaries_into(P, v, ptr, out_host);
 }
 
 // device copy of a host id list, grouped by level; calls fn(level, ptr, n)
@@ -874,13 +931,11 @@ static void by_level(Pool& P, int n_ids, const int* ids, F fn) {
       if (T.lvl[id] == lvl) sel.push_back(id);
     }
     if (sel.empty()) continue;
-    int* d = nullptr;
-    chk(hipMalloc((void**)&d, sel.size() * sizeof(int)), "hipMalloc");
+    int* d = (int*)P.work(sel.size() * sizeof(int));
     chk(hipMemcpyAsync(d, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice, P.stream),
         "ids");
     fn(lvl, d, (int)sel.size());
     chk(hipStreamSynchronize(P.stream), "sync");
-    chk(hipFree(d), "hipFree");
   }
 }
 

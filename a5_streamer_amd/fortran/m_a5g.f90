@@ -433,6 +433,14 @@ module m_a5g
        integer(c_int)     :: s
      end function c_box_summaries
 
+     function c_box_summaries_noalloc(t, out) bind(C, name="a5g_box_summaries_noalloc") &
+          result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: t
+       real(c_double)     :: out(*)
+       integer(c_int)     :: s
+     end function c_box_summaries_noalloc
+
      function c_prolong1_to(t, n, ids, iv) bind(C, name="a5g_prolong1_to") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -656,6 +664,7 @@ module m_a5g
   public :: st_setup, st_table_create, st_table_set_col, st_table_get
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
+  public :: st_box_summaries_noalloc
   public :: st_photoi_helmholtz_terms
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
@@ -1136,6 +1145,17 @@ contains
     allocate(summ(3, a5g_max_id(tree)))
     call check(c_box_summaries(tree%h, summ), "box_summaries")
   end subroutine st_box_summaries
+
+  !> st_box_summaries without any device allocation (the library's fixed
+  !> scratch, at most 131072 boxes; the tree must have been uploaded since
+  !> its last change). summ(3, max_id) is the caller's.
+  subroutine st_box_summaries_noalloc(tree, summ)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(inout)    :: summ(:, :)
+    if (size(summ, 1) /= 3 .or. size(summ, 2) < a5g_max_id(tree)) &
+         error stop "st_box_summaries_noalloc: summ must be (3, max_id)"
+    call check(c_box_summaries_noalloc(tree%h, summ), "box_summaries_noalloc")
+  end subroutine st_box_summaries_noalloc
 
   ! ------------------------------------------------------------------
   ! streamer diagnostics (src/streamer_cyl.f90:682-956)

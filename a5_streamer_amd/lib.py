@@ -315,6 +315,14 @@ class Tree:
         _call("a5g_box_summaries", self.h, _d(out)[1])
         return out
 
+    def box_summaries_noalloc(self):
+        """box_summaries through the library's fixed scratch: no device
+        allocation (at most 131072 boxes; the tree must be uploaded)."""
+        n = self.info()["max_id"]
+        out = np.zeros((n, 3))
+        _call("a5g_box_summaries_noalloc", self.h, _d(out)[1])
+        return out
+
     def prolong1_to(self, ids, iv, add=False):
         if add:
             raise A5GError("prolong1_to: only add=False (prolong_to_new_boxes) is provided")

@@ -202,6 +202,16 @@ int a5g_photoi_helmholtz_terms(a5g_tree *t, double coeff, int n_terms, const dou
 /* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
  * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
 int a5g_box_summaries(a5g_tree *t, double *out);
+/* a5g_box_summaries without any device allocation, for callers that cannot
+ * allocate (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference. The per-box maxima go through
+ * a fixed device scratch of the library (3 doubles for each of up to 131072
+ * boxes, held by each GPU until the process ends), shared by every call on
+ * that GPU: calls that overlap on one GPU race. Fails when max_id exceeds
+ * 131072, and when the tree changed since its last upload (after
+ * a5g_adjust_refinement or a5g_tidy_up, a call that may allocate, such as
+ * a5g_box_summaries or any kernel call, has to come first). */
+int a5g_box_summaries_noalloc(a5g_tree *t, double *out);
 /* a$D_prolong1_to (afivo/src/m_aX_prolong.f90:99-191, add = .false.) for
  * the boxes ids[0..n-1] (prolong_to_new_boxes, src/streamer_2d.f90:592-617) */
 int a5g_prolong1_to(a5g_tree *t, int n_ids, const int *ids, int iv);
