@@ -433,6 +433,10 @@ static thread_local std::string g_err;
 
 #define NEED(t) \
   if (!(t)) throw std::invalid_argument("null a5g_tree handle");
+// ghost cells the last multigrid call left unstored are stored before any
+// other call sees the data (flush_ghosts)
+#define FLUSH(t) \
+  if (!(t)->P.host_only) a5g::flush_ghosts((t)->P);
 
 static void check_id(const a5g::Tree& T, int id) {
   if (id < 1 || id > T.max_id || !T.in_use[id])
@@ -499,6 +503,11 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // A5G_GRAPHS=0 runs every multigrid call as plain launches
       const char* g = std::getenv("A5G_GRAPHS");
       h->P.graphs_on = !(g && g[0] == '0');
+      // A5G_DEFER_GHOSTS=0 stores the finest level's ghost cells at the end
+      // of every multigrid call, as the reference does, instead of leaving
+      // them to the next reader
+      const char* dg = std::getenv("A5G_DEFER_GHOSTS");
+      h->P.defer_on = !(dg && dg[0] == '0');
       const char* sb = std::getenv("A5G_STREAM_BYTES");
       if (sb && sb[0]) {
         char* end = nullptr;
@@ -532,6 +541,7 @@ int a5g_destroy(a5g_tree* t) {
 int a5g_set_base(a5g_tree* t, int n, const int* ix_list, const int* nb_list) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   t->P.tree.set_base(n, ix_list, nb_list);
   t->P.ensure_capacity(t->P.tree.cap);
   t->P.topo_dirty = true;
@@ -541,6 +551,7 @@ int a5g_set_base(a5g_tree* t, int n, const int* ix_list, const int* nb_list) {
 int a5g_adjust_refinement(a5g_tree* t, int n_flags, const int* flags, int* n_add, int* n_rm) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   int a = 0, r = 0;
   t->P.tree.adjust_refinement(n_flags, flags, &a, &r);
   t->P.ensure_capacity(std::max(t->P.tree.cap, t->P.tree.max_id));
@@ -569,6 +580,7 @@ int a5g_ref_info_lvl(a5g_tree* t, int lvl, int* add, int* rm, int* n_add, int* n
 int a5g_tidy_up(a5g_tree* t, double max_frac, double goal_frac, int n_clean_min, int reorder) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   if (P.tree.tidy_up(max_frac, goal_frac, n_clean_min, reorder != 0)) {
     P.apply_moves(P.tree.last_moves, P.tree.max_id);
@@ -642,6 +654,7 @@ int a5g_level_lists(a5g_tree* t, int lvl, int* ids, int* leaves, int* parents) {
 int a5g_set_cc(a5g_tree* t, int id, int iv, const double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_id(P.tree, id);
@@ -660,6 +673,7 @@ int a5g_set_cc(a5g_tree* t, int id, int iv, const double* data) {
 int a5g_get_cc(a5g_tree* t, int id, int iv, double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_id(P.tree, id);
@@ -677,6 +691,7 @@ int a5g_get_cc(a5g_tree* t, int id, int iv, double* data) {
 int a5g_set_cc_all(a5g_tree* t, int iv, const double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_iv(P.tree, iv);
@@ -698,6 +713,7 @@ int a5g_set_cc_all(a5g_tree* t, int iv, const double* data) {
 int a5g_get_cc_all(a5g_tree* t, int iv, double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_iv(P.tree, iv);
@@ -719,6 +735,7 @@ int a5g_get_cc_all(a5g_tree* t, int iv, double* data) {
 int a5g_set_fc(a5g_tree* t, int id, int dim, int ifc, const double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_id(P.tree, id);
@@ -734,6 +751,7 @@ int a5g_set_fc(a5g_tree* t, int id, int dim, int ifc, const double* data) {
 int a5g_get_fc(a5g_tree* t, int id, int dim, int ifc, double* data) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   check_id(P.tree, id);
@@ -749,6 +767,7 @@ int a5g_get_fc(a5g_tree* t, int id, int dim, int ifc, double* data) {
 int a5g_sync(a5g_tree* t) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::hcheck(hipStreamSynchronize(t->P.stream), "sync");
   API_END
@@ -758,6 +777,7 @@ int a5g_mg_setup(a5g_tree* t, int i_phi, int i_tmp, int i_rhs, int i_eps, int n_
                  int n_base, const int* bc_kind, const double* bc_val, int auto_ops) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   // mg$D_init_mg, m_aX_mg.f90:130-154
   if (i_phi < 0) throw std::invalid_argument("mg_init_mg: i_phi not set");
@@ -791,6 +811,7 @@ int a5g_mg_setup(a5g_tree* t, int i_phi, int i_tmp, int i_rhs, int i_eps, int n_
 int a5g_mg_set_helmholtz(a5g_tree* t, double kappa2) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   if (!(kappa2 >= 0)) throw std::invalid_argument("mg_set_helmholtz: need kappa2 >= 0");
   if (kappa2 > 0 && P.mg.auto_ops && P.mg.i_eps > 0)
@@ -802,6 +823,7 @@ int a5g_mg_set_helmholtz(a5g_tree* t, double kappa2) {
 int a5g_mg_set_bc_value(a5g_tree* t, int nb, double value) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   if (nb < 1 || nb > 2 * t->P.tree.dim) throw std::out_of_range("face index out of range");
   t->P.bc_val[nb - 1] = value;
   API_END
@@ -825,6 +847,7 @@ int a5g_mg_solve_mixed(a5g_tree* t, double tol, int max_outer, int n_inner, int 
                        int* n_outer, double* rel_res) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   auto& P = t->P;
   if (zero_start) {
@@ -846,6 +869,7 @@ int a5g_mg_solve_mixed(a5g_tree* t, double tol, int max_outer, int n_inner, int 
 int a5g_mg_residual_norms(a5g_tree* t, double* max_res, double* max_rhs) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   a5g::mg_residual_norms(t->P, max_res, max_rhs);
   API_END
 }
@@ -853,6 +877,7 @@ int a5g_mg_residual_norms(a5g_tree* t, double* max_res, double* max_rhs) {
 int a5g_gc_tree(a5g_tree* t, int iv, int rb, int bc) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   check_iv(t->P.tree, iv);
   if (rb < 0 || rb > 2 || bc < 0 || bc > 2) throw std::invalid_argument("bad ghost rule");
   a5g::gc_tree(t->P, iv, rb, bc);
@@ -862,6 +887,7 @@ int a5g_gc_tree(a5g_tree* t, int iv, int rb, int bc) {
 int a5g_restrict_tree(a5g_tree* t, int iv) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   check_iv(t->P.tree, iv);
   a5g::restrict_tree(t->P, iv);
   API_END
@@ -870,6 +896,7 @@ int a5g_restrict_tree(a5g_tree* t, int iv) {
 int a5g_tree_copy_cc(a5g_tree* t, int iv_from, int iv_to) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   check_iv(t->P.tree, iv_from);
   check_iv(t->P.tree, iv_to);
   a5g::tree_copy_cc(t->P, iv_from, iv_to);
@@ -879,6 +906,7 @@ int a5g_tree_copy_cc(a5g_tree* t, int iv_from, int iv_to) {
 int a5g_tree_maxdiff_cc(a5g_tree* t, int iv_a, int iv_b, double* maxdiff, double* maxabs_b) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   check_iv(t->P.tree, iv_a);
   check_iv(t->P.tree, iv_b);
@@ -889,6 +917,7 @@ int a5g_tree_maxdiff_cc(a5g_tree* t, int iv_a, int iv_b, double* maxdiff, double
 int a5g_tree_minmax_cc(a5g_tree* t, int iv, double* vmin, double* vmax) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   check_iv(t->P.tree, iv);
   a5g::minmax_boxes(t->P, iv, false, false, true, vmin, vmax);
   API_END
@@ -1015,6 +1044,7 @@ static void need_table(const a5g::Pool& P, const a5g::StVars& v) {
 int a5g_space_charge(a5g_tree* t, double fac) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::space_charge(t->P, t->P.sv, fac);
   API_END
@@ -1023,6 +1053,7 @@ int a5g_space_charge(a5g_tree* t, double fac) {
 int a5g_fld_from_pot(a5g_tree* t) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::fld_from_pot(t->P, t->P.sv);
   API_END
@@ -1031,6 +1062,7 @@ int a5g_fld_from_pot(a5g_tree* t) {
 int a5g_fluxes_koren(a5g_tree* t, double dt) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   need_table(t->P, t->P.sv);
   if (!(dt > 0)) throw std::invalid_argument("fluxes_koren: dt must be > 0");
@@ -1041,6 +1073,7 @@ int a5g_fluxes_koren(a5g_tree* t, double dt) {
 int a5g_consistent_fluxes(a5g_tree* t, int n, const int* f_ixs) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   for (int i = 0; i < n; i++) {
     if (f_ixs[i] < 1 || f_ixs[i] > t->P.tree.n_fc)
@@ -1053,6 +1086,7 @@ int a5g_consistent_fluxes(a5g_tree* t, int n, const int* f_ixs) {
 int a5g_update_solution(a5g_tree* t, double dt) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   need_table(t->P, t->P.sv);
   a5g::update_solution(t->P, t->P.sv, dt);
@@ -1064,6 +1098,7 @@ int a5g_photoi_helmholtz(a5g_tree* t, double coeff, int n_terms, const double* a
                          int max_cycles, int* cycles) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   need_table(P, P.sv);
@@ -1083,6 +1118,7 @@ int a5g_photoi_helmholtz_terms(a5g_tree* t, double coeff, int n_terms, const dou
                                int* cycles) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   need_table(P, P.sv);
@@ -1104,6 +1140,7 @@ int a5g_photoi_helmholtz_terms(a5g_tree* t, double coeff, int n_terms, const dou
 int a5g_average_dens(a5g_tree* t) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::average_dens(t->P, t->P.sv);
   API_END
@@ -1112,6 +1149,7 @@ int a5g_average_dens(a5g_tree* t) {
 int a5g_box_summaries(a5g_tree* t, double* out) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::box_summaries(t->P, t->P.sv, out);
   API_END
@@ -1120,6 +1158,7 @@ int a5g_box_summaries(a5g_tree* t, double* out) {
 int a5g_box_summaries_noalloc(a5g_tree* t, double* out) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   if (!out) throw std::invalid_argument("box_summaries_noalloc: null output");
   a5g::box_summaries_noalloc(t->P, t->P.sv, out);
@@ -1139,6 +1178,7 @@ int a5g_reduction_loc(a5g_tree* t, int kind, int var, int d, double thr, double 
                       double* val, int* id, int* ix) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   const auto& T = t->P.tree;
   if (!val || !id || !ix) throw std::invalid_argument("reduction_loc: null argument");
@@ -1159,6 +1199,7 @@ int a5g_cc_axis(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int* i
                 double* out, int* n_points) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   const auto& T = t->P.tree;
   if (!n_points) throw std::invalid_argument("cc_axis: null n_points");
   if (n_cc < 0 || n_fc < 0 || (n_cc && !ixs_cc) || (n_fc && !ixs_fc))
@@ -1186,6 +1227,7 @@ int a5g_leaf_cells(a5g_tree* t, int n_cc, const int* ixs_cc, int n_fc, const int
                    long long cap, double* out, long long* n_cells) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   const auto& T = t->P.tree;
   if (!n_cells) throw std::invalid_argument("leaf_cells: null n_cells");
   check_out_vars(T, n_cc, ixs_cc, n_fc, ixs_fc);
@@ -1212,6 +1254,7 @@ int a5g_write_vtk(a5g_tree* t, const char* filename, int n_cc, const char* const
                   int n_cycle, double time) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   const auto& T = t->P.tree;
   if (!T.ready) throw std::runtime_error("Tree not ready");
@@ -1234,6 +1277,7 @@ int a5g_dump_leaves(a5g_tree* t, const char* filename, int n_var, const int* ivs
                     double time) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   if (!filename) throw std::invalid_argument("dump_leaves: null filename");
   check_out_vars(t->P.tree, n_var, ivs, 0, nullptr);
@@ -1244,6 +1288,7 @@ int a5g_dump_leaves(a5g_tree* t, const char* filename, int n_var, const int* ivs
 int a5g_set_instances(a5g_tree* t, int n_inst, double stride) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   if (n_inst < 0) throw std::invalid_argument("set_instances: negative count");
   if (n_inst > 0 && !(stride > 0)) throw std::invalid_argument("set_instances: stride must be > 0");
@@ -1268,6 +1313,7 @@ int a5g_set_instances(a5g_tree* t, int n_inst, double stride) {
 int a5g_set_instance_values(a5g_tree* t, int what, int n, const double* values) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   need_gpu(P);
   if (!P.n_inst) throw std::logic_error("set_instance_values: no instances set");
@@ -1293,6 +1339,7 @@ int a5g_set_instance_values(a5g_tree* t, int what, int n, const double* values) 
 int a5g_set_active_instances(a5g_tree* t, int n, const int* active) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   if (!P.n_inst) throw std::logic_error("set_active_instances: no instances set");
   if (active && n != P.n_inst) throw std::invalid_argument("set_active_instances: one flag per instance");
@@ -1304,6 +1351,7 @@ int a5g_set_active_instances(a5g_tree* t, int n, const int* active) {
 int a5g_set_instance_groups(a5g_tree* t, int n, const int* group) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   auto& P = t->P;
   if (n != 0 && n != P.n_inst)
     throw std::invalid_argument("instance groups: need one entry per instance (or n = 0)");
@@ -1325,6 +1373,7 @@ int a5g_set_instance_groups(a5g_tree* t, int n, const int* group) {
 int a5g_fas_fmg_groups(a5g_tree* t, int set_residual, int first_call) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   a5g::mg_fmg_groups(t->P, set_residual != 0, first_call != 0);
   API_END
@@ -1333,6 +1382,7 @@ int a5g_fas_fmg_groups(a5g_tree* t, int set_residual, int first_call) {
 int a5g_prolong1_to(a5g_tree* t, int n_ids, const int* ids, int iv) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   check_iv(t->P.tree, iv);
   a5g::prolong1_boxes(t->P, n_ids, ids, iv);
@@ -1342,6 +1392,7 @@ int a5g_prolong1_to(a5g_tree* t, int n_ids, const int* ids, int iv) {
 int a5g_fill_cc(a5g_tree* t, int n_ids, const int* ids, int iv, double value) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   check_iv(t->P.tree, iv);
   a5g::fill_boxes(t->P, n_ids, ids, iv, value);
@@ -1351,6 +1402,7 @@ int a5g_fill_cc(a5g_tree* t, int n_ids, const int* ids, int iv, double value) {
 int a5g_gc_boxes(a5g_tree* t, int n_ids, const int* ids, int iv, int rb, int bc) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   check_iv(t->P.tree, iv);
   if (rb < 0 || rb > 2 || bc < 0 || bc > 2) throw std::invalid_argument("bad ghost rule");
@@ -1361,6 +1413,7 @@ int a5g_gc_boxes(a5g_tree* t, int n_ids, const int* ids, int iv, int rb, int bc)
 int a5g_set_gaussians(a5g_tree* t, int iv, int n_blob, const double* blobs, double base) {
   API_BEGIN
   NEED(t);
+  FLUSH(t);
   need_gpu(t->P);
   check_iv(t->P.tree, iv);
   if (n_blob < 0 || (n_blob > 0 && !blobs)) throw std::invalid_argument("bad blob list");

@@ -16,6 +16,8 @@ void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
 void mg_fmg(Pool& P, bool set_residual, bool first_call);
 void mg_fmg_groups(Pool& P, bool set_residual, bool first_call);
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
+// stores the ghost cells a multigrid call left to the next reader
+void flush_ghosts(Pool& P);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);
 void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b);
 void mg_solve_mixed(Pool& P, double tol, int max_outer, int n_inner, int* outer_done,

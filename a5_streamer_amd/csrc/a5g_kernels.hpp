@@ -71,6 +71,14 @@ __device__ __forceinline__ void child_off(const Dev& a, int id, int nc, int nb, 
   }
 }
 
+// The value V of the rule 2V - c on physical face nb of box id: the run's
+// own with instances, else the tree's
+template <int D>
+__device__ __forceinline__ double face_value(const Dev& a, int id, int nb) {
+  return a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstBc + nb - 1]
+                  : a.bc_val[nb - 1];
+}
+
 // Value of the ghost cell g = (gi, gj, gk) on face nb of box id for the
 // multigrid potential: neighbour copy (sides_from_nb, m_aX_gc.f90:368-402),
 // refinement-boundary extrapolation (mg$D_sides_rb, m_aX_mg.f90:262-356,
@@ -117,9 +125,7 @@ __device__ __forceinline__ T ghost_mg_u(const Dev& a, U own, int iv, int id, int
     const int kind = a.bc_kind[nb - 1];
     if (kind == 0) return cv;
     if (kind == 1) return -cv;
-    const double V = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstBc + nb - 1]
-                              : a.bc_val[nb - 1];
-    return T(2) * (T)V - cv;
+    return T(2) * (T)face_value<D>(a, id, nb) - cv;
   }
 }
 

@@ -602,9 +602,7 @@ __global__ __launch_bounds__(256) void k_gc(Dev a, const int* __restrict__ list,
       } else {
         xg = gc_boundary<D, T>(a, p, iv, id, nc, xnb, g, rb, bc);
       }
-      const double V = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstBc + nb - 1]
-                                : a.bc_val[nb - 1];
-      p[cidx<D>(nc, g[0], nb == 3 ? 0 : nc + 1, 0)] = T(2) * (T)V - xg;
+      p[cidx<D>(nc, g[0], nb == 3 ? 0 : nc + 1, 0)] = T(2) * (T)face_value<D>(a, id, nb) - xg;
     }
   }
 }
@@ -949,11 +947,15 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
 
 // The same pass with phi staged in LDS (lexicographic order, interior and
 // face ghosts) for the operator; the interior and the ghost slots of each
-// colour are visited as contiguous runs.
+// colour are visited as contiguous runs. fused = 1: the face ghosts of phi
+// are recomputed from the current state (neighbour copy, refinement-boundary
+// extrapolation, physical rule) instead of read, so that no ghost pass has
+// to store them after the restriction; with copy_tmp they also go to tmp's
+// face ghost slots, as the stored values would have.
 template <int D, bool CYL, int NC, typename T, bool EPS>
 __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
-                                                        int copy_tmp) {
+                                                        int copy_tmp, int fused) {
   constexpr int N2 = NC + 2;
   constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
   constexpr int HALF = BL / 2;
@@ -976,6 +978,17 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
     icell<D>(NC, q, col, i, j, k);
     return col * HALF + q;
   };
+  auto face = [](int q, int g[3]) {
+    const int nb = q / PER_FACE + 1;
+    const int t = q - (nb - 1) * PER_FACE;
+    const int d = d_nb_dim(nb);
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    g[0] = g[1] = g[2] = 0;
+    g[d] = d_nb_low(nb) ? 0 : NC + 1;
+    g[ta] = t % NC + 1;
+    if (D == 3) g[tb] = t / NC + 1;
+    return nb;
+  };
   const T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   if (act) {
     for (int e = lt; e < 2 * NCOL; e += TPB) {
@@ -983,49 +996,74 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
       const int x = cell(e, i, j, k);
       s[L(i, j, k)] = p[x];
     }
-    for (int q = lt; q < NGH; q += TPB) {
-      const int nb = q / PER_FACE + 1;
-      const int t = q - (nb - 1) * PER_FACE;
-      const int d = d_nb_dim(nb);
-      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-      int g[3] = {0, 0, 0};
-      g[d] = d_nb_low(nb) ? 0 : NC + 1;
-      g[ta] = t % NC + 1;
-      if (D == 3) g[tb] = t / NC + 1;
-      s[L(g[0], g[1], g[2])] = p[cidx<D>(NC, g[0], g[1], g[2])];
-    }
+    if (!fused)
+      for (int q = lt; q < NGH; q += TPB) {
+        int g[3];
+        face(q, g);
+        s[L(g[0], g[1], g[2])] = p[cidx<D>(NC, g[0], g[1], g[2])];
+      }
   }
   __syncthreads();
-  if (!act) return;
-  T* r = boxp_t<T>(a, i_rhs, id);
-  T* t = boxp_t<T>(a, i_tmp, id);
-  const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
-  const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
-  for (int e = lt; e < 2 * NCOL; e += TPB) {
-    int i, j, k;
-    const int x = cell(e, i, j, k);
-    T rv = EPS ? op_u<D, CYL, T>(a, id, NC, own, i, j, k, inv_dr_sq, ioff)
-               : lplh_u<D, CYL, T>(a, own, i, j, k, inv_dr_sq, ioff);
-    rv = rv + t[x];
-    r[x] = rv;
-    if (copy_tmp) t[x] = own(i, j, k);
+  if (act && fused)
+    for (int q = lt; q < NGH; q += TPB) {
+      int g[3];
+      const int nb = face(q, g);
+      // reads interior cells of s only, writes a ghost slot
+      s[L(g[0], g[1], g[2])] = ghost_mg_u<D, T>(a, own, i_phi, id, NC, nb, g[0], g[1], g[2]);
+    }
+  __syncthreads();
+  T* r = act ? boxp_t<T>(a, i_rhs, id) : nullptr;
+  T* t = act ? boxp_t<T>(a, i_tmp, id) : nullptr;
+  if (act) {
+    const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
+    const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
+    for (int e = lt; e < 2 * NCOL; e += TPB) {
+      int i, j, k;
+      const int x = cell(e, i, j, k);
+      T rv = EPS ? op_u<D, CYL, T>(a, id, NC, own, i, j, k, inv_dr_sq, ioff)
+                 : lplh_u<D, CYL, T>(a, own, i, j, k, inv_dr_sq, ioff);
+      rv = rv + t[x];
+      r[x] = rv;
+      if (copy_tmp) t[x] = own(i, j, k);
+    }
+    for (int e = lt; e < 2 * NGC; e += TPB) {
+      const int col = e >= NGC;
+      const int x = col * HALF + NCOL + (e - col * NGC);
+      r[x] = r[x] + t[x];
+      if (copy_tmp) t[x] = p[x];
+    }
   }
-  for (int e = lt; e < 2 * NGC; e += TPB) {
-    const int col = e >= NGC;
-    const int x = col * HALF + NCOL + (e - col * NGC);
-    r[x] = r[x] + t[x];
-    if (copy_tmp) t[x] = p[x];
+  if (!(copy_tmp && fused)) return;
+  // tmp's face ghosts take the recomputed values, after the loop above has
+  // used the old ones and copied the stored (stale) ones
+  __syncthreads();
+  if (act) {
+    for (int q = lt; q < NGH; q += TPB) {
+      int g[3];
+      face(q, g);
+      t[cidx<D>(NC, g[0], g[1], g[2])] = s[L(g[0], g[1], g[2])];
+    }
+    // the corner ghosts the rule 2V - c over a whole y ghost row writes
+    // (k_gc), from the x ghosts recomputed above
+    if (D == 2 && lt < 4) {
+      const int nb = 3 + (lt >> 1), side = lt & 1;
+      if (a.nbs[id * 6 + nb - 1] < 0 && a.bc_kind[nb - 1] == kBcRow2V) {
+        const int gx = side ? NC + 1 : 0, cy = nb == 3 ? 1 : NC;
+        t[cidx<D>(NC, gx, nb == 3 ? 0 : NC + 1, 0)] =
+            T(2) * (T)face_value<D>(a, id, nb) - s[L(gx, cy, 0)];
+      }
+    }
   }
 }
 
 template <int D, bool CYL, typename T, bool EPS>
 static bool launch_coarse_rhs_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc,
-                                    int i_phi, int i_rhs, int i_tmp, int copy_tmp) {
+                                    int i_phi, int i_rhs, int i_tmp, int copy_tmp, int fused) {
   auto go = [&](auto kern, int bl) {
     const int tpb = bl > 128 ? 256 : (bl > 64 ? 128 : 64);
     const int bpb = 256 / tpb;
     hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
-                       i_phi, i_rhs, i_tmp, copy_tmp);
+                       i_phi, i_rhs, i_tmp, copy_tmp, fused);
   };
   const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
   switch (nc) {
@@ -1435,6 +1473,11 @@ struct Ctx {
   bool eps;  // per-box operator tags (automatic operators and an eps variable)
   int n_down, n_up, n_base;  // smoothing cycles (mg$D_t%n_cycle_*)
   double helm;               // Helmholtz coefficient (0: Laplacian)
+  // The finest level of a V-cycle over the whole tree leaves its ghost
+  // cells to the next reader (defer_top), and the call after it starts from
+  // such a level (top_stale); see flush_ghosts.
+  bool defer_top = false;
+  bool top_stale = false;
 };
 
 static Ctx ctx64(const Pool& P) {
@@ -1602,29 +1645,35 @@ static void resid_rstr(Pool& P, const Ctx& c, int lvl, int keep_res) {
   P.prof_end(pr);
 }
 
+// The FAS right-hand side on the parents of level lvl. fused: the parents'
+// face ghosts of phi are recomputed in the pass (the LDS kernels); where no
+// such kernel exists they are stored first. Returns whether it recomputed.
 template <int D, typename T>
-static void coarse_rhs(Pool& P, const Ctx& c, int lvl, int copy_tmp) {
+static void coarse_rhs(Pool& P, const Ctx& c, int lvl, int copy_tmp, bool fused = false) {
   const int n = P.n_parents(lvl);
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
   Geo g = geo(n, cells_of(D, nc, true));
   Dev a = dev_for(P, c);
   const double cells = (double)n * cells_of(D, nc, false);
+  const int f = fused ? 1 : 0;
+  const bool lds = nc == 4 || nc == 8 || nc == 16 || (D == 2 && nc == 32);
+  if (fused && !lds) fill_gc_phi<D, T>(P, c, lvl);
   const int pr = P.prof_begin(PK_COARSE_RHS, lvl, 4.0 * sizeof(T) * cells);
   bool done;
   if (c.cyl)
     done = c.eps ? launch_coarse_rhs_lds_t<2, true, T, true>(P, a, P.parents(lvl), n, nc,
-                                                              c.i_phi, c.i_rhs, c.i_tmp, copy_tmp)
+                                                              c.i_phi, c.i_rhs, c.i_tmp, copy_tmp, f)
                  : launch_coarse_rhs_lds_t<2, true, T, false>(P, a, P.parents(lvl), n, nc,
                                                                c.i_phi, c.i_rhs, c.i_tmp,
-                                                               copy_tmp);
+                                                               copy_tmp, f);
   else
     done = c.eps ? launch_coarse_rhs_lds_t<D, false, T, true>(P, a, P.parents(lvl), n, nc,
                                                                c.i_phi, c.i_rhs, c.i_tmp,
-                                                               copy_tmp)
+                                                               copy_tmp, f)
                  : launch_coarse_rhs_lds_t<D, false, T, false>(P, a, P.parents(lvl), n, nc,
                                                                 c.i_phi, c.i_rhs, c.i_tmp,
-                                                                copy_tmp);
+                                                                copy_tmp, f);
   if (!done) {
     if (c.cyl)
       hipLaunchKernelGGL((k_coarse_rhs<2, true, T>), dim3(g.grid), dim3(256), 0, P.stream, a,
@@ -1639,10 +1688,13 @@ static void coarse_rhs(Pool& P, const Ctx& c, int lvl, int copy_tmp) {
 
 template <int D, typename T>
 static void update_coarse(Pool& P, const Ctx& c, int lvl) {
-  // update_coarse, m_aX_mg.f90:430-486
+  // update_coarse, m_aX_mg.f90:430-486. The reference stores the ghost cells
+  // of phi on lvl - 1 after the restriction; here their two readers, the
+  // coarse rhs and the first down half-sweep of lvl - 1, recompute them
+  // from the same state, so they are not stored (the up-smoothing of
+  // lvl - 1 stores them before anything else reads them).
   resid_rstr<D, T>(P, c, lvl, 0);
-  fill_gc_phi<D, T>(P, c, lvl - 1);
-  coarse_rhs<D, T>(P, c, lvl - 1, 1);
+  coarse_rhs<D, T>(P, c, lvl - 1, 1, true);
 }
 
 template <int D, typename T>
@@ -1742,16 +1794,24 @@ static void fas_vcycle(Pool& P, const Ctx& c, int max_lvl, bool set_residual) {
   // correction reads) does not store them after the up-smoothing, and the
   // next down-smoothing of that level recomputes them in its first
   // half-sweep.
+  // In FP64 the finest level of the tree does the same across calls
+  // (defer_top): its ghosts are stored by the next call that reads them
+  // (flush_ghosts), or recomputed by the first half-sweep of the next
+  // V-cycle (top_stale).
   const int min_lvl = P.tree.min_lvl;
   auto lazy = [&](int lvl) { return c.homogeneous && P.n_parents(lvl) == 0; };
+  const bool defer = c.defer_top && !set_residual && max_lvl == P.tree.max_lvl &&
+                     P.n_parents(max_lvl) == 0;
   for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
-    gsrb_boxes<D, T>(P, c, lvl, c.n_down, lvl == max_lvl && lazy(lvl), false);
+    // below max_lvl the stored ghosts are stale (update_coarse)
+    gsrb_boxes<D, T>(P, c, lvl, c.n_down, lvl < max_lvl || lazy(lvl) || c.top_stale, false);
     update_coarse<D, T>(P, c, lvl);
   }
-  gsrb_boxes<D, T>(P, c, min_lvl, c.n_base);
+  gsrb_boxes<D, T>(P, c, min_lvl, c.n_base, min_lvl < max_lvl || c.top_stale,
+                   !(min_lvl == max_lvl && defer));
   for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
     correct_children<D, T>(P, c, lvl - 1);
-    gsrb_boxes<D, T>(P, c, lvl, c.n_up, true, !(lvl == max_lvl && lazy(lvl)));
+    gsrb_boxes<D, T>(P, c, lvl, c.n_up, true, !(lvl == max_lvl && (lazy(lvl) || defer)));
   }
   if (set_residual)
     for (int lvl = min_lvl; lvl <= max_lvl; lvl++) residual_level<D, T>(P, c, lvl, false);
@@ -1939,24 +1999,64 @@ static void mg_check(Pool& P) {
   if (P.mg.auto_ops && P.mg.i_eps > 0) ensure_tags(P);
 }
 
+// Ghost cells of the potential left unstored by the last multigrid call
+// (a5g_fas_fmg / a5g_fas_vcycle over the whole tree): the reference stores
+// them at the end of every V-cycle (fill_gc_phi after the up-smoothing,
+// m_aX_mg.f90:418-423), but inside a solve nothing reads those of the
+// finest level before the next V-cycle's first half-sweep, which recomputes
+// them from the same state. So the store is left to whoever comes next: a
+// following V-cycle over the tree recomputes, and every other entry point
+// first calls flush_ghosts, which stores exactly what the skipped pass would
+// have stored (nothing has changed the level, its neighbours or its parents
+// in between). The state seen through the C ABI is the reference's.
+void flush_ghosts(Pool& P) {
+  if (P.defer_lvl == kNoDefer) return;
+  const int lvl = P.defer_lvl, iv = P.defer_iv;
+  P.defer_lvl = kNoDefer;
+  gc_list(P, lvl, P.ids(lvl), P.n_ids(lvl), iv, 0, 0);
+}
+
+// whether a call over the whole tree may leave the finest level's ghosts
+static bool may_defer(const Pool& P, bool set_residual) {
+  return !set_residual && P.n_inst == 0 && P.cur_group < 0 && P.defer_on;
+}
+
 void mg_fmg(Pool& P, bool set_residual, bool first_call) {
   mg_check(P);
-  const Ctx c = ctx64(P);
-  run_captured(P, call_key(1, set_residual, first_call, 0), [&] {
+  flush_ghosts(P);
+  Ctx c = ctx64(P);
+  c.defer_top = may_defer(P, set_residual);
+  run_captured(P, call_key(1, set_residual, first_call, c.defer_top), [&] {
     if (P.tree.dim == 2) fas_fmg<2, double>(P, c, set_residual, first_call);
     else fas_fmg<3, double>(P, c, set_residual, first_call);
   });
+  if (c.defer_top && P.n_parents(P.tree.max_lvl) == 0) {
+    P.defer_lvl = P.tree.max_lvl;
+    P.defer_iv = c.i_phi;
+  }
 }
 
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual) {
   mg_check(P);
   if (max_lvl < P.tree.min_lvl || max_lvl > P.tree.max_lvl)
     throw std::invalid_argument("fas_vcycle: max_lvl outside the tree");
-  const Ctx c = ctx64(P);
-  run_captured(P, call_key(2, max_lvl - P.tree.min_lvl, set_residual, 0), [&] {
-    if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, max_lvl, set_residual);
-    else fas_vcycle<3, double>(P, c, max_lvl, set_residual);
-  });
+  Ctx c = ctx64(P);
+  if (P.defer_lvl != kNoDefer && !(P.defer_lvl == max_lvl && P.defer_iv == c.i_phi))
+    flush_ghosts(P);
+  c.top_stale = P.defer_lvl == max_lvl;
+  P.defer_lvl = kNoDefer;
+  c.defer_top = may_defer(P, set_residual) && max_lvl == P.tree.max_lvl &&
+                P.n_parents(max_lvl) == 0;
+  run_captured(P, call_key(2, max_lvl - P.tree.min_lvl, set_residual,
+                           (c.top_stale ? 1 : 0) | (c.defer_top ? 2 : 0)),
+               [&] {
+                 if (P.tree.dim == 2) fas_vcycle<2, double>(P, c, max_lvl, set_residual);
+                 else fas_vcycle<3, double>(P, c, max_lvl, set_residual);
+               });
+  if (c.defer_top) {
+    P.defer_lvl = max_lvl;
+    P.defer_iv = c.i_phi;
+  }
 }
 
 // The slots of the reduction scratch: set every pair to (lo_init, hi_init)

@@ -56,6 +56,8 @@ constexpr int kInstDt = 0, kInstKorenFac = 1, kInstBc = 2;
 // face, and 2V - c over the whole ghost row of a 2D y face, corners included
 constexpr int kBcCopy = 0, kBcNeg = 1, kBc2V = 2, kBcRow2V = 3;
 
+constexpr int kNoDefer = -1000000;
+
 // Per-launch timing with HIP events on the pool's stream (kernel classes
 // below), used by the benchmark to measure kernel durations in place.
 enum ProfKind { PK_GSRB = 0, PK_GC, PK_RESID_RSTR, PK_COARSE_RHS, PK_CORR, PK_COPY, PK_OTHER,
@@ -164,6 +166,12 @@ struct Pool {
   // times the 256 MB Infinity Cache. A5G_STREAM_BYTES at a5g_init sets
   // another threshold (0: every level of 16^3 or 32^2 boxes streams).
   double stream_bytes = 768.0 * (1 << 20);
+  // The finest level whose ghost cells of variable defer_iv the last
+  // multigrid call left unstored (kNoDefer: none); flush_ghosts stores
+  // them. A5G_DEFER_GHOSTS=0 at a5g_init stores them at once instead.
+  int defer_lvl = kNoDefer;
+  int defer_iv = -1;
+  bool defer_on = true;
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   // waits for every stream of the handle, then destroys the graphs (one may
