@@ -2592,3 +2592,341 @@ void o_box_summaries(const otree *t, const o_vars *v, double *out) {
     out[(id - 1) * 3 + 2] = mn;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Reverse mode of one transport substep of the drivers on a frozen     */
+/* mesh: fluxes_koren, a$D_consistent_fluxes, update_solution           */
+/* (src/streamer_2d.f90:384-542, src/streamer_3d.f90:396-615,           */
+/* src/streamer_cyl.f90:416-570, afivo/src/m_aX_core.f90:818-930). No   */
+/* counterpart in the reference, which has no derivatives. Each routine */
+/* takes the adjoints of its outputs and adds those of its inputs; every */
+/* branch (upwind side, limiter case, flux clamp, table row, dielectric  */
+/* rule) is the one the forward pass took on the same data. Pinned by    */
+/* central finite differences of the forward routines above             */
+/* (tests/test_oracle_adjoint.py).                                      */
+/* ------------------------------------------------------------------ */
+
+static void lt_dcol(const otable *t, int col, double x, int li, double lf, double *dval) {
+  /* d/dx of lt_col_at at x: the slope of the row interval, 0 where get_loc
+   * clamps to the table ends */
+  const double *c = t->rc + (size_t)(col - 1) * t->n_rows;
+  double frac = (x - t->x_min) * t->inv_dx;
+  int raw = (int)ceil(frac);
+  (void)lf;
+  *dval = (raw < 1 || raw >= t->n_rows) ? 0.0 : (c[li] - c[li - 1]) * t->inv_dx;
+}
+
+static void lt_col_adj(const otable *t, int col, int li, double lf, double lam, double *tg) {
+  /* adjoint of lf c(li) + (1 - lf) c(li + 1) with respect to the rows */
+  if (!tg) return;
+  double *g = tg + (size_t)(col - 1) * t->n_rows;
+  g[li - 1] += lf * lam;
+  g[li] += (1 - lf) * lam;
+}
+
+void o_update_solution_adj(otree *t, const o_vars *v, const otable *tbl, double dt,
+                           const o_adj *a, double *tgrad) {
+  int D = t->dim;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id];
+      double inv_dr = 1 / t->dr[id];
+      const double *E = CC(t, id, v->fld), *eps = CC(t, id, v->eps);
+      double *le = CC(t, id, a->le), *lp = CC(t, id, a->li), *lE = CC(t, id, a->lE);
+      double *lph = v->photoi ? CC(t, id, a->lpho) : NULL;
+      const double *F[3];
+      double *lF[3];
+      for (int d = 0; d < D; d++) {
+        F[d] = FC(t, id, v->f_elec, d);
+        lF[d] = FC(t, id, a->lF, d);
+      }
+      int ioff = (IXP(t, id)[0] - 1) * nc;
+      int kmax = D == 3 ? nc : 1;
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            /* the dielectric rule: n_i' = n_i^* - n_e^*, n_e' = 0 */
+            double le_pre = le[c], lp_pre = lp[c];
+            if (v->dielectric_rule && eps[c] > 1.0) {
+              le_pre = -lp[c];
+              lp_pre = lp[c];
+            }
+            le[c] = le_pre;
+            lp[c] = lp_pre;
+            double lsrc = le_pre * dt + lp_pre * dt;
+            double lsflux = le_pre * dt;
+            int li;
+            double lf;
+            lt_loc(tbl, E[c], &li, &lf);
+            double alpha = lt_col_at(tbl, v->t_alpha, li, lf);
+            double eta = lt_col_at(tbl, v->t_eta, li, lf);
+            size_t fx[2] = {FI(nc, D, 0, i, j, k), FI(nc, D, 0, i + 1, j, k)};
+            size_t fy[2] = {FI(nc, D, 1, i, j, k), FI(nc, D, 1, i, j + 1, k)};
+            size_t fz[2] = {0, 0};
+            double df[3];
+            df[0] = F[0][fx[0]] + F[0][fx[1]];
+            df[1] = F[1][fy[0]] + F[1][fy[1]];
+            if (D == 3) {
+              fz[0] = FI(nc, D, 2, i, j, k);
+              fz[1] = FI(nc, D, 2, i, j, k + 1);
+              df[2] = F[2][fz[0]] + F[2][fz[1]];
+            }
+            double nrm = o_norm2(D, df);
+            /* src = 0.5 |df| (alpha - eta) [+ pho] */
+            double lnrm = lsrc * 0.5 * (alpha - eta);
+            double lal = lsrc * 0.5 * nrm, let = -lsrc * 0.5 * nrm;
+            if (lph) lph[c] += lsrc;
+            double da, de;
+            lt_dcol(tbl, v->t_alpha, E[c], li, lf, &da);
+            lt_dcol(tbl, v->t_eta, E[c], li, lf, &de);
+            lE[c] += lal * da + let * de;
+            lt_col_adj(tbl, v->t_alpha, li, lf, lal, tgrad);
+            lt_col_adj(tbl, v->t_eta, li, lf, let, tgrad);
+            double ldf[3] = {0, 0, 0};
+            if (nrm > 0)
+              for (int d = 0; d < D; d++) ldf[d] = lnrm * df[d] / nrm;
+            lF[0][fx[0]] += ldf[0];
+            lF[0][fx[1]] += ldf[0];
+            lF[1][fy[0]] += ldf[1];
+            lF[1][fy[1]] += ldf[1];
+            if (D == 3) {
+              lF[2][fz[0]] += ldf[2];
+              lF[2][fz[1]] += ldf[2];
+            }
+            double s = lsflux * inv_dr;
+            if (D == 2 && t->coord == 1) {
+              double den = i + ioff - 0.5;
+              double rf1 = (double)(i + ioff - 1) / den, rf2 = (double)(i + ioff) / den;
+              lF[0][fx[0]] += rf1 * s;
+              lF[0][fx[1]] -= rf2 * s;
+            } else {
+              lF[0][fx[0]] += s;
+              lF[0][fx[1]] -= s;
+            }
+            lF[1][fy[0]] += s;
+            lF[1][fy[1]] -= s;
+            if (D == 3) {
+              lF[2][fz[0]] += s;
+              lF[2][fz[1]] -= s;
+            }
+          }
+    }
+  }
+}
+
+void o_consistent_fluxes_adj(otree *t, int lF) {
+  /* the coarse face values of o_consistent_fluxes are means of fine faces:
+   * their adjoints go to those fine faces and are cleared */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl - 1; lvl++) {
+    ivec *pa = &t->parents[LV(t, lvl)];
+    for (int i = 0; i < pa->n; i++) {
+      int id = pa->v[i];
+      for (int nb = 1; nb <= t->nnb; nb++) {
+        int nb_id = NBP(t, id)[nb - 1];
+        if (nb_id <= NO_BOX || has_children(t, nb_id)) continue;
+        int nc = t->ncell[id], nch = nc >> 1, d = nb_dim(nb);
+        int ii = nb_low(nb) ? 1 : nc + 1, inb = nb_low(nb) ? nc + 1 : 1;
+        for (int ic = 0; ic < t->nch / 2; ic++) {
+          int ich = ch_adj_nb(t, ic, nb), cid = CHP(t, id)[ich - 1];
+          int ioff[3];
+          for (int dd = 0; dd < t->dim; dd++) ioff[dd] = nch * ch_dix(ich, dd);
+          double *dst = FC(t, nb_id, lF, d), *src = FC(t, cid, lF, d);
+          int ta = (d == 0) ? 1 : 0, tb = (d == 2) ? 1 : 2;
+          int bmax = t->dim == 3 ? nch : 1;
+          for (int b = 1; b <= bmax; b++)
+            for (int a = 1; a <= nch; a++) {
+              int pd[3], s[3];
+              pd[d] = inb; pd[ta] = ioff[ta] + a;
+              s[d] = ii; s[ta] = 2 * a - 1;
+              if (t->dim == 3) { pd[tb] = ioff[tb] + b; s[tb] = 2 * b - 1; }
+              size_t g = FI(nc, t->dim, d, pd[0], pd[1], t->dim == 3 ? pd[2] : 1);
+              double lam = dst[g];
+              dst[g] = 0;
+              if (t->dim == 2) {
+                src[FI(nc, 2, d, s[0], s[1], 1)] += 0.5 * lam;
+                s[ta] += 1;
+                src[FI(nc, 2, d, s[0], s[1], 1)] += 0.5 * lam;
+              } else {
+                for (int m = 0; m < 4; m++) {
+                  int s0[3] = {s[0], s[1], s[2]};
+                  s0[ta] += m & 1;
+                  s0[tb] += m >> 1;
+                  src[FI(nc, 3, d, s0[0], s0[1], s0[2])] += 0.25 * lam;
+                }
+              }
+            }
+        }
+      }
+    }
+  }
+}
+
+static void gc2_adj(otree *t, int id, int iv, int nb, int x, int b, double lam) {
+  /* adjoint of one second-layer ghost value of o_gc2_box (face nb,
+   * tangential cell (x, b)): lam goes to the cells it was taken from */
+  int nc = t->ncell[id], D = t->dim;
+  int nb_id = NBP(t, id)[nb - 1], d = nb_dim(nb);
+  int ta = (d == 0) ? 1 : 0, tb = (d == 2) ? 1 : 2;
+  if (nb_id != NO_BOX) {
+    double *dst = CC(t, nb_id > NO_BOX ? nb_id : id, iv);
+    int p[3];
+    if (nb_id > NO_BOX) p[d] = nb_low(nb) ? nc - 1 : 2;
+    else p[d] = nb_low(nb) ? 2 : nc - 1;
+    p[ta] = x;
+    if (D == 3) p[tb] = b;
+    dst[D == 2 ? I2(nc, p[0], p[1]) : I3(nc, p[0], p[1], p[2])] += lam;
+    return;
+  }
+  int p = t->parent[id], pn = NBP(t, p)[nb - 1], off[3];
+  child_offset(t, id, nb, off);
+  int pnc = t->ncell[pn];
+  double *q = CC(t, pn, iv);
+  int ix = nb_hi01(nb) * (nc + 3) - 1;
+  int n1 = off[d] + ((ix + 1) >> 1), n2 = n1 + 1 - 2 * (ix & 1);
+  int c1 = off[ta] + ((x + 1) >> 1), c2 = c1 + 1 - 2 * (x & 1);
+  if (D == 2) {
+    int P0[2], P1[2], P2[2];
+    P0[d] = n1; P0[ta] = c1;
+    P1[d] = n2; P1[ta] = c1;
+    P2[d] = n1; P2[ta] = c2;
+    q[I2(pnc, P0[0], P0[1])] += 0.5 * lam;
+    q[I2(pnc, P1[0], P1[1])] += 0.25 * lam;
+    q[I2(pnc, P2[0], P2[1])] += 0.25 * lam;
+    return;
+  }
+  int e1 = off[tb] + ((b + 1) >> 1), e2 = e1 + 1 - 2 * (b & 1);
+  int P[4][3];
+  for (int m = 0; m < 4; m++) { P[m][d] = n1; P[m][ta] = c1; P[m][tb] = e1; }
+  P[1][d] = n2; P[2][ta] = c2; P[3][tb] = e2;
+  for (int m = 0; m < 4; m++) q[I3(pnc, P[m][0], P[m][1], P[m][2])] += 0.25 * lam;
+}
+
+static void koren_mlim_d(double a, double b, double *da, double *db) {
+  /* the derivatives of o_koren_mlim in the case it takes for (a, b) */
+  double aa = a * a, ab = a * b;
+  if (ab <= 0) { *da = 0; *db = 0; }
+  else if (aa <= 0.25 * ab) { *da = 1; *db = 0; }
+  else if (aa <= 2.5 * ab) { *da = 1 / 3.0; *db = 1 / 6.0; }
+  else { *da = 0; *db = 1; }
+}
+
+void o_fluxes_koren_adj(otree *t, const o_vars *v, const otable *tbl, double dt,
+                        const o_adj *a, double *tgrad) {
+  const double eps0 = 8.8541878176e-12, qe = 1.6022e-19;
+  double fac = -0.8 * eps0 / (qe * dt);
+  int D = t->dim;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], nc = t->ncell[id];
+      double inv_dr = 1 / t->dr[id];
+      size_t fl = D == 2 ? (size_t)nc : (size_t)nc * nc;
+      double *gc = malloc(sizeof(double) * fl * t->nnb);
+      o_gc2_box(t, id, v->elec, gc);
+      const double *n = CC(t, id, v->elec), *E = CC(t, id, v->fld);
+      double *ln = CC(t, id, a->le), *lE = CC(t, id, a->lE);
+      for (int d = 0; d < D; d++) {
+        const double *Ff = FC(t, id, v->f_fld, d);
+        double *lFe = FC(t, id, a->lF, d), *lFf = FC(t, id, a->lfld, d);
+        int n0 = nc + (d == 0), n1 = nc + (d == 1), n2 = D == 3 ? nc + (d == 2) : 1;
+        int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        for (int k = 1; k <= n2; k++)
+          for (int j = 1; j <= n1; j++)
+            for (int i = 1; i <= n0; i++) {
+              int c[3] = {i, j, D == 3 ? k : 0};
+#define CI(a0, a1, a2) (D == 2 ? I2(nc, a0, a1) : I3(nc, a0, a1, a2))
+              int m1[3] = {c[0], c[1], c[2]}, m2[3] = {c[0], c[1], c[2]}, p1[3] = {c[0], c[1], c[2]};
+              m1[d] -= 1;
+              m2[d] -= 2;
+              p1[d] += 1;
+              int pos = c[d];
+              size_t ic = CI(c[0], c[1], c[2]), im = CI(m1[0], m1[1], m1[2]);
+              size_t fi = FI(nc, D, d, i, j, k);
+              double lam = lFe[fi];
+              lFe[fi] = 0;
+              if (lam == 0) continue;
+              double fld_avg = 0.5 * (E[ic] + E[im]);
+              int li;
+              double lf;
+              lt_loc(tbl, fld_avg, &li, &lf);
+              double mob = lt_col_at(tbl, v->t_mob, li, lf);
+              double dif = lt_col_at(tbl, v->t_dif, li, lf);
+              double fld = Ff[fi];
+              double vd = -mob * fld;
+              double nc0 = n[ic], nm1 = n[im];
+              double gradc = nc0 - nm1;
+              size_t gpos = (size_t)(c[ta] - 1) + (D == 3 ? (size_t)nc * (c[tb] - 1) : 0);
+              int tb_ = D == 3 ? c[tb] : 1;
+              /* F = f - dif gradc / dr */
+              double ldif = -lam * gradc * inv_dr;
+              double lgradc = -lam * dif * inv_dr;
+              double lvd = 0, lfld = 0;
+              if (vd < 0.0) {
+                int ghost = pos == nc + 1;
+                double tmp = ghost ? gc[(size_t)(2 * d + 1) * fl + gpos] : n[CI(p1[0], p1[1], p1[2])];
+                double gradn = tmp - nc0;
+                double lim = o_koren_mlim(gradc, gradn);
+                double f = vd * (nc0 - lim);
+                if (f < fac * fld) {
+                  lfld += lam * fac;
+                } else {
+                  double da, db;
+                  koren_mlim_d(gradc, gradn, &da, &db);
+                  lvd = lam * (nc0 - lim);
+                  double llim = -lam * vd;
+                  ln[ic] += lam * vd;
+                  lgradc += llim * da;
+                  double lgn = llim * db;
+                  ln[ic] -= lgn;
+                  if (ghost) gc2_adj(t, id, a->le, 2 * d + 2, c[ta], tb_, lgn);
+                  else ln[CI(p1[0], p1[1], p1[2])] += lgn;
+                }
+              } else {
+                int ghost = pos == 1;
+                double tmp = ghost ? gc[(size_t)(2 * d) * fl + gpos] : n[CI(m2[0], m2[1], m2[2])];
+                double gradp = nm1 - tmp;
+                double lim = o_koren_mlim(gradc, gradp);
+                double f = vd * (nm1 + lim);
+                if (f > fac * fld) {
+                  lfld += lam * fac;
+                } else {
+                  double da, db;
+                  koren_mlim_d(gradc, gradp, &da, &db);
+                  lvd = lam * (nm1 + lim);
+                  double llim = lam * vd;
+                  ln[im] += lam * vd;
+                  lgradc += llim * da;
+                  double lgp = llim * db;
+                  ln[im] += lgp;
+                  if (ghost) gc2_adj(t, id, a->le, 2 * d + 1, c[ta], tb_, -lgp);
+                  else ln[CI(m2[0], m2[1], m2[2])] -= lgp;
+                }
+              }
+              ln[ic] += lgradc;
+              ln[im] -= lgradc;
+              /* vd = -mob fld */
+              double lmob = -lvd * fld;
+              lfld += -lvd * mob;
+              lFf[fi] += lfld;
+              double dm, dd;
+              lt_dcol(tbl, v->t_mob, fld_avg, li, lf, &dm);
+              lt_dcol(tbl, v->t_dif, fld_avg, li, lf, &dd);
+              double lx = lmob * dm + ldif * dd;
+              lE[ic] += 0.5 * lx;
+              lE[im] += 0.5 * lx;
+              lt_col_adj(tbl, v->t_mob, li, lf, lmob, tgrad);
+              lt_col_adj(tbl, v->t_dif, li, lf, ldif, tgrad);
+#undef CI
+            }
+      }
+      free(gc);
+    }
+  }
+}
+
+void o_lt_set_rows(otable *t, const double *rc) {
+  /* the table's rows_cols(n_rows, n_cols) as given (test perturbations) */
+  memcpy(t->rc, rc, sizeof(double) * (size_t)t->n_rows * t->n_cols);
+}

@@ -122,6 +122,9 @@ double o_lt_get_col(const otable *t, int col, double x);
 /* rows_cols(n_rows, n_cols), column-major, and x_min / inv_dx */
 void o_lt_data(const otable *t, double *rows_cols, double *x_min, double *inv_dx);
 
+/* overwrite the table's rows_cols(n_rows, n_cols) (column-major) */
+void o_lt_set_rows(otable *t, const double *rc);
+
 /* norm2 of the flang runtime (scaled sum of squares) */
 double o_norm2(int n, const double *x);
 
@@ -158,6 +161,28 @@ void o_fluxes_koren(otree *t, const o_vars *v, const otable *tbl, double dt);
 void o_update_solution(otree *t, const o_vars *v, const otable *tbl, double dt);
 /* average_dens on every box, whole boxes */
 void o_average_dens(otree *t, const o_vars *v);
+/* Reverse mode of one transport substep on a frozen mesh (no reference
+ * counterpart). Adjoint variables: cell variables le, li (n_e, n_i), lE
+ * (|E|), lpho (photoionization source) and face variables lF (electron
+ * flux), lfld (field). Each routine takes the adjoints of its outputs and
+ * adds those of its inputs; tgrad (n_cols x n_rows, column-major, or NULL)
+ * accumulates the adjoints of the table rows.
+ *  update_solution_adj: in le, li the adjoints of n_e', n_i' on leaf
+ *    interiors; out: those of n_e, n_i, plus lF, lE, lpho, tgrad(alpha, eta);
+ *  consistent_fluxes_adj: moves lF from the coarse faces the forward pass
+ *    overwrote to the fine faces it averaged, clearing the coarse ones;
+ *  fluxes_koren_adj: consumes lF on leaves (clearing it) into le (cells and
+ *    first- and second-layer ghost sources), lE, lfld, tgrad(mobility,
+ *    diffusion). */
+typedef struct {
+  int le, li, lE, lpho; /* cell variables */
+  int lF, lfld;         /* face variables */
+} o_adj;
+void o_update_solution_adj(otree *t, const o_vars *v, const otable *tbl, double dt,
+                           const o_adj *a, double *tgrad);
+void o_consistent_fluxes_adj(otree *t, int lF);
+void o_fluxes_koren_adj(otree *t, const o_vars *v, const otable *tbl, double dt,
+                        const o_adj *a, double *tgrad);
 /* per-box maxima for the refinement criterion: out[id-1] = (dr^2 max|rhs|,
  * max fld, max elec) over the interior, boxes of levels >= 1 */
 void o_box_summaries(const otree *t, const o_vars *v, double *out);

@@ -344,6 +344,11 @@ class OTable:
     def get_col(self, col, x):
         return self.L.o_lt_get_col(self.h, col, x)
 
+    def set_rows(self, rows_cols):
+        """Overwrite the table with rows_cols (n_cols, n_rows), as data() gives it."""
+        a, p = _d(np.asarray(rows_cols, float).reshape(-1))
+        self.L.o_lt_set_rows(self.h, p)
+
     def data(self):
         rc = np.zeros(self.n_rows * self.n_cols)
         xm, idx = C.c_double(), C.c_double()
@@ -378,6 +383,28 @@ def average_dens(T, v):
 
 def photoi_rate(T, v, tbl, coeff):
     T.L.o_photoi_rate(T.h, C.byref(v), tbl.h, C.c_double(coeff))
+
+
+class OAdj(C.Structure):
+    """Adjoint variables of the reverse-mode transport substep: cell
+    variables le, li, lE, lpho and face variables lF, lfld."""
+    _fields_ = [(n, C.c_int) for n in ("le", "li", "lE", "lpho", "lF", "lfld")]
+
+
+def _tg(tgrad):
+    return None if tgrad is None else tgrad.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def update_solution_adj(T, v, tbl, dt, a, tgrad=None):
+    T.L.o_update_solution_adj(T.h, C.byref(v), tbl.h, C.c_double(dt), C.byref(a), _tg(tgrad))
+
+
+def consistent_fluxes_adj(T, lF):
+    T.L.o_consistent_fluxes_adj(T.h, C.c_int(lF))
+
+
+def fluxes_koren_adj(T, v, tbl, dt, a, tgrad=None):
+    T.L.o_fluxes_koren_adj(T.h, C.byref(v), tbl.h, C.c_double(dt), C.byref(a), _tg(tgrad))
 
 
 def box_summaries(T, v):
