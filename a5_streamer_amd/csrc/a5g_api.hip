@@ -382,6 +382,9 @@ void Pool::release() {
   f(d_work);
   d_work = nullptr;
   work_bytes = 0;
+  f(d_tgrad);
+  d_tgrad = nullptr;
+  tgrad_n = 0;
   d_inst = nullptr;
   d_flux_pairs = nullptr;
   d_diag = nullptr;
@@ -1152,6 +1155,60 @@ int a5g_box_summaries(a5g_tree* t, double* out) {
   FLUSH(t);
   need_gpu(t->P);
   a5g::box_summaries(t->P, t->P.sv, out);
+  API_END
+}
+
+static a5g::AdjVars adj_vars(const a5g::Tree& T, const int* adj, bool photoi) {
+  if (!adj) throw std::invalid_argument("adjoint variables not given");
+  a5g::AdjVars av;
+  av.le = adj[0];
+  av.li = adj[1];
+  av.lE = adj[2];
+  av.lpho = adj[3];
+  av.lF = adj[4];
+  av.lfld = adj[5];
+  check_iv(T, av.le);
+  check_iv(T, av.li);
+  check_iv(T, av.lE);
+  if (photoi) check_iv(T, av.lpho);
+  for (int f : {av.lF, av.lfld})
+    if (f < 1 || f > T.n_fc) throw std::out_of_range("face variable out of range");
+  return av;
+}
+
+int a5g_update_solution_adjoint(a5g_tree* t, double dt, const int* adj) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::update_solution_adj(t->P, t->P.sv, dt, adj_vars(t->P.tree, adj, t->P.sv.pho > 0));
+  API_END
+}
+
+int a5g_consistent_fluxes_adjoint(a5g_tree* t, int lF) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (lF < 1 || lF > t->P.tree.n_fc) throw std::out_of_range("face variable out of range");
+  a5g::consistent_fluxes_adj(t->P, lF);
+  API_END
+}
+
+int a5g_fluxes_koren_adjoint(a5g_tree* t, double dt, const int* adj) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::fluxes_koren_adj(t->P, t->P.sv, dt, adj_vars(t->P.tree, adj, false));
+  API_END
+}
+
+int a5g_table_adjoint(a5g_tree* t, int clear, double* rows_cols) {
+  API_BEGIN
+  NEED(t);
+  need_gpu(t->P);
+  a5g::table_adjoint(t->P, clear != 0, rows_cols);
   API_END
 }
 

@@ -34,6 +34,11 @@ void consistent_fluxes(Pool& P, int f_elec);
 void update_solution(Pool& P, const StVars& v, double dt);
 void average_dens(Pool& P, const StVars& v);
 void box_summaries(Pool& P, const StVars& v, double* out_host);
+// reverse mode of the transport substep (a5g_*_adjoint)
+void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av);
+void consistent_fluxes_adj(Pool& P, int lF);
+void fluxes_koren_adj(Pool& P, const StVars& v, double dt, const AdjVars& av);
+void table_adjoint(Pool& P, bool clear, double* out_host);
 // the same with the library's fixed device scratch instead of allocations
 // (a5g_box_summaries_noalloc); noalloc_init looks the scratch up for the
 // current device

@@ -82,6 +82,13 @@ struct StVars {
   int dielectric_rule = 1;
 };
 
+// adjoint variables of the reverse-mode transport substep: cell variables
+// le, li (densities), lE (|E|), lpho (photoionization source); face
+// variables lF (electron flux), lfld (field)
+struct AdjVars {
+  int le = -1, li = -1, lE = -1, lpho = -1, lF = -1, lfld = -1;
+};
+
 struct MgOpts {
   int i_phi = -1, i_tmp = -1, i_rhs = -1, i_eps = -1;
   int n_down = 2, n_up = 2, n_base = 2;
@@ -147,6 +154,9 @@ struct Pool {
   // transport lookup table on the device: rows_cols(n_rows, n_cols)
   double* d_table = nullptr;
   int tbl_rows = 0, tbl_cols = 0;
+  // adjoints of the table rows accumulated by the reverse-mode kernels
+  double* d_tgrad = nullptr;
+  size_t tgrad_n = 0;
   double tbl_xmin = 0, tbl_inv_dx = 0;
   std::vector<double> h_table;  // host copy while it is being built
   bool prof_on = false;

@@ -441,6 +441,339 @@ __global__ __launch_bounds__(256) void k_average(Dev a, const int* __restrict__ 
   }
 }
 
+// out[0..n) = 0
+__global__ void k_zero(double* __restrict__ out, size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    out[q] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of one transport substep on a frozen mesh: update_solution,
+// a$D_consistent_fluxes and fluxes_koren (no counterpart in the reference,
+// which has no derivatives). Each kernel takes the adjoints of its outputs
+// and adds those of its inputs, taking every branch (upwind side, limiter
+// case, flux clamp, table row, dielectric rule) the forward pass takes on
+// the same data. The adjoint variables: cell variables le, li (densities),
+// lE (|E|), lpho (photoionization source); face variables lF (electron flux),
+// lfld (field); table rows in tg[col][row].
+// ---------------------------------------------------------------------------
+
+// d/dx of a table column at x (lt_col at lt_loc(x)): the row interval's
+// slope, 0 where the location is clamped to the table ends
+__device__ __forceinline__ double lt_dcol(const Tbl& t, int col, double x, int li) {
+  const double frac = (x - t.x_min) * t.inv_dx;
+  const int raw = (int)ceil(frac);
+  if (raw < 1 || raw >= t.n_rows) return 0.0;
+  const double* c = t.rc + (size_t)(col - 1) * t.n_rows;
+  return (c[li] - c[li - 1]) * t.inv_dx;
+}
+
+__device__ __forceinline__ void lt_col_adj(const Tbl& t, double* tg, int col, int li, double lf,
+                                           double lam) {
+  if (lam == 0.0) return;
+  double* g = tg + (size_t)(col - 1) * t.n_rows;
+  atomicAdd(g + li - 1, lf * lam);
+  atomicAdd(g + li, (1 - lf) * lam);
+}
+
+// update_solution reverse: one lane per leaf cell. le, li hold the adjoints
+// of n_e', n_i' on entry and those of n_e, n_i on exit; each face of lF gets
+// the contributions of the (one or two) cells beside it, so with lF zero on
+// entry the result does not depend on their order.
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_update_adj(Dev a, Tbl tb, const int* __restrict__ list,
+                                                    int n, int nc, int tpb, int ifld, int ieps,
+                                                    int fel, int calpha, int ceta, int diel,
+                                                    int photoi, int le, int li, int lE, int lph,
+                                                    int lF, double* __restrict__ tg,
+                                                    double dt_all) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double dt = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstDt] : dt_all;
+  const double inv_dr = 1 / a.dr[id];
+  const double* E = boxp(a, ifld, id);
+  const double* eps = boxp(a, ieps, id);
+  double* Le = boxp(a, le, id);
+  double* Li = boxp(a, li, id);
+  double* LEf = boxp(a, lE, id);
+  double* Lph = photoi ? boxp(a, lph, id) : nullptr;
+  const double* F[3];
+  double* LF[3];
+  for (int d = 0; d < D; d++) {
+    F[d] = facep(a, fel, id, d, D);
+    LF[d] = facep(a, lF, id, d, D);
+  }
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c = cidx<D>(nc, i, j, k);
+    double le_pre = Le[c], lp_pre = Li[c];
+    if (diel && eps[c] > 1.0) le_pre = -lp_pre;  // n_i' = n_i* - n_e*, n_e' = 0
+    Le[c] = le_pre;
+    Li[c] = lp_pre;
+    const double lsrc = le_pre * dt + lp_pre * dt;
+    const double s = le_pre * dt * inv_dr;
+    int ix;
+    double lf;
+    lt_loc(tb, E[c], ix, lf);
+    const double alpha = lt_col(tb, calpha, ix, lf);
+    const double eta = lt_col(tb, ceta, ix, lf);
+    int fl[3], fh[3];
+    double df[3] = {0, 0, 0};
+    for (int d = 0; d < D; d++) {
+      int h[3] = {i, j, k};
+      h[d] += 1;
+      fl[d] = fidx<D>(nc, d, i, j, k);
+      fh[d] = fidx<D>(nc, d, h[0], h[1], h[2]);
+      df[d] = F[d][fl[d]] + F[d][fh[d]];
+    }
+    const double nrm = D == 2 ? norm2_flang<2>(df) : norm2_flang<3>(df);
+    const double lnrm = lsrc * 0.5 * (alpha - eta);
+    const double lal = lsrc * 0.5 * nrm, let = -lsrc * 0.5 * nrm;
+    if (Lph) Lph[c] += lsrc;
+    LEf[c] += lal * lt_dcol(tb, calpha, E[c], ix) + let * lt_dcol(tb, ceta, E[c], ix);
+    lt_col_adj(tb, tg, calpha, ix, lf, lal);
+    lt_col_adj(tb, tg, ceta, ix, lf, let);
+    for (int d = 0; d < D; d++) {
+      const double ldf = nrm > 0 ? lnrm * df[d] / nrm : 0.0;
+      double wl = 1.0, wh = 1.0;
+      if (CYL && d == 0) {
+        const double den = i + ioff - 0.5;
+        wl = (double)(i + ioff - 1) / den;
+        wh = (double)(i + ioff) / den;
+      }
+      atomicAdd(LF[d] + fl[d], ldf + wl * s);
+      atomicAdd(LF[d] + fh[d], ldf - wh * s);
+    }
+  }
+}
+
+// consistent fluxes reverse: one lane per coarse face cell of a (parent,
+// direction) pair; its adjoint goes to the fine faces it averaged and is
+// cleared (each fine face feeds one coarse face)
+template <int D>
+__global__ __launch_bounds__(256) void k_flux_fix_adj(Dev a, const int2* __restrict__ pairs,
+                                                      int n, int nc, int lF) {
+  const int hnc = nc >> 1;
+  const int per = D == 2 ? nc : nc * nc;
+  const long total = (long)n * per;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < total;
+       q += (long)gridDim.x * blockDim.x) {
+    const int pi = (int)(q / per), w = (int)(q - (long)pi * per);
+    const int id = pairs[pi].x, nb = pairs[pi].y;
+    const int d = d_nb_dim(nb);
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int nb_id = a.nbs[id * 6 + nb - 1];
+    const bool low = d_nb_low(nb);
+    const int ii = low ? 1 : nc + 1, inb = low ? nc + 1 : 1;
+    const int a_ = w % nc + 1, b_ = D == 3 ? w / nc + 1 : 1;
+    const int ca = (a_ - 1) / hnc, cb = D == 3 ? (b_ - 1) / hnc : 0;
+    int cdix[3] = {0, 0, 0};
+    cdix[d] = low ? 0 : 1;
+    cdix[ta] = ca;
+    if (D == 3) cdix[tb] = cb;
+    const int ich = 1 + cdix[0] + 2 * cdix[1] + 4 * cdix[2];
+    const int cid = a.children[id * 8 + ich - 1];
+    const int sa = 2 * (a_ - ca * hnc) - 1, sb = D == 3 ? 2 * (b_ - cb * hnc) - 1 : 1;
+    double* src = facep(a, lF, cid, d, D);
+    double* dst = facep(a, lF, nb_id, d, D);
+    int s[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+    s[d] = ii;
+    s[ta] = sa;
+    t[d] = inb;
+    t[ta] = a_;
+    if (D == 3) {
+      s[tb] = sb;
+      t[tb] = b_;
+    }
+    const int g = fidx<D>(nc, d, t[0], t[1], t[2]);
+    const double lam = dst[g];
+    dst[g] = 0;
+    if (D == 2) {
+      src[fidx<D>(nc, d, s[0], s[1], 0)] += 0.5 * lam;
+      s[ta] += 1;
+      src[fidx<D>(nc, d, s[0], s[1], 0)] += 0.5 * lam;
+    } else {
+      for (int m = 0; m < 4; m++) {
+        int s0[3] = {s[0], s[1], s[2]};
+        s0[ta] += m & 1;
+        s0[tb] += m >> 1;
+        src[fidx<D>(nc, d, s0[0], s0[1], s0[2])] += 0.25 * lam;
+      }
+    }
+  }
+}
+
+// the adjoint of one second-layer ghost value of gc2_value: lam goes to the
+// cells it was taken from (a neighbour's, the box's own, or the parent's
+// neighbour's)
+template <int D>
+__device__ void gc2_adj(const Dev& a, int iv, int id, int nc, int nb, int t1, int t2, double lam) {
+  const int d = d_nb_dim(nb);
+  const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+  const bool low = d_nb_low(nb);
+  const int nb_id = a.nbs[id * 6 + nb - 1];
+  int c[3] = {0, 0, 0};
+  c[ta] = t1;
+  if (D == 3) c[tb] = t2;
+  if (nb_id > 0) {
+    c[d] = low ? nc - 1 : 2;
+    atomicAdd(boxp(a, iv, nb_id) + cidx<D>(nc, c[0], c[1], c[2]), lam);
+    return;
+  }
+  if (nb_id < 0) {
+    c[d] = low ? 2 : nc - 1;
+    atomicAdd(boxp(a, iv, id) + cidx<D>(nc, c[0], c[1], c[2]), lam);
+    return;
+  }
+  const int pid = a.parent[id];
+  const int pn = a.nbs[pid * 6 + nb - 1];
+  const int pnc = a.ncell[pn];
+  int off[3];
+  child_off<D>(a, id, nc, nb, off);
+  double* q = boxp(a, iv, pn);
+  const int ix = low ? -1 : nc + 2;
+  const int n1 = off[d] + ((ix + 1) >> 1), n2 = n1 + 1 - 2 * (ix & 1);
+  const int c1 = off[ta] + ((t1 + 1) >> 1), c2 = c1 + 1 - 2 * (t1 & 1);
+  if (D == 2) {
+    int s0[3] = {0, 0, 0}, s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    s0[d] = n1; s0[ta] = c1;
+    s1[d] = n2; s1[ta] = c1;
+    s2[d] = n1; s2[ta] = c2;
+    atomicAdd(q + cidx<D>(pnc, s0[0], s0[1], 0), 0.5 * lam);
+    atomicAdd(q + cidx<D>(pnc, s1[0], s1[1], 0), 0.25 * lam);
+    atomicAdd(q + cidx<D>(pnc, s2[0], s2[1], 0), 0.25 * lam);
+    return;
+  }
+  const int e1 = off[tb] + ((t2 + 1) >> 1), e2 = e1 + 1 - 2 * (t2 & 1);
+  int P4[4][3];
+  for (int m = 0; m < 4; m++) {
+    P4[m][d] = n1;
+    P4[m][ta] = c1;
+    P4[m][tb] = e1;
+  }
+  P4[1][d] = n2;
+  P4[2][ta] = c2;
+  P4[3][tb] = e2;
+  for (int m = 0; m < 4; m++) atomicAdd(q + cidx<D>(pnc, P4[m][0], P4[m][1], P4[m][2]), 0.25 * lam);
+}
+
+__device__ __forceinline__ void koren_mlim_d(double x, double y, double& dx, double& dy) {
+  const double aa = x * x, ab = x * y;
+  if (ab <= 0) { dx = 0; dy = 0; }
+  else if (aa <= 0.25 * ab) { dx = 1; dy = 0; }
+  else if (aa <= 2.5 * ab) { dx = 1 / 3.0; dy = 1 / 6.0; }
+  else { dx = 0; dy = 1; }
+}
+
+// fluxes_koren reverse: one lane per leaf face; consumes lF (clearing it)
+template <int D>
+__global__ __launch_bounds__(256) void k_koren_adj(Dev a, Tbl tb, const int* __restrict__ list,
+                                                   int n, int nc, int tpb, int ie, int ifld,
+                                                   int ffld, int cmob, int cdif, int le, int lE,
+                                                   int lF, int lfld, double* __restrict__ tg,
+                                                   double fac_all) {
+  BOXES_PER_BLOCK
+  if (!act) return;
+  const double fac = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstKorenFac]
+                              : fac_all;
+  const double inv_dr = 1 / a.dr[id];
+  const double* ne = boxp(a, ie, id);
+  const double* E = boxp(a, ifld, id);
+  double* Ln = boxp(a, le, id);
+  double* LEf = boxp(a, lE, id);
+  const int per = (nc + 1) * (D == 2 ? nc : nc * nc);
+  for (int q = lt; q < D * per; q += tpb) {
+    const int d = q / per, w = q - d * per;
+    double* LFd = facep(a, lF, id, d, D);
+    const double lam = LFd[w];
+    LFd[w] = 0;
+    if (lam == 0.0) continue;
+    const int n0 = nc + (d == 0), n1 = nc + (d == 1);
+    int c[3];
+    c[0] = w % n0 + 1;
+    c[1] = (w / n0) % n1 + 1;
+    c[2] = D == 3 ? w / (n0 * n1) + 1 : 0;
+    int m1[3] = {c[0], c[1], c[2]};
+    m1[d] -= 1;
+    const int pos = c[d];
+    const int ic = cidx<D>(nc, c[0], c[1], c[2]);
+    const int im = cidx<D>(nc, m1[0], m1[1], m1[2]);
+    const double fld_avg = 0.5 * (E[ic] + E[im]);
+    int li;
+    double lf;
+    lt_loc(tb, fld_avg, li, lf);
+    const double mob = lt_col(tb, cmob, li, lf);
+    const double dif = lt_col(tb, cdif, li, lf);
+    const double fld = facep(a, ffld, id, d, D)[w];
+    const double vd = -mob * fld;
+    const double n0v = ne[ic], nm1 = ne[im];
+    const double gradc = n0v - nm1;
+    const int ta = d == 0 ? 1 : 0, tbb = d == 2 ? 1 : 2;
+    const int t2 = D == 3 ? c[tbb] : 0;
+    const double ldif = -lam * gradc * inv_dr;
+    double lgradc = -lam * dif * inv_dr;
+    double lvd = 0, lfl = 0;
+    if (vd < 0.0) {
+      const bool ghost = pos == nc + 1;
+      int p1[3] = {c[0], c[1], c[2]};
+      p1[d] += 1;
+      const double tmp =
+          ghost ? gc2_value<D>(a, ie, id, nc, 2 * d + 2, c[ta], t2) : ne[cidx<D>(nc, p1[0], p1[1], p1[2])];
+      const double gradn = tmp - n0v;
+      const double lim = koren_mlim(gradc, gradn);
+      const double f = vd * (n0v - lim);
+      if (f < fac * fld) {
+        lfl += lam * fac;
+      } else {
+        double da, db;
+        koren_mlim_d(gradc, gradn, da, db);
+        lvd = lam * (n0v - lim);
+        const double llim = -lam * vd;
+        lgradc += llim * da;
+        const double lgn = llim * db;
+        atomicAdd(Ln + ic, lam * vd - lgn);
+        if (ghost) gc2_adj<D>(a, le, id, nc, 2 * d + 2, c[ta], t2, lgn);
+        else atomicAdd(Ln + cidx<D>(nc, p1[0], p1[1], p1[2]), lgn);
+      }
+    } else {
+      const bool ghost = pos == 1;
+      int m2[3] = {c[0], c[1], c[2]};
+      m2[d] -= 2;
+      const double tmp =
+          ghost ? gc2_value<D>(a, ie, id, nc, 2 * d + 1, c[ta], t2) : ne[cidx<D>(nc, m2[0], m2[1], m2[2])];
+      const double gradp = nm1 - tmp;
+      const double lim = koren_mlim(gradc, gradp);
+      const double f = vd * (nm1 + lim);
+      if (f > fac * fld) {
+        lfl += lam * fac;
+      } else {
+        double da, db;
+        koren_mlim_d(gradc, gradp, da, db);
+        lvd = lam * (nm1 + lim);
+        const double llim = lam * vd;
+        lgradc += llim * da;
+        const double lgp = llim * db;
+        atomicAdd(Ln + im, lam * vd + lgp);
+        if (ghost) gc2_adj<D>(a, le, id, nc, 2 * d + 1, c[ta], t2, -lgp);
+        else atomicAdd(Ln + cidx<D>(nc, m2[0], m2[1], m2[2]), -lgp);
+      }
+    }
+    atomicAdd(Ln + ic, lgradc);
+    atomicAdd(Ln + im, -lgradc);
+    const double lmob = -lvd * fld;
+    lfl += -lvd * mob;
+    facep(a, lfld, id, d, D)[w] += lfl;
+    const double lx = lmob * lt_dcol(tb, cmob, fld_avg, li) + ldif * lt_dcol(tb, cdif, fld_avg, li);
+    atomicAdd(LEf + ic, 0.5 * lx);
+    atomicAdd(LEf + im, 0.5 * lx);
+    lt_col_adj(tb, tg, cmob, li, lf, lmob);
+    lt_col_adj(tb, tg, cdif, li, lf, ldif);
+  }
+}
+
 // per-box maxima of set_ref_flags (src/streamer_2d.f90:186-192):
 // (dr^2 max|rhs|, max fld, max elec) over the interior
 template <int D>
@@ -695,12 +1028,10 @@ void fluxes_koren(Pool& P, const StVars& v, double dt) {
   }
 }
 
-void consistent_fluxes(Pool& P, int f_elec) {
-  // a$D_consistent_fluxes: (parent, direction) pairs whose neighbour is a
-  // leaf, per level; the list is rebuilt when the topology changes
-  P.sync_topology();
+// a$D_consistent_fluxes: (parent, direction) pairs whose neighbour is a
+// leaf, per level; the list is rebuilt when the topology changes
+static void ensure_flux_pairs(Pool& P) {
   const auto& T = P.tree;
-  const int D = T.dim;
   const int nl = T.max_lvl - T.min_lvl;
   if (P.flux_pairs_version != P.topo_version) {
     std::vector<int2> pairs;
@@ -728,6 +1059,13 @@ void consistent_fluxes(Pool& P, int f_elec) {
     }
     P.flux_pairs_version = P.topo_version;
   }
+}
+
+void consistent_fluxes(Pool& P, int f_elec) {
+  P.sync_topology();
+  ensure_flux_pairs(P);
+  const auto& T = P.tree;
+  const int D = T.dim;
   for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
     const size_t np = P.flux_pairs_n[lvl - T.min_lvl];
     if (np == 0) continue;
@@ -835,13 +1173,6 @@ void axpy_leaves(Pool& P, int iin, int iout, double s, bool add) {
                          P.leaves(lvl), n, nc, g.tpb, iin, iout, s, (int)add);
     chk(hipGetLastError(), "axpy_leaves");
   }
-}
-
-// out[0..n) = 0
-__global__ void k_zero(double* __restrict__ out, size_t n) {
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
-       q += (size_t)gridDim.x * blockDim.x)
-    out[q] = 0.0;
 }
 
 // The per-box maxima of every box of levels >= 1 into d_out (3 max_id
@@ -971,6 +1302,106 @@ void fill_boxes(Pool& P, int n_ids, const int* ids, int iv, double value) {
 void gc_boxes(Pool& P, int n_ids, const int* ids, int iv, int rb, int bc) {
   P.sync_topology();
   by_level(P, n_ids, ids, [&](int lvl, const int* d, int n) { gc_list(P, lvl, d, n, iv, rb, bc); });
+}
+
+// host side of the reverse-mode kernels
+
+static double* table_grad(Pool& P) {
+  if (!P.d_table) throw std::logic_error("no transport table: call a5g_set_table first");
+  const size_t n = (size_t)P.tbl_rows * P.tbl_cols;
+  if (!P.d_tgrad || P.tgrad_n != n) {
+    if (P.d_tgrad) chk(hipFree(P.d_tgrad), "hipFree");
+    P.d_tgrad = nullptr;
+    chk(hipMalloc((void**)&P.d_tgrad, n * sizeof(double)), "hipMalloc table adjoint");
+    P.tgrad_n = n;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256),
+                       0, P.stream, P.d_tgrad, n);
+    chk(hipGetLastError(), "table adjoint zero");
+  }
+  return P.d_tgrad;
+}
+
+void table_adjoint(Pool& P, bool clear, double* out_host) {
+  double* g = table_grad(P);
+  if (out_host) {
+    chk(hipMemcpyAsync(out_host, g, P.tgrad_n * sizeof(double), hipMemcpyDeviceToHost, P.stream),
+        "table adjoint");
+    chk(hipStreamSynchronize(P.stream), "table adjoint sync");
+  }
+  if (clear) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((P.tgrad_n + 255) / 256, 1024)),
+                       dim3(256), 0, P.stream, g, P.tgrad_n);
+    chk(hipGetLastError(), "table adjoint zero");
+  }
+}
+
+void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  Tbl tb = device_table(P);
+  double* tg = table_grad(P);
+  const bool cyl = D == 2 && P.tree.coord == 1;
+  const int photoi = v.pho > 0 ? 1 : 0;
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, box_cells(D, nc, false));
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb, P.leaves(lvl), n,
+                         nc, g.tpb, v.fld, v.eps, v.f_elec, v.t_alpha, v.t_eta,
+                         v.dielectric_rule, photoi, av.le, av.li, av.lE, av.lpho, av.lF, tg, dt);
+    };
+    if (D == 3) go(k_update_adj<3, false>);
+    else if (cyl) go(k_update_adj<2, true>);
+    else go(k_update_adj<2, false>);
+    chk(hipGetLastError(), "update_solution_adj");
+  }
+}
+
+void consistent_fluxes_adj(Pool& P, int lF) {
+  P.sync_topology();
+  ensure_flux_pairs(P);
+  const auto& T = P.tree;
+  const int D = T.dim;
+  for (int lvl = T.min_lvl; lvl <= T.max_lvl - 1; lvl++) {
+    const size_t np = P.flux_pairs_n[lvl - T.min_lvl];
+    if (np == 0) continue;
+    const int2* d_pairs = P.d_flux_pairs + P.flux_pairs_off[lvl - T.min_lvl];
+    const int nc = P.nc_lvl(lvl);
+    const long work = (long)np * (D == 2 ? nc : nc * nc);
+    const int grid = (int)std::min<long>((work + 255) / 256, 65535);
+    if (D == 2)
+      hipLaunchKernelGGL((k_flux_fix_adj<2>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
+                         (int)np, nc, lF);
+    else
+      hipLaunchKernelGGL((k_flux_fix_adj<3>), dim3(grid), dim3(256), 0, P.stream, P.dev(), d_pairs,
+                         (int)np, nc, lF);
+    chk(hipGetLastError(), "consistent_fluxes_adj");
+  }
+}
+
+void fluxes_koren_adj(Pool& P, const StVars& v, double dt, const AdjVars& av) {
+  P.sync_topology();
+  const int D = P.tree.dim;
+  Tbl tb = device_table(P);
+  double* tg = table_grad(P);
+  const double eps0 = 8.8541878176e-12, qe = 1.6022e-19;
+  const double fac = -0.8 * eps0 / (qe * dt);
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_leaves(lvl);
+    if (!n) continue;
+    const int nc = P.nc_lvl(lvl);
+    G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb, P.leaves(lvl), n,
+                         nc, g.tpb, v.elec, v.fld, v.f_fld, v.t_mob, v.t_dif, av.le, av.lE, av.lF,
+                         av.lfld, tg, fac);
+    };
+    if (D == 2) go(k_koren_adj<2>);
+    else go(k_koren_adj<3>);
+    chk(hipGetLastError(), "fluxes_koren_adj");
+  }
 }
 
 }  // namespace a5g

@@ -433,6 +433,40 @@ module m_a5g
        integer(c_int)     :: s
      end function c_box_summaries
 
+     function c_update_solution_adjoint(t, dt, adj) &
+          bind(C, name="a5g_update_solution_adjoint") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: dt
+       integer(c_int)        :: adj(6)
+       integer(c_int)        :: s
+     end function c_update_solution_adjoint
+
+     function c_consistent_fluxes_adjoint(t, lf) &
+          bind(C, name="a5g_consistent_fluxes_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: lf
+       integer(c_int)        :: s
+     end function c_consistent_fluxes_adjoint
+
+     function c_fluxes_koren_adjoint(t, dt, adj) &
+          bind(C, name="a5g_fluxes_koren_adjoint") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: dt
+       integer(c_int)        :: adj(6)
+       integer(c_int)        :: s
+     end function c_fluxes_koren_adjoint
+
+     function c_table_adjoint(t, clear, rows_cols) bind(C, name="a5g_table_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: clear
+       type(c_ptr), value    :: rows_cols
+       integer(c_int)        :: s
+     end function c_table_adjoint
+
      function c_box_summaries_noalloc(t, out) bind(C, name="a5g_box_summaries_noalloc") &
           result(s)
        import :: c_ptr, c_int, c_double
@@ -665,6 +699,7 @@ module m_a5g
   public :: st_space_charge, st_fld_from_pot, st_fluxes_koren, st_update_solution
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
   public :: st_box_summaries_noalloc
+  public :: st_transport_adjoint, st_table_adjoint
   public :: st_photoi_helmholtz_terms
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
@@ -1145,6 +1180,32 @@ contains
     allocate(summ(3, a5g_max_id(tree)))
     call check(c_box_summaries(tree%h, summ), "box_summaries")
   end subroutine st_box_summaries
+
+  !> Reverse mode of one substep of the time loop (fluxes_koren,
+  !> a$D_consistent_fluxes, update_solution; src/streamer_2d.f90:384-542) on
+  !> a frozen mesh and field: adj = cell variables le, li, lE, lpho and face
+  !> variables lF, lfld. le, li hold the adjoints of the new densities and
+  !> get those of the old ones; lF must start at zero. No counterpart in the
+  !> reference.
+  subroutine st_transport_adjoint(tree, dt, adj)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: dt
+    integer, intent(in)        :: adj(6)
+    integer(c_int)             :: a(6)
+    a = adj
+    call check(c_update_solution_adjoint(tree%h, dt, a), "update_solution adjoint")
+    call check(c_consistent_fluxes_adjoint(tree%h, a(5)), "consistent_fluxes adjoint")
+    call check(c_fluxes_koren_adjoint(tree%h, dt, a), "fluxes_koren adjoint")
+  end subroutine st_transport_adjoint
+
+  !> The adjoints of the transport-table rows accumulated so far,
+  !> rows_cols(n_rows, n_cols); clear them after reading when clear
+  subroutine st_table_adjoint(tree, rows_cols, clear)
+    type(a5g_t), intent(inout)        :: tree
+    real(dp), intent(inout), target   :: rows_cols(:, :)
+    logical, intent(in)               :: clear
+    call check(c_table_adjoint(tree%h, merge(1, 0, clear), c_loc(rows_cols)), "table adjoint")
+  end subroutine st_table_adjoint
 
   !> st_box_summaries without any device allocation (the library's fixed
   !> scratch, at most 131072 boxes; the tree must have been uploaded since

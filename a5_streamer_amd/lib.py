@@ -315,6 +315,25 @@ class Tree:
         _call("a5g_box_summaries", self.h, _d(out)[1])
         return out
 
+    # --- reverse mode of the transport substep (a5g_*_adjoint) ------------
+    def update_solution_adjoint(self, dt, adj):
+        a, pa = _i(np.asarray(adj))
+        _call("a5g_update_solution_adjoint", self.h, C.c_double(dt), pa)
+
+    def consistent_fluxes_adjoint(self, lF):
+        _call("a5g_consistent_fluxes_adjoint", self.h, C.c_int(lF))
+
+    def fluxes_koren_adjoint(self, dt, adj):
+        a, pa = _i(np.asarray(adj))
+        _call("a5g_fluxes_koren_adjoint", self.h, C.c_double(dt), pa)
+
+    def table_adjoint(self, clear=False):
+        """The accumulated table-row adjoints as (n_cols, n_rows)."""
+        n_rows, n_cols = self._tbl
+        out = np.zeros(n_rows * n_cols)
+        _call("a5g_table_adjoint", self.h, C.c_int(int(bool(clear))), _d(out)[1])
+        return out.reshape(n_cols, n_rows)
+
     def box_summaries_noalloc(self):
         """box_summaries through the library's fixed scratch: no device
         allocation (at most 131072 boxes; the tree must be uploaded)."""

@@ -202,6 +202,37 @@ int a5g_photoi_helmholtz_terms(a5g_tree *t, double coeff, int n_terms, const dou
 /* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
  * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
 int a5g_box_summaries(a5g_tree *t, double *out);
+/* ---- reverse mode of the transport substep (frozen mesh) ------------- */
+
+/* The adjoints of one substep of the drivers' time loop, fluxes_koren, then
+ * a$D_consistent_fluxes, then update_solution (src/streamer_2d.f90:384-542,
+ * src/streamer_3d.f90:396-615, src/streamer_cyl.f90:416-570,
+ * afivo/src/m_aX_core.f90:818-930), with the mesh and the field held fixed.
+ * No counterpart in the reference, which computes no derivatives. Run them
+ * in the reverse order on the state the forward calls read (the fluxes of
+ * the forward pass in the flux variable). adj[6] = cell variables le, li,
+ * lE, lpho and face variables lF, lfld, which hold the adjoints of the
+ * densities, |E|, the photoionization source, the electron flux and the
+ * face field. Each call takes the adjoints of its outputs and adds those of
+ * its inputs, taking the branches (upwind side, limiter case, flux clamp,
+ * table row, dielectric rule) the forward pass takes on the same data; the
+ * adjoints of the table rows accumulate in the handle
+ * (a5g_table_adjoint).
+ *   update_solution: le, li hold the adjoints of n_e', n_i' on the leaves
+ *     and get those of n_e, n_i; adds to lF (start from zero), lE, lpho
+ *     (with photoionization) and the alpha and eta rows; */
+int a5g_update_solution_adjoint(a5g_tree *t, double dt, const int *adj);
+/*   consistent fluxes: the adjoints of the coarse faces the forward call
+ *     overwrote go to the fine faces it averaged, and are cleared; */
+int a5g_consistent_fluxes_adjoint(a5g_tree *t, int lF);
+/*   fluxes_koren: consumes lF on the leaves (clearing it) into le (cells,
+ *     first-layer ghosts and the cells the second ghost layer is taken
+ *     from), lE, lfld and the mobility and diffusion rows. */
+int a5g_fluxes_koren_adjoint(a5g_tree *t, double dt, const int *adj);
+/* The table-row adjoints accumulated so far, rows_cols(n_rows, n_cols)
+ * column-major (NULL: not read); clear = 1 then sets them to zero. */
+int a5g_table_adjoint(a5g_tree *t, int clear, double *rows_cols);
+
 /* a5g_box_summaries without any device allocation, for callers that cannot
  * allocate (code under stream capture, serving loops, harnesses that forbid
  * hipMalloc); no counterpart in the reference. The per-box maxima go through
