@@ -31,6 +31,9 @@ module m_a5g
   integer, parameter, public :: a5_bc_copy  = 0 ! ghost = interior (Neumann)
   integer, parameter, public :: a5_bc_neg   = 1 ! ghost = -interior (grounded)
   integer, parameter, public :: a5_bc_value = 2 ! ghost = 2 V - interior
+  ! ghost = 2 V - interior over the whole ghost row of a 2D y face, corners
+  ! included (the cylindrical driver's top face, src/streamer_cyl.f90:408-410)
+  integer, parameter, public :: a5_bc_row_value = 3
   ! refinement flags (afivo/src/m_afivo_types.f90)
   integer, parameter, public :: a5_rm_ref = -1, a5_kp_ref = 0, a5_do_ref = 1
   ! coordinate systems
