@@ -263,6 +263,17 @@ class Config:
         self.table_max_fld = 3e7
         self.n_boxes = 1000
         self.transport = None          # (x, [mob, dif, alpha, eta]) or None
+        # Driver variant for large runs (not in the reference's drivers):
+        # boxes whose centre lies in [lo, hi] are refined while their level
+        # is below lvl, for each (lvl, lo, hi) of refine_regions, on top of
+        # set_ref_flags' criteria
+        self.refine_regions = []
+        # Initial densities evaluated on the device (a5g_set_gaussians: the
+        # Gaussian seeds as exp(-|r - r0|^2 / w^2) with a plain sum of
+        # squares, equal to GM_gaussian of set_init_cond within rounding)
+        # instead of on the host in the reference's arithmetic; point seeds
+        # with falloff 2 and no dielectric only
+        self.init_on_device = False
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise KeyError("unknown config key %s" % k)
@@ -421,12 +432,29 @@ class StreamerRun:
             elec[q] = val
         return elec, eps
 
+    def _init_on_device(self):
+        c, T = self.c, self.T
+        if c.epsilon_diel != 1.0 or any(f != 2 for f in c.seed_falloff) or \
+                any(list(a) != list(b) for a, b in zip(c.seed_r0, c.seed_r1)):
+            raise ValueError("init_on_device: point seeds with falloff 2 and no dielectric only")
+        blobs = [list(c.seed_r0[k]) + [c.seed_width[k], c.seed_dens[k]]
+                 for k in range(len(c.seed_dens))]
+        T.set_gaussians(I_ELEC, np.array(blobs, float), c.bg_dens)
+        T.tree_copy_cc(I_ELEC, I_PION)
+        b = T.boxes()
+        ids = np.nonzero(b["in_use"])[0] + 1
+        T.fill_cc(ids, I_PHI, 0.0)
+        T.fill_cc(ids, I_EPS, 1.0)
+
     def set_init_cond(self):
         """set_init_cond (src/streamer_2d.f90:218-248) on every box in use.
         A tree with whole-variable uploads (set_cc_all) gets one upload per
         variable; box slots are box_len(n_cell) long, a coarser box filling
         the start of its slot."""
         T = self.T
+        if self.c.init_on_device:
+            self._init_on_device()
+            return
         b = T.boxes()
         n = len(b["lvl"])
         whole = hasattr(T, "set_cc_all")
@@ -496,7 +524,7 @@ class StreamerRun:
             f = 1
         return f
 
-    def flags_rows(self, time, s, dr, nc, rmin, ok):
+    def flags_rows(self, time, s, dr, nc, rmin, ok, lvl=None):
         """box_flag of many boxes of this run at once, with array forms of
         the same operations (each flag the same): s the per-box maxima, dr,
         nc, rmin (n, dim) the boxes, ok the boxes to flag (others keep)."""
@@ -516,6 +544,10 @@ class StreamerRun:
                 hit = (dist - c.seed_width[k] < boxlen) & (dr[idx] > 0.2 * c.seed_width[k])
                 f[idx[hit]] = 1
         f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
+        for lvl_r, lo, hi in c.refine_regions:
+            centre = np.stack([rmin[:, k] + 0.5 * nc * dr for k in range(self.dim)], axis=1)
+            inside = np.all((centre >= np.asarray(lo)) & (centre <= np.asarray(hi)), axis=1)
+            f[inside & (np.asarray(lvl) < lvl_r)] = 1
         f[~ok] = 0
         return f
 
@@ -524,7 +556,7 @@ class StreamerRun:
         b = T.boxes()
         s = T.box_summaries()
         ok = (b["in_use"] != 0) & (b["lvl"] >= 1)
-        return self.flags_rows(self.time, s, b["dr"], b["n_cell"], b["r_min"], ok)
+        return self.flags_rows(self.time, s, b["dr"], b["n_cell"], b["r_min"], ok, b["lvl"])
 
     def adjust(self):
         return self.T.adjust(self.ref_flags())

@@ -240,27 +240,46 @@ STEP_KINDS = {0: "gsrb_half_sweep", 1: "ghost_cells", 2: "residual_restrict",
               12: "average", 13: "restrict", 14: "photoi_rate"}
 
 
-def streamer_step(args):
+# the production 3D streamer: a 64 mm cube of one base box (the reference's
+# init_tree), uniform level 6 (0.25 mm cells) and level 7 in the top quarter
+# where the streamer starts, the seed refining further
+PRODUCTION_L = 64e-3
+PRODUCTION_REGIONS = [(6, [0.0, 0.0, 0.0], [PRODUCTION_L] * 3),
+                      (7, [0.0, 0.0, 0.75 * PRODUCTION_L], [PRODUCTION_L] * 3)]
+
+
+def streamer_step(args, steps=None, warmup=None):
     """The full 3D streamer step on one GPU: the 3D driver (a 64 mm cube of
-    8^3-cell boxes on a base of step_base^3 boxes, 2 MV/m, the air seed and transport of
-    the 2D benchmark moved to (32, 32, 56) mm, Helmholtz photoionization)
-    after its initial refinement, stepped through the library. Prints one
-    JSON line: leaf Mcell-steps/s and, per kernel class, the time share and
-    the rate of algorithmic bytes over the event-timed launch durations."""
+    8^3-cell boxes on a base of step_base^3 boxes, 2 MV/m, the air seed and
+    transport of the 2D benchmark moved to (32, 32, 56) mm, Helmholtz
+    photoionization) after its initial refinement, stepped through the
+    library. With one base box the mesh is refined at least to level 6
+    everywhere and to level 7 in the top quarter (the production size, tens
+    of millions of leaf cells) and the initial densities are evaluated on the
+    device. Returns one JSON-able dict: leaf Mcell-steps/s, the leaf cells at
+    the start and end, the whole step's algorithmic bytes over its time as a
+    fraction of the HBM peak, and per kernel class the time share and rate."""
     import a5_streamer_amd as A
     from a5_streamer_amd import streamer as ST
-    cfg = ST.Config(variant="3d", domain_len=64e-3, n_base=args.step_base,
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    production = args.step_base == 1
+    cfg = ST.Config(variant="3d", domain_len=PRODUCTION_L, n_base=args.step_base,
                     lvls_max=args.step_lvls,
                     applied_fld=2.0e6, seed_r0=[[32e-3, 32e-3, 56e-3]],
                     seed_r1=[[32e-3, 32e-3, 56e-3]], photoi=not args.no_photoi,
-                    photoi_warm=not args.photoi_cold, n_boxes=20000)
+                    photoi_warm=not args.photoi_cold, n_boxes=20000,
+                    refine_regions=PRODUCTION_REGIONS if production else [],
+                    init_on_device=production)
     run = ST.StreamerRun(A.Tree, cfg)
     t0 = time.perf_counter()
     run.initialise()
     t_init = time.perf_counter() - t0
     cells = run.leaf_cells()
-    blocks = max(1, args.steps // cfg.steps_amr)
-    for _ in range(args.warmup):
+    print("streamer step: initial refinement done in %.1f s, %d leaf cells, %d boxes" % (
+        t_init, cells, run.T.info()["max_id"]), file=sys.stderr, flush=True)
+    blocks = max(1, steps // cfg.steps_amr)
+    for _ in range(warmup):
         run.step_block()
     T = run.T
     T.sync()
@@ -282,26 +301,39 @@ def streamer_step(args):
     t_prof = time.perf_counter() - t1
     kinds = {}
     tot_ms = 0.0
+    tot_bytes = 0.0
     for k, name in STEP_KINDS.items():
         n, ms, by = T.prof_read(k, -999)
         if n:
             kinds[name] = {"launches": n, "ms_per_step": round(ms / cfg.steps_amr, 3),
                            "alg_GBps": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
             tot_ms += ms
+            tot_bytes += by
     T.prof_enable(False)
     for v in kinds.values():
         v["share"] = round(v["ms_per_step"] * cfg.steps_amr / tot_ms, 4)
-    line = {
+    ms_step = dt * 1e3 / n_steps
+    # algorithmic bytes of one step (the profiled block's, per step) over the
+    # time per step of the timed blocks
+    step_bytes = tot_bytes / cfg.steps_amr
+    step_ach = step_bytes / (ms_step * 1e-3) / 1e9
+    out = {
         "metric": "3D streamer step, leaf Mcell-steps/s",
         "value": round(0.5 * (c0 + c1) * n_steps / dt / 1e6, 3),
-        "unit": "Mcell-steps/s", "steps": n_steps, "warmup_blocks": args.warmup,
-        "ms_per_step": round(dt * 1e3 / n_steps, 3), "dtype": "f64",
-        "config": {"workload": "3D streamer: 64 mm cube, 8^3-cell boxes, %d^3 base boxes, "
-                               "2 MV/m, seed at (32, 32, 56) mm, lvls_max %d, %s" % (
-                                   args.step_base, args.step_lvls, "Helmholtz photoionization"
+        "unit": "Mcell-steps/s", "steps": n_steps, "warmup_blocks": warmup,
+        "ms_per_step": round(ms_step, 3), "dtype": "f64",
+        "config": {"workload": "3D streamer: 64 mm cube, 8^3-cell boxes, %d^3 base box(es), "
+                               "2 MV/m, seed at (32, 32, 56) mm, lvls_max %d, %s%s" % (
+                                   args.step_base, args.step_lvls,
+                                   "uniform level 6, level 7 for z > 48 mm, "
+                                   if production else "",
+                                   "Helmholtz photoionization"
                                    if cfg.photoi else "no photoionization"),
                    "leaf_cells_start": c0, "leaf_cells_end": c1, "leaf_cells_init": cells,
-                   "init_s": round(t_init, 1)},
+                   "boxes": T.info()["max_id"], "init_s": round(t_init, 1)},
+        "roofline": {"bound": "hbm", "step_bytes": round(step_bytes),
+                     "step_achieved": round(step_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "step_frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "photoi": {"start": "cold (FMG)" if args.photoi_cold else "warm (last step's)",
                    "v_cycles_per_step": round((cyc1 - cyc0) / n_steps, 2)}
         if cfg.photoi else None,
@@ -309,8 +341,8 @@ def streamer_step(args):
         "profiled_block_s": round(t_prof, 3),
         "kernel_ms_per_step": round(tot_ms / cfg.steps_amr, 3),
     }
-    print(json.dumps(line), flush=True)
     T.close()
+    return out
 
 
 def splitmix64(seed):
@@ -409,8 +441,12 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--streamer-step", action="store_true",
                     help="measure the full 3D streamer step instead (own JSON line)")
-    ap.add_argument("--step-lvls", type=int, default=7)
-    ap.add_argument("--step-base", type=int, default=16)
+    ap.add_argument("--step-lvls", type=int, default=9)
+    ap.add_argument("--step-base", type=int, default=1,
+                    help="base boxes per direction of the streamer step (1: the reference's "
+                         "single base box, refined to the production size)")
+    ap.add_argument("--no-streamer3d", action="store_true",
+                    help="leave the production 3D streamer step out of the headline line")
     ap.add_argument("--no-photoi", action="store_true")
     ap.add_argument("--photoi-cold", action="store_true",
                     help="solve every photoionization term by FMG from zero each step")
@@ -423,7 +459,7 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.streamer_step:
-        streamer_step(args)
+        print(json.dumps(streamer_step(args)), flush=True)
         return
     if args.batch:
         batch_runs(args)
@@ -514,6 +550,10 @@ def main():
         # the FP64 GPU path is bit-identical to the reference, so the
         # reference needs the same number of V-cycles
         out["cpu_baseline"] = cpu_baseline(W, n_v)
+    if not args.no_streamer3d and not args.small:
+        # the production 3D streamer step, a few timed steps after its
+        # initial refinement
+        out["streamer3d"] = streamer_step(args, steps=6, warmup=1)
     # keys first that the driver reads
     order = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
              "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
