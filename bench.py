@@ -57,51 +57,100 @@ def solve(T, max_lvl, n_v):
         T.fas_vcycle(max_lvl, False)
 
 
-def cpu_baseline(W_small, n_v_small, budget_s=20.0):
-    """The reference (oracle/_ref, built from the reference sources) timed on
-    this host's cores on the same octree (1-3 solves, bounded by budget_s);
-    the rhs comes from the GPU library on the bit-identical tree."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def host_cpu():
+    """The host's processor model and the cores this process may run on."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except Exception:
+        usable = os.cpu_count() or 1
+    return model, usable
+
+
+def _ref_octree(W, n_cc=3):
+    """The octree W on the reference harness, with the rhs the GPU library
+    evaluates on the bit-identical tree."""
     import refharness as R
-    if not R.available():
-        return None
     import a5_streamer_amd as A
-    Tg = A.Tree(3, W_small.n_cell, 3, 0, W_small.dr, coarsen_to=2, n_boxes=1000)
-    W_small.build(Tg)
-    Tg.set_gaussians(3, W_small.blobs(), 0.0)
+    Tg = A.Tree(3, W.n_cell, n_cc, 0, W.dr, coarsen_to=2, n_boxes=1000)
+    W.build(Tg)
+    Tg.set_gaussians(3, W.blobs(), 0.0)
     rhs = Tg.get_cc_all(3)
-    T = R.RefTree(3, W_small.n_cell, 3, 0, W_small.dr, coarsen_to=2, n_boxes=1000)
-    W_small.build(T)
+    Tg.close()
+    T = R.RefTree(3, W.n_cell, 3, 0, W.dr, coarsen_to=2, n_boxes=1000)
+    W.build(T)
     b = T.boxes()
     for i in range(len(b["lvl"])):
         if b["in_use"][i]:
             nc = int(b["n_cell"][i])
             T.set_cc(i + 1, 3, rhs[i][: (nc + 2) ** 3])
-    kinds, vals = W_small.bc()
-    T.mg_setup(1, 2, 3, n_base=W_small.n_cycle_base, bc_kind=kinds, bc_val=vals,
-               auto_ops=True)
+    kinds, vals = W.bc()
+    T.mg_setup(1, 2, 3, n_base=W.n_cycle_base, bc_kind=kinds, bc_val=vals, auto_ops=True)
+    return T
+
+
+def _ref_solves(T, n_v, reps):
+    """Wall seconds of each of reps reference solves (FMG + n_v V-cycles)."""
     max_lvl = T.info()["max_lvl"]
-    cells = W_small.leaf_cells(T)
-    n_solves = 0
-    t0 = time.perf_counter()
-    while True:
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
         T.fas_fmg(False, True)
-        for _ in range(n_v_small):
+        for _ in range(n_v):
             T.fas_vcycle(max_lvl, False)
-        n_solves += 1
-        if time.perf_counter() - t0 > budget_s / 3 or n_solves >= 3:
-            break
-    dt = time.perf_counter() - t0
-    Tg.close()
-    return {
-        "value": cells * n_solves / dt / 1e6,
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def cpu_baseline(W, n_v, small_W=None, n_v_small=None):
+    """The reference (oracle/_ref, built from the reference sources) timed on
+    this host: the median of 3 solves of the benchmark octree with OpenMP on
+    every usable core, and the median of 3 solves of a smaller octree of the
+    same construction on one thread (a bounded sample: one full-size solve
+    takes tens of seconds on one core). The rhs comes from the GPU library on
+    the bit-identical tree."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refharness as R
+    if not R.available():
+        return None
+    model, usable = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", usable))
+    T = _ref_octree(W)
+    cells = W.leaf_cells(T)
+    R.set_threads(threads)
+    times = _ref_solves(T, n_v, 3)
+    med = float(np.median(times))
+    out = {
+        "value": cells / med / 1e6,
         "unit": "Mcell-steps/s",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        "cores": threads,
         "kind": "reference",
         "sample": "reference Fortran (flang -O2 -fopenmp) mg3_fas_fmg + %d mg3_fas_vcycle to "
-                  "1e-10 on the benchmark octree (%d leaf cells), %d solve(s) in %.1f s" % (
-                      n_v_small, cells, n_solves, dt),
+                  "1e-10 on the benchmark octree (%d leaf cells), median of 3 solves "
+                  "(%s s) on %d threads" % (n_v, cells, ", ".join("%.2f" % t for t in times),
+                                            threads),
+        "host": {"cpu": model, "usable_cores": usable},
     }
+    if small_W is not None:
+        Ts = _ref_octree(small_W)
+        cs = small_W.leaf_cells(Ts)
+        R.set_threads(1)
+        ts = _ref_solves(Ts, n_v_small, 3)
+        R.set_threads(threads)
+        out["one_thread"] = {
+            "value": cs / float(np.median(ts)) / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+            "sample": "the same solve on the octree with uniform level 3 and a 5x5x8-box "
+                      "level-4 sub-volume (%d leaf cells), %d V-cycles, median of 3 "
+                      "(%s s) on one thread" % (cs, n_v_small,
+                                                ", ".join("%.2f" % t for t in ts))}
+    return out
 
 
 def converge_count(T, max_lvl, tol=1e-10, cap=60):
@@ -549,7 +598,14 @@ def main():
     if not args.no_cpu_baseline:
         # the FP64 GPU path is bit-identical to the reference, so the
         # reference needs the same number of V-cycles
-        out["cpu_baseline"] = cpu_baseline(W, n_v)
+        if args.small:
+            out["cpu_baseline"] = cpu_baseline(W, n_v)
+        else:
+            Ws = OctreePoisson(uniform_lvl=3, fine_lvl=4, refine_frac=(5, 5, 8))
+            Tq = build_gpu(Ws)
+            n_vs, _ = converge_count(Tq, Tq.info()["max_lvl"], tol)
+            Tq.close()
+            out["cpu_baseline"] = cpu_baseline(W, n_v, Ws, n_vs)
     if not args.no_streamer3d and not args.small:
         # the production 3D streamer step, a few timed steps after its
         # initial refinement
