@@ -522,6 +522,12 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       a5g::hcheck(hipMalloc((void**)&h->P.d_red, 2 * a5g::kRedSlots * sizeof(unsigned long long)),
                   "hipMalloc");
       a5g::noalloc_init();
+      int dev = 0, n_cu = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) ==
+              hipSuccess &&
+          n_cu > 0)
+        h->P.n_cu = n_cu;
       h->P.ensure_capacity(h->P.tree.cap);
     }
   } catch (...) {
@@ -843,6 +849,46 @@ int a5g_fas_vcycle(a5g_tree* t, int max_lvl, int set_residual) {
   API_BEGIN
   NEED(t);
   a5g::mg_vcycle(t->P, max_lvl, set_residual != 0);
+  API_END
+}
+
+int a5g_set_shape(a5g_tree* t, int kernel, int nc, int cls, int shape) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  auto& P = t->P;
+  if (kernel < 0 || kernel >= a5g::kShapeKernels) throw std::out_of_range("unknown kernel");
+  if (nc < 2 || nc > 32 || (nc & (nc - 1))) throw std::invalid_argument("nc must be 2..32, a power of 2");
+  if (cls < 0 || cls >= a5g::kSizeClasses) throw std::out_of_range("size class out of range");
+  if (shape < 0 || shape >= a5g::kShapes) throw std::out_of_range("shape out of range");
+  P.clear_graphs();
+  P.shape_tab[kernel][P.tree.dim - 2][a5g::log2i(nc)][cls] = (unsigned char)shape;
+  API_END
+}
+
+int a5g_get_shape(a5g_tree* t, int kernel, int nc, int cls, int* shape) {
+  API_BEGIN
+  NEED(t);
+  if (!shape) throw std::invalid_argument("null output");
+  if (kernel < 0 || kernel >= a5g::kShapeKernels) throw std::out_of_range("unknown kernel");
+  if (nc < 2 || nc > 32 || (nc & (nc - 1))) throw std::invalid_argument("nc must be 2..32, a power of 2");
+  if (cls < 0 || cls >= a5g::kSizeClasses) throw std::out_of_range("size class out of range");
+  *shape = t->P.shape_tab[kernel][t->P.tree.dim - 2][a5g::log2i(nc)][cls];
+  API_END
+}
+
+int a5g_autotune(a5g_tree* t, int reps, int max_rec, double* rec, int* n_rec) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!n_rec || (max_rec > 0 && !rec)) throw std::invalid_argument("null output");
+  std::vector<double> r;
+  a5g::autotune(t->P, reps, r);
+  const int n = (int)(r.size() / 8);
+  *n_rec = n;
+  for (int i = 0; i < std::min(n, max_rec); i++)
+    for (int k = 0; k < 8; k++) rec[i * 8 + k] = r[i * 8 + k];
   API_END
 }
 
@@ -1475,6 +1521,19 @@ int a5g_set_gaussians(a5g_tree* t, int iv, int n_blob, const double* blobs, doub
   check_iv(t->P.tree, iv);
   if (n_blob < 0 || (n_blob > 0 && !blobs)) throw std::invalid_argument("bad blob list");
   a5g::set_gaussians(t->P, iv, n_blob, blobs, base);
+  API_END
+}
+
+int a5g_set_gaussians_instances(a5g_tree* t, int iv, int n_blob, const double* blobs,
+                                const double* base) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  if (t->P.n_inst == 0) throw std::invalid_argument("no instances set");
+  if (n_blob < 0 || (n_blob > 0 && !blobs) || !base) throw std::invalid_argument("bad blob list");
+  a5g::set_gaussians(t->P, iv, n_blob, blobs, 0.0, base);
   API_END
 }
 

@@ -12,16 +12,24 @@
 namespace a5g {
 
 // v(x) = base + sum_k amp_k * exp(-|x - c_k|^2 / w_k^2) at the cell centres
-// r_min + (i - 0.5) dr (a$D_r_cc, m_aX_utils.f90:309-314)
+// r_min + (i - 0.5) dr (a$D_r_cc, m_aX_utils.f90:309-314). With bases set,
+// a box of instance g takes base bases[g] and blobs g * nblob .. (g + 1) *
+// nblob - 1.
 template <int D>
 __global__ __launch_bounds__(256) void k_gaussians(Dev a, const int* __restrict__ list, int n,
                                                    int nc, int iv, int nblob,
                                                    const double* __restrict__ blobs,
-                                                   double base) {
+                                                   double base,
+                                                   const double* __restrict__ bases) {
   const int li = blockIdx.x;
   if (li >= n) return;
   const int id = list[li];
   double* p = boxp(a, iv, id);
+  if (bases != nullptr) {
+    const int g = inst_of<D>(a, id);
+    base = bases[g];
+    blobs += (size_t)g * nblob * (D + 2);
+  }
   const int m = nc + 2;
   const int bl = D == 2 ? m * m : m * m * m;
   const double dr = a.dr[id];
@@ -46,26 +54,34 @@ __global__ __launch_bounds__(256) void k_gaussians(Dev a, const int* __restrict_
   }
 }
 
-void set_gaussians(Pool& P, int iv, int nblob, const double* blobs_host, double base) {
+void set_gaussians(Pool& P, int iv, int nblob, const double* blobs_host, double base,
+                   const double* bases_host) {
   P.sync_topology();
   const int D = P.tree.dim;
+  // per instance: n_inst blob sets and bases, stored after each other
+  const int sets = bases_host ? P.n_inst : 1;
+  const size_t n_blob_all = (size_t)nblob * sets;
   double* d_blobs = nullptr;
-  const size_t bytes = (size_t)std::max(nblob, 1) * (D + 2) * sizeof(double);
+  const size_t bytes = (std::max(n_blob_all, (size_t)1) * (D + 2) + sets) * sizeof(double);
   if (hipMalloc((void**)&d_blobs, bytes) != hipSuccess) throw std::runtime_error("hipMalloc");
-  if (nblob > 0 &&
-      hipMemcpyAsync(d_blobs, blobs_host, (size_t)nblob * (D + 2) * sizeof(double),
+  double* d_bases = bases_host ? d_blobs + std::max(n_blob_all, (size_t)1) * (D + 2) : nullptr;
+  if (n_blob_all > 0 &&
+      hipMemcpyAsync(d_blobs, blobs_host, n_blob_all * (D + 2) * sizeof(double),
                      hipMemcpyHostToDevice, P.stream) != hipSuccess)
     throw std::runtime_error("copy blobs");
+  if (d_bases && hipMemcpyAsync(d_bases, bases_host, sets * sizeof(double),
+                                hipMemcpyHostToDevice, P.stream) != hipSuccess)
+    throw std::runtime_error("copy bases");
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
     const int n = P.n_ids(lvl);
     if (n == 0) continue;
     const int nc = P.nc_lvl(lvl);
     if (D == 2)
       hipLaunchKernelGGL((k_gaussians<2>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
-                         n, nc, iv, nblob, d_blobs, base);
+                         n, nc, iv, nblob, d_blobs, base, d_bases);
     else
       hipLaunchKernelGGL((k_gaussians<3>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
-                         n, nc, iv, nblob, d_blobs, base);
+                         n, nc, iv, nblob, d_blobs, base, d_bases);
     if (hipGetLastError() != hipSuccess) throw std::runtime_error("gaussians launch");
   }
   if (hipStreamSynchronize(P.stream) != hipSuccess) throw std::runtime_error("gaussians sync");

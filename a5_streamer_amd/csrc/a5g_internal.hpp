@@ -16,6 +16,7 @@ void minmax_boxes(Pool& P, int iv, bool absval, bool whole, bool leaves_only,
 void mg_fmg(Pool& P, bool set_residual, bool first_call);
 void mg_fmg_groups(Pool& P, bool set_residual, bool first_call);
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
+void autotune(Pool& P, int reps, std::vector<double>& rec);
 // stores the ghost cells a multigrid call left to the next reader
 void flush_ghosts(Pool& P);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);
@@ -23,7 +24,8 @@ void maxdiff_leaves(Pool& P, int ia, int ib, double* maxdiff, double* maxabs_b);
 void mg_solve_mixed(Pool& P, double tol, int max_outer, int n_inner, int* outer_done,
                     double* rel_res);
 
-void set_gaussians(Pool& P, int iv, int nblob, const double* blobs, double base);
+void set_gaussians(Pool& P, int iv, int nblob, const double* blobs, double base,
+                   const double* bases = nullptr);
 void gc_list(Pool& P, int lvl, const int* d_ids, int n, int iv, int rb, int bc);
 
 void set_table(Pool& P, int n_rows, int n_cols, double x_min, double inv_dx, const double* rc);

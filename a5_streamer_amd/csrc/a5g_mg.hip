@@ -248,6 +248,14 @@ __device__ __forceinline__ int xcd_block() {
   return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
 }
 
+// xcd_block of workgroup slot v of a one-shot grid of nblk workgroups, for
+// a persistent grid whose size is a multiple of 8 (slot v then runs on the
+// XCD that workgroup v of the one-shot grid would run on)
+__device__ __forceinline__ int xcd_slot(int v, int nblk) {
+  const int per = nblk >> 3;
+  return v < (per << 3) ? (v & 7) * per + (v >> 3) : v;
+}
+
 // Box-size-specialised half-sweep. The other colour's interior cells and
 // face ghosts are staged in LDS, in an order used only there: slot
 // lex(i, j, k) / 2 of the whole (NC+2)^D block, one colour, in which the
@@ -256,15 +264,31 @@ __device__ __forceinline__ int xcd_block() {
 // current state (the values the reference's ghost pass after the previous
 // half-sweep would store). Then every lane updates cells of colour cntr&1
 // from LDS and rhs (loaded into registers before the barrier).
-template <int D, bool CYL, int NC, typename T, bool EPS, bool NT>
+//
+// Launch shapes (the autotuner's menu, kGsrbShapes): HALVE gives each box
+// half the lanes of the default (each lane then updates twice the cells) and
+// twice the boxes per 256-lane workgroup; PERSIST launches at most
+// persist_grid workgroups, each walking the list's workgroup slots with the
+// grid's stride (slot v of the one-shot grid, so the XCD mapping and every
+// value are those of the one-shot form).
+template <int D, int NC, bool HALVE>
+struct GsrbShape {
+  static constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
+  static constexpr int TPB0 = NCOL > 128 ? 256 : (NCOL > 64 ? 128 : 64);
+  static constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
+  static constexpr int BPB = 256 / TPB;
+};
+
+template <int D, bool CYL, int NC, typename T, bool EPS, bool NT, bool HALVE = false,
+          bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__ list, int n,
                                                   int cntr, int fused, int i_phi, int i_rhs) {
   constexpr int N2 = NC + 2;
   constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
   constexpr int HALF = BL / 2;
-  constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
-  constexpr int TPB = NCOL > 128 ? 256 : (NCOL > 64 ? 128 : 64);
-  constexpr int BPB = 256 / TPB;
+  constexpr int NCOL = GsrbShape<D, NC, HALVE>::NCOL;
+  constexpr int TPB = GsrbShape<D, NC, HALVE>::TPB;
+  constexpr int BPB = GsrbShape<D, NC, HALVE>::BPB;
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int HNC = NC / 2;
   constexpr int NIT = (NCOL + TPB - 1) / TPB;
@@ -273,18 +297,20 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
   constexpr int GPF = PER_FACE / 2;
   __shared__ T sh[BPB * HALF];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
-  const int li = xcd_block() * BPB + slot;
-  const bool act = li < n;
-  const int id = act ? list[li] : 0;
+  const int nblk = (n + BPB - 1) / BPB;
   const int c = cntr & 1, oc = c ^ 1;
   T* s = sh + slot * HALF;
-  T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   // the interior cell of colour col at slot q of its half; returns its
   // lexicographic index in the block
   auto cell_of = [&](int q, int col, int& i, int& j, int& k) {
     icell<D>(NC, q, col, i, j, k);
     return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k);
   };
+  for (int vb = PERSIST ? (int)blockIdx.x : 0;;) {
+  const int li = (PERSIST ? xcd_slot(vb, nblk) : xcd_block()) * BPB + slot;
+  const bool act = li < n;
+  const int id = act ? list[li] : 0;
+  T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
   T rv[NIT];
   if (act) {
     // Every global load (interior cells of colour oc, rhs of colour c, the
@@ -357,7 +383,7 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
     }
   }
   __syncthreads();
-  if (!act) return;
+  if (act) {
   T* pc = p + c * HALF;
   const T dx2 = (T)(a.dr[id] * a.dr[id]);
   const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
@@ -413,18 +439,35 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
     if (NT) __builtin_nontemporal_store(v, pc + q);
     else pc[q] = v;
   }
+  }
+  if (!PERSIST) break;
+  vb += gridDim.x;
+  if (vb >= nblk) break;
+  __syncthreads();  // the next slot's loads overwrite the LDS
+  }
+}
+
+// One half-sweep in launch shape `shape` (kShapes: bit 0 persistent grid,
+// bit 1 half the lanes per box)
+template <int D, bool CYL, int NC, typename T, bool EPS, bool NT>
+static void go_gsrb(Pool& P, const Dev& a, const int* list, int n, int cntr, int fused,
+                    int i_phi, int i_rhs, int shape) {
+  const bool persist = shape & 1, halve = shape & 2;
+  const int bpb = halve ? GsrbShape<D, NC, true>::BPB : GsrbShape<D, NC, false>::BPB;
+  const int nblk = (n + bpb - 1) / bpb;
+  const int grid = persist ? P.persist_grid(nblk) : nblk;
+  auto kern = halve ? (persist ? k_gsrb_lds<D, CYL, NC, T, EPS, NT, true, true>
+                               : k_gsrb_lds<D, CYL, NC, T, EPS, NT, true, false>)
+                    : (persist ? k_gsrb_lds<D, CYL, NC, T, EPS, NT, false, true>
+                               : k_gsrb_lds<D, CYL, NC, T, EPS, NT, false, false>);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, P.stream, a, list, n, cntr, fused, i_phi,
+                     i_rhs);
 }
 
 template <int D, bool CYL, typename T, bool EPS>
 static bool launch_gsrb_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc, int cntr,
                               int fused, int i_phi, int i_rhs) {
-  auto go = [&](auto kern, int ncol) {
-    const int tpb = ncol > 128 ? 256 : (ncol > 64 ? 128 : 64);
-    const int bpb = 256 / tpb;
-    hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
-                       cntr, fused, i_phi, i_rhs);
-  };
-  const int ncol = (D == 2 ? nc * nc : nc * nc * nc) / 2;
+  const int sh = P.shape(0, D, nc, n);
   // A level whose potential and rhs far exceed the 256 MB Infinity Cache
   // streams: the rhs is read and the new colour written with non-temporal
   // accesses, which keeps L2 for the neighbour faces the next boxes read.
@@ -433,17 +476,17 @@ static bool launch_gsrb_lds_t(Pool& P, const Dev& a, const int* list, int n, int
   const double level_bytes = 2.0 * n * box_cells * sizeof(T);
   const bool nt = level_bytes > P.stream_bytes;
   switch (nc) {
-    case 2: go(k_gsrb_lds<D, CYL, 2, T, EPS, false>, ncol); return true;
-    case 4: go(k_gsrb_lds<D, CYL, 4, T, EPS, false>, ncol); return true;
-    case 8: go(k_gsrb_lds<D, CYL, 8, T, EPS, false>, ncol); return true;
+    case 2: go_gsrb<D, CYL, 2, T, EPS, false>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh); return true;
+    case 4: go_gsrb<D, CYL, 4, T, EPS, false>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh); return true;
+    case 8: go_gsrb<D, CYL, 8, T, EPS, false>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh); return true;
     case 16:
-      if (nt) go(k_gsrb_lds<D, CYL, 16, T, EPS, true>, ncol);
-      else go(k_gsrb_lds<D, CYL, 16, T, EPS, false>, ncol);
+      if (nt) go_gsrb<D, CYL, 16, T, EPS, true>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh);
+      else go_gsrb<D, CYL, 16, T, EPS, false>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh);
       return true;
     case 32:
       if (D == 2) {
-        if (nt) go(k_gsrb_lds<2, CYL, 32, T, EPS, true>, ncol);
-        else go(k_gsrb_lds<2, CYL, 32, T, EPS, false>, ncol);
+        if (nt) go_gsrb<2, CYL, 32, T, EPS, true>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh);
+        else go_gsrb<2, CYL, 32, T, EPS, false>(P, a, list, n, cntr, fused, i_phi, i_rhs, sh);
         return true;
       }
       return false;
@@ -2741,6 +2784,103 @@ void restrict_tree(Pool& P, int iv) {
 void tree_copy_cc(Pool& P, int from, int to) {
   P.sync_topology();
   for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) copy_level(P, lvl, from, to);
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner: for every (box size, level-size class) that the
+// tree's levels hold, each shape of the half-sweep menu is timed on those
+// levels (reps half-sweeps each after two untimed ones, on the potential and
+// rhs of the multigrid, with HIP events on the handle's stream) and the
+// fastest is kept when it beats the default by more than 3%. The potential is
+// saved to a device copy first and restored afterwards; every shape computes
+// the same values, so only the time differs. Rows of rec: kernel, dim, nc,
+// size class, shape, boxes timed, microseconds per half-sweep, 1 if chosen.
+// ---------------------------------------------------------------------------
+
+template <int D>
+static void autotune_t(Pool& P, int reps, std::vector<double>& rec) {
+  const Ctx c = ctx64(P);
+  const Dev a = dev_for(P, c);
+  // levels of each (log2 nc, size class)
+  std::vector<std::vector<int>> groups(6 * kSizeClasses);
+  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+    const int n = P.n_ids(lvl);
+    if (n > 0) groups[log2i(P.nc_lvl(lvl)) * kSizeClasses + size_class(n)].push_back(lvl);
+  }
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "event");
+  check(hipEventCreate(&e1), "event");
+  for (int key = 0; key < (int)groups.size(); key++) {
+    if (groups[key].empty()) continue;
+    const int l2 = key / kSizeClasses, cls = key % kSizeClasses;
+    double t[kShapes];
+    int boxes = 0;
+    for (int sh = 0; sh < kShapes; sh++) {
+      P.shape_tab[0][D - 2][l2][cls] = (unsigned char)sh;
+      t[sh] = 0;
+      boxes = 0;
+      for (int lvl : groups[key]) {
+        const int n = P.n_ids(lvl), nc = P.nc_lvl(lvl);
+        boxes += n;
+        auto sweep = [&](int it) {
+          const bool ok = c.cyl ? launch_gsrb_lds<2, true, double>(P, a, P.ids(lvl), n, nc, it, 1,
+                                                                   c.i_phi, c.i_rhs)
+                                : launch_gsrb_lds<D, false, double>(P, a, P.ids(lvl), n, nc, it,
+                                                                    1, c.i_phi, c.i_rhs);
+          if (!ok) throw std::runtime_error("autotune: no half-sweep kernel for this box size");
+          post_launch("autotune");
+        };
+        sweep(1);
+        sweep(2);
+        check(hipEventRecord(e0, P.stream), "event");
+        for (int r = 0; r < reps; r++) sweep(r + 1);
+        check(hipEventRecord(e1, P.stream), "event");
+        check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        t[sh] += 1e3 * ms / reps;
+      }
+    }
+    int best = 0;
+    for (int sh = 1; sh < kShapes; sh++)
+      if (t[sh] < t[best]) best = sh;
+    if (!(t[best] < 0.97 * t[0])) best = 0;
+    P.shape_tab[0][D - 2][l2][cls] = (unsigned char)best;
+    for (int sh = 0; sh < kShapes; sh++) {
+      const double row[8] = {0, (double)D, (double)(1 << l2), (double)cls, (double)sh,
+                             (double)boxes, t[sh], sh == best ? 1.0 : 0.0};
+      rec.insert(rec.end(), row, row + 8);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
+void autotune(Pool& P, int reps, std::vector<double>& rec) {
+  if (P.mg.i_phi <= 0 || P.mg.i_rhs <= 0) throw std::runtime_error("autotune: multigrid not set up");
+  if (reps < 1) throw std::invalid_argument("autotune: reps must be >= 1");
+  P.sync_topology();
+  // captured graphs hold the old shapes
+  P.clear_graphs();
+  const Dev a = P.dev();
+  double* phi = a.cc + (size_t)(P.mg.i_phi - 1) * a.vs;
+  double* save = nullptr;
+  check(hipMalloc((void**)&save, a.vs * sizeof(double)), "hipMalloc");
+  check(hipMemcpyAsync(save, phi, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream),
+        "save");
+  try {
+    if (P.tree.dim == 2) autotune_t<2>(P, reps, rec);
+    else autotune_t<3>(P, reps, rec);
+  } catch (...) {
+    (void)hipMemcpyAsync(phi, save, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream);
+    (void)hipStreamSynchronize(P.stream);
+    (void)hipFree(save);
+    throw;
+  }
+  check(hipMemcpyAsync(phi, save, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream),
+        "restore");
+  check(hipStreamSynchronize(P.stream), "restore sync");
+  check(hipFree(save), "hipFree");
 }
 
 }  // namespace a5g

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <string>
 #include <vector>
 
@@ -101,6 +103,19 @@ constexpr int kRedSlots = 256;
 // independent solves that may run side by side (photoionization terms)
 constexpr int kMaxTerms = 4;
 
+// Launch shapes of the box kernels (a5g_set_shape, a5g_autotune): per
+// kernel, dimension, box size (log2 of the cells a side) and level-size
+// class, an index into the kernel's menu (0: the default shape).
+constexpr int kShapeKernels = 1;  // 0: multigrid half-sweep
+constexpr int kSizeClasses = 3;   // levels of < 1024, < 16384 and more boxes
+constexpr int kShapes = 4;        // bit 0: persistent grid, bit 1: half the lanes per box
+inline int size_class(int n) { return n < 1024 ? 0 : (n < 16384 ? 1 : 2); }
+inline int log2i(int v) {
+  int l = 0;
+  while ((1 << l) < v) l++;
+  return l;
+}
+
 struct Pool {
   Tree tree;
   // No GPU visible: the tree core still works (topology calls), every
@@ -165,6 +180,16 @@ struct Pool {
   // events around `count` consecutive launches of one kernel class
   int prof_begin(int kind, int lvl, double bytes, int count = 1);
   void prof_end(int idx);
+  unsigned char shape_tab[kShapeKernels][2][6][kSizeClasses] = {};
+  int shape(int kernel, int dim, int nc, int n) const {
+    return shape_tab[kernel][dim - 2][log2i(nc)][size_class(n)];
+  }
+  // workgroups of a persistent grid: 8 per compute unit
+  int n_cu = 256;
+  int persist_grid(int nblk) const {
+    const int g = std::min(nblk, 8 * n_cu);
+    return g >= 8 ? g - g % 8 : g;
+  }
   int bc_kind[6] = {0, 0, 0, 0, 0, 0};
   double bc_val[6] = {0, 0, 0, 0, 0, 0};
   // Multigrid call sequences captured as HIP graphs and replayed while the

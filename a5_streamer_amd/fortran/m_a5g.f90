@@ -276,6 +276,30 @@ module m_a5g
        integer(c_int)        :: s
      end function c_fas_vcycle
 
+     function c_set_shape(t, kernel, nc, cls, shape) bind(C, name="a5g_set_shape") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: kernel, nc, cls, shape
+       integer(c_int)        :: s
+     end function c_set_shape
+
+     function c_get_shape(t, kernel, nc, cls, shape) bind(C, name="a5g_get_shape") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: kernel, nc, cls
+       integer(c_int)        :: shape
+       integer(c_int)        :: s
+     end function c_get_shape
+
+     function c_autotune(t, reps, max_rec, rec, n_rec) bind(C, name="a5g_autotune") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: reps, max_rec
+       real(c_double)        :: rec(*)
+       integer(c_int)        :: n_rec
+       integer(c_int)        :: s
+     end function c_autotune
+
      function c_mg_solve_mixed(t, tol, max_outer, n_inner, zero_start, n_outer, &
           rel_res) bind(C, name="a5g_mg_solve_mixed") result(s)
        import :: c_ptr, c_int, c_double
@@ -521,6 +545,15 @@ module m_a5g
        real(c_double), value :: base
        integer(c_int)        :: s
      end function c_set_gaussians
+
+     function c_set_gaussians_instances(t, iv, n_blob, blobs, base) &
+          bind(C, name="a5g_set_gaussians_instances") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv, n_blob
+       real(c_double)        :: blobs(*), base(*)
+       integer(c_int)        :: s
+     end function c_set_gaussians_instances
 
      function c_get_loc(t, r, max_lvl, id, ix) bind(C, name="a5g_get_loc") result(s)
        import :: c_ptr, c_int, c_double

@@ -206,6 +206,31 @@ class Tree:
     def fas_vcycle(self, max_lvl, set_residual):
         _call("a5g_fas_vcycle", self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
 
+    SHAPE_FIELDS = ("kernel", "dim", "nc", "size_class", "shape", "boxes", "us", "chosen")
+
+    def set_shape(self, kernel, nc, cls, shape):
+        _call("a5g_set_shape", self.h, C.c_int(kernel), C.c_int(nc), C.c_int(cls),
+              C.c_int(shape))
+
+    def get_shape(self, kernel, nc, cls):
+        s = C.c_int()
+        _call("a5g_get_shape", self.h, C.c_int(kernel), C.c_int(nc), C.c_int(cls), C.byref(s))
+        return s.value
+
+    def autotune(self, reps=20):
+        """Launch-shape autotuner (a5g_autotune): a list of dicts, one per
+        (box size, size class, shape) timed, SHAPE_FIELDS as keys."""
+        n = C.c_int()
+        rec = np.zeros(8 * 6 * 3 * 4)
+        _call("a5g_autotune", self.h, C.c_int(reps), C.c_int(len(rec) // 8), _d(rec)[1],
+              C.byref(n))
+        out = []
+        for i in range(min(n.value, len(rec) // 8)):
+            row = rec[8 * i:8 * i + 8]
+            out.append({k: (float(v) if k == "us" else int(v))
+                        for k, v in zip(self.SHAPE_FIELDS, row)})
+        return out
+
     def solve_mixed(self, tol=1e-10, max_outer=30, n_inner=1, zero_start=True):
         it, rel = C.c_int(), C.c_double()
         _call("a5g_mg_solve_mixed", self.h, C.c_double(tol), C.c_int(max_outer),
@@ -361,6 +386,13 @@ class Tree:
         b, pb = _d(np.asarray(blobs, float).reshape(-1, self.dim + 2))
         _call("a5g_set_gaussians", self.h, C.c_int(iv), C.c_int(b.shape[0]), pb,
               C.c_double(base))
+
+    def set_gaussians_instances(self, iv, blobs, base):
+        """blobs (n_inst, n_blob, dim + 2) in tree coordinates, base (n_inst,)."""
+        base = np.asarray(base, float)
+        b, pb = _d(np.asarray(blobs, float).reshape(len(base), -1, self.dim + 2))
+        bb, pbb = _d(base)
+        _call("a5g_set_gaussians_instances", self.h, C.c_int(iv), C.c_int(b.shape[1]), pb, pbb)
 
     def prof_enable(self, on):
         _call("a5g_prof_enable", self.h, C.c_int(int(on)))

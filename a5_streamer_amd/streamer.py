@@ -150,6 +150,38 @@ def dist_line_rows_per(r, r0, r1):
     return np.where(temp <= 0.0, to0, np.where(temp >= line_len2, to1, top))
 
 
+def seed_hits(centre, boxlen, dr, r0, r1, w):
+    """The seed rule of set_ref_flags for boxes (n rows) and one seed each
+    (r0, r1 (n, d), w (n,)): dist_line(centre, r0, r1) - w < boxlen and
+    dr > 0.2 w (src/streamer_2d.f90:200-211). The distance is computed only
+    for boxes that the cheap tests leave open: the second condition, and the
+    box centre's gap to the segment's bounding box (a lower bound of the
+    distance) below (w + boxlen)(1 + 1e-9); every other box is provably no
+    hit, so each flag equals the one computed for every box."""
+    hit = np.zeros(len(boxlen), bool)
+    idx = np.nonzero(dr > 0.2 * w)[0]
+    if len(idx) == 0:
+        return hit
+    c, a, e = centre[idx], r0[idx], r1[idx]
+    gap = np.max(np.maximum(0.0, np.maximum(np.minimum(a, e) - c, c - np.maximum(a, e))),
+                 axis=1)
+    idx = idx[gap < (w[idx] + boxlen[idx]) * (1 + 1e-9)]
+    if len(idx):
+        dist = dist_line_rows_per(centre[idx], r0[idx], r1[idx])
+        hit[idx] = dist - w[idx] < boxlen[idx]
+    return hit
+
+
+def group_max(g, v, n, empty):
+    """max of v over the rows of each group 0 <= g < n (empty for a group
+    with no rows). np.maximum.at with the values in the result's dtype (a
+    mixed-dtype call takes numpy's slow generic loop)."""
+    v = np.ascontiguousarray(v)
+    res = np.full(n, empty, dtype=v.dtype)
+    np.maximum.at(res, np.asarray(g, np.int64), v)
+    return res
+
+
 _LOG_HALF_HUGE = math.log(0.5 * 1.7976931348623157e308)
 
 
@@ -539,9 +571,12 @@ class StreamerRun:
             boxlen = nc[idx] * dr[idx]
             centre = np.stack([rmin[idx, k] + 0.5 * nc[idx] * dr[idx] for k in range(self.dim)],
                               axis=1)
+            m = len(idx)
             for k in range(len(c.seed_dens)):
-                dist = dist_line_rows(centre, c.seed_r0[k], c.seed_r1[k])
-                hit = (dist - c.seed_width[k] < boxlen) & (dr[idx] > 0.2 * c.seed_width[k])
+                hit = seed_hits(centre, boxlen, dr[idx],
+                                np.broadcast_to(np.asarray(c.seed_r0[k], float), (m, self.dim)),
+                                np.broadcast_to(np.asarray(c.seed_r1[k], float), (m, self.dim)),
+                                np.full(m, float(c.seed_width[k])))
                 f[idx[hit]] = 1
         f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
         for lvl_r, lo, hi in c.refine_regions:
@@ -872,8 +907,7 @@ class BatchRun:
                 dix = (b["ix"][sel, D - 1] - 1) & 1
                 loc[sel, D - 1] = loc[p, D - 1] + 0.5 * b["dr"][p] * dix * b["n_cell"][p]
         b["run"], b["r_loc"] = g, loc
-        self.inst_lvl = np.zeros(self.K, np.int64)
-        np.maximum.at(self.inst_lvl, g[use], lv[use])
+        self.inst_lvl = group_max(g[use], lv[use].astype(np.int64), self.K, 0)
         return b
 
     def _set_active(self, mask):
@@ -886,6 +920,9 @@ class BatchRun:
         uploaded for every box (a settled run gets the values it holds), phi
         is cleared on the active runs' boxes only."""
         T = self.T
+        if all(c.init_on_device for c in self.cfgs):
+            self._init_on_device(b)
+            return
         n = len(b["lvl"])
         bl = T.box_len(T.n_cell)
         elec_all, eps_all = np.zeros((n, bl)), np.zeros((n, bl))
@@ -905,6 +942,31 @@ class BatchRun:
         T.set_cc_all(I_EPS, eps_all)
         if clear:
             T.fill_cc(np.array(clear, np.int32), I_PHI, 0.0)
+
+    def _init_on_device(self, b):
+        """set_init_cond of the active runs on the device
+        (a5g_set_gaussians_instances, Config.init_on_device): every run's
+        seeds moved to its place in the tree, its own background."""
+        T, D = self.T, self.dim
+        n_seed = {len(c.seed_dens) for c in self.cfgs}
+        if len(n_seed) != 1:
+            raise ValueError("runs of a batch must have the same number of seeds")
+        blobs = np.zeros((self.K, n_seed.pop(), D + 2))
+        for g, c in enumerate(self.cfgs):
+            if any(f != 2 for f in c.seed_falloff) or \
+                    any(list(a) != list(e) for a, e in zip(c.seed_r0, c.seed_r1)):
+                raise ValueError("init_on_device: point seeds with falloff 2 only")
+            for k in range(len(c.seed_dens)):
+                blobs[g, k, :D] = c.seed_r0[k]
+                blobs[g, k, D - 1] += g * self.H
+                blobs[g, k, D], blobs[g, k, D + 1] = c.seed_width[k], c.seed_dens[k]
+        T.set_gaussians_instances(I_ELEC, blobs, [c.bg_dens for c in self.cfgs])
+        T.tree_copy_cc(I_ELEC, I_PION)
+        use = b["in_use"] != 0
+        T.fill_cc((np.nonzero(use)[0] + 1).astype(np.int32), I_EPS, 1.0)
+        clear = np.nonzero(use & self.active[b["run"]])[0] + 1
+        if len(clear):
+            T.fill_cc(clear.astype(np.int32), I_PHI, 0.0)
 
     def compute_fld(self, no_guess):
         """compute_fld of the active runs. The FMG of a run goes up to its own
@@ -979,19 +1041,36 @@ class BatchRun:
         early = ok & (times[g] < v["init_window"])
         if early.any():
             idx = np.nonzero(early)[0]
-            boxlen = nc[idx] * dr[idx]
-            centre = np.stack([b["r_loc"][idx, k] + 0.5 * nc[idx] * dr[idx]
-                               for k in range(self.dim)], axis=1)
-            gi_of = g[idx]
             for k in range(n_seed.pop()):
-                # seed k of each box's run, gathered per box
-                r0 = np.array([c.seed_r0[k] for c in self.cfgs], float)[gi_of]
-                r1 = np.array([c.seed_r1[k] for c in self.cfgs], float)[gi_of]
-                w = np.array([c.seed_width[k] for c in self.cfgs], float)[gi_of]
-                dist = dist_line_rows_per(centre, r0, r1)
-                hit = (dist - w < boxlen) & (dr[idx] > 0.2 * w)
-                f[idx[hit]] = 1
+                # seed k of each box's run, gathered per box, for the boxes
+                # coarse enough for the rule (dr > 0.2 w)
+                w = np.array([c.seed_width[k] for c in self.cfgs], float)[g[idx]]
+                pre = dr[idx] > 0.2 * w
+                j = idx[pre]
+                if len(j) == 0:
+                    continue
+                gj = g[j]
+                r0 = np.array([c.seed_r0[k] for c in self.cfgs], float)[gj]
+                r1 = np.array([c.seed_r1[k] for c in self.cfgs], float)[gj]
+                centre = np.stack([b["r_loc"][j, d] + 0.5 * nc[j] * dr[j]
+                                   for d in range(self.dim)], axis=1)
+                f[j[seed_hits(centre, nc[j] * dr[j], dr[j], r0, r1, w[pre])]] = 1
         f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
+        # refine_regions in each run's own coordinates; runs with the same
+        # regions are treated together
+        by_regions = {}
+        for gi, c in enumerate(self.cfgs):
+            if c.refine_regions:
+                by_regions.setdefault(repr(c.refine_regions), []).append(gi)
+        if by_regions:
+            centre = np.stack([b["r_loc"][:, k] + 0.5 * nc * dr for k in range(self.dim)],
+                              axis=1)
+            for gis in by_regions.values():
+                mine = np.isin(g, gis)
+                for lvl_r, lo, hi in self.cfgs[gis[0]].refine_regions:
+                    inside = np.all((centre >= np.asarray(lo)) & (centre <= np.asarray(hi)),
+                                    axis=1)
+                    f[mine & inside & (np.asarray(b["lvl"]) < lvl_r)] = 1
         f[~ok] = 0
         return f
 
@@ -1001,10 +1080,8 @@ class BatchRun:
         dts = np.zeros(self.K)
         sel = (b["in_use"] != 0) & (b["lvl"] >= 1)
         leaf = sel & (b["children"][:, 0] == 0)
-        fld_k = np.full(self.K, -np.inf)
-        np.maximum.at(fld_k, b["run"][leaf], s[leaf, 1])
-        lvl_k = np.zeros(self.K, np.int64)
-        np.maximum.at(lvl_k, b["run"][sel], b["lvl"][sel])
+        fld_k = group_max(b["run"][leaf], s[leaf, 1], self.K, -np.inf)
+        lvl_k = group_max(b["run"][sel], b["lvl"][sel].astype(np.int64), self.K, 0)
         for g, r in enumerate(self.runs):
             max_fld = float(fld_k[g])
             dr_min = self.dr_base * 0.5 ** (int(lvl_k[g]) - 1)
@@ -1028,7 +1105,6 @@ class BatchRun:
             self.set_init_cond(b)
             self.compute_fld(True)
             add_k, _, _ = self.adjust(b)
-            self._boxes()
             settled |= self.active & (add_k == 0)
             if settled.all():
                 self._set_active(np.ones(self.K, bool))
@@ -1085,7 +1161,7 @@ class BatchRun:
             self.compute_fld(False)
             for r in self.runs:
                 r.step_no += 1
-        b = self._boxes()
+        # the steps change no topology: b still describes the tree
         add_k, rm_k, added = self.adjust(b)
         changed = (add_k > 0) | (rm_k > 0)
         if changed.any():

@@ -27,6 +27,35 @@ def splitmix64(seed):
         yield (z >> 11) * (1.0 / (1 << 53))
 
 
+# Refinement regions of the developed batched runs, (lvl, lo, hi) in (r, z):
+# everything to level 4, a 2 mm wide column to level 6 and a 0.6 mm wide
+# channel below the seed to level 8, the mesh of a streamer some 14 mm long
+# (about 1,100 boxes of 8 x 8 cells per run)
+BATCH_REGIONS = [(4, [0.0, 0.0], [16e-3, 32e-3]),
+                 (6, [0.0, 8e-3], [2e-3, 28e-3]),
+                 (8, [0.0, 12e-3], [0.6e-3, 26e-3])]
+
+
+def batch_configs(n, lvls_max, regions=None, init_on_device=False):
+    """The batched cylindrical workload: n runs of 16 x 32 mm (1 x 2 base
+    boxes of 8 x 8 cells), run k drawn from SplitMix64 seeded 20261203 + k:
+    applied field in [1.5, 2.5] MV/m, seed width in [0.2, 0.8] mm, background
+    in [1e14, 1e16] m^-3 (log-uniform), the seed on the axis at z = 24 mm.
+    regions: refine_regions of every run (None: the reference's criteria
+    only)."""
+    from . import streamer as ST
+    cfgs = []
+    for k in range(n):
+        g = splitmix64(20261203 + k)
+        fld, w, bg = 1.5e6 + 1.0e6 * next(g), 0.2e-3 + 0.6e-3 * next(g), 10 ** (14 + 2 * next(g))
+        cfgs.append(ST.Config(variant="cyl", domain_len=16e-3, n_base=(1, 2),
+                              lvls_max=lvls_max, applied_fld=fld, seed_width=[w], bg_dens=bg,
+                              seed_r0=[[0.0, 24e-3]], seed_r1=[[0.0, 24e-3]],
+                              refine_regions=list(regions or []),
+                              init_on_device=init_on_device))
+    return cfgs
+
+
 def base_grid(dim, n_base):
     """ix_list / nb_list of an n_base[0] x ... block of base boxes, physical
     boundaries (-1) on the domain faces (Afivo a$D_set_base conventions)."""

@@ -289,6 +289,32 @@ STEP_KINDS = {0: "gsrb_half_sweep", 1: "ghost_cells", 2: "residual_restrict",
               12: "average", 13: "restrict", 14: "photoi_rate"}
 
 
+def profile_block(T, step_block, steps_amr):
+    """One more block with the library's per-kind event timers on: per kind
+    the launches, ms per step, algorithmic GB/s and share of the kernel time;
+    the total kernel ms and algorithmic bytes of the block and its wall time."""
+    T.prof_enable(True)
+    T.sync()
+    t1 = time.perf_counter()
+    step_block()
+    T.sync()
+    t_prof = time.perf_counter() - t1
+    kinds = {}
+    tot_ms = 0.0
+    tot_bytes = 0.0
+    for k, name in STEP_KINDS.items():
+        n, ms, by = T.prof_read(k, -999)
+        if n:
+            kinds[name] = {"launches": n, "ms_per_step": round(ms / steps_amr, 3),
+                           "alg_GBps": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
+            tot_ms += ms
+            tot_bytes += by
+    T.prof_enable(False)
+    for v in kinds.values():
+        v["share"] = round(v["ms_per_step"] * steps_amr / tot_ms, 4)
+    return kinds, tot_ms, tot_bytes, t_prof
+
+
 # the production 3D streamer: a 64 mm cube of one base box (the reference's
 # init_tree), uniform level 6 (0.25 mm cells) and level 7 in the top quarter
 # where the streamer starts, the seed refining further
@@ -342,25 +368,7 @@ def streamer_step(args, steps=None, warmup=None):
     cyc1 = run.photoi_cycles
     c1 = run.leaf_cells()
     n_steps = blocks * cfg.steps_amr
-    T.prof_enable(True)
-    T.sync()
-    t1 = time.perf_counter()
-    run.step_block()
-    T.sync()
-    t_prof = time.perf_counter() - t1
-    kinds = {}
-    tot_ms = 0.0
-    tot_bytes = 0.0
-    for k, name in STEP_KINDS.items():
-        n, ms, by = T.prof_read(k, -999)
-        if n:
-            kinds[name] = {"launches": n, "ms_per_step": round(ms / cfg.steps_amr, 3),
-                           "alg_GBps": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
-            tot_ms += ms
-            tot_bytes += by
-    T.prof_enable(False)
-    for v in kinds.values():
-        v["share"] = round(v["ms_per_step"] * cfg.steps_amr / tot_ms, 4)
+    kinds, tot_ms, tot_bytes, t_prof = profile_block(T, run.step_block, cfg.steps_amr)
     ms_step = dt * 1e3 / n_steps
     # algorithmic bytes of one step (the profiled block's, per step) over the
     # time per step of the timed blocks
@@ -394,48 +402,29 @@ def streamer_step(args, steps=None, warmup=None):
     return out
 
 
-def splitmix64(seed):
-    """SplitMix64 (Steele, Lea and Flood 2014), as uniform doubles in [0, 1)."""
-    x = seed & 0xFFFFFFFFFFFFFFFF
-    while True:
-        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-        z = x
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-        yield (z ^ (z >> 31)) / 2.0 ** 64
-
-
-def batch_configs(n, lvls_max):
-    """The batched cylindrical workload: n runs of 16 x 32 mm (1 x 2 base
-    boxes of 8 x 8 cells), run k drawn from SplitMix64 seeded 20261203 + k:
-    applied field in [1.5, 2.5] MV/m, seed width in [0.2, 0.8] mm, background
-    in [1e14, 1e16] m^-3 (log-uniform), the seed on the axis at z = 24 mm."""
-    from a5_streamer_amd import streamer as ST
-    cfgs = []
-    for k in range(n):
-        g = splitmix64(20261203 + k)
-        fld, w, bg = 1.5e6 + 1.0e6 * next(g), 0.2e-3 + 0.6e-3 * next(g), 10 ** (14 + 2 * next(g))
-        cfgs.append(ST.Config(variant="cyl", domain_len=16e-3, n_base=(1, 2),
-                              lvls_max=lvls_max, applied_fld=fld, seed_width=[w], bg_dens=bg,
-                              seed_r0=[[0.0, 24e-3]], seed_r1=[[0.0, 24e-3]]))
-    return cfgs
-
-
-def batch_runs(args):
+def batch_runs(args, n=None, blocks=None, sample=None):
     """Many cylindrical runs in one tree (BatchRun over a5g_set_instances)
-    against the same runs stepped one by one, each on its own tree. Prints
-    one JSON line: aggregate leaf Mcell-steps/s of both and their ratio."""
+    against the same runs stepped one by one, each on its own tree (the first
+    `sample` of them, whose rate stands for all: the runs one by one take
+    the sum of their times). Returns the line's dict: aggregate leaf
+    Mcell-steps/s of both and their ratio."""
     import a5_streamer_amd as A
     from a5_streamer_amd import streamer as ST
-    cfgs = batch_configs(args.batch, args.batch_lvls)
-    blocks = max(1, args.steps // cfgs[0].steps_amr)
+    from a5_streamer_amd.workloads import BATCH_REGIONS, batch_configs
+    n = n or args.batch
+    regions = BATCH_REGIONS if args.batch_region else None
+    cfgs = batch_configs(n, args.batch_lvls, regions, init_on_device=True)
+    blocks = blocks or max(1, args.steps // cfgs[0].steps_amr)
+    sample = min(n, sample or args.batch_sample or n)
     t0 = time.perf_counter()
     B = ST.BatchRun(A.Tree, cfgs)
     B.initialise()
     t_init = time.perf_counter() - t0
+    print("batch of %d runs initialised in %.1f s" % (n, t_init), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         B.step_block()
     B.T.sync()
+    tune = B.T.autotune(reps=10) if args.autotune else []
     cells = []
     t0 = time.perf_counter()
     for _ in range(blocks):
@@ -445,11 +434,17 @@ def batch_runs(args):
     dt_b = time.perf_counter() - t0
     steps = blocks * cfgs[0].steps_amr
     v_b = sum(cells) * cfgs[0].steps_amr / dt_b / 1e6
-    boxes = int(B.T.info()["max_id"])
+    kinds, tot_ms, tot_bytes, t_prof = profile_block(B.T, B.step_block, cfgs[0].steps_amr)
+    step_bytes = tot_bytes / cfgs[0].steps_amr
+    step_ach = step_bytes / (dt_b / steps) / 1e9
+    nb = B._boxes()
+    per_run = np.bincount(nb["run"][nb["in_use"] != 0], minlength=n)
+    boxes = int((nb["in_use"] != 0).sum())
     B.T.close()
+    print("batch timed: %.1f ms per step" % (dt_b * 1e3 / steps), file=sys.stderr, flush=True)
     # the same runs one by one
     dt_s, cells_s = 0.0, 0
-    for c in cfgs:
+    for c in cfgs[:sample]:
         r = ST.StreamerRun(A.Tree, c)
         r.initialise()
         for _ in range(args.warmup):
@@ -463,21 +458,33 @@ def batch_runs(args):
         dt_s += time.perf_counter() - t0
         r.T.close()
     v_s = cells_s / dt_s / 1e6
-    line = {
+    return {
         "metric": "batched cylindrical runs, aggregate leaf Mcell-steps/s",
         "value": round(v_b, 3), "unit": "Mcell-steps/s", "steps": steps,
         "warmup_blocks": args.warmup, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "%d cylindrical runs of 16 x 32 mm, 1 x 2 base boxes of 8 x 8 "
                                "cells, lvls_max %d, SplitMix64 seed 20261203 + k: field "
-                               "1.5-2.5 MV/m, seed width 0.2-0.8 mm, background 1e14-1e16"
-                               % (len(cfgs), args.batch_lvls),
-                   "boxes_in_batch_tree": boxes, "leaf_cells_start": cells[0],
-                   "init_s": round(t_init, 1)},
+                               "1.5-2.5 MV/m, seed width 0.2-0.8 mm, background 1e14-1e16%s"
+                               % (n, args.batch_lvls,
+                                  ", refine regions %s" % regions if regions else ""),
+                   "boxes_in_batch_tree": boxes,
+                   "boxes_per_run": [int(per_run.min()), int(np.median(per_run)),
+                                     int(per_run.max())],
+                   "leaf_cells_start": cells[0], "init_s": round(t_init, 1)},
         "ms_per_step": round(dt_b * 1e3 / steps, 3),
-        "one_by_one": {"value": round(v_s, 3), "ms_per_step_all_runs": round(dt_s * 1e3 / steps, 3)},
+        "one_by_one": {"value": round(v_s, 3), "runs_timed": sample,
+                       "ms_per_step_all_runs": round(dt_s * n / sample * 1e3 / steps, 3)},
         "speedup_vs_one_by_one": round(v_b / v_s, 2),
+        "autotune": [{k: (round(r[k], 2) if k == "us" else r[k])
+                      for k in ("nc", "size_class", "shape", "boxes", "us", "chosen")}
+                     for r in tune],
+        "roofline": {"bound": "hbm", "step_bytes": round(step_bytes),
+                     "step_achieved": round(step_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "step_frac": round(step_ach / HBM_PEAK_GBS, 4)},
+        "kernels": kinds,
+        "profiled_block_s": round(t_prof, 3),
+        "kernel_ms_per_step": round(tot_ms / cfgs[0].steps_amr, 3),
     }
-    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -502,6 +509,13 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="measure this many batched cylindrical runs instead (own JSON line)")
     ap.add_argument("--batch-lvls", type=int, default=10)
+    ap.add_argument("--batch-region", type=int, default=1,
+                    help="1: refine every batched run to a developed streamer mesh (0: the "
+                         "reference's criteria only)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: run the launch-shape autotuner on the tree before timing")
+    ap.add_argument("--batch-sample", type=int, default=0,
+                    help="time this many of the runs one by one (0: all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -511,7 +525,7 @@ def main():
         print(json.dumps(streamer_step(args)), flush=True)
         return
     if args.batch:
-        batch_runs(args)
+        print(json.dumps(batch_runs(args)), flush=True)
         return
 
     from a5_streamer_amd.workloads import OctreePoisson

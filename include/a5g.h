@@ -98,6 +98,25 @@ int a5g_mg_set_helmholtz(a5g_tree *t, double kappa2);
 int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 /* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
 int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
+/* Launch shapes of the box kernels (no reference counterpart: the
+ * reference's loops over boxes have no launch geometry). kernel 0 is the
+ * multigrid half-sweep; for boxes of nc cells a side (2..32) on levels of
+ * size class cls (0: < 1024 boxes, 1: < 16384, 2: more), shape 0 is the
+ * default (lanes per box from the cells of one colour, 256-lane
+ * workgroups, one workgroup per group of boxes), 1 the same on a persistent
+ * grid (8 workgroups per compute unit walking the list), 2 half the lanes
+ * per box and twice the boxes per workgroup, 3 that on a persistent grid.
+ * Every shape computes the same values. */
+int a5g_set_shape(a5g_tree *t, int kernel, int nc, int cls, int shape);
+int a5g_get_shape(a5g_tree *t, int kernel, int nc, int cls, int *shape);
+/* Autotuner: times every shape of the half-sweep on the tree's current
+ * levels (reps half-sweeps per shape and level on the multigrid's
+ * potential, which is restored afterwards) and keeps, per box size and size
+ * class present, the fastest when it beats the default by more than 3%.
+ * Writes up to max_rec rows of 8 doubles (kernel, dim, nc, size class,
+ * shape, boxes timed, microseconds per half-sweep, 1 if chosen) and their
+ * count to *n_rec. */
+int a5g_autotune(a5g_tree *t, int reps, int max_rec, double *rec, int *n_rec);
 /* Mixed-precision solve of the same problem to max|r| <= tol max|rhs| over
  * the leaves: an FP64 defect-correction loop whose correction A e = r (zero
  * boundary data) is solved by FP32 FAS multigrid with the schedule of
@@ -265,6 +284,12 @@ int a5g_tree_maxdiff_cc(a5g_tree *t, int iv_a, int iv_b, double *maxdiff,
  * The Gaussian seed of m_geom.f90:82-86 (GM_gaussian) written for blobs. */
 int a5g_set_gaussians(a5g_tree *t, int iv, int n_blob, const double *blobs,
                       double base);
+/* a5g_set_gaussians per run of a tree with instances: run g's boxes take
+ * base[g] and blobs[g * n_blob .. (g + 1) * n_blob - 1] (centres in the
+ * tree's coordinates). The seeds of set_init_cond (src/streamer_cyl.f90,
+ * m_geom.f90:82-86) for every run of a batch in one call. */
+int a5g_set_gaussians_instances(a5g_tree *t, int iv, int n_blob, const double *blobs,
+                                const double *base);
 
 /* ---- independent runs in one tree ------------------------------------ */
 

@@ -78,6 +78,20 @@ def test_batch_flags_are_the_box_flags(oracle_lib):
             want = B.runs[g].box_flag(B.runs[g].time, s[i], dr[i], 8, b["r_loc"][i])
         assert got[i] == want, i
     assert set(got.tolist()) == {-1, 0, 1}
+    # refine regions (two runs share theirs, one has none): each run's
+    # flags_rows on its own boxes
+    before = got
+    regions = [(5, [0.0, 10e-3], [8e-3, 25e-3]), (7, [0.0, 18e-3], [4e-3, 25e-3])]
+    cfgs[0].refine_regions = cfgs[1].refine_regions = regions
+    got = B.flags(b, s)
+    ok = (b["in_use"] != 0) & (lvl >= 1)
+    for g in range(3):
+        mine = b["run"] == g
+        want = B.runs[g].flags_rows(B.runs[g].time, s[mine], dr[mine], b["n_cell"][mine],
+                                    b["r_loc"][mine], ok[mine] & B.active[g], lvl[mine])
+        np.testing.assert_array_equal(got[mine], want, err_msg="run %d" % g)
+    assert (got[b["run"] == 0] != before[b["run"] == 0]).any()
+    np.testing.assert_array_equal(got[b["run"] == 2], before[b["run"] == 2])
 
 
 def test_dist_line_rows_per_is_dist_line():

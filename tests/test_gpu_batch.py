@@ -66,6 +66,62 @@ def test_batch_matches_separate_runs(oracle_lib):
     assert list(batch.leaf_cells()) == [r.leaf_cells() for r in solo]
 
 
+def _batch_state(T, boxes, run):
+    """_state of one run of a batch tree, read with one get_cc_all per
+    variable."""
+    allv = {iv: T.get_cc_all(iv) for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD, S.I_RHS)}
+    out = {}
+    for i in np.nonzero((boxes["run"] == run) & (boxes["in_use"] != 0))[0]:
+        m = (int(boxes["n_cell"][i]) + 2) ** 2
+        out[(int(boxes["lvl"][i]), tuple(boxes["r_loc"][i]))] = np.concatenate(
+            [allv[iv][i, :m] for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD, S.I_RHS)])
+    return out
+
+
+@pytest.mark.parametrize("developed", [False, True])
+def test_batch_draws_match_separate_runs(oracle_lib, developed):
+    """16 seeded draws of the batched workload (field 1.5-2.5 MV/m, seed
+    width 0.2-0.8 mm, background 1e14-1e16 m^-3), batched, against each run
+    stepped alone through the oracle, bit for bit on every box and ghost.
+    developed: every run refined to the developed streamer mesh of the
+    benchmark (about 1,100 boxes a run) before it starts."""
+    import a5_streamer_amd as A
+    from a5_streamer_amd.workloads import BATCH_REGIONS, batch_configs
+    print("batch of 16 seeded draws, developed=%s" % developed, flush=True)
+    lvls_max = 12
+    cfgs = batch_configs(16, lvls_max, BATCH_REGIONS if developed else None)
+    batch = S.BatchRun(A.Tree, cfgs)
+    solo = [S.StreamerRun(O.OStreamTree, c) for c in cfgs]
+    batch.initialise()
+    for r in solo:
+        r.initialise()
+    n_blocks = 2 if developed else 5
+    for blk in range(n_blocks + 1):
+        if blk:
+            res = batch.step_block()
+            for g, r in enumerate(solo):
+                a, rm = r.step_block()
+                assert (res[0][g], res[1][g]) == (a, rm), (blk, g)
+        if blk not in (0, n_blocks):
+            continue
+        b = batch._boxes()
+        for g, r in enumerate(solo):
+            assert r.T.info()["max_lvl"] < lvls_max
+            assert batch.runs[g].time == r.time and batch.runs[g].dt == r.dt, (blk, g)
+            ob = r.T.boxes()
+            mine = _batch_state(batch.T, b, g)
+            ref = _state(r.T, ob, ob["in_use"] != 0, ob["r_min"])
+            assert mine.keys() == ref.keys(), (blk, g)
+            for k in ref:
+                np.testing.assert_array_equal(mine[k], ref[k], err_msg="block %d run %d box %s"
+                                              % (blk, g, k))
+    assert list(batch.leaf_cells()) == [r.leaf_cells() for r in solo]
+    if developed:
+        per_run = np.bincount(b["run"][b["in_use"] != 0], minlength=16)
+        assert per_run.min() >= 200 and per_run.max() <= 3000, per_run
+    batch.T.close()
+
+
 def test_groups_side_by_side_match_groups_in_turn():
     """The field solves of run groups side by side (a5g_fas_fmg_groups) give
     every box, ghosts included, the doubles that the groups solved one after
@@ -89,3 +145,38 @@ def test_groups_side_by_side_match_groups_in_turn():
             np.testing.assert_array_equal(runs[0].T.get_cc(int(i) + 1, iv),
                                           runs[1].T.get_cc(int(i) + 1, iv),
                                           err_msg="box %d var %d" % (i + 1, iv))
+
+
+def test_batch_init_on_device_matches_host_init():
+    """set_init_cond of every run on the device (a5g_set_gaussians_instances)
+    gives the host's densities within rounding (the device sums the squares
+    of the distance plainly, the host as the reference does: 1e-12 relative)
+    and the same eps and cleared potential, on the mesh after the initial
+    refinement."""
+    import a5_streamer_amd as A
+    print("batch device initial condition", flush=True)
+    B = S.BatchRun(A.Tree, _cfgs())
+    B.initialise()
+    b = B._boxes()
+    use = np.nonzero(b["in_use"])[0]
+    ivs = (S.I_ELEC, S.I_PION, S.I_EPS)
+    host = {iv: B.T.get_cc_all(iv)[use] for iv in ivs}
+    for c in B.cfgs:
+        c.init_on_device = True
+    B.T.fill_cc((use + 1).astype(np.int32), S.I_PHI, 7.0)
+    for iv in ivs:
+        B.T.fill_cc((use + 1).astype(np.int32), iv, -1.0)
+    B._set_active(np.ones(B.K, bool))
+    B.set_init_cond(b)
+    bl = B.T.box_len(B.T.n_cell)
+    for iv in ivs:
+        dev = B.T.get_cc_all(iv)[use]
+        for r, i in enumerate(use):
+            m = (int(b["n_cell"][i]) + 2) ** 2
+            np.testing.assert_allclose(dev[r, :m], host[iv][r, :m], rtol=1e-12, atol=0,
+                                       err_msg="var %d box %d" % (iv, i + 1))
+        assert dev.shape[1] == bl
+    phi = B.T.get_cc_all(S.I_PHI)[use]
+    for r, i in enumerate(use):
+        m = (int(b["n_cell"][i]) + 2) ** 2
+        assert (phi[r, :m] == 0.0).all()

@@ -111,3 +111,18 @@ def test_abi_exports_every_declared_symbol():
     lib = ctypes.CDLL(A.lib_path())
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_shape_arguments_are_checked():
+    """a5g_set_shape / a5g_get_shape work on the host tree and reject an
+    unknown kernel, box size, size class or shape."""
+    import a5_streamer_amd as A
+    T = A.Tree(2, 8, 3, 0, 1.0 / 8)
+    for args in ((1, 8, 0, 0), (0, 6, 0, 0), (0, 8, 3, 0), (0, 8, 0, 4)):
+        with pytest.raises(Exception):
+            T.set_shape(*args)
+    assert T.get_shape(0, 8, 2) == 0
+    T.set_shape(0, 8, 2, 3)
+    assert T.get_shape(0, 8, 2) == 3
+    assert T.get_shape(0, 16, 2) == 0
+    T.close()
