@@ -172,6 +172,26 @@ def seed_hits(centre, boxlen, dr, r0, r1, w):
     return hit
 
 
+_POOL = None
+
+
+def _chunked(n, fn, chunk=32768):
+    """fn(lo, hi) over [0, n) in chunks, on a small thread pool when there
+    is more than one chunk."""
+    global _POOL
+    bounds = [(lo, min(n, lo + chunk)) for lo in range(0, n, chunk)]
+    if len(bounds) <= 1:
+        for lo, hi in bounds:
+            fn(lo, hi)
+        return
+    if _POOL is None:
+        import concurrent.futures
+        import os
+        _POOL = concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1))
+    for fut in [_POOL.submit(fn, lo, hi) for lo, hi in bounds]:
+        fut.result()
+
+
 def group_max(g, v, n, empty):
     """max of v over the rows of each group 0 <= g < n (empty for a group
     with no rows). np.maximum.at with the values in the result's dtype (a
@@ -1024,54 +1044,69 @@ class BatchRun:
     def flags(self, b, s):
         """set_ref_flags of every box of the active runs (StreamerRun.box_flag
         with array operations over all runs at once, each flag the same), keep
-        for the others."""
-        c0, v = self.cfgs[0], self.v
-        g = b["run"]
-        ok = (b["in_use"] != 0) & (b["lvl"] >= 1) & self.active[g]
+        for the others. The boxes are split into chunks flagged on a thread
+        pool (numpy releases the interpreter lock inside its array
+        operations); every operation is per box, so chunking changes no flag."""
+        v = self.v
         n_seed = {len(c.seed_dens) for c in self.cfgs}
         if len(n_seed) != 1:
             raise ValueError("runs of a batch must have the same number of seeds")
-        dr, nc = b["dr"], b["n_cell"]
-        crv_phi, max_fld = s[:, 0], s[:, 1]
-        adx = dr * self.runs[0].tbl.col_rows(T_ALPHA, max_fld)
-        f = np.zeros(len(dr), np.int32)
-        f[(adx < 0.1) & (dr < 2.0e-5)] = -1
-        f[(adx < 0.1) & (crv_phi < 4.0) & (dr < 5.0e-5)] = -1
+        n_seed = n_seed.pop()
+        # per-run tables, gathered per box in the chunks
         times = np.array([r.time for r in self.runs])
-        early = ok & (times[g] < v["init_window"])
-        if early.any():
-            idx = np.nonzero(early)[0]
-            for k in range(n_seed.pop()):
-                # seed k of each box's run, gathered per box, for the boxes
-                # coarse enough for the rule (dr > 0.2 w)
-                w = np.array([c.seed_width[k] for c in self.cfgs], float)[g[idx]]
+        W = np.array([[c.seed_width[k] for k in range(n_seed)] for c in self.cfgs], float)
+        R0 = np.array([[c.seed_r0[k] for k in range(n_seed)] for c in self.cfgs], float)
+        R1 = np.array([[c.seed_r1[k] for k in range(n_seed)] for c in self.cfgs], float)
+        # refine_regions: runs with the same regions share a region group
+        reg_of, regions = np.full(self.K, -1, np.int64), []
+        seen = {}
+        for gi, c in enumerate(self.cfgs):
+            if c.refine_regions:
+                key = repr(c.refine_regions)
+                if key not in seen:
+                    seen[key] = len(regions)
+                    regions.append(c.refine_regions)
+                reg_of[gi] = seen[key]
+        tbl = self.runs[0].tbl
+        D = self.dim
+        f = np.zeros(len(b["dr"]), np.int32)
+
+        def rows(lo, hi):
+            g = b["run"][lo:hi]
+            dr, nc, lvl = b["dr"][lo:hi], b["n_cell"][lo:hi], b["lvl"][lo:hi]
+            rl = b["r_loc"][lo:hi]
+            crv_phi, max_fld = s[lo:hi, 0], s[lo:hi, 1]
+            ok = (b["in_use"][lo:hi] != 0) & (lvl >= 1) & self.active[g]
+            adx = dr * tbl.col_rows(T_ALPHA, max_fld)
+            fc = np.zeros(hi - lo, np.int32)
+            fc[(adx < 0.1) & (dr < 2.0e-5)] = -1
+            fc[(adx < 0.1) & (crv_phi < 4.0) & (dr < 5.0e-5)] = -1
+            idx = np.nonzero(ok & (times[g] < v["init_window"]))[0]
+            for k in range(n_seed if len(idx) else 0):
+                # seed k of each box's run, for the boxes coarse enough for
+                # the rule (dr > 0.2 w)
+                w = W[g[idx], k]
                 pre = dr[idx] > 0.2 * w
                 j = idx[pre]
                 if len(j) == 0:
                     continue
                 gj = g[j]
-                r0 = np.array([c.seed_r0[k] for c in self.cfgs], float)[gj]
-                r1 = np.array([c.seed_r1[k] for c in self.cfgs], float)[gj]
-                centre = np.stack([b["r_loc"][j, d] + 0.5 * nc[j] * dr[j]
-                                   for d in range(self.dim)], axis=1)
-                f[j[seed_hits(centre, nc[j] * dr[j], dr[j], r0, r1, w[pre])]] = 1
-        f[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
-        # refine_regions in each run's own coordinates; runs with the same
-        # regions are treated together
-        by_regions = {}
-        for gi, c in enumerate(self.cfgs):
-            if c.refine_regions:
-                by_regions.setdefault(repr(c.refine_regions), []).append(gi)
-        if by_regions:
-            centre = np.stack([b["r_loc"][:, k] + 0.5 * nc * dr for k in range(self.dim)],
-                              axis=1)
-            for gis in by_regions.values():
-                mine = np.isin(g, gis)
-                for lvl_r, lo, hi in self.cfgs[gis[0]].refine_regions:
-                    inside = np.all((centre >= np.asarray(lo)) & (centre <= np.asarray(hi)),
-                                    axis=1)
-                    f[mine & inside & (np.asarray(b["lvl"]) < lvl_r)] = 1
-        f[~ok] = 0
+                centre = np.stack([rl[j, d] + 0.5 * nc[j] * dr[j] for d in range(D)], axis=1)
+                fc[j[seed_hits(centre, nc[j] * dr[j], dr[j], R0[gj, k], R1[gj, k], w[pre])]] = 1
+            fc[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
+            if regions:
+                centre = np.stack([rl[:, d] + 0.5 * nc * dr for d in range(D)], axis=1)
+                rg = reg_of[g]
+                for ri, regs in enumerate(regions):
+                    mine = rg == ri
+                    for lvl_r, rlo, rhi in regs:
+                        inside = np.all((centre >= np.asarray(rlo)) & (centre <= np.asarray(rhi)),
+                                        axis=1)
+                        fc[mine & inside & (lvl < lvl_r)] = 1
+            fc[~ok] = 0
+            f[lo:hi] = fc
+
+        _chunked(len(f), rows)
         return f
 
     def max_dt(self, b):

@@ -289,6 +289,19 @@ STEP_KINDS = {0: "gsrb_half_sweep", 1: "ghost_cells", 2: "residual_restrict",
               12: "average", 13: "restrict", 14: "photoi_rate"}
 
 
+def tune_summary(rec):
+    """The autotuner's rows, per (box size, size class): the shape chosen and
+    the microseconds per half-sweep of shapes 0..3 over the boxes timed."""
+    out = {}
+    for r in rec:
+        k = "nc%d_class%d" % (r["nc"], r["size_class"])
+        e = out.setdefault(k, {"boxes": r["boxes"], "us": [None] * 4, "chosen": 0})
+        e["us"][r["shape"]] = round(r["us"], 2)
+        if r["chosen"]:
+            e["chosen"] = r["shape"]
+    return out
+
+
 def profile_block(T, step_block, steps_amr):
     """One more block with the library's per-kind event timers on: per kind
     the launches, ms per step, algorithmic GB/s and share of the kernel time;
@@ -354,6 +367,7 @@ def streamer_step(args, steps=None, warmup=None):
     print("streamer step: initial refinement done in %.1f s, %d leaf cells, %d boxes" % (
         t_init, cells, run.T.info()["max_id"]), file=sys.stderr, flush=True)
     blocks = max(1, steps // cfg.steps_amr)
+    tune = run.T.autotune(reps=10) if args.autotune else []
     for _ in range(warmup):
         run.step_block()
     T = run.T
@@ -397,6 +411,7 @@ def streamer_step(args, steps=None, warmup=None):
         "kernels": kinds,
         "profiled_block_s": round(t_prof, 3),
         "kernel_ms_per_step": round(tot_ms / cfg.steps_amr, 3),
+        "autotune": tune_summary(tune),
     }
     T.close()
     return out
@@ -421,10 +436,12 @@ def batch_runs(args, n=None, blocks=None, sample=None):
     B.initialise()
     t_init = time.perf_counter() - t0
     print("batch of %d runs initialised in %.1f s" % (n, t_init), file=sys.stderr, flush=True)
+    # the autotuner times shapes on the settled mesh; the warmup blocks then
+    # capture the graphs with the chosen shapes
+    tune = B.T.autotune(reps=10) if args.autotune else []
     for _ in range(args.warmup):
         B.step_block()
     B.T.sync()
-    tune = B.T.autotune(reps=10) if args.autotune else []
     cells = []
     t0 = time.perf_counter()
     for _ in range(blocks):
@@ -442,9 +459,9 @@ def batch_runs(args, n=None, blocks=None, sample=None):
     boxes = int((nb["in_use"] != 0).sum())
     B.T.close()
     print("batch timed: %.1f ms per step" % (dt_b * 1e3 / steps), file=sys.stderr, flush=True)
-    # the same runs one by one
+    # the same runs one by one (sample < 0: not timed)
     dt_s, cells_s = 0.0, 0
-    for c in cfgs[:sample]:
+    for c in cfgs[:max(sample, 0)]:
         r = ST.StreamerRun(A.Tree, c)
         r.initialise()
         for _ in range(args.warmup):
@@ -457,7 +474,7 @@ def batch_runs(args, n=None, blocks=None, sample=None):
         r.T.sync()
         dt_s += time.perf_counter() - t0
         r.T.close()
-    v_s = cells_s / dt_s / 1e6
+    v_s = cells_s / dt_s / 1e6 if dt_s > 0 else None
     return {
         "metric": "batched cylindrical runs, aggregate leaf Mcell-steps/s",
         "value": round(v_b, 3), "unit": "Mcell-steps/s", "steps": steps,
@@ -473,11 +490,10 @@ def batch_runs(args, n=None, blocks=None, sample=None):
                    "leaf_cells_start": cells[0], "init_s": round(t_init, 1)},
         "ms_per_step": round(dt_b * 1e3 / steps, 3),
         "one_by_one": {"value": round(v_s, 3), "runs_timed": sample,
-                       "ms_per_step_all_runs": round(dt_s * n / sample * 1e3 / steps, 3)},
-        "speedup_vs_one_by_one": round(v_b / v_s, 2),
-        "autotune": [{k: (round(r[k], 2) if k == "us" else r[k])
-                      for k in ("nc", "size_class", "shape", "boxes", "us", "chosen")}
-                     for r in tune],
+                       "ms_per_step_all_runs": round(dt_s * n / sample * 1e3 / steps, 3)}
+        if v_s else None,
+        "speedup_vs_one_by_one": round(v_b / v_s, 2) if v_s else None,
+        "autotune": tune_summary(tune),
         "roofline": {"bound": "hbm", "step_bytes": round(step_bytes),
                      "step_achieved": round(step_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "step_frac": round(step_ach / HBM_PEAK_GBS, 4)},
@@ -488,6 +504,17 @@ def batch_runs(args, n=None, blocks=None, sample=None):
 
 
 def main():
+    # The JSON line goes to the process's stdout; everything native code
+    # writes to file descriptor 1 (the reference library of the CPU baseline
+    # reports box storage growth there, buffered until exit) goes to stderr.
+    json_out = os.fdopen(os.dup(1), "w")
+    sys.stdout.flush()
+    os.dup2(2, 1)
+
+    def emit(d):
+        json_out.write(json.dumps(d) + "\n")
+        json_out.flush()
+
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -506,6 +533,8 @@ def main():
     ap.add_argument("--no-photoi", action="store_true")
     ap.add_argument("--photoi-cold", action="store_true",
                     help="solve every photoionization term by FMG from zero each step")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave the batched cylindrical runs out of the headline line")
     ap.add_argument("--batch", type=int, default=0,
                     help="measure this many batched cylindrical runs instead (own JSON line)")
     ap.add_argument("--batch-lvls", type=int, default=10)
@@ -515,17 +544,17 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: run the launch-shape autotuner on the tree before timing")
     ap.add_argument("--batch-sample", type=int, default=0,
-                    help="time this many of the runs one by one (0: all)")
+                    help="time this many of the runs one by one (0: all, -1: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.streamer_step:
-        print(json.dumps(streamer_step(args)), flush=True)
+        emit(streamer_step(args))
         return
     if args.batch:
-        print(json.dumps(batch_runs(args)), flush=True)
+        emit(batch_runs(args))
         return
 
     from a5_streamer_amd.workloads import OctreePoisson
@@ -539,6 +568,7 @@ def main():
 
     # --- FP64 ---------------------------------------------------------------
     n_v, _ = converge_count(T, max_lvl, tol)
+    tune = T.autotune(reps=10) if args.autotune else []
     for _ in range(args.warmup):
         solve(T, max_lvl, n_v)
     dt64 = timed(T, args.steps, lambda: solve(T, max_lvl, n_v))
@@ -566,7 +596,8 @@ def main():
         "fp64": {"value": round(v64, 3), "ms_per_step": round(dt64 * 1e3 / args.steps, 3),
                  "v_cycles_per_solve": n_v, "rel_residual": rel64,
                  "roofline": roofline(T, "fp64", lambda: solve(T, max_lvl, n_v)),
-                 "kernel_ms_per_step": {k: round(v[1], 3) for k, v in prof64.items()}},
+                 "kernel_ms_per_step": {k: round(v[1], 3) for k, v in prof64.items()},
+                 "autotune": tune_summary(tune)},
     }
     head = "fp64"
     if not args.fp64_only:
@@ -624,13 +655,17 @@ def main():
         # the production 3D streamer step, a few timed steps after its
         # initial refinement
         out["streamer3d"] = streamer_step(args, steps=6, warmup=1)
+    if not args.no_batched and not args.small:
+        # the batched cylindrical runs: 256 runs of about 1,100 boxes each,
+        # against a sample of them stepped one by one
+        out["batched"] = batch_runs(args, n=256, blocks=2, sample=4)
     # keys first that the driver reads
     order = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
              "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
              "roofline", "cpu_baseline"]
     line = {k: out[k] for k in order if k in out}
     line.update({k: v for k, v in out.items() if k not in line})
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 if __name__ == "__main__":
