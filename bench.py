@@ -503,6 +503,69 @@ def batch_runs(args, n=None, blocks=None, sample=None):
     }
 
 
+def level_breakdown(args):
+    """Per level of the benchmark octree: launches and event-timed ms of each
+    kernel class in one FP64 solve (FMG + the V-cycles to 1e-10) and in one
+    mixed solve (FP64 defect correction around FP32 V-cycles), with the wall
+    time of both (the mixed form's FP64 outer passes are kind 6)."""
+    from a5_streamer_amd.workloads import OctreePoisson
+    W = OctreePoisson()
+    T = build_gpu(W, n_cc=4)
+    max_lvl = T.info()["max_lvl"]
+    min_lvl = T.info()["min_lvl"]
+    tol = 1e-10
+    n_v, _ = converge_count(T, max_lvl, tol)
+    tune = T.autotune(reps=10) if args.autotune else []
+    legs = {"fp64": lambda: solve(T, max_lvl, n_v),
+            "mixed": lambda: T.solve_mixed(tol, 40, N_INNER, True)}
+    out = {"metric": "per-level kernel time of one solve, FP64 vs mixed",
+           "config": {"workload": "3D octree Poisson of the headline line", "tol": tol,
+                      "fp64_v_cycles": n_v, "mixed_inner": N_INNER,
+                      "boxes_per_level": {str(l): len(T.level(l)[0])
+                                          for l in range(min_lvl, max_lvl + 1)
+                                          if len(T.level(l)[0])}},
+           "autotune": tune_summary(tune)}
+    names = KIND_NAMES + ["fp64_outer"]
+    for leg, fn in legs.items():
+        for _ in range(2):
+            fn()
+        wall = timed(T, 3, fn) / 3
+        T.prof_enable(True)
+        T.sync()
+        fn()
+        T.sync()
+        lv = {}
+        for lvl in range(min_lvl, max_lvl + 1):
+            row = {}
+            for k, name in enumerate(names):
+                n, ms, _ = T.prof_read(k, lvl)
+                if n:
+                    row[name] = [n, round(ms, 4)]
+            if row:
+                lv[str(lvl)] = row
+        tot = {}
+        for k, name in enumerate(names):
+            n, ms, _ = T.prof_read(k, -999)
+            if n:
+                tot[name] = [n, round(ms, 4)]
+        T.prof_enable(False)
+        out[leg] = {"wall_ms": round(wall * 1e3, 3), "levels": lv, "total": tot}
+    # outer corrections and wall time of the mixed solve per number of FP32
+    # V-cycles in each correction
+    sweep = {}
+    for n_inner in (1, 2, 3, 4, 6):
+        info = {}
+
+        def one():
+            info["it"], info["rel"] = T.solve_mixed(tol, 60, n_inner, True)
+        one()
+        sweep[str(n_inner)] = {"outer": info["it"], "rel_residual": info["rel"],
+                               "wall_ms": round(timed(T, 3, one) / 3 * 1e3, 3)}
+    out["mixed_inner_sweep"] = sweep
+    T.close()
+    return out
+
+
 def main():
     # The JSON line goes to the process's stdout; everything native code
     # writes to file descriptor 1 (the reference library of the CPU baseline
@@ -533,6 +596,9 @@ def main():
     ap.add_argument("--no-photoi", action="store_true")
     ap.add_argument("--photoi-cold", action="store_true",
                     help="solve every photoionization term by FMG from zero each step")
+    ap.add_argument("--level-breakdown", action="store_true",
+                    help="per-level kernel times of one FP64 and one mixed solve of the "
+                         "octree instead (own JSON line)")
     ap.add_argument("--no-batched", action="store_true",
                     help="leave the batched cylindrical runs out of the headline line")
     ap.add_argument("--batch", type=int, default=0,
@@ -555,6 +621,9 @@ def main():
         return
     if args.batch:
         emit(batch_runs(args))
+        return
+    if args.level_breakdown:
+        emit(level_breakdown(args))
         return
 
     from a5_streamer_amd.workloads import OctreePoisson

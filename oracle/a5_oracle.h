@@ -27,7 +27,12 @@
  *    solution and far field (tests/test_oracle_helmholtz.py);
  *  - the axis walk and the output arrays: the reference's own a2_get_loc
  *    and m_vtk, geometry and brute force (tests/test_oracle_diag.py,
- *    tests/test_oracle_output.py).
+ *    tests/test_oracle_output.py);
+ *  - the reverse mode of the transport substep (no reference
+ *    counterpart): central differences of the restated forward substep
+ *    (itself pinned above) along 20 seeded directions, at two step sizes
+ *    whose error bounds follow the limiter's and the table's kinks
+ *    (tests/test_oracle_adjoint.py).
  * No function is left without such a check.
  *
  * Conventions follow the reference: box ids are 1-based, 0 = no box,
