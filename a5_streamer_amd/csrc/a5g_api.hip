@@ -127,6 +127,9 @@ void Pool::ensure_capacity(int need) {
 void Pool::sync_topology() {
   if (host_only) throw std::runtime_error("no GPU visible: cell data and kernels are unavailable");
   if (!topo_dirty && !lists_dirty) return;
+  // an upload belongs on the handle's stream, before any side-by-side section
+  if (stream != home)
+    throw std::logic_error("topology upload on a side stream: upload it before the fork");
   auto up = [&](void* dst, const void* src, size_t bytes) {
     hcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "topology upload");
   };
@@ -347,6 +350,11 @@ void Pool::ensure_diag(int n) {
 
 void* Pool::work(size_t bytes) {
   if (bytes > work_bytes || !d_work) {
+    // growing waits for the whole device, which a section running side by
+    // side on the side streams must not do (it would serialise them, and
+    // break a capture)
+    if (stream != home)
+      throw std::logic_error("work buffer growth on a side stream: grow it before the fork");
     // no kernel of an earlier call may still read the old buffer: hipFree
     // waits for the device
     if (d_work) hcheck(hipFree(d_work), "hipFree work");

@@ -2006,6 +2006,12 @@ static void run_captured(Pool& P, uint64_t key, F&& f) {
       return;
     }
   if (P.graphs.size() >= kMaxGraphs) {
+    if (P.stream != P.home) {
+      // inside a side-by-side section evicting would wait for every stream
+      // (branches makes room before the fork): run this call plainly
+      f();
+      return;
+    }
     // the oldest graph may still run on a side stream
     P.sync_streams();
     (void)hipGraphExecDestroy(P.graphs.front().second);
@@ -2538,6 +2544,19 @@ static void branches(Pool& P, const std::vector<int>& js, F&& f) {
     return;
   }
   P.ensure_side((int)js.size() - 1);
+  // Work that must wait for the device, or upload on the handle's stream,
+  // happens here, before the fork: the topology upload, and room in the
+  // graph cache for the captures of the section (a side stream never
+  // evicts; sync_topology and Pool::work refuse to run on one).
+  P.sync_topology();
+  const size_t room = 8 * js.size();
+  if (P.graphs_on && P.graphs.size() + room > kMaxGraphs) {
+    P.sync_streams();
+    while (!P.graphs.empty() && P.graphs.size() + room > kMaxGraphs) {
+      (void)hipGraphExecDestroy(P.graphs.front().second);
+      P.graphs.erase(P.graphs.begin());
+    }
+  }
   hipStream_t main = P.stream;
   check(hipEventRecord(P.ev_fork, main), "fork event");
   for (size_t k = 1; k < js.size(); k++)

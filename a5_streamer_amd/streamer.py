@@ -1095,14 +1095,19 @@ class BatchRun:
                 fc[j[seed_hits(centre, nc[j] * dr[j], dr[j], R0[gj, k], R1[gj, k], w[pre])]] = 1
             fc[(adx > 1.0) & (crv_phi > v["crv_refine"])] = 1
             if regions:
-                centre = np.stack([rl[:, d] + 0.5 * nc * dr for d in range(D)], axis=1)
                 rg = reg_of[g]
                 for ri, regs in enumerate(regions):
-                    mine = rg == ri
                     for lvl_r, rlo, rhi in regs:
-                        inside = np.all((centre >= np.asarray(rlo)) & (centre <= np.asarray(rhi)),
-                                        axis=1)
-                        fc[mine & inside & (lvl < lvl_r)] = 1
+                        # boxes of this region group below the region's level,
+                        # then their centres against the region, one axis at
+                        # a time
+                        j = np.nonzero((rg == ri) & (lvl < lvl_r))[0]
+                        for d in range(D):
+                            if len(j) == 0:
+                                break
+                            cd = rl[j, d] + 0.5 * nc[j] * dr[j]
+                            j = j[(cd >= rlo[d]) & (cd <= rhi[d])]
+                        fc[j] = 1
             fc[~ok] = 0
             f[lo:hi] = fc
 
