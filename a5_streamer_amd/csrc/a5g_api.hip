@@ -392,6 +392,10 @@ void Pool::release() {
   work_bytes = 0;
   f(d_tgrad);
   d_tgrad = nullptr;
+  f(d_rules);
+  d_rules = nullptr;
+  rules_cap = 0;
+  rules_set = false;
   tgrad_n = 0;
   d_inst = nullptr;
   d_flux_pairs = nullptr;
@@ -1209,6 +1213,39 @@ int a5g_box_summaries(a5g_tree* t, double* out) {
   FLUSH(t);
   need_gpu(t->P);
   a5g::box_summaries(t->P, t->P.sv, out);
+  API_END
+}
+
+int a5g_set_ref_rules(a5g_tree* t, int n_seed, const double* seeds, int n_reg,
+                      const double* regions, double init_window, double crv_refine,
+                      int n_base_last) {
+  API_BEGIN
+  NEED(t);
+  if (n_seed < 0 || n_reg < 0 || (n_seed > 0 && !seeds) || (n_reg > 0 && !regions))
+    throw std::invalid_argument("bad seed or region list");
+  if (n_base_last < 1) throw std::invalid_argument("n_base_last must be >= 1");
+  a5g::set_ref_rules(t->P, n_seed, seeds, n_reg, regions, init_window, crv_refine, n_base_last);
+  API_END
+}
+
+int a5g_ref_flags(a5g_tree* t, const double* times, int* flags) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!times || !flags) throw std::invalid_argument("null argument");
+  a5g::ref_flags(t->P, t->P.sv, times, flags);
+  API_END
+}
+
+int a5g_instance_maxima(a5g_tree* t, int iv, double* vmax, int* lvl_max) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  check_iv(t->P.tree, iv);
+  if (!vmax || !lvl_max) throw std::invalid_argument("null argument");
+  a5g::instance_maxima(t->P, iv, vmax, lvl_max);
   API_END
 }
 

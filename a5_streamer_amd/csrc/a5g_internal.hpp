@@ -36,6 +36,10 @@ void consistent_fluxes(Pool& P, int f_elec);
 void update_solution(Pool& P, const StVars& v, double dt);
 void average_dens(Pool& P, const StVars& v);
 void box_summaries(Pool& P, const StVars& v, double* out_host);
+void set_ref_rules(Pool& P, int n_seed, const double* seeds, int n_reg, const double* regions,
+                   double window, double crv_refine, int n_last);
+void ref_flags(Pool& P, const StVars& v, const double* times, int* flags_host);
+void instance_maxima(Pool& P, int iv, double* vmax, int* lvl_max);
 // reverse mode of the transport substep (a5g_*_adjoint)
 void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av);
 void consistent_fluxes_adj(Pool& P, int lF);

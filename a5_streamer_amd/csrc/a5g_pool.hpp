@@ -180,6 +180,18 @@ struct Pool {
   // events around `count` consecutive launches of one kernel class
   int prof_begin(int kind, int lvl, double bytes, int count = 1);
   void prof_end(int idx);
+  // The refinement rules of set_ref_flags for a5g_ref_flags
+  // (a5g_set_ref_rules): per run the seeds (r0, r1, width), the refinement
+  // regions shared by all runs (lvl, lo, hi), the time window of the seed
+  // rule, crv_refine and the base boxes per run along the last dimension.
+  // h_rules holds the seeds, then the regions; d_rules the same followed by
+  // one time per run, uploaded by each a5g_ref_flags.
+  std::vector<double> h_rules;
+  bool rules_set = false;
+  int rules_n_seed = 0, rules_n_reg = 0, rules_n_last = 1, rules_runs = 1;
+  double rules_window = 0, rules_crv = 0;
+  double* d_rules = nullptr;
+  size_t rules_cap = 0;
   unsigned char shape_tab[kShapeKernels][2][6][kSizeClasses] = {};
   int shape(int kernel, int dim, int nc, int n) const {
     return shape_tab[kernel][dim - 2][log2i(nc)][size_class(n)];

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -1401,6 +1402,250 @@ void fluxes_koren_adj(Pool& P, const StVars& v, double dt, const AdjVars& av) {
     if (D == 2) go(k_koren_adj<2>);
     else go(k_koren_adj<3>);
     chk(hipGetLastError(), "fluxes_koren_adj");
+  }
+}
+
+}  // namespace a5g
+
+namespace a5g {
+
+// ---------------------------------------------------------------------------
+// set_ref_flags on the device (src/streamer_2d.f90:177-216, with the 3D and
+// cylindrical drivers' crv_refine and seed windows): the flag of every box of
+// the active runs from its maxima (k_summary), in the arithmetic of the
+// host's rule, so that each flag is the host's. A box of run g is placed in
+// g's own coordinates the way a one-run tree places it: its level-1
+// ancestor at (ix - 1) dr box_size along the last dimension (ix counted
+// within the run), then each child's offset 0.5 dr dix n_cell added level by
+// level.
+// ---------------------------------------------------------------------------
+
+// GM_dist_line (src/m_geom.f90:29-46), the norm by norm2_flang
+template <int D>
+__device__ double dist_line_d(const double* r, const double* r0, const double* r1) {
+  double line_len2 = 0.0, temp = 0.0, d[3];
+  for (int k = 0; k < D; k++) {
+    const double t = r1[k] - r0[k];
+    line_len2 = line_len2 + t * t;
+  }
+  for (int k = 0; k < D; k++) temp = temp + (r[k] - r0[k]) * (r1[k] - r0[k]);
+  if (temp <= 0.0) {
+    for (int k = 0; k < D; k++) d[k] = r[k] - r0[k];
+  } else if (temp >= line_len2) {
+    for (int k = 0; k < D; k++) d[k] = r[k] - r1[k];
+  } else {
+    for (int k = 0; k < D; k++) d[k] = r[k] - (r0[k] + temp / line_len2 * (r1[k] - r0[k]));
+  }
+  return norm2_flang<D>(d);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, size_t n, int v) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n;
+       q += (size_t)gridDim.x * blockDim.x)
+    p[q] = v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_ref_flags(Dev a, Tbl tb, const int* __restrict__ list,
+                                                   int n, const double* __restrict__ sum,
+                                                   const double* __restrict__ rules, int n_seed,
+                                                   int n_reg, int n_runs, int n_last,
+                                                   double window, double crv_refine,
+                                                   int c_alpha, int* __restrict__ flags) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= n) return;
+  const int id = list[li];
+  const int g = a.n_inst ? inst_of<D>(a, id) : 0;
+  const double dr = a.dr[id];
+  const int nc = a.ncell[id];
+  const double crv_phi = sum[(size_t)(id - 1) * 3], max_fld = sum[(size_t)(id - 1) * 3 + 1];
+  int lti;
+  double ltf;
+  lt_loc(tb, max_fld, lti, ltf);
+  const double adx = dr * lt_col(tb, c_alpha, lti, ltf);
+  int f = 0;
+  if (adx < 0.1 && dr < 2.0e-5) f = -1;
+  if (adx < 0.1 && crv_phi < 4.0 && dr < 5.0e-5) f = -1;
+  // the box's r_min in its run's coordinates
+  double rl[3] = {a.rmin[id * 3], a.rmin[id * 3 + 1], a.rmin[id * 3 + 2]};
+  if (a.n_inst) {
+    int chain[64];
+    int m = 0, b = id;
+    while (a.lvl[b] > 1 && m < 64) {
+      chain[m++] = b;
+      b = a.parent[b];
+    }
+    const int ixl = a.ix[b * 3 + D - 1] - g * n_last;
+    double loc = 0.0 + (double)(ixl - 1) * a.dr[b] * (double)a.ncell[b];
+    for (int q = m - 1; q >= 0; q--) {
+      const int c = chain[q], p = a.parent[c];
+      const int dix = (a.ix[c * 3 + D - 1] - 1) & 1;
+      loc = loc + 0.5 * a.dr[p] * (double)dix * (double)a.ncell[p];
+    }
+    rl[D - 1] = loc;
+  }
+  double centre[3];
+  for (int k = 0; k < D; k++) centre[k] = rl[k] + 0.5 * nc * dr;
+  const double* seeds = rules;
+  const double* regions = rules + (size_t)n_runs * n_seed * (2 * D + 1);
+  const double* times = regions + (size_t)n_reg * (1 + 2 * D);
+  if (times[g] < window) {
+    const double boxlen = nc * dr;
+    for (int k = 0; k < n_seed; k++) {
+      const double* sd = seeds + ((size_t)g * n_seed + k) * (2 * D + 1);
+      const double w = sd[2 * D];
+      if (dist_line_d<D>(centre, sd, sd + D) - w < boxlen && dr > 0.2 * w) f = 1;
+    }
+  }
+  if (adx > 1.0 && crv_phi > crv_refine) f = 1;
+  for (int r = 0; r < n_reg; r++) {
+    const double* rg = regions + (size_t)r * (1 + 2 * D);
+    if (a.lvl[id] >= rg[0]) continue;
+    bool in = true;
+    for (int k = 0; k < D; k++) in = in && centre[k] >= rg[1 + k] && centre[k] <= rg[1 + D + k];
+    if (in) f = 1;
+  }
+  flags[id - 1] = f;
+}
+
+// Per run: the largest value of cc(iv) over the interiors of its leaf boxes
+// of levels >= 1 (ordered-integer keys) and its highest level holding boxes
+template <int D>
+__global__ __launch_bounds__(256) void k_inst_max(Dev a, const int* __restrict__ list, int n,
+                                                  int nc, int iv, int lvl, int leaves,
+                                                  unsigned long long* __restrict__ vkey,
+                                                  int* __restrict__ lmax) {
+  const int li = blockIdx.x;
+  if (li >= n) return;
+  const int id = list[li];
+  const int g = a.n_inst ? inst_of<D>(a, id) : 0;
+  if (threadIdx.x == 0) atomicMax(&lmax[g], lvl);
+  if (!leaves) return;
+  const double* v = boxp(a, iv, id);
+  double m = -__builtin_huge_val();
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    m = fmax(m, v[cidx<D>(nc, i, j, k)]);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) {
+    unsigned long long u = __double_as_longlong(m);
+    atomicMax(&vkey[g], (u >> 63) ? ~u : (u | 0x8000000000000000ull));
+  }
+}
+
+void set_ref_rules(Pool& P, int n_seed, const double* seeds, int n_reg, const double* regions,
+                   double window, double crv_refine, int n_last) {
+  const int D = P.tree.dim;
+  const int runs = P.n_inst ? P.n_inst : 1;
+  const size_t ns = (size_t)runs * n_seed * (2 * D + 1), nr = (size_t)n_reg * (1 + 2 * D);
+  P.h_rules.assign(seeds, seeds + ns);
+  P.h_rules.insert(P.h_rules.end(), regions, regions + nr);
+  P.rules_n_seed = n_seed;
+  P.rules_n_reg = n_reg;
+  P.rules_runs = runs;
+  P.rules_window = window;
+  P.rules_crv = crv_refine;
+  P.rules_n_last = n_last;
+  P.rules_set = true;
+}
+
+void ref_flags(Pool& P, const StVars& v, const double* times, int* flags_host) {
+  if (!P.rules_set) throw std::logic_error("ref_flags: call a5g_set_ref_rules first");
+  const int runs = P.n_inst ? P.n_inst : 1;
+  if (runs != P.rules_runs) throw std::logic_error("ref_flags: the rules were set for another number of runs");
+  P.sync_topology();
+  const auto& T = P.tree;
+  const int D = T.dim;
+  const Tbl tb = device_table(P);
+  const size_t n_id = (size_t)std::max(T.max_id, 1);
+  // the work buffer: maxima (3 per id), then the flags
+  double* d_sum = (double*)P.work(n_id * 3 * sizeof(double) + n_id * sizeof(int));
+  int* d_flags = (int*)(d_sum + n_id * 3);
+  // the rules and this call's times, in one upload (kept alive until the
+  // synchronisation at the end)
+  std::vector<double> buf(P.h_rules);
+  buf.insert(buf.end(), times, times + runs);
+  if (buf.size() > P.rules_cap) {
+    if (P.d_rules) chk(hipFree(P.d_rules), "hipFree rules");
+    P.d_rules = nullptr;
+    P.rules_cap = 0;
+    chk(hipMalloc((void**)&P.d_rules, buf.size() * sizeof(double)), "hipMalloc rules");
+    P.rules_cap = buf.size();
+  }
+  chk(hipMemcpyAsync(P.d_rules, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
+                     P.stream),
+      "rules upload");
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<size_t>((n_id + 255) / 256, 4096)),
+                     dim3(256), 0, P.stream, d_flags, n_id, 0);
+  chk(hipGetLastError(), "ref_flags fill");
+  for (int lvl = 1; lvl <= T.max_lvl; lvl++) {
+    const int n = P.n_ids(lvl);
+    if (!n) continue;
+    if (D == 2)
+      hipLaunchKernelGGL((k_summary<2>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, P.nc_lvl(lvl), v.rhs, v.fld, v.elec, d_sum);
+    else
+      hipLaunchKernelGGL((k_summary<3>), dim3(n), dim3(256), 0, P.stream, P.dev(), P.ids(lvl),
+                         n, P.nc_lvl(lvl), v.rhs, v.fld, v.elec, d_sum);
+    chk(hipGetLastError(), "ref_flags summaries");
+    const dim3 grid((n + 255) / 256);
+    if (D == 2)
+      hipLaunchKernelGGL((k_ref_flags<2>), grid, dim3(256), 0, P.stream, P.dev(), tb, P.ids(lvl),
+                         n, d_sum, P.d_rules, P.rules_n_seed, P.rules_n_reg, runs,
+                         P.rules_n_last, P.rules_window, P.rules_crv, v.t_alpha, d_flags);
+    else
+      hipLaunchKernelGGL((k_ref_flags<3>), grid, dim3(256), 0, P.stream, P.dev(), tb, P.ids(lvl),
+                         n, d_sum, P.d_rules, P.rules_n_seed, P.rules_n_reg, runs,
+                         P.rules_n_last, P.rules_window, P.rules_crv, v.t_alpha, d_flags);
+    chk(hipGetLastError(), "ref_flags");
+  }
+  chk(hipMemcpyAsync(flags_host, d_flags, (size_t)T.max_id * sizeof(int), hipMemcpyDeviceToHost,
+                     P.stream),
+      "flags download");
+  chk(hipStreamSynchronize(P.stream), "ref_flags sync");
+}
+
+void instance_maxima(Pool& P, int iv, double* vmax, int* lvl_max) {
+  P.sync_topology();
+  const auto& T = P.tree;
+  const int D = T.dim;
+  const int runs = P.n_inst ? P.n_inst : 1;
+  unsigned long long* d_key =
+      (unsigned long long*)P.work(runs * (sizeof(unsigned long long) + sizeof(int)));
+  int* d_lvl = (int*)(d_key + runs);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, P.stream, (double*)d_key, (size_t)runs);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, P.stream, d_lvl, (size_t)runs, 0);
+  chk(hipGetLastError(), "instance_maxima fill");
+  for (int lvl = 1; lvl <= T.max_lvl; lvl++) {
+    for (int leaves = 0; leaves < 2; leaves++) {
+      const int n = leaves ? P.n_leaves(lvl) : P.n_ids(lvl);
+      if (!n) continue;
+      const int* list = leaves ? P.leaves(lvl) : P.ids(lvl);
+      if (D == 2)
+        hipLaunchKernelGGL((k_inst_max<2>), dim3(n), dim3(256), 0, P.stream, P.dev(), list, n,
+                           P.nc_lvl(lvl), iv, lvl, leaves, d_key, d_lvl);
+      else
+        hipLaunchKernelGGL((k_inst_max<3>), dim3(n), dim3(256), 0, P.stream, P.dev(), list, n,
+                           P.nc_lvl(lvl), iv, lvl, leaves, d_key, d_lvl);
+      chk(hipGetLastError(), "instance_maxima");
+    }
+  }
+  std::vector<unsigned long long> key(runs);
+  chk(hipMemcpyAsync(key.data(), d_key, runs * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                     P.stream),
+      "maxima download");
+  chk(hipMemcpyAsync(lvl_max, d_lvl, runs * sizeof(int), hipMemcpyDeviceToHost, P.stream),
+      "levels download");
+  chk(hipStreamSynchronize(P.stream), "instance_maxima sync");
+  for (int g = 0; g < runs; g++) {
+    if (key[g] == 0) {
+      vmax[g] = -HUGE_VAL;
+      continue;
+    }
+    const unsigned long long u = (key[g] >> 63) ? (key[g] & 0x7fffffffffffffffull) : ~key[g];
+    std::memcpy(&vmax[g], &u, 8);
   }
 }
 

@@ -460,6 +460,34 @@ module m_a5g
        integer(c_int)     :: s
      end function c_box_summaries
 
+     function c_set_ref_rules(t, n_seed, seeds, n_reg, regions, init_window, crv_refine, &
+          n_base_last) bind(C, name="a5g_set_ref_rules") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: n_seed, n_reg, n_base_last
+       real(c_double)        :: seeds(*), regions(*)
+       real(c_double), value :: init_window, crv_refine
+       integer(c_int)        :: s
+     end function c_set_ref_rules
+
+     function c_ref_flags(t, times, flags) bind(C, name="a5g_ref_flags") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double)        :: times(*)
+       integer(c_int)        :: flags(*)
+       integer(c_int)        :: s
+     end function c_ref_flags
+
+     function c_instance_maxima(t, iv, vmax, lvl_max) bind(C, name="a5g_instance_maxima") &
+          result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv
+       real(c_double)        :: vmax(*)
+       integer(c_int)        :: lvl_max(*)
+       integer(c_int)        :: s
+     end function c_instance_maxima
+
      function c_update_solution_adjoint(t, dt, adj) &
           bind(C, name="a5g_update_solution_adjoint") result(s)
        import :: c_ptr, c_int, c_double

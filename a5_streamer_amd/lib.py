@@ -340,6 +340,29 @@ class Tree:
         _call("a5g_box_summaries", self.h, _d(out)[1])
         return out
 
+    def set_ref_rules(self, seeds, regions, init_window, crv_refine, n_base_last=1):
+        """seeds (runs, n_seed, 2 dim + 1), regions (n_reg, 1 + 2 dim)."""
+        sd = np.asarray(seeds, float)
+        rg = np.asarray(regions, float).reshape(-1, 1 + 2 * self.dim)
+        a, pa = _d(sd)
+        b, pb = _d(rg if len(rg) else np.zeros(1))
+        _call("a5g_set_ref_rules", self.h, C.c_int(sd.shape[1] if sd.ndim == 3 else 0), pa,
+              C.c_int(len(rg)), pb, C.c_double(init_window), C.c_double(crv_refine),
+              C.c_int(n_base_last))
+
+    def ref_flags(self, times):
+        n = self.info()["max_id"]
+        out = np.zeros(n, np.int32)
+        t, pt = _d(np.atleast_1d(np.asarray(times, float)))
+        _call("a5g_ref_flags", self.h, pt, _i(out)[1])
+        return out
+
+    def instance_maxima(self, iv, runs=1):
+        v = np.zeros(runs)
+        lv = np.zeros(runs, np.int32)
+        _call("a5g_instance_maxima", self.h, C.c_int(iv), _d(v)[1], _i(lv)[1])
+        return v, lv
+
     # --- reverse mode of the transport substep (a5g_*_adjoint) ------------
     def update_solution_adjoint(self, dt, adj):
         a, pa = _i(np.asarray(adj))

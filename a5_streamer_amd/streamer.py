@@ -427,6 +427,19 @@ class StreamerRun:
         T.mg_setup(I_PHI, I_FLD, I_RHS, i_eps=I_EPS, n_base=v["n_cycle_base"],
                    bc_kind=np.array(kinds, np.int32), bc_val=np.array(vals),
                    auto_ops=True)
+        # the refinement flags on the device when the tree has them
+        # (a5g_ref_flags: the same rule, each flag the same)
+        self.device_flags = hasattr(T, "set_ref_rules")
+        if self.device_flags:
+            D = self.dim
+            seeds = np.zeros((1, len(c.seed_dens), 2 * D + 1))
+            for k in range(len(c.seed_dens)):
+                seeds[0, k, :D] = c.seed_r0[k]
+                seeds[0, k, D:2 * D] = c.seed_r1[k]
+                seeds[0, k, 2 * D] = c.seed_width[k]
+            regions = [[lv] + list(lo) + list(hi) for lv, lo, hi in c.refine_regions]
+            T.set_ref_rules(seeds, np.array(regions, float).reshape(-1, 1 + 2 * D),
+                            v["init_window"], v["crv_refine"], 1)
 
     # --- set_init_cond / set_box_eps (src/streamer_2d.f90:218-268) -----------
     def _eps_at(self, r):
@@ -608,6 +621,8 @@ class StreamerRun:
 
     def ref_flags(self):
         T = self.T
+        if self.device_flags:
+            return T.ref_flags([self.time])
         b = T.boxes()
         s = T.box_summaries()
         ok = (b["in_use"] != 0) & (b["lvl"] >= 1)
@@ -904,6 +919,25 @@ class BatchRun:
         self.voltages = np.array([r.voltage for r in self.runs])
         T.set_instance_values(2 * D, self.voltages)
         self.active = np.ones(K, bool)
+        # refinement flags and time-step maxima on the device when the tree
+        # has them (a5g_ref_flags, a5g_instance_maxima) and every run shares
+        # its regions; the array forms on the host otherwise
+        self.device_flags = False
+        regs = {repr(c.refine_regions) for c in cfgs}
+        if hasattr(T, "set_ref_rules") and len(regs) == 1:
+            n_seed = {len(c.seed_dens) for c in cfgs}
+            if len(n_seed) == 1:
+                ns = n_seed.pop()
+                seeds = np.zeros((K, ns, 2 * D + 1))
+                for g, c in enumerate(cfgs):
+                    for k in range(ns):
+                        seeds[g, k, :D] = c.seed_r0[k]
+                        seeds[g, k, D:2 * D] = c.seed_r1[k]
+                        seeds[g, k, 2 * D] = c.seed_width[k]
+                regions = [[lv] + list(lo) + list(hi) for lv, lo, hi in c0.refine_regions]
+                T.set_ref_rules(seeds, np.array(regions, float).reshape(-1, 1 + 2 * D),
+                                v["init_window"], v["crv_refine"], self.n_last)
+                self.device_flags = True
 
     # --- geometry -------------------------------------------------------------
     def _boxes(self):
@@ -1025,8 +1059,10 @@ class BatchRun:
         """Refinement flags of the active runs (the others keep their mesh);
         returns the boxes added and removed per run."""
         T = self.T
-        s = T.box_summaries()
-        T.adjust(self.flags(b, s))
+        if self.device_flags:
+            T.adjust(T.ref_flags([r.time for r in self.runs]))
+        else:
+            T.adjust(self.flags(b, T.box_summaries()))
         add_k, rm_k = np.zeros(self.K, int), np.zeros(self.K, int)
         nb = T.boxes()
         added = []
@@ -1116,12 +1152,16 @@ class BatchRun:
 
     def max_dt(self, b):
         """get_max_dt per run from the per-box maxima of its leaves."""
-        s = self.T.box_summaries()
         dts = np.zeros(self.K)
-        sel = (b["in_use"] != 0) & (b["lvl"] >= 1)
-        leaf = sel & (b["children"][:, 0] == 0)
-        fld_k = group_max(b["run"][leaf], s[leaf, 1], self.K, -np.inf)
-        lvl_k = group_max(b["run"][sel], b["lvl"][sel].astype(np.int64), self.K, 0)
+        if self.device_flags and self.active.all():
+            # the same maxima reduced per run on the device
+            fld_k, lvl_k = self.T.instance_maxima(I_FLD, self.K)
+        else:
+            s = self.T.box_summaries()
+            sel = (b["in_use"] != 0) & (b["lvl"] >= 1)
+            leaf = sel & (b["children"][:, 0] == 0)
+            fld_k = group_max(b["run"][leaf], s[leaf, 1], self.K, -np.inf)
+            lvl_k = group_max(b["run"][sel], b["lvl"][sel].astype(np.int64), self.K, 0)
         for g, r in enumerate(self.runs):
             max_fld = float(fld_k[g])
             dr_min = self.dr_base * 0.5 ** (int(lvl_k[g]) - 1)
@@ -1218,4 +1258,10 @@ class BatchRun:
         b = self._boxes()
         leaf = (b["in_use"] != 0) & (b["lvl"] >= 1) & (b["children"][:, 0] == 0)
         nc = self.cfgs[0].box_size ** self.dim
-        return np.array([int(np.sum(leaf & (b["run"] == g))) * nc for g in range(self.K)])
+        return np.bincount(b["run"][leaf], minlength=self.K).astype(np.int64) * nc
+
+    def total_leaf_cells(self):
+        """Leaf cells of all runs together (no run attribution needed)."""
+        b = self.T.boxes()
+        leaf = (b["in_use"] != 0) & (b["lvl"] >= 1) & (b["children"][:, 0] == 0)
+        return int(leaf.sum()) * self.cfgs[0].box_size ** self.dim

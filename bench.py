@@ -445,7 +445,7 @@ def batch_runs(args, n=None, blocks=None, sample=None):
     cells = []
     t0 = time.perf_counter()
     for _ in range(blocks):
-        cells.append(int(B.leaf_cells().sum()))
+        cells.append(B.total_leaf_cells())
         B.step_block()
     B.T.sync()
     dt_b = time.perf_counter() - t0

@@ -221,6 +221,32 @@ int a5g_photoi_helmholtz_terms(a5g_tree *t, double coeff, int n_terms, const dou
 /* The maxima set_ref_flags reads per box (src/streamer_2d.f90:186-192):
  * out[id-1] = (dr^2 max|rhs|, max fld, max elec) for boxes of levels >= 1 */
 int a5g_box_summaries(a5g_tree *t, double *out);
+/* set_ref_flags (src/streamer_2d.f90:177-216; the 3D and cylindrical
+ * drivers differ in init_window and crv_refine) evaluated on the device for
+ * every box of levels >= 1 of the active runs, from the maxima of
+ * a5g_box_summaries, in the arithmetic of the drivers' rule (flang's NORM2
+ * for the seed distance, GM_dist_line of src/m_geom.f90:29-46), so each
+ * flag equals the host rule's. The reference evaluates the rule in its
+ * adjust_refinement callback; here the flags come back for
+ * a5g_adjust_refinement. Rules, set once: per run n_seed seeds of 2 dim + 1
+ * values (r0, r1, width), n_reg refinement regions shared by all runs of
+ * 1 + 2 dim values (level below which a box whose centre lies in [lo, hi]
+ * is refined; a driver variant for large runs, not in the reference),
+ * the time below which the seed rule applies, crv_refine, and the base
+ * boxes per run along the last dimension (a run's coordinates are those of
+ * a one-run tree). */
+int a5g_set_ref_rules(a5g_tree *t, int n_seed, const double *seeds, int n_reg,
+                      const double *regions, double init_window, double crv_refine,
+                      int n_base_last);
+/* times[g]: run g's time (one value without instances); flags[id-1] = -1,
+ * 0 or 1 for ids 1..max_id, 0 for boxes not flagged (levels < 1, runs not
+ * active). */
+int a5g_ref_flags(a5g_tree *t, const double *times, int *flags);
+/* Per run (one value without instances): the largest cc(iv) over the
+ * interiors of its leaf boxes of levels >= 1 (-inf if none), and its
+ * highest level holding boxes: the maxima get_max_dt reads
+ * (src/streamer_2d.f90:271-303). */
+int a5g_instance_maxima(a5g_tree *t, int iv, double *vmax, int *lvl_max);
 /* ---- reverse mode of the transport substep (frozen mesh) ------------- */
 
 /* The adjoints of one substep of the drivers' time loop, fluxes_koren, then
