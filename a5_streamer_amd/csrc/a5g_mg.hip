@@ -708,7 +708,9 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
 // reference's smoothing, fill_gc_phi after gsrb_boxes, would store: nothing
 // changes the child, its neighbours or the uncovered parent cells in
 // between), so that pass is not needed before this one.
-template <int D, bool CYL, int NC, typename T, bool EPS>
+// HALVE (a launch shape of the autotuner's menu): half the lanes per box,
+// twice the boxes per workgroup.
+template <int D, bool CYL, int NC, typename T, bool EPS, bool HALVE = false>
 __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
                                                         int keep_res) {
@@ -718,7 +720,8 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   constexpr int HNC = NC / 2;
   constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
   constexpr int TOTAL = D == 2 ? HNC * HNC : HNC * HNC * HNC;
-  constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int TPB0 = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
   constexpr int BPB = 256 / TPB;
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int NGH = 2 * D * PER_FACE;
@@ -995,7 +998,7 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
 // extrapolation, physical rule) instead of read, so that no ghost pass has
 // to store them after the restriction; with copy_tmp they also go to tmp's
 // face ghost slots, as the stored values would have.
-template <int D, bool CYL, int NC, typename T, bool EPS>
+template <int D, bool CYL, int NC, typename T, bool EPS, bool HALVE = false>
 __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
                                                         int copy_tmp, int fused) {
@@ -1006,7 +1009,8 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
   constexpr int NGC = HALF - NCOL;  // ghost slots of one colour
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int NGH = 2 * D * PER_FACE;
-  constexpr int TPB = BL > 128 ? 256 : (BL > 64 ? 128 : 64);
+  constexpr int TPB0 = BL > 128 ? 256 : (BL > 64 ? 128 : 64);
+  constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
   constexpr int BPB = 256 / TPB;
   __shared__ T sh[BPB * BL];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
@@ -1102,16 +1106,23 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
 template <int D, bool CYL, typename T, bool EPS>
 static bool launch_coarse_rhs_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc,
                                     int i_phi, int i_rhs, int i_tmp, int copy_tmp, int fused) {
+  const bool halve = nc <= 8 && (P.shape(kShapeCoarseRhs, D, nc, n) & 2);
   auto go = [&](auto kern, int bl) {
-    const int tpb = bl > 128 ? 256 : (bl > 64 ? 128 : 64);
+    const int tpb = (bl > 128 ? 256 : (bl > 64 ? 128 : 64)) >> (halve ? 1 : 0);
     const int bpb = 256 / tpb;
     hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
                        i_phi, i_rhs, i_tmp, copy_tmp, fused);
   };
   const int bl = D == 2 ? (nc + 2) * (nc + 2) : (nc + 2) * (nc + 2) * (nc + 2);
   switch (nc) {
-    case 4: go(k_coarse_rhs_lds<D, CYL, 4, T, EPS>, bl); return true;
-    case 8: go(k_coarse_rhs_lds<D, CYL, 8, T, EPS>, bl); return true;
+    case 4:
+      if (halve) go(k_coarse_rhs_lds<D, CYL, 4, T, EPS, true>, bl);
+      else go(k_coarse_rhs_lds<D, CYL, 4, T, EPS>, bl);
+      return true;
+    case 8:
+      if (halve) go(k_coarse_rhs_lds<D, CYL, 8, T, EPS, true>, bl);
+      else go(k_coarse_rhs_lds<D, CYL, 8, T, EPS>, bl);
+      return true;
     case 16: go(k_coarse_rhs_lds<D, CYL, 16, T, EPS>, bl); return true;
     case 32:
       if (D == 2) {
@@ -1195,13 +1206,14 @@ __global__ __launch_bounds__(256) void k_corr(Dev a, const int* __restrict__ lis
 // The same correction with the parent's region under the child staged in
 // LDS: (NC/2 + 2)^D values of the parent's tmp, ghosts included, read once
 // per child box instead of once per child cell.
-template <int D, int NC, typename T>
+template <int D, int NC, typename T, bool HALVE = false>
 __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__ list, int n,
                                                   int i_phi, int i_tmp) {
   constexpr int R = NC / 2 + 2;
   constexpr int RB = D == 2 ? R * R : R * R * R;
   constexpr int TOTAL = D == 2 ? NC * NC : NC * NC * NC;
-  constexpr int TPB = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int TPB0 = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
+  constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
   constexpr int BPB = 256 / TPB;
   constexpr int NIT = (TOTAL + TPB - 1) / TPB;
   constexpr int NCOL = TOTAL / 2;
@@ -1307,15 +1319,19 @@ __global__ __launch_bounds__(256) void k_corr_lds(Dev a, const int* __restrict__
 template <int D, typename T>
 static bool launch_corr_lds(Pool& P, const Dev& a, const int* list, int n, int nc, int i_phi,
                             int i_tmp) {
+  const bool halve = nc <= 8 && (P.shape(kShapeCorr, D, nc, n) & 2);
   auto go = [&](auto kern, int total) {
-    const int tpb = total > 128 ? 256 : (total > 64 ? 128 : 64);
+    const int tpb = (total > 128 ? 256 : (total > 64 ? 128 : 64)) >> (halve ? 1 : 0);
     const int bpb = 256 / tpb;
     hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
                        i_phi, i_tmp);
   };
   const int total = D == 2 ? nc * nc : nc * nc * nc;
   switch (nc) {
-    case 8: go(k_corr_lds<D, 8, T>, total); return true;
+    case 8:
+      if (halve) go(k_corr_lds<D, 8, T, true>, total);
+      else go(k_corr_lds<D, 8, T>, total);
+      return true;
     case 16: go(k_corr_lds<D, 16, T>, total); return true;
     case 32:
       if (D == 2) {
@@ -1555,6 +1571,11 @@ static void gc_list_t(Pool& P, const Dev& a, int lvl, const int* list, int n, in
   if (n == 0) return;
   const int nc = P.nc_lvl(lvl);
   Geo g = geo(n, 2 * D * (D == 2 ? nc : nc * nc));
+  if ((P.shape(kShapeGc, D, nc, n) & 2) && g.tpb > 16) {
+    // half the lanes per box, twice the boxes per workgroup
+    g.tpb /= 2;
+    g.grid = (n + 256 / g.tpb - 1) / (256 / g.tpb);
+  }
   const int pr = P.prof_begin(PK_GC, lvl, 0.0);
   hipLaunchKernelGGL((k_gc<D, T>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc, g.tpb,
                      iv, rb, bc);
@@ -1622,8 +1643,9 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
 template <int D, bool CYL, typename T, bool EPS>
 static bool launch_resid_rstr_lds_t(Pool& P, const Dev& a, const int* list, int n, int nc,
                                     int i_phi, int i_rhs, int i_tmp, int keep_res) {
+  const bool halve = nc <= 8 && (P.shape(kShapeResidRstr, D, nc, n) & 2);
   auto go = [&](auto kern, int total) {
-    const int tpb = total > 128 ? 256 : (total > 64 ? 128 : 64);
+    const int tpb = (total > 128 ? 256 : (total > 64 ? 128 : 64)) >> (halve ? 1 : 0);
     const int bpb = 256 / tpb;
     hipLaunchKernelGGL(kern, dim3((n + bpb - 1) / bpb), dim3(256), 0, P.stream, a, list, n,
                        i_phi, i_rhs, i_tmp, keep_res);
@@ -1631,9 +1653,18 @@ static bool launch_resid_rstr_lds_t(Pool& P, const Dev& a, const int* list, int 
   const int hnc = nc / 2;
   const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
   switch (nc) {
-    case 2: go(k_resid_rstr_lds<D, CYL, 2, T, EPS>, total); return true;
-    case 4: go(k_resid_rstr_lds<D, CYL, 4, T, EPS>, total); return true;
-    case 8: go(k_resid_rstr_lds<D, CYL, 8, T, EPS>, total); return true;
+    case 2:
+      if (halve) go(k_resid_rstr_lds<D, CYL, 2, T, EPS, true>, total);
+      else go(k_resid_rstr_lds<D, CYL, 2, T, EPS>, total);
+      return true;
+    case 4:
+      if (halve) go(k_resid_rstr_lds<D, CYL, 4, T, EPS, true>, total);
+      else go(k_resid_rstr_lds<D, CYL, 4, T, EPS>, total);
+      return true;
+    case 8:
+      if (halve) go(k_resid_rstr_lds<D, CYL, 8, T, EPS, true>, total);
+      else go(k_resid_rstr_lds<D, CYL, 8, T, EPS>, total);
+      return true;
     case 16: go(k_resid_rstr_lds<D, CYL, 16, T, EPS>, total); return true;
     case 32:
       if (D == 2) {
@@ -2806,69 +2837,117 @@ void tree_copy_cc(Pool& P, int from, int to) {
 }
 
 // ---------------------------------------------------------------------------
-// Launch-shape autotuner: for every (box size, level-size class) that the
-// tree's levels hold, each shape of the half-sweep menu is timed on those
-// levels (reps half-sweeps each after two untimed ones, on the potential and
-// rhs of the multigrid, with HIP events on the handle's stream) and the
-// fastest is kept when it beats the default by more than 3%. The potential is
-// saved to a device copy first and restored afterwards; every shape computes
-// the same values, so only the time differs. Rows of rec: kernel, dim, nc,
-// size class, shape, boxes timed, microseconds per half-sweep, 1 if chosen.
+// Launch-shape autotuner: for every kernel of the menu and every (box size,
+// level-size class) that the tree's levels hold, each shape is timed on
+// those levels (reps launches each after two untimed ones, on the
+// multigrid's potential, tmp and rhs, with HIP events on the handle's
+// stream), and the fastest is kept when it beats the default by more than
+// 3%. The half-sweep has four shapes; the ghost fill, the residual +
+// restriction, the coarse rhs and the correction have the default and half
+// the lanes per box (for boxes of 8 cells a side or fewer, where the default
+// leaves lanes idle). The three variables are saved to a device copy first
+// and restored afterwards; every shape computes the same values, so only
+// the time differs. Rows of rec: kernel, dim, nc, size class, shape, boxes
+// timed, microseconds per launch, 1 if chosen.
 // ---------------------------------------------------------------------------
 
 template <int D>
 static void autotune_t(Pool& P, int reps, std::vector<double>& rec) {
   const Ctx c = ctx64(P);
   const Dev a = dev_for(P, c);
-  // levels of each (log2 nc, size class)
-  std::vector<std::vector<int>> groups(6 * kSizeClasses);
-  for (int lvl = P.tree.min_lvl; lvl <= P.tree.max_lvl; lvl++) {
-    const int n = P.n_ids(lvl);
-    if (n > 0) groups[log2i(P.nc_lvl(lvl)) * kSizeClasses + size_class(n)].push_back(lvl);
-  }
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
-  for (int key = 0; key < (int)groups.size(); key++) {
-    if (groups[key].empty()) continue;
-    const int l2 = key / kSizeClasses, cls = key % kSizeClasses;
-    double t[kShapes];
-    int boxes = 0;
-    for (int sh = 0; sh < kShapes; sh++) {
-      P.shape_tab[0][D - 2][l2][cls] = (unsigned char)sh;
-      t[sh] = 0;
-      boxes = 0;
-      for (int lvl : groups[key]) {
-        const int n = P.n_ids(lvl), nc = P.nc_lvl(lvl);
-        boxes += n;
-        auto sweep = [&](int it) {
-          const bool ok = c.cyl ? launch_gsrb_lds<2, true, double>(P, a, P.ids(lvl), n, nc, it, 1,
-                                                                   c.i_phi, c.i_rhs)
-                                : launch_gsrb_lds<D, false, double>(P, a, P.ids(lvl), n, nc, it,
-                                                                    1, c.i_phi, c.i_rhs);
-          if (!ok) throw std::runtime_error("autotune: no half-sweep kernel for this box size");
-          post_launch("autotune");
-        };
-        sweep(1);
-        sweep(2);
-        check(hipEventRecord(e0, P.stream), "event");
-        for (int r = 0; r < reps; r++) sweep(r + 1);
-        check(hipEventRecord(e1, P.stream), "event");
-        check(hipEventSynchronize(e1), "event sync");
-        float ms = 0;
-        check(hipEventElapsedTime(&ms, e0, e1), "event time");
-        t[sh] += 1e3 * ms / reps;
+  const int min_lvl = P.tree.min_lvl;
+  for (int kern = 0; kern < kShapeKernels; kern++) {
+    // the boxes one launch of this kernel treats on level lvl (0: none)
+    auto n_of = [&](int lvl) {
+      const int nc = P.nc_lvl(lvl);
+      switch (kern) {
+        case kShapeGsrb: return P.n_ids(lvl);
+        case kShapeGc: return nc <= 8 ? P.n_ids(lvl) : 0;
+        case kShapeResidRstr: return lvl > min_lvl && nc <= 8 ? P.n_ids(lvl) : 0;
+        case kShapeCoarseRhs: return nc >= 4 && nc <= 8 ? P.n_parents(lvl) : 0;
+        default: return lvl > min_lvl && nc == 8 ? P.n_ids(lvl) : 0;
       }
+    };
+    auto launch = [&](int lvl, int it) {
+      const int n = n_of(lvl), nc = P.nc_lvl(lvl);
+      bool ok = true;
+      switch (kern) {
+        case kShapeGsrb:
+          ok = c.cyl ? launch_gsrb_lds<2, true, double>(P, a, P.ids(lvl), n, nc, it, 1, c.i_phi,
+                                                        c.i_rhs)
+                     : launch_gsrb_lds<D, false, double>(P, a, P.ids(lvl), n, nc, it, 1,
+                                                         c.i_phi, c.i_rhs);
+          break;
+        case kShapeGc:
+          gc_list_t<D, double>(P, a, lvl, P.ids(lvl), n, c.i_phi, 0, 0);
+          break;
+        case kShapeResidRstr:
+          ok = c.cyl ? launch_resid_rstr_lds<2, true, double>(P, a, P.ids(lvl), n, nc, c.i_phi,
+                                                              c.i_rhs, c.i_tmp, 0)
+                     : launch_resid_rstr_lds<D, false, double>(P, a, P.ids(lvl), n, nc,
+                                                               c.i_phi, c.i_rhs, c.i_tmp, 0);
+          break;
+        case kShapeCoarseRhs:
+          if (c.cyl)
+            ok = c.eps ? launch_coarse_rhs_lds_t<2, true, double, true>(
+                             P, a, P.parents(lvl), n, nc, c.i_phi, c.i_rhs, c.i_tmp, 1, 1)
+                       : launch_coarse_rhs_lds_t<2, true, double, false>(
+                             P, a, P.parents(lvl), n, nc, c.i_phi, c.i_rhs, c.i_tmp, 1, 1);
+          else
+            ok = c.eps ? launch_coarse_rhs_lds_t<D, false, double, true>(
+                             P, a, P.parents(lvl), n, nc, c.i_phi, c.i_rhs, c.i_tmp, 1, 1)
+                       : launch_coarse_rhs_lds_t<D, false, double, false>(
+                             P, a, P.parents(lvl), n, nc, c.i_phi, c.i_rhs, c.i_tmp, 1, 1);
+          break;
+        default:
+          ok = launch_corr_lds<D, double>(P, a, P.ids(lvl), n, nc, c.i_phi, c.i_tmp);
+      }
+      if (!ok) throw std::runtime_error("autotune: no kernel for this box size");
+      post_launch("autotune");
+    };
+    std::vector<std::vector<int>> groups(6 * kSizeClasses);
+    for (int lvl = min_lvl; lvl <= P.tree.max_lvl; lvl++) {
+      const int n = n_of(lvl);
+      if (n > 0) groups[log2i(P.nc_lvl(lvl)) * kSizeClasses + size_class(n)].push_back(lvl);
     }
-    int best = 0;
-    for (int sh = 1; sh < kShapes; sh++)
-      if (t[sh] < t[best]) best = sh;
-    if (!(t[best] < 0.97 * t[0])) best = 0;
-    P.shape_tab[0][D - 2][l2][cls] = (unsigned char)best;
-    for (int sh = 0; sh < kShapes; sh++) {
-      const double row[8] = {0, (double)D, (double)(1 << l2), (double)cls, (double)sh,
-                             (double)boxes, t[sh], sh == best ? 1.0 : 0.0};
-      rec.insert(rec.end(), row, row + 8);
+    const int shapes[4] = {0, 2, 1, 3};
+    const int n_shapes = kern == kShapeGsrb ? 4 : 2;
+    for (int key = 0; key < (int)groups.size(); key++) {
+      if (groups[key].empty()) continue;
+      const int l2 = key / kSizeClasses, cls = key % kSizeClasses;
+      double t[kShapes] = {0, 0, 0, 0};
+      int boxes = 0;
+      for (int si = 0; si < n_shapes; si++) {
+        const int sh = shapes[si];
+        P.shape_tab[kern][D - 2][l2][cls] = (unsigned char)sh;
+        boxes = 0;
+        for (int lvl : groups[key]) {
+          boxes += n_of(lvl);
+          launch(lvl, 1);
+          launch(lvl, 2);
+          check(hipEventRecord(e0, P.stream), "event");
+          for (int r = 0; r < reps; r++) launch(lvl, r + 1);
+          check(hipEventRecord(e1, P.stream), "event");
+          check(hipEventSynchronize(e1), "event sync");
+          float ms = 0;
+          check(hipEventElapsedTime(&ms, e0, e1), "event time");
+          t[sh] += 1e3 * ms / reps;
+        }
+      }
+      int best = 0;
+      for (int si = 1; si < n_shapes; si++)
+        if (t[shapes[si]] < t[best]) best = shapes[si];
+      if (!(t[best] < 0.97 * t[0])) best = 0;
+      P.shape_tab[kern][D - 2][l2][cls] = (unsigned char)best;
+      for (int si = 0; si < n_shapes; si++) {
+        const int sh = shapes[si];
+        const double row[8] = {(double)kern, (double)D, (double)(1 << l2), (double)cls,
+                               (double)sh, (double)boxes, t[sh], sh == best ? 1.0 : 0.0};
+        rec.insert(rec.end(), row, row + 8);
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -2876,29 +2955,36 @@ static void autotune_t(Pool& P, int reps, std::vector<double>& rec) {
 }
 
 void autotune(Pool& P, int reps, std::vector<double>& rec) {
-  if (P.mg.i_phi <= 0 || P.mg.i_rhs <= 0) throw std::runtime_error("autotune: multigrid not set up");
+  if (P.mg.i_phi <= 0 || P.mg.i_rhs <= 0 || P.mg.i_tmp <= 0)
+    throw std::runtime_error("autotune: multigrid not set up");
   if (reps < 1) throw std::invalid_argument("autotune: reps must be >= 1");
-  P.sync_topology();
+  mg_check(P);  // topology uploaded, operator tags set
   // captured graphs hold the old shapes
   P.clear_graphs();
   const Dev a = P.dev();
-  double* phi = a.cc + (size_t)(P.mg.i_phi - 1) * a.vs;
+  const int ivs[3] = {P.mg.i_phi, P.mg.i_tmp, P.mg.i_rhs};
   double* save = nullptr;
-  check(hipMalloc((void**)&save, a.vs * sizeof(double)), "hipMalloc");
-  check(hipMemcpyAsync(save, phi, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream),
-        "save");
+  check(hipMalloc((void**)&save, 3 * a.vs * sizeof(double)), "hipMalloc");
+  auto copy_all = [&](bool back) {
+    for (int k = 0; k < 3; k++) {
+      double* v = a.cc + (size_t)(ivs[k] - 1) * a.vs;
+      double* sv = save + (size_t)k * a.vs;
+      const hipError_t e = hipMemcpyAsync(back ? v : sv, back ? sv : v, a.vs * sizeof(double),
+                                          hipMemcpyDeviceToDevice, P.stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(P.stream);
+  };
+  check(copy_all(false), "save");
   try {
     if (P.tree.dim == 2) autotune_t<2>(P, reps, rec);
     else autotune_t<3>(P, reps, rec);
   } catch (...) {
-    (void)hipMemcpyAsync(phi, save, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream);
-    (void)hipStreamSynchronize(P.stream);
+    (void)copy_all(true);
     (void)hipFree(save);
     throw;
   }
-  check(hipMemcpyAsync(phi, save, a.vs * sizeof(double), hipMemcpyDeviceToDevice, P.stream),
-        "restore");
-  check(hipStreamSynchronize(P.stream), "restore sync");
+  check(copy_all(true), "restore");
   check(hipFree(save), "hipFree");
 }
 

@@ -106,7 +106,11 @@ constexpr int kMaxTerms = 4;
 // Launch shapes of the box kernels (a5g_set_shape, a5g_autotune): per
 // kernel, dimension, box size (log2 of the cells a side) and level-size
 // class, an index into the kernel's menu (0: the default shape).
-constexpr int kShapeKernels = 1;  // 0: multigrid half-sweep
+// kernels: the multigrid half-sweep (all four shapes), the ghost fill, the
+// residual + restriction, the coarse rhs and the correction (shapes 0 and 2)
+constexpr int kShapeGsrb = 0, kShapeGc = 1, kShapeResidRstr = 2, kShapeCoarseRhs = 3,
+              kShapeCorr = 4;
+constexpr int kShapeKernels = 5;
 constexpr int kSizeClasses = 3;   // levels of < 1024, < 16384 and more boxes
 constexpr int kShapes = 4;        // bit 0: persistent grid, bit 1: half the lanes per box
 inline int size_class(int n) { return n < 1024 ? 0 : (n < 16384 ? 1 : 2); }

@@ -219,9 +219,9 @@ class Tree:
 
     def autotune(self, reps=20):
         """Launch-shape autotuner (a5g_autotune): a list of dicts, one per
-        (box size, size class, shape) timed, SHAPE_FIELDS as keys."""
+        (kernel, box size, size class, shape) timed, SHAPE_FIELDS as keys."""
         n = C.c_int()
-        rec = np.zeros(8 * 6 * 3 * 4)
+        rec = np.zeros(8 * 5 * 6 * 3 * 4)
         _call("a5g_autotune", self.h, C.c_int(reps), C.c_int(len(rec) // 8), _d(rec)[1],
               C.byref(n))
         out = []

@@ -289,12 +289,16 @@ STEP_KINDS = {0: "gsrb_half_sweep", 1: "ghost_cells", 2: "residual_restrict",
               12: "average", 13: "restrict", 14: "photoi_rate"}
 
 
+TUNE_KERNELS = ("gsrb", "gc", "resid_rstr", "coarse_rhs", "corr")
+
+
 def tune_summary(rec):
-    """The autotuner's rows, per (box size, size class): the shape chosen and
-    the microseconds per half-sweep of shapes 0..3 over the boxes timed."""
+    """The autotuner's rows, per (kernel, box size, size class): the shape
+    chosen and the microseconds per launch of shapes 0..3 (None: not on that
+    kernel's menu) over the boxes timed."""
     out = {}
     for r in rec:
-        k = "nc%d_class%d" % (r["nc"], r["size_class"])
+        k = "%s_nc%d_class%d" % (TUNE_KERNELS[r["kernel"]], r["nc"], r["size_class"])
         e = out.setdefault(k, {"boxes": r["boxes"], "us": [None] * 4, "chosen": 0})
         e["us"][r["shape"]] = round(r["us"], 2)
         if r["chosen"]:

@@ -100,21 +100,24 @@ int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
 /* Launch shapes of the box kernels (no reference counterpart: the
  * reference's loops over boxes have no launch geometry). kernel 0 is the
- * multigrid half-sweep; for boxes of nc cells a side (2..32) on levels of
- * size class cls (0: < 1024 boxes, 1: < 16384, 2: more), shape 0 is the
- * default (lanes per box from the cells of one colour, 256-lane
- * workgroups, one workgroup per group of boxes), 1 the same on a persistent
- * grid (8 workgroups per compute unit walking the list), 2 half the lanes
- * per box and twice the boxes per workgroup, 3 that on a persistent grid.
- * Every shape computes the same values. */
+ * multigrid half-sweep, 1 the ghost fill, 2 the residual + restriction, 3
+ * the coarse rhs, 4 the correction; for boxes of nc cells a side (2..32) on
+ * levels of size class cls (0: < 1024 boxes, 1: < 16384, 2: more), shape 0
+ * is the default (lanes per box from the cells the kernel treats, 256-lane
+ * workgroups, one workgroup per group of boxes), 2 half the lanes per box
+ * and twice the boxes per workgroup (kernels 1-4: boxes of up to 8 cells a
+ * side); for the half-sweep also 1, the default on a persistent grid (8
+ * workgroups per compute unit walking the list), and 3, shape 2 on a
+ * persistent grid. Every shape computes the same values. */
 int a5g_set_shape(a5g_tree *t, int kernel, int nc, int cls, int shape);
 int a5g_get_shape(a5g_tree *t, int kernel, int nc, int cls, int *shape);
-/* Autotuner: times every shape of the half-sweep on the tree's current
- * levels (reps half-sweeps per shape and level on the multigrid's
- * potential, which is restored afterwards) and keeps, per box size and size
- * class present, the fastest when it beats the default by more than 3%.
+/* Autotuner: times every shape of every kernel above on the tree's current
+ * levels (reps launches per shape and level on the multigrid's potential,
+ * tmp and rhs, which are restored afterwards) and keeps, per kernel, box
+ * size and size class present, the fastest when it beats the default by
+ * more than 3%.
  * Writes up to max_rec rows of 8 doubles (kernel, dim, nc, size class,
- * shape, boxes timed, microseconds per half-sweep, 1 if chosen) and their
+ * shape, boxes timed, microseconds per launch, 1 if chosen) and their
  * count to *n_rec. */
 int a5g_autotune(a5g_tree *t, int reps, int max_rec, double *rec, int *n_rec);
 /* Mixed-precision solve of the same problem to max|r| <= tol max|rhs| over

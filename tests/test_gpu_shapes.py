@@ -1,7 +1,8 @@
-"""Launch shapes of the multigrid half-sweep (a5g_set_shape) and the launch
+"""Launch shapes of the multigrid kernels (a5g_set_shape) and the launch
 autotuner (a5g_autotune). Every shape (default; persistent grid; half the
-lanes per box and twice the boxes per workgroup; both) computes the same
-doubles: an FMG and a V-cycle with one shape forced everywhere leave every
+lanes per box and twice the boxes per workgroup; both; for the ghost fill,
+residual + restriction, coarse rhs and correction the default and half the
+lanes) computes the same doubles: an FMG and a V-cycle with one shape forced everywhere leave every
 variable of every box, ghosts included, equal to the default's. The first
 and third trees hold levels of more workgroup slots (32,768 boxes of 8^2,
 4,096 boxes of 8^3) than a persistent grid has workgroups (2,048 on 256
@@ -46,9 +47,10 @@ def _tree(dim, nc, coord, base, lvl_u, eps):
 
 
 def _force(T, shape):
-    for nc in (2, 4, 8, 16, 32):
-        for cls in range(3):
-            T.set_shape(0, nc, cls, shape)
+    for kernel in range(5):
+        for nc in (2, 4, 8, 16, 32):
+            for cls in range(3):
+                T.set_shape(kernel, nc, cls, shape)
 
 
 def _solve(T, start):
@@ -83,14 +85,15 @@ def test_autotune_keeps_the_state_and_reports_its_choice():
     rec = T.autotune(reps=3)
     for iv, v in before.items():
         np.testing.assert_array_equal(T.get_cc_all(iv), v, err_msg="var %d" % iv)
-    keys = {(r["nc"], r["size_class"]) for r in rec}
-    assert (8, 2) in keys
+    keys = {(r["kernel"], r["nc"], r["size_class"]) for r in rec}
+    # the finest level (32,768 boxes) is of class 2; its parents (8,192) of 1
+    assert {(0, 8, 2), (1, 8, 2), (2, 8, 2), (3, 8, 1), (4, 8, 2)} <= keys
     for k in keys:
-        rows = [r for r in rec if (r["nc"], r["size_class"]) == k]
-        assert sorted(r["shape"] for r in rows) == [0, 1, 2, 3]
+        rows = [r for r in rec if (r["kernel"], r["nc"], r["size_class"]) == k]
+        assert sorted(r["shape"] for r in rows) == ([0, 1, 2, 3] if k[0] == 0 else [0, 2])
         chosen = [r for r in rows if r["chosen"]]
         assert len(chosen) == 1 and all(r["us"] > 0 for r in rows)
-        assert T.get_shape(0, k[0], k[1]) == chosen[0]["shape"]
+        assert T.get_shape(*k) == chosen[0]["shape"]
         best = min(r["us"] for r in rows)
         default = [r for r in rows if r["shape"] == 0][0]["us"]
         assert chosen[0]["us"] == best or chosen[0]["shape"] == 0

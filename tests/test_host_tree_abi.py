@@ -118,11 +118,13 @@ def test_shape_arguments_are_checked():
     unknown kernel, box size, size class or shape."""
     import a5_streamer_amd as A
     T = A.Tree(2, 8, 3, 0, 1.0 / 8)
-    for args in ((1, 8, 0, 0), (0, 6, 0, 0), (0, 8, 3, 0), (0, 8, 0, 4)):
+    for args in ((5, 8, 0, 0), (-1, 8, 0, 0), (0, 6, 0, 0), (0, 8, 3, 0), (0, 8, 0, 4)):
         with pytest.raises(Exception):
             T.set_shape(*args)
     assert T.get_shape(0, 8, 2) == 0
     T.set_shape(0, 8, 2, 3)
     assert T.get_shape(0, 8, 2) == 3
     assert T.get_shape(0, 16, 2) == 0
+    T.set_shape(4, 8, 1, 2)
+    assert T.get_shape(4, 8, 1) == 2 and T.get_shape(3, 8, 1) == 0
     T.close()
