@@ -150,6 +150,66 @@ def cpu_baseline(W, n_v, small_W=None, n_v_small=None):
                       "level-4 sub-volume (%d leaf cells), %d V-cycles, median of 3 "
                       "(%s s) on one thread" % (cs, n_v_small,
                                                 ", ".join("%.2f" % t for t in ts))}
+    out["drivers"] = driver_baselines(R, threads)
+    return out
+
+
+# The reference's own driver programs (built as modules into oracle/_ref)
+# on the physics of the 2D workload and of the 3D production streamer, each
+# on its own single base box and refinement; rates over the steps between
+# two end times, so that initialisation cancels
+DRIVER_BASELINES = {
+    "2d_air_streamer": (dict(variant="2d", n_base=1, lvls_max=30), 20e-9, 40e-9),
+    "3d_streamer_first_steps": (dict(variant="3d", n_base=1, lvls_max=30, domain_len=64e-3,
+                                     applied_fld=2.0e6, seed_r0=[[32e-3, 32e-3, 56e-3]],
+                                     seed_r1=[[32e-3, 32e-3, 56e-3]], n_boxes=20000),
+                                5e-9, 12e-9),
+}
+
+
+def _ref_driver_pair(R, cfg, end1, end2):
+    """(leaf cell-steps, seconds, steps) of the reference driver's steps
+    between end1 and end2: two runs from t = 0, the difference of their
+    step counters and wall times."""
+    res = []
+    for end in (end1, end2):
+        D = R.RefDriver(cfg.variant)
+        D.config(cfg.domain_len, cfg.box_size, cfg.applied_fld, dt_max=cfg.dt_max,
+                 end_time=end, epsilon_diel=cfg.epsilon_diel, steps_amr=cfg.steps_amr)
+        D.init_cond(cfg.bg_dens, cfg.seed_r0, cfg.seed_r1, cfg.seed_dens, cfg.seed_width,
+                    cfg.seed_falloff)
+        x, cols = cfg.transport_columns()
+        D.table(cfg.table_size, cfg.table_max_fld, x, cols)
+        D.counters(reset=True)
+        t0 = time.perf_counter()
+        D.run(cfg.box_size)
+        w = time.perf_counter() - t0
+        n, c = D.counters()
+        res.append((c, w, n))
+    return res[1][0] - res[0][0], res[1][1] - res[0][1], res[1][2] - res[0][2]
+
+
+def driver_baselines(R, threads):
+    """Mcell-steps/s of the reference driver programs on this host's cores:
+    on every usable core the median of 3 step-range measurements, on one
+    core one (a bounded sample)."""
+    from a5_streamer_amd import streamer as ST
+    out = {}
+    for name, (kw, end1, end2) in DRIVER_BASELINES.items():
+        cfg = ST.Config(**kw)
+        row = {}
+        for thr, reps in ((threads, 3), (1, 1)):
+            R.set_threads(thr)
+            runs = [_ref_driver_pair(R, cfg, end1, end2) for _ in range(reps)]
+            rates = [c / w / 1e6 for c, w, _ in runs]
+            k = int(np.argsort(rates)[len(rates) // 2])
+            c, w, n = runs[k]
+            row["all_cores" if thr == threads else "one_thread"] = {
+                "value": rates[k], "unit": "Mcell-steps/s", "cores": thr,
+                "sample": "steps %d after t = %g s, %.3g leaf cell-steps in %.2f s%s" % (
+                    n, end1, c, w, ", median of %d" % reps if reps > 1 else "")}
+        out[name] = row
+    R.set_threads(threads)
     return out
 
 

@@ -16,8 +16,12 @@
 #    `call aD_destroy(tree)` becomes the module procedure drv_main, placed
 #    first after `contains`. The lines before init_tree read the config
 #    files and the transport data; the harness sets those module variables
-#    (ST_*, ST_td_tbl, ST_init_cond) itself before it calls drv_main.
-BEGIN { state = 0; skipping = 0; nbuf = 0 }
+#    (ST_*, ST_td_tbl, ST_init_cond) itself before it calls drv_main;
+#  - two counters for the CPU baseline: at the start of every time step
+#    (the line after `do n = 1, ST_steps_amr`) drv_steps is incremented and
+#    the step's leaf cells (leaf boxes of levels >= 1 times box_size^dim)
+#    are added to drv_cells. They read the tree only.
+BEGIN { state = 0; skipping = 0; nbuf = 0; dim = (name == "3d") ? 3 : 2 }
 
 function cont(line) { return line ~ /&[ \t]*$/ }
 
@@ -39,15 +43,32 @@ function cont(line) { return line ~ /&[ \t]*$/ }
     if (line ~ /^[ \t]*call a[23]_destroy\(tree\)/) { state = 3; next }
     if (line ~ /write_silo|write_streamer_properties/) { skipping = cont(line); next }
     buf[++nbuf] = line
+    if (line ~ /^[ \t]*do n = 1, ST_steps_amr[ \t]*$/) {
+      buf[++nbuf] = "        drv_steps = drv_steps + 1"
+      buf[++nbuf] = "        drv_cells = drv_cells + drv_leaf_cells()"
+      ncount++
+    }
     next
   }
   if (state == 3) {
     if (line ~ /^contains[ \t]*$/) {
+      print "  integer :: drv_steps = 0"
+      print "  real(kind(0.0d0)) :: drv_cells = 0"
       print "contains"
       print ""
       print "  subroutine drv_main()"
       for (k = 1; k <= nbuf; k++) print buf[k]
       print "  end subroutine drv_main"
+      print ""
+      print "  function drv_leaf_cells() result(c)"
+      print "    real(kind(0.0d0)) :: c"
+      print "    integer :: l"
+      print "    c = 0"
+      print "    do l = 1, tree%max_lvl"
+      print "       c = c + size(tree%lvls(l)%leaves)"
+      print "    end do"
+      print "    c = c * real(ST_box_size, kind(0.0d0))**" dim
+      print "  end function drv_leaf_cells"
       state = 4
     }
     next
@@ -57,7 +78,7 @@ function cont(line) { return line ~ /&[ \t]*$/ }
 }
 
 END {
-  if (state != 4) {
+  if (state != 4 || ncount != 1) {
     print "drv2mod.awk: " FILENAME " does not have the expected program layout" > "/dev/stderr"
     exit 1
   }

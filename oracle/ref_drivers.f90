@@ -28,19 +28,22 @@ module ref_drivers
        d2_update_solution => update_solution, d2_average_dens => average_dens, &
        d2_photoi_rate => set_photoi_rate, d2_ref_flags => set_ref_flags, &
        d2_max_dt => get_max_dt, d2_init_cond => set_init_cond, &
-       d2_compute_fld => compute_fld, d2_sides_bc => sides_bc_pot
+       d2_compute_fld => compute_fld, d2_sides_bc => sides_bc_pot, &
+       d2_steps => drv_steps, d2_cells => drv_cells
   use ref_drv_3d, only: d3_main => drv_main, d3_tree => tree, d3_mg => mg, &
        d3_fld_from_pot => fld_from_pot, d3_fluxes_koren => fluxes_koren, &
        d3_update_solution => update_solution, d3_average_dens => average_dens, &
        d3_photoi_rate => set_photoi_rate, d3_ref_flags => set_ref_flags, &
        d3_max_dt => get_max_dt, d3_init_cond => set_init_cond, &
-       d3_compute_fld => compute_fld, d3_sides_bc => sides_bc_pot
+       d3_compute_fld => compute_fld, d3_sides_bc => sides_bc_pot, &
+       d3_steps => drv_steps, d3_cells => drv_cells
   use ref_drv_cyl, only: dc_main => drv_main, dc_tree => tree, dc_mg => mg, &
        dc_fld_from_pot => fld_from_pot, dc_fluxes_koren => fluxes_koren, &
        dc_update_solution => update_solution, dc_average_dens => average_dens, &
        dc_photoi_rate => set_photoi_rate, dc_ref_flags => set_ref_flags, &
        dc_max_dt => get_max_dt, dc_init_cond => set_init_cond, &
-       dc_compute_fld => compute_fld, dc_sides_bc => sides_bc_pot
+       dc_compute_fld => compute_fld, dc_sides_bc => sides_bc_pot, &
+       dc_steps => drv_steps, dc_cells => drv_cells
   implicit none
   private
 
@@ -141,6 +144,38 @@ contains
        h2 = dc_tree
     end select
   end subroutine h_run
+
+  ! The time steps and leaf cell-steps (leaf cells summed over the steps)
+  ! that the driver's main loop has run since the last reset; reset = 1 sets
+  ! them to zero after reading
+  subroutine h_counters(variant, steps, cells, reset) bind(C, name="refdrv_counters")
+    integer(c_int), value :: variant, reset
+    integer(c_int)        :: steps
+    real(c_double)        :: cells
+    select case (variant)
+    case (1)
+       steps = d2_steps
+       cells = d2_cells
+       if (reset /= 0) then
+          d2_steps = 0
+          d2_cells = 0
+       end if
+    case (2)
+       steps = d3_steps
+       cells = d3_cells
+       if (reset /= 0) then
+          d3_steps = 0
+          d3_cells = 0
+       end if
+    case (3)
+       steps = dc_steps
+       cells = dc_cells
+       if (reset /= 0) then
+          dc_steps = 0
+          dc_cells = 0
+       end if
+    end select
+  end subroutine h_counters
 
   ! The multigrid set-up of the driver's main program (src/streamer_2d.f90:
   ! 36-52, src/streamer_3d.f90:36-52, src/streamer_cyl.f90:39-55), for calls

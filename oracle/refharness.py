@@ -374,6 +374,13 @@ class RefDriver:
         self.L.refdrv_run(C.c_int(self.vid))
         return RefTree.view(self.dim, n_cell)
 
+    def counters(self, reset=True):
+        """(time steps, leaf cell-steps) the driver's main loop has run
+        since the last reset."""
+        n, c = C.c_int(), C.c_double()
+        self.L.refdrv_counters(C.c_int(self.vid), C.byref(n), C.byref(c), C.c_int(int(reset)))
+        return n.value, c.value
+
     def mg_init(self):
         self.L.refdrv_mg_init(C.c_int(self.vid))
 

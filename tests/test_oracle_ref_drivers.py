@@ -117,14 +117,19 @@ def test_run_matches_reference_driver(ref, name):
     cfg = S.Config(**CFGS[name])
     end = END_TIME[name]
     D = _driver(ref, cfg, end_time=end)
+    D.counters(reset=True)
     RT = D.run(cfg.box_size)
     r = S.StreamerRun(O.OStreamTree, cfg)
     r.initialise()
     blocks = 0
+    cell_steps = 0
     while r.time <= end:
+        cell_steps += r.leaf_cells() * cfg.steps_amr
         r.step_block()
         blocks += 1
     assert blocks >= 2
+    # the step and leaf-cell counters of the CPU baseline
+    assert D.counters() == (blocks * cfg.steps_amr, float(cell_steps))
     t_ref, dt_ref = D.time()
     assert r.time == t_ref
     # the reference computed the next time step before leaving its loop
