@@ -864,6 +864,16 @@ int a5g_fas_vcycle(a5g_tree* t, int max_lvl, int set_residual) {
   API_END
 }
 
+int a5g_fas_fmg_adjoint(a5g_tree* t, const int* adj, double* vbar) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!adj || !vbar) throw std::invalid_argument("null argument");
+  a5g::fmg_adjoint(t->P, adj, vbar);
+  API_END
+}
+
 int a5g_set_shape(a5g_tree* t, int kernel, int nc, int cls, int shape) {
   API_BEGIN
   NEED(t);

@@ -2988,4 +2988,462 @@ void autotune(Pool& P, int reps, std::vector<double>& rec) {
   check(hipFree(save), "hipFree");
 }
 
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the FAS multigrid (a5g_fas_fmg_adjoint). mg$D_fas_fmg
+// (m_aX_mg.f90:164-203) with first_call false and without the residual is an
+// affine map of the potential, tmp and rhs it starts from and of the
+// boundary values V of the rule 2V - c (every operation is linear in them:
+// the Gauss-Seidel half-sweeps, the ghost rules, residual, restriction,
+// coarse rhs, correction, copies). Its adjoint is the transpose of each
+// operation, applied in the reverse order of the reference's schedule
+// (not of this library's fused one, which computes the same values), and
+// needs no state of the forward solve. Adjoint variables: lp (potential),
+// lt (tmp), lr (rhs), lw (work: the residual restricted by update_coarse);
+// the adjoints of V go to vbar (one per face). Normal boxes (no permittivity
+// tags) and the Laplacian only. The transposes scatter with atomics, so the
+// sums arrive in an order of their own.
+// ---------------------------------------------------------------------------
+
+// a$D_gc_box's rules on one box (m_aX_gc.f90:50-68, with mg$D_sides_rb and
+// the physical rules), transposed: faces in the reverse order, every face
+// ghost's adjoint sent to the values it was made from and cleared.
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_gc(Dev a, const int* __restrict__ list, int n,
+                                                int nc, int il, double* __restrict__ vbar) {
+  if ((int)blockIdx.x >= n) return;
+  const int id = list[blockIdx.x];
+  double* L = boxp(a, il, id);
+  const int per_face = D == 2 ? nc : nc * nc;
+  for (int nb = 2 * D; nb >= 1; nb--) {
+    const int d = d_nb_dim(nb);
+    const bool low = d_nb_low(nb);
+    const int gix = low ? 0 : nc + 1, cix = low ? 1 : nc, dix = low ? 1 : -1;
+    const int nb_id = a.nbs[id * 6 + nb - 1];
+    if (D == 2 && nb_id < 0 && d == 1 && a.bc_kind[nb - 1] == kBcRow2V) {
+      // the corner ghosts of the whole-row rule, written last from the x
+      // ghosts of this pass: a(gx, gy) = 2V - a(gx, cy)
+      if (threadIdx.x < 2) {
+        const int gx = threadIdx.x == 0 ? 0 : nc + 1;
+        double* g = L + cidx<D>(nc, gx, gix, 0);
+        const double gb = *g;
+        *g = 0;
+        atomicAdd(L + cidx<D>(nc, gx, cix, 0), -gb);
+        atomicAdd(vbar + nb - 1, 2 * gb);
+      }
+      __syncthreads();
+    }
+    for (int q = threadIdx.x; q < per_face; q += blockDim.x) {
+      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int g[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+      g[d] = gix;
+      c[d] = cix;
+      g[ta] = c[ta] = q % nc + 1;
+      if (D == 3) g[tb] = c[tb] = q / nc + 1;
+      double* gp = L + cidx<D>(nc, g[0], g[1], g[2]);
+      const double gb = *gp;
+      *gp = 0;
+      if (gb == 0) continue;
+      if (nb_id > 0) {
+        // sides_from_nb (m_aX_gc.f90:368-402): the neighbour's adjacent layer
+        int s[3] = {c[0], c[1], c[2]};
+        s[d] = low ? nc : 1;
+        atomicAdd(boxp(a, il, nb_id) + cidx<D>(nc, s[0], s[1], s[2]), gb);
+      } else if (nb_id == 0) {
+        // mg$D_sides_rb (m_aX_mg.f90:262-356): 0.5 times the parent's cell
+        // (a$D_gc_prolong0) plus an extrapolation from the box's own cells
+        const int pid = a.parent[id];
+        const int pnc = a.ncell[pid];
+        int off[3];
+        child_off<D>(a, id, nc, 0, off);
+        int pc[3] = {0, 0, 0};
+        for (int e = 0; e < D; e++) pc[e] = off[e] + ((g[e] + 1) >> 1);
+        atomicAdd(boxp(a, il, pid) + cidx<D>(pnc, pc[0], pc[1], pc[2]), 0.5 * gb);
+        atomicAdd(L + cidx<D>(nc, c[0], c[1], c[2]), (D == 2 ? 1.0 : 1.25) * gb);
+        int e1[3] = {c[0], c[1], c[2]};
+        e1[d] += dix;
+        atomicAdd(L + cidx<D>(nc, e1[0], e1[1], e1[2]), -0.25 * gb);
+        int e2[3] = {c[0], c[1], c[2]};
+        e2[ta] += -1 + 2 * (c[ta] & 1);
+        atomicAdd(L + cidx<D>(nc, e2[0], e2[1], e2[2]), -0.25 * gb);
+        if (D == 3) {
+          int e3[3] = {c[0], c[1], c[2]};
+          e3[tb] += -1 + 2 * (c[tb] & 1);
+          atomicAdd(L + cidx<D>(nc, e3[0], e3[1], e3[2]), -0.25 * gb);
+        }
+      } else {
+        // the drivers' rules: copy, -c, 2V - c
+        const int kind = a.bc_kind[nb - 1];
+        double* cp = L + cidx<D>(nc, c[0], c[1], c[2]);
+        if (kind == kBcCopy) {
+          atomicAdd(cp, gb);
+        } else if (kind == kBcNeg) {
+          atomicAdd(cp, -gb);
+        } else {
+          atomicAdd(cp, -gb);
+          atomicAdd(vbar + nb - 1, 2 * gb);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One half-sweep (mg$D_box_gsrb_lpl, m_aX_mg.f90:769-811; mg2_box_gsrb_clpl,
+// :1413-1439) transposed: each updated cell's adjoint goes to its
+// neighbours and to its rhs, and is cleared
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__ list, int n,
+                                                  int nc, int cntr, int tpb, int il, int ir) {
+  BOX_PROLOGUE
+  double* L = boxp(a, il, id);
+  double* R = boxp(a, ir, id);
+  const double dx2 = a.dr[id] * a.dr[id];
+  const int half = nc >> 1;
+  const int ncol = (D == 2 ? nc * nc : nc * nc * nc) >> 1;
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+  const double w = D == 2 ? 0.25 : 1.0 / 6.0;
+  for (int q = lt; q < ncol; q += tpb) {
+    const int m = q % half, row = q / half;
+    const int j = row % nc + 1;
+    const int k = D == 3 ? row / nc + 1 : 0;
+    const int i = 2 - ((cntr ^ (D == 3 ? k + j : j)) & 1) + 2 * m;
+    double* yp = L + cidx<D>(nc, i, j, k);
+    const double yb = *yp;
+    *yp = 0;
+    if (yb == 0) continue;
+    double wxm = w, wxp = w;
+    if (CYL) {
+      const double den = i + ioff - 0.5;
+      wxm = 0.25 * ((double)(i + ioff - 1) / den);
+      wxp = 0.25 * ((double)(i + ioff) / den);
+    }
+    atomicAdd(L + cidx<D>(nc, i - 1, j, k), wxm * yb);
+    atomicAdd(L + cidx<D>(nc, i + 1, j, k), wxp * yb);
+    atomicAdd(L + cidx<D>(nc, i, j - 1, k), w * yb);
+    atomicAdd(L + cidx<D>(nc, i, j + 1, k), w * yb);
+    if (D == 3) {
+      atomicAdd(L + cidx<D>(nc, i, j, k - 1), w * yb);
+      atomicAdd(L + cidx<D>(nc, i, j, k + 1), w * yb);
+    }
+    R[cidx<D>(nc, i, j, k)] += -w * dx2 * yb;
+  }
+}
+
+// dst += s A^T src for the box operator A (mg$D_box_lpl, m_aX_mg.f90:814-848;
+// mg2_box_clpl, :1442-1464): src on the interior; then src = 0 on the
+// interior when clear_src
+template <int D, bool CYL>
+__global__ __launch_bounds__(256) void k_adj_op(Dev a, const int* __restrict__ list, int n,
+                                                int nc, int tpb, int isrc, double sgn, int idst,
+                                                int clear_src) {
+  BOX_PROLOGUE
+  double* S = boxp(a, isrc, id);
+  double* Dp = boxp(a, idst, id);
+  const double inv = 1 / (a.dr[id] * a.dr[id]);
+  const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c0 = cidx<D>(nc, i, j, k);
+    const double sb = sgn * S[c0];
+    if (clear_src) S[c0] = 0;
+    if (sb == 0) continue;
+    double wxm = inv, wxp = inv;
+    if (CYL) {
+      const double den = i + ioff - 0.5;
+      wxm = ((double)(i + ioff - 1) / den) * inv;
+      wxp = ((double)(i + ioff) / den) * inv;
+    }
+    atomicAdd(Dp + cidx<D>(nc, i - 1, j, k), wxm * sb);
+    atomicAdd(Dp + cidx<D>(nc, i + 1, j, k), wxp * sb);
+    atomicAdd(Dp + cidx<D>(nc, i, j - 1, k), inv * sb);
+    atomicAdd(Dp + cidx<D>(nc, i, j + 1, k), inv * sb);
+    if (D == 3) {
+      atomicAdd(Dp + cidx<D>(nc, i, j, k - 1), inv * sb);
+      atomicAdd(Dp + cidx<D>(nc, i, j, k + 1), inv * sb);
+    }
+    atomicAdd(Dp + c0, -(D == 2 ? 4.0 : 6.0) * inv * sb);
+  }
+}
+
+// mg$D_box_rstr_lpl (m_aX_mg.f90:851-893) of the children in list,
+// transposed: the parent cells' adjoint in variable ip goes to the child's
+// fine cells in variable idst (added, or assigned when assign) and is
+// cleared
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_rstr(Dev a, const int* __restrict__ list, int n,
+                                                  int nc, int tpb, int ip, int idst,
+                                                  int assign) {
+  BOX_PROLOGUE
+  const int pid = a.parent[id];
+  const int pnc = a.ncell[pid];
+  const int hnc = nc >> 1;
+  int off[3];
+  child_off<D>(a, id, nc, 0, off);
+  double* Pp = boxp(a, ip, pid);
+  double* C = boxp(a, idst, id);
+  const int total = D == 2 ? hnc * hnc : hnc * hnc * hnc;
+  const double w = D == 2 ? 0.25 : 0.125;
+  for (int q = lt; q < total; q += tpb) {
+    const int ci = q % hnc + 1, cj = (q / hnc) % hnc + 1, ck = D == 3 ? q / (hnc * hnc) + 1 : 0;
+    double* pp = Pp + cidx<D>(pnc, off[0] + ci, off[1] + cj, D == 3 ? off[2] + ck : 0);
+    const double v = w * *pp;
+    *pp = 0;
+    for (int f = 0; f < (D == 2 ? 4 : 8); f++) {
+      const int fi = 2 * ci - 1 + (f & 1), fj = 2 * cj - 1 + ((f >> 1) & 1);
+      const int fk = D == 3 ? 2 * ck - 1 + (f >> 2) : 0;
+      double* cp = C + cidx<D>(nc, fi, fj, fk);
+      *cp = assign ? v : *cp + v;
+    }
+  }
+}
+
+// mg$D_box_corr_lpl (m_aX_mg.f90:714-766) of the children in list,
+// transposed: the child's potential adjoint (il, unchanged) is added to the
+// adjoint of the parent's tmp (ilt) with the prolongation's weights
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_corr(Dev a, const int* __restrict__ list, int n,
+                                                  int nc, int tpb, int il, int ilt) {
+  BOX_PROLOGUE
+  const int pid = a.parent[id];
+  const int pnc = a.ncell[pid];
+  int off[3];
+  child_off<D>(a, id, nc, 0, off);
+  const double* F = boxp(a, il, id);
+  double* Q = boxp(a, ilt, pid);
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const double fb = F[cidx<D>(nc, i, j, k)];
+    if (fb == 0) continue;
+    const int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+    const int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+    if (D == 2) {
+      atomicAdd(Q + cidx<D>(pnc, ic1, jc1, 0), 0.5 * fb);
+      atomicAdd(Q + cidx<D>(pnc, ic2, jc1, 0), 0.25 * fb);
+      atomicAdd(Q + cidx<D>(pnc, ic1, jc2, 0), 0.25 * fb);
+    } else {
+      const int kc1 = off[2] + ((k + 1) >> 1), kc2 = kc1 + 1 - 2 * (k & 1);
+      atomicAdd(Q + cidx<D>(pnc, ic1, jc1, kc1), 0.25 * fb);
+      atomicAdd(Q + cidx<D>(pnc, ic2, jc1, kc1), 0.25 * fb);
+      atomicAdd(Q + cidx<D>(pnc, ic1, jc2, kc1), 0.25 * fb);
+      atomicAdd(Q + cidx<D>(pnc, ic1, jc1, kc2), 0.25 * fb);
+    }
+  }
+}
+
+// whole-box transposes: mode 0, tmp = phi: lp += lt, lt = 0; mode 1,
+// tmp = phi - tmp: lp += lt, lt = -lt; mode 2, rhs += tmp: lt += lr
+__global__ __launch_bounds__(256) void k_adj_whole(Dev a, const int* __restrict__ list, int n,
+                                                   int bl, int tpb, int mode, int il, int ilt,
+                                                   int ilr) {
+  BOX_PROLOGUE
+  double* Lp = boxp(a, il, id);
+  double* Lt = boxp(a, ilt, id);
+  const double* Lr = boxp(a, ilr, id);
+  for (int q = lt; q < bl; q += tpb) {
+    if (mode == 2) {
+      Lt[q] += Lr[q];
+    } else {
+      const double tb = Lt[q];
+      Lp[q] += tb;
+      Lt[q] = mode == 0 ? 0.0 : -tb;
+    }
+  }
+}
+
+// dst += src on the interior of the boxes in list; then src = 0 there when
+// clear_src
+template <int D>
+__global__ __launch_bounds__(256) void k_adj_int_add(Dev a, const int* __restrict__ list, int n,
+                                                     int nc, int tpb, int isrc, int idst,
+                                                     int clear_src) {
+  BOX_PROLOGUE
+  double* S = boxp(a, isrc, id);
+  double* Dp = boxp(a, idst, id);
+  const int total = D == 2 ? nc * nc : nc * nc * nc;
+  for (int q = lt; q < total; q += tpb) {
+    const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
+    const int c0 = cidx<D>(nc, i, j, k);
+    Dp[c0] += S[c0];
+    if (clear_src) S[c0] = 0;
+  }
+}
+
+namespace {
+struct AdjMg {
+  int lp, lt, lr, lw;
+  double* vbar;
+};
+}  // namespace
+
+template <int D>
+static void adj_gc_level(Pool& P, const Dev& a, const AdjMg& m, int lvl) {
+  const int n = P.n_ids(lvl);
+  if (!n) return;
+  hipLaunchKernelGGL(k_adj_gc<D>, dim3(n), dim3(256), 0, P.stream, a, P.ids(lvl), n,
+                     P.nc_lvl(lvl), m.lp, m.vbar);
+  post_launch("adj_gc");
+}
+
+template <int D>
+static void adj_gsrb_boxes(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int lvl,
+                           int n_cycle) {
+  const int n = P.n_ids(lvl);
+  if (!n) return;
+  const int nc = P.nc_lvl(lvl);
+  const Geo g = geo(n, cells_of(D, nc, false) / 2);
+  for (int it = 2 * n_cycle; it >= 1; it--) {
+    adj_gc_level<D>(P, a, m, lvl);
+    if (c.cyl)
+      hipLaunchKernelGGL((k_adj_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr);
+    else
+      hipLaunchKernelGGL((k_adj_gsrb<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                         P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr);
+    post_launch("adj_gsrb");
+  }
+}
+
+template <int D>
+static void adj_op(Pool& P, const Ctx& c, const Dev& a, const int* list, int n, int nc,
+                   int isrc, double sgn, int idst, bool clear) {
+  if (!n) return;
+  const Geo g = geo(n, cells_of(D, nc, false));
+  if (c.cyl)
+    hipLaunchKernelGGL((k_adj_op<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n, nc,
+                       g.tpb, isrc, sgn, idst, clear ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_adj_op<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a, list, n,
+                       nc, g.tpb, isrc, sgn, idst, clear ? 1 : 0);
+  post_launch("adj_op");
+}
+
+static void adj_whole(Pool& P, const Dev& a, const int* list, int n, int nc, int mode,
+                      const AdjMg& m) {
+  if (!n) return;
+  const int bl = cells_of(P.tree.dim, nc, true);
+  const Geo g = geo(n, bl);
+  hipLaunchKernelGGL(k_adj_whole, dim3(g.grid), dim3(256), 0, P.stream, a, list, n, bl, g.tpb,
+                     mode, m.lp, m.lt, m.lr);
+  post_launch("adj_whole");
+}
+
+template <int D>
+static void adj_rstr(Pool& P, const Dev& a, int lvl, int ip, int idst, bool assign) {
+  const int n = P.n_ids(lvl);
+  if (!n) return;
+  const int nc = P.nc_lvl(lvl);
+  const Geo g = geo(n, cells_of(D, nc / 2, false));
+  hipLaunchKernelGGL(k_adj_rstr<D>, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
+                     g.tpb, ip, idst, assign ? 1 : 0);
+  post_launch("adj_rstr");
+}
+
+template <int D>
+static void adj_interior_add(Pool& P, const Dev& a, int lvl, int isrc, int idst, bool clear) {
+  const int n = P.n_ids(lvl);
+  if (!n) return;
+  const int nc = P.nc_lvl(lvl);
+  const Geo g = geo(n, cells_of(D, nc, false));
+  hipLaunchKernelGGL(k_adj_int_add<D>, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n,
+                     nc, g.tpb, isrc, idst, clear ? 1 : 0);
+  post_launch("adj_int_add");
+}
+
+// correct_children over the parents of lvl (m_aX_mg.f90:374-401), transposed
+template <int D>
+static void adj_correct_children(Pool& P, const Dev& a, const AdjMg& m, int lvl) {
+  const int np = P.n_parents(lvl);
+  if (!np) return;
+  const int nch = P.n_ids(lvl + 1), cnc = P.nc_lvl(lvl + 1);
+  const Geo g = geo(nch, cells_of(D, cnc, false));
+  hipLaunchKernelGGL(k_adj_corr<D>, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl + 1),
+                     nch, cnc, g.tpb, m.lp, m.lt);
+  post_launch("adj_corr");
+  adj_whole(P, a, P.parents(lvl), np, P.nc_lvl(lvl), 1, m);
+}
+
+// update_coarse (m_aX_mg.f90:430-486; the fine tmp is restored there) and
+// set_coarse_phi_rhs (:490-523; the fine tmp keeps the residual), transposed
+template <int D>
+static void adj_update_coarse(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int lvl,
+                              bool keep_res) {
+  const int np = P.n_parents(lvl - 1);
+  const int pnc = P.nc_lvl(lvl - 1);
+  if (!keep_res) adj_whole(P, a, P.parents(lvl - 1), np, pnc, 0, m);  // tmp = phi
+  adj_whole(P, a, P.parents(lvl - 1), np, pnc, 2, m);                 // rhs += tmp
+  adj_op<D>(P, c, a, P.parents(lvl - 1), np, pnc, m.lr, 1.0, m.lp, true);  // rhs = A phi
+  adj_gc_level<D>(P, a, m, lvl - 1);
+  const int n = P.n_ids(lvl), nc = P.nc_lvl(lvl);
+  adj_rstr<D>(P, a, lvl, m.lp, m.lp, false);  // phi_p = R phi
+  if (keep_res) {
+    adj_rstr<D>(P, a, lvl, m.lt, m.lt, false);  // tmp_p = R tmp
+    // tmp = rhs - A phi on the interior
+    adj_op<D>(P, c, a, P.ids(lvl), n, nc, m.lt, -1.0, m.lp, false);
+    adj_interior_add<D>(P, a, lvl, m.lt, m.lr, true);
+  } else {
+    adj_rstr<D>(P, a, lvl, m.lt, m.lw, true);  // tmp_p = R (rhs - A phi)
+    adj_op<D>(P, c, a, P.ids(lvl), n, nc, m.lw, -1.0, m.lp, false);
+    adj_interior_add<D>(P, a, lvl, m.lw, m.lr, false);
+  }
+  if (keep_res && lvl == P.tree.max_lvl) adj_gc_level<D>(P, a, m, lvl);
+}
+
+template <int D>
+static void adj_vcycle(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int max_lvl) {
+  const int min_lvl = P.tree.min_lvl;
+  for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
+    adj_gsrb_boxes<D>(P, c, a, m, lvl, c.n_up);
+    adj_gc_level<D>(P, a, m, lvl);
+    adj_correct_children<D>(P, a, m, lvl - 1);
+  }
+  adj_gsrb_boxes<D>(P, c, a, m, min_lvl, c.n_base);
+  for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
+    adj_update_coarse<D>(P, c, a, m, lvl, false);
+    adj_gsrb_boxes<D>(P, c, a, m, lvl, c.n_down);
+  }
+}
+
+template <int D>
+static void fmg_adjoint_t(Pool& P, const AdjMg& m) {
+  const Ctx c = ctx64(P);
+  const Dev a = P.dev();
+  const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
+  for (int lvl = max_lvl; lvl >= min_lvl; lvl--) {
+    adj_vcycle<D>(P, c, a, m, lvl);
+    if (lvl > min_lvl) {
+      adj_gc_level<D>(P, a, m, lvl);
+      adj_correct_children<D>(P, a, m, lvl - 1);
+    }
+    adj_whole(P, a, P.ids(lvl), P.n_ids(lvl), P.nc_lvl(lvl), 0, m);  // tmp = phi
+  }
+  for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) adj_update_coarse<D>(P, c, a, m, lvl, true);
+}
+
+void fmg_adjoint(Pool& P, const int* adj, double* vbar_host) {
+  if (!P.mg.ready) throw std::logic_error("fmg_adjoint: you haven't called mg_init_mg");
+  if (P.mg.helm != 0) throw std::invalid_argument("fmg_adjoint: the Laplacian only");
+  if (P.n_inst) throw std::invalid_argument("fmg_adjoint: one run per tree only");
+  mg_check(P);
+  for (int id = 1; id <= P.tree.max_id; id++)
+    if (P.tree.in_use[id] && P.mg.auto_ops && P.mg.i_eps > 0 && P.tree.tag[id] > kTagNormal)
+      throw std::invalid_argument("fmg_adjoint: boxes with a permittivity are not supported");
+  const int nv = P.tree.n_cc;
+  for (int k = 0; k < 4; k++)
+    if (adj[k] < 1 || adj[k] > nv) throw std::out_of_range("fmg_adjoint: bad adjoint variable");
+  double* d_vbar = (double*)P.work(8 * sizeof(double));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, P.stream, (unsigned long long*)d_vbar, 8,
+                     0ull);
+  post_launch("fill");
+  const AdjMg m{adj[0], adj[1], adj[2], adj[3], d_vbar};
+  if (P.tree.dim == 2) fmg_adjoint_t<2>(P, m);
+  else fmg_adjoint_t<3>(P, m);
+  check(hipMemcpyAsync(vbar_host, d_vbar, 2 * P.tree.dim * sizeof(double), hipMemcpyDeviceToHost,
+                       P.stream),
+        "vbar");
+  check(hipStreamSynchronize(P.stream), "fmg_adjoint sync");
+}
+
 }  // namespace a5g

@@ -276,6 +276,14 @@ module m_a5g
        integer(c_int)        :: s
      end function c_fas_vcycle
 
+     function c_fas_fmg_adjoint(t, adj, vbar) bind(C, name="a5g_fas_fmg_adjoint") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int)        :: adj(*)
+       real(c_double)        :: vbar(*)
+       integer(c_int)        :: s
+     end function c_fas_fmg_adjoint
+
      function c_set_shape(t, kernel, nc, cls, shape) bind(C, name="a5g_set_shape") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t

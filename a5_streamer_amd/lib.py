@@ -206,6 +206,15 @@ class Tree:
     def fas_vcycle(self, max_lvl, set_residual):
         _call("a5g_fas_vcycle", self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
 
+    def fas_fmg_adjoint(self, adj):
+        """Reverse mode of fas_fmg(False, False) (a5g_fas_fmg_adjoint):
+        adj = (lp, lt, lr, lw) cell variables; returns the adjoints of the
+        boundary values V per face."""
+        a, pa = _i(np.asarray(adj))
+        vbar = np.zeros(2 * self.dim)
+        _call("a5g_fas_fmg_adjoint", self.h, pa, _d(vbar)[1])
+        return vbar
+
     SHAPE_FIELDS = ("kernel", "dim", "nc", "size_class", "shape", "boxes", "us", "chosen")
 
     def set_shape(self, kernel, nc, cls, shape):

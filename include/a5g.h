@@ -98,6 +98,18 @@ int a5g_mg_set_helmholtz(a5g_tree *t, double kappa2);
 int a5g_fas_fmg(a5g_tree *t, int set_residual, int first_call);
 /* mg$D_fas_vcycle, m_aX_mg.f90:208-255 */
 int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
+/* Reverse mode of a5g_fas_fmg(t, 0, 0) (mg$D_fas_fmg, m_aX_mg.f90:164-203,
+ * with first_call false and no residual): the solve is an affine map of the
+ * potential, tmp and rhs it starts from and of the values V of the rule
+ * 2V - c; this applies its transpose. adj[4] = cell variables lp, lt, lr,
+ * lw: on entry lp, lt, lr hold the adjoints of the potential, tmp and rhs
+ * the solve leaves (every cell, ghosts included); on return the adjoints of
+ * the values it started from; lw is work space. vbar[2 dim] receives the
+ * adjoints of V per face. The transpose of each operation of the
+ * reference's schedule, in reverse order; no state of the forward solve is
+ * needed. Normal boxes (no permittivity), the Laplacian, one run per tree.
+ * No counterpart in the reference, which computes no derivatives. */
+int a5g_fas_fmg_adjoint(a5g_tree *t, const int *adj, double *vbar);
 /* Launch shapes of the box kernels (no reference counterpart: the
  * reference's loops over boxes have no launch geometry). kernel 0 is the
  * multigrid half-sweep, 1 the ghost fill, 2 the residual + restriction, 3

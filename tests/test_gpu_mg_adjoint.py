@@ -1,0 +1,92 @@
+"""The reverse mode of the multigrid solve (a5g_fas_fmg_adjoint) against its
+forward map. mg$D_fas_fmg with first_call false is linear in the potential,
+tmp and rhs it starts from and in the boundary values V (every operation of
+the FMG and its V-cycles is), so its transpose satisfies the dot-product
+identity <w, F(x)> = <F^T w, x> for every x and w. The forward map is the
+library's FMG, bit for bit the oracle's and the reference's
+(tests/test_gpu_mg.py); x and w are seeded random values on every cell,
+ghosts included, and random V. A transpose that drops, doubles or misplaces
+any term of any operation breaks the identity far above the bound
+(1e-12 of the sum of the absolute products)."""
+import numpy as np
+import pytest
+
+import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dim, nc, coord, base, face rules
+    (2, 8, 0, (2, 1, 1), (0, 1, 2, 0)),
+    (2, 8, 1, (1, 2, 1), (0, 0, 1, 3)),
+    (2, 4, 0, (1, 1, 1), (1, 2, 0, 2)),
+    (3, 8, 0, (1, 1, 1), (0, 0, 1, 2, 0, 2)),
+    (3, 4, 0, (2, 1, 1), (2, 1, 0, 0, 1, 2)),
+]
+LP, LT, LR, LW = 5, 6, 7, 8
+
+
+def _rules(lvl, c):
+    if lvl < 2:
+        return 1
+    return 1 if (lvl < 4 and np.linalg.norm(c - 0.35) < 0.22) else -1
+
+
+def _masks(T, dim):
+    """True on the cells of each box slot that the box has ((nc + 2)^dim of
+    box_len(n_cell); coarser boxes fill the start of their slot)."""
+    b = T.boxes()
+    bl = T.box_len(T.n_cell)
+    m = np.zeros((len(b["lvl"]), bl), bool)
+    for i in range(len(b["lvl"])):
+        if b["in_use"][i]:
+            m[i, :(int(b["n_cell"][i]) + 2) ** dim] = True
+    return m
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("dim,nc,coord,base,kinds", CASES)
+def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed):
+    import a5_streamer_amd as A
+    print("fmg adjoint", dim, nc, coord, base, kinds, seed, flush=True)
+    T = A.Tree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
+    S.build_tree(T, dim, nc, _rules, n_base=base)
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal(2 * dim)
+    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    n, bl = T.info()["max_id"], T.box_len(nc)
+    mask = _masks(T, dim)
+    x = [rng.standard_normal((n, bl)) * mask for _ in range(3)]
+    w = [rng.standard_normal((n, bl)) * mask for _ in range(3)]
+    # forward: F(x) from the state x and the values V
+    for iv, v in zip((1, 2, 3), x):
+        T.set_cc_all(iv, v)
+    T.fas_fmg(False, False)
+    y = [T.get_cc_all(iv) * mask for iv in (1, 2, 3)]
+    lhs = sum(float(np.sum(a * b)) for a, b in zip(w, y))
+    scale = sum(float(np.sum(np.abs(a * b))) for a, b in zip(w, y))
+    # reverse: F^T w
+    for iv, v in zip((LP, LT, LR), w):
+        T.set_cc_all(iv, v)
+    T.set_cc_all(LW, np.zeros((n, bl)))
+    vbar = T.fas_fmg_adjoint([LP, LT, LR, LW])
+    xb = [T.get_cc_all(iv) * mask for iv in (LP, LT, LR)]
+    rhs = sum(float(np.sum(a * b)) for a, b in zip(xb, x)) + float(np.dot(vbar, V))
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+    # V enters where a face has the rule 2V - c, and only there
+    for nb in range(2 * dim):
+        if kinds[nb] < 2:
+            assert vbar[nb] == 0.0
+    assert any(vbar[nb] != 0.0 for nb in range(2 * dim) if kinds[nb] >= 2)
+    T.close()
+
+
+def test_fmg_adjoint_refuses_what_it_does_not_cover():
+    import a5_streamer_amd as A
+    T = A.Tree(2, 8, 8, 1, 1.0 / 8, coarsen_to=2)
+    S.build_tree(T, 2, 8, _rules)
+    T.mg_setup(1, 2, 3, bc_kind=np.zeros(4, np.int32), auto_ops=True)
+    T.mg_set_helmholtz(4.0)
+    with pytest.raises(Exception):
+        T.fas_fmg_adjoint([LP, LT, LR, LW])
+    T.close()
