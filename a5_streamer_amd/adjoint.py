@@ -1,5 +1,7 @@
-"""Differentiation of the streamer's transport substep on a frozen mesh,
-as a torch.autograd.Function over the a5g C ABI.
+"""Differentiation of the streamer's time steps on a frozen mesh, as
+torch.autograd.Functions over the a5g C ABI: one transport substep
+(TransportSubstep), and a window of whole time steps with checkpoints
+(FrozenWindow, StreamerWindow).
 
 One substep of the drivers' time loop is fluxes_koren, then
 a$D_consistent_fluxes, then update_solution (src/streamer_2d.f90:105-110,
@@ -18,8 +20,23 @@ density includes its ghost cells and the cells the second ghost layer is
 taken from; rows of boxes the substep does not change pass their gradient
 through.
 
+A window of whole steps: FrozenWindow steps a run of the 2D or 3D driver
+(src/streamer_2d.f90:91-129, src/streamer_3d.f90 alike) with its mesh and
+its time steps frozen, and runs the reverse of every operation of a step
+in the opposite order with the hand-written reverse kernels
+(a5g_*_adjoint, a5g_fas_fmg_adjoint). States are checkpointed in device
+memory (a5g_state_store) every `every` steps; the reverse pass recomputes
+each segment from its checkpoint, keeping the start of every step, and
+each step once more, keeping the two states inside it that its reverse
+reads. StreamerWindow maps the transport table and the applied voltage to
+the final densities and field; window_loss is the loss of the adjoint
+workload (integrated electron count plus the softmax-smoothed head
+position).
+
 No counterpart in the reference, which computes no derivatives.
 """
+import math
+
 import numpy as np
 
 from . import streamer as S
@@ -115,3 +132,298 @@ def transport_substep(tree, dt, layout, ne, ni, table):
     """n_e', n_i' of one transport substep, differentiable in ne, ni and the
     table (torch float64 tensors)."""
     return transport_substep_function().apply(ne, ni, table, tree, dt, layout)
+
+
+# ---------------------------------------------------------------------------
+# a window of whole time steps
+# ---------------------------------------------------------------------------
+
+# the state a step reads and writes: cell and face variables
+PRIMAL_CC = (S.I_ELEC, S.I_PION, S.I_ELEC_OLD, S.I_PION_OLD, S.I_PHI, S.I_FLD, S.I_RHS)
+PRIMAL_FC = (S.F_ELEC, S.F_FLD)
+
+
+class StepAdjointVars:
+    """The reverse mode's variables of a run made with Config(adjoint=True):
+    cell variables le, li (densities), leo, lio (the old densities), lphi,
+    lE (|E|, which is also the multigrid's tmp), lrhs, lw (multigrid work)
+    after the run's own, and face variables lF (flux), lfld (field)."""
+
+    def __init__(self, run):
+        b = run.n_cc_run
+        (self.le, self.li, self.leo, self.lio, self.lphi, self.lE, self.lrhs,
+         self.lw) = range(b + 1, b + 1 + S.N_ADJ_CELL)
+        self.lF, self.lfld = S.N_VAR_FACE + 1, S.N_VAR_FACE + 2
+        self.cells = tuple(range(b + 1, b + 1 + S.N_ADJ_CELL))
+        self.faces = (self.lF, self.lfld)
+
+    def transport(self):
+        """adj[6] of the transport substep's reverse kernels (no
+        photoionization source)."""
+        return np.array([self.le, self.li, self.lE, -1, self.lF, self.lfld], np.int32)
+
+
+class FrozenWindow:
+    """n_steps time steps of a run on its current mesh, with the time steps
+    dts (recorded by the first forward pass when None: max_dt every
+    steps_amr steps, as the drivers take it), differentiable in the
+    transport table, the applied voltage and the densities it starts from.
+
+    The run must have been made with Config(adjoint=True), without
+    photoionization, a dielectric or a time-dependent field. The window
+    starts from the run's state at construction every time it runs."""
+
+    SLOT_START, SLOT_S1, SLOT_S2, SLOT_CKPT = 0, 1, 2, 3
+
+    def __init__(self, run, n_steps, every=10, dts=None):
+        c = run.c
+        if not c.adjoint:
+            raise ValueError("FrozenWindow: the run needs Config(adjoint=True)")
+        if c.photoi or c.epsilon_diel != 1.0 or c.fld_mod_t0 < 1e98:
+            raise ValueError("FrozenWindow: no photoionization, dielectric or field modulation")
+        if run.v["coord"] != 0:
+            raise ValueError("FrozenWindow: Cartesian runs only")
+        if n_steps < 1 or every < 1:
+            raise ValueError("FrozenWindow: n_steps and every must be positive")
+        self.run, self.T = run, run.T
+        self.n, self.k = int(n_steps), int(every)
+        self.n_seg = (self.n + self.k - 1) // self.k
+        self.v = StepAdjointVars(run)
+        self.dts = None if dts is None else [float(x) for x in dts]
+        rc, self.x_min, self.inv_dx = self.T.table_get()
+        self.table0 = rc.copy()
+        self.voltage0 = run.voltage
+        self.top = 2 * run.dim                 # the face of the rule 2V - c
+        self._ckpt_key = None
+        self.T.state_store(self.SLOT_START, PRIMAL_CC, PRIMAL_FC)
+
+    def _slot_step(self, j):
+        return self.SLOT_CKPT + self.n_seg + j
+
+    def _save(self, slot):
+        self.T.state_store(slot, PRIMAL_CC, PRIMAL_FC)
+
+    def _load(self, slot):
+        self.T.state_store(slot, PRIMAL_CC, PRIMAL_FC, load=True)
+
+    # --- forward ----------------------------------------------------------
+    def _compute_fld(self):
+        """compute_fld (src/streamer_2d.f90:307-343) at the window's fixed
+        boundary value."""
+        T = self.T
+        T.space_charge(S.ELEC / S.EPS0)
+        for _ in range(S.N_FMG_CYCLES):
+            T.fas_fmg(False, False)
+        T.fld_from_pot()
+        T.gc_tree(S.I_FLD, S.RB_INTERP, S.BC_NEUMANN)
+
+    def _substep(self, dt):
+        T = self.T
+        T.fluxes_koren(dt)
+        T.consistent_fluxes([S.F_ELEC])
+        T.update_solution(dt)
+        T.restrict_tree(S.I_ELEC)
+        T.restrict_tree(S.I_PION)
+        T.gc_tree(S.I_ELEC, S.RB_INTERP_LIM, S.BC_NEUMANN)
+        T.gc_tree(S.I_PION, S.RB_INTERP_LIM, S.BC_NEUMANN)
+
+    def _step(self, dt, keep=False):
+        """One time step of the drivers' loop (src/streamer_2d.f90:91-129);
+        keep saves the states after the first substep's field solve (S1) and
+        after the second substep (S2)."""
+        T = self.T
+        T.tree_copy_cc(S.I_ELEC, S.I_ELEC_OLD)
+        T.tree_copy_cc(S.I_PION, S.I_PION_OLD)
+        self._substep(dt)
+        self._compute_fld()
+        if keep:
+            self._save(self.SLOT_S1)
+        self._substep(dt)
+        if keep:
+            self._save(self.SLOT_S2)
+        T.average_dens()
+        self._compute_fld()
+
+    def _set_params(self, table, voltage):
+        T = self.T
+        T.set_table(np.asarray(table, float), self.x_min, self.inv_dx)
+        T.mg_set_bc_value(self.top, float(voltage))
+
+    def forward(self, table=None, voltage=None):
+        """Runs the window from its start state with the transport table
+        (n_cols, n_rows) and the applied voltage (the value V of the rule
+        2V - c; defaults: the run's), keeping the checkpoints. The run's
+        tree then holds the final state."""
+        table = self.table0 if table is None else np.asarray(table, float)
+        voltage = self.voltage0 if voltage is None else float(voltage)
+        self._load(self.SLOT_START)
+        self._set_params(table, voltage)
+        record = self.dts is None
+        dts = [] if record else self.dts
+        amr = self.run.c.steps_amr
+        for s in range(self.n):
+            if s % self.k == 0:
+                self._save(self.SLOT_CKPT + s // self.k)
+            if record:
+                dts.append(self.run.max_dt() if s % amr == 0 else dts[-1])
+            self._step(dts[s])
+        self.dts = dts
+        self._ckpt_key = (table.tobytes(), voltage)
+        return self
+
+    # --- reverse ----------------------------------------------------------
+    def _compute_fld_reverse(self):
+        T, v = self.T, self.v
+        T.gc_tree_adjoint(S.I_FLD, v.lE, S.RB_INTERP, S.BC_NEUMANN)
+        T.fld_from_pot_adjoint(v.lE, v.lfld, v.lphi)
+        vbar = 0.0
+        for _ in range(S.N_FMG_CYCLES):
+            vbar += T.fas_fmg_adjoint([v.lphi, v.lE, v.lrhs, v.lw])[self.top - 1]
+        T.space_charge_adjoint(S.ELEC / S.EPS0, v.lrhs, v.le, v.li)
+        return vbar
+
+    def _transport_reverse(self, dt):
+        """The reverse of fluxes_koren, consistent fluxes and update_solution
+        on the state in the primal variables (the substep's start), whose
+        fluxes are recomputed first."""
+        T, adj = self.T, self.v.transport()
+        T.fluxes_koren(dt)
+        T.consistent_fluxes([S.F_ELEC])
+        T.update_solution_adjoint(dt, adj)
+        T.consistent_fluxes_adjoint(self.v.lF)
+        T.fluxes_koren_adjoint(dt, adj)
+
+    def _ghosts_restrict_reverse(self):
+        T, v = self.T, self.v
+        T.gc_tree_adjoint(S.I_ELEC, v.le, S.RB_INTERP_LIM, S.BC_NEUMANN)
+        T.gc_tree_adjoint(S.I_PION, v.li, S.RB_INTERP_LIM, S.BC_NEUMANN)
+        T.restrict_tree_adjoint(v.le)
+        T.restrict_tree_adjoint(v.li)
+
+    def _step_reverse(self, dt, s0_slot):
+        """The reverse of one step from its start state S0, which is in the
+        primal variables and in store slot s0_slot: the step is recomputed
+        keeping S1 and S2, then every operation's reverse runs, last first.
+        Returns the voltage's share."""
+        T, v = self.T, self.v
+        self._step(dt, keep=True)
+        vbar = self._compute_fld_reverse()                 # the last field solve
+        T.average_dens_adjoint(v.le, v.leo, v.li, v.lio)
+        self._load(self.SLOT_S2)                           # the second substep
+        self._ghosts_restrict_reverse()
+        self._load(self.SLOT_S1)
+        self._transport_reverse(dt)
+        vbar += self._compute_fld_reverse()                # the first substep's field solve
+        self._ghosts_restrict_reverse()
+        self._load(s0_slot)                                # the first substep
+        self._transport_reverse(dt)
+        T.tree_copy_cc_adjoint(v.le, v.leo)                # the copies to the old densities
+        T.tree_copy_cc_adjoint(v.li, v.lio)
+        return vbar
+
+    def backward(self, g_ne, g_E, table=None, voltage=None):
+        """The gradient of a loss whose gradients with respect to the final
+        n_e and |E| (arrays shaped as Tree.get_cc_all) are g_ne, g_E, at the
+        given table and voltage (defaults: the run's). Returns (d table
+        (n_cols, n_rows), d voltage, d n_e, d n_i of the start state)."""
+        table = self.table0 if table is None else np.asarray(table, float)
+        voltage = self.voltage0 if voltage is None else float(voltage)
+        if self.dts is None or self._ckpt_key != (table.tobytes(), voltage):
+            self.forward(table, voltage)
+        T, v = self.T, self.v
+        self._set_params(table, voltage)
+        T.state_store(0, v.cells, v.faces, clear=True)
+        T.set_cc_all(v.le, np.asarray(g_ne, float))
+        T.set_cc_all(v.lE, np.asarray(g_E, float))
+        T.table_adjoint(clear=True)
+        vbar = 0.0
+        for seg in reversed(range(self.n_seg)):
+            lo, hi = seg * self.k, min(self.n, (seg + 1) * self.k)
+            self._load(self.SLOT_CKPT + seg)
+            for s in range(lo, hi):
+                self._save(self._slot_step(s - lo))
+                if s < hi - 1:
+                    self._step(self.dts[s])
+            for s in reversed(range(lo, hi)):
+                slot = self._slot_step(s - lo)
+                self._load(slot)
+                vbar += self._step_reverse(self.dts[s], slot)
+        g_table = T.table_adjoint(clear=True)
+        return g_table, vbar, T.get_cc_all(v.le), T.get_cc_all(v.li)
+
+
+def streamer_window_function():
+    """The torch.autograd.Function of a FrozenWindow: (table, voltage,
+    window) -> (n_e, |E|) of the final state, float64 tensors shaped as
+    Tree.get_cc_all; its backward runs the window's reverse pass."""
+    torch = _torch()
+
+    class StreamerWindow(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, table, voltage, window):
+            tb = table.detach().cpu().numpy().astype(float)
+            vo = float(voltage.detach().cpu())
+            window.forward(tb, vo)
+            ne = torch.from_numpy(window.T.get_cc_all(S.I_ELEC))
+            E = torch.from_numpy(window.T.get_cc_all(S.I_FLD))
+            ctx.window, ctx.params = window, (tb, vo)
+            return ne, E
+
+        @staticmethod
+        def backward(ctx, g_ne, g_E):
+            tb, vo = ctx.params
+            gt, gv, _, _ = ctx.window.backward(g_ne.detach().cpu().numpy(),
+                                               g_E.detach().cpu().numpy(), tb, vo)
+            return (torch.from_numpy(gt), torch.tensor(gv, dtype=torch.float64), None)
+
+    return StreamerWindow
+
+
+def streamer_window(window, table, voltage):
+    """(n_e, |E|) after the window's steps, differentiable in the table
+    (n_cols, n_rows) and the voltage (torch float64 tensors)."""
+    return streamer_window_function().apply(table, voltage, window)
+
+
+class LeafGeometry:
+    """Per cell of the Tree.get_cc_all layout: the leaf-interior mask, the
+    cell volume (dr^D) and the coordinate along the last axis (the
+    propagation axis of the drivers' streamers)."""
+
+    def __init__(self, T):
+        b = T.boxes()
+        n, nc, D = len(b["lvl"]), T.n_cell, T.dim
+        m = nc + 2
+        self.mask = np.zeros((n, m ** D), bool)
+        self.vol = np.zeros((n, m ** D))
+        self.z = np.zeros((n, m ** D))
+        inner = np.zeros([m] * D, bool)
+        inner[(slice(1, -1),) * D] = True
+        inner = inner.ravel(order="F")
+        # the last axis' index of each cell of a box (x fastest)
+        last = (np.arange(m ** D) // m ** (D - 1)) if D > 1 else np.arange(m)
+        for lvl in range(1, T.info()["max_lvl"] + 1):
+            for i in T.level(lvl)[1]:
+                r = int(i) - 1
+                dr = b["dr"][r]
+                self.mask[r] = inner
+                self.vol[r, inner] = dr ** D
+                self.z[r] = b["r_min"][r, D - 1] + (last - 0.5) * dr
+
+
+def window_loss(ne, E, geo, domain_len, e_ref, tau=1e-4):
+    """The adjoint workload's loss: the electron count (sum of n_e dr^D over
+    the leaf cells) plus the head position, the last-axis coordinate of the
+    leaf cells weighted by softmax(L |E| / (tau e_ref)): a smooth stand-in
+    for the location of the maximum field (src/streamer_cyl.f90:703-723),
+    with temperature tau (1e-4 m) on the length scale where the domain
+    length L corresponds to the reference field e_ref. Returns (loss,
+    count, head) as torch scalars."""
+    torch = _torch()
+    vol = torch.from_numpy(geo.vol)
+    count = (ne * vol).sum()
+    m = torch.from_numpy(geo.mask)
+    beta = domain_len / (tau * e_ref)
+    p = torch.softmax(beta * E[m], dim=0)
+    head = (p * torch.from_numpy(geo.z)[m]).sum()
+    return count + head, count, head

@@ -387,6 +387,7 @@ void Pool::release() {
   f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag); f(d_inst);
   f(d_term_red);
   d_term_red = nullptr;
+  store_release(*this);
   f(d_work);
   d_work = nullptr;
   work_bytes = 0;
@@ -1310,6 +1311,73 @@ int a5g_table_adjoint(a5g_tree* t, int clear, double* rows_cols) {
   NEED(t);
   need_gpu(t->P);
   a5g::table_adjoint(t->P, clear != 0, rows_cols);
+  API_END
+}
+
+int a5g_tree_copy_cc_adjoint(a5g_tree* t, int lfrom, int lto) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::copy_cc_adj(t->P, lfrom, lto);
+  API_END
+}
+
+int a5g_average_dens_adjoint(a5g_tree* t, const int* adj) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!adj) throw std::invalid_argument("null argument");
+  a5g::average_dens_adj(t->P, adj[0], adj[1], adj[2], adj[3]);
+  API_END
+}
+
+int a5g_space_charge_adjoint(a5g_tree* t, double fac, const int* adj) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!adj) throw std::invalid_argument("null argument");
+  a5g::space_charge_adj(t->P, fac, adj[0], adj[1], adj[2]);
+  API_END
+}
+
+int a5g_restrict_tree_adjoint(a5g_tree* t, int il) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::restrict_tree_adj(t->P, il);
+  API_END
+}
+
+int a5g_gc_tree_adjoint(a5g_tree* t, int iv, int il, int rb, int bc) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::gc_tree_adj(t->P, iv, il, rb, bc);
+  API_END
+}
+
+int a5g_fld_from_pot_adjoint(a5g_tree* t, const int* adj) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!adj) throw std::invalid_argument("null argument");
+  a5g::fld_from_pot_adj(t->P, t->P.sv, adj[0], adj[1], adj[2]);
+  API_END
+}
+
+int a5g_state_store(a5g_tree* t, int slot, int n_cc, const int* ivs, int n_fc, const int* fvs,
+                    int op) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  a5g::state_store(t->P, slot, n_cc, ivs, n_fc, fvs, op);
   API_END
 }
 

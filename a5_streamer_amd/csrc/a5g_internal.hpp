@@ -46,6 +46,15 @@ void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av)
 void consistent_fluxes_adj(Pool& P, int lF);
 void fluxes_koren_adj(Pool& P, const StVars& v, double dt, const AdjVars& av);
 void table_adjoint(Pool& P, bool clear, double* out_host);
+// reverse mode of the rest of the step (a5g_adjoint.hip)
+void copy_cc_adj(Pool& P, int lfrom, int lto);
+void average_dens_adj(Pool& P, int le, int leo, int li, int lio);
+void space_charge_adj(Pool& P, double fac, int lr, int le, int li);
+void restrict_tree_adj(Pool& P, int il);
+void gc_tree_adj(Pool& P, int iv, int il, int rb, int bc);
+void fld_from_pot_adj(Pool& P, const StVars& v, int lE, int lfld, int lphi);
+void state_store(Pool& P, int slot, int n_cc, const int* ivs, int n_fc, const int* fvs, int op);
+void store_release(Pool& P);
 // the same with the library's fixed device scratch instead of allocations
 // (a5g_box_summaries_noalloc); noalloc_init looks the scratch up for the
 // current device

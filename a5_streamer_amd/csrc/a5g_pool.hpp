@@ -275,6 +275,18 @@ struct Pool {
   std::vector<std::vector<std::vector<int>>> g_ids, g_leaves, g_parents;
   std::vector<std::vector<size_t>> g_off_ids, g_off_leaves, g_off_parents;
 
+  // device store of whole variables (a5g_state_store): per slot one buffer
+  // with the copies, the box slots it covers, the variable lists and the
+  // topology version it was taken at
+  struct Slot {
+    double* d = nullptr;
+    size_t bytes = 0;
+    int max_id = 0;
+    std::vector<int> key;
+    uint64_t topo = 0;
+  };
+  std::vector<Slot> store;
+
   int bs() const;
   int fs() const;
   Dev dev();

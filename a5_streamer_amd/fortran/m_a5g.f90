@@ -530,6 +530,60 @@ module m_a5g
        integer(c_int)        :: s
      end function c_table_adjoint
 
+     function c_tree_copy_cc_adjoint(t, lfrom, lto) &
+          bind(C, name="a5g_tree_copy_cc_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: lfrom, lto
+       integer(c_int)        :: s
+     end function c_tree_copy_cc_adjoint
+
+     function c_average_dens_adjoint(t, adj) bind(C, name="a5g_average_dens_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: adj(4)
+       integer(c_int)     :: s
+     end function c_average_dens_adjoint
+
+     function c_space_charge_adjoint(t, fac, adj) &
+          bind(C, name="a5g_space_charge_adjoint") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       real(c_double), value :: fac
+       integer(c_int)        :: adj(3)
+       integer(c_int)        :: s
+     end function c_space_charge_adjoint
+
+     function c_restrict_tree_adjoint(t, il) bind(C, name="a5g_restrict_tree_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: il
+       integer(c_int)        :: s
+     end function c_restrict_tree_adjoint
+
+     function c_gc_tree_adjoint(t, iv, il, rb, bc) bind(C, name="a5g_gc_tree_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: iv, il, rb, bc
+       integer(c_int)        :: s
+     end function c_gc_tree_adjoint
+
+     function c_fld_from_pot_adjoint(t, adj) bind(C, name="a5g_fld_from_pot_adjoint") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: t
+       integer(c_int)     :: adj(3)
+       integer(c_int)     :: s
+     end function c_fld_from_pot_adjoint
+
+     function c_state_store(t, slot, n_cc, ivs, n_fc, fvs, op) &
+          bind(C, name="a5g_state_store") result(s)
+       import :: c_ptr, c_int
+       type(c_ptr), value    :: t
+       integer(c_int), value :: slot, n_cc, n_fc, op
+       integer(c_int)        :: ivs(*), fvs(*)
+       integer(c_int)        :: s
+     end function c_state_store
+
      function c_box_summaries_noalloc(t, out) bind(C, name="a5g_box_summaries_noalloc") &
           result(s)
        import :: c_ptr, c_int, c_double
@@ -772,6 +826,9 @@ module m_a5g
   public :: st_average_dens, st_box_summaries, st_photoi_helmholtz, a5g_set_helmholtz
   public :: st_box_summaries_noalloc
   public :: st_transport_adjoint, st_table_adjoint
+  public :: st_copy_cc_adjoint, st_average_dens_adjoint, st_space_charge_adjoint
+  public :: st_restrict_tree_adjoint, st_gc_tree_adjoint, st_fld_from_pot_adjoint
+  public :: a5g_state_store
   public :: st_photoi_helmholtz_terms
   public :: a2_get_loc, a3_get_loc, st_reduction_loc, st_get_cc_axis
   public :: a2_write_vtk, a3_write_vtk, a5g_dump_leaves
@@ -1278,6 +1335,70 @@ contains
     logical, intent(in)               :: clear
     call check(c_table_adjoint(tree%h, merge(1, 0, clear), c_loc(rows_cols)), "table adjoint")
   end subroutine st_table_adjoint
+
+  !> Reverse mode of a$D_tree_copy_cc(from, to): lfrom += lto, lto = 0
+  subroutine st_copy_cc_adjoint(tree, lfrom, lto)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: lfrom, lto
+    call check(c_tree_copy_cc_adjoint(tree%h, lfrom, lto), "tree_copy_cc adjoint")
+  end subroutine st_copy_cc_adjoint
+
+  !> Reverse mode of average_dens: adj = le, le_old, li, li_old
+  subroutine st_average_dens_adjoint(tree, adj)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: adj(4)
+    integer(c_int)             :: a(4)
+    a = adj
+    call check(c_average_dens_adjoint(tree%h, a), "average_dens adjoint")
+  end subroutine st_average_dens_adjoint
+
+  !> Reverse mode of compute_fld's rhs = fac (n_e - n_i): adj = lrhs, le, li
+  subroutine st_space_charge_adjoint(tree, fac, adj)
+    type(a5g_t), intent(inout) :: tree
+    real(dp), intent(in)       :: fac
+    integer, intent(in)        :: adj(3)
+    integer(c_int)             :: a(3)
+    a = adj
+    call check(c_space_charge_adjoint(tree%h, fac, a), "space charge adjoint")
+  end subroutine st_space_charge_adjoint
+
+  !> Reverse mode of a$D_restrict_tree for the adjoint variable il
+  subroutine st_restrict_tree_adjoint(tree, il)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: il
+    call check(c_restrict_tree_adjoint(tree%h, il), "restrict_tree adjoint")
+  end subroutine st_restrict_tree_adjoint
+
+  !> Reverse mode of a$D_gc_tree(iv, rb, bc), rb 1 (interp) or 2
+  !> (interp_lim), bc 1 (Neumann) or 2 (Dirichlet); il the adjoint of iv
+  subroutine st_gc_tree_adjoint(tree, iv, il, rb, bc)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: iv, il, rb, bc
+    call check(c_gc_tree_adjoint(tree%h, iv, il, rb, bc), "gc_tree adjoint")
+  end subroutine st_gc_tree_adjoint
+
+  !> Reverse mode of fld_from_pot: adj = lE (cell), lfld (face), lphi (cell)
+  subroutine st_fld_from_pot_adjoint(tree, adj)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: adj(3)
+    integer(c_int)             :: a(3)
+    a = adj
+    call check(c_fld_from_pot_adjoint(tree%h, a), "fld_from_pot adjoint")
+  end subroutine st_fld_from_pot_adjoint
+
+  !> Whole cell variables ivs and face variables fvs: op 0 saves them in the
+  !> device store slot `slot`, op 1 restores them from it, op 2 sets them to
+  !> zero
+  subroutine a5g_state_store(tree, slot, ivs, fvs, op)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: slot, ivs(:), fvs(:), op
+    integer(c_int)             :: a(max(1, size(ivs))), b(max(1, size(fvs)))
+    a = 0
+    b = 0
+    a(1:size(ivs)) = ivs
+    b(1:size(fvs)) = fvs
+    call check(c_state_store(tree%h, slot, size(ivs), a, size(fvs), b, op), "state store")
+  end subroutine a5g_state_store
 
   !> st_box_summaries without any device allocation (the library's fixed
   !> scratch, at most 131072 boxes; the tree must have been uploaded since

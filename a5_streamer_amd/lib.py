@@ -391,6 +391,38 @@ class Tree:
         _call("a5g_table_adjoint", self.h, C.c_int(int(bool(clear))), _d(out)[1])
         return out.reshape(n_cols, n_rows)
 
+    # --- reverse mode of the rest of the step (a5g_*_adjoint) -------------
+    def tree_copy_cc_adjoint(self, lfrom, lto):
+        _call("a5g_tree_copy_cc_adjoint", self.h, C.c_int(lfrom), C.c_int(lto))
+
+    def average_dens_adjoint(self, le, le_old, li, li_old):
+        a, pa = _i(np.array([le, le_old, li, li_old]))
+        _call("a5g_average_dens_adjoint", self.h, pa)
+
+    def space_charge_adjoint(self, fac, lrhs, le, li):
+        a, pa = _i(np.array([lrhs, le, li]))
+        _call("a5g_space_charge_adjoint", self.h, C.c_double(fac), pa)
+
+    def restrict_tree_adjoint(self, il):
+        _call("a5g_restrict_tree_adjoint", self.h, C.c_int(il))
+
+    def gc_tree_adjoint(self, iv, il, rb, bc):
+        _call("a5g_gc_tree_adjoint", self.h, C.c_int(iv), C.c_int(il), C.c_int(rb), C.c_int(bc))
+
+    def fld_from_pot_adjoint(self, lE, lfld, lphi):
+        a, pa = _i(np.array([lE, lfld, lphi]))
+        _call("a5g_fld_from_pot_adjoint", self.h, pa)
+
+    def state_store(self, slot, ivs, fvs=(), load=False, clear=False):
+        """Saves (default) or restores (load) whole cell variables ivs and
+        face variables fvs in device store slot `slot`; clear sets them to
+        zero instead (slot unused)."""
+        a, pa = _i(np.asarray(list(ivs) or [0]))
+        b, pb = _i(np.asarray(list(fvs) or [0]))
+        op = 2 if clear else (1 if load else 0)
+        _call("a5g_state_store", self.h, C.c_int(slot), C.c_int(len(ivs)), pa, C.c_int(len(fvs)),
+              pb, C.c_int(op))
+
     def box_summaries_noalloc(self):
         """box_summaries through the library's fixed scratch: no device
         allocation (at most 131072 boxes; the tree must be uploaded)."""

@@ -49,6 +49,8 @@ BC_MG, BC_NEUMANN = 0, 1                    # the solver's sides_bc, gc_neumann
 # z face, src/streamer_cyl.f90:408-410)
 BC_COPY, BC_NEG, BC_2V, BC_ROW_2V = 0, 1, 2, 3
 N_FMG_CYCLES = 1                            # src/m_streamer.f90:46
+# variables of the reverse mode (Config.adjoint), after the run's own
+N_ADJ_CELL, N_ADJ_FACE = 8, 2
 
 
 def norm2(v):
@@ -326,6 +328,9 @@ class Config:
         # instead of on the host in the reference's arithmetic; point seeds
         # with falloff 2 and no dielectric only
         self.init_on_device = False
+        # The variables of the reverse mode (adjoint.FrozenWindow): eight
+        # cell and two face variables after the run's own
+        self.adjoint = False
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise KeyError("unknown config key %s" % k)
@@ -403,9 +408,14 @@ class StreamerRun:
         n0 = c.n_base if np.isscalar(c.n_base) else c.n_base[0]
         dr = c.domain_len / (c.box_size * n0)
         n_cc = N_VAR_CELL + (3 * len(I_PH_TERMS) if c.photoi else 0)
+        self.n_cc_run = n_cc
+        n_fc = N_VAR_FACE
+        if c.adjoint:
+            n_cc += N_ADJ_CELL
+            n_fc += N_ADJ_FACE
         self.photoi_solved = False     # the term variables hold solutions
         self.photoi_cycles = 0         # V-cycles of the photoionization solves
-        self.T = T = tree_cls(self.dim, c.box_size, n_cc, N_VAR_FACE, dr,
+        self.T = T = tree_cls(self.dim, c.box_size, n_cc, n_fc, dr,
                               coarsen_to=2, lvls_max=c.lvls_max, coord=v["coord"],
                               n_boxes=c.n_boxes)
         T.set_base(*base_grid(self.dim, c.n_base))

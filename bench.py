@@ -567,6 +567,91 @@ def batch_runs(args, n=None, blocks=None, sample=None):
     }
 
 
+# the adjoint workload: the 2D air streamer (4 x 4 base boxes of 8 x 8
+# cells on 32 mm, 2.5 MV/m, seed at (16, 28) mm) on a frozen mesh of about
+# 1.5M leaf cells: uniform level 6 and level 7 over a 10 x 14 mm block
+# around the seed and the channel below it
+ADJ_REGIONS = [(6, [0.0, 0.0], [32e-3, 32e-3]), (7, [11e-3, 18e-3], [21e-3, 32e-3])]
+
+
+def adjoint_window(args, steps=100, every=10, fd_dirs=2):
+    """Gradient of the adjoint workload's loss (electron count + softmax
+    head position after `steps` steps on the frozen mesh) with respect to
+    the transport table, the mobility and the applied voltage: forward pass
+    with a checkpoint every `every` steps, backward pass by
+    checkpoint-recompute, and central differences (h = 1e-6 relative) on
+    fd_dirs seeded directions."""
+    import torch
+    import a5_streamer_amd as A
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    cfg = St.Config("2d", adjoint=True, refine_regions=ADJ_REGIONS, n_boxes=40000)
+    t0 = time.perf_counter()
+    run = St.StreamerRun(A.Tree, cfg)
+    run.initialise()
+    T = run.T
+    if args.autotune:
+        T.autotune(reps=5)
+    win = AD.FrozenWindow(run, steps, every=every)
+    init_s = time.perf_counter() - t0
+    cells = int(sum(len(T.level(l)[1]) for l in range(1, T.info()["max_lvl"] + 1))) * 64
+    geo = AD.LeafGeometry(T)
+    win.forward()                      # records the time steps (warm-up)
+    T.sync()
+    t1 = time.perf_counter()
+    win.forward()
+    T.sync()
+    t_fwd = time.perf_counter() - t1
+    ne = torch.from_numpy(T.get_cc_all(St.I_ELEC)).requires_grad_(True)
+    E = torch.from_numpy(T.get_cc_all(St.I_FLD)).requires_grad_(True)
+    loss, count, head = AD.window_loss(ne, E, geo, cfg.domain_len, cfg.applied_fld)
+    loss.backward()
+    t1 = time.perf_counter()
+    gt, gv, _, _ = win.backward(ne.grad.numpy(), E.grad.numpy())
+    T.sync()
+    t_bwd = time.perf_counter() - t1
+    # the gradient with respect to the scalar mobility and the alpha column
+    g_mob = float(np.sum(gt[St.T_MOB - 1]))
+    rng = np.random.default_rng(20261204)
+    tb0, v0 = win.table0, win.voltage0
+    fd = []
+    for k in range(fd_dirs):
+        dtb = tb0 * rng.standard_normal(tb0.shape)
+        dv = v0 * rng.standard_normal()
+        vals = []
+        for sgn in (1.0, -1.0):
+            win.forward(tb0 + sgn * 1e-6 * dtb, v0 + sgn * 1e-6 * dv)
+            ne_ = torch.from_numpy(T.get_cc_all(St.I_ELEC))
+            E_ = torch.from_numpy(T.get_cc_all(St.I_FLD))
+            vals.append(float(AD.window_loss(ne_, E_, geo, cfg.domain_len, cfg.applied_fld)[0]))
+        f = (vals[0] - vals[1]) / 2e-6
+        a = float(np.sum(gt * dtb)) + gv * dv
+        sc = float(np.sum(np.abs(gt * dtb))) + abs(gv * dv)
+        fd.append({"fd": f, "adjoint": a, "rel_err_of_scale": abs(f - a) / sc})
+    slots = win.SLOT_CKPT + win.n_seg + every
+    n_boxes = int(np.sum(T.boxes()["in_use"]))
+    T.close()
+    return {
+        "metric": "adjoint of the 2D streamer on a frozen mesh, leaf Mcell-steps/s of "
+                  "forward + backward",
+        "value": round(cells * steps / (t_fwd + t_bwd) / 1e6, 3),
+        "unit": "Mcell-steps/s", "dtype": "f64",
+        "config": {"workload": "2D air streamer (32 mm, 4 x 4 base boxes of 8 x 8 cells, "
+                               "2.5 MV/m), frozen mesh: uniform level 6 + level 7 on "
+                               "[11, 21] x [18, 32] mm, %d steps, checkpoint every %d"
+                               % (steps, every),
+                   "leaf_cells": cells, "boxes": n_boxes, "init_s": round(init_s, 1)},
+        "forward_ms_per_step": round(t_fwd * 1e3 / steps, 3),
+        "backward_ms_per_step": round(t_bwd * 1e3 / steps, 3),
+        "backward_over_forward": round(t_bwd / t_fwd, 2),
+        "store_slots": slots,
+        "loss": {"electron_count": float(count), "head_position_m": float(head)},
+        "grad": {"voltage": gv, "mobility": g_mob,
+                 "alpha_table_norm": float(np.linalg.norm(gt[St.T_ALPHA - 1]))},
+        "central_differences": fd,
+    }
+
+
 def level_breakdown(args):
     """Per level of the benchmark octree: launches and event-timed ms of each
     kernel class in one FP64 solve (FMG + the V-cycles to 1e-10) and in one
@@ -675,6 +760,11 @@ def main():
                     help="1: run the launch-shape autotuner on the tree before timing")
     ap.add_argument("--batch-sample", type=int, default=0,
                     help="time this many of the runs one by one (0: all, -1: none)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="measure the adjoint workload instead (own JSON line)")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="leave the adjoint workload out of the headline line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -688,6 +778,9 @@ def main():
         return
     if args.level_breakdown:
         emit(level_breakdown(args))
+        return
+    if args.adjoint:
+        emit(adjoint_window(args, steps=args.adjoint_steps))
         return
 
     from a5_streamer_amd.workloads import OctreePoisson
@@ -792,6 +885,10 @@ def main():
         # the batched cylindrical runs: 256 runs of about 1,100 boxes each,
         # against a sample of them stepped one by one
         out["batched"] = batch_runs(args, n=256, blocks=2, sample=4)
+    if not args.no_adjoint and not args.small:
+        # the adjoint workload: 100 steps of the 2D streamer on a frozen mesh
+        # of about 1.5M leaf cells, differentiated with checkpoints every 10
+        out["adjoint"] = adjoint_window(args, steps=args.adjoint_steps)
     # keys first that the driver reads
     order = ["metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
              "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",

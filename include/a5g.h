@@ -293,6 +293,47 @@ int a5g_fluxes_koren_adjoint(a5g_tree *t, double dt, const int *adj);
  * column-major (NULL: not read); clear = 1 then sets them to zero. */
 int a5g_table_adjoint(a5g_tree *t, int clear, double *rows_cols);
 
+/* ---- reverse mode of the rest of a time step (frozen mesh) ------------ */
+
+/* The adjoints of the remaining calls of the drivers' time step
+ * (src/streamer_2d.f90:98-129 with compute_fld, :307-381). No counterpart
+ * in the reference, which computes no derivatives. Each call takes the
+ * adjoints of the values its forward call writes (and clears them: the
+ * forward call overwrites those values) and adds those of the values it
+ * reads. Calls whose forward call branches on its data (the limited
+ * interpolation, the field norm) are linearised on the state in the primal
+ * variables, which must be the state the forward call read.
+ *   a$D_tree_copy_cc(from, to) (m_aX_utils.f90:432-453): lfrom += lto on
+ *     whole boxes, lto = 0; */
+int a5g_tree_copy_cc_adjoint(a5g_tree *t, int lfrom, int lto);
+/*   average_dens (src/streamer_2d.f90:492-496): adj[4] = le, le_old, li,
+ *     li_old; */
+int a5g_average_dens_adjoint(a5g_tree *t, const int *adj);
+/*   compute_fld's rhs = fac (n_e - n_i) on the leaves
+ *     (src/streamer_2d.f90:318-329): adj[3] = lrhs, le, li; */
+int a5g_space_charge_adjoint(a5g_tree *t, double fac, const int *adj);
+/*   a$D_restrict_tree (m_aX_restrict.f90:54-142): parents' adjoints go to
+ *     their children, coarsest first; */
+int a5g_restrict_tree_adjoint(a5g_tree *t, int il);
+/*   a$D_gc_tree(iv, rb, bc) (m_aX_gc.f90:28-46) with rb 1 a$D_gc_interp or
+ *     2 a$D_gc_interp_lim and bc 1 a$D_gc_neumann or 2 a$D_gc_dirichlet:
+ *     every face ghost's adjoint in il goes to the interior cells it was made
+ *     from; the limiter's case is taken from iv; */
+int a5g_gc_tree_adjoint(a5g_tree *t, int iv, int il, int rb, int bc);
+/*   fld_from_pot (src/streamer_2d.f90:346-381, src/streamer_3d.f90:348-395):
+ *     adj[3] = lE (cell), lfld (face), lphi (cell): the adjoints of the
+ *     interior |E| and of the face field go to the potential (cells and
+ *     ghosts), linearised at the field the forward call left. */
+int a5g_fld_from_pot_adjoint(a5g_tree *t, const int *adj);
+/* A device store of whole variables for checkpoint-recompute: op = 0
+ * copies the cell variables ivs[n_cc] and face variables fvs[n_fc] of every
+ * box slot into store slot `slot` (0..4096, grown as needed, freed with the
+ * handle); op = 1 writes them back; op = 2 sets the variables to zero
+ * (slot unused). A load needs the same variables and an unchanged mesh.
+ * No counterpart in the reference, which keeps no states. */
+int a5g_state_store(a5g_tree *t, int slot, int n_cc, const int *ivs, int n_fc, const int *fvs,
+                    int op);
+
 /* a5g_box_summaries without any device allocation, for callers that cannot
  * allocate (code under stream capture, serving loops, harnesses that forbid
  * hipMalloc); no counterpart in the reference. The per-box maxima go through
