@@ -26,9 +26,9 @@ its time steps frozen, and runs the reverse of every operation of a step
 in the opposite order with the hand-written reverse kernels
 (a5g_*_adjoint, a5g_fas_fmg_adjoint). States are checkpointed in device
 memory (a5g_state_store) every `every` steps; the reverse pass recomputes
-each segment from its checkpoint, keeping the start of every step, and
-each step once more, keeping the two states inside it that its reverse
-reads. StreamerWindow maps the transport table and the applied voltage to
+each segment from its checkpoint once, keeping the three states of every
+step that its reverse reads (the start, the state after the first
+substep's field solve and the state after the second substep). StreamerWindow maps the transport table and the applied voltage to
 the final densities and field; window_loss is the loss of the adjoint
 workload (integrated electron count plus the softmax-smoothed head
 position).
@@ -173,7 +173,7 @@ class FrozenWindow:
     photoionization, a dielectric or a time-dependent field. The window
     starts from the run's state at construction every time it runs."""
 
-    SLOT_START, SLOT_S1, SLOT_S2, SLOT_CKPT = 0, 1, 2, 3
+    SLOT_START, SLOT_FINAL, SLOT_CKPT = 0, 1, 2
 
     def __init__(self, run, n_steps, every=10, dts=None):
         c = run.c
@@ -197,8 +197,14 @@ class FrozenWindow:
         self._ckpt_key = None
         self.T.state_store(self.SLOT_START, PRIMAL_CC, PRIMAL_FC)
 
-    def _slot_step(self, j):
-        return self.SLOT_CKPT + self.n_seg + j
+    def _slot_step(self, j, which):
+        """Store slot of state S0, S1 or S2 (which 0, 1, 2) of step j of a
+        segment."""
+        return self.SLOT_CKPT + self.n_seg + 3 * j + which
+
+    def store_slots(self):
+        """Store slots the window uses (each holds the whole primal state)."""
+        return self.SLOT_CKPT + self.n_seg + 3 * min(self.k, self.n)
 
     def _save(self, slot):
         self.T.state_store(slot, PRIMAL_CC, PRIMAL_FC)
@@ -227,20 +233,20 @@ class FrozenWindow:
         T.gc_tree(S.I_ELEC, S.RB_INTERP_LIM, S.BC_NEUMANN)
         T.gc_tree(S.I_PION, S.RB_INTERP_LIM, S.BC_NEUMANN)
 
-    def _step(self, dt, keep=False):
+    def _step(self, dt, keep=None):
         """One time step of the drivers' loop (src/streamer_2d.f90:91-129);
-        keep saves the states after the first substep's field solve (S1) and
-        after the second substep (S2)."""
+        keep = (slot1, slot2) saves the states after the first substep's
+        field solve (S1) and after the second substep (S2)."""
         T = self.T
         T.tree_copy_cc(S.I_ELEC, S.I_ELEC_OLD)
         T.tree_copy_cc(S.I_PION, S.I_PION_OLD)
         self._substep(dt)
         self._compute_fld()
         if keep:
-            self._save(self.SLOT_S1)
+            self._save(keep[0])
         self._substep(dt)
         if keep:
-            self._save(self.SLOT_S2)
+            self._save(keep[1])
         T.average_dens()
         self._compute_fld()
 
@@ -267,6 +273,7 @@ class FrozenWindow:
             if record:
                 dts.append(self.run.max_dt() if s % amr == 0 else dts[-1])
             self._step(dts[s])
+        self._save(self.SLOT_FINAL)
         self.dts = dts
         self._ckpt_key = (table.tobytes(), voltage)
         return self
@@ -300,22 +307,22 @@ class FrozenWindow:
         T.restrict_tree_adjoint(v.le)
         T.restrict_tree_adjoint(v.li)
 
-    def _step_reverse(self, dt, s0_slot):
-        """The reverse of one step from its start state S0, which is in the
-        primal variables and in store slot s0_slot: the step is recomputed
-        keeping S1 and S2, then every operation's reverse runs, last first.
+    def _step_reverse(self, dt, s0, s1, s2):
+        """The reverse of one step whose end state is in the primal
+        variables and whose states S0 (start), S1 (after the first
+        substep's field solve) and S2 (after the second substep) are in
+        store slots s0, s1, s2: every operation's reverse, last first.
         Returns the voltage's share."""
         T, v = self.T, self.v
-        self._step(dt, keep=True)
         vbar = self._compute_fld_reverse()                 # the last field solve
         T.average_dens_adjoint(v.le, v.leo, v.li, v.lio)
-        self._load(self.SLOT_S2)                           # the second substep
+        self._load(s2)                                     # the second substep
         self._ghosts_restrict_reverse()
-        self._load(self.SLOT_S1)
+        self._load(s1)
         self._transport_reverse(dt)
         vbar += self._compute_fld_reverse()                # the first substep's field solve
         self._ghosts_restrict_reverse()
-        self._load(s0_slot)                                # the first substep
+        self._load(s0)                                     # the first substep
         self._transport_reverse(dt)
         T.tree_copy_cc_adjoint(v.le, v.leo)                # the copies to the old densities
         T.tree_copy_cc_adjoint(v.li, v.lio)
@@ -339,15 +346,24 @@ class FrozenWindow:
         vbar = 0.0
         for seg in reversed(range(self.n_seg)):
             lo, hi = seg * self.k, min(self.n, (seg + 1) * self.k)
+            # recompute the segment from its checkpoint, keeping the three
+            # states of every step that its reverse reads
             self._load(self.SLOT_CKPT + seg)
             for s in range(lo, hi):
-                self._save(self._slot_step(s - lo))
-                if s < hi - 1:
-                    self._step(self.dts[s])
+                j = s - lo
+                self._save(self._slot_step(j, 0))
+                self._step(self.dts[s], keep=(self._slot_step(j, 1), self._slot_step(j, 2)))
             for s in reversed(range(lo, hi)):
-                slot = self._slot_step(s - lo)
-                self._load(slot)
-                vbar += self._step_reverse(self.dts[s], slot)
+                j = s - lo
+                # the step's end state: the next step's start, the next
+                # segment's checkpoint or the window's final state
+                if s + 1 < hi:
+                    self._load(self._slot_step(j + 1, 0))
+                elif seg + 1 < self.n_seg:
+                    self._load(self.SLOT_CKPT + seg + 1)
+                else:
+                    self._load(self.SLOT_FINAL)
+                vbar += self._step_reverse(self.dts[s], *(self._slot_step(j, w) for w in range(3)))
         g_table = T.table_adjoint(clear=True)
         return g_table, vbar, T.get_cc_all(v.le), T.get_cc_all(v.li)
 

@@ -387,6 +387,8 @@ void Pool::release() {
   f(d_tag); f(d_dr); f(d_rmin); f(d_lists); f(d_red); f(d_flux_pairs); f(d_diag); f(d_inst);
   f(d_term_red);
   d_term_red = nullptr;
+  f(d_vbar);
+  d_vbar = nullptr;
   store_release(*this);
   f(d_work);
   d_work = nullptr;

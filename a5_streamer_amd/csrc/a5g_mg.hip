@@ -3006,86 +3006,95 @@ void autotune(Pool& P, int reps, std::vector<double>& rec) {
 // ---------------------------------------------------------------------------
 
 // a$D_gc_box's rules on one box (m_aX_gc.f90:50-68, with mg$D_sides_rb and
-// the physical rules), transposed: faces in the reverse order, every face
-// ghost's adjoint sent to the values it was made from and cleared.
+// the physical rules), transposed: every face ghost's adjoint is sent to the
+// values it was made from and cleared. The forward pass makes face ghosts
+// from interior cells only, except the corner ghosts of the whole-row rule
+// 2V - c, which it writes last from the x ghosts of the same pass; so those
+// corners go first (into their x ghosts), then every face ghost at once.
+// Several boxes share a workgroup (tpb lanes each).
 template <int D>
 __global__ __launch_bounds__(256) void k_adj_gc(Dev a, const int* __restrict__ list, int n,
-                                                int nc, int il, double* __restrict__ vbar) {
-  if ((int)blockIdx.x >= n) return;
-  const int id = list[blockIdx.x];
-  double* L = boxp(a, il, id);
+                                                int nc, int tpb, int il, double* __restrict__ vbar) {
+  const int slot = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot * tpb;
+  const int li_ = blockIdx.x * (blockDim.x / tpb) + slot;
+  const bool act = li_ < n;
+  const int id = act ? list[li_] : 0;
+  double* L = act ? boxp(a, il, id) : nullptr;
+  if (D == 2 && act && lt < 4) {
+    // a(gx, gy) = 2V - a(gx, cy) on the y faces with the whole-row rule
+    const int nb = 3 + (lt >> 1);
+    if (a.nbs[id * 6 + nb - 1] < 0 && a.bc_kind[nb - 1] == kBcRow2V) {
+      const bool low = d_nb_low(nb);
+      const int gy = low ? 0 : nc + 1, cy = low ? 1 : nc;
+      const int gx = (lt & 1) ? nc + 1 : 0;
+      double* g = L + cidx<D>(nc, gx, gy, 0);
+      const double gb = *g;
+      *g = 0;
+      L[cidx<D>(nc, gx, cy, 0)] -= gb;
+      atomicAdd(vbar + nb - 1, 2 * gb);
+    }
+  }
+  __syncthreads();
+  if (!act) return;
   const int per_face = D == 2 ? nc : nc * nc;
-  for (int nb = 2 * D; nb >= 1; nb--) {
+  for (int q = lt; q < 2 * D * per_face; q += tpb) {
+    const int nb = q / per_face + 1;
+    const int t = q - (nb - 1) * per_face;
     const int d = d_nb_dim(nb);
     const bool low = d_nb_low(nb);
     const int gix = low ? 0 : nc + 1, cix = low ? 1 : nc, dix = low ? 1 : -1;
     const int nb_id = a.nbs[id * 6 + nb - 1];
-    if (D == 2 && nb_id < 0 && d == 1 && a.bc_kind[nb - 1] == kBcRow2V) {
-      // the corner ghosts of the whole-row rule, written last from the x
-      // ghosts of this pass: a(gx, gy) = 2V - a(gx, cy)
-      if (threadIdx.x < 2) {
-        const int gx = threadIdx.x == 0 ? 0 : nc + 1;
-        double* g = L + cidx<D>(nc, gx, gix, 0);
-        const double gb = *g;
-        *g = 0;
-        atomicAdd(L + cidx<D>(nc, gx, cix, 0), -gb);
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    int g[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    g[d] = gix;
+    c[d] = cix;
+    g[ta] = c[ta] = t % nc + 1;
+    if (D == 3) g[tb] = c[tb] = t / nc + 1;
+    double* gp = L + cidx<D>(nc, g[0], g[1], g[2]);
+    const double gb = *gp;
+    *gp = 0;
+    if (gb == 0) continue;
+    if (nb_id > 0) {
+      // sides_from_nb (m_aX_gc.f90:368-402): the neighbour's adjacent layer
+      int s[3] = {c[0], c[1], c[2]};
+      s[d] = low ? nc : 1;
+      atomicAdd(boxp(a, il, nb_id) + cidx<D>(nc, s[0], s[1], s[2]), gb);
+    } else if (nb_id == 0) {
+      // mg$D_sides_rb (m_aX_mg.f90:262-356): 0.5 times the parent's cell
+      // (a$D_gc_prolong0) plus an extrapolation from the box's own cells
+      const int pid = a.parent[id];
+      const int pnc = a.ncell[pid];
+      int off[3];
+      child_off<D>(a, id, nc, 0, off);
+      int pc[3] = {0, 0, 0};
+      for (int e = 0; e < D; e++) pc[e] = off[e] + ((g[e] + 1) >> 1);
+      atomicAdd(boxp(a, il, pid) + cidx<D>(pnc, pc[0], pc[1], pc[2]), 0.5 * gb);
+      atomicAdd(L + cidx<D>(nc, c[0], c[1], c[2]), (D == 2 ? 1.0 : 1.25) * gb);
+      int e1[3] = {c[0], c[1], c[2]};
+      e1[d] += dix;
+      atomicAdd(L + cidx<D>(nc, e1[0], e1[1], e1[2]), -0.25 * gb);
+      int e2[3] = {c[0], c[1], c[2]};
+      e2[ta] += -1 + 2 * (c[ta] & 1);
+      atomicAdd(L + cidx<D>(nc, e2[0], e2[1], e2[2]), -0.25 * gb);
+      if (D == 3) {
+        int e3[3] = {c[0], c[1], c[2]};
+        e3[tb] += -1 + 2 * (c[tb] & 1);
+        atomicAdd(L + cidx<D>(nc, e3[0], e3[1], e3[2]), -0.25 * gb);
+      }
+    } else {
+      // the drivers' rules: copy, -c, 2V - c
+      const int kind = a.bc_kind[nb - 1];
+      double* cp = L + cidx<D>(nc, c[0], c[1], c[2]);
+      if (kind == kBcCopy) {
+        atomicAdd(cp, gb);
+      } else if (kind == kBcNeg) {
+        atomicAdd(cp, -gb);
+      } else {
+        atomicAdd(cp, -gb);
         atomicAdd(vbar + nb - 1, 2 * gb);
       }
-      __syncthreads();
     }
-    for (int q = threadIdx.x; q < per_face; q += blockDim.x) {
-      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-      int g[3] = {0, 0, 0}, c[3] = {0, 0, 0};
-      g[d] = gix;
-      c[d] = cix;
-      g[ta] = c[ta] = q % nc + 1;
-      if (D == 3) g[tb] = c[tb] = q / nc + 1;
-      double* gp = L + cidx<D>(nc, g[0], g[1], g[2]);
-      const double gb = *gp;
-      *gp = 0;
-      if (gb == 0) continue;
-      if (nb_id > 0) {
-        // sides_from_nb (m_aX_gc.f90:368-402): the neighbour's adjacent layer
-        int s[3] = {c[0], c[1], c[2]};
-        s[d] = low ? nc : 1;
-        atomicAdd(boxp(a, il, nb_id) + cidx<D>(nc, s[0], s[1], s[2]), gb);
-      } else if (nb_id == 0) {
-        // mg$D_sides_rb (m_aX_mg.f90:262-356): 0.5 times the parent's cell
-        // (a$D_gc_prolong0) plus an extrapolation from the box's own cells
-        const int pid = a.parent[id];
-        const int pnc = a.ncell[pid];
-        int off[3];
-        child_off<D>(a, id, nc, 0, off);
-        int pc[3] = {0, 0, 0};
-        for (int e = 0; e < D; e++) pc[e] = off[e] + ((g[e] + 1) >> 1);
-        atomicAdd(boxp(a, il, pid) + cidx<D>(pnc, pc[0], pc[1], pc[2]), 0.5 * gb);
-        atomicAdd(L + cidx<D>(nc, c[0], c[1], c[2]), (D == 2 ? 1.0 : 1.25) * gb);
-        int e1[3] = {c[0], c[1], c[2]};
-        e1[d] += dix;
-        atomicAdd(L + cidx<D>(nc, e1[0], e1[1], e1[2]), -0.25 * gb);
-        int e2[3] = {c[0], c[1], c[2]};
-        e2[ta] += -1 + 2 * (c[ta] & 1);
-        atomicAdd(L + cidx<D>(nc, e2[0], e2[1], e2[2]), -0.25 * gb);
-        if (D == 3) {
-          int e3[3] = {c[0], c[1], c[2]};
-          e3[tb] += -1 + 2 * (c[tb] & 1);
-          atomicAdd(L + cidx<D>(nc, e3[0], e3[1], e3[2]), -0.25 * gb);
-        }
-      } else {
-        // the drivers' rules: copy, -c, 2V - c
-        const int kind = a.bc_kind[nb - 1];
-        double* cp = L + cidx<D>(nc, c[0], c[1], c[2]);
-        if (kind == kBcCopy) {
-          atomicAdd(cp, gb);
-        } else if (kind == kBcNeg) {
-          atomicAdd(cp, -gb);
-        } else {
-          atomicAdd(cp, -gb);
-          atomicAdd(vbar + nb - 1, 2 * gb);
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -3282,8 +3291,10 @@ template <int D>
 static void adj_gc_level(Pool& P, const Dev& a, const AdjMg& m, int lvl) {
   const int n = P.n_ids(lvl);
   if (!n) return;
-  hipLaunchKernelGGL(k_adj_gc<D>, dim3(n), dim3(256), 0, P.stream, a, P.ids(lvl), n,
-                     P.nc_lvl(lvl), m.lp, m.vbar);
+  const int nc = P.nc_lvl(lvl);
+  const Geo g = geo(n, 2 * D * (D == 2 ? nc : nc * nc));
+  hipLaunchKernelGGL(k_adj_gc<D>, dim3(g.grid), dim3(256), 0, P.stream, a, P.ids(lvl), n, nc,
+                     g.tpb, m.lp, m.vbar);
   post_launch("adj_gc");
 }
 
@@ -3433,13 +3444,20 @@ void fmg_adjoint(Pool& P, const int* adj, double* vbar_host) {
   const int nv = P.tree.n_cc;
   for (int k = 0; k < 4; k++)
     if (adj[k] < 1 || adj[k] > nv) throw std::out_of_range("fmg_adjoint: bad adjoint variable");
-  double* d_vbar = (double*)P.work(8 * sizeof(double));
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, P.stream, (unsigned long long*)d_vbar, 8,
-                     0ull);
-  post_launch("fill");
+  // the adjoints of V in a buffer of their own, kept until release (a
+  // captured graph holds its address)
+  if (!P.d_vbar) check(hipMalloc((void**)&P.d_vbar, 8 * sizeof(double)), "hipMalloc vbar");
+  double* d_vbar = P.d_vbar;
   const AdjMg m{adj[0], adj[1], adj[2], adj[3], d_vbar};
-  if (P.tree.dim == 2) fmg_adjoint_t<2>(P, m);
-  else fmg_adjoint_t<3>(P, m);
+  // the launches depend on the tree, its lists and the variables only: one
+  // graph per variable set, replayed on later calls
+  run_captured(P, fnv(call_key(9, adj[0], adj[1], adj[2]), (uint64_t)adj[3]), [&] {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, P.stream, (unsigned long long*)d_vbar,
+                       8, 0ull);
+    post_launch("fill");
+    if (P.tree.dim == 2) fmg_adjoint_t<2>(P, m);
+    else fmg_adjoint_t<3>(P, m);
+  });
   check(hipMemcpyAsync(vbar_host, d_vbar, 2 * P.tree.dim * sizeof(double), hipMemcpyDeviceToHost,
                        P.stream),
         "vbar");

@@ -238,6 +238,9 @@ struct Pool {
   // 2 kRedSlots reduction keys per independent solve (allocated once and
   // kept: captured graphs hold the pointer)
   unsigned long long* d_term_red = nullptr;
+  // adjoints of the boundary values of the multigrid's reverse mode (8
+  // doubles, allocated once and kept: a captured graph holds the pointer)
+  double* d_vbar = nullptr;
   void ensure_term_red();
 
   // instances (a5g_set_instances): the device level lists hold only the

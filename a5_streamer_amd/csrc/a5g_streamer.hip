@@ -478,19 +478,102 @@ __device__ __forceinline__ void lt_col_adj(const Tbl& t, double* tg, int col, in
   atomicAdd(g + li, (1 - lf) * lam);
 }
 
+// Table-row adjoints of two columns (c0, c1) gathered per workgroup: every
+// cell of a level feeds the few rows its field falls in, so global atomics
+// from all lanes would meet on a handful of addresses. With the two
+// columns' rows in LDS (lds, 2 n_rows doubles: kTgLdsRows rows at most),
+// the lanes add there, each lane keeps the range of rows it touched, and
+// the workgroup adds the touched range to the global rows once at the end
+// (tg_flush). Without LDS room the rows are added to global memory
+// directly.
+constexpr int kTgLdsRows = 2048;
+struct TgAcc {
+  double* lds;  // nullptr: global atomics
+  double* tg;
+  int n_rows, c0, c1;
+  int lo = 1 << 30, hi = -1;  // rows this lane touched (0-based)
+  __device__ void add(int which, int li, double lf, double lam) {
+    if (lam == 0.0) return;
+    if (!lds) {
+      double* g = tg + (size_t)((which ? c1 : c0) - 1) * n_rows;
+      atomicAdd(g + li - 1, lf * lam);
+      atomicAdd(g + li, (1 - lf) * lam);
+      return;
+    }
+    double* g = lds + which * n_rows;
+    atomicAdd(g + li - 1, lf * lam);
+    atomicAdd(g + li, (1 - lf) * lam);
+    lo = min(lo, li - 1);
+    hi = max(hi, li);
+  }
+};
+
+// zero the LDS rows and the workgroup's row range (all lanes, then a barrier)
+__device__ __forceinline__ void tg_begin(double* lds, int n_rows, int* range) {
+  if (lds)
+    for (int q = threadIdx.x; q < 2 * n_rows; q += blockDim.x) lds[q] = 0.0;
+  if (threadIdx.x == 0) {
+    range[0] = 1 << 30;
+    range[1] = -1;
+  }
+  __syncthreads();
+}
+
+// the workgroup's touched rows to global memory (all lanes)
+__device__ __forceinline__ void tg_flush(TgAcc& acc, int* range) {
+  if (!acc.lds) return;
+  if (acc.hi >= 0) {
+    atomicMin(range, acc.lo);
+    atomicMax(range + 1, acc.hi);
+  }
+  __syncthreads();
+  const int lo = range[0], hi = range[1];
+  const int w = hi - lo + 1;
+  for (int q = threadIdx.x; q < 2 * max(w, 0); q += blockDim.x) {
+    const int which = q / w, r = lo + q - which * w;
+    const double v = acc.lds[which * acc.n_rows + r];
+    if (v != 0.0) atomicAdd(acc.tg + (size_t)((which ? acc.c1 : acc.c0) - 1) * acc.n_rows + r, v);
+  }
+}
+
 // update_solution reverse: one lane per leaf cell. le, li hold the adjoints
 // of n_e', n_i' on entry and those of n_e, n_i on exit; each face of lF gets
 // the contributions of the (one or two) cells beside it, so with lF zero on
 // entry the result does not depend on their order.
+template <int D, bool CYL>
+__device__ void update_adj_box(const Dev& a, const Tbl& tb, int id, int nc, int tpb, int lt,
+                               int ifld, int ieps, int fel, int calpha, int ceta, int diel,
+                               int photoi, int le, int li, int lE, int lph, int lF, TgAcc& acc,
+                               double dt_all);
+
 template <int D, bool CYL>
 __global__ __launch_bounds__(256) void k_update_adj(Dev a, Tbl tb, const int* __restrict__ list,
                                                     int n, int nc, int tpb, int ifld, int ieps,
                                                     int fel, int calpha, int ceta, int diel,
                                                     int photoi, int le, int li, int lE, int lph,
                                                     int lF, double* __restrict__ tg,
-                                                    double dt_all) {
-  BOXES_PER_BLOCK
-  if (!act) return;
+                                                    double dt_all, int use_lds) {
+  extern __shared__ double s_tg[];
+  __shared__ int s_range[2];
+  TgAcc acc{use_lds ? s_tg : nullptr, tg, tb.n_rows, calpha, ceta};
+  tg_begin(acc.lds, tb.n_rows, s_range);
+  const int bpb = blockDim.x / tpb;
+  const int slot = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot * tpb;
+  const int n_groups = (n + bpb - 1) / bpb;
+  for (int grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const int li_ = grp * bpb + slot;
+    if (li_ < n) update_adj_box<D, CYL>(a, tb, list[li_], nc, tpb, lt, ifld, ieps, fel, calpha,
+                                        ceta, diel, photoi, le, li, lE, lph, lF, acc, dt_all);
+  }
+  tg_flush(acc, s_range);
+}
+
+template <int D, bool CYL>
+__device__ void update_adj_box(const Dev& a, const Tbl& tb, int id, int nc, int tpb, int lt,
+                               int ifld, int ieps, int fel, int calpha, int ceta, int diel,
+                               int photoi, int le, int li, int lE, int lph, int lF, TgAcc& acc,
+                               double dt_all) {
   const double dt = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstDt] : dt_all;
   const double inv_dr = 1 / a.dr[id];
   const double* E = boxp(a, ifld, id);
@@ -535,8 +618,8 @@ __global__ __launch_bounds__(256) void k_update_adj(Dev a, Tbl tb, const int* __
     const double lal = lsrc * 0.5 * nrm, let = -lsrc * 0.5 * nrm;
     if (Lph) Lph[c] += lsrc;
     LEf[c] += lal * lt_dcol(tb, calpha, E[c], ix) + let * lt_dcol(tb, ceta, E[c], ix);
-    lt_col_adj(tb, tg, calpha, ix, lf, lal);
-    lt_col_adj(tb, tg, ceta, ix, lf, let);
+    acc.add(0, ix, lf, lal);
+    acc.add(1, ix, lf, let);
     for (int d = 0; d < D; d++) {
       const double ldf = nrm > 0 ? lnrm * df[d] / nrm : 0.0;
       double wl = 1.0, wh = 1.0;
@@ -671,13 +754,36 @@ __device__ __forceinline__ void koren_mlim_d(double x, double y, double& dx, dou
 
 // fluxes_koren reverse: one lane per leaf face; consumes lF (clearing it)
 template <int D>
+__device__ void koren_adj_box(const Dev& a, const Tbl& tb, int id, int nc, int tpb, int lt,
+                              int ie, int ifld, int ffld, int cmob, int cdif, int le, int lE,
+                              int lF, int lfld, TgAcc& acc, double fac_all);
+
+template <int D>
 __global__ __launch_bounds__(256) void k_koren_adj(Dev a, Tbl tb, const int* __restrict__ list,
                                                    int n, int nc, int tpb, int ie, int ifld,
                                                    int ffld, int cmob, int cdif, int le, int lE,
                                                    int lF, int lfld, double* __restrict__ tg,
-                                                   double fac_all) {
-  BOXES_PER_BLOCK
-  if (!act) return;
+                                                   double fac_all, int use_lds) {
+  extern __shared__ double s_tg[];
+  __shared__ int s_range[2];
+  TgAcc acc{use_lds ? s_tg : nullptr, tg, tb.n_rows, cmob, cdif};
+  tg_begin(acc.lds, tb.n_rows, s_range);
+  const int bpb = blockDim.x / tpb;
+  const int slot = threadIdx.x / tpb;
+  const int lt = threadIdx.x - slot * tpb;
+  const int n_groups = (n + bpb - 1) / bpb;
+  for (int grp = blockIdx.x; grp < n_groups; grp += gridDim.x) {
+    const int li_ = grp * bpb + slot;
+    if (li_ < n) koren_adj_box<D>(a, tb, list[li_], nc, tpb, lt, ie, ifld, ffld, cmob, cdif, le,
+                                  lE, lF, lfld, acc, fac_all);
+  }
+  tg_flush(acc, s_range);
+}
+
+template <int D>
+__device__ void koren_adj_box(const Dev& a, const Tbl& tb, int id, int nc, int tpb, int lt,
+                              int ie, int ifld, int ffld, int cmob, int cdif, int le, int lE,
+                              int lF, int lfld, TgAcc& acc, double fac_all) {
   const double fac = a.n_inst ? a.inst_val[inst_of<D>(a, id) * kInstVals + kInstKorenFac]
                               : fac_all;
   const double inv_dr = 1 / a.dr[id];
@@ -770,8 +876,8 @@ __global__ __launch_bounds__(256) void k_koren_adj(Dev a, Tbl tb, const int* __r
     const double lx = lmob * lt_dcol(tb, cmob, fld_avg, li) + ldif * lt_dcol(tb, cdif, fld_avg, li);
     atomicAdd(LEf + ic, 0.5 * lx);
     atomicAdd(LEf + im, 0.5 * lx);
-    lt_col_adj(tb, tg, cmob, li, lf, lmob);
-    lt_col_adj(tb, tg, cdif, li, lf, ldif);
+    acc.add(0, li, lf, lmob);
+    acc.add(1, li, lf, ldif);
   }
 }
 
@@ -1336,6 +1442,20 @@ void table_adjoint(Pool& P, bool clear, double* out_host) {
   }
 }
 
+// Launch of the reverse kernels that gather table-row adjoints: with the
+// two columns' rows fitting in LDS, 4 workgroups per compute unit at most
+// walk the box groups (so each workgroup adds its touched rows to global
+// memory once for many boxes); otherwise the one-shot grid with global
+// atomics.
+struct TgLaunch {
+  int grid, use_lds;
+  size_t lds_bytes;
+};
+static TgLaunch tg_launch(const Pool& P, int grid) {
+  if (P.tbl_rows > kTgLdsRows) return {grid, 0, 0};
+  return {std::min(grid, 4 * P.n_cu), 1, (size_t)2 * P.tbl_rows * sizeof(double)};
+}
+
 void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av) {
   P.sync_topology();
   const int D = P.tree.dim;
@@ -1348,10 +1468,12 @@ void update_solution_adj(Pool& P, const StVars& v, double dt, const AdjVars& av)
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, box_cells(D, nc, false));
+    const TgLaunch tl = tg_launch(P, g.grid);
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb, P.leaves(lvl), n,
-                         nc, g.tpb, v.fld, v.eps, v.f_elec, v.t_alpha, v.t_eta,
-                         v.dielectric_rule, photoi, av.le, av.li, av.lE, av.lpho, av.lF, tg, dt);
+      hipLaunchKernelGGL(kern, dim3(tl.grid), dim3(256), tl.lds_bytes, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.fld, v.eps, v.f_elec, v.t_alpha, v.t_eta,
+                         v.dielectric_rule, photoi, av.le, av.li, av.lE, av.lpho, av.lF, tg, dt,
+                         tl.use_lds);
     };
     if (D == 3) go(k_update_adj<3, false>);
     else if (cyl) go(k_update_adj<2, true>);
@@ -1394,10 +1516,11 @@ void fluxes_koren_adj(Pool& P, const StVars& v, double dt, const AdjVars& av) {
     if (!n) continue;
     const int nc = P.nc_lvl(lvl);
     G2 g = g2(n, D * (nc + 1) * (D == 2 ? nc : nc * nc));
+    const TgLaunch tl = tg_launch(P, g.grid);
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(g.grid), dim3(256), 0, P.stream, P.dev(), tb, P.leaves(lvl), n,
-                         nc, g.tpb, v.elec, v.fld, v.f_fld, v.t_mob, v.t_dif, av.le, av.lE, av.lF,
-                         av.lfld, tg, fac);
+      hipLaunchKernelGGL(kern, dim3(tl.grid), dim3(256), tl.lds_bytes, P.stream, P.dev(), tb,
+                         P.leaves(lvl), n, nc, g.tpb, v.elec, v.fld, v.f_fld, v.t_mob, v.t_dif,
+                         av.le, av.lE, av.lF, av.lfld, tg, fac, tl.use_lds);
     };
     if (D == 2) go(k_koren_adj<2>);
     else go(k_koren_adj<3>);

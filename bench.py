@@ -628,7 +628,7 @@ def adjoint_window(args, steps=100, every=10, fd_dirs=2):
         a = float(np.sum(gt * dtb)) + gv * dv
         sc = float(np.sum(np.abs(gt * dtb))) + abs(gv * dv)
         fd.append({"fd": f, "adjoint": a, "rel_err_of_scale": abs(f - a) / sc})
-    slots = win.SLOT_CKPT + win.n_seg + every
+    slots = win.store_slots()
     n_boxes = int(np.sum(T.boxes()["in_use"]))
     T.close()
     return {
@@ -645,7 +645,7 @@ def adjoint_window(args, steps=100, every=10, fd_dirs=2):
         "backward_ms_per_step": round(t_bwd * 1e3 / steps, 3),
         "backward_over_forward": round(t_bwd / t_fwd, 2),
         "store_slots": slots,
-        "loss": {"electron_count": float(count), "head_position_m": float(head)},
+        "loss": {"electron_count": float(count.detach()), "head_position_m": float(head.detach())},
         "grad": {"voltage": gv, "mobility": g_mob,
                  "alpha_table_norm": float(np.linalg.norm(gt[St.T_ALPHA - 1]))},
         "central_differences": fd,
