@@ -817,7 +817,7 @@ module m_a5g
   public :: a2_tree_max_cc, a3_tree_max_cc, a2_tree_min_cc, a3_tree_min_cc
   public :: a2_consistent_fluxes, a3_consistent_fluxes, a2_prolong1_to, a3_prolong1_to
   public :: mg2_init_mg, mg3_init_mg, mg2_fas_fmg, mg3_fas_fmg
-  public :: mg2_fas_vcycle, mg3_fas_vcycle
+  public :: mg2_fas_vcycle, mg3_fas_vcycle, mg_fas_fmg_adjoint
   public :: a5g_max_id, a5g_max_lvl, a5g_min_dr, a5g_level_leaves, a5g_box_info
   public :: a5g_set_cc, a5g_get_cc, a5g_fill_cc, a5g_sync
   public :: a5g_mixed_solve, a5g_residual_norms, a5g_set_bc_value
@@ -1086,6 +1086,23 @@ contains
     logical, intent(in)        :: set_residual, first_call
     call check(c_fas_fmg(tree%h, lg(set_residual), lg(first_call)), "mg$D_fas_fmg")
   end subroutine t_fas_fmg
+
+  !> Reverse mode of mg$D_fas_fmg(tree, mg, .false., .false.): adj = cell
+  !> variables lp, lt, lr, lw holding the adjoints of the potential, tmp and
+  !> rhs the solve leaves (lw: work space); on return those of the values
+  !> it started from, and vbar(2 dim) the adjoints of the face values V.
+  !> No counterpart in the reference.
+  subroutine mg_fas_fmg_adjoint(tree, adj, vbar)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: adj(4)
+    real(dp), intent(out)      :: vbar(:)
+    integer(c_int)             :: a(4)
+    real(c_double)             :: v(6)
+    a = adj
+    v = 0
+    call check(c_fas_fmg_adjoint(tree%h, a, v), "mg$D_fas_fmg adjoint")
+    vbar = v(1:size(vbar))
+  end subroutine mg_fas_fmg_adjoint
 
   !> mg$D_fas_vcycle (m_aX_mg.f90:208-255)
   subroutine t_fas_vcycle(tree, mg, max_lvl, set_residual)

@@ -102,3 +102,28 @@ def test_run_matches_oracle(oracle_lib, name):
         assert changed > 0
     if name == "2d_modulated":
         assert runs[1].voltage != -runs[1].c.domain_len * runs[1].c.applied_fld
+
+
+def test_2d_workload_as_specified(oracle_lib):
+    """The 2D workload as its configuration states it: 32 mm square of 4 x 4
+    base boxes of 8 x 8 cells, refinement to lvls_max 9 (3.9 um), 2.5 MV/m,
+    the 1e20 m^-3 seed of width 0.4 mm at (16, 28) mm over 1e15 m^-3, the
+    air transport table, no photoionization: 200 time steps (100 blocks of
+    steps_amr = 2, each with its mesh update), the whole state and tree
+    compared with the oracle bit for bit every 20 steps."""
+    import a5_streamer_amd as A
+    print("2d workload, 200 steps", flush=True)
+    runs = [S.StreamerRun(M, S.Config("2d")) for M in (O.OStreamTree, A.Tree)]
+    for r in runs:
+        r.initialise()
+    _same_state(runs[0].T, runs[1].T, 2)
+    for blk in range(100):
+        res = [r.step_block() for r in runs]
+        assert res[0] == res[1], (blk, res)
+        assert runs[0].dt == runs[1].dt and runs[0].time == runs[1].time
+        if blk % 10 == 9:
+            _same_state(runs[0].T, runs[1].T, 2)
+            print("  step %d: t = %.4e s, %d boxes, max level %d" % (
+                runs[1].step_no, runs[1].time, runs[1].T.info()["max_id"],
+                runs[1].T.info()["max_lvl"]), flush=True)
+    assert runs[1].step_no == 200
