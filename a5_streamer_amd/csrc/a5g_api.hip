@@ -526,6 +526,10 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // them to the next reader
       const char* dg = std::getenv("A5G_DEFER_GHOSTS");
       h->P.defer_on = !(dg && dg[0] == '0');
+      // A5G_SMALL_LEVELS=0 launches every half-sweep of the small coarse
+      // levels instead of one workgroup per smoothing call
+      const char* sl = std::getenv("A5G_SMALL_LEVELS");
+      h->P.small_on = !(sl && sl[0] == '0');
       const char* sb = std::getenv("A5G_STREAM_BYTES");
       if (sb && sb[0]) {
         char* end = nullptr;

@@ -449,6 +449,144 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
 
 // One half-sweep in launch shape `shape` (kShapes: bit 0 persistent grid,
 // bit 1 half the lanes per box)
+// ---------------------------------------------------------------------------
+// A whole smoothing call (gsrb_boxes: 2 n_cycle half-sweeps, each followed by
+// the ghost pass, and the final stored ghost pass when store) on a level
+// small enough that its potential and rhs fit in the LDS of one workgroup:
+// the coarse levels of a V-cycle, where a launch per half-sweep costs more
+// than the work. The level is loaded once (the potential with its stored
+// ghost layer, the interior of the rhs), every half-sweep and ghost pass
+// runs in LDS with a barrier between them, and the interior (and the ghost
+// layer when store) goes back at the end. Each value is computed from the
+// same operands in the same order as by k_gsrb and k_gc, so the result is
+// bit for bit theirs. For levels whose boxes have no refinement boundary,
+// no permittivity tags and the face rules copy, -c and 2V - c (the host
+// checks that: small_level_ok).
+// LDS: the potential, lexicographic per box ((nc+2)^D per box), then the
+// rhs interiors (nc^D per box), then the neighbour slots (2 D per box, -1 a
+// physical face).
+// ---------------------------------------------------------------------------
+constexpr size_t kSmallLevelLds = 64 * 1024;
+
+template <int D, bool CYL, typename T>
+__global__ __launch_bounds__(1024) void k_gsrb_small(Dev a, const int* __restrict__ list, int n,
+                                                     int nc, int n_half, int first_fused,
+                                                     int store, int i_phi, int i_rhs) {
+  extern __shared__ __align__(16) unsigned char small_lds[];
+  const int m = nc + 2;
+  const int bl = D == 2 ? m * m : m * m * m;
+  const int ni = D == 2 ? nc * nc : nc * nc * nc;
+  const int nf = D == 2 ? nc : nc * nc;  // cells of a face
+  T* sp = (T*)small_lds;
+  T* sr = sp + (size_t)n * bl;
+  int* snb = (int*)(sr + (size_t)n * ni);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  auto L = [&](int i, int j, int k) { return D == 2 ? i + m * j : i + m * (j + m * k); };
+  // load: the whole potential block, the rhs interior, the neighbour slots
+  for (int q = tid; q < n * bl; q += nt) {
+    const int b = q / bl, c = q - b * bl;
+    const int i = c % m, j = (c / m) % m, k = D == 3 ? c / (m * m) : 0;
+    sp[q] = boxp_t<T>(a, i_phi, list[b])[cidx<D>(nc, i, j, k)];
+  }
+  for (int q = tid; q < n * ni; q += nt) {
+    const int b = q / ni, c = q - b * ni;
+    const int i = c % nc + 1, j = (c / nc) % nc + 1, k = D == 3 ? c / (nc * nc) + 1 : 0;
+    sr[q] = boxp_t<T>(a, i_rhs, list[b])[cidx_in<D>(nc, i, j, k)];
+  }
+  for (int q = tid; q < n * 2 * D; q += nt) {
+    const int b = q / (2 * D), nb = q - b * 2 * D + 1;
+    const int nb_id = a.nbs[list[b] * 6 + nb - 1];
+    int slot = -1;
+    if (nb_id > 0)
+      for (int o = 0; o < n; o++)
+        if (list[o] == nb_id) slot = o;
+    snb[q] = slot;
+  }
+  __syncthreads();
+  // the face ghosts of every box from the current state (k_gc's rules for
+  // a box with same-level neighbours and physical faces)
+  auto ghosts = [&]() {
+    for (int q = tid; q < n * 2 * D * nf; q += nt) {
+      const int b = q / (2 * D * nf), r = q - b * 2 * D * nf;
+      const int nb = r / nf + 1, t = r - (nb - 1) * nf;
+      const int d = d_nb_dim(nb);
+      const bool low = d_nb_low(nb);
+      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int g[3] = {0, 0, 0};
+      g[d] = low ? 0 : nc + 1;
+      g[ta] = t % nc + 1;
+      if (D == 3) g[tb] = t / nc + 1;
+      const int ns = snb[b * 2 * D + nb - 1];
+      T v;
+      if (ns >= 0) {
+        int sc[3] = {g[0], g[1], g[2]};
+        sc[d] = low ? nc : 1;
+        v = sp[(size_t)ns * bl + L(sc[0], sc[1], sc[2])];
+      } else {
+        int c[3] = {g[0], g[1], g[2]};
+        c[d] = low ? 1 : nc;
+        const T cv = sp[(size_t)b * bl + L(c[0], c[1], c[2])];
+        const int kind = a.bc_kind[nb - 1];
+        v = kind == 0 ? cv : (kind == 1 ? -cv : T(2) * (T)a.bc_val[nb - 1] - cv);
+      }
+      sp[(size_t)b * bl + L(g[0], g[1], g[2])] = v;
+    }
+  };
+  const int half = nc >> 1;
+  const int ncol = ni >> 1;
+  for (int cntr = 1; cntr <= n_half; cntr++) {
+    if (cntr > 1 || first_fused) {
+      ghosts();
+      __syncthreads();
+    }
+    for (int q = tid; q < n * ncol; q += nt) {
+      const int b = q / ncol, w = q - b * ncol;
+      const int id = list[b];
+      const int mm = w % half, row = w / half;
+      const int j = row % nc + 1;
+      const int k = D == 3 ? row / nc + 1 : 0;
+      const int i0 = 2 - ((cntr ^ (D == 3 ? k + j : j)) & 1);
+      const int i = i0 + 2 * mm;
+      const T* u = sp + (size_t)b * bl;
+      auto nbv = [&](int di, int dj, int dk) { return u[L(i + di, j + dj, k + dk)]; };
+      const T dx2 = (T)(a.dr[id] * a.dr[id]);
+      const T rhs = sr[(size_t)b * ni + (i - 1) + nc * ((j - 1) + (D == 3 ? nc * (k - 1) : 0))];
+      T num;
+      if (D == 2) {
+        if (CYL) {
+          const int ioff = (a.ix[id * 3] - 1) * nc;
+          const double den = i + ioff - 0.5;
+          const T rf1 = (T)((double)(i + ioff - 1) / den);
+          const T rf2 = (T)((double)(i + ioff) / den);
+          num = rf1 * nbv(-1, 0, 0) + rf2 * nbv(1, 0, 0) + nbv(0, 1, 0) + nbv(0, -1, 0) -
+                dx2 * rhs;
+        } else {
+          num = nbv(1, 0, 0) + nbv(-1, 0, 0) + nbv(0, 1, 0) + nbv(0, -1, 0) - dx2 * rhs;
+        }
+      } else {
+        num = nbv(1, 0, 0) + nbv(-1, 0, 0) + nbv(0, 1, 0) + nbv(0, -1, 0) + nbv(0, 0, 1) +
+              nbv(0, 0, -1) - dx2 * rhs;
+      }
+      T* pc = sp + (size_t)b * bl + L(i, j, k);
+      if (a.helm != 0) *pc = num / (T(2 * D) + (T)a.helm * dx2);
+      else *pc = (D == 2 ? T(0.25) : T(1) / T(6)) * num;
+    }
+    __syncthreads();
+  }
+  if (store) {
+    ghosts();
+    __syncthreads();
+  }
+  // back to HBM: the interior, and the face ghosts when they were stored
+  for (int q = tid; q < n * bl; q += nt) {
+    const int b = q / bl, c = q - b * bl;
+    const int i = c % m, j = (c / m) % m, k = D == 3 ? c / (m * m) : 0;
+    const int out = (i < 1 || i > nc) + (j < 1 || j > nc) + (D == 3 && (k < 1 || k > nc));
+    if (out == 0 || (store && out == 1))
+      boxp_t<T>(a, i_phi, list[b])[cidx<D>(nc, i, j, k)] = sp[q];
+  }
+}
+
 template <int D, bool CYL, int NC, typename T, bool EPS, bool NT>
 static void go_gsrb(Pool& P, const Dev& a, const int* list, int n, int cntr, int fused,
                     int i_phi, int i_rhs, int shape) {
@@ -1605,6 +1743,27 @@ static void fill_gc_phi(Pool& P, const Ctx& c, int lvl) {
 // skipped a ghost pass that would have stored exactly what is recomputed).
 // store = false leaves the stored ghost layer as it was, for callers whose
 // next reader of these ghosts recomputes them (the residual pass).
+// Whether a smoothing call on lvl can run in one workgroup's LDS
+// (k_gsrb_small): its boxes' potential and rhs fit, none has a refinement
+// boundary or a permittivity tag, no run of instances, and the face rules
+// are copy, -c and 2V - c. Returns the LDS bytes (0: no).
+template <typename T>
+static size_t small_level_lds(const Pool& P, const Ctx& c, const Dev& a, int lvl, int n, int nc) {
+  if (!P.small_on || a.n_inst) return 0;
+  const int D = P.tree.dim;
+  const size_t bytes = (size_t)n * (cells_of(D, nc, true) + cells_of(D, nc, false)) * sizeof(T) +
+                       (size_t)n * 2 * D * sizeof(int);
+  if (bytes > kSmallLevelLds) return 0;
+  for (int nb = 1; nb <= 2 * D; nb++)
+    if (P.bc_kind[nb - 1] == kBcRow2V) return 0;
+  for (int id : P.lids(lvl)) {
+    if (c.eps && P.tree.tag[id] > kTagNormal) return 0;
+    for (int nb = 0; nb < 2 * D; nb++)
+      if (P.tree.nbs[id * 6 + nb] == 0) return 0;
+  }
+  return bytes;
+}
+
 template <int D, typename T>
 static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_fused = false,
                        bool store = true) {
@@ -1621,6 +1780,24 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
   const int pr = n_cycle > 0
                      ? P.prof_begin(PK_GSRB, lvl, 2 * n_cycle * 1.5 * sizeof(T) * cells, 2 * n_cycle)
                      : -1;
+  if (n_cycle > 0) {
+    const size_t lds = small_level_lds<T>(P, c, a, lvl, n, nc);
+    if (lds > 0) {
+      const int work = std::max(n * ncol, n * 2 * D * (D == 2 ? nc : nc * nc));
+      const int nt = std::min(1024, std::max(64, pow2ceil(work)));
+      if (c.cyl)
+        hipLaunchKernelGGL((k_gsrb_small<2, true, T>), dim3(1), dim3(nt), lds, P.stream, a,
+                           P.ids(lvl), n, nc, 2 * n_cycle, first_fused ? 1 : 0, store ? 1 : 0,
+                           c.i_phi, c.i_rhs);
+      else
+        hipLaunchKernelGGL((k_gsrb_small<D, false, T>), dim3(1), dim3(nt), lds, P.stream, a,
+                           P.ids(lvl), n, nc, 2 * n_cycle, first_fused ? 1 : 0, store ? 1 : 0,
+                           c.i_phi, c.i_rhs);
+      post_launch("gsrb_small");
+      P.prof_end(pr);
+      return;
+    }
+  }
   for (int it = 1; it <= 2 * n_cycle; it++) {
     const int fused = it > 1 || first_fused;
     const bool done =

@@ -223,6 +223,9 @@ struct Pool {
   int defer_lvl = kNoDefer;
   int defer_iv = -1;
   bool defer_on = true;
+  // smoothing calls of small levels in one workgroup's LDS (k_gsrb_small);
+  // A5G_SMALL_LEVELS=0 at a5g_init launches every half-sweep instead
+  bool small_on = true;
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   // waits for every stream of the handle, then destroys the graphs (one may

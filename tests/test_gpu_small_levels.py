@@ -1,0 +1,93 @@
+"""Smoothing of small coarse levels in one workgroup's LDS (k_gsrb_small:
+all half-sweeps and ghost passes of one smoothing call in one launch)
+against the same calls with every half-sweep launched on its own
+(A5G_SMALL_LEVELS=0 at a5g_init): FMG, V-cycles, the mixed-precision solve
+and a Helmholtz solve leave every variable of every box, ghosts included,
+equal bit for bit. The trees hold coarse levels small enough for the LDS
+path (the coarsened levels below the base and small base levels) and
+larger ones that keep the per-half-sweep launches; with permittivity
+tags, a refinement boundary or the whole-row rule on a level the per-
+half-sweep launches run there. Both paths are also bit for bit the
+oracle's (tests/test_gpu_mg.py, tests/test_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+
+import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # dim, nc, coord, base, face rules
+    (2, 8, 0, (4, 4, 1), (0, 0, 1, 2)),
+    (2, 8, 1, (1, 2, 1), (0, 0, 1, 2)),
+    (2, 8, 1, (1, 2, 1), (0, 0, 1, 3)),
+    (3, 8, 0, (2, 2, 2), (0, 0, 0, 0, 1, 2)),
+    (3, 16, 0, (2, 2, 2), (0, 1, 0, 2, 1, 2)),
+    (2, 4, 0, (2, 2, 1), (1, 1, 2, 2)),
+]
+
+
+def _tree(dim, nc, coord, base, kinds, small, eps=False):
+    import a5_streamer_amd as A
+    os.environ["A5G_SMALL_LEVELS"] = "1" if small else "0"
+    try:
+        T = A.Tree(dim, nc, 5, 1, 1.0 / nc, coarsen_to=2, coord=coord)
+    finally:
+        del os.environ["A5G_SMALL_LEVELS"]
+
+    def rules(lvl, c):
+        if lvl < 2:
+            return 1
+        return 1 if (lvl < 4 and np.linalg.norm(c - 0.3) < 0.2) else 0
+    S.build_tree(T, dim, nc, rules, n_base=base)
+    rng = np.random.default_rng(5)
+    n, bl = T.info()["max_id"], T.box_len(nc)
+    T.set_cc_all(3, rng.standard_normal((n, bl)))
+    T.set_cc_all(1, np.zeros((n, bl)))
+    T.set_cc_all(5, np.where(rng.random((n, bl)) < 0.05, 2.0, 1.0) if eps else np.ones((n, bl)))
+    T.mg_setup(1, 2, 3, i_eps=5 if eps else -1, n_base=8, bc_kind=np.array(kinds, np.int32),
+               bc_val=np.arange(2 * dim, dtype=float) + 0.5, auto_ops=True)
+    return T
+
+
+def _run(T, kind):
+    mx = T.info()["max_lvl"]
+    if kind == "fmg":
+        T.fas_fmg(True, True)
+        T.fas_vcycle(mx, True)
+        T.fas_fmg(False, False)
+    elif kind == "mixed":
+        T.solve_mixed(1e-9, 20, 2, True)
+    else:
+        T.mg_set_helmholtz(30.0)
+        T.fas_fmg(False, True)
+        T.fas_vcycle(mx, False)
+    return {iv: T.get_cc_all(iv) for iv in (1, 2, 3)}
+
+
+@pytest.mark.parametrize("kind", ["fmg", "mixed", "helmholtz"])
+@pytest.mark.parametrize("dim,nc,coord,base,kinds", CASES)
+def test_small_levels_match_per_half_sweep_launches(dim, nc, coord, base, kinds, kind):
+    print("small levels", dim, nc, coord, base, kinds, kind, flush=True)
+    if kind == "helmholtz" and coord == 1:
+        pytest.skip("the Helmholtz operator is Cartesian")
+    out = []
+    for small in (True, False):
+        T = _tree(dim, nc, coord, base, kinds, small)
+        out.append(_run(T, kind))
+        T.close()
+    for iv in (1, 2, 3):
+        np.testing.assert_array_equal(out[0][iv], out[1][iv], err_msg="variable %d" % iv)
+
+
+def test_small_levels_with_permittivity_tags():
+    print("small levels with eps", flush=True)
+    out = []
+    for small in (True, False):
+        T = _tree(2, 8, 0, (4, 4, 1), (0, 0, 1, 2), small, eps=True)
+        out.append(_run(T, "fmg"))
+        T.close()
+    for iv in (1, 2, 3):
+        np.testing.assert_array_equal(out[0][iv], out[1][iv], err_msg="variable %d" % iv)
