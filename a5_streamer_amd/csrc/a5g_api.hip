@@ -530,6 +530,14 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // levels instead of one workgroup per smoothing call
       const char* sl = std::getenv("A5G_SMALL_LEVELS");
       h->P.small_on = !(sl && sl[0] == '0');
+      const char* sx = std::getenv("A5G_SMALL_LDS");
+      if (sx && sx[0]) {
+        char* end = nullptr;
+        const long v = std::strtol(sx, &end, 10);
+        if (end == sx || *end != 0 || v < 0)
+          throw std::invalid_argument("A5G_SMALL_LDS must be a byte count >= 0");
+        h->P.small_lds = v;
+      }
       const char* sb = std::getenv("A5G_STREAM_BYTES");
       if (sb && sb[0]) {
         char* end = nullptr;
@@ -541,6 +549,15 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       a5g::hcheck(hipMalloc((void**)&h->P.d_red, 2 * a5g::kRedSlots * sizeof(unsigned long long)),
                   "hipMalloc");
       a5g::noalloc_init();
+      {
+        int d0 = 0, lds = 0;
+        if (hipGetDevice(&d0) == hipSuccess &&
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, d0) ==
+                hipSuccess &&
+            lds > 0)
+          h->P.max_lds = lds;
+        (void)hipGetLastError();
+      }
       int dev = 0, n_cu = 0;
       if (hipGetDevice(&dev) == hipSuccess &&
           hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) ==

@@ -459,14 +459,15 @@ __global__ __launch_bounds__(256) void k_gsrb_lds(Dev a, const int* __restrict__
 // runs in LDS with a barrier between them, and the interior (and the ghost
 // layer when store) goes back at the end. Each value is computed from the
 // same operands in the same order as by k_gsrb and k_gc, so the result is
-// bit for bit theirs. For levels whose boxes have no refinement boundary,
-// no permittivity tags and the face rules copy, -c and 2V - c (the host
-// checks that: small_level_ok).
+// bit for bit theirs. For levels whose boxes have no refinement boundary
+// and no permittivity tags (the host checks that: small_level_lds). The
+// whole-row rule of a y face (2V - c on its corner ghosts too, from the x
+// ghosts of the same pass, as k_gc writes them) is applied in the final
+// stored pass.
 // LDS: the potential, lexicographic per box ((nc+2)^D per box), then the
 // rhs interiors (nc^D per box), then the neighbour slots (2 D per box, -1 a
 // physical face).
 // ---------------------------------------------------------------------------
-constexpr size_t kSmallLevelLds = 64 * 1024;
 
 template <int D, bool CYL, typename T>
 __global__ __launch_bounds__(1024) void k_gsrb_small(Dev a, const int* __restrict__ list, int n,
@@ -573,16 +574,31 @@ __global__ __launch_bounds__(1024) void k_gsrb_small(Dev a, const int* __restric
     }
     __syncthreads();
   }
+  // the corner ghosts of the whole-row rule on a y face of box b
+  auto row_corner = [&](int b, int nb) {
+    return D == 2 && snb[b * 2 * D + nb - 1] < 0 && a.bc_kind[nb - 1] == kBcRow2V;
+  };
   if (store) {
     ghosts();
     __syncthreads();
+    if (D == 2) {
+      for (int q = tid; q < n * 4; q += nt) {
+        const int b = q >> 2, nb = 3 + ((q >> 1) & 1), side = q & 1;
+        if (!row_corner(b, nb)) continue;
+        const int gx = side ? nc + 1 : 0, gy = nb == 3 ? 0 : nc + 1, cy = nb == 3 ? 1 : nc;
+        T* u = sp + (size_t)b * bl;
+        u[L(gx, gy, 0)] = T(2) * (T)a.bc_val[nb - 1] - u[L(gx, cy, 0)];
+      }
+      __syncthreads();
+    }
   }
-  // back to HBM: the interior, and the face ghosts when they were stored
+  // back to HBM: the interior, and the ghosts the stored pass wrote
   for (int q = tid; q < n * bl; q += nt) {
     const int b = q / bl, c = q - b * bl;
     const int i = c % m, j = (c / m) % m, k = D == 3 ? c / (m * m) : 0;
     const int out = (i < 1 || i > nc) + (j < 1 || j > nc) + (D == 3 && (k < 1 || k > nc));
-    if (out == 0 || (store && out == 1))
+    const bool corner = D == 2 && store && out == 2 && row_corner(b, j == 0 ? 3 : 4);
+    if (out == 0 || (store && out == 1) || corner)
       boxp_t<T>(a, i_phi, list[b])[cidx<D>(nc, i, j, k)] = sp[q];
   }
 }
@@ -1753,9 +1769,7 @@ static size_t small_level_lds(const Pool& P, const Ctx& c, const Dev& a, int lvl
   const int D = P.tree.dim;
   const size_t bytes = (size_t)n * (cells_of(D, nc, true) + cells_of(D, nc, false)) * sizeof(T) +
                        (size_t)n * 2 * D * sizeof(int);
-  if (bytes > kSmallLevelLds) return 0;
-  for (int nb = 1; nb <= 2 * D; nb++)
-    if (P.bc_kind[nb - 1] == kBcRow2V) return 0;
+  if (bytes > (size_t)std::min<long>(P.max_lds, P.small_lds)) return 0;
   for (int id : P.lids(lvl)) {
     if (c.eps && P.tree.tag[id] > kTagNormal) return 0;
     for (int nb = 0; nb < 2 * D; nb++)
@@ -1785,6 +1799,17 @@ static void gsrb_boxes(Pool& P, const Ctx& c, int lvl, int n_cycle, bool first_f
     if (lds > 0) {
       const int work = std::max(n * ncol, n * 2 * D * (D == 2 ? nc : nc * nc));
       const int nt = std::min(1024, std::max(64, pow2ceil(work)));
+      // more than 64 KB of dynamic LDS is granted per kernel on request
+      // (once per instantiation, before any capture can hold the launch)
+      auto allow = [&](const void* fn) {
+        static std::vector<const void*> done;
+        if (lds > 65536 && std::find(done.begin(), done.end(), fn) == done.end()) {
+          check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, P.max_lds),
+                "LDS of the small-level smoother");
+          done.push_back(fn);
+        }
+      };
+      allow(c.cyl ? (const void*)k_gsrb_small<2, true, T> : (const void*)k_gsrb_small<D, false, T>);
       if (c.cyl)
         hipLaunchKernelGGL((k_gsrb_small<2, true, T>), dim3(1), dim3(nt), lds, P.stream, a,
                            P.ids(lvl), n, nc, 2 * n_cycle, first_fused ? 1 : 0, store ? 1 : 0,

@@ -226,6 +226,11 @@ struct Pool {
   // smoothing calls of small levels in one workgroup's LDS (k_gsrb_small);
   // A5G_SMALL_LEVELS=0 at a5g_init launches every half-sweep instead
   bool small_on = true;
+  // LDS bytes one workgroup may use (hipDeviceAttributeMaxSharedMemoryPerBlock)
+  int max_lds = 64 * 1024;
+  // the most LDS a level of the small-level smoother may take
+  // (A5G_SMALL_LDS at a5g_init; default 64 KB)
+  long small_lds = 64 * 1024;
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   // waits for every stream of the handle, then destroys the graphs (one may
