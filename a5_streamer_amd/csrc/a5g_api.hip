@@ -530,6 +530,8 @@ int a5g_init(a5g_tree** t, int dim, int n_cell, int n_var_cell, int n_var_face, 
       // levels instead of one workgroup per smoothing call
       const char* sl = std::getenv("A5G_SMALL_LEVELS");
       h->P.small_on = !(sl && sl[0] == '0');
+      const char* rs = std::getenv("A5G_RSTR_SPLIT");
+      h->P.rstr_split = rs && rs[0] == '1';
       const char* sx = std::getenv("A5G_SMALL_LDS");
       if (sx && sx[0]) {
         char* end = nullptr;

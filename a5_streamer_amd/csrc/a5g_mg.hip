@@ -864,32 +864,55 @@ __global__ __launch_bounds__(256) void k_resid_rstr(Dev a, const int* __restrict
 // between), so that pass is not needed before this one.
 // HALVE (a launch shape of the autotuner's menu): half the lanes per box,
 // twice the boxes per workgroup.
-template <int D, bool CYL, int NC, typename T, bool EPS, bool HALVE = false>
+// SPLIT > 1 (3D): each box is shared by SPLIT workgroups, slab h taking the
+// fine planes k = h NC/SPLIT + 1 .. (h+1) NC/SPLIT and staging those planes
+// with one more on either side (the z neighbours of its first and last
+// plane) and the face ghosts they read. A slab holds whole pairs of planes
+// (odd, even), so the refinement-boundary extrapolation of its x and y face
+// ghosts reads planes inside it. The staging then takes (NC/SPLIT + 2)
+// planes instead of NC + 2: for 16^3 FP64 boxes 26 KB of LDS instead of
+// 47 KB, twice the workgroups per compute unit. Every value is the same.
+template <int D, bool CYL, int NC, typename T, bool EPS, bool HALVE = false, int SPLIT = 1>
 __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
                                                         int keep_res) {
+  static_assert(SPLIT == 1 || (D == 3 && (NC / 2) % SPLIT == 0), "z slabs of whole plane pairs");
   constexpr int N2 = NC + 2;
-  constexpr int BL = D == 2 ? N2 * N2 : N2 * N2 * N2;
-  constexpr int HALF = BL / 2;
+  constexpr int HALF = (D == 2 ? N2 * N2 : N2 * N2 * N2) / 2;
   constexpr int HNC = NC / 2;
+  constexpr int KS = NC / SPLIT;  // fine planes per slab
+  constexpr int BLS = D == 2 ? N2 * N2 : N2 * N2 * (SPLIT == 1 ? N2 : KS + 2);
   constexpr int NCOL = (D == 2 ? NC * NC : NC * NC * NC) / 2;
-  constexpr int TOTAL = D == 2 ? HNC * HNC : HNC * HNC * HNC;
+  constexpr int TOTAL = (D == 2 ? HNC * HNC : HNC * HNC * HNC) / SPLIT;
   constexpr int TPB0 = TOTAL > 128 ? 256 : (TOTAL > 64 ? 128 : 64);
   constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
   constexpr int BPB = 256 / TPB;
   constexpr int PER_FACE = D == 2 ? NC : NC * NC;
   constexpr int NGH = 2 * D * PER_FACE;
-  __shared__ T sh[BPB * BL];
+  __shared__ T sh[BPB * BLS];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
-  const int li = xcd_block() * BPB + slot;
-  const bool act = li < n;
+  const int lw = xcd_block() * BPB + slot;  // (box, slab) work item
+  const bool act = lw < n * SPLIT;
+  const int li = lw / SPLIT, h = lw - li * SPLIT;
   const int id = act ? list[li] : 0;
+  // fine planes of the slab, and the first staged plane
+  const int kf0 = SPLIT == 1 ? 1 : h * KS + 1, kf1 = SPLIT == 1 ? NC : kf0 + KS - 1;
+  const int kbase = SPLIT == 1 ? 0 : kf0 - 1;
   constexpr int NCELL = 2 * NCOL;
   constexpr int NITC = (NCELL + TPB - 1) / TPB;
   constexpr int NITG = (NGH + TPB - 1) / TPB;
-  T* s = sh + slot * BL;
-  auto L = [](int i, int j, int k) { return D == 2 ? i + N2 * j : i + N2 * (j + N2 * k); };
+  T* s = sh + slot * BLS;
+  auto L = [&](int i, int j, int k) {
+    return D == 2 ? i + N2 * j : i + N2 * (j + N2 * (k - kbase));
+  };
   auto own = [&](int i, int j, int k) { return s[L(i, j, k)]; };
+  // whether a face ghost is read by this slab
+  auto ghost_needed = [&](int nb, int gk) {
+    if (SPLIT == 1) return true;
+    const int d = d_nb_dim(nb);
+    if (d < 2) return gk >= kf0 && gk <= kf1;
+    return d_nb_low(nb) ? kf0 == 1 : kf1 == NC;
+  };
   // Face ghosts that copy a neighbour's cell are loaded first, then the
   // interior cells (each colour a contiguous run in HBM) in batches; all
   // these LDS slots are disjoint. Ghosts of the other rules read the staged
@@ -900,9 +923,10 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   constexpr int NITP = (TOTAL + TPB - 1) / TPB;
   constexpr int NF = D == 2 ? 4 : 8;
   T rq[NITP][NF];
-  auto fine = [](int q, int f, int& ii, int& jj, int& kk) {
+  const int ck0 = SPLIT == 1 ? 0 : h * (HNC / SPLIT);  // parent planes before the slab
+  auto fine = [&](int q, int f, int& ii, int& jj, int& kk) {
     const int ci = q % HNC + 1, cj = (q / HNC) % HNC + 1;
-    const int ck = D == 3 ? q / (HNC * HNC) + 1 : 0;
+    const int ck = D == 3 ? q / (HNC * HNC) + 1 + ck0 : 0;
     ii = 2 * ci - 1 + (f & 1);
     jj = 2 * cj - 1 + ((f >> 1) & 1);
     kk = D == 3 ? 2 * ck - 1 + (f >> 2) : 0;
@@ -938,14 +962,16 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
         g[d] = d_nb_low(nb) ? 0 : NC + 1;
         g[ta] = t % NC + 1;
         if (D == 3) g[tb] = t / NC + 1;
-        const int nb_id = a.nbs[id * 6 + nb - 1];
-        if (nb_id > 0) {
-          int sc[3] = {g[0], g[1], g[2]};
-          sc[d] = d_nb_low(nb) ? NC : 1;
-          gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx_in<D>(NC, sc[0], sc[1], sc[2])];
-          gs[u] = L(g[0], g[1], g[2]);
-        } else {
-          gq[u] = q;
+        if (ghost_needed(nb, g[2])) {
+          const int nb_id = a.nbs[id * 6 + nb - 1];
+          if (nb_id > 0) {
+            int sc[3] = {g[0], g[1], g[2]};
+            sc[d] = d_nb_low(nb) ? NC : 1;
+            gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx_in<D>(NC, sc[0], sc[1], sc[2])];
+            gs[u] = L(g[0], g[1], g[2]);
+          } else {
+            gq[u] = q;
+          }
         }
       }
     }
@@ -963,8 +989,10 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
           const int col = e >= NCOL, q = e - col * NCOL;
           int i, j, k;
           icell<D>(NC, q, col, i, j, k);
-          at[u] = L(i, j, k);
-          v[u] = p[col * HALF + q];
+          if (SPLIT == 1 || (k >= kf0 - 1 && k <= kf1 + 1)) {
+            at[u] = L(i, j, k);
+            v[u] = p[col * HALF + q];
+          }
         }
       }
 #pragma unroll
@@ -1010,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
     if (q >= TOTAL) break;
     const int ci = q % HNC + 1;
     const int cj = (q / HNC) % HNC + 1;
-    const int ck = D == 3 ? q / (HNC * HNC) + 1 : 0;
+    const int ck = D == 3 ? q / (HNC * HNC) + 1 + ck0 : 0;
     // Fortran's sum over cc(i_f:i_f+1, j_f:j_f+1[, k_f:k_f+1]) runs in array
     // order, i fastest (m_aX_mg.f90:872-890)
     T sr = 0, sp = 0;
@@ -1867,7 +1895,18 @@ static bool launch_resid_rstr_lds_t(Pool& P, const Dev& a, const int* list, int 
       if (halve) go(k_resid_rstr_lds<D, CYL, 8, T, EPS, true>, total);
       else go(k_resid_rstr_lds<D, CYL, 8, T, EPS>, total);
       return true;
-    case 16: go(k_resid_rstr_lds<D, CYL, 16, T, EPS>, total); return true;
+    case 16:
+      if constexpr (D == 3 && sizeof(T) == 8) {
+        if (P.rstr_split) {
+          // two z slabs per box (26 KB of LDS each instead of 47 KB), one
+          // 256-lane workgroup per slab
+          hipLaunchKernelGGL((k_resid_rstr_lds<D, CYL, 16, T, EPS, false, 2>), dim3(2 * n),
+                             dim3(256), 0, P.stream, a, list, n, i_phi, i_rhs, i_tmp, keep_res);
+          return true;
+        }
+      }
+      go(k_resid_rstr_lds<D, CYL, 16, T, EPS>, total);
+      return true;
     case 32:
       if (D == 2) {
         go(k_resid_rstr_lds<2, CYL, 32, T, EPS>, total);

@@ -231,6 +231,11 @@ struct Pool {
   // the most LDS a level of the small-level smoother may take
   // (A5G_SMALL_LDS at a5g_init; default 64 KB)
   long small_lds = 64 * 1024;
+  // residual + restriction of 16^3 FP64 boxes in two z slabs per box
+  // (A5G_RSTR_SPLIT=1 at a5g_init; by default one workgroup per box, which
+  // measures faster: the slabs double the workgroups per compute unit but
+  // re-read a plane each and filter the staging loop)
+  bool rstr_split = false;
   uint64_t graphs_state = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
   // waits for every stream of the handle, then destroys the graphs (one may

@@ -91,3 +91,42 @@ def test_small_levels_with_permittivity_tags():
         T.close()
     for iv in (1, 2, 3):
         np.testing.assert_array_equal(out[0][iv], out[1][iv], err_msg="variable %d" % iv)
+
+
+@pytest.mark.parametrize("eps", [False, True])
+@pytest.mark.parametrize("base", [(1, 1, 1), (2, 1, 1)])
+def test_residual_restriction_slabs_match_whole_boxes(base, eps):
+    """The residual + restriction of 16^3 FP64 boxes in two z slabs per box
+    (A5G_RSTR_SPLIT=1) against one workgroup per box (the default): FMG and
+    V-cycles with the residual kept, on trees with refinement boundaries,
+    every variable of every box equal bit for bit."""
+    import a5_streamer_amd as A
+    print("residual slabs", base, eps, flush=True)
+    out = []
+    for split in ("1", "0"):
+        os.environ["A5G_RSTR_SPLIT"] = split
+        try:
+            T = A.Tree(3, 16, 5, 1, 1.0 / 16, coarsen_to=2)
+        finally:
+            del os.environ["A5G_RSTR_SPLIT"]
+
+        def rules(lvl, c):
+            if lvl < 2:
+                return 1
+            return 1 if (lvl < 3 and np.linalg.norm(c - 0.35) < 0.3) else 0
+        S.build_tree(T, 3, 16, rules, n_base=base)
+        rng = np.random.default_rng(9)
+        n, bl = T.info()["max_id"], T.box_len(16)
+        T.set_cc_all(3, rng.standard_normal((n, bl)))
+        T.set_cc_all(1, np.zeros((n, bl)))
+        T.set_cc_all(5, np.where(rng.random((n, bl)) < 0.05, 2.0, 1.0) if eps else np.ones((n, bl)))
+        T.mg_setup(1, 2, 3, i_eps=5 if eps else -1, n_base=4,
+                   bc_kind=np.array([0, 1, 2, 0, 1, 2], np.int32),
+                   bc_val=np.arange(6, dtype=float) + 0.5, auto_ops=True)
+        T.fas_fmg(True, True)
+        T.fas_vcycle(T.info()["max_lvl"], True)
+        T.fas_fmg(False, False)
+        out.append({iv: T.get_cc_all(iv) for iv in (1, 2, 3)})
+        T.close()
+    for iv in (1, 2, 3):
+        np.testing.assert_array_equal(out[0][iv], out[1][iv], err_msg="variable %d" % iv)
