@@ -2930,3 +2930,653 @@ void o_lt_set_rows(otable *t, const double *rc) {
   /* the table's rows_cols(n_rows, n_cols) as given (test perturbations) */
   memcpy(t->rc, rc, sizeof(double) * (size_t)t->n_rows * t->n_cols);
 }
+
+/* ------------------------------------------------------------------ */
+/* reverse mode of the rest of a time step and of the FAS FMG          */
+/* (no counterpart in the reference, which computes no derivatives)   */
+/* ------------------------------------------------------------------ */
+/* Each routine takes the adjoints of the values its forward routine writes
+ * (and clears them) and adds those of the values it reads, one loop per
+ * forward loop, in plain sequential code. The GPU's reverse kernels scatter
+ * the same terms with atomics; these sums come in loop order. */
+
+void o_tree_copy_cc_adj(otree *t, int lfrom, int lto) {
+  /* a$D_tree_copy_cc (m_aX_utils.f90:432-453), to = from on whole boxes */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = 0; q < ids->n; q++) {
+      int id = ids->v[q], n = blen(t, id);
+      double *f = CC(t, id, lfrom), *o = CC(t, id, lto);
+      for (int m = 0; m < n; m++) {
+        f[m] += o[m];
+        o[m] = 0;
+      }
+    }
+  }
+}
+
+void o_average_dens_adj(otree *t, int le, int leo, int li, int lio) {
+  /* average_dens (src/streamer_2d.f90:492-496), n = 0.5 (n + n_old) */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = 0; q < ids->n; q++) {
+      int id = ids->v[q], n = blen(t, id);
+      double *e = CC(t, id, le), *eo = CC(t, id, leo), *p = CC(t, id, li), *po = CC(t, id, lio);
+      for (int m = 0; m < n; m++) {
+        eo[m] += 0.5 * e[m];
+        e[m] = 0.5 * e[m];
+        po[m] += 0.5 * p[m];
+        p[m] = 0.5 * p[m];
+      }
+    }
+  }
+}
+
+void o_space_charge_adj(otree *t, double fac, int lr, int le, int li) {
+  /* rhs = fac (n_e - n_i) on leaf boxes (src/streamer_2d.f90:318-329) */
+  for (int lvl = 1; lvl <= t->max_lvl; lvl++) {
+    ivec *lv = &t->leaves[LV(t, lvl)];
+    for (int q = 0; q < lv->n; q++) {
+      int id = lv->v[q], n = blen(t, id);
+      double *r = CC(t, id, lr), *e = CC(t, id, le), *p = CC(t, id, li);
+      for (int m = 0; m < n; m++) {
+        e[m] += fac * r[m];
+        p[m] -= fac * r[m];
+        r[m] = 0;
+      }
+    }
+  }
+}
+
+static void restrict_box_adj(otree *t, int c_id, int p_id, int il) {
+  /* a$D_restrict_box (m_aX_restrict.f90:67-142) transposed */
+  int nc = t->ncell[c_id], hnc = nc >> 1, pnc = t->ncell[p_id], off[3];
+  child_offset(t, c_id, 0, off);
+  double *c = CC(t, c_id, il), *p = CC(t, p_id, il);
+  if (t->dim == 2) {
+    double dr16 = 0.0625 * t->dr[p_id];
+    for (int j = 1; j <= hnc; j++) {
+      int jc = off[1] + j, jf = 2 * j - 1;
+      for (int i = 1; i <= hnc; i++) {
+        int ic = off[0] + i, fi = 2 * i - 1;
+        double lam = p[I2(pnc, ic, jc)];
+        p[I2(pnc, ic, jc)] = 0;
+        if (t->coord == 1) {
+          double r = RMP(t, p_id)[0] + (i - 0.5) * t->dr[p_id];
+          double rfac = dr16 / r;
+          c[I2(nc, fi, jf)] += (0.25 - rfac) * lam;
+          c[I2(nc, fi, jf + 1)] += (0.25 - rfac) * lam;
+          c[I2(nc, fi + 1, jf)] += (0.25 + rfac) * lam;
+          c[I2(nc, fi + 1, jf + 1)] += (0.25 + rfac) * lam;
+        } else {
+          c[I2(nc, fi, jf)] += 0.25 * lam;
+          c[I2(nc, fi + 1, jf)] += 0.25 * lam;
+          c[I2(nc, fi, jf + 1)] += 0.25 * lam;
+          c[I2(nc, fi + 1, jf + 1)] += 0.25 * lam;
+        }
+      }
+    }
+  } else {
+    for (int k = 1; k <= hnc; k++)
+      for (int j = 1; j <= hnc; j++)
+        for (int i = 1; i <= hnc; i++) {
+          size_t pc = I3(pnc, off[0] + i, off[1] + j, off[2] + k);
+          double lam = p[pc];
+          p[pc] = 0;
+          for (int dk = 0; dk < 2; dk++)
+            for (int dj = 0; dj < 2; dj++)
+              for (int di = 0; di < 2; di++)
+                c[I3(nc, 2 * i - 1 + di, 2 * j - 1 + dj, 2 * k - 1 + dk)] += 0.125 * lam;
+        }
+  }
+}
+
+void o_restrict_tree_adj(otree *t, int il) {
+  /* a$D_restrict_tree (m_aX_restrict.f90:54-64) transposed: coarsest first */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl - 1; lvl++) {
+    ivec *pa = &t->parents[LV(t, lvl)];
+    for (int q = 0; q < pa->n; q++) {
+      int id = pa->v[q];
+      for (int c = 0; c < t->nch; c++) {
+        int cid = CHP(t, id)[c];
+        if (cid != NO_BOX) restrict_box_adj(t, cid, id, il);
+      }
+    }
+  }
+}
+
+/* a$D_gc_interp[_lim] (m_aX_gc.f90:90-295) on face nb, transposed; the
+ * limiter's case is recomputed from the primal iv */
+static void gc_interp_adj(otree *t, int id, int nb, int iv, int il, int lim) {
+  const double sixth = 1 / 6.0, third = 1 / 3.0;
+  int nc = t->ncell[id], p = t->parent[id], pn = NBP(t, p)[nb - 1], off[3];
+  child_offset(t, id, nb, off);
+  int ix, ixf, ixc;
+  if (nb_low(nb)) { ix = 0; ixf = 1; ixc = nc; }
+  else { ix = nc + 1; ixf = nc; ixc = 1; }
+  double *a = CC(t, id, iv), *q = CC(t, pn, iv);
+  double *la = CC(t, id, il), *lq = CC(t, pn, il);
+  int pnc = t->ncell[pn], d = nb_dim(nb);
+  if (t->dim == 2) {
+    for (int x = 1; x <= nc; x++) {
+      int o = (d == 0) ? off[1] : off[0];
+      int c1 = o + ((x + 1) >> 1), c2 = c1 + 1 - 2 * (x & 1);
+      size_t q1, q2, g, f;
+      if (d == 0) {
+        q1 = I2(pnc, ixc, c1); q2 = I2(pnc, ixc, c2); g = I2(nc, ix, x); f = I2(nc, ixf, x);
+      } else {
+        q1 = I2(pnc, c1, ixc); q2 = I2(pnc, c2, ixc); g = I2(nc, x, ix); f = I2(nc, x, ixf);
+      }
+      double lam = la[g];
+      la[g] = 0;
+      double v = 0.5 * q[q1] + sixth * q[q2] + third * a[f];
+      if (lim && v > 2 * q[q1]) {
+        lq[q1] += 2 * lam;
+      } else {
+        lq[q1] += 0.5 * lam;
+        lq[q2] += sixth * lam;
+        la[f] += third * lam;
+      }
+    }
+  } else {
+    for (int y = 1; y <= nc; y++)
+      for (int x = 1; x <= nc; x++) {
+        size_t q1, q2, q3, g, f;
+        if (d == 0) {
+          int k = y, j = x;
+          int kc1 = off[2] + ((k + 1) >> 1), kc2 = kc1 + 1 - 2 * (k & 1);
+          int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+          q1 = I3(pnc, ixc, jc1, kc1); q2 = I3(pnc, ixc, jc2, kc1); q3 = I3(pnc, ixc, jc1, kc2);
+          g = I3(nc, ix, j, k); f = I3(nc, ixf, j, k);
+        } else if (d == 1) {
+          int k = y, i = x;
+          int kc1 = off[2] + ((k + 1) >> 1), kc2 = kc1 + 1 - 2 * (k & 1);
+          int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+          q1 = I3(pnc, ic1, ixc, kc1); q2 = I3(pnc, ic2, ixc, kc1); q3 = I3(pnc, ic1, ixc, kc2);
+          g = I3(nc, i, ix, k); f = I3(nc, i, ixf, k);
+        } else {
+          int j = y, i = x;
+          int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+          int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+          q1 = I3(pnc, ic1, jc1, ixc); q2 = I3(pnc, ic1, jc2, ixc); q3 = I3(pnc, ic2, jc1, ixc);
+          g = I3(nc, i, j, ix); f = I3(nc, i, j, ixf);
+        }
+        double lam = la[g];
+        la[g] = 0;
+        double v = third * q[q1] + sixth * q[q2] + sixth * q[q3] + third * a[f];
+        if (lim && v > 2 * q[q1]) {
+          lq[q1] += 2 * lam;
+        } else {
+          lq[q1] += third * lam;
+          lq[q2] += sixth * lam;
+          lq[q3] += sixth * lam;
+          la[f] += third * lam;
+        }
+      }
+  }
+}
+
+/* sides_from_nb (m_aX_gc.f90:368-402) transposed */
+static void sides_from_nb_adj(otree *t, int id, int nb_id, int nb, int il) {
+  int nc = t->ncell[id];
+  double *a = CC(t, id, il), *o = CC(t, nb_id, il);
+  int d = nb_dim(nb);
+  int src = nb_low(nb) ? nc : 1, dst = nb_low(nb) ? 0 : nc + 1;
+  int bmax = t->dim == 3 ? nc : 1;
+  for (int b = 1; b <= bmax; b++)
+    for (int x = 1; x <= nc; x++) {
+      int p[3], q[3];
+      int ta = (d == 0) ? 1 : 0, tb = (d == 2) ? 1 : 2;
+      p[d] = dst; q[d] = src;
+      p[ta] = q[ta] = x;
+      if (t->dim == 3) p[tb] = q[tb] = b;
+      size_t pi = t->dim == 2 ? I2(nc, p[0], p[1]) : I3(nc, p[0], p[1], p[2]);
+      size_t qi = t->dim == 2 ? I2(nc, q[0], q[1]) : I3(nc, q[0], q[1], q[2]);
+      o[qi] += a[pi];
+      a[pi] = 0;
+    }
+}
+
+void o_gc_tree_adj(otree *t, int iv, int il, int rb, int bc) {
+  /* a$D_gc_tree (m_aX_gc.f90:28-46) with a$D_gc_interp (rb 1) or
+   * a$D_gc_interp_lim (rb 2) and a$D_gc_neumann (bc 1) or a$D_gc_dirichlet
+   * (bc 2), transposed: levels and faces in the reverse order */
+  for (int lvl = t->max_lvl; lvl >= t->min_lvl; lvl--) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = ids->n - 1; q >= 0; q--) {
+      int id = ids->v[q];
+      for (int nb = t->nnb; nb >= 1; nb--) {
+        int nb_id = NBP(t, id)[nb - 1];
+        if (nb_id > NO_BOX) {
+          sides_from_nb_adj(t, id, nb_id, nb, il);
+        } else if (nb_id == NO_BOX) {
+          gc_interp_adj(t, id, nb, iv, il, rb == 2);
+        } else {
+          double *a = CC(t, id, il);
+          FACE_LOOP_BEGIN(t, id, nb)
+          a[c] += bc == 1 ? a[g] : -a[g];
+          a[g] = 0;
+          FACE_LOOP_END
+        }
+      }
+    }
+  }
+}
+
+void o_fld_from_pot_adj(otree *t, const o_vars *v, int lE, int lfld, int lphi) {
+  /* fld_from_pot (src/streamer_2d.f90:346-381, src/streamer_3d.f90:348-395)
+   * transposed on every box: the interior |E| adjoints to the faces
+   * (d|E|/dF = (F_lo + F_hi) / (4 |E|), zero where |E| = 0), then every
+   * face adjoint to the potential of its two cells */
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = 0; q < ids->n; q++) {
+      int id = ids->v[q], nc = t->ncell[id], D = t->dim;
+      double inv_dr = 1 / t->dr[id];
+      const double *e = CC(t, id, v->eps), *E = CC(t, id, v->fld);
+      double *LE = CC(t, id, lE), *LP = CC(t, id, lphi);
+      const double *F[3];
+      double *LF[3];
+      for (int d = 0; d < D; d++) {
+        F[d] = FC(t, id, v->f_fld, d);
+        LF[d] = FC(t, id, lfld, d);
+      }
+      int kmax = D == 3 ? nc : 1;
+      for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= nc; j++)
+          for (int i = 1; i <= nc; i++) {
+            size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+            double le = LE[c], ev = E[c];
+            LE[c] = 0;
+            if (le == 0 || !(ev > 0)) continue;
+            for (int d = 0; d < D; d++) {
+              int h[3] = {i, j, k};
+              h[d] += 1;
+              size_t f0 = FI(nc, D, d, i, j, k), f1 = FI(nc, D, d, h[0], h[1], h[2]);
+              double s = F[d][f0] + F[d][f1];
+              LF[d][f0] += le * (0.25 * s / ev);
+              LF[d][f1] += le * (0.25 * s / ev);
+            }
+          }
+      for (int d = 0; d < D; d++) {
+        int n0 = nc + (d == 0), n1 = nc + (d == 1), n2 = D == 3 ? nc + (d == 2) : 1;
+        for (int k = 1; k <= n2; k++)
+          for (int j = 1; j <= n1; j++)
+            for (int i = 1; i <= n0; i++) {
+              int a[3] = {i, j, D == 3 ? k : 0}, b[3] = {i, j, D == 3 ? k : 0};
+              a[d] -= 1;
+              size_t ia = D == 2 ? I2(nc, a[0], a[1]) : I3(nc, a[0], a[1], a[2]);
+              size_t ib = D == 2 ? I2(nc, b[0], b[1]) : I3(nc, b[0], b[1], b[2]);
+              int fpos = (d == 0) ? i : (d == 1) ? j : k;
+              double kf;
+              if (fpos == 1) kf = 2 * inv_dr * e[ia] / (e[ib] + e[ia]);
+              else if (fpos == nc + 1) kf = 2 * inv_dr * e[ib] / (e[ib] + e[ia]);
+              else kf = inv_dr;
+              size_t fi = FI(nc, D, d, i, j, k);
+              double lam = LF[d][fi];
+              LF[d][fi] = 0;
+              LP[ia] += kf * lam;
+              LP[ib] -= kf * lam;
+            }
+      }
+    }
+  }
+}
+
+/* --- the FAS FMG (m_aX_mg.f90:164-255) transposed: Cartesian or        */
+/* cylindrical Laplacian, no permittivity, no Helmholtz term              */
+
+typedef struct {
+  int lp, lt, lr, lw;
+  double *vbar;
+} o_adjmg;
+
+/* a$D_gc_box with mg$D_sides_rb and the multigrid face rules, transposed:
+ * faces in the reverse order; on a y face with the whole-row rule its
+ * corner ghosts (written last, from the x ghosts) go first */
+static void gc_box_mg_adj(otree *t, int id, const o_adjmg *m) {
+  int nc = t->ncell[id], D = t->dim;
+  double *a = CC(t, id, m->lp);
+  for (int nb = t->nnb; nb >= 1; nb--) {
+    int nb_id = NBP(t, id)[nb - 1];
+    if (nb_id > NO_BOX) {
+      sides_from_nb_adj(t, id, nb_id, nb, m->lp);
+    } else if (nb_id == NO_BOX) {
+      /* 0.5 times the parent's cell (a$D_gc_prolong0) plus an
+       * extrapolation from the box's own cells (mg$D_sides_rb) */
+      int d = nb_dim(nb), dix = nb_low(nb) ? 1 : -1;
+      int p = t->parent[id], pnc = t->ncell[p], off[3];
+      child_offset(t, id, 0, off);
+      double *lpp = CC(t, p, m->lp);
+      FACE_LOOP_BEGIN(t, id, nb)
+      double lam = a[g];
+      a[g] = 0;
+      if (lam != 0) {
+        int gi[3] = {q_[0], q_[1], D_ == 3 ? q_[2] : 0};
+        int pc[3] = {0, 0, 0};
+        for (int e2 = 0; e2 < D; e2++) pc[e2] = off[e2] + ((gi[e2] + 1) >> 1);
+        lpp[D == 2 ? I2(pnc, pc[0], pc[1]) : I3(pnc, pc[0], pc[1], pc[2])] += 0.5 * lam;
+        a[c] += (D == 2 ? 1.0 : 1.25) * lam;
+        int e1[3] = {r_[0], r_[1], D_ == 3 ? r_[2] : 0};
+        e1[d] += dix;
+        a[D == 2 ? I2(nc, e1[0], e1[1]) : I3(nc, e1[0], e1[1], e1[2])] -= 0.25 * lam;
+        int ta = (d == 0) ? 1 : 0, tb = (d == 2) ? 1 : 2;
+        int e2[3] = {r_[0], r_[1], D_ == 3 ? r_[2] : 0};
+        e2[ta] += -1 + 2 * (e2[ta] & 1);
+        a[D == 2 ? I2(nc, e2[0], e2[1]) : I3(nc, e2[0], e2[1], e2[2])] -= 0.25 * lam;
+        if (D == 3) {
+          int e3[3] = {r_[0], r_[1], r_[2]};
+          e3[tb] += -1 + 2 * (e3[tb] & 1);
+          a[I3(nc, e3[0], e3[1], e3[2])] -= 0.25 * lam;
+        }
+      }
+      FACE_LOOP_END
+    } else {
+      int kind = t->bc_kind[nb - 1];
+      if (kind == 3 && D == 2 && nb_dim(nb) == 1) {
+        int gy = nb_low(nb) ? 0 : nc + 1, cy = nb_low(nb) ? 1 : nc;
+        for (int s = 0; s < 2; s++) {
+          int gx = s ? nc + 1 : 0;
+          double lam = a[I2(nc, gx, gy)];
+          a[I2(nc, gx, gy)] = 0;
+          a[I2(nc, gx, cy)] -= lam;
+          m->vbar[nb - 1] += 2 * lam;
+        }
+      }
+      FACE_LOOP_BEGIN(t, id, nb)
+      double lam = a[g];
+      a[g] = 0;
+      if (kind == 0) a[c] += lam;
+      else if (kind == 1) a[c] -= lam;
+      else {
+        a[c] -= lam;
+        m->vbar[nb - 1] += 2 * lam;
+      }
+      FACE_LOOP_END
+    }
+  }
+}
+
+static void adj_gc_level(otree *t, int lvl, const o_adjmg *m) {
+  ivec *ids = &t->ids[LV(t, lvl)];
+  for (int q = ids->n - 1; q >= 0; q--) gc_box_mg_adj(t, ids->v[q], m);
+}
+
+/* one half-sweep (mg$D_box_gsrb_lpl, mg2_box_gsrb_clpl) transposed */
+static void gsrb_box_adj(otree *t, int id, int cntr, const o_adjmg *m) {
+  int nc = t->ncell[id], D = t->dim;
+  int cyl = D == 2 && t->coord == 1 && t->auto_ops;
+  double dx2 = t->dr[id] * t->dr[id];
+  double *L = CC(t, id, m->lp), *R = CC(t, id, m->lr);
+  double w = D == 2 ? 0.25 : 1 / 6.0;
+  int kmax = D == 3 ? nc : 1;
+  int ioff = cyl ? (IXP(t, id)[0] - 1) * nc : 0;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++) {
+      int i0 = 2 - ((cntr ^ (D == 3 ? k + j : j)) & 1);
+      for (int i = i0; i <= nc; i += 2) {
+        size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+        double y = L[c];
+        L[c] = 0;
+        if (y == 0) continue;
+        double wxm = w, wxp = w;
+        if (cyl) {
+          double den = i + ioff - 0.5;
+          wxm = 0.25 * ((double)(i + ioff - 1) / den);
+          wxp = 0.25 * ((double)(i + ioff) / den);
+        }
+        if (D == 2) {
+          L[I2(nc, i - 1, j)] += wxm * y;
+          L[I2(nc, i + 1, j)] += wxp * y;
+          L[I2(nc, i, j - 1)] += w * y;
+          L[I2(nc, i, j + 1)] += w * y;
+        } else {
+          L[I3(nc, i - 1, j, k)] += w * y;
+          L[I3(nc, i + 1, j, k)] += w * y;
+          L[I3(nc, i, j - 1, k)] += w * y;
+          L[I3(nc, i, j + 1, k)] += w * y;
+          L[I3(nc, i, j, k - 1)] += w * y;
+          L[I3(nc, i, j, k + 1)] += w * y;
+        }
+        R[c] += -w * dx2 * y;
+      }
+    }
+}
+
+static void adj_gsrb_boxes(otree *t, int lvl, int n_cycle, const o_adjmg *m) {
+  ivec *ids = &t->ids[LV(t, lvl)];
+  for (int it = 2 * n_cycle; it >= 1; it--) {
+    adj_gc_level(t, lvl, m);
+    for (int q = ids->n - 1; q >= 0; q--) gsrb_box_adj(t, ids->v[q], it, m);
+  }
+}
+
+/* dst += s A^T src on box id (A the box Laplacian); src cleared if asked */
+static void op_box_adj(otree *t, int id, int isrc, double s, int idst, int clear) {
+  int nc = t->ncell[id], D = t->dim;
+  int cyl = D == 2 && t->coord == 1 && t->auto_ops;
+  double inv = 1 / (t->dr[id] * t->dr[id]);
+  double *S = CC(t, id, isrc), *Dp = CC(t, id, idst);
+  int kmax = D == 3 ? nc : 1;
+  int ioff = cyl ? (IXP(t, id)[0] - 1) * nc : 0;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) {
+        size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+        double sb = s * S[c];
+        if (clear) S[c] = 0;
+        if (sb == 0) continue;
+        double wxm = inv, wxp = inv;
+        if (cyl) {
+          double den = i + ioff - 0.5;
+          wxm = ((double)(i + ioff - 1) / den) * inv;
+          wxp = ((double)(i + ioff) / den) * inv;
+        }
+        if (D == 2) {
+          Dp[I2(nc, i - 1, j)] += wxm * sb;
+          Dp[I2(nc, i + 1, j)] += wxp * sb;
+          Dp[I2(nc, i, j - 1)] += inv * sb;
+          Dp[I2(nc, i, j + 1)] += inv * sb;
+          Dp[c] += -4.0 * inv * sb;
+        } else {
+          Dp[I3(nc, i - 1, j, k)] += inv * sb;
+          Dp[I3(nc, i + 1, j, k)] += inv * sb;
+          Dp[I3(nc, i, j - 1, k)] += inv * sb;
+          Dp[I3(nc, i, j + 1, k)] += inv * sb;
+          Dp[I3(nc, i, j, k - 1)] += inv * sb;
+          Dp[I3(nc, i, j, k + 1)] += inv * sb;
+          Dp[c] += -6.0 * inv * sb;
+        }
+      }
+}
+
+/* mg$D_box_rstr_lpl of child id transposed: the parent's adjoint in ip to
+ * the child's cells in idst (added or assigned), the parent's cleared */
+static void rstr_box_adj(otree *t, int id, int ip, int idst, int assign) {
+  int p_id = t->parent[id], nc = t->ncell[id], hnc = nc >> 1, pnc = t->ncell[p_id], off[3];
+  child_offset(t, id, 0, off);
+  double *P = CC(t, p_id, ip), *C = CC(t, id, idst);
+  int D = t->dim;
+  double w = D == 2 ? 0.25 : 0.125;
+  int kmax = D == 3 ? hnc : 1;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= hnc; j++)
+      for (int i = 1; i <= hnc; i++) {
+        size_t pc = D == 2 ? I2(pnc, off[0] + i, off[1] + j)
+                           : I3(pnc, off[0] + i, off[1] + j, off[2] + k);
+        double v = w * P[pc];
+        P[pc] = 0;
+        for (int f = 0; f < (D == 2 ? 4 : 8); f++) {
+          int fi = 2 * i - 1 + (f & 1), fj = 2 * j - 1 + ((f >> 1) & 1);
+          int fk = D == 3 ? 2 * k - 1 + (f >> 2) : 0;
+          size_t c = D == 2 ? I2(nc, fi, fj) : I3(nc, fi, fj, fk);
+          C[c] = assign ? v : C[c] + v;
+        }
+      }
+}
+
+/* mg$D_box_corr_lpl of child id transposed: the child's potential adjoint
+ * (unchanged) into the parent's tmp adjoint with the prolongation weights */
+static void corr_box_adj(otree *t, int id, const o_adjmg *m) {
+  int p_id = t->parent[id], nc = t->ncell[id], pnc = t->ncell[p_id], off[3], D = t->dim;
+  child_offset(t, id, 0, off);
+  const double *F = CC(t, id, m->lp);
+  double *Q = CC(t, p_id, m->lt);
+  int kmax = D == 3 ? nc : 1;
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) {
+        double fb = F[D == 2 ? I2(nc, i, j) : I3(nc, i, j, k)];
+        if (fb == 0) continue;
+        int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
+        int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+        if (D == 2) {
+          Q[I2(pnc, ic1, jc1)] += 0.5 * fb;
+          Q[I2(pnc, ic2, jc1)] += 0.25 * fb;
+          Q[I2(pnc, ic1, jc2)] += 0.25 * fb;
+        } else {
+          int kc1 = off[2] + ((k + 1) >> 1), kc2 = kc1 + 1 - 2 * (k & 1);
+          Q[I3(pnc, ic1, jc1, kc1)] += 0.25 * fb;
+          Q[I3(pnc, ic2, jc1, kc1)] += 0.25 * fb;
+          Q[I3(pnc, ic1, jc2, kc1)] += 0.25 * fb;
+          Q[I3(pnc, ic1, jc1, kc2)] += 0.25 * fb;
+        }
+      }
+}
+
+/* whole-box transposes: mode 0 tmp = phi; mode 1 tmp = phi - tmp; mode 2
+ * rhs += tmp */
+static void whole_adj(otree *t, int id, int mode, const o_adjmg *m) {
+  int n = blen(t, id);
+  double *Lp = CC(t, id, m->lp), *Lt = CC(t, id, m->lt), *Lr = CC(t, id, m->lr);
+  for (int q = 0; q < n; q++) {
+    if (mode == 2) {
+      Lt[q] += Lr[q];
+    } else {
+      double tb = Lt[q];
+      Lp[q] += tb;
+      Lt[q] = mode == 0 ? 0.0 : -tb;
+    }
+  }
+}
+
+/* dst += src on the interior of box id; src cleared there if asked */
+static void int_add_adj(otree *t, int id, int isrc, int idst, int clear) {
+  int nc = t->ncell[id], D = t->dim, kmax = D == 3 ? nc : 1;
+  double *S = CC(t, id, isrc), *Dp = CC(t, id, idst);
+  for (int k = 1; k <= kmax; k++)
+    for (int j = 1; j <= nc; j++)
+      for (int i = 1; i <= nc; i++) {
+        size_t c = D == 2 ? I2(nc, i, j) : I3(nc, i, j, k);
+        Dp[c] += S[c];
+        if (clear) S[c] = 0;
+      }
+}
+
+static void adj_correct_children(otree *t, int lvl, const o_adjmg *m) {
+  /* correct_children over the parents of lvl (m_aX_mg.f90:374-401) */
+  ivec *pa = &t->parents[LV(t, lvl)];
+  for (int q = pa->n - 1; q >= 0; q--) {
+    int id = pa->v[q];
+    for (int c = t->nch - 1; c >= 0; c--) {
+      int cid = CHP(t, id)[c];
+      if (cid != NO_BOX) corr_box_adj(t, cid, m);
+    }
+    whole_adj(t, id, 1, m);
+  }
+}
+
+static void adj_update_coarse(otree *t, int lvl, int keep_res, const o_adjmg *m) {
+  /* update_coarse (keep_res 0, the fine tmp restored; m_aX_mg.f90:430-486)
+   * and set_coarse_phi_rhs (keep_res 1; :490-523) transposed */
+  ivec *pa = &t->parents[LV(t, lvl - 1)];
+  for (int q = pa->n - 1; q >= 0; q--) {
+    int id = pa->v[q];
+    if (!keep_res) whole_adj(t, id, 0, m);           /* tmp = phi */
+    whole_adj(t, id, 2, m);                          /* rhs += tmp */
+    op_box_adj(t, id, m->lr, 1.0, m->lp, 1);         /* rhs = A phi */
+  }
+  adj_gc_level(t, lvl - 1, m);
+  ivec *ids = &t->ids[LV(t, lvl)];
+  for (int q = ids->n - 1; q >= 0; q--) {
+    int id = ids->v[q];
+    rstr_box_adj(t, id, m->lp, m->lp, 0);            /* phi_p = R phi */
+    if (keep_res) {
+      rstr_box_adj(t, id, m->lt, m->lt, 0);          /* tmp_p = R tmp */
+      op_box_adj(t, id, m->lt, -1.0, m->lp, 0);      /* tmp = rhs - A phi */
+      int_add_adj(t, id, m->lt, m->lr, 1);
+    } else {
+      rstr_box_adj(t, id, m->lt, m->lw, 1);          /* tmp_p = R (rhs - A phi) */
+      op_box_adj(t, id, m->lw, -1.0, m->lp, 0);
+      int_add_adj(t, id, m->lw, m->lr, 0);
+    }
+  }
+  if (keep_res && lvl == t->max_lvl) adj_gc_level(t, lvl, m);
+}
+
+static void adj_vcycle(otree *t, int max_lvl, const o_adjmg *m) {
+  int min_lvl = t->min_lvl;
+  for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
+    adj_gsrb_boxes(t, lvl, t->n_up, m);
+    adj_gc_level(t, lvl, m);
+    adj_correct_children(t, lvl - 1, m);
+  }
+  adj_gsrb_boxes(t, min_lvl, t->n_base, m);
+  for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) {
+    adj_update_coarse(t, lvl, 0, m);
+    adj_gsrb_boxes(t, lvl, t->n_down, m);
+  }
+}
+
+int o_fas_fmg_adj(otree *t, const int *adj, double *vbar) {
+  /* mg$D_fas_fmg(tree, mg, .false., .false.) (m_aX_mg.f90:164-203)
+   * transposed, in the reverse of the reference's schedule. adj = lp, lt,
+   * lr, lw; vbar[nb-1] receives the adjoint of the value V of face nb.
+   * Returns 1 (not done) for permittivity tags or a Helmholtz term. */
+  if (t->helm != 0) return 1;
+  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = 0; q < ids->n; q++)
+      if (box_tag(t, ids->v[q]) != TAG_NORMAL) return 1;
+  }
+  o_adjmg m = {adj[0], adj[1], adj[2], adj[3], vbar};
+  for (int nb = 0; nb < t->nnb; nb++) vbar[nb] = 0;
+  int min_lvl = t->min_lvl, max_lvl = t->max_lvl;
+  for (int lvl = max_lvl; lvl >= min_lvl; lvl--) {
+    adj_vcycle(t, lvl, &m);
+    if (lvl > min_lvl) {
+      adj_gc_level(t, lvl, &m);
+      adj_correct_children(t, lvl - 1, &m);
+    }
+    ivec *ids = &t->ids[LV(t, lvl)];
+    for (int q = ids->n - 1; q >= 0; q--) whole_adj(t, ids->v[q], 0, &m);  /* tmp = phi */
+  }
+  for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) adj_update_coarse(t, lvl, 1, &m);
+  return 0;
+}
+
+void o_store_vars(otree *t, int n_cc, const int *ivs, int n_fc, const int *fvs, double *buf,
+                  int load) {
+  /* whole cell and face variables of box slots 0..max_id to (load 0) or
+   * from (load 1) buf, in the order given */
+  size_t o = 0;
+  for (int k = 0; k < n_cc; k++)
+    for (int id = 0; id <= t->max_id; id++) {
+      double *c = CC(t, id, ivs[k]);
+      if (load) memcpy(c, buf + o, sizeof(double) * t->box_len);
+      else memcpy(buf + o, c, sizeof(double) * t->box_len);
+      o += t->box_len;
+    }
+  for (int k = 0; k < n_fc; k++)
+    for (int id = 0; id <= t->max_id; id++)
+      for (int d = 0; d < t->dim; d++) {
+        double *f = FC(t, id, fvs[k], d);
+        if (load) memcpy(f, buf + o, sizeof(double) * t->face_len);
+        else memcpy(buf + o, f, sizeof(double) * t->face_len);
+        o += t->face_len;
+      }
+}
+
+size_t o_store_size(const otree *t, int n_cc, int n_fc) {
+  return (size_t)(t->max_id + 1) * ((size_t)n_cc * t->box_len + (size_t)n_fc * t->dim * t->face_len);
+}

@@ -32,7 +32,15 @@
  *    counterpart): central differences of the restated forward substep
  *    (itself pinned above) along 20 seeded directions, at two step sizes
  *    whose error bounds follow the limiter's and the table's kinks
- *    (tests/test_oracle_adjoint.py).
+ *    (tests/test_oracle_adjoint.py);
+ *  - the reverse mode of the rest of a time step and of the FAS-FMG solve
+ *    (copy, average, space charge, restriction, ghost cells, field from
+ *    potential, o_fas_fmg_adj; no reference counterpart): the dot-product
+ *    identity <w, F(x)> = <J^T w, x> against the forward routines above,
+ *    which are positively homogeneous in what they read (affine in the
+ *    boundary values for the FMG), a central difference off the ray for the
+ *    field, and the gradient of whole windows of steps against central
+ *    differences of the forward pass (tests/test_oracle_step_adjoint.py).
  * No function is left without such a check.
  *
  * Conventions follow the reference: box ids are 1-based, 0 = no box,
@@ -42,6 +50,8 @@
  */
 #ifndef A5_ORACLE_H
 #define A5_ORACLE_H
+
+#include <stddef.h>
 
 typedef struct otree otree;
 
@@ -188,6 +198,29 @@ void o_update_solution_adj(otree *t, const o_vars *v, const otable *tbl, double 
 void o_consistent_fluxes_adj(otree *t, int lF);
 void o_fluxes_koren_adj(otree *t, const o_vars *v, const otable *tbl, double dt,
                         const o_adj *a, double *tgrad);
+/* Reverse mode of the rest of a time step (no reference counterpart), in
+ * the forms of the library's a5g_*_adjoint calls: each takes the adjoints
+ * of what its forward routine writes (clearing them) and adds those of
+ * what it reads. gc_tree_adj covers rb 1 (interp) and 2 (interp_lim, the
+ * limiter's case taken from the primal iv) with bc 1 (Neumann) and 2
+ * (Dirichlet); fld_from_pot_adj is linearised at the field the forward
+ * routine left. */
+void o_tree_copy_cc_adj(otree *t, int lfrom, int lto);
+void o_average_dens_adj(otree *t, int le, int leo, int li, int lio);
+void o_space_charge_adj(otree *t, double fac, int lr, int le, int li);
+void o_restrict_tree_adj(otree *t, int il);
+void o_gc_tree_adj(otree *t, int iv, int il, int rb, int bc);
+void o_fld_from_pot_adj(otree *t, const o_vars *v, int lE, int lfld, int lphi);
+/* o_fas_fmg(t, 0, 0) transposed: adj = lp, lt, lr, lw (the adjoints of the
+ * potential, tmp and rhs it leaves, in; of those it starts from, out; lw
+ * work); vbar[nb-1] the adjoint of face nb's value V. Returns 1 without
+ * doing anything on boxes with a permittivity or a Helmholtz term. */
+int o_fas_fmg_adj(otree *t, const int *adj, double *vbar);
+/* whole variables of box slots 0..max_id to (load 0) or from (load 1) buf,
+ * which holds o_store_size doubles */
+void o_store_vars(otree *t, int n_cc, const int *ivs, int n_fc, const int *fvs, double *buf,
+                  int load);
+size_t o_store_size(const otree *t, int n_cc, int n_fc);
 /* per-box maxima for the refinement criterion: out[id-1] = (dr^2 max|rhs|,
  * max fld, max elec) over the interior, boxes of levels >= 1 */
 void o_box_summaries(const otree *t, const o_vars *v, double *out);

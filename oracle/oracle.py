@@ -53,6 +53,9 @@ def lib():
                                       C.c_double, dp, ip, ip]
         L.o_cc_axis.restype = C.c_int
         L.o_leaf_cells.restype = C.c_longlong
+        L.o_space_charge_adj.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.o_store_size.restype = C.c_size_t
+        L.o_store_size.argtypes = [C.c_void_p, C.c_int, C.c_int]
         for name in ("o_space_charge", "o_fld_from_pot", "o_average_dens"):
             getattr(L, name).argtypes = None
         _lib = L
@@ -481,6 +484,103 @@ class OStreamTree(OTree):
 
     def box_summaries(self):
         return box_summaries(self, self.v)
+
+    # --- the interface of a5_streamer_amd.adjoint.FrozenWindow -------------
+    def set_cc_all(self, iv, data):
+        """Tree.set_cc_all: row id-1 holds box id's block at its start."""
+        b = self.boxes()
+        for i in range(len(b["lvl"])):
+            if b["in_use"][i]:
+                n = self.box_len(int(b["n_cell"][i]))
+                self.set_cc(i + 1, iv, np.asarray(data[i][:n], float))
+
+    def get_cc_all(self, iv):
+        b = self.boxes()
+        out = np.zeros((len(b["lvl"]), self.box_len(self.n_cell)))
+        for i in range(len(b["lvl"])):
+            if b["in_use"][i]:
+                nc = int(b["n_cell"][i])
+                out[i, :self.box_len(nc)] = self.get_cc(i + 1, iv, nc)
+        return out
+
+    def set_table(self, rows_cols, x_min, inv_dx):
+        self.tbl.set_rows(rows_cols)
+
+    def table_adjoint(self, clear=False):
+        """The accumulated table-row adjoints as (n_cols, n_rows)."""
+        if getattr(self, "tgrad", None) is None:
+            self.tgrad = np.zeros(self.tbl.n_rows * self.tbl.n_cols)
+        out = self.tgrad.reshape(self.tbl.n_cols, self.tbl.n_rows).copy()
+        if clear:
+            self.tgrad[:] = 0
+        return out
+
+    def _adj(self, adj):
+        a = OAdj(*[int(x) for x in adj])
+        if getattr(self, "tgrad", None) is None:
+            self.tgrad = np.zeros(self.tbl.n_rows * self.tbl.n_cols)
+        return a
+
+    def update_solution_adjoint(self, dt, adj):
+        update_solution_adj(self, self.v, self.tbl, dt, self._adj(adj), self.tgrad)
+
+    def consistent_fluxes_adjoint(self, lF):
+        consistent_fluxes_adj(self, lF)
+
+    def fluxes_koren_adjoint(self, dt, adj):
+        fluxes_koren_adj(self, self.v, self.tbl, dt, self._adj(adj), self.tgrad)
+
+    def tree_copy_cc_adjoint(self, lfrom, lto):
+        self.L.o_tree_copy_cc_adj(self.h, C.c_int(lfrom), C.c_int(lto))
+
+    def average_dens_adjoint(self, le, le_old, li, li_old):
+        self.L.o_average_dens_adj(self.h, C.c_int(le), C.c_int(le_old), C.c_int(li),
+                                  C.c_int(li_old))
+
+    def space_charge_adjoint(self, fac, lrhs, le, li):
+        self.L.o_space_charge_adj(self.h, fac, lrhs, le, li)
+
+    def restrict_tree_adjoint(self, il):
+        self.L.o_restrict_tree_adj(self.h, C.c_int(il))
+
+    def gc_tree_adjoint(self, iv, il, rb, bc):
+        if rb not in (1, 2) or bc not in (1, 2) or iv == il:
+            raise ValueError("gc_tree_adjoint: rb, bc in (1, 2) and il != iv")
+        self.L.o_gc_tree_adj(self.h, C.c_int(iv), C.c_int(il), C.c_int(rb), C.c_int(bc))
+
+    def fld_from_pot_adjoint(self, lE, lfld, lphi):
+        self.L.o_fld_from_pot_adj(self.h, C.byref(self.v), C.c_int(lE), C.c_int(lfld),
+                                  C.c_int(lphi))
+
+    def fas_fmg_adjoint(self, adj):
+        a, pa = _i(np.asarray(adj))
+        vbar = np.zeros(6)
+        if self.L.o_fas_fmg_adj(self.h, pa, _d(vbar)[1]) != 0:
+            raise ValueError("fas_fmg_adjoint: no permittivity or Helmholtz term")
+        return vbar[:2 * self.dim].copy()
+
+    def state_store(self, slot, ivs, fvs=(), load=False, clear=False):
+        """Tree.state_store: whole variables kept in host memory per slot."""
+        ivs, fvs = [int(x) for x in ivs], [int(x) for x in fvs]
+        a, pa = _i(np.asarray(ivs or [0]))
+        f, pf = _i(np.asarray(fvs or [0]))
+        n = int(self.L.o_store_size(self.h, len(ivs), len(fvs)))
+        if clear:
+            buf = np.zeros(n)
+            self.L.o_store_vars(self.h, len(ivs), pa, len(fvs), pf, _d(buf)[1], 1)
+            return
+        if not hasattr(self, "_store"):
+            self._store = {}
+        key = (tuple(ivs), tuple(fvs), self.info()["max_id"])
+        if load:
+            if slot not in self._store or self._store[slot][0] != key:
+                raise ValueError("state_store: slot %d holds no such state" % slot)
+            buf = self._store[slot][1]
+            self.L.o_store_vars(self.h, len(ivs), pa, len(fvs), pf, _d(buf)[1], 1)
+        else:
+            buf = np.zeros(n)
+            self.L.o_store_vars(self.h, len(ivs), pa, len(fvs), pf, _d(buf)[1], 0)
+            self._store[slot] = (key, buf)
 
     def prolong1_to(self, ids, iv, add=False):
         for i in np.atleast_1d(ids):

@@ -21,8 +21,10 @@ reaches the rounding floor instead of levelling off at a fixed error,
 which a wrong term would leave). At h = 1e-6 it stays below 1.5e-6; the
 bound is 1e-5. Each reverse operation is pinned exactly on its own
 (tests/test_gpu_step_adjoint.py, tests/test_gpu_mg_adjoint.py,
-tests/test_gpu_adjoint.py). The electron count and the head position are
-checked apart, since their sizes differ by 15 orders."""
+tests/test_gpu_adjoint.py), and the whole gradient is compared with the
+oracle's FP64 reverse mode of the same window. The electron count and
+the head position are checked apart, since their sizes differ by 15
+orders."""
 import numpy as np
 import pytest
 
@@ -120,6 +122,46 @@ def test_window_gradient_against_central_differences(which):
               flush=True)
         worst = max(worst, abs(fd - ad) / scale)
     assert worst <= 1e-5, worst
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_window_gradient_matches_oracle_reverse_mode(which):
+    """The same window's gradient computed by the oracle's FP64 reverse mode
+    (the same FrozenWindow code over OStreamTree, whose reverse routines are
+    pinned by the dot-product identity and by central differences on the
+    CPU, tests/test_oracle_step_adjoint.py) on the same time steps: the
+    forward passes agree bit for bit, and the reverse passes differ only in
+    the order of the library's atomic sums (table rows, the potential's
+    adjoint through ghost and restriction transposes), so the gradients of
+    the table, of V and of the start densities agree to 1e-10 of their
+    largest entry."""
+    import torch
+    from a5_streamer_amd import adjoint as AD
+    print("window gradient vs oracle, loss term", which, flush=True)
+    run, win = _window(6, 4)
+    geo = AD.LeafGeometry(win.T)
+    win.forward()
+    ref = S.StreamerRun(O.OStreamTree, S.Config(**CFG))
+    ref.initialise()
+    rwin = AD.FrozenWindow(ref, 6, every=4, dts=win.dts)
+    rwin.forward()
+    c = run.c
+    grads = []
+    for w in (win, rwin):
+        ne = torch.from_numpy(w.T.get_cc_all(S.I_ELEC)).requires_grad_(True)
+        E = torch.from_numpy(w.T.get_cc_all(S.I_FLD)).requires_grad_(True)
+        AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1 + which].backward()
+        gne = ne.grad.numpy() if ne.grad is not None else np.zeros(ne.shape)
+        gE = E.grad.numpy() if E.grad is not None else np.zeros(E.shape)
+        grads.append(w.backward(gne, gE))
+    (gt, gv, gn, gi), (rt, rv, rn, ri) = grads
+    assert np.abs(rt).max() > 0 and rv != 0 and np.abs(rn).max() > 0
+    for a, r, name in ((gt, rt, "table"), (gn, rn, "n_e start"), (gi, ri, "n_i start")):
+        err = np.abs(a - r).max() / np.abs(r).max()
+        print("  %s: max |gpu - oracle| / max |oracle| = %.2e" % (name, err), flush=True)
+        assert err <= 1e-10, (name, err)
+    print("  V: %.16e %.16e" % (gv, rv), flush=True)
+    assert abs(gv - rv) <= 1e-10 * abs(rv)
 
 
 def test_window_gradient_does_not_depend_on_checkpoint_spacing():

@@ -1,0 +1,221 @@
+"""The oracle's reverse mode of a whole time step (o_*_adj in
+oracle/a5_oracle.c, test infrastructure): each reverse routine against its
+forward routine by the dot-product identity, and the gradient of a window
+of steps stepped by the oracle (adjoint.FrozenWindow over OStreamTree)
+against central differences of its forward pass.
+
+Every forward routine here is positively homogeneous of degree one in the
+values it reads (linear; piecewise linear with a data-dependent case for
+the limited interpolation; the field norm of faces linear in the
+potential), so its Jacobian J at x satisfies J x = F(x) and the reverse
+routine, J^T at x, must give <w, F(x)> = <J^T w, x> for seeded x, w on every
+cell, ghosts included. The FMG with first_call false is affine in the
+potential, tmp, rhs and the face values V, so its transpose satisfies the
+identity with V's adjoints added. A dropped, doubled, misplaced or
+mis-signed term breaks it far above the bound (1e-12 of the sum of the
+absolute products). The window's forward pass is the oracle's step, bit for
+bit the reference drivers' (tests/test_oracle_ref_drivers.py)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import scenarios as S
+
+CASES = [(2, 0, (2, 1, 1)), (2, 1, (1, 2, 1)), (3, 0, (1, 1, 1))]
+I_E, I_I, I_EO, I_IO, I_PHI, I_FLD, I_RHS, I_EPS = 1, 2, 3, 4, 5, 6, 7, 9
+L_A, L_B, L_C, L_D = 11, 12, 13, 14
+F_FLD, LF_FLD = 2, 4
+
+
+def _rules(lvl, c):
+    if lvl < 2:
+        return 1
+    return 1 if (lvl < 4 and np.linalg.norm(c - 0.4) < 0.22) else -1
+
+
+def _tree(dim, coord, base, nc=8):
+    T = O.OStreamTree(dim, nc, 20, 4, 1.0 / nc, coarsen_to=2, coord=coord)
+    S.build_tree(T, dim, nc, _rules, n_base=base)
+    T.streamer_setup()
+    return T
+
+
+def _dot(a, b):
+    return float(np.sum(a * b)), float(np.sum(np.abs(a * b)))
+
+
+def _check(pairs_y, pairs_x, extra=0.0):
+    lhs = sum(_dot(w, y)[0] for w, y in pairs_y)
+    rhs = sum(_dot(xb, x)[0] for xb, x in pairs_x) + extra
+    scale = sum(_dot(w, y)[1] for w, y in pairs_y) + sum(_dot(xb, x)[1] for xb, x in pairs_x)
+    assert scale > 0
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs, scale)
+
+
+def _faces(T, ifc):
+    b = T.boxes()
+    return np.concatenate([T.get_fc(i + 1, d, ifc) for i in range(len(b["lvl"]))
+                           if b["in_use"][i] for d in range(1, T.dim + 1)])
+
+
+def _set_faces(T, ifc, rng):
+    b = T.boxes()
+    nf = (T.n_cell + 1) * T.n_cell ** (T.dim - 1)
+    vals = []
+    for i in range(len(b["lvl"])):
+        if b["in_use"][i]:
+            for d in range(1, T.dim + 1):
+                v = rng.standard_normal(nf)
+                T.set_fc(i + 1, d, ifc, v)
+                vals.append(v)
+    return np.concatenate(vals)
+
+
+def _rand(T, rng, iv, positive=False):
+    """Seeded values on every cell of every box; returns what the tree holds."""
+    sh = (T.info()["max_id"], T.box_len(T.n_cell))
+    v = np.exp(2.0 * rng.standard_normal(sh)) if positive else rng.standard_normal(sh)
+    T.set_cc_all(iv, v)
+    return T.get_cc_all(iv)
+
+
+@pytest.mark.parametrize("dim,coord,base", CASES)
+def test_oracle_step_reverse_routines(dim, coord, base):
+    T = _tree(dim, coord, base)
+    rng = np.random.default_rng(4)
+    # tree copy
+    x = [_rand(T, rng, iv) for iv in (I_E, I_EO)]
+    T.tree_copy_cc(I_E, I_EO)
+    y = [T.get_cc_all(iv) for iv in (I_E, I_EO)]
+    w = [_rand(T, rng, lv) for lv in (L_A, L_B)]
+    T.tree_copy_cc_adjoint(L_A, L_B)
+    _check(list(zip(w, y)), list(zip([T.get_cc_all(lv) for lv in (L_A, L_B)], x)))
+    # average
+    ivs, lvs = (I_E, I_EO, I_I, I_IO), (L_A, L_B, L_C, L_D)
+    x = [_rand(T, rng, iv) for iv in ivs]
+    T.average_dens()
+    y = [T.get_cc_all(iv) for iv in ivs]
+    w = [_rand(T, rng, lv) for lv in lvs]
+    T.average_dens_adjoint(*lvs)
+    _check(list(zip(w, y)), list(zip([T.get_cc_all(lv) for lv in lvs], x)))
+    # space charge
+    fac = 1.6022e-19 / 8.8541878176e-12
+    ivs, lvs = (I_RHS, I_E, I_I), (L_A, L_B, L_C)
+    x = [_rand(T, rng, iv) for iv in ivs]
+    T.space_charge(fac)
+    y = [T.get_cc_all(iv) for iv in ivs]
+    w = [_rand(T, rng, lv) for lv in lvs]
+    T.space_charge_adjoint(fac, *lvs)
+    _check(list(zip(w, y)), list(zip([T.get_cc_all(lv) for lv in lvs], x)))
+    # restriction
+    x = _rand(T, rng, I_E)
+    T.restrict_tree(I_E)
+    y = T.get_cc_all(I_E)
+    w = _rand(T, rng, L_A)
+    T.restrict_tree_adjoint(L_A)
+    _check([(w, y)], [(T.get_cc_all(L_A), x)])
+    # ghost cells, limited and plain interpolation, Neumann and Dirichlet
+    for rb, bc in ((1, 1), (2, 1), (2, 2), (1, 2)):
+        x = _rand(T, rng, I_E, positive=True)
+        T.gc_tree(I_E, rb, bc)
+        y = T.get_cc_all(I_E)
+        w = _rand(T, rng, L_A)
+        T.gc_tree_adjoint(I_E, L_A, rb, bc)
+        _check([(w, y)], [(T.get_cc_all(L_A), x)])
+    # field from potential
+    _rand(T, rng, I_EPS, positive=True)
+    phi, fld0 = _rand(T, rng, I_PHI), _rand(T, rng, I_FLD)
+    F0 = _set_faces(T, F_FLD, rng)
+    T.fld_from_pot()
+    y_phi, y_fld, y_F = T.get_cc_all(I_PHI), T.get_cc_all(I_FLD), _faces(T, F_FLD)
+    w_phi, w_fld = _rand(T, rng, L_A), _rand(T, rng, L_B)
+    w_F = _set_faces(T, LF_FLD, rng)
+    T.fld_from_pot_adjoint(L_B, LF_FLD, L_A)
+    xb_phi = T.get_cc_all(L_A)
+    _check([(w_phi, y_phi), (w_fld, y_fld), (w_F, y_F)],
+           [(xb_phi, phi), (T.get_cc_all(L_B), fld0), (_faces(T, LF_FLD), F0)])
+    # and off the ray: a central difference of <w, F(phi)>
+    delta = rng.standard_normal(phi.shape)
+    vals = []
+    for s in (1.0, -1.0):
+        T.set_cc_all(I_PHI, phi + s * 1e-6 * delta)
+        T.fld_from_pot()
+        vals.append(_dot(w_fld, T.get_cc_all(I_FLD))[0] + _dot(w_F, _faces(T, F_FLD))[0])
+    ad, sc = _dot(xb_phi - w_phi, delta)
+    assert abs((vals[0] - vals[1]) / 2e-6 - ad) <= 1e-6 * sc
+
+
+FMG_CASES = [
+    (2, 8, 0, (2, 1, 1), (0, 1, 2, 0)),
+    (2, 8, 1, (1, 2, 1), (0, 0, 1, 3)),
+    (3, 4, 0, (2, 1, 1), (2, 1, 0, 0, 1, 2)),
+]
+
+
+@pytest.mark.parametrize("dim,nc,coord,base,kinds", FMG_CASES)
+def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds):
+    T = O.OTree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
+    S.build_tree(T, dim, nc, _rules, n_base=base)
+    rng = np.random.default_rng(2)
+    V = rng.standard_normal(2 * dim)
+    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    T.__class__ = O.OStreamTree       # the whole-variable helpers
+    x = [_rand(T, rng, iv) for iv in (1, 2, 3)]
+    T.fas_fmg(False, False)
+    y = [T.get_cc_all(iv) for iv in (1, 2, 3)]
+    w = [_rand(T, rng, lv) for lv in (5, 6, 7)]
+    T.set_cc_all(8, np.zeros_like(w[0]))
+    vbar = T.fas_fmg_adjoint([5, 6, 7, 8])
+    xb = [T.get_cc_all(lv) for lv in (5, 6, 7)]
+    _check(list(zip(w, y)), list(zip(xb, x)), extra=float(np.dot(vbar, V)))
+    assert all(vbar[nb] == 0.0 for nb in range(2 * dim) if kinds[nb] < 2)
+
+
+def test_oracle_window_gradient_against_central_differences():
+    """The oracle's window gradient (4 steps of the 2D streamer on a frozen
+    mesh, a checkpoint every 2) against central differences of its forward
+    pass along seeded relative directions in (table, voltage): both loss
+    terms, bound 1e-5 of the directional derivative's absolute scale. The
+    difference's error is truncation above h = 1e-5 and rounding below it
+    (for the head position, measured over h = 1e-4 .. 1e-9: 1e-4 .. 6e-6 at
+    h = 1e-4, 4e-6 .. 9e-7 at 1e-5, 3e-5 .. 1.5e-5 at 1e-6, growing as h
+    shrinks); the smaller of the errors at h = 1e-5 and 1e-6 is checked. A
+    wrong term leaves an error that no h removes."""
+    import torch
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    run = St.StreamerRun(O.OStreamTree, St.Config("2d", lvls_max=5, adjoint=True))
+    run.initialise()
+    win = AD.FrozenWindow(run, 4, every=2)
+    geo = AD.LeafGeometry(run.T)
+    c = run.c
+
+    def losses():
+        ne = torch.from_numpy(run.T.get_cc_all(St.I_ELEC))
+        E = torch.from_numpy(run.T.get_cc_all(St.I_FLD))
+        return [float(v) for v in AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1:]]
+
+    rng = np.random.default_rng(11)
+    t0, v0 = win.table0, win.voltage0
+    for which in (0, 1):
+        win.forward()
+        ne = torch.from_numpy(run.T.get_cc_all(St.I_ELEC)).requires_grad_(True)
+        E = torch.from_numpy(run.T.get_cc_all(St.I_FLD)).requires_grad_(True)
+        AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1 + which].backward()
+        gne = ne.grad.numpy() if ne.grad is not None else np.zeros(ne.shape)
+        gE = E.grad.numpy() if E.grad is not None else np.zeros(E.shape)
+        gt, gv, _, _ = win.backward(gne, gE)
+        assert np.abs(gt).max() > 0 and gv != 0
+        for _ in range(3):
+            dt_ = t0 * rng.standard_normal(t0.shape)
+            dv = v0 * rng.standard_normal()
+            ad = float(np.sum(gt * dt_)) + gv * dv
+            sc = float(np.sum(np.abs(gt * dt_))) + abs(gv * dv)
+            errs = []
+            for h in (1e-5, 1e-6):
+                vals = []
+                for s in (1.0, -1.0):
+                    win.forward(t0 + s * h * dt_, v0 + s * h * dv)
+                    vals.append(losses()[which])
+                errs.append(abs((vals[0] - vals[1]) / (2 * h) - ad) / sc)
+            assert min(errs) <= 1e-5, (which, errs)
