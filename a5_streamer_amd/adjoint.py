@@ -171,7 +171,11 @@ class FrozenWindow:
 
     The run must have been made with Config(adjoint=True), without
     photoionization, a dielectric or a time-dependent field. The window
-    starts from the run's state at construction every time it runs."""
+    starts from the run's state at construction every time it runs.
+    Windows on one tree use the same store slots: a window made later
+    replaces an earlier one's start state and checkpoints, so only the
+    latest is live (reset() first to start the new one where the old one
+    started)."""
 
     SLOT_START, SLOT_FINAL, SLOT_CKPT = 0, 1, 2
 
@@ -254,6 +258,14 @@ class FrozenWindow:
         T = self.T
         T.set_table(np.asarray(table, float), self.x_min, self.inv_dx)
         T.mg_set_bc_value(self.top, float(voltage))
+
+    def reset(self):
+        """Puts the run's tree back where the window started: its start
+        state, and the table and voltage the window was made with. A pass
+        at other parameters (forward(table, voltage)) leaves them in the
+        tree, and a window made later takes the tree's as its defaults."""
+        self._load(self.SLOT_START)
+        self._set_params(self.table0, self.voltage0)
 
     def forward(self, table=None, voltage=None):
         """Runs the window from its start state with the transport table

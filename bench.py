@@ -574,13 +574,69 @@ def batch_runs(args, n=None, blocks=None, sample=None):
 ADJ_REGIONS = [(6, [0.0, 0.0], [32e-3, 32e-3]), (7, [11e-3, 18e-3], [21e-3, 32e-3])]
 
 
-def adjoint_window(args, steps=100, every=10, fd_dirs=2):
+def adjoint_oracle(run, win, cfg, cells, k=2):
+    """The oracle's FP64 reverse mode (the same FrozenWindow code over the
+    oracle's tree) on the first k steps of the adjoint window: the same
+    frozen mesh and the same time steps. Returns its forward + backward rate
+    on one host core (the adjoint's CPU baseline, a bounded sample of the
+    window) and the library's gradient of the same k-step window against
+    it: the forward states agree bit for bit, the gradients to the order of
+    the library's atomic sums."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    win.reset()
+    gw = AD.FrozenWindow(run, k, every=k, dts=win.dts[:k])
+    t0 = time.perf_counter()
+    ref = St.StreamerRun(O.OStreamTree, cfg)
+    ref.initialise()
+    t_init = time.perf_counter() - t0
+    rw = AD.FrozenWindow(ref, k, every=k, dts=win.dts[:k])
+    res = []
+    for w in (gw, rw):
+        geo = AD.LeafGeometry(w.T)
+        t1 = time.perf_counter()
+        w.forward()
+        t_f = time.perf_counter() - t1
+        ne = torch.from_numpy(w.T.get_cc_all(St.I_ELEC)).requires_grad_(True)
+        E = torch.from_numpy(w.T.get_cc_all(St.I_FLD)).requires_grad_(True)
+        AD.window_loss(ne, E, geo, cfg.domain_len, cfg.applied_fld)[0].backward()
+        t1 = time.perf_counter()
+        g = w.backward(ne.grad.numpy(), E.grad.numpy())
+        res.append((g, ne.detach().numpy(), t_f, time.perf_counter() - t1))
+    (gt, gv, gn, _), ne_g, _, _ = res[0]
+    (rt, rv, rn, _), ne_r, tf, tb = res[1]
+    del rw, ref
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+
+    return {
+        "cpu_baseline": {
+            "value": round(cells * k / (tf + tb) / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": "the oracle's FP64 reverse mode (C, one thread) on the first %d steps of "
+                      "the window, forward %.2f s + backward %.2f s (tree set-up %.1f s "
+                      "not counted)" % (k, tf, tb, t_init)},
+        "oracle_check": {"steps": k, "forward_bit_identical": bool(np.array_equal(ne_g, ne_r)),
+                         "table_grad_rel": rel(gt, rt), "voltage_grad_rel": abs(gv - rv) / abs(rv),
+                         "n_e_start_grad_rel": rel(gn, rn)},
+    }
+
+
+def adjoint_window(args, steps=100, every=10, fd_dirs=20):
     """Gradient of the adjoint workload's loss (electron count + softmax
     head position after `steps` steps on the frozen mesh) with respect to
     the transport table, the mobility and the applied voltage: forward pass
     with a checkpoint every `every` steps, backward pass by
     checkpoint-recompute, and central differences (h = 1e-6 relative) on
-    fd_dirs seeded directions."""
+    fd_dirs seeded directions. Per direction: the difference's error
+    relative to the directional derivative's absolute scale, and the gap
+    between its one-sided differences, |f(h) + f(-h) - 2 f(0)| / (h scale),
+    which is h f'' / scale on a smooth stretch and about the slope jump of
+    a limiter or clamp kink that the pair straddles."""
     import torch
     import a5_streamer_amd as A
     from a5_streamer_amd import adjoint as AD
@@ -605,6 +661,7 @@ def adjoint_window(args, steps=100, every=10, fd_dirs=2):
     ne = torch.from_numpy(T.get_cc_all(St.I_ELEC)).requires_grad_(True)
     E = torch.from_numpy(T.get_cc_all(St.I_FLD)).requires_grad_(True)
     loss, count, head = AD.window_loss(ne, E, geo, cfg.domain_len, cfg.applied_fld)
+    loss0 = float(loss.detach())
     loss.backward()
     t1 = time.perf_counter()
     gt, gv, _, _ = win.backward(ne.grad.numpy(), E.grad.numpy())
@@ -627,9 +684,11 @@ def adjoint_window(args, steps=100, every=10, fd_dirs=2):
         f = (vals[0] - vals[1]) / 2e-6
         a = float(np.sum(gt * dtb)) + gv * dv
         sc = float(np.sum(np.abs(gt * dtb))) + abs(gv * dv)
-        fd.append({"fd": f, "adjoint": a, "rel_err_of_scale": abs(f - a) / sc})
+        fd.append((abs(f - a) / sc, abs(vals[0] + vals[1] - 2 * loss0) / (1e-6 * sc)))
+    errs = np.array([e for e, _ in fd])
     slots = win.store_slots()
     n_boxes = int(np.sum(T.boxes()["in_use"]))
+    oracle_leg = {} if args.no_cpu_baseline else adjoint_oracle(run, win, cfg, cells)
     T.close()
     return {
         "metric": "adjoint of the 2D streamer on a frozen mesh, leaf Mcell-steps/s of "
@@ -648,7 +707,12 @@ def adjoint_window(args, steps=100, every=10, fd_dirs=2):
         "loss": {"electron_count": float(count.detach()), "head_position_m": float(head.detach())},
         "grad": {"voltage": gv, "mobility": g_mob,
                  "alpha_table_norm": float(np.linalg.norm(gt[St.T_ALPHA - 1]))},
-        "central_differences": fd,
+        "central_differences": {
+            "directions": fd_dirs, "h": 1e-6,
+            "rel_err_of_scale": {"max": float(errs.max()), "median": float(np.median(errs))},
+            "per_direction": [{"rel_err": float("%.3g" % e), "one_sided_gap": float("%.3g" % g)}
+                              for e, g in fd]} if fd_dirs else None,
+        **oracle_leg,
     }
 
 

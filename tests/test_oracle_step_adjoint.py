@@ -219,3 +219,40 @@ def test_oracle_window_gradient_against_central_differences():
                     vals.append(losses()[which])
                 errs.append(abs((vals[0] - vals[1]) / (2 * h) - ad) / sc)
             assert min(errs) <= 1e-5, (which, errs)
+
+
+def test_window_reset_and_checkpoint_spacing():
+    """A pass at other parameters leaves them in the run's tree, so a window
+    made afterwards takes them as its defaults; FrozenWindow.reset() puts
+    back the start state and the window's own table and voltage. After it, a
+    window with other checkpoint spacing (made on the same tree, so on the
+    same store slots, which windows use in turn) gives the same gradient bit
+    for bit: the recomputed segments repeat the forward pass exactly and the
+    oracle's sums run in one order."""
+    import torch
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    run = St.StreamerRun(O.OStreamTree, St.Config("2d", lvls_max=5, adjoint=True))
+    run.initialise()
+    geo = AD.LeafGeometry(run.T)
+    c = run.c
+
+    def grads(w):
+        w.forward()
+        ne = torch.from_numpy(run.T.get_cc_all(St.I_ELEC)).requires_grad_(True)
+        E = torch.from_numpy(run.T.get_cc_all(St.I_FLD)).requires_grad_(True)
+        AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[0].backward()
+        return w.backward(ne.grad.numpy(), E.grad.numpy())
+
+    win = AD.FrozenWindow(run, 3, every=3)
+    g1 = grads(win)
+    win.forward(win.table0 * 1.001, win.voltage0 * 1.001)
+    # what a window made now would take as its table
+    assert not np.array_equal(run.T.table_get()[0], win.table0)
+    win.reset()
+    assert np.array_equal(run.T.table_get()[0], win.table0)
+    w2 = AD.FrozenWindow(run, 3, every=1, dts=win.dts)
+    assert np.array_equal(w2.table0, win.table0) and w2.voltage0 == win.voltage0
+    g2 = grads(w2)
+    for a, b in zip(g1, g2):
+        np.testing.assert_array_equal(a, b)
