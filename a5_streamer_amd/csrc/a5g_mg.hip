@@ -1179,7 +1179,12 @@ __global__ __launch_bounds__(256) void k_coarse_rhs(Dev a, const int* __restrict
 // are recomputed from the current state (neighbour copy, refinement-boundary
 // extrapolation, physical rule) instead of read, so that no ghost pass has
 // to store them after the restriction; with copy_tmp they also go to tmp's
-// face ghost slots, as the stored values would have.
+// face ghost slots, as the stored values would have. Every global load of a
+// box (its potential, tmp of its interior, rhs and tmp of its ghost slots,
+// the face ghosts or the neighbour cells they copy) is issued before any
+// store: on a level of a few boxes the kernel's time is one workgroup's
+// chain of memory latencies, which loads taken one loop iteration at a time
+// (each behind a store to rhs or tmp) would lengthen many times over.
 template <int D, bool CYL, int NC, typename T, bool EPS, bool HALVE = false>
 __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __restrict__ list,
                                                         int n, int i_phi, int i_rhs, int i_tmp,
@@ -1194,6 +1199,9 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
   constexpr int TPB0 = BL > 128 ? 256 : (BL > 64 ? 128 : 64);
   constexpr int TPB = HALVE ? TPB0 / 2 : TPB0;
   constexpr int BPB = 256 / TPB;
+  constexpr int NITC = (2 * NCOL + TPB - 1) / TPB;  // interior cells per lane
+  constexpr int NITH = (NGH + TPB - 1) / TPB;       // face ghosts per lane
+  constexpr int NITX = (2 * NGC + TPB - 1) / TPB;   // ghost slots per lane
   __shared__ T sh[BPB * BL];
   const int slot = threadIdx.x / TPB, lt = threadIdx.x % TPB;
   const int li = xcd_block() * BPB + slot;
@@ -1218,59 +1226,118 @@ __global__ __launch_bounds__(256) void k_coarse_rhs_lds(Dev a, const int* __rest
     if (D == 3) g[tb] = t / NC + 1;
     return nb;
   };
+  auto gslot = [](int e) {
+    const int col = e >= NGC;
+    return col * HALF + NCOL + (e - col * NGC);
+  };
   const T* p = act ? boxp_t<T>(a, i_phi, id) : nullptr;
-  if (act) {
-    for (int e = lt; e < 2 * NCOL; e += TPB) {
-      int i, j, k;
-      const int x = cell(e, i, j, k);
-      s[L(i, j, k)] = p[x];
-    }
-    if (!fused)
-      for (int q = lt; q < NGH; q += TPB) {
-        int g[3];
-        face(q, g);
-        s[L(g[0], g[1], g[2])] = p[cidx<D>(NC, g[0], g[1], g[2])];
-      }
-  }
-  __syncthreads();
-  if (act && fused)
-    for (int q = lt; q < NGH; q += TPB) {
-      int g[3];
-      const int nb = face(q, g);
-      // reads interior cells of s only, writes a ghost slot
-      s[L(g[0], g[1], g[2])] = ghost_mg_u<D, T>(a, own, i_phi, id, NC, nb, g[0], g[1], g[2]);
-    }
-  __syncthreads();
   T* r = act ? boxp_t<T>(a, i_rhs, id) : nullptr;
   T* t = act ? boxp_t<T>(a, i_tmp, id) : nullptr;
+  T pv[NITC], tv[NITC], gv[NITH], rg[NITX], tg[NITX], pg[NITX];
+  int gnb[NITH];
+  if (act) {
+#pragma unroll
+    for (int u = 0; u < NITC; u++) {
+      const int e = lt + u * TPB;
+      if (e < 2 * NCOL) {
+        int i, j, k;
+        const int x = cell(e, i, j, k);
+        pv[u] = p[x];
+        tv[u] = t[x];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NITH; u++) {
+      const int q = lt + u * TPB;
+      gnb[u] = 0;
+      if (q < NGH) {
+        int g[3];
+        const int nb = face(q, g);
+        if (!fused) {
+          gv[u] = p[cidx<D>(NC, g[0], g[1], g[2])];
+        } else {
+          const int nb_id = a.nbs[id * 6 + nb - 1];
+          if (nb_id > 0) {
+            const int d = d_nb_dim(nb);
+            int sc[3] = {g[0], g[1], g[2]};
+            sc[d] = d_nb_low(nb) ? NC : 1;
+            gv[u] = boxp_t<T>(a, i_phi, nb_id)[cidx_in<D>(NC, sc[0], sc[1], sc[2])];
+          } else {
+            gnb[u] = nb;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NITX; u++) {
+      const int e = lt + u * TPB;
+      if (e < 2 * NGC) {
+        const int x = gslot(e);
+        rg[u] = r[x];
+        tg[u] = t[x];
+        if (copy_tmp) pg[u] = p[x];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NITC; u++) {
+      const int e = lt + u * TPB;
+      if (e < 2 * NCOL) {
+        int i, j, k;
+        cell(e, i, j, k);
+        s[L(i, j, k)] = pv[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NITH; u++) {
+      const int q = lt + u * TPB;
+      if (q < NGH) {
+        int g[3];
+        face(q, g);
+        // refinement-boundary and physical rules from the box's own cells
+        // (read from HBM: the LDS copy is not complete before the barrier)
+        if (gnb[u] > 0) gv[u] = ghost_mg<D, T>(a, p, i_phi, id, NC, gnb[u], g[0], g[1], g[2]);
+        s[L(g[0], g[1], g[2])] = gv[u];
+      }
+    }
+  }
+  __syncthreads();
   if (act) {
     const T inv_dr_sq = (T)(1 / (a.dr[id] * a.dr[id]));
     const int ioff = CYL ? (a.ix[id * 3] - 1) * NC : 0;
-    for (int e = lt; e < 2 * NCOL; e += TPB) {
+#pragma unroll
+    for (int u = 0; u < NITC; u++) {
+      const int e = lt + u * TPB;
+      if (e >= 2 * NCOL) break;
       int i, j, k;
       const int x = cell(e, i, j, k);
       T rv = EPS ? op_u<D, CYL, T>(a, id, NC, own, i, j, k, inv_dr_sq, ioff)
                  : lplh_u<D, CYL, T>(a, own, i, j, k, inv_dr_sq, ioff);
-      rv = rv + t[x];
+      rv = rv + tv[u];
       r[x] = rv;
-      if (copy_tmp) t[x] = own(i, j, k);
+      if (copy_tmp) t[x] = pv[u];
     }
-    for (int e = lt; e < 2 * NGC; e += TPB) {
-      const int col = e >= NGC;
-      const int x = col * HALF + NCOL + (e - col * NGC);
-      r[x] = r[x] + t[x];
-      if (copy_tmp) t[x] = p[x];
+#pragma unroll
+    for (int u = 0; u < NITX; u++) {
+      const int e = lt + u * TPB;
+      if (e >= 2 * NGC) break;
+      const int x = gslot(e);
+      r[x] = rg[u] + tg[u];
+      if (copy_tmp) t[x] = pg[u];
     }
   }
   if (!(copy_tmp && fused)) return;
   // tmp's face ghosts take the recomputed values, after the loop above has
-  // used the old ones and copied the stored (stale) ones
+  // copied the stored (stale) ones (possibly from another lane)
   __syncthreads();
   if (act) {
-    for (int q = lt; q < NGH; q += TPB) {
-      int g[3];
-      face(q, g);
-      t[cidx<D>(NC, g[0], g[1], g[2])] = s[L(g[0], g[1], g[2])];
+#pragma unroll
+    for (int u = 0; u < NITH; u++) {
+      const int q = lt + u * TPB;
+      if (q < NGH) {
+        int g[3];
+        face(q, g);
+        t[cidx<D>(NC, g[0], g[1], g[2])] = gv[u];
+      }
     }
     // the corner ghosts the rule 2V - c over a whole y ghost row writes
     // (k_gc), from the x ghosts recomputed above
