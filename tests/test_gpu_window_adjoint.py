@@ -133,8 +133,11 @@ def test_window_gradient_matches_oracle_reverse_mode(which):
     forward passes agree bit for bit, and the reverse passes differ only in
     the order of the library's atomic sums (table rows, the potential's
     adjoint through ghost and restriction transposes), so the gradients of
-    the table, of V and of the start densities agree to 1e-12 of their
-    largest entry (measured: 8e-15 at most)."""
+    the table and of the start densities agree to 1e-12 of their largest
+    entry (measured: 8e-15 at most). V's adjoint is a sum of boundary
+    contributions of both signs that largely cancel, so a reordered sum
+    moves it more relative to itself: bound 1e-10 (measured 1e-13 here,
+    1.4e-12 in 3D, 2.3e-12 on the 1.5M-cell window of the bench)."""
     import torch
     from a5_streamer_amd import adjoint as AD
     print("window gradient vs oracle, loss term", which, flush=True)
@@ -161,7 +164,53 @@ def test_window_gradient_matches_oracle_reverse_mode(which):
         print("  %s: max |gpu - oracle| / max |oracle| = %.2e" % (name, err), flush=True)
         assert err <= 1e-12, (name, err)
     print("  V: %.16e %.16e" % (gv, rv), flush=True)
-    assert abs(gv - rv) <= 1e-12 * abs(rv)
+    assert abs(gv - rv) <= 1e-10 * abs(rv)
+
+
+def test_3d_window_matches_oracle():
+    """The 3D driver's step in a window (an 8 mm cube of 2^3 base boxes,
+    4 levels; 4 steps, a checkpoint every 2): the forward pass equals the
+    oracle's window bit for bit, and the gradient of both loss terms with
+    respect to the table and the start densities equals the oracle's FP64
+    reverse mode (pinned by central differences on the CPU,
+    tests/test_oracle_step_adjoint.py) to 1e-12 of the largest entry, and
+    V's to 1e-10 of itself (a sum that cancels; see the 2D test)."""
+    import torch
+    import a5_streamer_amd as A
+    from a5_streamer_amd import adjoint as AD
+    print("3d window vs oracle", flush=True)
+    cfg = dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4,
+               seed_r0=[[4e-3, 4e-3, 5e-3]], seed_r1=[[4e-3, 4e-3, 6e-3]], seed_width=[0.6e-3],
+               applied_fld=2.5e6, n_boxes=4000, adjoint=True)
+    run = S.StreamerRun(A.Tree, S.Config(**cfg))
+    run.initialise()
+    win = AD.FrozenWindow(run, 4, every=2)
+    win.forward()
+    ref = S.StreamerRun(O.OStreamTree, S.Config(**cfg))
+    ref.initialise()
+    rwin = AD.FrozenWindow(ref, 4, every=2, dts=win.dts)
+    rwin.forward()
+    for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD):
+        np.testing.assert_array_equal(run.T.get_cc_all(iv), ref.T.get_cc_all(iv), err_msg=str(iv))
+    geo = AD.LeafGeometry(run.T)
+    c = run.c
+    for which in (0, 1):
+        grads = []
+        for w in (win, rwin):
+            w.forward()
+            ne = torch.from_numpy(w.T.get_cc_all(S.I_ELEC)).requires_grad_(True)
+            E = torch.from_numpy(w.T.get_cc_all(S.I_FLD)).requires_grad_(True)
+            AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1 + which].backward()
+            gne = ne.grad.numpy() if ne.grad is not None else np.zeros(ne.shape)
+            gE = E.grad.numpy() if E.grad is not None else np.zeros(E.shape)
+            grads.append(w.backward(gne, gE))
+        (gt, gv, gn, gi), (rt, rv, rn, ri) = grads
+        assert np.abs(rt).max() > 0 and rv != 0
+        for a, r, name in ((gt, rt, "table"), (gn, rn, "n_e start"), (gi, ri, "n_i start")):
+            err = np.abs(a - r).max() / np.abs(r).max()
+            print("  term %d %s: %.2e" % (which, name, err), flush=True)
+            assert err <= 1e-12, (which, name, err)
+        assert abs(gv - rv) <= 1e-10 * abs(rv), (which, gv, rv)
 
 
 def test_window_gradient_does_not_depend_on_checkpoint_spacing():

@@ -256,3 +256,56 @@ def test_window_reset_and_checkpoint_spacing():
     g2 = grads(w2)
     for a, b in zip(g1, g2):
         np.testing.assert_array_equal(a, b)
+
+
+CFG3D = dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4, seed_r0=[[4e-3, 4e-3, 5e-3]],
+             seed_r1=[[4e-3, 4e-3, 6e-3]], seed_width=[0.6e-3], applied_fld=2.5e6,
+             n_boxes=4000, adjoint=True)
+
+
+def test_oracle_3d_window_gradient_against_central_differences():
+    """The same check on the 3D driver's step (an 8 mm cube of 2^3 base
+    boxes, 408 boxes): 4 steps, a checkpoint every 2, two seeded relative
+    directions per loss term. Measured over h = 1e-4 .. 1e-8, the
+    difference's error falls to 1e-9 for the electron count and to
+    4e-10 .. 2e-7 for the head position at its best h (1e-6; truncation
+    above, rounding below); the smaller of the errors at h = 1e-6 and 1e-7
+    is checked against 1e-6."""
+    import torch
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    run = St.StreamerRun(O.OStreamTree, St.Config(**CFG3D))
+    run.initialise()
+    win = AD.FrozenWindow(run, 4, every=2)
+    geo = AD.LeafGeometry(run.T)
+    c = run.c
+
+    def term(which):
+        ne = torch.from_numpy(run.T.get_cc_all(St.I_ELEC))
+        E = torch.from_numpy(run.T.get_cc_all(St.I_FLD))
+        return float(AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1 + which])
+
+    rng = np.random.default_rng(3)
+    t0, v0 = win.table0, win.voltage0
+    for which in (0, 1):
+        win.forward()
+        ne = torch.from_numpy(run.T.get_cc_all(St.I_ELEC)).requires_grad_(True)
+        E = torch.from_numpy(run.T.get_cc_all(St.I_FLD)).requires_grad_(True)
+        AD.window_loss(ne, E, geo, c.domain_len, c.applied_fld)[1 + which].backward()
+        gne = ne.grad.numpy() if ne.grad is not None else np.zeros(ne.shape)
+        gE = E.grad.numpy() if E.grad is not None else np.zeros(E.shape)
+        gt, gv, _, _ = win.backward(gne, gE)
+        assert np.abs(gt).max() > 0 and gv != 0
+        for _ in range(2):
+            dtb = t0 * rng.standard_normal(t0.shape)
+            dv = v0 * rng.standard_normal()
+            ad = float(np.sum(gt * dtb)) + gv * dv
+            sc = float(np.sum(np.abs(gt * dtb))) + abs(gv * dv)
+            errs = []
+            for h in (1e-6, 1e-7):
+                vals = []
+                for s in (1.0, -1.0):
+                    win.forward(t0 + s * h * dtb, v0 + s * h * dv)
+                    vals.append(term(which))
+                errs.append(abs((vals[0] - vals[1]) / (2 * h) - ad) / sc)
+            assert min(errs) <= 1e-6, (which, errs)
