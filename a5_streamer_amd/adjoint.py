@@ -20,16 +20,18 @@ density includes its ghost cells and the cells the second ghost layer is
 taken from; rows of boxes the substep does not change pass their gradient
 through.
 
-A window of whole steps: FrozenWindow steps a run of the 2D or 3D driver
-(src/streamer_2d.f90:91-129, src/streamer_3d.f90 alike) with its mesh and
-its time steps frozen, and runs the reverse of every operation of a step
+A window of whole steps: FrozenWindow steps a run of the 2D, 3D or
+cylindrical driver (src/streamer_2d.f90:91-129; src/streamer_3d.f90 alike;
+src/streamer_cyl.f90 advances the time inside the substeps, which a
+constant field does not see) with its mesh and its time steps frozen, and runs the reverse of every operation of a step
 in the opposite order with the hand-written reverse kernels
 (a5g_*_adjoint, a5g_fas_fmg_adjoint). States are checkpointed in device
 memory (a5g_state_store) every `every` steps; the reverse pass recomputes
 each segment from its checkpoint once, keeping the three states of every
 step that its reverse reads (the start, the state after the first
-substep's field solve and the state after the second substep). StreamerWindow maps the transport table and the applied voltage to
-the final densities and field; window_loss is the loss of the adjoint
+substep's field solve and the state after the second substep).
+StreamerWindow maps the transport table and the applied voltage to the
+final densities and field; window_loss is the loss of the adjoint
 workload (integrated electron count plus the softmax-smoothed head
 position).
 
@@ -169,9 +171,10 @@ class FrozenWindow:
     steps_amr steps, as the drivers take it), differentiable in the
     transport table, the applied voltage and the densities it starts from.
 
-    The run must have been made with Config(adjoint=True), without
-    photoionization, a dielectric or a time-dependent field. The window
-    starts from the run's state at construction every time it runs.
+    The run (2D, 3D or cylindrical) must have been made with
+    Config(adjoint=True), without photoionization, a dielectric or a
+    time-dependent field. The window starts from the run's state at
+    construction every time it runs.
     Windows on one tree use the same store slots: a window made later
     replaces an earlier one's start state and checkpoints, so only the
     latest is live (reset() first to start the new one where the old one
@@ -185,8 +188,6 @@ class FrozenWindow:
             raise ValueError("FrozenWindow: the run needs Config(adjoint=True)")
         if c.photoi or c.epsilon_diel != 1.0 or c.fld_mod_t0 < 1e98:
             raise ValueError("FrozenWindow: no photoionization, dielectric or field modulation")
-        if run.v["coord"] != 0:
-            raise ValueError("FrozenWindow: Cartesian runs only")
         if n_steps < 1 or every < 1:
             raise ValueError("FrozenWindow: n_steps and every must be positive")
         self.run, self.T = run, run.T
@@ -415,7 +416,8 @@ def streamer_window(window, table, voltage):
 
 class LeafGeometry:
     """Per cell of the Tree.get_cc_all layout: the leaf-interior mask, the
-    cell volume (dr^D) and the coordinate along the last axis (the
+    cell volume (dr^D; on a cylindrical (r, z) tree the ring 2 pi r dr^2,
+    r the cell centre's radius) and the coordinate along the last axis (the
     propagation axis of the drivers' streamers)."""
 
     def __init__(self, T):
@@ -428,15 +430,24 @@ class LeafGeometry:
         inner = np.zeros([m] * D, bool)
         inner[(slice(1, -1),) * D] = True
         inner = inner.ravel(order="F")
-        # the last axis' index of each cell of a box (x fastest)
+        # the first and the last axis' index of each cell of a box (x fastest)
+        first = np.arange(m ** D) % m
         last = (np.arange(m ** D) // m ** (D - 1)) if D > 1 else np.arange(m)
-        for lvl in range(1, T.info()["max_lvl"] + 1):
-            for i in T.level(lvl)[1]:
-                r = int(i) - 1
-                dr = b["dr"][r]
-                self.mask[r] = inner
+        cyl = T.info()["coord"] == 1
+        # the leaves from the box records (in use, no children): the level
+        # lists of the top level are not rebuilt once refinement reaches
+        # lvls_max (m_aX_core.f90:497-531), and its new leaves are missing
+        # there until a tidy_up
+        leaves = np.nonzero((b["in_use"] != 0) & (b["children"][:, 0] == 0) & (b["lvl"] >= 1))[0]
+        for r in leaves:
+            dr = b["dr"][r]
+            self.mask[r] = inner
+            if cyl:
+                rad = b["r_min"][r, 0] + (first[inner] - 0.5) * dr
+                self.vol[r, inner] = 2 * np.pi * rad * dr ** 2
+            else:
                 self.vol[r, inner] = dr ** D
-                self.z[r] = b["r_min"][r, D - 1] + (last - 0.5) * dr
+            self.z[r] = b["r_min"][r, D - 1] + (last - 0.5) * dr
 
 
 def window_loss(ne, E, geo, domain_len, e_ref, tau=1e-4):

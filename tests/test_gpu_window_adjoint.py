@@ -30,6 +30,7 @@ import pytest
 
 import oracle as O
 from a5_streamer_amd import streamer as S
+from window_cases import WINDOWS
 
 pytestmark = pytest.mark.gpu
 
@@ -167,28 +168,27 @@ def test_window_gradient_matches_oracle_reverse_mode(which):
     assert abs(gv - rv) <= 1e-10 * abs(rv)
 
 
-def test_3d_window_matches_oracle():
-    """The 3D driver's step in a window (an 8 mm cube of 2^3 base boxes,
-    4 levels; 4 steps, a checkpoint every 2): the forward pass equals the
-    oracle's window bit for bit, and the gradient of both loss terms with
-    respect to the table and the start densities equals the oracle's FP64
-    reverse mode (pinned by central differences on the CPU,
+@pytest.mark.parametrize("name", sorted(WINDOWS))
+def test_window_3d_and_cyl_matches_oracle(name):
+    """The 3D and the cylindrical driver's step in a window (tests/
+    window_cases.py; a checkpoint every 2 steps): the forward pass equals
+    the oracle's window bit for bit, and the gradient of both loss terms
+    with respect to the table and the start densities equals the oracle's
+    FP64 reverse mode (pinned by central differences on the CPU,
     tests/test_oracle_step_adjoint.py) to 1e-12 of the largest entry, and
     V's to 1e-10 of itself (a sum that cancels; see the 2D test)."""
     import torch
     import a5_streamer_amd as A
     from a5_streamer_amd import adjoint as AD
-    print("3d window vs oracle", flush=True)
-    cfg = dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4,
-               seed_r0=[[4e-3, 4e-3, 5e-3]], seed_r1=[[4e-3, 4e-3, 6e-3]], seed_width=[0.6e-3],
-               applied_fld=2.5e6, n_boxes=4000, adjoint=True)
+    print("window vs oracle,", name, flush=True)
+    cfg, n_steps = WINDOWS[name]
     run = S.StreamerRun(A.Tree, S.Config(**cfg))
     run.initialise()
-    win = AD.FrozenWindow(run, 4, every=2)
+    win = AD.FrozenWindow(run, n_steps, every=2)
     win.forward()
     ref = S.StreamerRun(O.OStreamTree, S.Config(**cfg))
     ref.initialise()
-    rwin = AD.FrozenWindow(ref, 4, every=2, dts=win.dts)
+    rwin = AD.FrozenWindow(ref, n_steps, every=2, dts=win.dts)
     rwin.forward()
     for iv in (S.I_ELEC, S.I_PION, S.I_PHI, S.I_FLD):
         np.testing.assert_array_equal(run.T.get_cc_all(iv), ref.T.get_cc_all(iv), err_msg=str(iv))
@@ -206,11 +206,11 @@ def test_3d_window_matches_oracle():
             grads.append(w.backward(gne, gE))
         (gt, gv, gn, gi), (rt, rv, rn, ri) = grads
         assert np.abs(rt).max() > 0 and rv != 0
-        for a, r, name in ((gt, rt, "table"), (gn, rn, "n_e start"), (gi, ri, "n_i start")):
+        for a, r, what in ((gt, rt, "table"), (gn, rn, "n_e start"), (gi, ri, "n_i start")):
             err = np.abs(a - r).max() / np.abs(r).max()
-            print("  term %d %s: %.2e" % (which, name, err), flush=True)
-            assert err <= 1e-12, (which, name, err)
-        assert abs(gv - rv) <= 1e-10 * abs(rv), (which, gv, rv)
+            print("  term %d %s: %.2e" % (which, what, err), flush=True)
+            assert err <= 1e-12, (name, which, what, err)
+        assert abs(gv - rv) <= 1e-10 * abs(rv), (name, which, gv, rv)
 
 
 def test_window_gradient_does_not_depend_on_checkpoint_spacing():

@@ -258,25 +258,26 @@ def test_window_reset_and_checkpoint_spacing():
         np.testing.assert_array_equal(a, b)
 
 
-CFG3D = dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4, seed_r0=[[4e-3, 4e-3, 5e-3]],
-             seed_r1=[[4e-3, 4e-3, 6e-3]], seed_width=[0.6e-3], applied_fld=2.5e6,
-             n_boxes=4000, adjoint=True)
+from window_cases import WINDOWS  # noqa: E402
 
 
-def test_oracle_3d_window_gradient_against_central_differences():
-    """The same check on the 3D driver's step (an 8 mm cube of 2^3 base
-    boxes, 408 boxes): 4 steps, a checkpoint every 2, two seeded relative
-    directions per loss term. Measured over h = 1e-4 .. 1e-8, the
-    difference's error falls to 1e-9 for the electron count and to
-    4e-10 .. 2e-7 for the head position at its best h (1e-6; truncation
-    above, rounding below); the smaller of the errors at h = 1e-6 and 1e-7
-    is checked against 1e-6."""
+@pytest.mark.parametrize("name", sorted(WINDOWS))
+def test_oracle_window_gradient_3d_and_cyl(name):
+    """The same check on the 3D and the cylindrical driver's step: a
+    checkpoint every 2 steps, two seeded relative directions per loss term.
+    Measured over h = 1e-4 .. 1e-8, the difference's error at its best h
+    (1e-6 .. 1e-7; truncation above, rounding below) is 1e-9 (3D) and
+    2e-10 .. 9e-9 (cylindrical) for the electron count, 4e-10 .. 2e-7 and
+    6e-8 .. 2e-7 for the head position; the smaller of the errors at
+    h = 1e-6 and 1e-7 is checked against 1e-6. The cylindrical count is the
+    ring volume 2 pi r dr dz's (LeafGeometry)."""
     import torch
     from a5_streamer_amd import adjoint as AD
     from a5_streamer_amd import streamer as St
-    run = St.StreamerRun(O.OStreamTree, St.Config(**CFG3D))
+    cfg, n_steps = WINDOWS[name]
+    run = St.StreamerRun(O.OStreamTree, St.Config(**cfg))
     run.initialise()
-    win = AD.FrozenWindow(run, 4, every=2)
+    win = AD.FrozenWindow(run, n_steps, every=2)
     geo = AD.LeafGeometry(run.T)
     c = run.c
 
@@ -309,3 +310,25 @@ def test_oracle_3d_window_gradient_against_central_differences():
                     vals.append(term(which))
                 errs.append(abs((vals[0] - vals[1]) / (2 * h) - ad) / sc)
             assert min(errs) <= 1e-6, (which, errs)
+
+
+@pytest.mark.parametrize("variant", ["2d", "3d", "cyl"])
+def test_leaf_geometry_volumes(variant):
+    """LeafGeometry's leaf-cell volumes add up to the domain's: L^2, L^3,
+    and on the cylindrical (r, z) tree pi R^2 Z (rings 2 pi r dr dz); its
+    mask holds the leaf interiors only and its coordinate is the last
+    axis' cell centre."""
+    from a5_streamer_amd import adjoint as AD
+    from a5_streamer_amd import streamer as St
+    kw = dict(WINDOWS["3d"][0]) if variant == "3d" else (
+        dict(WINDOWS["cyl"][0]) if variant == "cyl" else dict(variant="2d", lvls_max=5, adjoint=True))
+    run = St.StreamerRun(O.OStreamTree, St.Config(**kw))
+    run.initialise()
+    geo = AD.LeafGeometry(run.T)
+    L = run.c.domain_len
+    want = {"2d": L * L, "3d": L ** 3, "cyl": np.pi * L ** 2 * (2 * L)}[variant]
+    assert abs(geo.vol.sum() - want) <= 1e-12 * want
+    assert (geo.vol[~geo.mask] == 0).all() and (geo.vol[geo.mask] > 0).all()
+    z = geo.z[geo.mask]
+    top = 2 * L if variant == "cyl" else L
+    assert z.min() > 0 and z.max() < top

@@ -1,0 +1,15 @@
+"""Windows of the 3D and the cylindrical driver's steps for the reverse-mode
+tests: (Config keywords, steps). The 3D one is an 8 mm cube of 2^3 base
+boxes (408 boxes); the cylindrical one 16 x 32 mm with the seed on the axis
+(46 boxes) at 4 MV/m, so that the electron count depends on the parameters
+well above its rounding (at 2 MV/m its directional derivatives are 2.5e-5
+of it)."""
+
+WINDOWS = {
+    "3d": (dict(variant="3d", domain_len=8e-3, n_base=2, lvls_max=4,
+                seed_r0=[[4e-3, 4e-3, 5e-3]], seed_r1=[[4e-3, 4e-3, 6e-3]],
+                seed_width=[0.6e-3], applied_fld=2.5e6, n_boxes=4000, adjoint=True), 4),
+    "cyl": (dict(variant="cyl", domain_len=16e-3, n_base=(1, 2), lvls_max=8,
+                 seed_r0=[[0.0, 24e-3]], seed_r1=[[0.0, 24e-3]], applied_fld=4.0e6,
+                 adjoint=True), 6),
+}
