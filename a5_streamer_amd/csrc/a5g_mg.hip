@@ -902,7 +902,20 @@ __global__ __launch_bounds__(256) void k_resid_rstr_lds(Dev a, const int* __rest
   constexpr int NITC = (NCELL + TPB - 1) / TPB;
   constexpr int NITG = (NGH + TPB - 1) / TPB;
   T* s = sh + slot * BLS;
+  // The tile in LDS: a whole box (SPLIT 1) split by parity into 2^D blocks
+  // of (N2/2)^D, cell (i, j, k) in block (i&1, j&1, k&1) at (i>>1, j>>1,
+  // k>>1). A lane restricts to one parent cell, so neighbouring lanes take
+  // fine cells two apart: in lexicographic order their stencil reads were
+  // two doubles apart and conflicted in the LDS banks (3.9 stall cycles per
+  // active LDS cycle); here they are adjacent. Slabs (SPLIT > 1) keep the
+  // lexicographic order.
+  constexpr int H2 = N2 / 2;
+  constexpr int PBLK = D == 2 ? H2 * H2 : H2 * H2 * H2;
   auto L = [&](int i, int j, int k) {
+    if (SPLIT == 1) {
+      const int sub = (i & 1) + 2 * (j & 1) + (D == 3 ? 4 * (k & 1) : 0);
+      return sub * PBLK + (i >> 1) + H2 * ((j >> 1) + (D == 3 ? H2 * (k >> 1) : 0));
+    }
     return D == 2 ? i + N2 * j : i + N2 * (j + N2 * (k - kbase));
   };
   auto own = [&](int i, int j, int k) { return s[L(i, j, k)]; };
