@@ -171,8 +171,8 @@ class FrozenWindow:
     steps_amr steps, as the drivers take it), differentiable in the
     transport table, the applied voltage and the densities it starts from.
 
-    The run (2D, 3D or cylindrical) must have been made with
-    Config(adjoint=True), without photoionization, a dielectric or a
+    The run (2D, 3D or cylindrical, with or without a dielectric) must have
+    been made with Config(adjoint=True), without photoionization or a
     time-dependent field. The window starts from the run's state at
     construction every time it runs.
     Windows on one tree use the same store slots: a window made later
@@ -186,8 +186,8 @@ class FrozenWindow:
         c = run.c
         if not c.adjoint:
             raise ValueError("FrozenWindow: the run needs Config(adjoint=True)")
-        if c.photoi or c.epsilon_diel != 1.0 or c.fld_mod_t0 < 1e98:
-            raise ValueError("FrozenWindow: no photoionization, dielectric or field modulation")
+        if c.photoi or c.fld_mod_t0 < 1e98:
+            raise ValueError("FrozenWindow: no photoionization or field modulation")
         if n_steps < 1 or every < 1:
             raise ValueError("FrozenWindow: n_steps and every must be positive")
         self.run, self.T = run, run.T

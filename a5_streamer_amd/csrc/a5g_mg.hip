@@ -3321,9 +3321,12 @@ void autotune(Pool& P, int reps, std::vector<double>& rec) {
 // (not of this library's fused one, which computes the same values), and
 // needs no state of the forward solve. Adjoint variables: lp (potential),
 // lt (tmp), lr (rhs), lw (work: the residual restricted by update_coarse);
-// the adjoints of V go to vbar (one per face). Normal boxes (no permittivity
-// tags) and the Laplacian only. The transposes scatter with atomics, so the
-// sums arrive in an order of their own.
+// the adjoints of V go to vbar (one per face). The operators are those the
+// forward solve used per box: the Laplacian, or for boxes tagged with a
+// permittivity mg$D_box_lpld, gsrb_lpld and corr_lpld (their weights depend
+// on eps only, which the window holds fixed); no Helmholtz term. The
+// transposes scatter with atomics, so the sums arrive in an order of their
+// own.
 // ---------------------------------------------------------------------------
 
 // a$D_gc_box's rules on one box (m_aX_gc.f90:50-68, with mg$D_sides_rb and
@@ -3422,12 +3425,26 @@ __global__ __launch_bounds__(256) void k_adj_gc(Dev a, const int* __restrict__ l
 // One half-sweep (mg$D_box_gsrb_lpl, m_aX_mg.f90:769-811; mg2_box_gsrb_clpl,
 // :1413-1439) transposed: each updated cell's adjoint goes to its
 // neighbours and to its rhs, and is cleared
+// the cidx of the neighbours of interior cell (i, j, k) in the order i-1,
+// i+1, j-1, j+1, k-1, k+1 of the reference's u(:) and a(:)
+template <int D>
+__device__ __forceinline__ void nb_cells(int nc, int i, int j, int k, int nbx[6]) {
+  nbx[0] = cidx<D>(nc, i - 1, j, k);
+  nbx[1] = cidx<D>(nc, i + 1, j, k);
+  nbx[2] = cidx<D>(nc, i, j - 1, k);
+  nbx[3] = cidx<D>(nc, i, j + 1, k);
+  nbx[4] = D == 3 ? cidx<D>(nc, i, j, k - 1) : 0;
+  nbx[5] = D == 3 ? cidx<D>(nc, i, j, k + 1) : 0;
+}
+
 template <int D, bool CYL>
 __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__ list, int n,
                                                   int nc, int cntr, int tpb, int il, int ir) {
   BOX_PROLOGUE
   double* L = boxp(a, il, id);
   double* R = boxp(a, ir, id);
+  const double* ev =
+      (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
   const double dx2 = a.dr[id] * a.dr[id];
   const int half = nc >> 1;
   const int ncol = (D == 2 ? nc * nc : nc * nc * nc) >> 1;
@@ -3442,6 +3459,28 @@ __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__
     const double yb = *yp;
     *yp = 0;
     if (yb == 0) continue;
+    if (ev != nullptr) {
+      // mg$D_box_gsrb_lpld: (sum c_m rf_m u_m - h^2 rhs) / sum c_m rf_m with
+      // c_m = 2 eps_0 eps_m / (eps_0 + eps_m)
+      const int c0 = cidx<D>(nc, i, j, k);
+      int nbx[6];
+      nb_cells<D>(nc, i, j, k, nbx);
+      double c[6], s2 = 0;
+      const double a0 = ev[c0];
+#pragma unroll
+      for (int m = 0; m < 2 * D; m++) {
+        c[m] = 2 * a0 * ev[nbx[m]] / (a0 + ev[nbx[m]]);
+        if (CYL && m < 2) {
+          const double den = i + ioff - 0.5;
+          c[m] = c[m] * ((double)(i + ioff - 1 + m) / den);
+        }
+        s2 = s2 + c[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 2 * D; m++) atomicAdd(L + nbx[m], c[m] / s2 * yb);
+      R[c0] += -dx2 / s2 * yb;
+      continue;
+    }
     double wxm = w, wxp = w;
     if (CYL) {
       const double den = i + ioff - 0.5;
@@ -3470,6 +3509,8 @@ __global__ __launch_bounds__(256) void k_adj_op(Dev a, const int* __restrict__ l
   BOX_PROLOGUE
   double* S = boxp(a, isrc, id);
   double* Dp = boxp(a, idst, id);
+  const double* ev =
+      (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, id) : nullptr;
   const double inv = 1 / (a.dr[id] * a.dr[id]);
   const int ioff = CYL ? (a.ix[id * 3] - 1) * nc : 0;
   const int total = D == 2 ? nc * nc : nc * nc * nc;
@@ -3479,6 +3520,25 @@ __global__ __launch_bounds__(256) void k_adj_op(Dev a, const int* __restrict__ l
     const double sb = sgn * S[c0];
     if (clear_src) S[c0] = 0;
     if (sb == 0) continue;
+    if (ev != nullptr) {
+      // mg$D_box_lpld: 2 inv sum_m rf_m eps_0 eps_m / (eps_0 + eps_m) (u_m - u_0)
+      int nbx[6];
+      nb_cells<D>(nc, i, j, k, nbx);
+      const double a0 = ev[c0];
+      double sum = 0;
+#pragma unroll
+      for (int m = 0; m < 2 * D; m++) {
+        double cm = a0 * ev[nbx[m]] / (a0 + ev[nbx[m]]);
+        if (CYL && m < 2) {
+          const double den = i + ioff - 0.5;
+          cm = ((double)(i + ioff - 1 + m) / den) * cm;
+        }
+        atomicAdd(Dp + nbx[m], 2 * inv * cm * sb);
+        sum = sum + cm;
+      }
+      atomicAdd(Dp + c0, -2 * inv * sum * sb);
+      continue;
+    }
     double wxm = inv, wxp = inv;
     if (CYL) {
       const double den = i + ioff - 0.5;
@@ -3542,6 +3602,10 @@ __global__ __launch_bounds__(256) void k_adj_corr(Dev a, const int* __restrict__
   child_off<D>(a, id, nc, 0, off);
   const double* F = boxp(a, il, id);
   double* Q = boxp(a, ilt, pid);
+  // mg$D_box_corr_lpld for a child tagged with a permittivity (the parent's
+  // eps weights the prolongation)
+  const double* ep =
+      (a.tag != nullptr && a.tag[id] > kTagNormal) ? boxp(a, a.i_eps, pid) : nullptr;
   const int total = D == 2 ? nc * nc : nc * nc * nc;
   for (int q = lt; q < total; q += tpb) {
     const int i = q % nc + 1, j = (q / nc) % nc + 1, k = D == 3 ? q / (nc * nc) + 1 : 0;
@@ -3549,6 +3613,25 @@ __global__ __launch_bounds__(256) void k_adj_corr(Dev a, const int* __restrict__
     if (fb == 0) continue;
     const int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
     const int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+    if (ep != nullptr) {
+      // 2D: f += 1/2 sum_m (a0 u0 + a_m u_m) / (a0 + a_m); 3D: f += 1/3 sum_m
+      // (a0 u0 + a_m (3/2 u_m - 1/2 u0)) / (a0 + a_m)
+      const int kc1 = D == 3 ? off[2] + ((k + 1) >> 1) : 0;
+      const int kc2 = D == 3 ? kc1 + 1 - 2 * (k & 1) : 0;
+      const int p0 = cidx<D>(pnc, ic1, jc1, kc1);
+      const int pm[3] = {cidx<D>(pnc, ic2, jc1, kc1), cidx<D>(pnc, ic1, jc2, kc1),
+                         D == 3 ? cidx<D>(pnc, ic1, jc1, kc2) : 0};
+      const double a0 = ep[p0];
+      double w0 = 0;
+#pragma unroll
+      for (int m = 0; m < D; m++) {
+        const double am = ep[pm[m]];
+        atomicAdd(Q + pm[m], 0.5 * am / (a0 + am) * fb);
+        w0 = w0 + (D == 2 ? 0.5 * a0 : (a0 - 0.5 * am) / 3.0) / (a0 + am);
+      }
+      atomicAdd(Q + p0, w0 * fb);
+      continue;
+    }
     if (D == 2) {
       atomicAdd(Q + cidx<D>(pnc, ic1, jc1, 0), 0.5 * fb);
       atomicAdd(Q + cidx<D>(pnc, ic2, jc1, 0), 0.25 * fb);
@@ -3741,7 +3824,8 @@ static void adj_vcycle(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int 
 template <int D>
 static void fmg_adjoint_t(Pool& P, const AdjMg& m) {
   const Ctx c = ctx64(P);
-  const Dev a = P.dev();
+  // with the permittivity tags and eps when the operators use them
+  const Dev a = dev_for(P, c);
   const int min_lvl = P.tree.min_lvl, max_lvl = P.tree.max_lvl;
   for (int lvl = max_lvl; lvl >= min_lvl; lvl--) {
     adj_vcycle<D>(P, c, a, m, lvl);
@@ -3759,9 +3843,6 @@ void fmg_adjoint(Pool& P, const int* adj, double* vbar_host) {
   if (P.mg.helm != 0) throw std::invalid_argument("fmg_adjoint: the Laplacian only");
   if (P.n_inst) throw std::invalid_argument("fmg_adjoint: one run per tree only");
   mg_check(P);
-  for (int id = 1; id <= P.tree.max_id; id++)
-    if (P.tree.in_use[id] && P.mg.auto_ops && P.mg.i_eps > 0 && P.tree.tag[id] > kTagNormal)
-      throw std::invalid_argument("fmg_adjoint: boxes with a permittivity are not supported");
   const int nv = P.tree.n_cc;
   for (int k = 0; k < 4; k++)
     if (adj[k] < 1 || adj[k] > nv) throw std::out_of_range("fmg_adjoint: bad adjoint variable");

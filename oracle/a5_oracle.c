@@ -3302,12 +3302,28 @@ static void adj_gc_level(otree *t, int lvl, const o_adjmg *m) {
   for (int q = ids->n - 1; q >= 0; q--) gc_box_mg_adj(t, ids->v[q], m);
 }
 
-/* one half-sweep (mg$D_box_gsrb_lpl, mg2_box_gsrb_clpl) transposed */
+/* the neighbours of interior cell (i, j, k), in the order i-1, i+1, j-1,
+ * j+1, k-1, k+1 of the reference's u(:) and a(:) */
+static int nb_index(int D, int nc, int i, int j, int k, size_t *nbi) {
+  if (D == 2) {
+    nbi[0] = I2(nc, i - 1, j); nbi[1] = I2(nc, i + 1, j);
+    nbi[2] = I2(nc, i, j - 1); nbi[3] = I2(nc, i, j + 1);
+  } else {
+    nbi[0] = I3(nc, i - 1, j, k); nbi[1] = I3(nc, i + 1, j, k);
+    nbi[2] = I3(nc, i, j - 1, k); nbi[3] = I3(nc, i, j + 1, k);
+    nbi[4] = I3(nc, i, j, k - 1); nbi[5] = I3(nc, i, j, k + 1);
+  }
+  return 2 * D;
+}
+
+/* one half-sweep (mg$D_box_gsrb_lpl, mg2_box_gsrb_clpl; for a box tagged
+ * with a permittivity gsrb_lpld above) transposed */
 static void gsrb_box_adj(otree *t, int id, int cntr, const o_adjmg *m) {
   int nc = t->ncell[id], D = t->dim;
   int cyl = D == 2 && t->coord == 1 && t->auto_ops;
   double dx2 = t->dr[id] * t->dr[id];
   double *L = CC(t, id, m->lp), *R = CC(t, id, m->lr);
+  const double *e = box_tag(t, id) == TAG_NORMAL ? NULL : CC(t, id, t->i_eps);
   double w = D == 2 ? 0.25 : 1 / 6.0;
   int kmax = D == 3 ? nc : 1;
   int ioff = cyl ? (IXP(t, id)[0] - 1) * nc : 0;
@@ -3319,6 +3335,23 @@ static void gsrb_box_adj(otree *t, int id, int cntr, const o_adjmg *m) {
         double y = L[c];
         L[c] = 0;
         if (y == 0) continue;
+        if (e) {
+          /* p = (sum c_m rf_m u_m - dx2 rhs) / sum c_m rf_m */
+          size_t nbi[6];
+          int nn = nb_index(D, nc, i, j, k, nbi);
+          double a0 = e[c], cm[6], s2 = 0;
+          for (int q = 0; q < nn; q++) {
+            cm[q] = 2 * a0 * e[nbi[q]] / (a0 + e[nbi[q]]);
+            if (D == 2 && t->coord == 1 && q < 2) {
+              double den = i + (IXP(t, id)[0] - 1) * nc - 0.5;
+              cm[q] = cm[q] * ((double)(i + (IXP(t, id)[0] - 1) * nc - 1 + q) / den);
+            }
+            s2 = s2 + cm[q];
+          }
+          for (int q = 0; q < nn; q++) L[nbi[q]] += cm[q] / s2 * y;
+          R[c] += -dx2 / s2 * y;
+          continue;
+        }
         double wxm = w, wxp = w;
         if (cyl) {
           double den = i + ioff - 0.5;
@@ -3351,12 +3384,14 @@ static void adj_gsrb_boxes(otree *t, int lvl, int n_cycle, const o_adjmg *m) {
   }
 }
 
-/* dst += s A^T src on box id (A the box Laplacian); src cleared if asked */
+/* dst += s A^T src on box id (A the box Laplacian, or box_lpld for a box
+ * tagged with a permittivity); src cleared if asked */
 static void op_box_adj(otree *t, int id, int isrc, double s, int idst, int clear) {
   int nc = t->ncell[id], D = t->dim;
   int cyl = D == 2 && t->coord == 1 && t->auto_ops;
   double inv = 1 / (t->dr[id] * t->dr[id]);
   double *S = CC(t, id, isrc), *Dp = CC(t, id, idst);
+  const double *e = box_tag(t, id) == TAG_NORMAL ? NULL : CC(t, id, t->i_eps);
   int kmax = D == 3 ? nc : 1;
   int ioff = cyl ? (IXP(t, id)[0] - 1) * nc : 0;
   for (int k = 1; k <= kmax; k++)
@@ -3366,6 +3401,23 @@ static void op_box_adj(otree *t, int id, int isrc, double s, int idst, int clear
         double sb = s * S[c];
         if (clear) S[c] = 0;
         if (sb == 0) continue;
+        if (e) {
+          /* o = 2 inv sum_m rf_m a0 a_m / (a0 + a_m) (u_m - u0) */
+          size_t nbi[6];
+          int nn = nb_index(D, nc, i, j, k, nbi);
+          double a0 = e[c], sum = 0;
+          for (int q = 0; q < nn; q++) {
+            double cm = a0 * e[nbi[q]] / (a0 + e[nbi[q]]);
+            if (D == 2 && t->coord == 1 && q < 2) {
+              double den = i + (IXP(t, id)[0] - 1) * nc - 0.5;
+              cm = ((double)(i + (IXP(t, id)[0] - 1) * nc - 1 + q) / den) * cm;
+            }
+            Dp[nbi[q]] += 2 * inv * cm * sb;
+            sum = sum + cm;
+          }
+          Dp[c] += -2 * inv * sum * sb;
+          continue;
+        }
         double wxm = inv, wxp = inv;
         if (cyl) {
           double den = i + ioff - 0.5;
@@ -3422,6 +3474,9 @@ static void corr_box_adj(otree *t, int id, const o_adjmg *m) {
   child_offset(t, id, 0, off);
   const double *F = CC(t, id, m->lp);
   double *Q = CC(t, p_id, m->lt);
+  /* mg$D_box_corr_lpld for a child tagged with a permittivity: the parent's
+   * eps weights the prolongation (o_box_corr above) */
+  const double *e = box_tag(t, id) == TAG_NORMAL ? NULL : CC(t, p_id, t->i_eps);
   int kmax = D == 3 ? nc : 1;
   for (int k = 1; k <= kmax; k++)
     for (int j = 1; j <= nc; j++)
@@ -3430,6 +3485,27 @@ static void corr_box_adj(otree *t, int id, const o_adjmg *m) {
         if (fb == 0) continue;
         int ic1 = off[0] + ((i + 1) >> 1), ic2 = ic1 + 1 - 2 * (i & 1);
         int jc1 = off[1] + ((j + 1) >> 1), jc2 = jc1 + 1 - 2 * (j & 1);
+        if (e) {
+          int kc1 = D == 3 ? off[2] + ((k + 1) >> 1) : 0, kc2 = D == 3 ? kc1 + 1 - 2 * (k & 1) : 0;
+          size_t p0 = D == 2 ? I2(pnc, ic1, jc1) : I3(pnc, ic1, jc1, kc1);
+          size_t pm[3];
+          if (D == 2) {
+            pm[0] = I2(pnc, ic2, jc1); pm[1] = I2(pnc, ic1, jc2);
+          } else {
+            pm[0] = I3(pnc, ic2, jc1, kc1); pm[1] = I3(pnc, ic1, jc2, kc1);
+            pm[2] = I3(pnc, ic1, jc1, kc2);
+          }
+          double a0 = e[p0], w0 = 0;
+          for (int q = 0; q < D; q++) {
+            double am = e[pm[q]];
+            /* 2D: 1/2 (a0 u0 + a_m u_m) / (a0 + a_m); 3D: 1/3 (a0 u0 + a_m
+             * (3/2 u_m - 1/2 u0)) / (a0 + a_m), summed over m */
+            Q[pm[q]] += 0.5 * am / (a0 + am) * fb;
+            w0 += (D == 2 ? 0.5 * a0 : (a0 - 0.5 * am) / 3.0) / (a0 + am);
+          }
+          Q[p0] += w0 * fb;
+          continue;
+        }
         if (D == 2) {
           Q[I2(pnc, ic1, jc1)] += 0.5 * fb;
           Q[I2(pnc, ic2, jc1)] += 0.25 * fb;
@@ -3532,13 +3608,10 @@ int o_fas_fmg_adj(otree *t, const int *adj, double *vbar) {
   /* mg$D_fas_fmg(tree, mg, .false., .false.) (m_aX_mg.f90:164-203)
    * transposed, in the reverse of the reference's schedule. adj = lp, lt,
    * lr, lw; vbar[nb-1] receives the adjoint of the value V of face nb.
-   * Returns 1 (not done) for permittivity tags or a Helmholtz term. */
+   * The operators are those of the forward solve per box (the Laplacian, or
+   * the permittivity forms for tagged boxes). Returns 1 (not done) for a
+   * Helmholtz term. */
   if (t->helm != 0) return 1;
-  for (int lvl = t->min_lvl; lvl <= t->max_lvl; lvl++) {
-    ivec *ids = &t->ids[LV(t, lvl)];
-    for (int q = 0; q < ids->n; q++)
-      if (box_tag(t, ids->v[q]) != TAG_NORMAL) return 1;
-  }
   o_adjmg m = {adj[0], adj[1], adj[2], adj[3], vbar};
   for (int nb = 0; nb < t->nnb; nb++) vbar[nb] = 0;
   int min_lvl = t->min_lvl, max_lvl = t->max_lvl;

@@ -7,7 +7,11 @@ library's FMG, bit for bit the oracle's and the reference's
 (tests/test_gpu_mg.py); x and w are seeded random values on every cell,
 ghosts included, and random V. A transpose that drops, doubles or misplaces
 any term of any operation breaks the identity far above the bound
-(1e-12 of the sum of the absolute products)."""
+(1e-12 of the sum of the absolute products). With a permittivity variable
+(eps = 1, a constant 2 or random values in [1, 3], box by box) the forward
+solve takes the permittivity forms of the operator, the smoother and the
+correction on the tagged boxes (mg$D_box_lpld, gsrb_lpld, corr_lpld), and
+so does the transpose."""
 import numpy as np
 import pytest
 
@@ -44,16 +48,38 @@ def _masks(T, dim):
     return m
 
 
+I_EPS = 4
+
+
+def set_eps(T, dim, rng):
+    """eps box by box in turn: 1 (no tag), a constant 2, random in [1, 3]."""
+    b = T.boxes()
+    n, bl = T.info()["max_id"], T.box_len(T.n_cell)
+    e = np.ones((n, bl))
+    for i in range(n):
+        if b["in_use"][i]:
+            m = (int(b["n_cell"][i]) + 2) ** dim
+            if i % 3 == 1:
+                e[i, :m] = 2.0
+            elif i % 3 == 2:
+                e[i, :m] = 1.0 + 2.0 * rng.random(m)
+    T.set_cc_all(I_EPS, e)
+
+
+@pytest.mark.parametrize("eps", [False, True])
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("dim,nc,coord,base,kinds", CASES)
-def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed):
+def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed, eps):
     import a5_streamer_amd as A
-    print("fmg adjoint", dim, nc, coord, base, kinds, seed, flush=True)
+    print("fmg adjoint", dim, nc, coord, base, kinds, seed, eps, flush=True)
     T = A.Tree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
     S.build_tree(T, dim, nc, _rules, n_base=base)
     rng = np.random.default_rng(seed)
     V = rng.standard_normal(2 * dim)
-    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    if eps:
+        set_eps(T, dim, rng)
+    T.mg_setup(1, 2, 3, i_eps=I_EPS if eps else -1, n_base=4,
+               bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
     n, bl = T.info()["max_id"], T.box_len(nc)
     mask = _masks(T, dim)
     x = [rng.standard_normal((n, bl)) * mask for _ in range(3)]

@@ -169,14 +169,16 @@ def test_window_gradient_matches_oracle_reverse_mode(which):
 
 
 @pytest.mark.parametrize("name", sorted(WINDOWS))
-def test_window_3d_and_cyl_matches_oracle(name):
-    """The 3D and the cylindrical driver's step in a window (tests/
-    window_cases.py; a checkpoint every 2 steps): the forward pass equals
-    the oracle's window bit for bit, and the gradient of both loss terms
-    with respect to the table and the start densities equals the oracle's
-    FP64 reverse mode (pinned by central differences on the CPU,
-    tests/test_oracle_step_adjoint.py) to 1e-12 of the largest entry, and
-    V's to 1e-10 of itself (a sum that cancels; see the 2D test)."""
+def test_window_other_drivers_match_oracle(name):
+    """The 3D driver's step, the cylindrical driver's and the 2D driver's
+    with a dielectric (the field solve's permittivity forms on tagged boxes)
+    in a window (tests/window_cases.py; a checkpoint every 2 steps): the
+    forward pass equals the oracle's window bit for bit, and the gradient of
+    both loss terms with respect to the table and the start densities
+    equals the oracle's FP64 reverse mode (pinned by central differences on
+    the CPU, tests/test_oracle_step_adjoint.py) to 1e-12 of the largest
+    entry, and V's to 1e-10 of itself (a sum that cancels; see the 2D
+    test)."""
     import torch
     import a5_streamer_amd as A
     from a5_streamer_amd import adjoint as AD

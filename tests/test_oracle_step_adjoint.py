@@ -152,14 +152,27 @@ FMG_CASES = [
 ]
 
 
+@pytest.mark.parametrize("eps", [False, True])
 @pytest.mark.parametrize("dim,nc,coord,base,kinds", FMG_CASES)
-def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds):
+def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds, eps):
+    """With eps (box by box in turn: 1, a constant 2, random in [1, 3]) the
+    forward solve and its transpose take the permittivity forms on the
+    tagged boxes."""
     T = O.OTree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
     S.build_tree(T, dim, nc, _rules, n_base=base)
     rng = np.random.default_rng(2)
     V = rng.standard_normal(2 * dim)
-    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
     T.__class__ = O.OStreamTree       # the whole-variable helpers
+    if eps:
+        b = T.boxes()
+        e = np.ones((T.info()["max_id"], T.box_len(T.n_cell)))
+        for i in range(len(b["lvl"])):
+            if b["in_use"][i]:
+                m = (int(b["n_cell"][i]) + 2) ** dim
+                e[i, :m] = 1.0 if i % 3 == 0 else (2.0 if i % 3 == 1 else 1.0 + 2.0 * rng.random(m))
+        T.set_cc_all(4, e)
+    T.mg_setup(1, 2, 3, i_eps=4 if eps else -1, n_base=4, bc_kind=np.array(kinds, np.int32),
+               bc_val=V, auto_ops=True)
     x = [_rand(T, rng, iv) for iv in (1, 2, 3)]
     T.fas_fmg(False, False)
     y = [T.get_cc_all(iv) for iv in (1, 2, 3)]
@@ -262,15 +275,17 @@ from window_cases import WINDOWS  # noqa: E402
 
 
 @pytest.mark.parametrize("name", sorted(WINDOWS))
-def test_oracle_window_gradient_3d_and_cyl(name):
-    """The same check on the 3D and the cylindrical driver's step: a
-    checkpoint every 2 steps, two seeded relative directions per loss term.
-    Measured over h = 1e-4 .. 1e-8, the difference's error at its best h
-    (1e-6 .. 1e-7; truncation above, rounding below) is 1e-9 (3D) and
-    2e-10 .. 9e-9 (cylindrical) for the electron count, 4e-10 .. 2e-7 and
-    6e-8 .. 2e-7 for the head position; the smaller of the errors at
-    h = 1e-6 and 1e-7 is checked against 1e-6. The cylindrical count is the
-    ring volume 2 pi r dr dz's (LeafGeometry)."""
+def test_oracle_window_gradient_other_drivers(name):
+    """The same check on the 3D driver's step, the cylindrical driver's and
+    the 2D driver's with a dielectric (tests/window_cases.py): a checkpoint
+    every 2 steps, two seeded relative directions per loss term. Measured
+    over h = 1e-4 .. 1e-8, the difference's error at its best h (1e-6 ..
+    1e-7; kinks and truncation above, rounding below) is, for the electron
+    count, 1e-9 (3D), 2e-10 .. 9e-9 (cylindrical) and 5e-9 .. 7e-8
+    (dielectric), and for the head position 4e-10 .. 2e-7, 6e-8 .. 2e-7 and
+    2e-8 .. 1.4e-7; the smaller of the errors at h = 1e-6 and 1e-7 is
+    checked against 1e-6. The cylindrical count is the ring volume
+    2 pi r dr dz's (LeafGeometry)."""
     import torch
     from a5_streamer_amd import adjoint as AD
     from a5_streamer_amd import streamer as St
