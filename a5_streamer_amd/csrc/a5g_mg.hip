@@ -3329,6 +3329,63 @@ void autotune(Pool& P, int reps, std::vector<double>& rec) {
 // own.
 // ---------------------------------------------------------------------------
 
+// One face ghost's rule (a$D_gc_box with mg$D_sides_rb and the physical
+// rules) transposed: the adjoint gb of ghost g of face nb of box id (nb_id its
+// neighbour entry) is added to the values the ghost was made from.
+template <int D>
+__device__ __forceinline__ void adj_ghost(const Dev& a, int il, int id, int nc, int nb,
+                                          int nb_id, const int g[3], double gb,
+                                          double* __restrict__ vbar) {
+  double* L = boxp(a, il, id);
+  const int d = d_nb_dim(nb);
+  const bool low = d_nb_low(nb);
+  const int cix = low ? 1 : nc, dix = low ? 1 : -1;
+  const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+  int c[3] = {g[0], g[1], g[2]};
+  c[d] = cix;
+  if (nb_id > 0) {
+    // sides_from_nb (m_aX_gc.f90:368-402): the neighbour's adjacent layer
+    int s[3] = {c[0], c[1], c[2]};
+    s[d] = low ? nc : 1;
+    atomicAdd(boxp(a, il, nb_id) + cidx<D>(nc, s[0], s[1], s[2]), gb);
+  } else if (nb_id == 0) {
+    // mg$D_sides_rb (m_aX_mg.f90:262-356): 0.5 times the parent's cell
+    // (a$D_gc_prolong0) plus an extrapolation from the box's own cells
+    const int pid = a.parent[id];
+    const int pnc = a.ncell[pid];
+    int off[3];
+    child_off<D>(a, id, nc, 0, off);
+    int pc[3] = {0, 0, 0};
+    for (int e = 0; e < D; e++) pc[e] = off[e] + ((g[e] + 1) >> 1);
+    atomicAdd(boxp(a, il, pid) + cidx<D>(pnc, pc[0], pc[1], pc[2]), 0.5 * gb);
+    atomicAdd(L + cidx<D>(nc, c[0], c[1], c[2]), (D == 2 ? 1.0 : 1.25) * gb);
+    int e1[3] = {c[0], c[1], c[2]};
+    e1[d] += dix;
+    atomicAdd(L + cidx<D>(nc, e1[0], e1[1], e1[2]), -0.25 * gb);
+    int e2[3] = {c[0], c[1], c[2]};
+    e2[ta] += -1 + 2 * (c[ta] & 1);
+    atomicAdd(L + cidx<D>(nc, e2[0], e2[1], e2[2]), -0.25 * gb);
+    if (D == 3) {
+      int e3[3] = {c[0], c[1], c[2]};
+      e3[tb] += -1 + 2 * (c[tb] & 1);
+      atomicAdd(L + cidx<D>(nc, e3[0], e3[1], e3[2]), -0.25 * gb);
+    }
+  } else {
+    // the drivers' rules: copy, -c, 2V - c (also the face ghosts of the
+    // whole-row rule)
+    const int kind = a.bc_kind[nb - 1];
+    double* cp = L + cidx<D>(nc, c[0], c[1], c[2]);
+    if (kind == kBcCopy) {
+      atomicAdd(cp, gb);
+    } else if (kind == kBcNeg) {
+      atomicAdd(cp, -gb);
+    } else {
+      atomicAdd(cp, -gb);
+      atomicAdd(vbar + nb - 1, 2 * gb);
+    }
+  }
+}
+
 // a$D_gc_box's rules on one box (m_aX_gc.f90:50-68, with mg$D_sides_rb and
 // the physical rules), transposed: every face ghost's adjoint is sent to the
 // values it was made from and cleared. The forward pass makes face ghosts
@@ -3366,59 +3423,17 @@ __global__ __launch_bounds__(256) void k_adj_gc(Dev a, const int* __restrict__ l
     const int nb = q / per_face + 1;
     const int t = q - (nb - 1) * per_face;
     const int d = d_nb_dim(nb);
-    const bool low = d_nb_low(nb);
-    const int gix = low ? 0 : nc + 1, cix = low ? 1 : nc, dix = low ? 1 : -1;
     const int nb_id = a.nbs[id * 6 + nb - 1];
     const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    int g[3] = {0, 0, 0}, c[3] = {0, 0, 0};
-    g[d] = gix;
-    c[d] = cix;
-    g[ta] = c[ta] = t % nc + 1;
-    if (D == 3) g[tb] = c[tb] = t / nc + 1;
+    int g[3] = {0, 0, 0};
+    g[d] = d_nb_low(nb) ? 0 : nc + 1;
+    g[ta] = t % nc + 1;
+    if (D == 3) g[tb] = t / nc + 1;
     double* gp = L + cidx<D>(nc, g[0], g[1], g[2]);
     const double gb = *gp;
     *gp = 0;
     if (gb == 0) continue;
-    if (nb_id > 0) {
-      // sides_from_nb (m_aX_gc.f90:368-402): the neighbour's adjacent layer
-      int s[3] = {c[0], c[1], c[2]};
-      s[d] = low ? nc : 1;
-      atomicAdd(boxp(a, il, nb_id) + cidx<D>(nc, s[0], s[1], s[2]), gb);
-    } else if (nb_id == 0) {
-      // mg$D_sides_rb (m_aX_mg.f90:262-356): 0.5 times the parent's cell
-      // (a$D_gc_prolong0) plus an extrapolation from the box's own cells
-      const int pid = a.parent[id];
-      const int pnc = a.ncell[pid];
-      int off[3];
-      child_off<D>(a, id, nc, 0, off);
-      int pc[3] = {0, 0, 0};
-      for (int e = 0; e < D; e++) pc[e] = off[e] + ((g[e] + 1) >> 1);
-      atomicAdd(boxp(a, il, pid) + cidx<D>(pnc, pc[0], pc[1], pc[2]), 0.5 * gb);
-      atomicAdd(L + cidx<D>(nc, c[0], c[1], c[2]), (D == 2 ? 1.0 : 1.25) * gb);
-      int e1[3] = {c[0], c[1], c[2]};
-      e1[d] += dix;
-      atomicAdd(L + cidx<D>(nc, e1[0], e1[1], e1[2]), -0.25 * gb);
-      int e2[3] = {c[0], c[1], c[2]};
-      e2[ta] += -1 + 2 * (c[ta] & 1);
-      atomicAdd(L + cidx<D>(nc, e2[0], e2[1], e2[2]), -0.25 * gb);
-      if (D == 3) {
-        int e3[3] = {c[0], c[1], c[2]};
-        e3[tb] += -1 + 2 * (c[tb] & 1);
-        atomicAdd(L + cidx<D>(nc, e3[0], e3[1], e3[2]), -0.25 * gb);
-      }
-    } else {
-      // the drivers' rules: copy, -c, 2V - c
-      const int kind = a.bc_kind[nb - 1];
-      double* cp = L + cidx<D>(nc, c[0], c[1], c[2]);
-      if (kind == kBcCopy) {
-        atomicAdd(cp, gb);
-      } else if (kind == kBcNeg) {
-        atomicAdd(cp, -gb);
-      } else {
-        atomicAdd(cp, -gb);
-        atomicAdd(vbar + nb - 1, 2 * gb);
-      }
-    }
+    adj_ghost<D>(a, il, id, nc, nb, nb_id, g, gb, vbar);
   }
 }
 
@@ -3437,9 +3452,17 @@ __device__ __forceinline__ void nb_cells(int nc, int i, int j, int k, int nbx[6]
   nbx[5] = D == 3 ? cidx<D>(nc, i, j, k + 1) : 0;
 }
 
-template <int D, bool CYL>
+// GC: the ghost pass that precedes this half-sweep in the forward schedule
+// is transposed here too: a contribution to a face ghost goes straight
+// through that ghost's rule (adj_ghost) instead of into the ghost slot for a
+// k_adj_gc launch. The rule sends it to the cell itself (written only by
+// this lane, after its read), to cells of the other colour and to the
+// parent, none of which this launch reads; the transpose is linear, so the
+// sum is that of the two launches.
+template <int D, bool CYL, bool GC = false>
 __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__ list, int n,
-                                                  int nc, int cntr, int tpb, int il, int ir) {
+                                                  int nc, int cntr, int tpb, int il, int ir,
+                                                  double* __restrict__ vbar) {
   BOX_PROLOGUE
   double* L = boxp(a, il, id);
   double* R = boxp(a, ir, id);
@@ -3459,6 +3482,19 @@ __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__
     const double yb = *yp;
     *yp = 0;
     if (yb == 0) continue;
+    // add v to the adjoint of (x, y, z), a neighbour of (i, j, k)
+    auto put = [&](int x, int y, int z, double v) {
+      if (GC) {
+        const int nb = x < 1 ? 1 : x > nc ? 2 : y < 1 ? 3 : y > nc ? 4 : (D == 3 && z < 1) ? 5
+                     : (D == 3 && z > nc) ? 6 : 0;
+        if (nb > 0) {
+          const int g[3] = {x, y, z};
+          adj_ghost<D>(a, il, id, nc, nb, a.nbs[id * 6 + nb - 1], g, v, vbar);
+          return;
+        }
+      }
+      atomicAdd(L + cidx<D>(nc, x, y, z), v);
+    };
     if (ev != nullptr) {
       // mg$D_box_gsrb_lpld: (sum c_m rf_m u_m - h^2 rhs) / sum c_m rf_m with
       // c_m = 2 eps_0 eps_m / (eps_0 + eps_m)
@@ -3476,8 +3512,10 @@ __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__
         }
         s2 = s2 + c[m];
       }
+      const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0};
+      const int dk[6] = {0, 0, 0, 0, -1, 1};
 #pragma unroll
-      for (int m = 0; m < 2 * D; m++) atomicAdd(L + nbx[m], c[m] / s2 * yb);
+      for (int m = 0; m < 2 * D; m++) put(i + di[m], j + dj[m], k + dk[m], c[m] / s2 * yb);
       R[c0] += -dx2 / s2 * yb;
       continue;
     }
@@ -3487,13 +3525,13 @@ __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__
       wxm = 0.25 * ((double)(i + ioff - 1) / den);
       wxp = 0.25 * ((double)(i + ioff) / den);
     }
-    atomicAdd(L + cidx<D>(nc, i - 1, j, k), wxm * yb);
-    atomicAdd(L + cidx<D>(nc, i + 1, j, k), wxp * yb);
-    atomicAdd(L + cidx<D>(nc, i, j - 1, k), w * yb);
-    atomicAdd(L + cidx<D>(nc, i, j + 1, k), w * yb);
+    put(i - 1, j, k, wxm * yb);
+    put(i + 1, j, k, wxp * yb);
+    put(i, j - 1, k, w * yb);
+    put(i, j + 1, k, w * yb);
     if (D == 3) {
-      atomicAdd(L + cidx<D>(nc, i, j, k - 1), w * yb);
-      atomicAdd(L + cidx<D>(nc, i, j, k + 1), w * yb);
+      put(i, j, k - 1, w * yb);
+      put(i, j, k + 1, w * yb);
     }
     R[cidx<D>(nc, i, j, k)] += -w * dx2 * yb;
   }
@@ -3709,14 +3747,27 @@ static void adj_gsrb_boxes(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, 
   if (!n) return;
   const int nc = P.nc_lvl(lvl);
   const Geo g = geo(n, cells_of(D, nc, false) / 2);
+  // forward: (half-sweep it, ghost pass) for it = 1 .. 2 n_cycle; the last
+  // ghost pass is transposed on its own, each earlier one inside the
+  // transpose of the half-sweep that follows it
+  adj_gc_level<D>(P, a, m, lvl);
   for (int it = 2 * n_cycle; it >= 1; it--) {
-    adj_gc_level<D>(P, a, m, lvl);
-    if (c.cyl)
-      hipLaunchKernelGGL((k_adj_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                         P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr);
-    else
-      hipLaunchKernelGGL((k_adj_gsrb<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
-                         P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr);
+    const bool gc = it > 1;
+    if (c.cyl) {
+      if (gc)
+        hipLaunchKernelGGL((k_adj_gsrb<2, true, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                           P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr, m.vbar);
+      else
+        hipLaunchKernelGGL((k_adj_gsrb<2, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                           P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr, m.vbar);
+    } else {
+      if (gc)
+        hipLaunchKernelGGL((k_adj_gsrb<D, false, true>), dim3(g.grid), dim3(256), 0, P.stream,
+                           a, P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr, m.vbar);
+      else
+        hipLaunchKernelGGL((k_adj_gsrb<D, false>), dim3(g.grid), dim3(256), 0, P.stream, a,
+                           P.ids(lvl), n, nc, it, g.tpb, m.lp, m.lr, m.vbar);
+    }
     post_launch("adj_gsrb");
   }
 }
