@@ -3324,9 +3324,9 @@ void autotune(Pool& P, int reps, std::vector<double>& rec) {
 // the adjoints of V go to vbar (one per face). The operators are those the
 // forward solve used per box: the Laplacian, or for boxes tagged with a
 // permittivity mg$D_box_lpld, gsrb_lpld and corr_lpld (their weights depend
-// on eps only, which the window holds fixed); no Helmholtz term. The
-// transposes scatter with atomics, so the sums arrive in an order of their
-// own.
+// on eps only, which the window holds fixed), and the Helmholtz term
+// lpl(u) - helm u of the photoionization solves. The transposes scatter with
+// atomics, so the sums arrive in an order of their own.
 // ---------------------------------------------------------------------------
 
 // One face ghost's rule (a$D_gc_box with mg$D_sides_rb and the physical
@@ -3519,21 +3519,24 @@ __global__ __launch_bounds__(256) void k_adj_gsrb(Dev a, const int* __restrict__
       R[c0] += -dx2 / s2 * yb;
       continue;
     }
-    double wxm = w, wxp = w;
+    // (sum of the neighbours - h^2 rhs) / 2D, or with a Helmholtz term
+    // divided by 2D + helm h^2
+    const double wc = a.helm != 0 ? 1 / (2 * D + a.helm * dx2) : w;
+    double wxm = wc, wxp = wc;
     if (CYL) {
       const double den = i + ioff - 0.5;
-      wxm = 0.25 * ((double)(i + ioff - 1) / den);
-      wxp = 0.25 * ((double)(i + ioff) / den);
+      wxm = wc * ((double)(i + ioff - 1) / den);
+      wxp = wc * ((double)(i + ioff) / den);
     }
     put(i - 1, j, k, wxm * yb);
     put(i + 1, j, k, wxp * yb);
-    put(i, j - 1, k, w * yb);
-    put(i, j + 1, k, w * yb);
+    put(i, j - 1, k, wc * yb);
+    put(i, j + 1, k, wc * yb);
     if (D == 3) {
-      put(i, j, k - 1, w * yb);
-      put(i, j, k + 1, w * yb);
+      put(i, j, k - 1, wc * yb);
+      put(i, j, k + 1, wc * yb);
     }
-    R[cidx<D>(nc, i, j, k)] += -w * dx2 * yb;
+    R[cidx<D>(nc, i, j, k)] += -wc * dx2 * yb;
   }
 }
 
@@ -3592,6 +3595,7 @@ __global__ __launch_bounds__(256) void k_adj_op(Dev a, const int* __restrict__ l
       atomicAdd(Dp + cidx<D>(nc, i, j, k + 1), inv * sb);
     }
     atomicAdd(Dp + c0, -(D == 2 ? 4.0 : 6.0) * inv * sb);
+    if (a.helm != 0) atomicAdd(Dp + c0, -a.helm * sb);  // lpl(u) - helm u
   }
 }
 
@@ -3891,7 +3895,6 @@ static void fmg_adjoint_t(Pool& P, const AdjMg& m) {
 
 void fmg_adjoint(Pool& P, const int* adj, double* vbar_host) {
   if (!P.mg.ready) throw std::logic_error("fmg_adjoint: you haven't called mg_init_mg");
-  if (P.mg.helm != 0) throw std::invalid_argument("fmg_adjoint: the Laplacian only");
   if (P.n_inst) throw std::invalid_argument("fmg_adjoint: one run per tree only");
   mg_check(P);
   const int nv = P.tree.n_cc;

@@ -107,9 +107,10 @@ int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
  * the values it started from; lw is work space. vbar[2 dim] receives the
  * adjoints of V per face. The transpose of each operation of the
  * reference's schedule, in reverse order; no state of the forward solve is
- * needed. The operators of the forward solve per box (the Laplacian, or
- * the permittivity forms mg$D_box_lpld / gsrb_lpld / corr_lpld on boxes
- * tagged with eps); no Helmholtz term; one run per tree.
+ * needed. The operators of the forward solve per box (the Laplacian, with
+ * the Helmholtz term of a5g_mg_set_helmholtz, or the permittivity forms
+ * mg$D_box_lpld / gsrb_lpld / corr_lpld on boxes tagged with eps); one run
+ * per tree.
  * No counterpart in the reference, which computes no derivatives. */
 int a5g_fas_fmg_adjoint(a5g_tree *t, const int *adj, double *vbar);
 /* Launch shapes of the box kernels (no reference counterpart: the

@@ -3324,7 +3324,8 @@ static void gsrb_box_adj(otree *t, int id, int cntr, const o_adjmg *m) {
   double dx2 = t->dr[id] * t->dr[id];
   double *L = CC(t, id, m->lp), *R = CC(t, id, m->lr);
   const double *e = box_tag(t, id) == TAG_NORMAL ? NULL : CC(t, id, t->i_eps);
-  double w = D == 2 ? 0.25 : 1 / 6.0;
+  /* 1 / 2D, or with a Helmholtz term 1 / (2D + helm dx2) */
+  double w = t->helm != 0 ? 1 / (2 * D + t->helm * dx2) : (D == 2 ? 0.25 : 1 / 6.0);
   int kmax = D == 3 ? nc : 1;
   int ioff = cyl ? (IXP(t, id)[0] - 1) * nc : 0;
   for (int k = 1; k <= kmax; k++)
@@ -3355,8 +3356,8 @@ static void gsrb_box_adj(otree *t, int id, int cntr, const o_adjmg *m) {
         double wxm = w, wxp = w;
         if (cyl) {
           double den = i + ioff - 0.5;
-          wxm = 0.25 * ((double)(i + ioff - 1) / den);
-          wxp = 0.25 * ((double)(i + ioff) / den);
+          wxm = w * ((double)(i + ioff - 1) / den);
+          wxp = w * ((double)(i + ioff) / den);
         }
         if (D == 2) {
           L[I2(nc, i - 1, j)] += wxm * y;
@@ -3439,6 +3440,7 @@ static void op_box_adj(otree *t, int id, int isrc, double s, int idst, int clear
           Dp[I3(nc, i, j, k + 1)] += inv * sb;
           Dp[c] += -6.0 * inv * sb;
         }
+        if (t->helm != 0) Dp[c] += -t->helm * sb;  /* lpl(u) - helm u */
       }
 }
 
@@ -3608,10 +3610,9 @@ int o_fas_fmg_adj(otree *t, const int *adj, double *vbar) {
   /* mg$D_fas_fmg(tree, mg, .false., .false.) (m_aX_mg.f90:164-203)
    * transposed, in the reverse of the reference's schedule. adj = lp, lt,
    * lr, lw; vbar[nb-1] receives the adjoint of the value V of face nb.
-   * The operators are those of the forward solve per box (the Laplacian, or
-   * the permittivity forms for tagged boxes). Returns 1 (not done) for a
-   * Helmholtz term. */
-  if (t->helm != 0) return 1;
+   * The operators are those of the forward solve per box: the Laplacian
+   * (with the Helmholtz term of o_mg_set_helmholtz), or the permittivity
+   * forms for tagged boxes. Returns 0. */
   o_adjmg m = {adj[0], adj[1], adj[2], adj[3], vbar};
   for (int nb = 0; nb < t->nnb; nb++) vbar[nb] = 0;
   int min_lvl = t->min_lvl, max_lvl = t->max_lvl;

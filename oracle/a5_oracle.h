@@ -213,8 +213,9 @@ void o_gc_tree_adj(otree *t, int iv, int il, int rb, int bc);
 void o_fld_from_pot_adj(otree *t, const o_vars *v, int lE, int lfld, int lphi);
 /* o_fas_fmg(t, 0, 0) transposed: adj = lp, lt, lr, lw (the adjoints of the
  * potential, tmp and rhs it leaves, in; of those it starts from, out; lw
- * work); vbar[nb-1] the adjoint of face nb's value V. Returns 1 without
- * doing anything on boxes with a permittivity or a Helmholtz term. */
+ * work); vbar[nb-1] the adjoint of face nb's value V. The operators are
+ * the forward solve's per box (Laplacian with its Helmholtz term, or the
+ * permittivity forms on tagged boxes). Returns 0. */
 int o_fas_fmg_adj(otree *t, const int *adj, double *vbar);
 /* whole variables of box slots 0..max_id to (load 0) or from (load 1) buf,
  * which holds o_store_size doubles */

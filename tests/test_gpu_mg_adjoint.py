@@ -11,7 +11,8 @@ any term of any operation breaks the identity far above the bound
 (eps = 1, a constant 2 or random values in [1, 3], box by box) the forward
 solve takes the permittivity forms of the operator, the smoother and the
 correction on the tagged boxes (mg$D_box_lpld, gsrb_lpld, corr_lpld), and
-so does the transpose."""
+so does the transpose; with a Helmholtz term (the photoionization solves'
+operator lpl(u) - kappa^2 u) the smoother divides by 2D + kappa^2 h^2."""
 import numpy as np
 import pytest
 
@@ -66,20 +67,23 @@ def set_eps(T, dim, rng):
     T.set_cc_all(I_EPS, e)
 
 
-@pytest.mark.parametrize("eps", [False, True])
+@pytest.mark.parametrize("op", ["lpl", "eps", "helmholtz"])
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("dim,nc,coord,base,kinds", CASES)
-def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed, eps):
+def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed, op):
     import a5_streamer_amd as A
-    print("fmg adjoint", dim, nc, coord, base, kinds, seed, eps, flush=True)
+    print("fmg adjoint", dim, nc, coord, base, kinds, seed, op, flush=True)
     T = A.Tree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
     S.build_tree(T, dim, nc, _rules, n_base=base)
     rng = np.random.default_rng(seed)
     V = rng.standard_normal(2 * dim)
-    if eps:
+    if op == "eps":
         set_eps(T, dim, rng)
-    T.mg_setup(1, 2, 3, i_eps=I_EPS if eps else -1, n_base=4,
+    T.mg_setup(1, 2, 3, i_eps=I_EPS if op == "eps" else -1, n_base=4,
                bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    if op == "helmholtz":
+        # lpl(u) - kappa^2 u with kappa^2 h^2 of order one on the finest level
+        T.mg_set_helmholtz(float(nc * 8) ** 2)
     n, bl = T.info()["max_id"], T.box_len(nc)
     mask = _masks(T, dim)
     x = [rng.standard_normal((n, bl)) * mask for _ in range(3)]
@@ -107,12 +111,11 @@ def test_fmg_adjoint_dot_product(dim, nc, coord, base, kinds, seed, eps):
     T.close()
 
 
-def test_fmg_adjoint_refuses_what_it_does_not_cover():
+def test_fmg_adjoint_refuses_bad_variables():
     import a5_streamer_amd as A
     T = A.Tree(2, 8, 8, 1, 1.0 / 8, coarsen_to=2)
     S.build_tree(T, 2, 8, _rules)
     T.mg_setup(1, 2, 3, bc_kind=np.zeros(4, np.int32), auto_ops=True)
-    T.mg_set_helmholtz(4.0)
     with pytest.raises(Exception):
-        T.fas_fmg_adjoint([LP, LT, LR, LW])
+        T.fas_fmg_adjoint([LP, LT, LR, 99])
     T.close()

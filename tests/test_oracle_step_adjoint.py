@@ -152,12 +152,15 @@ FMG_CASES = [
 ]
 
 
-@pytest.mark.parametrize("eps", [False, True])
+@pytest.mark.parametrize("op", ["lpl", "eps", "helmholtz"])
 @pytest.mark.parametrize("dim,nc,coord,base,kinds", FMG_CASES)
-def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds, eps):
+def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds, op):
     """With eps (box by box in turn: 1, a constant 2, random in [1, 3]) the
     forward solve and its transpose take the permittivity forms on the
-    tagged boxes."""
+    tagged boxes; with a Helmholtz term lpl(u) - kappa^2 u (the
+    photoionization solves' operator) the smoother divides by
+    2D + kappa^2 h^2."""
+    eps = op == "eps"
     T = O.OTree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
     S.build_tree(T, dim, nc, _rules, n_base=base)
     rng = np.random.default_rng(2)
@@ -173,6 +176,8 @@ def test_oracle_fmg_reverse_dot_product(dim, nc, coord, base, kinds, eps):
         T.set_cc_all(4, e)
     T.mg_setup(1, 2, 3, i_eps=4 if eps else -1, n_base=4, bc_kind=np.array(kinds, np.int32),
                bc_val=V, auto_ops=True)
+    if op == "helmholtz":
+        T.mg_set_helmholtz(float(nc * 8) ** 2)
     x = [_rand(T, rng, iv) for iv in (1, 2, 3)]
     T.fas_fmg(False, False)
     y = [T.get_cc_all(iv) for iv in (1, 2, 3)]
