@@ -900,6 +900,16 @@ int a5g_fas_fmg_adjoint(a5g_tree* t, const int* adj, double* vbar) {
   API_END
 }
 
+int a5g_fas_vcycle_adjoint(a5g_tree* t, int max_lvl, const int* adj, double* vbar) {
+  API_BEGIN
+  NEED(t);
+  FLUSH(t);
+  need_gpu(t->P);
+  if (!adj || !vbar) throw std::invalid_argument("null argument");
+  a5g::vcycle_adjoint(t->P, max_lvl, adj, vbar);
+  API_END
+}
+
 int a5g_set_shape(a5g_tree* t, int kernel, int nc, int cls, int shape) {
   API_BEGIN
   NEED(t);

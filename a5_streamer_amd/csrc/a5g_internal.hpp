@@ -18,6 +18,7 @@ void mg_fmg_groups(Pool& P, bool set_residual, bool first_call);
 void mg_vcycle(Pool& P, int max_lvl, bool set_residual);
 void autotune(Pool& P, int reps, std::vector<double>& rec);
 void fmg_adjoint(Pool& P, const int* adj, double* vbar_host);
+void vcycle_adjoint(Pool& P, int max_lvl, const int* adj, double* vbar_host);
 // stores the ghost cells a multigrid call left to the next reader
 void flush_ghosts(Pool& P);
 void mg_residual_norms(Pool& P, double* max_res, double* max_rhs);

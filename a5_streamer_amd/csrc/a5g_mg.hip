@@ -3893,6 +3893,42 @@ static void fmg_adjoint_t(Pool& P, const AdjMg& m) {
   for (int lvl = min_lvl + 1; lvl <= max_lvl; lvl++) adj_update_coarse<D>(P, c, a, m, lvl, true);
 }
 
+// Reverse mode of one FAS V-cycle up to max_lvl without the residual
+// (mg$D_fas_vcycle, m_aX_mg.f90:208-255): adj_vcycle, with the checks,
+// variables and vbar buffer of fmg_adjoint.
+template <int D>
+static void vcycle_adjoint_t(Pool& P, const AdjMg& m, int max_lvl) {
+  const Ctx c = ctx64(P);
+  const Dev a = dev_for(P, c);
+  adj_vcycle<D>(P, c, a, m, max_lvl);
+}
+
+void vcycle_adjoint(Pool& P, int max_lvl, const int* adj, double* vbar_host) {
+  if (!P.mg.ready) throw std::logic_error("vcycle_adjoint: you haven't called mg_init_mg");
+  if (P.n_inst) throw std::invalid_argument("vcycle_adjoint: one run per tree only");
+  mg_check(P);
+  if (max_lvl < P.tree.min_lvl || max_lvl > P.tree.max_lvl)
+    throw std::invalid_argument("vcycle_adjoint: max_lvl outside the tree");
+  const int nv = P.tree.n_cc;
+  for (int k = 0; k < 4; k++)
+    if (adj[k] < 1 || adj[k] > nv) throw std::out_of_range("vcycle_adjoint: bad adjoint variable");
+  if (!P.d_vbar) check(hipMalloc((void**)&P.d_vbar, 8 * sizeof(double)), "hipMalloc vbar");
+  const AdjMg m{adj[0], adj[1], adj[2], adj[3], P.d_vbar};
+  run_captured(P, fnv(call_key(10, adj[0], adj[1], max_lvl - P.tree.min_lvl),
+                      fnv((uint64_t)adj[2], (uint64_t)adj[3])),
+               [&] {
+                 hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, P.stream,
+                                    (unsigned long long*)P.d_vbar, 8, 0ull);
+                 post_launch("fill");
+                 if (P.tree.dim == 2) vcycle_adjoint_t<2>(P, m, max_lvl);
+                 else vcycle_adjoint_t<3>(P, m, max_lvl);
+               });
+  check(hipMemcpyAsync(vbar_host, P.d_vbar, 2 * P.tree.dim * sizeof(double),
+                       hipMemcpyDeviceToHost, P.stream),
+        "vbar");
+  check(hipStreamSynchronize(P.stream), "vcycle_adjoint sync");
+}
+
 void fmg_adjoint(Pool& P, const int* adj, double* vbar_host) {
   if (!P.mg.ready) throw std::logic_error("fmg_adjoint: you haven't called mg_init_mg");
   if (P.n_inst) throw std::invalid_argument("fmg_adjoint: one run per tree only");

@@ -284,6 +284,16 @@ module m_a5g
        integer(c_int)        :: s
      end function c_fas_fmg_adjoint
 
+     function c_fas_vcycle_adjoint(t, max_lvl, adj, vbar) &
+          bind(C, name="a5g_fas_vcycle_adjoint") result(s)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value    :: t
+       integer(c_int), value :: max_lvl
+       integer(c_int)        :: adj(*)
+       real(c_double)        :: vbar(*)
+       integer(c_int)        :: s
+     end function c_fas_vcycle_adjoint
+
      function c_set_shape(t, kernel, nc, cls, shape) bind(C, name="a5g_set_shape") result(s)
        import :: c_ptr, c_int
        type(c_ptr), value    :: t
@@ -817,7 +827,7 @@ module m_a5g
   public :: a2_tree_max_cc, a3_tree_max_cc, a2_tree_min_cc, a3_tree_min_cc
   public :: a2_consistent_fluxes, a3_consistent_fluxes, a2_prolong1_to, a3_prolong1_to
   public :: mg2_init_mg, mg3_init_mg, mg2_fas_fmg, mg3_fas_fmg
-  public :: mg2_fas_vcycle, mg3_fas_vcycle, mg_fas_fmg_adjoint
+  public :: mg2_fas_vcycle, mg3_fas_vcycle, mg_fas_fmg_adjoint, mg_fas_vcycle_adjoint
   public :: a5g_max_id, a5g_max_lvl, a5g_min_dr, a5g_level_leaves, a5g_box_info
   public :: a5g_set_cc, a5g_get_cc, a5g_fill_cc, a5g_sync
   public :: a5g_mixed_solve, a5g_residual_norms, a5g_set_bc_value
@@ -1103,6 +1113,22 @@ contains
     call check(c_fas_fmg_adjoint(tree%h, a, v), "mg$D_fas_fmg adjoint")
     vbar = v(1:size(vbar))
   end subroutine mg_fas_fmg_adjoint
+
+  !> Reverse mode of mg$D_fas_vcycle(tree, mg, .false., max_lvl)
+  !> (m_aX_mg.f90:208-255): adj and vbar as for mg_fas_fmg_adjoint.
+  !> No counterpart in the reference.
+  subroutine mg_fas_vcycle_adjoint(tree, max_lvl, adj, vbar)
+    type(a5g_t), intent(inout) :: tree
+    integer, intent(in)        :: max_lvl
+    integer, intent(in)        :: adj(4)
+    real(dp), intent(out)      :: vbar(:)
+    integer(c_int)             :: a(4)
+    real(c_double)             :: v(6)
+    a = adj
+    v = 0
+    call check(c_fas_vcycle_adjoint(tree%h, int(max_lvl, c_int), a, v), "mg$D_fas_vcycle adjoint")
+    vbar = v(1:size(vbar))
+  end subroutine mg_fas_vcycle_adjoint
 
   !> mg$D_fas_vcycle (m_aX_mg.f90:208-255)
   subroutine t_fas_vcycle(tree, mg, max_lvl, set_residual)

@@ -206,6 +206,15 @@ class Tree:
     def fas_vcycle(self, max_lvl, set_residual):
         _call("a5g_fas_vcycle", self.h, C.c_int(max_lvl), C.c_int(int(set_residual)))
 
+    def fas_vcycle_adjoint(self, max_lvl, adj):
+        """Reverse mode of fas_vcycle(max_lvl, False)
+        (a5g_fas_vcycle_adjoint): adj and the returned boundary-value
+        adjoints as for fas_fmg_adjoint."""
+        a, pa = _i(np.asarray(adj))
+        vbar = np.zeros(2 * self.dim)
+        _call("a5g_fas_vcycle_adjoint", self.h, C.c_int(max_lvl), pa, _d(vbar)[1])
+        return vbar
+
     def fas_fmg_adjoint(self, adj):
         """Reverse mode of fas_fmg(False, False) (a5g_fas_fmg_adjoint):
         adj = (lp, lt, lr, lw) cell variables; returns the adjoints of the

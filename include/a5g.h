@@ -113,6 +113,13 @@ int a5g_fas_vcycle(a5g_tree *t, int max_lvl, int set_residual);
  * per tree.
  * No counterpart in the reference, which computes no derivatives. */
 int a5g_fas_fmg_adjoint(a5g_tree *t, const int *adj, double *vbar);
+/* Reverse mode of a5g_fas_vcycle(t, max_lvl, 0) (mg$D_fas_vcycle,
+ * m_aX_mg.f90:208-255, without the residual), an affine map of the same
+ * values: the same adj[4] and vbar as a5g_fas_fmg_adjoint, the same
+ * operators. With it the reverse of a whole solve (an FMG and V-cycles) is
+ * the V-cycles' reverses, last first, then the FMG's. No counterpart in the
+ * reference, which computes no derivatives. */
+int a5g_fas_vcycle_adjoint(a5g_tree *t, int max_lvl, const int *adj, double *vbar);
 /* Launch shapes of the box kernels (no reference counterpart: the
  * reference's loops over boxes have no launch geometry). kernel 0 is the
  * multigrid half-sweep, 1 the ghost fill, 2 the residual + restriction, 3

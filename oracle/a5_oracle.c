@@ -3606,6 +3606,15 @@ static void adj_vcycle(otree *t, int max_lvl, const o_adjmg *m) {
   }
 }
 
+int o_fas_vcycle_adj(otree *t, int max_lvl, const int *adj, double *vbar) {
+  /* mg$D_fas_vcycle(tree, mg, .false., max_lvl) (m_aX_mg.f90:208-255)
+   * transposed; adj and vbar as for o_fas_fmg_adj. Returns 0. */
+  o_adjmg m = {adj[0], adj[1], adj[2], adj[3], vbar};
+  for (int nb = 0; nb < t->nnb; nb++) vbar[nb] = 0;
+  adj_vcycle(t, max_lvl, &m);
+  return 0;
+}
+
 int o_fas_fmg_adj(otree *t, const int *adj, double *vbar) {
   /* mg$D_fas_fmg(tree, mg, .false., .false.) (m_aX_mg.f90:164-203)
    * transposed, in the reverse of the reference's schedule. adj = lp, lt,

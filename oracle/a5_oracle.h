@@ -217,6 +217,8 @@ void o_fld_from_pot_adj(otree *t, const o_vars *v, int lE, int lfld, int lphi);
  * the forward solve's per box (Laplacian with its Helmholtz term, or the
  * permittivity forms on tagged boxes). Returns 0. */
 int o_fas_fmg_adj(otree *t, const int *adj, double *vbar);
+/* o_fas_vcycle(t, max_lvl, 0) transposed; adj and vbar as above */
+int o_fas_vcycle_adj(otree *t, int max_lvl, const int *adj, double *vbar);
 /* whole variables of box slots 0..max_id to (load 0) or from (load 1) buf,
  * which holds o_store_size doubles */
 void o_store_vars(otree *t, int n_cc, const int *ivs, int n_fc, const int *fvs, double *buf,

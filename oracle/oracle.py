@@ -552,6 +552,12 @@ class OStreamTree(OTree):
         self.L.o_fld_from_pot_adj(self.h, C.byref(self.v), C.c_int(lE), C.c_int(lfld),
                                   C.c_int(lphi))
 
+    def fas_vcycle_adjoint(self, max_lvl, adj):
+        a, pa = _i(np.asarray(adj))
+        vbar = np.zeros(6)
+        self.L.o_fas_vcycle_adj(self.h, C.c_int(max_lvl), pa, _d(vbar)[1])
+        return vbar[:2 * self.dim].copy()
+
     def fas_fmg_adjoint(self, adj):
         a, pa = _i(np.asarray(adj))
         vbar = np.zeros(6)

@@ -119,3 +119,56 @@ def test_fmg_adjoint_refuses_bad_variables():
     with pytest.raises(Exception):
         T.fas_fmg_adjoint([LP, LT, LR, 99])
     T.close()
+
+
+def _xw(T, dim, rng, mask):
+    n, bl = T.info()["max_id"], T.box_len(T.n_cell)
+    return [rng.standard_normal((n, bl)) * mask for _ in range(3)]
+
+
+@pytest.mark.parametrize("op", ["lpl", "helmholtz"])
+@pytest.mark.parametrize("below", [0, 1])
+@pytest.mark.parametrize("dim,nc,coord,base,kinds", CASES)
+def test_vcycle_adjoint_dot_product(dim, nc, coord, base, kinds, below, op):
+    """a5g_fas_vcycle_adjoint against the forward V-cycle up to the finest
+    level or the one below (the same affine map and identity as the FMG's),
+    and the reverse of a whole solve (FMG + 2 V-cycles): the V-cycles'
+    reverses, last first, then the FMG's."""
+    import a5_streamer_amd as A
+    print("vcycle adjoint", dim, nc, coord, base, kinds, below, op, flush=True)
+    T = A.Tree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
+    S.build_tree(T, dim, nc, _rules, n_base=base)
+    rng = np.random.default_rng(5 + below)
+    V = rng.standard_normal(2 * dim)
+    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    if op == "helmholtz":
+        T.mg_set_helmholtz(float(nc * 8) ** 2)
+    n, bl = T.info()["max_id"], T.box_len(nc)
+    mask = _masks(T, dim)
+    top = T.info()["max_lvl"] - below
+    for whole in (False, True):
+        x, w = _xw(T, dim, rng, mask), _xw(T, dim, rng, mask)
+        for iv, v in zip((1, 2, 3), x):
+            T.set_cc_all(iv, v)
+        if whole:
+            T.fas_fmg(False, False)
+            T.fas_vcycle(top, False)
+            T.fas_vcycle(top, False)
+        else:
+            T.fas_vcycle(top, False)
+        y = [T.get_cc_all(iv) * mask for iv in (1, 2, 3)]
+        lhs = sum(float(np.sum(a * b)) for a, b in zip(w, y))
+        scale = sum(float(np.sum(np.abs(a * b))) for a, b in zip(w, y))
+        for iv, v in zip((LP, LT, LR), w):
+            T.set_cc_all(iv, v)
+        T.set_cc_all(LW, np.zeros((n, bl)))
+        vbar = T.fas_vcycle_adjoint(top, [LP, LT, LR, LW])
+        if whole:
+            vbar = vbar + T.fas_vcycle_adjoint(top, [LP, LT, LR, LW])
+            vbar = vbar + T.fas_fmg_adjoint([LP, LT, LR, LW])
+        xb = [T.get_cc_all(iv) * mask for iv in (LP, LT, LR)]
+        rhs = sum(float(np.sum(a * b)) for a, b in zip(xb, x)) + float(np.dot(vbar, V))
+        assert abs(lhs - rhs) <= 1e-12 * scale, (whole, lhs, rhs, scale)
+    with pytest.raises(Exception):
+        T.fas_vcycle_adjoint(T.info()["max_lvl"] + 1, [LP, LT, LR, LW])
+    T.close()

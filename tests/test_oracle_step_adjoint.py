@@ -352,3 +352,27 @@ def test_leaf_geometry_volumes(variant):
     z = geo.z[geo.mask]
     top = 2 * L if variant == "cyl" else L
     assert z.min() > 0 and z.max() < top
+
+
+@pytest.mark.parametrize("op", ["lpl", "helmholtz"])
+@pytest.mark.parametrize("dim,nc,coord,base,kinds", FMG_CASES)
+def test_oracle_vcycle_reverse_dot_product(dim, nc, coord, base, kinds, op):
+    """o_fas_vcycle_adj against the oracle's forward V-cycle up to the
+    finest level (the same affine map and identity as the FMG's)."""
+    T = O.OTree(dim, nc, 8, 1, 1.0 / nc, coarsen_to=2, coord=coord)
+    S.build_tree(T, dim, nc, _rules, n_base=base)
+    rng = np.random.default_rng(6)
+    V = rng.standard_normal(2 * dim)
+    T.__class__ = O.OStreamTree
+    T.mg_setup(1, 2, 3, n_base=4, bc_kind=np.array(kinds, np.int32), bc_val=V, auto_ops=True)
+    if op == "helmholtz":
+        T.mg_set_helmholtz(float(nc * 8) ** 2)
+    top = T.info()["max_lvl"]
+    x = [_rand(T, rng, iv) for iv in (1, 2, 3)]
+    T.fas_vcycle(top, False)
+    y = [T.get_cc_all(iv) for iv in (1, 2, 3)]
+    w = [_rand(T, rng, lv) for lv in (5, 6, 7)]
+    T.set_cc_all(8, np.zeros_like(w[0]))
+    vbar = T.fas_vcycle_adjoint(top, [5, 6, 7, 8])
+    xb = [T.get_cc_all(lv) for lv in (5, 6, 7)]
+    _check(list(zip(w, y)), list(zip(xb, x)), extra=float(np.dot(vbar, V)))
