@@ -3689,7 +3689,8 @@ __global__ __launch_bounds__(256) void k_adj_corr(Dev a, const int* __restrict__
 }
 
 // whole-box transposes: mode 0, tmp = phi: lp += lt, lt = 0; mode 1,
-// tmp = phi - tmp: lp += lt, lt = -lt; mode 2, rhs += tmp: lt += lr
+// tmp = phi - tmp: lp += lt, lt = -lt; mode 2, rhs += tmp: lt += lr;
+// mode 3, modes 0 and 2 in turn: lp += lt, lt = lr
 __global__ __launch_bounds__(256) void k_adj_whole(Dev a, const int* __restrict__ list, int n,
                                                    int bl, int tpb, int mode, int il, int ilt,
                                                    int ilr) {
@@ -3700,6 +3701,10 @@ __global__ __launch_bounds__(256) void k_adj_whole(Dev a, const int* __restrict_
   for (int q = lt; q < bl; q += tpb) {
     if (mode == 2) {
       Lt[q] += Lr[q];
+    } else if (mode == 3) {
+      // mode 0 then mode 2 of the same cell: tmp = phi, then rhs += tmp
+      Lp[q] += Lt[q];
+      Lt[q] = Lr[q];
     } else {
       const double tb = Lt[q];
       Lp[q] += tb;
@@ -3842,8 +3847,8 @@ static void adj_update_coarse(Pool& P, const Ctx& c, const Dev& a, const AdjMg& 
                               bool keep_res) {
   const int np = P.n_parents(lvl - 1);
   const int pnc = P.nc_lvl(lvl - 1);
-  if (!keep_res) adj_whole(P, a, P.parents(lvl - 1), np, pnc, 0, m);  // tmp = phi
-  adj_whole(P, a, P.parents(lvl - 1), np, pnc, 2, m);                 // rhs += tmp
+  // tmp = phi (update_coarse only), then rhs += tmp
+  adj_whole(P, a, P.parents(lvl - 1), np, pnc, keep_res ? 2 : 3, m);
   adj_op<D>(P, c, a, P.parents(lvl - 1), np, pnc, m.lr, 1.0, m.lp, true);  // rhs = A phi
   adj_gc_level<D>(P, a, m, lvl - 1);
   const int n = P.n_ids(lvl), nc = P.nc_lvl(lvl);
