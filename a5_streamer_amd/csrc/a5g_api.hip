@@ -1,4 +1,5 @@
-// GPU box pool management and the C ABI of the a5g library (include/a5g.h).
+// The device pool of the tree's boxes and the C ABI of the a5g library
+// (include/a5g.h).
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
