@@ -182,7 +182,7 @@ class FrozenWindow:
 
     SLOT_START, SLOT_FINAL, SLOT_CKPT = 0, 1, 2
 
-    def __init__(self, run, n_steps, every=10, dts=None):
+    def __init__(self, run, n_steps, every=10, dts=None, keep_all=False):
         c = run.c
         if not c.adjoint:
             raise ValueError("FrozenWindow: the run needs Config(adjoint=True)")
@@ -191,7 +191,13 @@ class FrozenWindow:
         if n_steps < 1 or every < 1:
             raise ValueError("FrozenWindow: n_steps and every must be positive")
         self.run, self.T = run, run.T
-        self.n, self.k = int(n_steps), int(every)
+        # keep_all: the forward pass keeps the three states of every step in
+        # device memory (3 n_steps whole states: 72 GB for 100 steps of 1.5M
+        # cells, within one MI355X's 288 GB), and the reverse pass recomputes
+        # nothing; one segment spans the window
+        self.keep_all = bool(keep_all)
+        self.n = int(n_steps)
+        self.k = self.n if self.keep_all else int(every)
         self.n_seg = (self.n + self.k - 1) // self.k
         self.v = StepAdjointVars(run)
         self.dts = None if dts is None else [float(x) for x in dts]
@@ -281,10 +287,14 @@ class FrozenWindow:
         dts = [] if record else self.dts
         amr = self.run.c.steps_amr
         for s in range(self.n):
-            if s % self.k == 0:
-                self._save(self.SLOT_CKPT + s // self.k)
             if record:
                 dts.append(self.run.max_dt() if s % amr == 0 else dts[-1])
+            if self.keep_all:
+                self._save(self._slot_step(s, 0))
+                self._step(dts[s], keep=(self._slot_step(s, 1), self._slot_step(s, 2)))
+                continue
+            if s % self.k == 0:
+                self._save(self.SLOT_CKPT + s // self.k)
             self._step(dts[s])
         self._save(self.SLOT_FINAL)
         self.dts = dts
@@ -360,12 +370,14 @@ class FrozenWindow:
         for seg in reversed(range(self.n_seg)):
             lo, hi = seg * self.k, min(self.n, (seg + 1) * self.k)
             # recompute the segment from its checkpoint, keeping the three
-            # states of every step that its reverse reads
-            self._load(self.SLOT_CKPT + seg)
-            for s in range(lo, hi):
-                j = s - lo
-                self._save(self._slot_step(j, 0))
-                self._step(self.dts[s], keep=(self._slot_step(j, 1), self._slot_step(j, 2)))
+            # states of every step that its reverse reads (kept by the
+            # forward pass already with keep_all)
+            if not self.keep_all:
+                self._load(self.SLOT_CKPT + seg)
+                for s in range(lo, hi):
+                    j = s - lo
+                    self._save(self._slot_step(j, 0))
+                    self._step(self.dts[s], keep=(self._slot_step(j, 1), self._slot_step(j, 2)))
             for s in reversed(range(lo, hi)):
                 j = s - lo
                 # the step's end state: the next step's start, the next

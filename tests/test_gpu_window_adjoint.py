@@ -221,11 +221,12 @@ def test_window_gradient_does_not_depend_on_checkpoint_spacing():
     run, win = _window(5, 5)
     geo = AD.LeafGeometry(win.T)
     ga = _grads(win, geo, 0)
-    for every in (1, 2):
+    for every in (1, 2, "all"):
         # a new window on the same tree (the same store slots, used in turn)
         # starts from the state and parameters the run holds: rewind it
         win.reset()
-        w2 = AD.FrozenWindow(run, 5, every=every, dts=win.dts)
+        w2 = (AD.FrozenWindow(run, 5, keep_all=True, dts=win.dts) if every == "all"
+              else AD.FrozenWindow(run, 5, every=every, dts=win.dts))
         gb = _grads(w2, geo, 0)
         np.testing.assert_allclose(gb[0], ga[0], rtol=0, atol=1e-12 * np.abs(ga[0]).max())
         assert abs(gb[1] - ga[1]) <= 1e-12 * abs(ga[1])

@@ -274,6 +274,13 @@ def test_window_reset_and_checkpoint_spacing():
     g2 = grads(w2)
     for a, b in zip(g1, g2):
         np.testing.assert_array_equal(a, b)
+    # every state kept by the forward pass, nothing recomputed
+    w2.reset()
+    w3 = AD.FrozenWindow(run, 3, keep_all=True, dts=win.dts)
+    assert w3.store_slots() == 3 + 3 * 3
+    g3 = grads(w3)
+    for a, b in zip(g1, g3):
+        np.testing.assert_array_equal(a, b)
 
 
 from window_cases import WINDOWS  # noqa: E402
