@@ -3751,17 +3751,19 @@ static void adj_gc_level(Pool& P, const Dev& a, const AdjMg& m, int lvl) {
 
 template <int D>
 static void adj_gsrb_boxes(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int lvl,
-                           int n_cycle) {
+                           int n_cycle, bool gc_before = false) {
   const int n = P.n_ids(lvl);
   if (!n) return;
   const int nc = P.nc_lvl(lvl);
   const Geo g = geo(n, cells_of(D, nc, false) / 2);
   // forward: (half-sweep it, ghost pass) for it = 1 .. 2 n_cycle; the last
   // ghost pass is transposed on its own, each earlier one inside the
-  // transpose of the half-sweep that follows it
+  // transpose of the half-sweep that follows it. gc_before: the caller's
+  // ghost pass just before the call (the one after a correction) goes
+  // inside the first half-sweep's transpose as well.
   adj_gc_level<D>(P, a, m, lvl);
   for (int it = 2 * n_cycle; it >= 1; it--) {
-    const bool gc = it > 1;
+    const bool gc = it > 1 || gc_before;
     if (c.cyl) {
       if (gc)
         hipLaunchKernelGGL((k_adj_gsrb<2, true, true>), dim3(g.grid), dim3(256), 0, P.stream, a,
@@ -3870,8 +3872,8 @@ template <int D>
 static void adj_vcycle(Pool& P, const Ctx& c, const Dev& a, const AdjMg& m, int max_lvl) {
   const int min_lvl = P.tree.min_lvl;
   for (int lvl = max_lvl; lvl >= min_lvl + 1; lvl--) {
-    adj_gsrb_boxes<D>(P, c, a, m, lvl, c.n_up);
-    adj_gc_level<D>(P, a, m, lvl);
+    // the up-smoothing and the ghost pass after the correction before it
+    adj_gsrb_boxes<D>(P, c, a, m, lvl, c.n_up, true);
     adj_correct_children<D>(P, a, m, lvl - 1);
   }
   adj_gsrb_boxes<D>(P, c, a, m, min_lvl, c.n_base);
